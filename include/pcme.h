@@ -1,0 +1,185 @@
+/*
+ * pcme.h -- C ABI of the MI355X (gfx950) H.264 P-frame motion search.
+ *
+ * The reference (pito-codecs 5.3.0) has no FFI on this path: its search is
+ * private Rust inside the CPU H.264 encoder.  Each entry point below names
+ * the Rust item it replaces (file:line under src/h264/enc/) and keeps that
+ * item's argument meaning.  Plain pointers and sizes only.
+ *
+ * Device pointers: every pointer marked (device) must be device memory
+ * reachable by the current HIP device.  Launches go to `stream` (a
+ * hipStream_t, NULL = the default stream) and are asynchronous unless
+ * stated otherwise.
+ *
+ * Errors: 0 = success, negative = failure (PCME_E*).  The reference's search
+ * is infallible and its frame threads turn a panic into a sticky
+ * Error::Failed for the frame (enc/mod.rs:228-233, 630-642); a caller maps a
+ * non-zero status onto that same path.  pcme_last_error() returns a
+ * thread-local message for the last failure.  There is no silent fallback:
+ * without a usable gfx950 device every compute entry point fails with
+ * PCME_ENODEV.
+ *
+ * Picture layout (picture.rs:7-51, 150-180): a padded luma plane of a coded
+ * picture of mbw x mbh macroblocks has stride 16*mbw + 128 and
+ * 16*mbh + 128 rows; picture pixel (0,0) is at byte 64*stride + 64.  The
+ * three half-pel planes (picture.rs:318-380) use the same layout.  Chroma
+ * planes: stride 8*mbw + 64, 8*mbh + 64 rows, pad 32.
+ */
+#ifndef PCME_H
+#define PCME_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCME_LUMA_PAD 64
+#define PCME_CHROMA_PAD 32
+
+#define PCME_OK 0
+#define PCME_EINVAL (-1)   /* bad argument (sizes, ranges, null pointers) */
+#define PCME_ENODEV (-2)   /* no usable gfx950 device */
+#define PCME_EHIP (-3)     /* a HIP runtime call failed */
+#define PCME_ESTALL (-4)   /* a wavefront dependency wait exceeded its bound */
+#define PCME_ENOMEM (-5)   /* device allocation failed */
+
+/* Quarter-pel motion vector; layout of `#[repr(Rust)] Mv {x: i16, y: i16}`
+ * (mbinfo.rs:1-16). */
+typedef struct pcme_mv { int16_t x, y; } pcme_mv;
+
+/* Partition shapes, in the order of `enum Part` (mbinfo.rs:47-72). */
+enum { PCME_P16x16 = 0, PCME_P16x8 = 1, PCME_P8x16 = 2, PCME_P8x8 = 3 };
+
+/* Per-macroblock result of the ME-only field analysis: what analyse_p
+ * commits for an inter MB (slice.rs:1921-1945, cand_inter_info
+ * slice.rs:1751-1785) plus the per-mode search_part totals.  128 bytes. */
+typedef struct pcme_mb_out {
+  pcme_mv mv[16];       /* MbInfo.mv[0], 4x4 blocks in raster order */
+  pcme_mv mvs[4];       /* best_mvs (one per partition, rest zero) */
+  pcme_mv mvps[4];      /* best_mvps: the MV predictor of each partition */
+  uint32_t cost[4];     /* search_part total per mode P16x16..P8x8;
+                           0xFFFFFFFF = mode not searched */
+  uint32_t best_cost;   /* chosen cost incl. the mode penalty (lam*k)>>8 */
+  pcme_mv skip_mv;      /* pskip_mv (slice.rs:484-494), the 16x16 seed */
+  pcme_mv mv16;         /* 16x16 result mv16[0], the seed of the other modes */
+  uint8_t part;         /* best_part (PCME_P*) */
+  uint8_t reserved[3];
+} pcme_mb_out;
+
+/* Search parameters (EncParams subset, slice.rs:21-34). */
+typedef struct pcme_params {
+  int32_t mbw, mbh;        /* coded size in macroblocks */
+  int32_t me_range;        /* hex-iteration cap and clamp, int-pel */
+  int32_t subpel_rounds;   /* extra quarter-pel rounds */
+  int32_t partitions;      /* 0/1: search 16x8, 8x16, 8x8 when cost16>1024 */
+  int32_t qp;              /* lambda = pcme_lambda(qp) */
+} pcme_params;
+
+/* One P frame of the batched ME-only field search.  All pointers (device). */
+typedef struct pcme_frame_job {
+  const uint8_t *src;      /* padded source luma of the frame */
+  const uint8_t *ref;      /* padded reference luma, pads edge-extended */
+  const uint8_t *ref_h;    /* half-pel planes of the reference (H, V, C) */
+  const uint8_t *ref_v;
+  const uint8_t *ref_c;
+  pcme_mb_out *out;        /* mbw*mbh records, raster order */
+  pcme_params prm;
+} pcme_frame_job;
+
+/* ---- library ------------------------------------------------------- */
+int pcme_version(void);
+const char *pcme_last_error(void);
+/* Number of visible gfx950 devices (0 if none); never fails. */
+int pcme_device_count(void);
+
+/* Lambda table (slice.rs:52-68, Lambdas::new): the u32 `sad` entry. */
+uint32_t pcme_lambda(int32_t qp);
+/* The packed quarter-pel source table the kernels use (picture.rs:387-404),
+ * 16 entries of 8 bits: [1:0] plane A, [2] dx A, [3] dy A, [5:4] plane B,
+ * [6] dx B, [7] dy B; planes 0 full, 1 H, 2 V, 3 C.  Writes 4 words. */
+void pcme_qpel_table(uint32_t out[4]);
+
+/* ---- picture preparation -------------------------------------------- */
+/* Import + edge extension of n planes (mod.rs:842-873 import, frame.rs:58-97
+ * extend): plane i of src (device) starts at src + i*src_pitch, is w x h with
+ * row stride sstride; plane i of dst (device) starts at dst + i*dst_pitch and
+ * is a padded plane of stride dstride around an aw x ah aligned picture with
+ * `pad` pad pixels per side (luma: aw = 16*mbw, dstride = aw + 128, pad 64;
+ * chroma: aw = 8*mbw, dstride = aw + 64, pad 32).  padded(x, y) =
+ * src[clamp(y, 0, h-1)][clamp(x, 0, w-1)] for x in [-pad, dstride-pad),
+ * y in [-pad, ah+pad).  Requires w <= aw, h <= ah. */
+int pcme_pad_planes(const uint8_t *src, int64_t src_pitch, int32_t sstride,
+                    int32_t w, int32_t h, uint8_t *dst, int64_t dst_pitch,
+                    int32_t dstride, int32_t aw, int32_t ah, int32_t pad,
+                    int32_t n, void *stream);
+/* Half-pel planes of n padded luma planes (picture.rs:318-380, run by
+ * frame.rs:99-122 over the whole padded plane).  Plane i of ref (device) is
+ * at ref + i*pitch; its H, V and C planes are written at h + i*pitch,
+ * v + i*pitch, c + i*pitch (device).  Layout as above for mbw x mbh. */
+int pcme_hpel(const uint8_t *ref, uint8_t *h, uint8_t *v, uint8_t *c,
+              int64_t pitch, int32_t n, int32_t mbw, int32_t mbh,
+              void *stream);
+
+/* ---- the search ----------------------------------------------------- */
+/* ME-only field over a batch of P frames: for every macroblock of every
+ * job, pskip_mv -> search_part(P16x16, seed = skip_mv) -> when partitions
+ * and cost16 > 1024, search_part(P16x8 / P8x16 / P8x8, seed = mv16) ->
+ * strict-< mode choice -> commit (slice.rs:1870-1945 without early skip,
+ * intra and RD; search_part slice.rs:1810-1868; search slice.rs:1590-1723;
+ * mvp slice.rs:392-482).  Each job's neighbours are its own committed MBs,
+ * its vlim rule that of frame.rs:152-156 / picture.rs:261-265.  Jobs are
+ * independent and may differ in size and parameters.  `jobs` is a HOST
+ * array; the library copies it.  Requires 1 <= me_range <= 1024,
+ * 0 <= subpel_rounds <= 8, 0 <= qp <= 51, mbw, mbh >= 1.  Blocks until the
+ * job table is uploaded; the search itself runs asynchronously on stream.
+ * Calls on one device are serialised by the library. */
+int pcme_search_frames(const pcme_frame_job *jobs, int njobs, void *stream);
+
+/* Same as pcme_search_frames, then waits for the stream and reports a
+ * dependency-wait stall (PCME_ESTALL) if one happened. */
+int pcme_search_frames_sync(const pcme_frame_job *jobs, int njobs,
+                            void *stream);
+
+/* Per-call form of SliceEnc::search (slice.rs:1590-1723), batched: job i
+ * searches one w x h block at (bx, by) of a reference with predictor mvp and
+ * candidates cands[0..ncands) (ncands <= 2, as search_part passes
+ * [seed, mvs[0]]), under the vlim of the caller's current MB row.  Results
+ * go to out_mv[i], out_cost[i] (device). */
+typedef struct pcme_search_job {
+  const uint8_t *src;      /* (device) top-left of the w x h source block */
+  int32_t sstride;
+  const uint8_t *ref;      /* (device) padded planes full, H, V, C */
+  const uint8_t *ref_h;
+  const uint8_t *ref_v;
+  const uint8_t *ref_c;
+  int32_t mbw, mbh;        /* coded picture size in MBs */
+  int32_t bx, by, w, h;    /* block position/size in picture pixels */
+  pcme_mv mvp;
+  pcme_mv cands[2];
+  int32_t ncands;
+  uint32_t lam;
+  int32_t me_range, subpel_rounds;
+  int64_t vlim;            /* SliceEnc::vlim for the block's MB row */
+} pcme_search_job;
+int pcme_search_batch(const pcme_search_job *jobs, int njobs,
+                      pcme_mv *out_mv, uint32_t *out_cost, void *stream);
+
+/* Motion-compensated prediction of the chosen partitions (pred_block,
+ * slice.rs:1725-1749 = mc_luma picture.rs:415-449 + mc_chroma
+ * picture.rs:451-487) for every MB of a searched frame.  ref planes and
+ * chroma planes (device, padded), mb (device) = search output.  Writes
+ * pred_y (16x16 per MB, raster MB order, 256 B each) and pred_u/pred_v
+ * (8x8 per MB, 64 B each). */
+int pcme_pred_frame(const uint8_t *ref, const uint8_t *ref_h,
+                    const uint8_t *ref_v, const uint8_t *ref_c,
+                    const uint8_t *ref_u, const uint8_t *ref_vv,
+                    const pcme_mb_out *mb, int32_t mbw, int32_t mbh,
+                    uint8_t *pred_y, uint8_t *pred_u, uint8_t *pred_v,
+                    void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,221 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check and the
+benchmark's CPU-baseline leg, as the checker -- never by the product path.
+The functions restate pito-codecs src/h264/enc/{dsp,picture,slice}.rs; see
+the header of pcme_oracle.c for the line map and for what pins each one.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+LUMA_PAD = 64
+CHROMA_PAD = 32
+
+OK_NONE, OK_I4x4, OK_I8x8, OK_I16x16, OK_PSKIP, OK_PINTER = range(6)
+
+
+class omv(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_int16), ("y", ctypes.c_int16)]
+
+
+class ombinfo(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int8), ("ref_idx", ctypes.c_int8 * 4), ("mv", omv * 16)]
+
+
+class oref(ctypes.Structure):
+    _fields_ = [("pl", ctypes.c_void_p * 4), ("stride", ctypes.c_int), ("pad", ctypes.c_int)]
+
+
+class oparams(ctypes.Structure):
+    _fields_ = [("mbw", ctypes.c_int), ("mbh", ctypes.c_int), ("me_range", ctypes.c_int),
+                ("subpel_rounds", ctypes.c_int), ("partitions", ctypes.c_int), ("vlim", ctypes.c_long)]
+
+
+class ocounters(ctypes.Structure):
+    _fields_ = [("sad_px", ctypes.c_uint64), ("satd_px", ctypes.c_uint64),
+                ("sad_evals", ctypes.c_uint64), ("satd_evals", ctypes.c_uint64),
+                ("searches", ctypes.c_uint64), ("mbs", ctypes.c_uint64)]
+
+
+class ojob(ctypes.Structure):
+    _fields_ = [("src", ctypes.c_void_p), ("ref", ctypes.c_void_p), ("out", ctypes.c_void_p),
+                ("mbw", ctypes.c_int), ("mbh", ctypes.c_int), ("me_range", ctypes.c_int),
+                ("subpel_rounds", ctypes.c_int), ("partitions", ctypes.c_int), ("qp", ctypes.c_int)]
+
+
+# omb_out has the layout of pcme_mb_out
+OMB_DTYPE = np.dtype([
+    ("mv", "<i2", (16, 2)), ("mvs", "<i2", (4, 2)), ("mvps", "<i2", (4, 2)),
+    ("cost", "<u4", (4,)), ("best_cost", "<u4"), ("skip_mv", "<i2", (2,)),
+    ("mv16", "<i2", (2,)), ("part", "u1"), ("reserved", "u1", (3,)),
+])
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-B", "-C", HERE], check=True, stdout=subprocess.DEVNULL)
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    lib = ctypes.CDLL(LIB)
+    c = ctypes
+    P = c.c_void_p
+    lib.o_sad.restype = c.c_uint32
+    lib.o_sad.argtypes = [P, c.c_int, P, c.c_int, c.c_int, c.c_int]
+    lib.o_sad16.restype = c.c_uint32
+    lib.o_sad16.argtypes = [P, c.c_int, P, c.c_int, c.c_int]
+    lib.o_satd.restype = c.c_uint32
+    lib.o_satd.argtypes = [P, c.c_int, P, c.c_int, c.c_int, c.c_int]
+    lib.o_lambda.restype = c.c_uint32
+    lib.o_lambda.argtypes = [c.c_int]
+    lib.o_se_bits.restype = c.c_uint32
+    lib.o_se_bits.argtypes = [c.c_int32]
+    lib.o_mv_bits.restype = c.c_uint32
+    lib.o_mv_bits.argtypes = [omv, omv]
+    lib.o_median.restype = c.c_int16
+    lib.o_median.argtypes = [c.c_int16, c.c_int16, c.c_int16]
+    lib.o_import_plane.restype = None
+    lib.o_import_plane.argtypes = [P, c.c_int, c.c_int, c.c_int, P, c.c_int, c.c_int, c.c_int, c.c_int]
+    lib.o_extend_plane.restype = None
+    lib.o_extend_plane.argtypes = [P, c.c_int, c.c_int, c.c_int, c.c_int]
+    lib.o_hpel.restype = None
+    lib.o_hpel.argtypes = [P, c.c_int, c.c_int, P, P, P]
+    lib.o_mc_luma.restype = None
+    lib.o_mc_luma.argtypes = [c.POINTER(oref), c.c_int, c.c_int, omv, c.c_int, c.c_int, P, c.c_int]
+    lib.o_mc_chroma.restype = None
+    lib.o_mc_chroma.argtypes = [P, P, c.c_int, c.c_int, c.c_int, c.c_int, omv, c.c_int, c.c_int, P, P, c.c_int]
+    lib.o_vlim.restype = c.c_long
+    lib.o_vlim.argtypes = [c.c_int, c.c_int]
+    lib.o_mv_ok.restype = c.c_int
+    lib.o_mv_ok.argtypes = [c.POINTER(oparams), c.c_int, c.c_int, c.c_int, c.c_int, omv]
+    lib.o_nb_part.restype = c.c_int
+    lib.o_nb_part.argtypes = [c.POINTER(oparams), P, c.POINTER(ombinfo), c.c_uint16, c.c_int, c.c_int,
+                              c.c_int, c.c_int, c.POINTER(c.c_int8), c.POINTER(omv)]
+    lib.o_mvp.restype = omv
+    lib.o_mvp.argtypes = [c.POINTER(oparams), P, c.POINTER(ombinfo), c.c_uint16, c.c_int, c.c_int,
+                          c.c_int, c.c_int, c.c_int, c.c_int8, c.c_int]
+    lib.o_pskip_mv.restype = omv
+    lib.o_pskip_mv.argtypes = [c.POINTER(oparams), P, c.c_int, c.c_int]
+    lib.o_search.restype = omv
+    lib.o_search.argtypes = [c.POINTER(oparams), c.POINTER(oref), c.c_int, c.c_int, c.c_int, c.c_int, P,
+                             c.c_int, omv, c.POINTER(omv), c.c_int, c.c_uint32, c.POINTER(c.c_uint32),
+                             c.POINTER(ocounters)]
+    lib.o_search_part.restype = c.c_uint32
+    lib.o_search_part.argtypes = [c.POINTER(oparams), P, c.c_int, c.c_int, c.POINTER(oref), c.c_int, P,
+                                  c.c_uint32, omv, c.POINTER(omv), c.POINTER(omv), c.POINTER(ocounters)]
+    lib.o_analyse_mb.restype = None
+    lib.o_analyse_mb.argtypes = [c.POINTER(oparams), P, c.c_int, c.c_int, c.POINTER(oref), P, c.c_int,
+                                 c.c_int, c.c_uint32, P, c.POINTER(ocounters)]
+    lib.o_search_frame.restype = None
+    lib.o_search_frame.argtypes = [c.c_int, c.c_int, c.c_int, c.c_int, c.c_int, c.c_uint32, P, c.c_int,
+                                   c.c_int, c.POINTER(oref), P, c.POINTER(ocounters)]
+    lib.o_search_jobs_mt.restype = c.c_double
+    lib.o_search_jobs_mt.argtypes = [c.POINTER(ojob), c.c_int, c.c_int, c.POINTER(ocounters)]
+    _lib = lib
+    return lib
+
+
+def _p(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+# ------------------------------------------------------------ convenience
+
+def pad_luma(frame: np.ndarray, mbw: int, mbh: int) -> np.ndarray:
+    """Import (mod.rs:842-873) + extend (frame.rs:58-97) one luma plane."""
+    lib = load()
+    h, w = frame.shape
+    stride, rows = 16 * mbw + 2 * LUMA_PAD, 16 * mbh + 2 * LUMA_PAD
+    out = np.zeros((rows, stride), np.uint8)
+    f = np.ascontiguousarray(frame)
+    lib.o_import_plane(_p(f), w, w, h, _p(out), stride, LUMA_PAD, 16 * mbw, 16 * mbh)
+    lib.o_extend_plane(_p(out), stride, LUMA_PAD, 16 * mbw, 16 * mbh)
+    return out
+
+
+def pad_chroma(frame: np.ndarray, mbw: int, mbh: int) -> np.ndarray:
+    lib = load()
+    h, w = frame.shape
+    stride, rows = 8 * mbw + 2 * CHROMA_PAD, 8 * mbh + 2 * CHROMA_PAD
+    out = np.zeros((rows, stride), np.uint8)
+    f = np.ascontiguousarray(frame)
+    lib.o_import_plane(_p(f), w, w, h, _p(out), stride, CHROMA_PAD, 8 * mbw, 8 * mbh)
+    lib.o_extend_plane(_p(out), stride, CHROMA_PAD, 8 * mbw, 8 * mbh)
+    return out
+
+
+def hpel(plane: np.ndarray):
+    """(H, V, C) half-pel planes of a padded plane (picture.rs:318-380)."""
+    lib = load()
+    rows, stride = plane.shape
+    p = np.ascontiguousarray(plane)
+    h = np.empty_like(p)
+    v = np.empty_like(p)
+    c = np.empty_like(p)
+    lib.o_hpel(_p(p), stride, rows, _p(h), _p(v), _p(c))
+    return h, v, c
+
+
+class Ref:
+    """A reference picture: padded plane + its half-pel planes, kept alive."""
+
+    def __init__(self, padded: np.ndarray):
+        self.planes = [np.ascontiguousarray(padded), *hpel(padded)]
+        self.view = oref()
+        for i, a in enumerate(self.planes):
+            self.view.pl[i] = _p(a)
+        self.view.stride = padded.shape[1]
+        self.view.pad = LUMA_PAD
+
+
+def search_frame(src_padded: np.ndarray, ref: Ref, mbw: int, mbh: int, me_range: int,
+                 subpel_rounds: int, partitions: bool, qp: int, counters: ocounters | None = None):
+    lib = load()
+    out = np.zeros(mbw * mbh, OMB_DTYPE)
+    s = np.ascontiguousarray(src_padded)
+    lib.o_search_frame(mbw, mbh, me_range, subpel_rounds, 1 if partitions else 0, lib.o_lambda(qp),
+                       _p(s), s.shape[1], LUMA_PAD, ctypes.byref(ref.view), _p(out),
+                       ctypes.byref(counters) if counters is not None else None)
+    return out.reshape(mbh, mbw)
+
+
+def search_clip_me(y_frames: np.ndarray, me_range: int, subpel_rounds: int, partitions: bool, qp: int,
+                   counters: ocounters | None = None):
+    """ME-only field of every P frame of an IPPP clip, frame t vs t-1."""
+    n, h, w = y_frames.shape
+    mbw, mbh = (w + 15) // 16, (h + 15) // 16
+    padded = [pad_luma(y_frames[i], mbw, mbh) for i in range(n)]
+    res = []
+    for t in range(1, n):
+        ref = Ref(padded[t - 1])
+        res.append(search_frame(padded[t], ref, mbw, mbh, me_range, subpel_rounds, partitions, qp, counters))
+    return np.stack(res) if res else np.zeros((0, mbh, mbw), OMB_DTYPE)
+
+
+def search_jobs_mt(srcs, refs, mbw, mbh, me_range, subpel_rounds, partitions, qp, threads):
+    """Frame-threaded CPU run (half-pel build + search per job); returns
+    (results, wall_seconds, counters)."""
+    lib = load()
+    n = len(srcs)
+    outs = [np.zeros(mbw * mbh, OMB_DTYPE) for _ in range(n)]
+    keep = [np.ascontiguousarray(a) for a in list(srcs) + list(refs)]
+    jobs = (ojob * n)()
+    for i in range(n):
+        jobs[i] = ojob(_p(keep[i]), _p(keep[n + i]), _p(outs[i]), mbw, mbh, me_range, subpel_rounds,
+                       1 if partitions else 0, qp)
+    ctr = ocounters()
+    secs = lib.o_search_jobs_mt(jobs, n, threads, ctypes.byref(ctr))
+    return [o.reshape(mbh, mbw) for o in outs], secs, ctr

@@ -1,0 +1,754 @@
+/*
+ * pcme_oracle.c -- CPU oracle for the H.264 P-frame motion search.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the CPU
+ * baseline of the benchmark.  Nothing in the product library calls it.
+ *
+ * A plain-C restatement of pito-codecs 5.3.0 (Rust, not buildable here: no
+ * cargo/rustc in the image), file by file:
+ *   src/h264/enc/dsp.rs:1-88          sad, sad16, satd_strip, satd
+ *   src/h264/enc/slice.rs:52-68       Lambdas::new
+ *   src/h264/enc/slice.rs:130-147     se_bits, mv_bits, median
+ *   src/h264/enc/slice.rs:392-507     nb_part, mvp, pskip_mv, mv_ok
+ *   src/h264/enc/slice.rs:1590-1723   search
+ *   src/h264/enc/slice.rs:1751-1868   cand_inter_info, mask_of, shape_of,
+ *                                     search_part
+ *   src/h264/enc/slice.rs:1870-1945   analyse_p (skip_mv, 16x16 search,
+ *                                     partition choice)
+ *   src/h264/enc/picture.rs:7-51,285-487  planes, extend, hpel_rows,
+ *                                     QPEL_SRC, mc_luma, mc_chroma
+ *   src/h264/enc/frame.rs:13,58-97,152-156  VMAX, extend, the ref view row
+ *                                     limit; picture.rs:211-268 (lim)
+ *   src/h264/enc/mod.rs:842-873       import (alignment replication)
+ *
+ * How each part is pinned (tests/):
+ *   o_sad/o_sad16            closed form (numpy |a-b| sum), random + extremes
+ *   o_satd                   textbook 4x4 Hadamard sum(|H D H^T|)/2 in numpy,
+ *                            constant-residual KAT (8a per 4x4 block)
+ *   o_lambda                 closed form and the published table of values
+ *   o_se_bits/o_mv_bits      Exp-Golomb se(v) length from its definition
+ *   o_hpel/o_mc_luma         H.264 8.4.2.2.1 equations (b, h, j and the
+ *                            quarter-sample averages) in numpy with
+ *                            clamp-to-plane; impulse KAT
+ *   o_mc_chroma              H.264 8.4.2.2.2 bilinear equation in numpy
+ *   o_nb_part/o_mvp/o_pskip  H.264 8.4.1.3 / 8.4.1.1 restated in Python in
+ *                            the decoder's structure (dec/slicedec.rs:581-681)
+ *                            over random neighbour configurations
+ *   o_search/o_search_part/  a second, independently written Python
+ *   o_analyse_mb/            restatement on tiny frames (golden vectors
+ *   o_search_frame           under tests/golden/), plus invariants: returned
+ *                            cost == SATD(mc(best)) + lambda*bits, mv_ok(best),
+ *                            brute-force recovery of a pure integer shift
+ *   o_import_plane/o_extend  numpy edge replication
+ */
+#include "pcme_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+int o_version(void) { return 1; }
+
+/* ------------------------------------------------------------------ dsp */
+
+/* dsp.rs:1-14 */
+uint32_t o_sad(const uint8_t *a, int sa, const uint8_t *b, int sb, int w,
+               int h) {
+  uint32_t s = 0;
+  for (int r = 0; r < h; r++)
+    for (int i = 0; i < w; i++) {
+      int d = (int)a[r * sa + i] - (int)b[r * sb + i];
+      s += (uint32_t)(d < 0 ? -d : d);
+    }
+  return s;
+}
+
+/* dsp.rs:16-33: per-row u16 accumulator, 16 wide */
+uint32_t o_sad16(const uint8_t *a, int sa, const uint8_t *b, int sb, int h) {
+  uint32_t s = 0;
+  for (int r = 0; r < h; r++) {
+    uint16_t acc = 0;
+    for (int i = 0; i < 16; i++) {
+      int d = (int)a[r * sa + i] - (int)b[r * sb + i];
+      acc = (uint16_t)(acc + (d < 0 ? -d : d));
+    }
+    s += acc;
+  }
+  return s;
+}
+
+static inline uint32_t uabs16(int16_t v) {
+  return (uint32_t)(v < 0 ? -(int32_t)v : (int32_t)v);
+}
+
+/* dsp.rs:35-70: one 4-row strip of width n (4, 8 or 16), i16 internals */
+static uint32_t satd_strip(const uint8_t *a, int sa, const uint8_t *b, int sb,
+                           int n) {
+  int16_t d[4][16], v[4][16];
+  for (int r = 0; r < 4; r++)
+    for (int i = 0; i < n; i++)
+      d[r][i] = (int16_t)((int16_t)a[r * sa + i] - (int16_t)b[r * sb + i]);
+  for (int i = 0; i < n; i++) {
+    int16_t s01 = (int16_t)(d[0][i] + d[1][i]);
+    int16_t d01 = (int16_t)(d[0][i] - d[1][i]);
+    int16_t s23 = (int16_t)(d[2][i] + d[3][i]);
+    int16_t d23 = (int16_t)(d[2][i] - d[3][i]);
+    v[0][i] = (int16_t)(s01 + s23);
+    v[1][i] = (int16_t)(s01 - s23);
+    v[2][i] = (int16_t)(d01 + d23);
+    v[3][i] = (int16_t)(d01 - d23);
+  }
+  uint32_t total = 0;
+  int hh = n / 2;
+  for (int r = 0; r < 4; r++) {
+    int16_t hs[8], hd[8];
+    for (int k = 0; k < hh; k++) {
+      hs[k] = (int16_t)(v[r][2 * k] + v[r][2 * k + 1]);
+      hd[k] = (int16_t)(v[r][2 * k] - v[r][2 * k + 1]);
+    }
+    for (int k = 0; k < hh / 2; k++) {
+      uint32_t x0 = uabs16(hs[2 * k]), x1 = uabs16(hs[2 * k + 1]);
+      uint32_t y0 = uabs16(hd[2 * k]), y1 = uabs16(hd[2 * k + 1]);
+      total += x0 > x1 ? x0 : x1;
+      total += y0 > y1 ? y0 : y1;
+    }
+  }
+  return total;
+}
+
+/* dsp.rs:72-88 */
+uint32_t o_satd(const uint8_t *a, int sa, const uint8_t *b, int sb, int w,
+                int h) {
+  uint32_t total = 0;
+  for (int by = 0; by < h; by += 4) {
+    const uint8_t *ao = a + by * sa, *bo = b + by * sb;
+    if (w == 16 || w == 8 || w == 4)
+      total += satd_strip(ao, sa, bo, sb, w);
+    else
+      for (int bx = 0; bx < w; bx += 4)
+        total += satd_strip(ao + bx, sa, bo + bx, sb, 4);
+  }
+  return total;
+}
+
+/* ------------------------------------------------------------ constants */
+
+/* slice.rs:52-68: sad[qp] = max(16, round(256*sqrt(0.85*2^((qp-12)/3)))) */
+uint32_t o_lambda(int qp) {
+  double l = 0.85 * pow(2.0, ((double)qp - 12.0) / 3.0);
+  double v = round(sqrt(l) * 256.0);
+  if (v < 16.0) v = 16.0;
+  return (uint32_t)v;
+}
+
+/* slice.rs:130-139 */
+uint32_t o_se_bits(int32_t v) {
+  uint32_t k = v > 0 ? 2u * (uint32_t)v - 1u : (uint32_t)(-2 * v);
+  uint32_t kp = k + 1;
+  int lz = __builtin_clz(kp);
+  return 2u * (uint32_t)(31 - lz) + 1u;
+}
+
+/* slice.rs:141-143 */
+uint32_t o_mv_bits(omv mv, omv mvp) {
+  return o_se_bits((int32_t)mv.x - (int32_t)mvp.x) +
+         o_se_bits((int32_t)mv.y - (int32_t)mvp.y);
+}
+
+/* slice.rs:145-147 */
+int16_t o_median(int16_t a, int16_t b, int16_t c) {
+  int16_t mx = a > b ? a : b, mn = a < b ? a : b;
+  int16_t t = mn > c ? mn : c;
+  return mx < t ? mx : t;
+}
+
+/* --------------------------------------------------------------- planes */
+
+/* mod.rs:842-873: rows 0..ah copy source row min(y, h-1); columns w..aw
+ * repeat the last copied pixel.  dst = first byte of the padded plane. */
+void o_import_plane(const uint8_t *src, int sstride, int w, int h,
+                    uint8_t *dst, int dstride, int pad, int aw, int ah) {
+  for (int y = 0; y < ah; y++) {
+    int sy = y < h - 1 ? y : h - 1;
+    uint8_t *o = dst + (size_t)(y + pad) * dstride + pad;
+    memcpy(o, src + (size_t)sy * sstride, (size_t)w);
+    uint8_t last = o[w - 1];
+    for (int x = w; x < aw; x++) o[x] = last;
+  }
+}
+
+/* picture.rs:285-316 + frame.rs:58-97: horizontal pad fill of every picture
+ * row, then the top pad rows copy the first row and the bottom pad rows copy
+ * the last one (both including their already-filled side pads). */
+void o_extend_plane(uint8_t *plane, int stride, int pad, int w, int h) {
+  for (int r = pad; r < pad + h; r++) {
+    uint8_t *row = plane + (size_t)r * stride;
+    uint8_t l = row[pad], rr = row[pad + w - 1];
+    memset(row, l, (size_t)pad);
+    memset(row + pad + w, rr, (size_t)(stride - pad - w));
+  }
+  for (int r = 0; r < pad; r++)
+    memcpy(plane + (size_t)r * stride, plane + (size_t)pad * stride,
+           (size_t)stride);
+  int last = pad + h - 1, rows = h + 2 * pad;
+  for (int r = last + 1; r < rows; r++)
+    memcpy(plane + (size_t)r * stride, plane + (size_t)last * stride,
+           (size_t)stride);
+}
+
+static inline uint8_t clip8(int32_t v) {
+  return (uint8_t)(v < 0 ? 0 : v > 255 ? 255 : v);
+}
+
+/* picture.rs:318-380 over rows 0..rows: H = horizontal 6-tap, V = vertical
+ * 6-tap, C = vertical 6-tap over the i16 horizontal intermediates.  Row and
+ * column indices clamp to the plane. */
+void o_hpel(const uint8_t *src, int stride, int rows, uint8_t *hout,
+            uint8_t *vout, uint8_t *cout) {
+  int16_t *hs = (int16_t *)malloc(sizeof(int16_t) * (size_t)stride * rows);
+#define AT(row, xx) \
+  ((int16_t)(row)[(xx) < 0 ? 0 : ((xx) > stride - 1 ? stride - 1 : (xx))])
+  for (int r = 0; r < rows; r++) {
+    const uint8_t *row = src + (size_t)r * stride;
+    int16_t *o = hs + (size_t)r * stride;
+    for (int x = 0; x < stride; x++) {
+      o[x] = (int16_t)(AT(row, x - 2) + AT(row, x + 3) -
+                       5 * (AT(row, x - 1) + AT(row, x + 2)) +
+                       20 * (AT(row, x) + AT(row, x + 1)));
+    }
+  }
+#undef AT
+  for (int r = 0; r < rows; r++) {
+    int rr[6];
+    for (int i = 0; i < 6; i++) {
+      int q = r - 2 + i;
+      rr[i] = q < 0 ? 0 : (q > rows - 1 ? rows - 1 : q);
+    }
+    for (int x = 0; x < stride; x++) {
+      size_t o = (size_t)r * stride + x;
+      hout[o] = clip8(((int32_t)hs[o] + 16) >> 5);
+      int32_t s = (int32_t)src[(size_t)rr[0] * stride + x] +
+                  (int32_t)src[(size_t)rr[5] * stride + x] -
+                  5 * ((int32_t)src[(size_t)rr[1] * stride + x] +
+                       (int32_t)src[(size_t)rr[4] * stride + x]) +
+                  20 * ((int32_t)src[(size_t)rr[2] * stride + x] +
+                        (int32_t)src[(size_t)rr[3] * stride + x]);
+      vout[o] = clip8((s + 16) >> 5);
+      int32_t c = (int32_t)hs[(size_t)rr[0] * stride + x] +
+                  (int32_t)hs[(size_t)rr[5] * stride + x] -
+                  5 * ((int32_t)hs[(size_t)rr[1] * stride + x] +
+                       (int32_t)hs[(size_t)rr[4] * stride + x]) +
+                  20 * ((int32_t)hs[(size_t)rr[2] * stride + x] +
+                        (int32_t)hs[(size_t)rr[3] * stride + x]);
+      cout[o] = clip8((c + 512) >> 10);
+    }
+  }
+  free(hs);
+}
+
+/* picture.rs:387-404: per q = (mv.y&3)*4 + (mv.x&3), two (plane, dx, dy)
+ * sources; plane 0 = full, 1 = H, 2 = V, 3 = C. */
+static const uint8_t QPEL_SRC[16][2][3] = {
+    {{0, 0, 0}, {0, 0, 0}}, {{0, 0, 0}, {1, 0, 0}}, {{1, 0, 0}, {1, 0, 0}},
+    {{1, 0, 0}, {0, 1, 0}}, {{0, 0, 0}, {2, 0, 0}}, {{1, 0, 0}, {2, 0, 0}},
+    {{1, 0, 0}, {3, 0, 0}}, {{1, 0, 0}, {2, 1, 0}}, {{2, 0, 0}, {2, 0, 0}},
+    {{2, 0, 0}, {3, 0, 0}}, {{3, 0, 0}, {3, 0, 0}}, {{3, 0, 0}, {2, 1, 0}},
+    {{2, 0, 0}, {0, 0, 1}}, {{2, 0, 0}, {1, 0, 1}}, {{3, 0, 0}, {1, 0, 1}},
+    {{2, 1, 0}, {1, 0, 1}},
+};
+
+static inline size_t vidx(const oref *r, long x, long y) {
+  return (size_t)((y + r->pad) * (long)r->stride + (x + r->pad));
+}
+
+/* picture.rs:415-449 */
+void o_mc_luma(const oref *r, int x, int y, omv mv, int w, int h,
+               uint8_t *out, int ostride) {
+  long xi = (long)x + (mv.x >> 2);
+  long yi = (long)y + (mv.y >> 2);
+  int q = (mv.y & 3) * 4 + (mv.x & 3);
+  const uint8_t *sa = QPEL_SRC[q][0], *sb = QPEL_SRC[q][1];
+  const uint8_t *a = r->pl[sa[0]] + vidx(r, xi + sa[1], yi + sa[2]);
+  if (sa[0] == sb[0] && sa[1] == sb[1] && sa[2] == sb[2]) {
+    for (int rr = 0; rr < h; rr++)
+      memcpy(out + rr * ostride, a + (size_t)rr * r->stride, (size_t)w);
+  } else {
+    const uint8_t *b = r->pl[sb[0]] + vidx(r, xi + sb[1], yi + sb[2]);
+    for (int rr = 0; rr < h; rr++)
+      for (int i = 0; i < w; i++)
+        out[rr * ostride + i] =
+            (uint8_t)(((uint16_t)a[(size_t)rr * r->stride + i] +
+                       (uint16_t)b[(size_t)rr * r->stride + i] + 1) >> 1);
+  }
+}
+
+/* picture.rs:451-487 */
+void o_mc_chroma(const uint8_t *u, const uint8_t *v, int cstride, int cpad,
+                 int x, int y, omv mv, int w, int h, uint8_t *out_u,
+                 uint8_t *out_v, int ostride) {
+  long xi = (long)x + (mv.x >> 3);
+  long yi = (long)y + (mv.y >> 3);
+  uint32_t fx = (uint32_t)(mv.x & 7), fy = (uint32_t)(mv.y & 7);
+  uint32_t w00 = (8 - fx) * (8 - fy), w01 = fx * (8 - fy);
+  uint32_t w10 = (8 - fx) * fy, w11 = fx * fy;
+  const uint8_t *pls[2] = {u, v};
+  uint8_t *outs[2] = {out_u, out_v};
+  for (int k = 0; k < 2; k++) {
+    const uint8_t *p = pls[k] + (size_t)((yi + cpad) * cstride + xi + cpad);
+    for (int r = 0; r < h; r++) {
+      const uint8_t *a = p + (size_t)r * cstride;
+      const uint8_t *b = a + cstride;
+      for (int i = 0; i < w; i++)
+        outs[k][r * ostride + i] =
+            (uint8_t)((w00 * a[i] + w01 * a[i + 1] + w10 * b[i] +
+                       w11 * b[i + 1] + 32) >> 6);
+    }
+  }
+}
+
+/* --------------------------------------------------------------- search */
+
+/* frame.rs:152 (need = 16*(my+1) + VMAX) with picture.rs:211-268: a view
+ * of a picture of coded height 16*mbh gives lim = need while need < h, else
+ * h + LUMA_PAD - 20. */
+long o_vlim(int my, int mbh) {
+  long need = 16L * (my + 1) + O_VMAX;
+  long h = 16L * mbh;
+  return need >= h ? h + O_LUMA_PAD - 20 : need;
+}
+
+/* slice.rs:496-507 */
+int o_mv_ok(const oparams *p, int x, int y, int w, int h, omv mv) {
+  long pw = (long)p->mbw * 16;
+  long m = O_LUMA_PAD - 20;
+  long x0 = (long)x + (mv.x >> 2);
+  long y0 = (long)y + (mv.y >> 2);
+  int ax = mv.x < 0 ? -(int)mv.x : mv.x;
+  int ay = mv.y < 0 ? -(int)mv.y : mv.y;
+  return ax < 2000 && ay < 2000 && x0 >= -m && y0 >= -m && x0 + w <= pw + m &&
+         y0 + h < p->vlim;
+}
+
+static inline int is_intra(int8_t k) {
+  return k == OK_I4x4 || k == OK_I8x8 || k == OK_I16x16;
+}
+
+/* mbinfo.rs:109-115 */
+static inline int8_t ref_at(const ombinfo *m, int r) {
+  int x = r & 3, y = r >> 2;
+  return m->ref_idx[(y >> 1) * 2 + (x >> 1)];
+}
+
+/* slice.rs:392-432 */
+int o_nb_part(const oparams *p, const ombinfo *mbs, const ombinfo *cur,
+              uint16_t mask, int mx, int my, int bx, int by, int8_t *ref,
+              omv *mv) {
+  static const omv ZERO = {0, 0};
+  if (bx >= 0 && bx < 4 && by >= 0 && by < 4) {
+    int r = by * 4 + bx;
+    if (mask & (1u << r)) {
+      *ref = ref_at(cur, r);
+      *mv = cur->mv[r];
+      return 1;
+    }
+    return 0;
+  }
+  int dx = bx < 0 ? -1 : (bx >= 4 ? 1 : 0);
+  int dy = by < 0 ? -1 : 0;
+  if (by >= 4 || (dy == 0 && dx == 1)) return 0;
+  int nx = mx + dx, ny = my + dy;
+  if (nx < 0 || nx >= p->mbw || ny < 0) return 0;
+  const ombinfo *m = &mbs[ny * p->mbw + nx];
+  int r = ((by + 4) % 4) * 4 + (bx + 4) % 4;
+  if (is_intra(m->kind)) {
+    *ref = -1;
+    *mv = ZERO;
+    return 1;
+  }
+  int8_t rf = ref_at(m, r);
+  *ref = rf;
+  *mv = rf >= 0 ? m->mv[r] : ZERO;
+  return 1;
+}
+
+/* slice.rs:434-482 */
+omv o_mvp(const oparams *p, const ombinfo *mbs, const ombinfo *cur,
+          uint16_t mask, int mx, int my, int x, int y, int w, int8_t rf,
+          int shape) {
+  int8_t ar = -1, br = -1, cr = -1;
+  omv am = {0, 0}, bm = {0, 0}, cm = {0, 0};
+  int ha = o_nb_part(p, mbs, cur, mask, mx, my, x - 1, y, &ar, &am);
+  int hb = o_nb_part(p, mbs, cur, mask, mx, my, x, y - 1, &br, &bm);
+  int hc = o_nb_part(p, mbs, cur, mask, mx, my, x + w, y - 1, &cr, &cm);
+  if (!hc) hc = o_nb_part(p, mbs, cur, mask, mx, my, x - 1, y - 1, &cr, &cm);
+  if (!hb && !hc && ha) {
+    hb = hc = 1;
+    br = cr = ar;
+    bm = cm = am;
+  }
+  if (!ha) { ar = -1; am.x = am.y = 0; }
+  if (!hb) { br = -1; bm.x = bm.y = 0; }
+  if (!hc) { cr = -1; cm.x = cm.y = 0; }
+  switch (shape) {
+  case 1: if (br == rf) return bm; break;
+  case 2: if (ar == rf) return am; break;
+  case 3: if (ar == rf) return am; break;
+  case 4: if (cr == rf) return cm; break;
+  default: break;
+  }
+  int n = (ar == rf) + (br == rf) + (cr == rf);
+  if (n == 1) {
+    if (ar == rf) return am;
+    if (br == rf) return bm;
+    return cm;
+  }
+  omv o;
+  o.x = o_median(am.x, bm.x, cm.x);
+  o.y = o_median(am.y, bm.y, cm.y);
+  return o;
+}
+
+/* slice.rs:484-494 */
+omv o_pskip_mv(const oparams *p, const ombinfo *mbs, int mx, int my) {
+  ombinfo cur;
+  memset(&cur, 0, sizeof cur);
+  for (int i = 0; i < 4; i++) cur.ref_idx[i] = -1;
+  omv z = {0, 0};
+  int8_t ar, br;
+  omv am, bm;
+  int ha = o_nb_part(p, mbs, &cur, 0, mx, my, -1, 0, &ar, &am);
+  int hb = o_nb_part(p, mbs, &cur, 0, mx, my, 0, -1, &br, &bm);
+  if (!ha || !hb) return z;
+  if (ar == 0 && am.x == 0 && am.y == 0) return z;
+  if (br == 0 && bm.x == 0 && bm.y == 0) return z;
+  return o_mvp(p, mbs, &cur, 0, mx, my, 0, 0, 4, 0, 0);
+}
+
+static inline omv mvnew(int32_t x, int32_t y) {
+  omv m;
+  m.x = (int16_t)x;
+  m.y = (int16_t)y;
+  return m;
+}
+static inline int mveq(omv a, omv b) { return a.x == b.x && a.y == b.y; }
+
+typedef struct {
+  const oparams *p;
+  const oref *r;
+  int bx, by, w, h;
+  const uint8_t *src;
+  int sstride;
+  omv mvp;
+  uint32_t lam;
+  ocounters *ctr;
+} sctx;
+
+/* slice.rs:1605-1616 (fcost) */
+static uint32_t fcost(const sctx *s, omv mv) {
+  const uint8_t *pl =
+      s->r->pl[0] + vidx(s->r, (long)s->bx + (mv.x >> 2), (long)s->by + (mv.y >> 2));
+  uint32_t d = s->w == 16 ? o_sad16(s->src, s->sstride, pl, s->r->stride, s->h)
+                          : o_sad(s->src, s->sstride, pl, s->r->stride, s->w, s->h);
+  if (s->ctr) {
+    s->ctr->sad_px += (uint64_t)s->w * s->h;
+    s->ctr->sad_evals++;
+  }
+  return d + ((s->lam * o_mv_bits(mv, s->mvp)) >> 8);
+}
+
+/* slice.rs:1684-1687 (scost) */
+static uint32_t scost(const sctx *s, omv mv) {
+  uint8_t buf[256];
+  o_mc_luma(s->r, s->bx, s->by, mv, s->w, s->h, buf, 16);
+  if (s->ctr) {
+    s->ctr->satd_px += (uint64_t)s->w * s->h;
+    s->ctr->satd_evals++;
+  }
+  return o_satd(s->src, s->sstride, buf, 16, s->w, s->h) +
+         ((s->lam * o_mv_bits(mv, s->mvp)) >> 8);
+}
+
+static const int SQ8[8][2] = {{-1, 0}, {1, 0},  {0, -1}, {0, 1},
+                              {-1, -1}, {1, -1}, {-1, 1}, {1, 1}};
+
+/* slice.rs:1590-1723 */
+omv o_search(const oparams *p, const oref *r, int bx, int by, int w, int h,
+             const uint8_t *src, int sstride, omv mvp, const omv *cands,
+             int ncands, uint32_t lam, uint32_t *cost_out, ocounters *ctr) {
+  sctx s = {p, r, bx, by, w, h, src, sstride, mvp, lam, ctr};
+  if (ctr) ctr->searches++;
+  omv best = mvnew(((int32_t)mvp.x + 2) & ~3, ((int32_t)mvp.y + 2) & ~3);
+  if (!o_mv_ok(p, bx, by, w, h, best)) best = mvnew(0, 0);
+  uint32_t bcost = fcost(&s, best);
+  for (int i = 0; i < ncands; i++) {
+    omv c = mvnew(((int32_t)cands[i].x + 2) & ~3, ((int32_t)cands[i].y + 2) & ~3);
+    if (!mveq(c, best) && o_mv_ok(p, bx, by, w, h, c)) {
+      uint32_t k = fcost(&s, c);
+      if (k < bcost) {
+        bcost = k;
+        best = c;
+      }
+    }
+  }
+  int range = p->me_range;
+  omv centre0 = best;
+  static const int HEX[6][2] = {{-2, 0}, {2, 0}, {-1, -2}, {1, -2}, {-1, 2}, {1, 2}};
+  for (int it = 0; it < range; it++) {
+    int moved = 0;
+    omv c = best;
+    for (int j = 0; j < 6; j++) {
+      omv m = mvnew((int32_t)c.x + HEX[j][0] * 4, (int32_t)c.y + HEX[j][1] * 4);
+      int ddx = (int32_t)m.x - (int32_t)centre0.x;
+      int ddy = (int32_t)m.y - (int32_t)centre0.y;
+      if (((ddx < 0 ? -ddx : ddx) >> 2) > range ||
+          ((ddy < 0 ? -ddy : ddy) >> 2) > range ||
+          !o_mv_ok(p, bx, by, w, h, m))
+        continue;
+      uint32_t k = fcost(&s, m);
+      if (k < bcost) {
+        bcost = k;
+        best = m;
+        moved = 1;
+      }
+    }
+    if (!moved) break;
+  }
+  for (int it = 0; it < 2; it++) {
+    omv c = best;
+    for (int j = 0; j < 8; j++) {
+      omv m = mvnew((int32_t)c.x + SQ8[j][0] * 4, (int32_t)c.y + SQ8[j][1] * 4);
+      if (!o_mv_ok(p, bx, by, w, h, m)) continue;
+      uint32_t k = fcost(&s, m);
+      if (k < bcost) {
+        bcost = k;
+        best = m;
+      }
+    }
+    if (mveq(best, c)) break;
+  }
+  uint32_t bs = scost(&s, best);
+  for (int step = 2; step >= 1; step--) {
+    int rounds = step == 1 ? 1 + p->subpel_rounds : 1;
+    for (int it = 0; it < rounds; it++) {
+      omv c = best;
+      for (int j = 0; j < 8; j++) {
+        omv m = mvnew((int32_t)c.x + SQ8[j][0] * step, (int32_t)c.y + SQ8[j][1] * step);
+        if (!o_mv_ok(p, bx, by, w, h, m)) continue;
+        uint32_t k = scost(&s, m);
+        if (k < bs) {
+          bs = k;
+          best = m;
+        }
+      }
+      if (mveq(best, c)) break;
+    }
+  }
+  if (cost_out) *cost_out = bs;
+  return best;
+}
+
+/* mbinfo.rs:64-71 */
+static void part_rect(int part, int i, int *x, int *y, int *w, int *h) {
+  switch (part) {
+  case OP_16x16: *x = 0; *y = 0; *w = 16; *h = 16; break;
+  case OP_16x8: *x = 0; *y = 8 * i; *w = 16; *h = 8; break;
+  case OP_8x16: *x = 8 * i; *y = 0; *w = 8; *h = 16; break;
+  default: *x = 8 * (i & 1); *y = 8 * (i >> 1); *w = 8; *h = 8; break;
+  }
+}
+static int part_count(int part) {
+  return part == OP_16x16 ? 1 : (part == OP_8x8 ? 4 : 2);
+}
+
+/* slice.rs:1787-1798 */
+static uint16_t mask_of(int part, int upto) {
+  uint16_t m = 0;
+  for (int q = 0; q < upto; q++) {
+    int x, y, w, h;
+    part_rect(part, q, &x, &y, &w, &h);
+    for (int by = y / 4; by < (y + h) / 4; by++)
+      for (int bx = x / 4; bx < (x + w) / 4; bx++) m |= (uint16_t)(1u << (by * 4 + bx));
+  }
+  return m;
+}
+
+/* slice.rs:1800-1808 */
+static int shape_of(int part, int q) {
+  if (part == OP_16x8) return q == 0 ? 1 : 2;
+  if (part == OP_8x16) return q == 0 ? 3 : 4;
+  return 0;
+}
+
+/* slice.rs:1810-1868 (list 0) */
+uint32_t o_search_part(const oparams *p, const ombinfo *mbs, int mx, int my,
+                       const oref *r, int part, const uint8_t *src16,
+                       uint32_t lam, omv seed, omv mvs[4], omv mvps[4],
+                       ocounters *ctr) {
+  int px = mx * 16, py = my * 16;
+  ombinfo info;
+  memset(&info, 0, sizeof info);
+  info.kind = OK_NONE;
+  for (int i = 0; i < 4; i++) info.ref_idx[i] = -1;
+  for (int i = 0; i < 4; i++) mvs[i] = mvps[i] = mvnew(0, 0);
+  uint32_t total = 0;
+  for (int q = 0; q < part_count(part); q++) {
+    int x, y, w, h;
+    part_rect(part, q, &x, &y, &w, &h);
+    uint16_t mask = mask_of(part, q);
+    omv mvp = o_mvp(p, mbs, &info, mask, mx, my, x / 4, y / 4, w / 4, 0,
+                    shape_of(part, q));
+    omv cands[2] = {seed, mvs[0]};
+    uint32_t cost;
+    omv mv = o_search(p, r, px + x, py + y, w, h, src16 + y * 16 + x, 16, mvp,
+                      cands, 2, lam, &cost, ctr);
+    mvs[q] = mv;
+    mvps[q] = mvp;
+    total += cost;
+    for (int by = y / 4; by < (y + h) / 4; by++)
+      for (int bx = x / 4; bx < (x + w) / 4; bx++) info.mv[by * 4 + bx] = mv;
+    for (int by8 = y / 8; by8 < (y + h + 7) / 8; by8++)
+      for (int bx8 = x / 8; bx8 < (x + w + 7) / 8; bx8++)
+        info.ref_idx[by8 * 2 + bx8] = 0;
+  }
+  return total;
+}
+
+/* slice.rs:1884-1945 without early skip / intra / RD, and the commit of
+ * cand_inter_info (slice.rs:1751-1785) with refs [[0;4],[-1;4]]. */
+void o_analyse_mb(const oparams *p, ombinfo *mbs, int mx, int my,
+                  const oref *r, const uint8_t *src_plane, int sstride,
+                  int spad, uint32_t lam, omb_out *out, ocounters *ctr) {
+  uint8_t src16[256];
+  int px = mx * 16, py = my * 16;
+  for (int rr = 0; rr < 16; rr++)
+    memcpy(src16 + rr * 16,
+           src_plane + (size_t)(py + rr + spad) * sstride + px + spad, 16);
+  omv skip = o_pskip_mv(p, mbs, mx, my);
+  omv mv16[4], mvp16[4];
+  uint32_t cost16 = o_search_part(p, mbs, mx, my, r, OP_16x16, src16, lam,
+                                  skip, mv16, mvp16, ctr);
+  int best_part = OP_16x16;
+  omv best_mvs[4], best_mvps[4];
+  memcpy(best_mvs, mv16, sizeof mv16);
+  memcpy(best_mvps, mvp16, sizeof mvp16);
+  uint64_t best_cost = (uint64_t)cost16 + (uint64_t)(lam >> 8);
+  out->cost[0] = cost16;
+  out->cost[1] = out->cost[2] = out->cost[3] = 0xFFFFFFFFu;
+  if (p->partitions && (uint64_t)cost16 > 256u * 4u) {
+    static const int parts[3] = {OP_16x8, OP_8x16, OP_8x8};
+    for (int k = 0; k < 3; k++) {
+      omv m[4], pp[4];
+      uint32_t c = o_search_part(p, mbs, mx, my, r, parts[k], src16, lam,
+                                 mv16[0], m, pp, ctr);
+      out->cost[parts[k]] = c;
+      uint32_t extra = parts[k] == OP_8x8 ? 10 : 3;
+      uint64_t cc = (uint64_t)c + (uint64_t)((lam * extra) >> 8);
+      if (cc < best_cost) {
+        best_cost = cc;
+        best_part = parts[k];
+        memcpy(best_mvs, m, sizeof m);
+        memcpy(best_mvps, pp, sizeof pp);
+      }
+    }
+  }
+  ombinfo *mi = &mbs[my * p->mbw + mx];
+  memset(mi, 0, sizeof *mi);
+  mi->kind = OK_PINTER;
+  for (int q = 0; q < part_count(best_part); q++) {
+    int x, y, w, h;
+    part_rect(best_part, q, &x, &y, &w, &h);
+    for (int by = y / 4; by < (y + h) / 4; by++)
+      for (int bx = x / 4; bx < (x + w) / 4; bx++) mi->mv[by * 4 + bx] = best_mvs[q];
+    for (int by8 = y / 8; by8 < (y + h + 7) / 8; by8++)
+      for (int bx8 = x / 8; bx8 < (x + w + 7) / 8; bx8++)
+        mi->ref_idx[by8 * 2 + bx8] = 0;
+  }
+  memcpy(out->mv, mi->mv, sizeof mi->mv);
+  memcpy(out->mvs, best_mvs, sizeof best_mvs);
+  memcpy(out->mvps, best_mvps, sizeof best_mvps);
+  out->best_cost = (uint32_t)best_cost;
+  out->skip_mv = skip;
+  out->mv16 = mv16[0];
+  out->part = (uint8_t)best_part;
+  out->pad_[0] = out->pad_[1] = out->pad_[2] = 0;
+  if (ctr) ctr->mbs++;
+}
+
+void o_search_frame(int mbw, int mbh, int me_range, int subpel_rounds,
+                    int partitions, uint32_t lam, const uint8_t *src,
+                    int sstride, int spad, const oref *r, omb_out *out,
+                    ocounters *ctr) {
+  ombinfo *mbs = (ombinfo *)calloc((size_t)mbw * mbh, sizeof(ombinfo));
+  oparams p = {mbw, mbh, me_range, subpel_rounds, partitions, 0};
+  for (int my = 0; my < mbh; my++) {
+    p.vlim = o_vlim(my, mbh);
+    for (int mx = 0; mx < mbw; mx++)
+      o_analyse_mb(&p, mbs, mx, my, r, src, sstride, spad, lam,
+                   &out[my * mbw + mx], ctr);
+  }
+  free(mbs);
+}
+
+/* ------------------------------------------------- frame-thread driver */
+
+typedef struct {
+  const ojob *jobs;
+  int njobs;
+  int next;
+  pthread_mutex_t mu;
+  ocounters total;
+} pool_t;
+
+static void *worker(void *arg) {
+  pool_t *pl = (pool_t *)arg;
+  ocounters c;
+  memset(&c, 0, sizeof c);
+  for (;;) {
+    pthread_mutex_lock(&pl->mu);
+    int j = pl->next++;
+    pthread_mutex_unlock(&pl->mu);
+    if (j >= pl->njobs) break;
+    const ojob *jb = &pl->jobs[j];
+    int stride = 16 * jb->mbw + 2 * O_LUMA_PAD;
+    int rows = 16 * jb->mbh + 2 * O_LUMA_PAD;
+    size_t n = (size_t)stride * rows;
+    uint8_t *hp = (uint8_t *)malloc(3 * n);
+    o_hpel(jb->ref, stride, rows, hp, hp + n, hp + 2 * n);
+    oref r = {{jb->ref, hp, hp + n, hp + 2 * n}, stride, O_LUMA_PAD};
+    o_search_frame(jb->mbw, jb->mbh, jb->me_range, jb->subpel_rounds,
+                   jb->partitions, o_lambda(jb->qp), jb->src, stride,
+                   O_LUMA_PAD, &r, jb->out, &c);
+    free(hp);
+  }
+  pthread_mutex_lock(&pl->mu);
+  pl->total.sad_px += c.sad_px;
+  pl->total.satd_px += c.satd_px;
+  pl->total.sad_evals += c.sad_evals;
+  pl->total.satd_evals += c.satd_evals;
+  pl->total.searches += c.searches;
+  pl->total.mbs += c.mbs;
+  pthread_mutex_unlock(&pl->mu);
+  return NULL;
+}
+
+double o_search_jobs_mt(const ojob *jobs, int njobs, int threads,
+                        ocounters *ctr) {
+  if (threads < 1) threads = 1;
+  pool_t pl;
+  memset(&pl, 0, sizeof pl);
+  pl.jobs = jobs;
+  pl.njobs = njobs;
+  pthread_mutex_init(&pl.mu, NULL);
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)threads);
+  for (int i = 0; i < threads; i++) pthread_create(&th[i], NULL, worker, &pl);
+  for (int i = 0; i < threads; i++) pthread_join(th[i], NULL);
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  free(th);
+  pthread_mutex_destroy(&pl.mu);
+  if (ctr) *ctr = pl.total;
+  return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}

@@ -1,0 +1,175 @@
+/*
+ * pcme_oracle.h -- CPU oracle for the H.264 P-frame motion search.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the
+ * reference encoder's inter-prediction search (pito-codecs 5.3.0,
+ * src/h264/enc/{dsp,picture,mbinfo,slice,frame}.rs).  Only the tests, the
+ * smoke check and the benchmark's CPU-baseline leg may link or call it, and
+ * only as the checker / CPU baseline -- never as the product path.
+ *
+ * Every function cites the reference lines it restates.  What pins each
+ * function (golden vectors, closed forms, invariants) is listed in the
+ * header comment of pcme_oracle.c.
+ */
+#ifndef PCME_ORACLE_H
+#define PCME_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* picture.rs:7-8 */
+#define O_LUMA_PAD 64
+#define O_CHROMA_PAD 32
+/* frame.rs:13 */
+#define O_VMAX 96
+
+/* mbinfo.rs:1-16 -- quarter-pel motion vector */
+typedef struct { int16_t x, y; } omv;
+
+/* mbinfo.rs:18-45 (subset used by the P-frame predictor) */
+enum { OK_NONE = 0, OK_I4x4, OK_I8x8, OK_I16x16, OK_PSKIP, OK_PINTER };
+/* mbinfo.rs:47-72 */
+enum { OP_16x16 = 0, OP_16x8 = 1, OP_8x16 = 2, OP_8x8 = 3 };
+
+/* mbinfo.rs:74-107, list 0 only (P slices use one list) */
+typedef struct {
+  int8_t kind;
+  int8_t ref_idx[4]; /* per 8x8, raster */
+  omv mv[16];        /* per 4x4, raster */
+} ombinfo;
+
+/* Reference view (picture.rs:116-138): four planes of identical padded
+ * layout: full-pel luma, then the H ('b'), V ('h') and C ('j') half-pel
+ * planes.  Coordinates are picture coordinates; (0,0) sits at
+ * data + pad*stride + pad. */
+typedef struct {
+  const uint8_t *pl[4];
+  int stride;
+  int pad;
+} oref;
+
+/* Search parameters (slice.rs:21-34 EncParams subset + SliceEnc::vlim) */
+typedef struct {
+  int mbw, mbh;
+  int me_range;
+  int subpel_rounds;
+  int partitions;
+  long vlim;
+} oparams;
+
+/* Work counters (evaluated candidates per phase, weighted by pixels). */
+typedef struct {
+  uint64_t sad_px;     /* pixels through SAD */
+  uint64_t satd_px;    /* pixels through MC + SATD */
+  uint64_t sad_evals;  /* SAD candidate evaluations */
+  uint64_t satd_evals; /* SATD candidate evaluations */
+  uint64_t searches;   /* calls of search() */
+  uint64_t mbs;        /* macroblocks analysed */
+} ocounters;
+
+/* Per-MB result of the ME-only field analysis. */
+typedef struct {
+  omv mv[16];      /* committed MbInfo.mv[0] */
+  omv mvs[4];      /* best_mvs */
+  omv mvps[4];     /* best_mvps */
+  uint32_t cost[4];/* search_part totals per mode; 0xFFFFFFFF = not searched */
+  uint32_t best_cost;
+  omv skip_mv;
+  omv mv16;        /* 16x16 search result, seed of the other modes */
+  uint8_t part;
+  uint8_t pad_[3];
+} omb_out;
+
+/* ---- dsp.rs ---- */
+uint32_t o_sad(const uint8_t *a, int sa, const uint8_t *b, int sb, int w, int h);
+uint32_t o_sad16(const uint8_t *a, int sa, const uint8_t *b, int sb, int h);
+uint32_t o_satd(const uint8_t *a, int sa, const uint8_t *b, int sb, int w, int h);
+
+/* ---- slice.rs:52-68, 130-147 ---- */
+uint32_t o_lambda(int qp);
+uint32_t o_se_bits(int32_t v);
+uint32_t o_mv_bits(omv mv, omv mvp);
+int16_t o_median(int16_t a, int16_t b, int16_t c);
+
+/* ---- picture.rs / mod.rs / frame.rs: planes ---- */
+/* Import (mod.rs:842-873): copy a w x h plane into the MB-aligned area
+ * (aw x ah) of a padded plane, replicating the right column and bottom row.
+ * dst points at the padded plane's first byte. */
+void o_import_plane(const uint8_t *src, int sstride, int w, int h,
+                    uint8_t *dst, int dstride, int pad, int aw, int ah);
+/* Edge extension of the pad (frame.rs:58-97, picture.rs:285-316). */
+void o_extend_plane(uint8_t *plane, int stride, int pad, int w, int h);
+/* Half-pel planes over the whole padded plane (picture.rs:318-380). */
+void o_hpel(const uint8_t *src, int stride, int rows,
+            uint8_t *hout, uint8_t *vout, uint8_t *cout);
+/* picture.rs:415-449 */
+void o_mc_luma(const oref *r, int x, int y, omv mv, int w, int h,
+               uint8_t *out, int ostride);
+/* picture.rs:451-487; u/v planes padded by pad */
+void o_mc_chroma(const uint8_t *u, const uint8_t *v, int cstride, int cpad,
+                 int x, int y, omv mv, int w, int h,
+                 uint8_t *out_u, uint8_t *out_v, int ostride);
+
+/* ---- slice.rs: search machinery ---- */
+/* vlim rule: frame.rs:13,152-156 with picture.rs:211-268 */
+long o_vlim(int my, int mbh);
+int o_mv_ok(const oparams *p, int x, int y, int w, int h, omv mv);
+/* nb_part (slice.rs:392-432).  mbs = the current picture's committed
+ * MbInfo (mbw*mbh, raster), cur = partial info of the MB being analysed.
+ * Returns 1 if available and writes (ref, mv). */
+int o_nb_part(const oparams *p, const ombinfo *mbs, const ombinfo *cur,
+              uint16_t mask, int mx, int my, int bx, int by,
+              int8_t *ref, omv *mv);
+/* mvp (slice.rs:434-482); shape: 0 Normal 1 Top16x8 2 Bottom16x8
+ * 3 Left8x16 4 Right8x16 */
+omv o_mvp(const oparams *p, const ombinfo *mbs, const ombinfo *cur,
+          uint16_t mask, int mx, int my, int x, int y, int w, int8_t rf,
+          int shape);
+/* pskip_mv (slice.rs:484-494) */
+omv o_pskip_mv(const oparams *p, const ombinfo *mbs, int mx, int my);
+/* search (slice.rs:1590-1723) */
+omv o_search(const oparams *p, const oref *r, int bx, int by, int w, int h,
+             const uint8_t *src, int sstride, omv mvp, const omv *cands,
+             int ncands, uint32_t lam, uint32_t *cost, ocounters *ctr);
+/* search_part (slice.rs:1810-1868) */
+uint32_t o_search_part(const oparams *p, const ombinfo *mbs, int mx, int my,
+                       const oref *r, int part, const uint8_t *src16,
+                       uint32_t lam, omv seed, omv mvs[4], omv mvps[4],
+                       ocounters *ctr);
+/* ME-only analysis of one MB: pskip_mv, search_part(16x16), partition
+ * choice (slice.rs:1884-1945) and the commit of cand_inter_info
+ * (slice.rs:1751-1785, 1980-1986) into mbs[my*mbw+mx]. */
+void o_analyse_mb(const oparams *p, ombinfo *mbs, int mx, int my,
+                  const oref *r, const uint8_t *src_plane, int sstride,
+                  int spad, uint32_t lam, omb_out *out, ocounters *ctr);
+/* ME-only field over one frame, raster order.  src = padded source luma,
+ * r = reference view whose planes are complete, vlim per row from o_vlim. */
+void o_search_frame(int mbw, int mbh, int me_range, int subpel_rounds,
+                    int partitions, uint32_t lam, const uint8_t *src,
+                    int sstride, int spad, const oref *r, omb_out *out,
+                    ocounters *ctr);
+
+/* Multi-frame CPU driver (frame threads, as enc/mod.rs:189-263): for each
+ * job, build the reference's half-pel planes then search the frame.  All
+ * planes are padded luma of the given geometry.  Returns wall seconds. */
+typedef struct {
+  const uint8_t *src;   /* padded source luma */
+  const uint8_t *ref;   /* padded reference luma (edges extended) */
+  omb_out *out;         /* mbw*mbh */
+  int mbw, mbh, me_range, subpel_rounds, partitions, qp;
+} ojob;
+double o_search_jobs_mt(const ojob *jobs, int njobs, int threads,
+                        ocounters *ctr);
+
+/* Scratch size in bytes for o_hpel row buffers (none needed); kept for ABI
+ * symmetry with the product library. */
+int o_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,384 @@
+// pcme_api.hip -- host side of libpcme: the extern "C" entry points of
+// include/pcme.h, argument checks, device workspaces and kernel launches.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "pcme_internal.h"
+
+namespace pcme {
+
+static thread_local char g_err[512] = "";
+
+static int fail(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(call)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail(PCME_EHIP, "%s failed: %s", #call, hipGetErrorString(e_));         \
+  } while (0)
+
+// Per-device state: workspaces of the search, grown on demand.
+struct Dev {
+  int id = -1;
+  std::mutex mu;
+  int resident_waves = 0;
+  uint32_t *ctl = nullptr;          // [0] ticket counter, [1] stall flag
+  DevJob *djobs = nullptr;
+  size_t djobs_cap = 0;
+  int2 *dtickets = nullptr;
+  size_t dtickets_cap = 0;
+  uint32_t *xchg = nullptr;
+  size_t xchg_cap = 0;              // words
+  bool xchg_dirty = true;
+  pcme_search_job *dsj = nullptr;
+  size_t dsj_cap = 0;
+  unsigned char *hstage = nullptr;  // pinned upload staging
+  size_t hstage_cap = 0;
+  hipEvent_t stage_ev = nullptr;
+  bool stage_pending = false;
+  hipEvent_t done_ev = nullptr;     // end of the last search launch
+  bool done_recorded = false;
+  uint32_t tag = 0;
+  std::vector<DevJob> last_jobs;
+  std::vector<int2> last_tickets;
+};
+
+static std::mutex g_mu;
+static std::map<int, Dev *> g_devs;
+
+static bool is_gfx950(int d) {
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, d) != hipSuccess) return false;
+  return strncmp(p.gcnArchName, "gfx950", 6) == 0;
+}
+
+static int get_dev(Dev **out) {
+  int d = 0;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return fail(PCME_ENODEV, "no HIP device visible");
+  HIPCHK(hipGetDevice(&d));
+  if (!is_gfx950(d)) return fail(PCME_ENODEV, "device %d is not a gfx950 (MI355X)", d);
+  std::lock_guard<std::mutex> g(g_mu);
+  auto it = g_devs.find(d);
+  if (it != g_devs.end()) {
+    *out = it->second;
+    return PCME_OK;
+  }
+  Dev *dv = new Dev();
+  dv->id = d;
+  hipDeviceProp_t p;
+  HIPCHK(hipGetDeviceProperties(&p, d));
+  int per_cu = 0;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, search_frames_kernel, 64, 0));
+  if (per_cu < 1) per_cu = 1;
+  dv->resident_waves = per_cu * p.multiProcessorCount;
+  HIPCHK(hipMalloc(&dv->ctl, 64));
+  HIPCHK(hipMemset(dv->ctl, 0, 64));
+  HIPCHK(hipEventCreateWithFlags(&dv->stage_ev, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&dv->done_ev, hipEventDisableTiming));
+  g_devs[d] = dv;
+  *out = dv;
+  return PCME_OK;
+}
+
+template <typename T>
+static int grow(T **p, size_t *cap, size_t n) {
+  if (*cap >= n && *p) return PCME_OK;
+  size_t nc = std::max(n, *cap * 2);
+  if (*p) {
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipFree(*p));
+    *p = nullptr;
+    *cap = 0;
+  }
+  if (hipMalloc((void **)p, nc * sizeof(T)) != hipSuccess)
+    return fail(PCME_ENOMEM, "hipMalloc of %zu bytes failed", nc * sizeof(T));
+  *cap = nc;
+  return PCME_OK;
+}
+
+// Copy `bytes` from host memory to device memory on `stream` through the
+// pinned staging buffer (waits for the previous staged copy first).
+static int upload(Dev *dv, void *dst, const void *src, size_t bytes, hipStream_t stream) {
+  if (dv->stage_pending) {
+    HIPCHK(hipEventSynchronize(dv->stage_ev));
+    dv->stage_pending = false;
+  }
+  if (dv->hstage_cap < bytes) {
+    if (dv->hstage) HIPCHK(hipHostFree(dv->hstage));
+    dv->hstage = nullptr;
+    size_t nc = std::max(bytes, dv->hstage_cap * 2);
+    HIPCHK(hipHostMalloc((void **)&dv->hstage, nc, hipHostMallocDefault));
+    dv->hstage_cap = nc;
+  }
+  memcpy(dv->hstage, src, bytes);
+  HIPCHK(hipMemcpyAsync(dst, dv->hstage, bytes, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipEventRecord(dv->stage_ev, stream));
+  dv->stage_pending = true;
+  return PCME_OK;
+}
+
+// slice.rs:52-68
+static uint32_t lambda_of(int qp) {
+  double l = 0.85 * pow(2.0, ((double)qp - 12.0) / 3.0);
+  double v = round(sqrt(l) * 256.0);
+  if (v < 16.0) v = 16.0;
+  return (uint32_t)v;
+}
+
+static int check_params(const pcme_params &p) {
+  if (p.mbw < 1 || p.mbh < 1 || p.mbw > 4096 || p.mbh > 4096)
+    return fail(PCME_EINVAL, "mbw/mbh out of range (%d x %d)", p.mbw, p.mbh);
+  if (p.me_range < 1 || p.me_range > 1024) return fail(PCME_EINVAL, "me_range %d not in [1,1024]", p.me_range);
+  if (p.subpel_rounds < 0 || p.subpel_rounds > 8)
+    return fail(PCME_EINVAL, "subpel_rounds %d not in [0,8]", p.subpel_rounds);
+  if (p.qp < 0 || p.qp > 51) return fail(PCME_EINVAL, "qp %d not in [0,51]", p.qp);
+  return PCME_OK;
+}
+
+static const uint32_t kSpinLimit = 1u << 22;
+
+static int search_frames_impl(const pcme_frame_job *jobs, int njobs, hipStream_t stream, Dev **dvo) {
+  if (njobs < 0 || (njobs > 0 && !jobs)) return fail(PCME_EINVAL, "bad job array");
+  if (njobs == 0) return PCME_OK;
+  int rc;
+  for (int i = 0; i < njobs; i++) {
+    if ((rc = check_params(jobs[i].prm))) return rc;
+    if (!jobs[i].src || !jobs[i].ref || !jobs[i].ref_h || !jobs[i].ref_v || !jobs[i].ref_c || !jobs[i].out)
+      return fail(PCME_EINVAL, "job %d has a null pointer", i);
+    if (((uintptr_t)jobs[i].src & 3u) != 0) return fail(PCME_EINVAL, "job %d: src not 4-byte aligned", i);
+  }
+  Dev *dv = nullptr;
+  if ((rc = get_dev(&dv))) return rc;
+  *dvo = dv;
+  std::lock_guard<std::mutex> g(dv->mu);
+  std::vector<DevJob> dj((size_t)njobs);
+  size_t words = 0;
+  int maxh = 0;
+  for (int i = 0; i < njobs; i++) {
+    const pcme_frame_job &J = jobs[i];
+    DevJob &d = dj[(size_t)i];
+    d.src = J.src;
+    d.pl[0] = J.ref; d.pl[1] = J.ref_h; d.pl[2] = J.ref_v; d.pl[3] = J.ref_c;
+    d.out = J.out;
+    d.xchg = nullptr;
+    d.mbw = J.prm.mbw;
+    d.mbh = J.prm.mbh;
+    d.stride = 16 * J.prm.mbw + 2 * PCME_PAD;
+    d.me_range = J.prm.me_range;
+    d.subpel_rounds = J.prm.subpel_rounds;
+    d.partitions = J.prm.partitions ? 1 : 0;
+    d.lam = lambda_of(J.prm.qp);
+    d.pad_ = 0;
+    words += (size_t)J.prm.mbw * J.prm.mbh * 4;
+    maxh = std::max(maxh, J.prm.mbh);
+  }
+  {
+    uint32_t *old = dv->xchg;
+    if ((rc = grow(&dv->xchg, &dv->xchg_cap, words))) return rc;
+    if (dv->xchg != old) dv->xchg_dirty = true;  // fresh memory: clear before use
+  }
+  size_t off = 0;
+  for (int i = 0; i < njobs; i++) {
+    dj[(size_t)i].xchg = dv->xchg + off;
+    off += (size_t)dj[(size_t)i].mbw * dj[(size_t)i].mbh * 4;
+  }
+  // Tickets in (row, job) order: a row's dependency (row-1 of the same job)
+  // always has a smaller ticket.
+  std::vector<int2> tk;
+  tk.reserve(words / 4);
+  for (int y = 0; y < maxh; y++)
+    for (int i = 0; i < njobs; i++)
+      if (y < dj[(size_t)i].mbh) tk.push_back(make_int2(i, y));
+  const bool same = dv->last_jobs.size() == dj.size() && dv->last_tickets.size() == tk.size() &&
+                    memcmp(dv->last_jobs.data(), dj.data(), dj.size() * sizeof(DevJob)) == 0 &&
+                    memcmp(dv->last_tickets.data(), tk.data(), tk.size() * sizeof(int2)) == 0;
+  if (!same) {
+    if ((rc = grow(&dv->djobs, &dv->djobs_cap, dj.size()))) return rc;
+    if ((rc = grow(&dv->dtickets, &dv->dtickets_cap, tk.size()))) return rc;
+    if ((rc = upload(dv, dv->djobs, dj.data(), dj.size() * sizeof(DevJob), stream))) return rc;
+    // second upload waits for the first staged copy to finish
+    if ((rc = upload(dv, dv->dtickets, tk.data(), tk.size() * sizeof(int2), stream))) return rc;
+    dv->last_jobs = dj;
+    dv->last_tickets = tk;
+  }
+  dv->tag++;
+  if (dv->tag > 255u) {
+    dv->tag = 1;
+    dv->xchg_dirty = true;
+  }
+  if (dv->xchg_dirty) dv->tag = 1;
+  // Launches share the ticket counter and the exchange words: order this one
+  // after the previous one even when they come on different streams.
+  if (dv->done_recorded) HIPCHK(hipStreamWaitEvent(stream, dv->done_ev, 0));
+  {
+    size_t nw = dv->xchg_dirty ? dv->xchg_cap : 0;
+    int blocks = dv->xchg_dirty ? 1024 : 1;
+    hipLaunchKernelGGL(prep_kernel, dim3(blocks), dim3(256), 0, stream, dv->ctl,
+                       dv->xchg_dirty ? dv->xchg : nullptr, nw);
+    HIPCHK(hipGetLastError());
+    dv->xchg_dirty = false;
+  }
+  int grid = (int)std::min<size_t>(tk.size(), (size_t)dv->resident_waves);
+  hipLaunchKernelGGL(search_frames_kernel, dim3(grid), dim3(64), 0, stream, dv->djobs, dv->dtickets,
+                     (int)tk.size(), dv->ctl, dv->tag, kSpinLimit);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(dv->done_ev, stream));
+  dv->done_recorded = true;
+  return PCME_OK;
+}
+
+}  // namespace pcme
+
+using namespace pcme;
+
+extern "C" {
+
+int pcme_version(void) { return 1; }
+
+const char *pcme_last_error(void) { return g_err; }
+
+int pcme_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  int k = 0;
+  for (int i = 0; i < n; i++) k += is_gfx950(i) ? 1 : 0;
+  return k;
+}
+
+uint32_t pcme_lambda(int32_t qp) {
+  if (qp < 0) qp = 0;
+  if (qp > 51) qp = 51;
+  return lambda_of(qp);
+}
+
+void pcme_qpel_table(uint32_t out[4]) {
+  out[0] = 0x41111000u;
+  out[1] = 0x61312120u;
+  out[2] = 0x63333222u;
+  out[3] = 0x96939282u;
+}
+
+int pcme_pad_planes(const uint8_t *src, int64_t src_pitch, int32_t sstride, int32_t w, int32_t h,
+                    uint8_t *dst, int64_t dst_pitch, int32_t dstride, int32_t aw, int32_t ah, int32_t pad,
+                    int32_t n, void *stream) {
+  if (n < 0 || w < 1 || h < 1 || w > aw || h > ah || sstride < w || pad < 0 || dstride < aw + 2 * pad ||
+      !src || !dst)
+    return fail(PCME_EINVAL, "pcme_pad_planes: bad geometry");
+  if ((dstride & 3) || (dst_pitch & 3) || ((uintptr_t)dst & 3))
+    return fail(PCME_EINVAL, "pcme_pad_planes: dst stride/pitch/pointer must be 4-byte aligned");
+  if (n == 0) return PCME_OK;
+  Dev *dv;
+  int rc = get_dev(&dv);
+  if (rc) return rc;
+  const int rows = ah + 2 * pad;
+  dim3 grid((unsigned)((dstride / 4 + 255) / 256), (unsigned)rows, (unsigned)n);
+  hipLaunchKernelGGL(pad_planes_kernel, grid, dim3(256), 0, (hipStream_t)stream, src, src_pitch, sstride, w,
+                     h, dst, dst_pitch, dstride, rows, pad);
+  HIPCHK(hipGetLastError());
+  return PCME_OK;
+}
+
+int pcme_hpel(const uint8_t *ref, uint8_t *h, uint8_t *v, uint8_t *c, int64_t pitch, int32_t n, int32_t mbw,
+              int32_t mbh, void *stream) {
+  if (n < 0 || mbw < 1 || mbh < 1 || !ref || !h || !v || !c) return fail(PCME_EINVAL, "pcme_hpel: bad arguments");
+  const int stride = 16 * mbw + 2 * PCME_PAD, rows = 16 * mbh + 2 * PCME_PAD;
+  if (pitch < (int64_t)stride * rows || (pitch & 3) || (((uintptr_t)h | (uintptr_t)v | (uintptr_t)c) & 3))
+    return fail(PCME_EINVAL, "pcme_hpel: pitch smaller than a plane or not 4-byte aligned");
+  if (n == 0) return PCME_OK;
+  Dev *dv;
+  int rc = get_dev(&dv);
+  if (rc) return rc;
+  dim3 grid((unsigned)((stride + HT_W - 1) / HT_W), (unsigned)((rows + HT_H - 1) / HT_H), (unsigned)n);
+  hipLaunchKernelGGL(hpel_kernel, grid, dim3(256), 0, (hipStream_t)stream, ref, h, v, c, pitch, stride, rows);
+  HIPCHK(hipGetLastError());
+  return PCME_OK;
+}
+
+int pcme_search_frames(const pcme_frame_job *jobs, int njobs, void *stream) {
+  Dev *dv = nullptr;
+  return search_frames_impl(jobs, njobs, (hipStream_t)stream, &dv);
+}
+
+int pcme_search_frames_sync(const pcme_frame_job *jobs, int njobs, void *stream) {
+  Dev *dv = nullptr;
+  int rc = search_frames_impl(jobs, njobs, (hipStream_t)stream, &dv);
+  if (rc || !dv) return rc;
+  HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+  uint32_t flag = 0;
+  {
+    std::lock_guard<std::mutex> g(dv->mu);
+    HIPCHK(hipMemcpy(&flag, dv->ctl + 1, 4, hipMemcpyDeviceToHost));
+  }
+  if (flag) return fail(PCME_ESTALL, "a wavefront dependency wait exceeded its bound");
+  return PCME_OK;
+}
+
+int pcme_search_batch(const pcme_search_job *jobs, int njobs, pcme_mv *out_mv, uint32_t *out_cost,
+                      void *stream) {
+  if (njobs < 0 || (njobs > 0 && (!jobs || !out_mv || !out_cost)))
+    return fail(PCME_EINVAL, "pcme_search_batch: bad arguments");
+  for (int i = 0; i < njobs; i++) {
+    const pcme_search_job &J = jobs[i];
+    bool shape = (J.w == 16 || J.w == 8) && (J.h == 16 || J.h == 8);
+    if (!shape) return fail(PCME_EINVAL, "job %d: block %dx%d not one of 16x16/16x8/8x16/8x8", i, J.w, J.h);
+    if (!J.src || !J.ref || !J.ref_h || !J.ref_v || !J.ref_c)
+      return fail(PCME_EINVAL, "job %d has a null pointer", i);
+    if (((uintptr_t)J.src & 3u) || (J.sstride & 3) || J.sstride < J.w)
+      return fail(PCME_EINVAL, "job %d: src must be 4-byte aligned with a stride multiple of 4", i);
+    if (J.mbw < 1 || J.mbh < 1 || J.bx < 0 || J.by < 0 || J.bx + J.w > 16 * J.mbw || J.by + J.h > 16 * J.mbh)
+      return fail(PCME_EINVAL, "job %d: block outside the picture", i);
+    if (J.ncands < 0 || J.ncands > 2) return fail(PCME_EINVAL, "job %d: ncands %d not in [0,2]", i, J.ncands);
+    if (J.me_range < 0 || J.me_range > 1024 || J.subpel_rounds < 0 || J.subpel_rounds > 8)
+      return fail(PCME_EINVAL, "job %d: me_range/subpel_rounds out of range", i);
+    if (J.vlim < J.by + J.h + 1 || J.vlim > 16LL * J.mbh + 44)
+      return fail(PCME_EINVAL, "job %d: vlim %lld outside [by+h+1, 16*mbh+44]", i, (long long)J.vlim);
+    int ax = J.mvp.x < 0 ? -J.mvp.x : J.mvp.x, ay = J.mvp.y < 0 ? -J.mvp.y : J.mvp.y;
+    if (ax >= 4096 || ay >= 4096) return fail(PCME_EINVAL, "job %d: mvp out of range", i);
+  }
+  if (njobs == 0) return PCME_OK;
+  Dev *dv;
+  int rc = get_dev(&dv);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(dv->mu);
+  if ((rc = grow(&dv->dsj, &dv->dsj_cap, (size_t)njobs))) return rc;
+  if ((rc = upload(dv, dv->dsj, jobs, sizeof(pcme_search_job) * (size_t)njobs, (hipStream_t)stream))) return rc;
+  hipLaunchKernelGGL(search_batch_kernel, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream, dv->dsj, njobs,
+                     out_mv, out_cost);
+  HIPCHK(hipGetLastError());
+  return PCME_OK;
+}
+
+int pcme_pred_frame(const uint8_t *ref, const uint8_t *ref_h, const uint8_t *ref_v, const uint8_t *ref_c,
+                    const uint8_t *ref_u, const uint8_t *ref_vv, const pcme_mb_out *mb, int32_t mbw, int32_t mbh,
+                    uint8_t *pred_y, uint8_t *pred_u, uint8_t *pred_v, void *stream) {
+  if (!ref || !ref_h || !ref_v || !ref_c || !ref_u || !ref_vv || !mb || !pred_y || !pred_u || !pred_v ||
+      mbw < 1 || mbh < 1)
+    return fail(PCME_EINVAL, "pcme_pred_frame: bad arguments");
+  Dev *dv;
+  int rc = get_dev(&dv);
+  if (rc) return rc;
+  hipLaunchKernelGGL(pred_frame_kernel, dim3((unsigned)(mbw * mbh)), dim3(256), 0, (hipStream_t)stream, ref,
+                     ref_h, ref_v, ref_c, ref_u, ref_vv, mb, mbw, mbh, pred_y, pred_u, pred_v);
+  HIPCHK(hipGetLastError());
+  return PCME_OK;
+}
+
+}  // extern "C"

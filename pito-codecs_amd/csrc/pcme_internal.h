@@ -1,0 +1,53 @@
+// pcme_internal.h -- types shared by the kernels and the host side of libpcme.
+#ifndef PCME_INTERNAL_H
+#define PCME_INTERNAL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/pcme.h"
+
+#define PCME_PAD 64
+#define PCME_CPAD 32
+
+namespace pcme {
+
+// Device copy of one frame job of the ME-only field search.
+struct DevJob {
+  const uint8_t *src;    // padded source luma
+  const uint8_t *pl[4];  // reference planes: full, H, V, C
+  pcme_mb_out *out;      // mbw*mbh records
+  uint32_t *xchg;        // mbw*mbh*4 exchange words (bottom-row MVs)
+  int mbw, mbh, stride;
+  int me_range, subpel_rounds, partitions;
+  uint32_t lam;
+  int pad_;
+};
+
+#ifdef __HIP__
+__global__ void search_frames_kernel(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
+                                     int ntickets, uint32_t *ctl, uint32_t tag, uint32_t spin_limit);
+__global__ void search_batch_kernel(const pcme_search_job *__restrict__ jobs, int njobs,
+                                    pcme_mv *out_mv, uint32_t *out_cost);
+__global__ void pad_planes_kernel(const uint8_t *__restrict__ src, int64_t src_pitch, int sstride, int w,
+                                  int h, uint8_t *__restrict__ dst, int64_t dst_pitch, int dstride,
+                                  int rows, int pad);
+__global__ void hpel_kernel(const uint8_t *__restrict__ ref, uint8_t *__restrict__ ho,
+                            uint8_t *__restrict__ vo, uint8_t *__restrict__ co, int64_t pitch,
+                            int stride, int rows);
+__global__ void pred_frame_kernel(const uint8_t *__restrict__ pl0, const uint8_t *__restrict__ pl1,
+                                  const uint8_t *__restrict__ pl2, const uint8_t *__restrict__ pl3,
+                                  const uint8_t *__restrict__ pu, const uint8_t *__restrict__ pv,
+                                  const pcme_mb_out *__restrict__ mbs, int mbw, int mbh,
+                                  uint8_t *__restrict__ oy, uint8_t *__restrict__ ou,
+                                  uint8_t *__restrict__ ov);
+__global__ void prep_kernel(uint32_t *ctl, uint32_t *xchg, size_t nwords);
+#endif
+
+// Tile of the half-pel kernel.
+#define HT_W 64
+#define HT_H 32
+
+}  // namespace pcme
+
+#endif

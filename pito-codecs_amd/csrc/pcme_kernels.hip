@@ -1,0 +1,911 @@
+// pcme_kernels.hip -- gfx950 kernels of the H.264 P-frame motion search.
+//
+// Kernels:
+//   pad_planes_kernel     import + edge extension (mod.rs:842-873,
+//                         frame.rs:58-97); HBM-bound copy.
+//   hpel_kernel           H/V/C half-pel planes over the padded plane
+//                         (picture.rs:318-380); HBM-bound, LDS-tiled.
+//   search_frames_kernel  persistent row walkers: one wave64 walks one MB row
+//                         of one frame left to right and runs, per MB,
+//                         pskip_mv -> search_part(16x16) -> partition choice
+//                         (slice.rs:1870-1945) with search (slice.rs:1590-1723)
+//                         evaluated 8 candidates per pass.  Rows are taken
+//                         from an atomic ticket queue in (row, frame) order;
+//                         row y of a frame waits for the bottom-row MVs of MB
+//                         x+1 of row y-1, published as tagged 32-bit words.
+//   search_batch_kernel   one wave per independent search() call.
+//   pred_frame_kernel     mc_luma + mc_chroma of the chosen partitions
+//                         (slice.rs:1725-1749).
+//
+// Candidate evaluation layout: lane = 8*slot + sub.  A slot holds one
+// candidate MV; its 8 lanes each cover a 4-row x 8-column strip of the
+// partition (two 4x4 blocks side by side, packed as i16 pairs for SATD).
+// Costs are reduced over the 8 lanes with DPP and the strict-< scan over the
+// slots runs on scalar registers, which reproduces the reference's
+// sequential "first strictly smaller cost wins" order exactly.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pcme_internal.h"
+
+namespace pcme {
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// DPP controls (gfx9 encoding).
+#define DPP_QP_1032 0xB1   // quad_perm [1,0,3,2]
+#define DPP_QP_2301 0x4E   // quad_perm [2,3,0,1]
+#define DPP_ROW_HALF_MIRROR 0x141
+
+// Sum over each aligned group of 8 lanes; every lane of the group receives
+// the sum.  Requires a full exec mask.
+__device__ __forceinline__ uint32_t red8(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_QP_1032, 0xF, 0xF, false);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_QP_2301, 0xF, 0xF, false);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_ROW_HALF_MIRROR, 0xF, 0xF, false);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t rdlane(uint32_t v, int l) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+
+// Exp-Golomb se(v) length (slice.rs:130-139).
+__device__ __forceinline__ uint32_t se_bits(int v) {
+  uint32_t k = v > 0 ? 2u * (uint32_t)v - 1u : (uint32_t)(-2 * v);
+  return 2u * (31u - (uint32_t)__builtin_clz(k + 1u)) + 1u;
+}
+
+// 8 bytes from an arbitrary address of a padded plane, as two dwords, read
+// with three aligned dword loads (in bounds: every caller keeps 3 bytes of
+// plane after the window, guaranteed by the 64-pixel pad and mv_ok).
+__device__ __forceinline__ void ld8u(const uint8_t *p, uint32_t &lo, uint32_t &hi) {
+  uintptr_t a = (uintptr_t)p;
+  const uint32_t *q = (const uint32_t *)(a & ~(uintptr_t)3);
+  uint32_t sh = (uint32_t)a & 3u;
+  uint32_t d0 = q[0], d1 = q[1], d2 = q[2];
+  lo = __builtin_amdgcn_alignbyte(d1, d0, sh);
+  hi = __builtin_amdgcn_alignbyte(d2, d1, sh);
+}
+
+typedef short v2s __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ v2s as_v2s(uint32_t u) { return __builtin_bit_cast(v2s, u); }
+
+// v_perm selector: [lo.byte c, 0, hi.byte c, 0] -> u16 pair (lo block, hi block)
+#define SELC(c) ((uint32_t)(c) | (0x0Cu << 8) | ((4u + (uint32_t)(c)) << 16) | (0x0Cu << 24))
+
+// SATD of one 4-row x 8-column strip = two 4x4 blocks (dsp.rs:35-70 for
+// one 4x4: vertical 4-point butterfly, horizontal pair butterfly, then
+// sum max(|x|,|y|)), both blocks at once in packed i16.  sp = source pairs
+// [r*4+c] = (left block [r][c], right block [r][c]); P[2r], P[2r+1] = the
+// prediction row r, columns 0-3 and 4-7.
+__device__ __forceinline__ uint32_t satd_strip8(const v2s *sp, const uint32_t *P) {
+  v2s d[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+      d[r][c] = sp[r * 4 + c] - as_v2s(__builtin_amdgcn_perm(P[2 * r + 1], P[2 * r], SELC(c)));
+  v2s v[4][4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    v2s a0 = d[0][c] + d[1][c], a1 = d[0][c] - d[1][c];
+    v2s a2 = d[2][c] + d[3][c], a3 = d[2][c] - d[3][c];
+    v[0][c] = a0 + a2;
+    v[1][c] = a0 - a2;
+    v[2][c] = a1 + a3;
+    v[3][c] = a1 - a3;
+  }
+  v2s acc = {0, 0};
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    v2s hs0 = v[k][0] + v[k][1], hs1 = v[k][2] + v[k][3];
+    v2s hd0 = v[k][0] - v[k][1], hd1 = v[k][2] - v[k][3];
+    acc += __builtin_elementwise_max(__builtin_elementwise_abs(hs0), __builtin_elementwise_abs(hs1));
+    acc += __builtin_elementwise_max(__builtin_elementwise_abs(hd0), __builtin_elementwise_abs(hd1));
+  }
+  return (uint32_t)(uint16_t)acc.x + (uint32_t)(uint16_t)acc.y;
+}
+
+// Quarter-pel source table (picture.rs:387-404), 8 bits per position q:
+// [1:0] plane A, [2] dx A, [3] dy A, [5:4] plane B, [6] dx B, [7] dy B.
+#define QW0 0x41111000u
+#define QW1 0x61312120u
+#define QW2 0x63333222u
+#define QW3 0x96939282u
+
+__device__ __forceinline__ uint32_t qpel_entry(uint32_t q) {
+  uint32_t w = (q & 8u) ? ((q & 4u) ? QW3 : QW2) : ((q & 4u) ? QW1 : QW0);
+  return (w >> ((q & 3u) * 8u)) & 0xFFu;
+}
+
+// --------------------------------------------------------- search context
+
+struct SCtx {            // wave-uniform
+  const uint8_t *pl[4];  // padded planes: full, H, V, C
+  int stride;
+  int pw;                // 16*mbw
+  int vlim;
+  uint32_t lam;
+  int range;
+  int subpel;
+};
+
+// slice.rs:496-507
+__device__ __forceinline__ bool mv_ok(const SCtx &c, int x, int y, int w, int h, int mx, int my) {
+  int x0 = x + (mx >> 2), y0 = y + (my >> 2);
+  int ax = mx < 0 ? -mx : mx, ay = my < 0 ? -my : my;
+  return ax < 2000 && ay < 2000 && x0 >= -44 && y0 >= -44 && x0 + w <= c.pw + 44 &&
+         y0 + h < c.vlim;
+}
+
+// Lane constants of one search: strip origin inside the block and whether
+// the lane's strip lies inside the block.
+struct Lane {
+  int slot, sub, sx, sy;
+  bool active;
+};
+
+template <int W, int H>
+__device__ __forceinline__ Lane lane_layout() {
+  Lane L;
+  uint32_t l = lane_id();
+  L.slot = (int)(l >> 3);
+  L.sub = (int)(l & 7u);
+  if (W == 16) {
+    L.sx = (L.sub & 1) * 8;
+    L.sy = (L.sub >> 1) * 4;
+    L.active = L.sub < 2 * (H / 4);
+  } else {
+    L.sx = 0;
+    L.sy = L.sub * 4;
+    L.active = L.sub < H / 4;
+  }
+  if (!L.active) L.sy = 0;
+  return L;
+}
+
+struct SrcRegs {
+  uint32_t raw[8];  // row r: raw[2r] cols 0-3, raw[2r+1] cols 4-7
+  v2s sp[16];       // u16 pairs for SATD
+};
+
+// Source strip of the lane; blk = block top-left, 4-byte aligned, sstride a
+// multiple of 4.
+__device__ __forceinline__ void load_src(const uint8_t *blk, int sstride, const Lane &L, SrcRegs &s) {
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const uint32_t *q = (const uint32_t *)(blk + (L.sy + r) * sstride + L.sx);
+    s.raw[2 * r] = q[0];
+    s.raw[2 * r + 1] = q[1];
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+      s.sp[r * 4 + c] = as_v2s(__builtin_amdgcn_perm(s.raw[2 * r + 1], s.raw[2 * r], SELC(c)));
+}
+
+// Integer-pel SAD cost of the lane's slot candidate (slice.rs:1605-1616):
+// SAD over the block + (lam * mv_bits) >> 8.  (ux, uy) is the address MV:
+// the candidate when valid, else a valid fallback (its cost is ignored).
+__device__ __forceinline__ uint32_t sad_cost(const SCtx &c, const Lane &L, const SrcRegs &s, int bx,
+                                             int by, int cx, int cy, int ux, int uy, int mvpx,
+                                             int mvpy) {
+  int x = bx + L.sx + (ux >> 2), y = by + L.sy + (uy >> 2);
+  const uint8_t *p = c.pl[0] + (size_t)(y + PCME_PAD) * c.stride + (x + PCME_PAD);
+  uint32_t acc = 0;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    uint32_t a, b;
+    ld8u(p + r * c.stride, a, b);
+    acc = __builtin_amdgcn_sad_u8(a, s.raw[2 * r], acc);
+    acc = __builtin_amdgcn_sad_u8(b, s.raw[2 * r + 1], acc);
+  }
+  acc = L.active ? acc : 0u;
+  acc = red8(acc);
+  return acc + ((c.lam * (se_bits(cx - mvpx) + se_bits(cy - mvpy))) >> 8);
+}
+
+__device__ __forceinline__ const uint8_t *plane_sel(const SCtx &c, uint32_t p) {
+  const uint8_t *a = (p & 1u) ? c.pl[1] : c.pl[0];
+  const uint8_t *b = (p & 1u) ? c.pl[3] : c.pl[2];
+  return (p & 2u) ? b : a;
+}
+
+// Sub-pel cost (slice.rs:1684-1687): mc_luma (picture.rs:415-449) as the
+// rounding average of two planes, SATD, + (lam * mv_bits) >> 8.  Full and
+// half positions read the same plane twice, and (a+a+1)>>1 == a.
+__device__ __forceinline__ uint32_t satd_cost(const SCtx &c, const Lane &L, const SrcRegs &s, int bx,
+                                              int by, int cx, int cy, int ux, int uy, int mvpx,
+                                              int mvpy) {
+  int xi = bx + L.sx + (ux >> 2), yi = by + L.sy + (uy >> 2);
+  uint32_t e = qpel_entry((uint32_t)((uy & 3) * 4 + (ux & 3)));
+  const uint8_t *A = plane_sel(c, e & 3u) +
+                     (size_t)(yi + (int)((e >> 3) & 1u) + PCME_PAD) * c.stride +
+                     (xi + (int)((e >> 2) & 1u) + PCME_PAD);
+  const uint8_t *B = plane_sel(c, (e >> 4) & 3u) +
+                     (size_t)(yi + (int)((e >> 7) & 1u) + PCME_PAD) * c.stride +
+                     (xi + (int)((e >> 6) & 1u) + PCME_PAD);
+  uint32_t P[8];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    uint32_t a0, a1, b0, b1;
+    ld8u(A + r * c.stride, a0, a1);
+    ld8u(B + r * c.stride, b0, b1);
+    P[2 * r] = __builtin_amdgcn_lerp(a0, b0, 0x01010101u);
+    P[2 * r + 1] = __builtin_amdgcn_lerp(a1, b1, 0x01010101u);
+  }
+  uint32_t acc = satd_strip8(s.sp, P);
+  acc = L.active ? acc : 0u;
+  acc = red8(acc);
+  return acc + ((c.lam * (se_bits(cx - mvpx) + se_bits(cy - mvpy))) >> 8);
+}
+
+// Per-slot candidate offsets.  HEX: (-2,0),(2,0),(-1,-2),(1,-2),(-1,2),(1,2)
+// (slice.rs:1634).  SQ8: (-1,0),(1,0),(0,-1),(0,1),(-1,-1),(1,-1),(-1,1),
+// (1,1) (slice.rs:1660-1669, 1693-1702).
+__device__ __forceinline__ int hex_dx(int s) { return s < 2 ? (s == 0 ? -2 : 2) : ((s & 1) ? 1 : -1); }
+__device__ __forceinline__ int hex_dy(int s) { return s < 2 ? 0 : (s < 4 ? -2 : 2); }
+__device__ __forceinline__ int sq_dx(int s) { return s < 2 ? (s ? 1 : -1) : (s < 4 ? 0 : ((s & 1) ? 1 : -1)); }
+__device__ __forceinline__ int sq_dy(int s) { return s < 2 ? 0 : (s < 4 ? (s == 2 ? -1 : 1) : (s < 6 ? -1 : 1)); }
+
+struct SRes {
+  int x, y;
+  uint32_t cost;
+};
+
+// SliceEnc::search (slice.rs:1590-1723) for a W x H block at picture
+// position (bx, by).  mvp, cands are wave-uniform; returns (best, cost).
+template <int W, int H>
+__device__ SRes search(const SCtx &c, const SrcRegs &s, const Lane &L, int bx, int by, int mvpx,
+                       int mvpy, int c0x, int c0y, int c1x, int c1y, int ncand) {
+  // slice.rs:1617-1631: rounded predictor, then the candidates.
+  int bxv = (mvpx + 2) & ~3, byv = (mvpy + 2) & ~3;
+  if (!mv_ok(c, bx, by, W, H, bxv, byv)) { bxv = 0; byv = 0; }
+  int r0x = (c0x + 2) & ~3, r0y = (c0y + 2) & ~3;
+  int r1x = (c1x + 2) & ~3, r1y = (c1y + 2) & ~3;
+  uint32_t bcost;
+  {
+    int cx = L.slot == 1 ? r0x : (L.slot == 2 ? r1x : bxv);
+    int cy = L.slot == 1 ? r0y : (L.slot == 2 ? r1y : byv);
+    bool valid = L.slot == 0 || (L.slot <= ncand && mv_ok(c, bx, by, W, H, cx, cy));
+    uint64_t vm = __ballot(valid);
+    uint32_t k = sad_cost(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy);
+    bcost = rdlane(k, 0);
+    if (ncand >= 1 && ((vm >> 8) & 1u) && !(r0x == bxv && r0y == byv)) {
+      uint32_t k1 = rdlane(k, 8);
+      if (k1 < bcost) { bcost = k1; bxv = r0x; byv = r0y; }
+    }
+    if (ncand >= 2 && ((vm >> 16) & 1u) && !(r1x == bxv && r1y == byv)) {
+      uint32_t k2 = rdlane(k, 16);
+      if (k2 < bcost) { bcost = k2; bxv = r1x; byv = r1y; }
+    }
+  }
+  // slice.rs:1632-1656: hexagon descent clamped to +-range around centre0.
+  const int c0xx = bxv, c0yy = byv;
+  const int range = c.range;
+  const int hdx = hex_dx(L.slot & 7), hdy = hex_dy(L.slot & 7);
+  for (int it = 0; it < range; it++) {
+    int cx = bxv + hdx * 4, cy = byv + hdy * 4;
+    int ddx = cx - c0xx, ddy = cy - c0yy;
+    bool valid = L.slot < 6 && ((ddx < 0 ? -ddx : ddx) >> 2) <= range &&
+                 ((ddy < 0 ? -ddy : ddy) >> 2) <= range && mv_ok(c, bx, by, W, H, cx, cy);
+    uint64_t vm = __ballot(valid);
+    uint32_t k = sad_cost(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy);
+    const int ox = bxv, oy = byv;
+    bool moved = false;
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+      if ((vm >> (8 * j)) & 1u) {
+        uint32_t kj = rdlane(k, 8 * j);
+        if (kj < bcost) {
+          bcost = kj;
+          bxv = ox + hex_dx(j) * 4;
+          byv = oy + hex_dy(j) * 4;
+          moved = true;
+        }
+      }
+    }
+    if (!moved) break;
+  }
+  // slice.rs:1657-1682: two rounds of 8-neighbour integer refinement.
+  const int qdx = sq_dx(L.slot), qdy = sq_dy(L.slot);
+  for (int it = 0; it < 2; it++) {
+    int cx = bxv + qdx * 4, cy = byv + qdy * 4;
+    bool valid = mv_ok(c, bx, by, W, H, cx, cy);
+    uint64_t vm = __ballot(valid);
+    uint32_t k = sad_cost(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy);
+    const int ox = bxv, oy = byv;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      if ((vm >> (8 * j)) & 1u) {
+        uint32_t kj = rdlane(k, 8 * j);
+        if (kj < bcost) {
+          bcost = kj;
+          bxv = ox + sq_dx(j) * 4;
+          byv = oy + sq_dy(j) * 4;
+        }
+      }
+    }
+    if (bxv == ox && byv == oy) break;
+  }
+  // slice.rs:1688: SATD cost at the integer best.
+  uint32_t bs = rdlane(satd_cost(c, L, s, bx, by, bxv, byv, bxv, byv, mvpx, mvpy), 0);
+  // slice.rs:1689-1721: half-pel round, then 1 + subpel_rounds quarter-pel.
+#pragma unroll
+  for (int step = 2; step >= 1; step--) {
+    int rounds = step == 1 ? 1 + c.subpel : 1;
+    for (int it = 0; it < rounds; it++) {
+      int cx = bxv + qdx * step, cy = byv + qdy * step;
+      bool valid = mv_ok(c, bx, by, W, H, cx, cy);
+      uint64_t vm = __ballot(valid);
+      uint32_t k = satd_cost(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy);
+      const int ox = bxv, oy = byv;
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        if ((vm >> (8 * j)) & 1u) {
+          uint32_t kj = rdlane(k, 8 * j);
+          if (kj < bs) {
+            bs = kj;
+            bxv = ox + sq_dx(j) * step;
+            byv = oy + sq_dy(j) * step;
+          }
+        }
+      }
+      if (bxv == ox && byv == oy) break;
+    }
+  }
+  SRes r;
+  r.x = bxv;
+  r.y = byv;
+  r.cost = bs;
+  return r;
+}
+
+// ------------------------------------------------------------ neighbours
+
+// Neighbour state of the MB being analysed in the ME-only field: every
+// coded MB is PInter with ref 0, so an available neighbour has ref 0.
+struct Nbr {
+  bool aL, aT, aTR, aTL;
+  int Lx[4], Ly[4];  // left MB, blocks 3, 7, 11, 15
+  int Tx[4], Ty[4];  // top MB, blocks 12..15
+  int TRx, TRy;      // top-right MB, block 12
+  int TLx, TLy;      // top-left MB, block 15
+};
+
+struct NbRes {
+  bool avail;
+  int ref;
+  int x, y;
+};
+
+// Partitions already searched in the current mode (search_part's `info`).
+struct Cur {
+  int x[4], y[4];
+};
+
+__host__ __device__ constexpr int part_count(int part) { return part == 0 ? 1 : (part == 3 ? 4 : 2); }
+__host__ __device__ constexpr int rect_x(int part, int i) { return part == 2 ? 8 * i : (part == 3 ? 8 * (i & 1) : 0); }
+__host__ __device__ constexpr int rect_y(int part, int i) { return part == 1 ? 8 * i : (part == 3 ? 8 * (i >> 1) : 0); }
+__host__ __device__ constexpr int rect_w(int part) { return (part == 0 || part == 1) ? 16 : 8; }
+__host__ __device__ constexpr int rect_h(int part) { return (part == 0 || part == 2) ? 16 : 8; }
+// Partition of `part` covering 4x4 block r.
+__host__ __device__ constexpr int part_of(int part, int r) {
+  return part == 0 ? 0 : part == 1 ? (r >> 3) : part == 2 ? ((r & 3) >> 1) : (((r >> 3) << 1) | ((r & 3) >> 1));
+}
+
+// nb_part (slice.rs:392-432) for the partition P of mode PART, block offset
+// (BX, BY) in 4x4 units relative to the MB.
+template <int PART, int P, int BX, int BY>
+__device__ __forceinline__ NbRes nb(const Nbr &n, const Cur &cur) {
+  NbRes r = {false, -1, 0, 0};
+  if constexpr (BX >= 0 && BX < 4 && BY >= 0 && BY < 4) {
+    constexpr int k = part_of(PART, BY * 4 + BX);
+    if constexpr (k < P) {
+      r.avail = true; r.ref = 0; r.x = cur.x[k]; r.y = cur.y[k];
+    }
+  } else if constexpr (BY >= 4 || (BY >= 0 && BX >= 4)) {
+  } else if constexpr (BY < 0) {
+    if constexpr (BX < 0) {
+      if (n.aTL) { r.avail = true; r.ref = 0; r.x = n.TLx; r.y = n.TLy; }
+    } else if constexpr (BX < 4) {
+      if (n.aT) { r.avail = true; r.ref = 0; r.x = n.Tx[BX]; r.y = n.Ty[BX]; }
+    } else {
+      if (n.aTR) { r.avail = true; r.ref = 0; r.x = n.TRx; r.y = n.TRy; }
+    }
+  } else {
+    if (n.aL) { r.avail = true; r.ref = 0; r.x = n.Lx[BY]; r.y = n.Ly[BY]; }
+  }
+  return r;
+}
+
+__device__ __forceinline__ int median3(int a, int b, int c) {
+  int mx = a > b ? a : b, mn = a < b ? a : b;
+  int t = mn > c ? mn : c;
+  return mx < t ? mx : t;
+}
+
+// mvp (slice.rs:434-482) with rf = 0.  SHAPE: 0 Normal, 1 Top16x8,
+// 2 Bottom16x8, 3 Left8x16, 4 Right8x16.
+template <int PART, int P, int X, int Y, int WB, int SHAPE>
+__device__ __forceinline__ void mvp(const Nbr &n, const Cur &cur, int &ox, int &oy) {
+  NbRes a = nb<PART, P, X - 1, Y>(n, cur);
+  NbRes b = nb<PART, P, X, Y - 1>(n, cur);
+  NbRes cc = nb<PART, P, X + WB, Y - 1>(n, cur);
+  if (!cc.avail) cc = nb<PART, P, X - 1, Y - 1>(n, cur);
+  if (!b.avail && !cc.avail && a.avail) { b = a; cc = a; }
+  if (!a.avail) { a.ref = -1; a.x = a.y = 0; }
+  if (!b.avail) { b.ref = -1; b.x = b.y = 0; }
+  if (!cc.avail) { cc.ref = -1; cc.x = cc.y = 0; }
+  const int rf = 0;
+  if (SHAPE == 1 && b.ref == rf) { ox = b.x; oy = b.y; return; }
+  if (SHAPE == 2 && a.ref == rf) { ox = a.x; oy = a.y; return; }
+  if (SHAPE == 3 && a.ref == rf) { ox = a.x; oy = a.y; return; }
+  if (SHAPE == 4 && cc.ref == rf) { ox = cc.x; oy = cc.y; return; }
+  int cnt = (a.ref == rf) + (b.ref == rf) + (cc.ref == rf);
+  if (cnt == 1) {
+    if (a.ref == rf) { ox = a.x; oy = a.y; return; }
+    if (b.ref == rf) { ox = b.x; oy = b.y; return; }
+    ox = cc.x; oy = cc.y; return;
+  }
+  ox = median3(a.x, b.x, cc.x);
+  oy = median3(a.y, b.y, cc.y);
+}
+
+// pskip_mv (slice.rs:484-494)
+__device__ __forceinline__ void pskip(const Nbr &n, int &ox, int &oy) {
+  Cur cur = {};
+  NbRes a = nb<0, 0, -1, 0>(n, cur);
+  NbRes b = nb<0, 0, 0, -1>(n, cur);
+  ox = oy = 0;
+  if (!a.avail || !b.avail) return;
+  if (a.ref == 0 && a.x == 0 && a.y == 0) return;
+  if (b.ref == 0 && b.x == 0 && b.y == 0) return;
+  mvp<0, 0, 0, 0, 4, 0>(n, cur, ox, oy);
+}
+
+__host__ __device__ constexpr int shape_of(int part, int p) {
+  return part == 1 ? (p == 0 ? 1 : 2) : part == 2 ? (p == 0 ? 3 : 4) : 0;
+}
+
+struct PartRes {
+  int mx[4], my[4];    // mvs
+  int px[4], py[4];    // mvps
+  uint32_t total;
+};
+
+// One partition of search_part (slice.rs:1826-1866), then the next.
+template <int PART, int P>
+__device__ __forceinline__ void part_step(const SCtx &c, const Nbr &n, Cur &cur, const uint8_t *src,
+                                          int sstride, int mbx, int mby, int seedx, int seedy,
+                                          PartRes &res) {
+  if constexpr (P < part_count(PART)) {
+    constexpr int X = rect_x(PART, P), Y = rect_y(PART, P);
+    constexpr int W = rect_w(PART), H = rect_h(PART);
+    int mvpx, mvpy;
+    mvp<PART, P, X / 4, Y / 4, W / 4, shape_of(PART, P)>(n, cur, mvpx, mvpy);
+    Lane L = lane_layout<W, H>();
+    SrcRegs s;
+    load_src(src + Y * sstride + X, sstride, L, s);
+    int m0x = P == 0 ? 0 : res.mx[0], m0y = P == 0 ? 0 : res.my[0];
+    SRes r = search<W, H>(c, s, L, mbx + X, mby + Y, mvpx, mvpy, seedx, seedy, m0x, m0y, 2);
+    res.mx[P] = r.x; res.my[P] = r.y;
+    res.px[P] = mvpx; res.py[P] = mvpy;
+    res.total += r.cost;
+    cur.x[P] = r.x; cur.y[P] = r.y;
+    part_step<PART, P + 1>(c, n, cur, src, sstride, mbx, mby, seedx, seedy, res);
+  }
+}
+
+// search_part (slice.rs:1810-1868), list 0.  src = MB top-left.
+template <int PART>
+__device__ __forceinline__ void search_part(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
+                                            int mbx, int mby, int seedx, int seedy, PartRes &res) {
+  Cur cur = {};
+#pragma unroll
+  for (int i = 0; i < 4; i++) { res.mx[i] = res.my[i] = res.px[i] = res.py[i] = 0; }
+  res.total = 0;
+  part_step<PART, 0>(c, n, cur, src, sstride, mbx, mby, seedx, seedy, res);
+}
+
+__device__ __forceinline__ uint32_t pack_mv(int x, int y) {
+  return ((uint32_t)x & 0xFFFFu) | ((uint32_t)y << 16);
+}
+__device__ __forceinline__ int sel4(int k, int a, int b, int cc, int d) {
+  return k == 0 ? a : (k == 1 ? b : (k == 2 ? cc : d));
+}
+
+struct MbRes {
+  int part;
+  PartRes best;
+  uint32_t cost[4];
+  uint32_t best_cost;
+  int skx, sky, m16x, m16y;
+};
+
+// ME-only analysis of one MB (slice.rs:1884-1945 without early skip,
+// intra and RD).
+__device__ __forceinline__ void analyse_mb(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
+                                           int mbx, int mby, int partitions, MbRes &o) {
+  int skx, sky;
+  pskip(n, skx, sky);
+  PartRes r16;
+  search_part<0>(c, n, src, sstride, mbx, mby, skx, sky, r16);
+  const uint32_t cost16 = r16.total;
+  o.part = 0;
+  o.best = r16;
+  o.best_cost = cost16 + (c.lam >> 8);
+  o.cost[0] = cost16;
+  o.cost[1] = o.cost[2] = o.cost[3] = 0xFFFFFFFFu;
+  o.skx = skx; o.sky = sky;
+  o.m16x = r16.mx[0]; o.m16y = r16.my[0];
+  if (partitions && cost16 > 1024u) {
+    PartRes r;
+    search_part<1>(c, n, src, sstride, mbx, mby, r16.mx[0], r16.my[0], r);
+    o.cost[1] = r.total;
+    uint32_t cc = r.total + ((c.lam * 3u) >> 8);
+    if (cc < o.best_cost) { o.best_cost = cc; o.part = 1; o.best = r; }
+    search_part<2>(c, n, src, sstride, mbx, mby, r16.mx[0], r16.my[0], r);
+    o.cost[2] = r.total;
+    cc = r.total + ((c.lam * 3u) >> 8);
+    if (cc < o.best_cost) { o.best_cost = cc; o.part = 2; o.best = r; }
+    search_part<3>(c, n, src, sstride, mbx, mby, r16.mx[0], r16.my[0], r);
+    o.cost[3] = r.total;
+    cc = r.total + ((c.lam * 10u) >> 8);
+    if (cc < o.best_cost) { o.best_cost = cc; o.part = 3; o.best = r; }
+  }
+}
+
+// MV of 4x4 block r after cand_inter_info (slice.rs:1751-1785).
+__device__ __forceinline__ int blk_mvx(const MbRes &o, int r) {
+  int k = o.part == 0 ? 0 : o.part == 1 ? (r >> 3) : o.part == 2 ? ((r & 3) >> 1) : (((r >> 3) << 1) | ((r & 3) >> 1));
+  return sel4(k, o.best.mx[0], o.best.mx[1], o.best.mx[2], o.best.mx[3]);
+}
+__device__ __forceinline__ int blk_mvy(const MbRes &o, int r) {
+  int k = o.part == 0 ? 0 : o.part == 1 ? (r >> 3) : o.part == 2 ? ((r & 3) >> 1) : (((r >> 3) << 1) | ((r & 3) >> 1));
+  return sel4(k, o.best.my[0], o.best.my[1], o.best.my[2], o.best.my[3]);
+}
+
+// 32 dwords of pcme_mb_out, lane i < 32 stores dword i.
+__device__ __forceinline__ void store_out(pcme_mb_out *out, const MbRes &o) {
+  int l = (int)lane_id();
+  uint32_t v;
+  if (l < 16) {
+    v = pack_mv(blk_mvx(o, l), blk_mvy(o, l));
+  } else if (l < 20) {
+    int k = l - 16;
+    v = pack_mv(sel4(k, o.best.mx[0], o.best.mx[1], o.best.mx[2], o.best.mx[3]),
+                sel4(k, o.best.my[0], o.best.my[1], o.best.my[2], o.best.my[3]));
+  } else if (l < 24) {
+    int k = l - 20;
+    v = pack_mv(sel4(k, o.best.px[0], o.best.px[1], o.best.px[2], o.best.px[3]),
+                sel4(k, o.best.py[0], o.best.py[1], o.best.py[2], o.best.py[3]));
+  } else if (l < 28) {
+    int k = l - 24;
+    v = (uint32_t)sel4(k, (int)o.cost[0], (int)o.cost[1], (int)o.cost[2], (int)o.cost[3]);
+  } else if (l == 28) {
+    v = o.best_cost;
+  } else if (l == 29) {
+    v = pack_mv(o.skx, o.sky);
+  } else if (l == 30) {
+    v = pack_mv(o.m16x, o.m16y);
+  } else {
+    v = (uint32_t)o.part;
+  }
+  if (l < 32) ((uint32_t *)out)[l] = v;
+}
+
+// ------------------------------------------------------ frame search kernel
+
+__device__ __forceinline__ int sext12(uint32_t v) { return (int)((v & 0xFFFu) ^ 0x800u) - 0x800; }
+__device__ __forceinline__ uint32_t xword(uint32_t tag, int x, int y) {
+  return (tag << 24) | (((uint32_t)x & 0xFFFu) << 12) | ((uint32_t)y & 0xFFFu);
+}
+
+// Wait until the 4 exchange words of one MB carry `tag`; returns them in
+// w[0..3] (uniform).  False on stall or on a raised error flag.
+__device__ __forceinline__ bool fetch_words(const uint32_t *xw, uint32_t tag, uint32_t *err,
+                                            uint32_t spin_limit, uint32_t w[4]) {
+  int l = (int)lane_id();
+  uint32_t spins = 0;
+  uint32_t v = 0;
+  for (;;) {
+    if (l < 4) v = __hip_atomic_load(xw + l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    bool ok = l >= 4 || (v >> 24) == tag;
+    if (__ballot(!ok) == 0) break;
+    if (++spins > spin_limit) {
+      if (l == 0) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    if ((spins & 63u) == 0) {
+      uint32_t e = 0;
+      if (l == 0) e = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (rdlane(e, 0) != 0) return false;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) w[i] = rdlane(v, i);
+  return true;
+}
+
+__global__ void __launch_bounds__(64) search_frames_kernel(const DevJob *__restrict__ jobs,
+                                                           const int2 *__restrict__ tickets, int ntickets,
+                                                           uint32_t *ctl, uint32_t tag,
+                                                           uint32_t spin_limit) {
+  uint32_t *counter = ctl;
+  uint32_t *err = ctl + 1;
+  const int l = (int)lane_id();
+  for (;;) {
+    uint32_t t = 0;
+    if (l == 0) t = atomicAdd(counter, 1u);
+    t = rdlane(t, 0);
+    if (t >= (uint32_t)ntickets) return;
+    {
+      uint32_t e = 0;
+      if (l == 0) e = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (rdlane(e, 0) != 0) return;
+    }
+    const int2 tk = tickets[t];
+    const DevJob &J = jobs[tk.x];
+    const int my = tk.y;
+    const int mbw = J.mbw, mbh = J.mbh, stride = J.stride;
+    SCtx c;
+    c.pl[0] = J.pl[0]; c.pl[1] = J.pl[1]; c.pl[2] = J.pl[2]; c.pl[3] = J.pl[3];
+    c.stride = stride;
+    c.pw = 16 * mbw;
+    {
+      // frame.rs:152 with picture.rs:261-265
+      int need = 16 * (my + 1) + 96;
+      c.vlim = need >= 16 * mbh ? 16 * mbh + 44 : need;
+    }
+    c.lam = J.lam;
+    c.range = J.me_range;
+    c.subpel = J.subpel_rounds;
+    uint32_t *xrow = J.xchg + (size_t)my * mbw * 4;
+    const uint32_t *xup = J.xchg + (size_t)(my - 1) * mbw * 4;
+    Nbr n;
+    n.aL = false;
+#pragma unroll
+    for (int i = 0; i < 4; i++) { n.Lx[i] = n.Ly[i] = 0; n.Tx[i] = n.Ty[i] = 0; }
+    n.TRx = n.TRy = n.TLx = n.TLy = 0;
+    // Rolling cache of the row above: words of MB x-1 (prev), x (cur), x+1 (next).
+    uint32_t prev[4] = {0, 0, 0, 0}, cur[4] = {0, 0, 0, 0}, nxt[4] = {0, 0, 0, 0};
+    bool ok = true;
+    if (my > 0) {
+      ok = fetch_words(xup, tag, err, spin_limit, cur);
+      if (ok && mbw > 1) ok = fetch_words(xup + 4, tag, err, spin_limit, nxt);
+    }
+    if (!ok) return;
+    for (int mx = 0; mx < mbw; mx++) {
+      if (my > 0 && mx > 0) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) { prev[i] = cur[i]; cur[i] = nxt[i]; }
+        if (mx + 1 < mbw) {
+          if (!fetch_words(xup + (size_t)(mx + 1) * 4, tag, err, spin_limit, nxt)) return;
+        }
+      }
+      n.aT = my > 0;
+      n.aTL = my > 0 && mx > 0;
+      n.aTR = my > 0 && mx + 1 < mbw;
+#pragma unroll
+      for (int i = 0; i < 4; i++) { n.Tx[i] = sext12(cur[i] >> 12); n.Ty[i] = sext12(cur[i]); }
+      n.TLx = sext12(prev[3] >> 12); n.TLy = sext12(prev[3]);
+      n.TRx = sext12(nxt[0] >> 12); n.TRy = sext12(nxt[0]);
+      const uint8_t *src = J.src + (size_t)(16 * my + PCME_PAD) * stride + 16 * mx + PCME_PAD;
+      MbRes o;
+      analyse_mb(c, n, src, stride, 16 * mx, 16 * my, J.partitions, o);
+      store_out(J.out + (size_t)my * mbw + mx, o);
+      // Publish the bottom row (blocks 12..15) for the row below.
+      if (l < 4) {
+        uint32_t wv = xword(tag, blk_mvx(o, 12 + l), blk_mvy(o, 12 + l));
+        __hip_atomic_store(xrow + (size_t)mx * 4 + l, wv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      // Left column (blocks 3, 7, 11, 15) for the next MB of this row.
+      n.aL = true;
+#pragma unroll
+      for (int i = 0; i < 4; i++) { n.Lx[i] = blk_mvx(o, 4 * i + 3); n.Ly[i] = blk_mvy(o, 4 * i + 3); }
+    }
+  }
+}
+
+// -------------------------------------------------- per-call search batch
+
+__global__ void __launch_bounds__(64) search_batch_kernel(const pcme_search_job *__restrict__ jobs,
+                                                          int njobs, pcme_mv *out_mv,
+                                                          uint32_t *out_cost) {
+  const int j = blockIdx.x;
+  if (j >= njobs) return;
+  const pcme_search_job &J = jobs[j];
+  SCtx c;
+  c.pl[0] = J.ref; c.pl[1] = J.ref_h; c.pl[2] = J.ref_v; c.pl[3] = J.ref_c;
+  c.stride = 16 * J.mbw + 2 * PCME_PAD;
+  c.pw = 16 * J.mbw;
+  c.vlim = (int)J.vlim;
+  c.lam = J.lam;
+  c.range = J.me_range;
+  c.subpel = J.subpel_rounds;
+  int c0x = J.ncands > 0 ? J.cands[0].x : 0, c0y = J.ncands > 0 ? J.cands[0].y : 0;
+  int c1x = J.ncands > 1 ? J.cands[1].x : 0, c1y = J.ncands > 1 ? J.cands[1].y : 0;
+  SRes r;
+  r.x = r.y = 0; r.cost = 0;
+  if (J.w == 16 && J.h == 16) {
+    Lane L = lane_layout<16, 16>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
+    r = search<16, 16>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands);
+  } else if (J.w == 16 && J.h == 8) {
+    Lane L = lane_layout<16, 8>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
+    r = search<16, 8>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands);
+  } else if (J.w == 8 && J.h == 16) {
+    Lane L = lane_layout<8, 16>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
+    r = search<8, 16>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands);
+  } else {
+    Lane L = lane_layout<8, 8>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
+    r = search<8, 8>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands);
+  }
+  if (lane_id() == 0) {
+    pcme_mv m;
+    m.x = (int16_t)r.x;
+    m.y = (int16_t)r.y;
+    out_mv[j] = m;
+    out_cost[j] = r.cost;
+  }
+}
+
+// ------------------------------------------------------------- pad kernel
+
+// One thread per 4 output bytes of a padded plane row; grid.z = plane.
+__global__ void __launch_bounds__(256) pad_planes_kernel(const uint8_t *__restrict__ src, int64_t src_pitch,
+                                                         int sstride, int w, int h, uint8_t *__restrict__ dst,
+                                                         int64_t dst_pitch, int dstride, int rows, int pad) {
+  const int gx = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  const int row = blockIdx.y;
+  const int pl = blockIdx.z;
+  if (gx >= dstride || row >= rows) return;
+  const uint8_t *s = src + (size_t)pl * src_pitch;
+  uint8_t *d = dst + (size_t)pl * dst_pitch + (size_t)row * dstride;
+  int y = row - pad;
+  y = y < 0 ? 0 : (y > h - 1 ? h - 1 : y);
+  const uint8_t *sr = s + (size_t)y * sstride;
+  uint32_t v = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    int x = gx + i - pad;
+    x = x < 0 ? 0 : (x > w - 1 ? w - 1 : x);
+    v |= (uint32_t)sr[x] << (8 * i);
+  }
+  if (gx + 4 <= dstride) {
+    *(uint32_t *)(d + gx) = v;
+  } else {
+    for (int i = 0; gx + i < dstride; i++) d[gx + i] = (uint8_t)(v >> (8 * i));
+  }
+}
+
+// ------------------------------------------------------------ hpel kernel
+
+// Tile of HT_W columns x HT_H rows per 256-thread block: source rows and
+// columns (+-2/3 taps, clamped to the plane) and the i16 horizontal
+// intermediates are staged in LDS; each thread then writes 4 consecutive
+// bytes of one row of each output plane per pass.
+__global__ void __launch_bounds__(256) hpel_kernel(const uint8_t *__restrict__ ref, uint8_t *__restrict__ ho,
+                                                   uint8_t *__restrict__ vo, uint8_t *__restrict__ co,
+                                                   int64_t pitch, int stride, int rows) {
+  __shared__ uint8_t s_src[HT_H + 5][HT_W + 8];
+  __shared__ int16_t s_h[HT_H + 5][HT_W];
+  const int x0 = blockIdx.x * HT_W, y0 = blockIdx.y * HT_H;
+  const size_t poff = (size_t)blockIdx.z * pitch;
+  const uint8_t *src = ref + poff;
+  const int t = threadIdx.x;
+  // Source rows y0-2 .. y0+HT_H+2, columns x0-2 .. x0+HT_W+5 (clamped).
+  for (int i = t; i < (HT_H + 5) * (HT_W + 8); i += 256) {
+    int r = i / (HT_W + 8), cx = i % (HT_W + 8);
+    int yy = y0 - 2 + r, xx = x0 - 2 + cx;
+    yy = yy < 0 ? 0 : (yy > rows - 1 ? rows - 1 : yy);
+    xx = xx < 0 ? 0 : (xx > stride - 1 ? stride - 1 : xx);
+    s_src[r][cx] = src[(size_t)yy * stride + xx];
+  }
+  __syncthreads();
+  // Horizontal intermediates (i16) for all HT_H+5 rows.
+  for (int i = t; i < (HT_H + 5) * HT_W; i += 256) {
+    int r = i / HT_W, cx = i % HT_W;
+    const uint8_t *e = &s_src[r][cx];  // e[0] = x-2
+    s_h[r][cx] = (int16_t)((int)e[0] + (int)e[5] - 5 * ((int)e[1] + (int)e[4]) + 20 * ((int)e[2] + (int)e[3]));
+  }
+  __syncthreads();
+  // Outputs: thread -> 4 consecutive columns of one row, 8 rows per pass.
+  const int cg = (t & 15) * 4;
+  for (int r = t >> 4; r < HT_H; r += 16) {
+    const int y = y0 + r;
+    if (y >= rows) break;
+    uint32_t hv = 0, vv = 0, cv = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int cx = cg + i;
+      const int k = r + 2;
+      int hval = ((int)s_h[k][cx] + 16) >> 5;
+      hval = hval < 0 ? 0 : (hval > 255 ? 255 : hval);
+      int sv = (int)s_src[k - 2][cx + 2] + (int)s_src[k + 3][cx + 2] -
+               5 * ((int)s_src[k - 1][cx + 2] + (int)s_src[k + 2][cx + 2]) +
+               20 * ((int)s_src[k][cx + 2] + (int)s_src[k + 1][cx + 2]);
+      int vval = (sv + 16) >> 5;
+      vval = vval < 0 ? 0 : (vval > 255 ? 255 : vval);
+      int sc = (int)s_h[k - 2][cx] + (int)s_h[k + 3][cx] - 5 * ((int)s_h[k - 1][cx] + (int)s_h[k + 2][cx]) +
+               20 * ((int)s_h[k][cx] + (int)s_h[k + 1][cx]);
+      int cval = (sc + 512) >> 10;
+      cval = cval < 0 ? 0 : (cval > 255 ? 255 : cval);
+      hv |= (uint32_t)hval << (8 * i);
+      vv |= (uint32_t)vval << (8 * i);
+      cv |= (uint32_t)cval << (8 * i);
+    }
+    const int x = x0 + cg;
+    const size_t o = poff + (size_t)y * stride + x;
+    if (x + 4 <= stride) {
+      *(uint32_t *)(ho + o) = hv;
+      *(uint32_t *)(vo + o) = vv;
+      *(uint32_t *)(co + o) = cv;
+    } else {
+      for (int i = 0; x + i < stride; i++) {
+        ho[o + i] = (uint8_t)(hv >> (8 * i));
+        vo[o + i] = (uint8_t)(vv >> (8 * i));
+        co[o + i] = (uint8_t)(cv >> (8 * i));
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ pred kernel
+
+// One 256-thread block per MB: threads make the 16x16 luma prediction
+// (mc_luma of the partition covering each pixel) and threads 0..127 the two
+// 8x8 chroma predictions (mc_chroma, picture.rs:451-487).
+__global__ void __launch_bounds__(256) pred_frame_kernel(const uint8_t *__restrict__ pl0, const uint8_t *__restrict__ pl1,
+                                                         const uint8_t *__restrict__ pl2, const uint8_t *__restrict__ pl3,
+                                                         const uint8_t *__restrict__ pu, const uint8_t *__restrict__ pv,
+                                                         const pcme_mb_out *__restrict__ mbs, int mbw, int mbh,
+                                                         uint8_t *__restrict__ oy, uint8_t *__restrict__ ou,
+                                                         uint8_t *__restrict__ ov) {
+  const int mb = blockIdx.x;
+  if (mb >= mbw * mbh) return;
+  const int mx = mb % mbw, my = mb / mbw;
+  const pcme_mb_out &m = mbs[mb];
+  const int t = threadIdx.x;
+  const int ys = 16 * mbw + 2 * PCME_PAD;
+  {
+    const int px = t & 15, py = t >> 4;
+    const pcme_mv mv = m.mv[(py >> 2) * 4 + (px >> 2)];
+    const int xi = 16 * mx + px + (mv.x >> 2), yi = 16 * my + py + (mv.y >> 2);
+    const uint32_t e = qpel_entry((uint32_t)((mv.y & 3) * 4 + (mv.x & 3)));
+    SCtx pc;
+    pc.pl[0] = pl0; pc.pl[1] = pl1; pc.pl[2] = pl2; pc.pl[3] = pl3;
+    const uint8_t a = plane_sel(pc, e & 3u)[(size_t)(yi + ((e >> 3) & 1) + PCME_PAD) * ys + xi + ((e >> 2) & 1) + PCME_PAD];
+    const uint8_t b = plane_sel(pc, (e >> 4) & 3u)[(size_t)(yi + ((e >> 7) & 1) + PCME_PAD) * ys + xi + ((e >> 6) & 1) + PCME_PAD];
+    oy[(size_t)mb * 256 + t] = (uint8_t)(((uint32_t)a + (uint32_t)b + 1u) >> 1);
+  }
+  if (t < 128) {
+    const int cs = 8 * mbw + 2 * PCME_CPAD;
+    const int k = t & 63;
+    const int cx = k & 7, cy = k >> 3;
+    const pcme_mv mv = m.mv[(cy >> 1) * 4 + (cx >> 1)];
+    const int xi = 8 * mx + cx + (mv.x >> 3), yi = 8 * my + cy + (mv.y >> 3);
+    const uint32_t fx = (uint32_t)(mv.x & 7), fy = (uint32_t)(mv.y & 7);
+    const uint8_t *p = (t < 64 ? pu : pv) + (size_t)(yi + PCME_CPAD) * cs + xi + PCME_CPAD;
+    const uint32_t v = ((8 - fx) * (8 - fy) * p[0] + fx * (8 - fy) * p[1] + (8 - fx) * fy * p[cs] +
+                        fx * fy * p[cs + 1] + 32) >> 6;
+    (t < 64 ? ou : ov)[(size_t)mb * 64 + k] = (uint8_t)v;
+  }
+}
+
+// Reset the ticket counter and the error flag; optionally clear the
+// exchange words (on tag wrap-around).
+__global__ void __launch_bounds__(256) prep_kernel(uint32_t *ctl, uint32_t *xchg, size_t nwords) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 2) ctl[i] = 0;
+  if (xchg) {
+    for (; i < nwords; i += (size_t)gridDim.x * blockDim.x) xchg[i] = 0;
+  }
+}
+
+}  // namespace pcme

@@ -1,0 +1,319 @@
+"""pcme -- Python binding of libpcme, the gfx950 H.264 P-frame motion search.
+
+libpcme's C ABI (include/pcme.h) is the product boundary; this module is a
+thin ctypes layer over it plus device-memory plumbing through PyTorch
+(tensors on ``cuda:0`` are only used as allocations and streams).
+
+The search is the reference encoder's ``SliceEnc::search`` /
+``search_part`` / partition choice (pito-codecs src/h264/enc/slice.rs:
+1590-1723, 1810-1868, 1921-1945) run as an ME-only field over whole P
+frames.  There is no CPU fallback: without the built library or a gfx950
+device every call raises ``PcmeError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "lib", "libpcme.so"))
+
+LUMA_PAD = 64
+CHROMA_PAD = 32
+
+PCME_OK = 0
+PCME_EINVAL = -1
+PCME_ENODEV = -2
+PCME_EHIP = -3
+PCME_ESTALL = -4
+PCME_ENOMEM = -5
+
+P16x16, P16x8, P8x16, P8x8 = 0, 1, 2, 3
+
+
+class PcmeError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"pcme error {code}: {msg}")
+        self.code = code
+
+
+# --------------------------------------------------------------- C structs
+
+class Mv(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_int16), ("y", ctypes.c_int16)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("mbw", ctypes.c_int32), ("mbh", ctypes.c_int32),
+                ("me_range", ctypes.c_int32), ("subpel_rounds", ctypes.c_int32),
+                ("partitions", ctypes.c_int32), ("qp", ctypes.c_int32)]
+
+
+class FrameJob(ctypes.Structure):
+    _fields_ = [("src", ctypes.c_void_p), ("ref", ctypes.c_void_p),
+                ("ref_h", ctypes.c_void_p), ("ref_v", ctypes.c_void_p),
+                ("ref_c", ctypes.c_void_p), ("out", ctypes.c_void_p),
+                ("prm", Params)]
+
+
+class SearchJob(ctypes.Structure):
+    _fields_ = [("src", ctypes.c_void_p), ("sstride", ctypes.c_int32),
+                ("ref", ctypes.c_void_p), ("ref_h", ctypes.c_void_p),
+                ("ref_v", ctypes.c_void_p), ("ref_c", ctypes.c_void_p),
+                ("mbw", ctypes.c_int32), ("mbh", ctypes.c_int32),
+                ("bx", ctypes.c_int32), ("by", ctypes.c_int32),
+                ("w", ctypes.c_int32), ("h", ctypes.c_int32),
+                ("mvp", Mv), ("cands", Mv * 2), ("ncands", ctypes.c_int32),
+                ("lam", ctypes.c_uint32), ("me_range", ctypes.c_int32),
+                ("subpel_rounds", ctypes.c_int32), ("vlim", ctypes.c_int64)]
+
+
+class SynthCfg(ctypes.Structure):
+    _fields_ = [("w", ctypes.c_int32), ("h", ctypes.c_int32),
+                ("frames", ctypes.c_int32), ("nrect", ctypes.c_int32),
+                ("rect_lo", ctypes.c_int32), ("rect_hi", ctypes.c_int32),
+                ("vmax", ctypes.c_float), ("pan_max", ctypes.c_float),
+                ("zoom", ctypes.c_float), ("noise_sigma", ctypes.c_float),
+                ("seed", ctypes.c_uint64)]
+
+
+# pcme_mb_out as a numpy record (128 bytes)
+MB_DTYPE = np.dtype([
+    ("mv", "<i2", (16, 2)), ("mvs", "<i2", (4, 2)), ("mvps", "<i2", (4, 2)),
+    ("cost", "<u4", (4,)), ("best_cost", "<u4"), ("skip_mv", "<i2", (2,)),
+    ("mv16", "<i2", (2,)), ("part", "u1"), ("reserved", "u1", (3,)),
+])
+assert MB_DTYPE.itemsize == 128
+
+# Every symbol include/pcme.h and include/pcme_synth.h declare.
+EXPORTS = (
+    "pcme_version", "pcme_last_error", "pcme_device_count", "pcme_lambda",
+    "pcme_qpel_table", "pcme_pad_planes", "pcme_hpel", "pcme_search_frames",
+    "pcme_search_frames_sync", "pcme_search_batch", "pcme_pred_frame",
+    "pcme_synth_clip",
+)
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libpcme.so (built in-tree).  PyTorch is imported first so that
+    the library binds to the same HIP runtime as torch's allocations."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    try:
+        import torch  # noqa: F401  (shares its libamdhip64 with libpcme)
+    except Exception:  # pragma: no cover - torch is part of the image
+        pass
+    if not os.path.exists(LIB_PATH):
+        raise PcmeError(PCME_ENODEV, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    lib = ctypes.CDLL(LIB_PATH)
+    c = ctypes
+    vp = c.c_void_p
+    sig = {
+        "pcme_version": (c.c_int, []),
+        "pcme_last_error": (c.c_char_p, []),
+        "pcme_device_count": (c.c_int, []),
+        "pcme_lambda": (c.c_uint32, [c.c_int32]),
+        "pcme_qpel_table": (None, [c.POINTER(c.c_uint32)]),
+        "pcme_pad_planes": (c.c_int, [vp, c.c_int64, c.c_int32, c.c_int32, c.c_int32, vp, c.c_int64,
+                                      c.c_int32, c.c_int32, c.c_int32, c.c_int32, c.c_int32, vp]),
+        "pcme_hpel": (c.c_int, [vp, vp, vp, vp, c.c_int64, c.c_int32, c.c_int32, c.c_int32, vp]),
+        "pcme_search_frames": (c.c_int, [c.POINTER(FrameJob), c.c_int, vp]),
+        "pcme_search_frames_sync": (c.c_int, [c.POINTER(FrameJob), c.c_int, vp]),
+        "pcme_search_batch": (c.c_int, [c.POINTER(SearchJob), c.c_int, vp, vp, vp]),
+        "pcme_pred_frame": (c.c_int, [vp, vp, vp, vp, vp, vp, vp, c.c_int32, c.c_int32, vp, vp, vp, vp]),
+        "pcme_synth_clip": (c.c_int, [c.POINTER(SynthCfg), vp, vp, vp, c.c_int32]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != PCME_OK:
+        msg = load().pcme_last_error().decode(errors="replace")
+        raise PcmeError(rc, msg)
+
+
+def lam(qp: int) -> int:
+    """Lambdas::new().sad[qp] (slice.rs:52-68)."""
+    return int(load().pcme_lambda(int(qp)))
+
+
+def qpel_table() -> list[int]:
+    out = (ctypes.c_uint32 * 4)()
+    load().pcme_qpel_table(out)
+    return list(out)
+
+
+def device_count() -> int:
+    return int(load().pcme_device_count())
+
+
+# ----------------------------------------------------------- synthetic video
+
+@dataclass
+class ClipSpec:
+    """A seeded synthetic clip (include/pcme_synth.h)."""
+    w: int
+    h: int
+    frames: int
+    nrect: int = 4
+    rect_lo: int = 24
+    rect_hi: int = 80
+    vmax: float = 3.0
+    pan_max: float = 1.0
+    zoom: float = 0.0
+    noise_sigma: float = 2.0
+    seed: int = 0x5EED0000
+
+
+def synth_clip(spec: ClipSpec, threads: int = 0):
+    """Generate the clip on the host: returns (Y[f,h,w], U[f,h/2,w/2], V)."""
+    lib = load()
+    y = np.empty((spec.frames, spec.h, spec.w), np.uint8)
+    u = np.empty((spec.frames, spec.h // 2, spec.w // 2), np.uint8)
+    v = np.empty_like(u)
+    cfg = SynthCfg(spec.w, spec.h, spec.frames, spec.nrect, spec.rect_lo, spec.rect_hi,
+                   spec.vmax, spec.pan_max, spec.zoom, spec.noise_sigma, spec.seed)
+    rc = lib.pcme_synth_clip(ctypes.byref(cfg), y.ctypes.data, u.ctypes.data, v.ctypes.data, threads)
+    _check(rc)
+    return y, u, v
+
+
+# --------------------------------------------------------------- geometry
+
+def mb_dims(w: int, h: int) -> tuple[int, int]:
+    return (w + 15) // 16, (h + 15) // 16
+
+
+def luma_geom(mbw: int, mbh: int) -> tuple[int, int]:
+    """(stride, rows) of a padded luma plane (picture.rs:150-155)."""
+    return 16 * mbw + 2 * LUMA_PAD, 16 * mbh + 2 * LUMA_PAD
+
+
+def chroma_geom(mbw: int, mbh: int) -> tuple[int, int]:
+    return 8 * mbw + 2 * CHROMA_PAD, 8 * mbh + 2 * CHROMA_PAD
+
+
+# ---------------------------------------------------------- device plumbing
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise PcmeError(PCME_ENODEV, "no GPU visible to torch")
+    return torch
+
+
+def _stream_ptr(stream) -> int | None:
+    if stream is None:
+        torch = _torch()
+        return torch.cuda.current_stream().cuda_stream
+    return int(stream)
+
+
+class DeviceClip:
+    """A clip resident in HBM: padded luma planes, their half-pel planes and
+    the padded chroma planes, laid out frame after frame."""
+
+    def __init__(self, y_frames, mbw: int, mbh: int, u_frames=None, v_frames=None):
+        torch = _torch()
+        self.torch = torch
+        self.n, self.h, self.w = (int(s) for s in y_frames.shape)
+        self.mbw, self.mbh = mbw, mbh
+        self.stride, self.rows = luma_geom(mbw, mbh)
+        self.pitch = self.stride * self.rows
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self.raw = torch.as_tensor(np.ascontiguousarray(y_frames)).to(dev) if not torch.is_tensor(y_frames) \
+            else y_frames.contiguous().to(dev)
+        self.planes = torch.empty((4, self.n, self.rows, self.stride), dtype=torch.uint8, device=dev)
+        self.has_chroma = u_frames is not None
+        if self.has_chroma:
+            self.cstride, self.crows = chroma_geom(mbw, mbh)
+            self.craw = [torch.as_tensor(np.ascontiguousarray(a)).to(dev) if not torch.is_tensor(a)
+                         else a.contiguous().to(dev) for a in (u_frames, v_frames)]
+            self.cplanes = torch.empty((2, self.n, self.crows, self.cstride), dtype=torch.uint8, device=dev)
+
+    def plane_ptr(self, k: int, i: int) -> int:
+        return self.planes[k, i].data_ptr()
+
+    def pad(self, stream=None) -> None:
+        """Import + edge-extend every frame (mod.rs:842-873, frame.rs:58-97)."""
+        lib, s = load(), _stream_ptr(stream)
+        _check(lib.pcme_pad_planes(self.raw.data_ptr(), self.h * self.w, self.w, self.w, self.h,
+                                   self.planes[0].data_ptr(), self.pitch, self.stride,
+                                   16 * self.mbw, 16 * self.mbh, LUMA_PAD, self.n, s))
+        if self.has_chroma:
+            ch, cw = self.h // 2, self.w // 2
+            for k in range(2):
+                _check(lib.pcme_pad_planes(self.craw[k].data_ptr(), ch * cw, cw, cw, ch,
+                                           self.cplanes[k].data_ptr(), self.crows * self.cstride,
+                                           self.cstride, 8 * self.mbw, 8 * self.mbh, CHROMA_PAD,
+                                           self.n, s))
+
+    def hpel(self, first: int = 0, count: int | None = None, stream=None) -> None:
+        """Half-pel planes of frames [first, first+count) (picture.rs:318-380)."""
+        count = self.n - first if count is None else count
+        if count <= 0:
+            return
+        lib, s = load(), _stream_ptr(stream)
+        _check(lib.pcme_hpel(self.plane_ptr(0, first), self.plane_ptr(1, first), self.plane_ptr(2, first),
+                             self.plane_ptr(3, first), self.pitch, count, self.mbw, self.mbh, s))
+
+
+def frame_job(clip: DeviceClip, cur: int, ref: int, out_ptr: int, me_range: int, subpel_rounds: int,
+              partitions: bool, qp: int, ref_clip: DeviceClip | None = None) -> FrameJob:
+    rc = clip if ref_clip is None else ref_clip
+    return FrameJob(clip.plane_ptr(0, cur), rc.plane_ptr(0, ref), rc.plane_ptr(1, ref),
+                    rc.plane_ptr(2, ref), rc.plane_ptr(3, ref), out_ptr,
+                    Params(clip.mbw, clip.mbh, me_range, subpel_rounds, 1 if partitions else 0, qp))
+
+
+def search_frames(jobs: list[FrameJob], stream=None, sync: bool = True) -> None:
+    """pcme_search_frames[_sync] over a list of FrameJob."""
+    lib = load()
+    arr = (FrameJob * len(jobs))(*jobs)
+    s = _stream_ptr(stream)
+    fn = lib.pcme_search_frames_sync if sync else lib.pcme_search_frames
+    _check(fn(arr, len(jobs), s))
+
+
+def search_batch(jobs: list[SearchJob], out_mv_ptr: int, out_cost_ptr: int, stream=None) -> None:
+    lib = load()
+    arr = (SearchJob * len(jobs))(*jobs)
+    _check(lib.pcme_search_batch(arr, len(jobs), out_mv_ptr, out_cost_ptr, _stream_ptr(stream)))
+
+
+def pred_frame(clip: DeviceClip, ref: int, mb_ptr: int, pred_y_ptr: int, pred_u_ptr: int,
+               pred_v_ptr: int, stream=None) -> None:
+    if not clip.has_chroma:
+        raise PcmeError(PCME_EINVAL, "clip has no chroma planes")
+    lib = load()
+    _check(lib.pcme_pred_frame(clip.plane_ptr(0, ref), clip.plane_ptr(1, ref), clip.plane_ptr(2, ref),
+                               clip.plane_ptr(3, ref), clip.cplanes[0, ref].data_ptr(),
+                               clip.cplanes[1, ref].data_ptr(), mb_ptr, clip.mbw, clip.mbh,
+                               pred_y_ptr, pred_u_ptr, pred_v_ptr, _stream_ptr(stream)))
+
+
+def search_clip_me(clip: DeviceClip, me_range: int, subpel_rounds: int, partitions: bool, qp: int,
+                   stream=None):
+    """ME-only field of every P frame of an IPPP clip (frame t searched
+    against frame t-1).  Returns the (n-1, mbh, mbw) MB_DTYPE records."""
+    torch = clip.torch
+    clip.pad(stream)
+    clip.hpel(0, clip.n - 1, stream)
+    nb = clip.n - 1
+    out = torch.empty((max(nb, 1), clip.mbh * clip.mbw * MB_DTYPE.itemsize), dtype=torch.uint8,
+                      device=clip.raw.device)
+    jobs = [frame_job(clip, t, t - 1, out[t - 1].data_ptr(), me_range, subpel_rounds, partitions, qp)
+            for t in range(1, clip.n)]
+    search_frames(jobs, stream, sync=True)
+    return out[:nb].cpu().numpy().view(MB_DTYPE).reshape(nb, clip.mbh, clip.mbw)

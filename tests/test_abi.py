@@ -1,0 +1,117 @@
+"""The C-ABI library: loads, exports every declared symbol, host-side
+helpers and argument checks (no GPU needed)."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for hdr in ("pcme.h", "pcme_synth.h"):
+        text = open(os.path.join(ROOT, "include", hdr)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(pcme_\w+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol(pcme_lib):
+    lib = pcme_lib.load()
+    names = declared_symbols()
+    assert names, "no declarations parsed"
+    assert names == set(pcme_lib.EXPORTS), names ^ set(pcme_lib.EXPORTS)
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+
+
+def test_version_and_lambda(pcme_lib, oracle):
+    assert pcme_lib.load().pcme_version() == 1
+    lib = oracle.load()
+    for qp in range(52):
+        assert pcme_lib.lam(qp) == lib.o_lambda(qp)
+
+
+def test_qpel_table_matches_spec():
+    """The packed table the kernels use equals H.264 8.4.2.2.1's choice of
+    the two samples averaged for each quarter position (planes: 0 = G full,
+    1 = b (H), 2 = h (V), 3 = j (C); offsets +1 in x or y mean the sample to
+    the right / below)."""
+    import pcme
+    words = pcme.qpel_table()
+    got = [(w >> (8 * i)) & 0xFF for w in words for i in range(4)]
+
+    def enc(a, b):
+        return a[0] | a[1] << 2 | a[2] << 3 | b[0] << 4 | b[1] << 6 | b[2] << 7
+
+    G, H, V, C = 0, 1, 2, 3
+    spec = {
+        (0, 0): ((G, 0, 0), (G, 0, 0)),  # G
+        (1, 0): ((G, 0, 0), (H, 0, 0)),  # a = (G + b + 1) >> 1
+        (2, 0): ((H, 0, 0), (H, 0, 0)),  # b
+        (3, 0): ((H, 0, 0), (G, 1, 0)),  # c = (H + b + 1) >> 1, H = G right
+        (0, 1): ((G, 0, 0), (V, 0, 0)),  # d = (G + h + 1) >> 1
+        (1, 1): ((H, 0, 0), (V, 0, 0)),  # e = (b + h + 1) >> 1
+        (2, 1): ((H, 0, 0), (C, 0, 0)),  # f = (b + j + 1) >> 1
+        (3, 1): ((H, 0, 0), (V, 1, 0)),  # g = (b + m + 1) >> 1, m = h right
+        (0, 2): ((V, 0, 0), (V, 0, 0)),  # h
+        (1, 2): ((V, 0, 0), (C, 0, 0)),  # i = (h + j + 1) >> 1
+        (2, 2): ((C, 0, 0), (C, 0, 0)),  # j
+        (3, 2): ((C, 0, 0), (V, 1, 0)),  # k = (j + m + 1) >> 1
+        (0, 3): ((V, 0, 0), (G, 0, 1)),  # n = (M + h + 1) >> 1, M = G below
+        (1, 3): ((V, 0, 0), (H, 0, 1)),  # p = (h + s + 1) >> 1, s = b below
+        (2, 3): ((C, 0, 0), (H, 0, 1)),  # q = (j + s + 1) >> 1
+        (3, 3): ((V, 1, 0), (H, 0, 1)),  # r = (m + s + 1) >> 1
+    }
+    for (fx, fy), (a, b) in spec.items():
+        q = fy * 4 + fx
+        assert got[q] == enc(a, b), f"q={q} ({fx},{fy})"
+
+
+def test_synth_is_deterministic_and_in_range(pcme_lib):
+    spec = pcme_lib.ClipSpec(64, 48, 3, nrect=2, rect_lo=8, rect_hi=20, vmax=3.0)
+    a = pcme_lib.synth_clip(spec, threads=1)
+    b = pcme_lib.synth_clip(spec, threads=3)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    assert a[0].shape == (3, 48, 64) and a[1].shape == (3, 24, 32)
+    other = pcme_lib.synth_clip(pcme_lib.ClipSpec(64, 48, 3, nrect=2, rect_lo=8, rect_hi=20, seed=1))
+    assert not np.array_equal(other[0], a[0])
+
+
+def test_synth_rejects_bad_config(pcme_lib):
+    with pytest.raises(pcme_lib.PcmeError):
+        pcme_lib.synth_clip(pcme_lib.ClipSpec(63, 48, 1))
+
+
+def test_compute_entry_points_fail_loudly_without_device(pcme_lib):
+    """No silent fallback: without a gfx950 device the search refuses."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    lib = pcme_lib.load()
+    job = pcme_lib.FrameJob()
+    job.prm = pcme_lib.Params(1, 1, 16, 1, 1, 28)
+    for f in ("src", "ref", "ref_h", "ref_v", "ref_c", "out"):
+        setattr(job, f, 64)
+    arr = (pcme_lib.FrameJob * 1)(job)
+    rc = lib.pcme_search_frames(arr, 1, None)
+    assert rc == pcme_lib.PCME_ENODEV, rc
+    assert b"device" in lib.pcme_last_error()
+
+
+def test_search_frames_rejects_bad_params(pcme_lib):
+    lib = pcme_lib.load()
+    job = pcme_lib.FrameJob()
+    job.prm = pcme_lib.Params(1, 1, 0, 1, 1, 28)  # me_range 0
+    for f in ("src", "ref", "ref_h", "ref_v", "ref_c", "out"):
+        setattr(job, f, 64)
+    arr = (pcme_lib.FrameJob * 1)(job)
+    rc = lib.pcme_search_frames(arr, 1, None)
+    assert rc in (pcme_lib.PCME_EINVAL, pcme_lib.PCME_ENODEV)
