@@ -87,6 +87,22 @@ def test_search_frames_match_oracle(gpu, oracle, case):
     assert_same_field(got, want, f"{w}x{h}")
 
 
+def test_search_frames_golden(gpu):
+    """The committed golden vectors (tests/golden/), through the GPU path."""
+    import os
+    gdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    for f in sorted(x for x in os.listdir(gdir) if x.endswith(".npz")):
+        say("golden", f)
+        z = np.load(os.path.join(gdir, f))
+        frames, prm = z["frames"], z["params"]
+        n, h, w = frames.shape
+        mbw, mbh = gpu.mb_dims(w, h)
+        dc = gpu.DeviceClip(frames, mbw, mbh)
+        got = gpu.search_clip_me(dc, int(prm[0]), int(prm[1]), bool(prm[2]), int(prm[3]))
+        want = z["expected"].view(gpu.MB_DTYPE).reshape(got.shape)
+        assert_same_field(got, want, f)
+
+
 @pytest.mark.parametrize("scene", ["flat", "gradient", "noise", "motion"])
 def test_search_frames_reference_scenes(gpu, oracle, scene):
     say("scene", scene)
