@@ -836,6 +836,10 @@ __global__ void __launch_bounds__(256) hpel_kernel(const uint8_t *__restrict__ r
                20 * ((int)s_h[k][cx] + (int)s_h[k + 1][cx]);
       int cval = (sc + 512) >> 10;
       cval = cval < 0 ? 0 : (cval > 255 ? 255 : cval);
+      // Keep each clamped byte opaque: fused into v_ashr_pk_u8_i32 and
+      // packed with v_lshl_or_b32 the upper bytes came out 0xFF on gfx950
+      // (the fused op's bits 16-31 were not zero).
+      asm volatile("" : "+v"(hval), "+v"(vval), "+v"(cval));
       hv |= (uint32_t)hval << (8 * i);
       vv |= (uint32_t)vval << (8 * i);
       cv |= (uint32_t)cval << (8 * i);
