@@ -40,6 +40,9 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=16,
                     help="P frames of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a kernel of known "
+                         "traffic, for scaling the FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -57,6 +60,12 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.copy_calib_mib:
+        a = torch.empty(args.copy_calib_mib << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     spec = pcme.ClipSpec(W, H, FRAMES, nrect=12, rect_lo=64, rect_hi=400, vmax=8.0, pan_max=1.0,
                          zoom=0.0, noise_sigma=2.0, seed=0x5EED0000)
     t0 = time.time()

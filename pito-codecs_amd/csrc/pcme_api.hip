@@ -163,7 +163,9 @@ static int search_frames_impl(const pcme_frame_job *jobs, int njobs, hipStream_t
     if ((rc = check_params(jobs[i].prm))) return rc;
     if (!jobs[i].src || !jobs[i].ref || !jobs[i].ref_h || !jobs[i].ref_v || !jobs[i].ref_c || !jobs[i].out)
       return fail(PCME_EINVAL, "job %d has a null pointer", i);
-    if (((uintptr_t)jobs[i].src & 3u) != 0) return fail(PCME_EINVAL, "job %d: src not 4-byte aligned", i);
+    if ((((uintptr_t)jobs[i].src | (uintptr_t)jobs[i].ref | (uintptr_t)jobs[i].ref_h | (uintptr_t)jobs[i].ref_v |
+          (uintptr_t)jobs[i].ref_c) & 3u) != 0)
+      return fail(PCME_EINVAL, "job %d: plane pointers must be 4-byte aligned", i);
   }
   Dev *dv = nullptr;
   if ((rc = get_dev(&dv))) return rc;
@@ -341,6 +343,8 @@ int pcme_search_batch(const pcme_search_job *jobs, int njobs, pcme_mv *out_mv, u
     if (!shape) return fail(PCME_EINVAL, "job %d: block %dx%d not one of 16x16/16x8/8x16/8x8", i, J.w, J.h);
     if (!J.src || !J.ref || !J.ref_h || !J.ref_v || !J.ref_c)
       return fail(PCME_EINVAL, "job %d has a null pointer", i);
+    if ((((uintptr_t)J.ref | (uintptr_t)J.ref_h | (uintptr_t)J.ref_v | (uintptr_t)J.ref_c) & 3u) != 0)
+      return fail(PCME_EINVAL, "job %d: reference planes must be 4-byte aligned", i);
     if (((uintptr_t)J.src & 3u) || (J.sstride & 3) || J.sstride < J.w)
       return fail(PCME_EINVAL, "job %d: src must be 4-byte aligned with a stride multiple of 4", i);
     if (J.mbw < 1 || J.mbh < 1 || J.bx < 0 || J.by < 0 || J.bx + J.w > 16 * J.mbw || J.by + J.h > 16 * J.mbh)

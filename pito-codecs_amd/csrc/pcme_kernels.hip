@@ -58,18 +58,6 @@ __device__ __forceinline__ uint32_t se_bits(int v) {
   return 2u * (31u - (uint32_t)__builtin_clz(k + 1u)) + 1u;
 }
 
-// 8 bytes from an arbitrary address of a padded plane, as two dwords, read
-// with three aligned dword loads (in bounds: every caller keeps 3 bytes of
-// plane after the window, guaranteed by the 64-pixel pad and mv_ok).
-__device__ __forceinline__ void ld8u(const uint8_t *p, uint32_t &lo, uint32_t &hi) {
-  uintptr_t a = (uintptr_t)p;
-  const uint32_t *q = (const uint32_t *)(a & ~(uintptr_t)3);
-  uint32_t sh = (uint32_t)a & 3u;
-  uint32_t d0 = q[0], d1 = q[1], d2 = q[2];
-  lo = __builtin_amdgcn_alignbyte(d1, d0, sh);
-  hi = __builtin_amdgcn_alignbyte(d2, d1, sh);
-}
-
 typedef short v2s __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ v2s as_v2s(uint32_t u) { return __builtin_bit_cast(v2s, u); }
 
@@ -127,6 +115,7 @@ struct SCtx {            // wave-uniform
   const uint8_t *pl[4];  // padded planes: full, H, V, C
   int stride;
   int pw;                // 16*mbw
+  int ph;                // 16*mbh
   int vlim;
   uint32_t lam;
   int range;
@@ -169,8 +158,16 @@ __device__ __forceinline__ Lane lane_layout() {
 
 struct SrcRegs {
   uint32_t raw[8];  // row r: raw[2r] cols 0-3, raw[2r+1] cols 4-7
-  v2s sp[16];       // u16 pairs for SATD
 };
+
+// u16 pairs (left block, right block) of the source strip, for SATD.
+__device__ __forceinline__ void src_pairs(const SrcRegs &s, v2s *sp) {
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+      sp[r * 4 + c] = as_v2s(__builtin_amdgcn_perm(s.raw[2 * r + 1], s.raw[2 * r], SELC(c)));
+}
 
 // Source strip of the lane; blk = block top-left, 4-byte aligned, sstride a
 // multiple of 4.
@@ -181,26 +178,79 @@ __device__ __forceinline__ void load_src(const uint8_t *blk, int sstride, const 
     s.raw[2 * r] = q[0];
     s.raw[2 * r + 1] = q[1];
   }
+}
+
+// ---- staged windows (LDS) ------------------------------------------------
+// Per wave: the full-pel window of the integer phase and the four planes
+// around the integer best for the sub-pel phase.  Before every pass a
+// wave-uniform check re-stages the region when the pass's candidates could
+// leave it; a region centred on the current best always covers every legal
+// candidate of the next pass (mv_ok keeps blocks 44 px inside the 64 px pad),
+// so all hex, square and sub-pel passes read LDS.
+#define WIN_M 10
+#define WIN_W 44   // 16 + 2*WIN_M + 8 bytes of alignment / over-read slack
+#define WIN_H 37   // 16 + 2*WIN_M + 1
+#define SP_W 32
+#define SP_H 21
+#define WIN_BYTES (WIN_W * WIN_H)
+#define SP_PLANE (SP_W * SP_H)
+#define LDS_PER_WAVE (((WIN_BYTES + 4 * SP_PLANE) + 15) & ~15)
+
+typedef const __attribute__((address_space(1))) uint32_t gu32;
+
+// 8 bytes at byte offset `off` of a 4-byte-aligned base (global or LDS),
+// read as three aligned dwords and shifted into place with v_alignbyte.
+template <typename P>
+__device__ __forceinline__ void ld8_at(P base, int off, uint32_t &lo, uint32_t &hi) {
+  const int a = off >> 2;
+  const uint32_t sh = (uint32_t)off & 3u;
+  uint32_t d0 = base[a], d1 = base[a + 1], d2 = base[a + 2];
+  lo = __builtin_amdgcn_alignbyte(d1, d0, sh);
+  hi = __builtin_amdgcn_alignbyte(d2, d1, sh);
+}
+
+__device__ __forceinline__ gu32 *gplane(const uint8_t *p) { return (gu32 *)p; }
+
+// Copy a DW x DH byte region (DW multiple of 4, origin ox 4-aligned, picture
+// coordinates) of a padded plane into LDS.  All loads are issued before the
+// stores (fully unrolled).
+template <int DW, int DH>
+__device__ __forceinline__ void stage_region(const uint8_t *plane, int stride, int ox, int oy, uint32_t *dst) {
+  constexpr int DPR = DW / 4, N = DPR * DH, IT = (N + 63) / 64;
+  const int l = (int)lane_id();
+  gu32 *src = gplane(plane);
+  const int base = ((oy + PCME_PAD) * stride + ox + PCME_PAD) >> 2;
+  const int sdw = stride >> 2;
+  uint32_t v[IT];
 #pragma unroll
-  for (int r = 0; r < 4; r++)
+  for (int k = 0; k < IT; k++) {
+    int i = l + 64 * k;
+    int r = i / DPR, cc = i - r * DPR;
+    v[k] = i < N ? src[base + r * sdw + cc] : 0u;
+  }
 #pragma unroll
-    for (int c = 0; c < 4; c++)
-      s.sp[r * 4 + c] = as_v2s(__builtin_amdgcn_perm(s.raw[2 * r + 1], s.raw[2 * r], SELC(c)));
+  for (int k = 0; k < IT; k++) {
+    int i = l + 64 * k;
+    if (i < N) dst[i] = v[k];
+  }
 }
 
 // Integer-pel SAD cost of the lane's slot candidate (slice.rs:1605-1616):
 // SAD over the block + (lam * mv_bits) >> 8.  (ux, uy) is the address MV:
 // the candidate when valid, else a valid fallback (its cost is ignored).
-__device__ __forceinline__ uint32_t sad_cost(const SCtx &c, const Lane &L, const SrcRegs &s, int bx,
-                                             int by, int cx, int cy, int ux, int uy, int mvpx,
-                                             int mvpy) {
-  int x = bx + L.sx + (ux >> 2), y = by + L.sy + (uy >> 2);
-  const uint8_t *p = c.pl[0] + (size_t)(y + PCME_PAD) * c.stride + (x + PCME_PAD);
+template <bool LDS>
+__device__ __forceinline__ uint32_t sad_cost(const SCtx &c, const Lane &L, const SrcRegs &s, int bx, int by,
+                                             int cx, int cy, int ux, int uy, int mvpx, int mvpy,
+                                             const uint32_t *lw, int wx0, int wy0) {
+  const int x = bx + L.sx + (ux >> 2), y = by + L.sy + (uy >> 2);
   uint32_t acc = 0;
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     uint32_t a, b;
-    ld8u(p + r * c.stride, a, b);
+    if (LDS)
+      ld8_at(lw, (y - wy0 + r) * WIN_W + (x - wx0), a, b);
+    else
+      ld8_at(gplane(c.pl[0]), (y + r + PCME_PAD) * c.stride + x + PCME_PAD, a, b);
     acc = __builtin_amdgcn_sad_u8(a, s.raw[2 * r], acc);
     acc = __builtin_amdgcn_sad_u8(b, s.raw[2 * r + 1], acc);
   }
@@ -216,29 +266,25 @@ __device__ __forceinline__ const uint8_t *plane_sel(const SCtx &c, uint32_t p) {
 }
 
 // Sub-pel cost (slice.rs:1684-1687): mc_luma (picture.rs:415-449) as the
-// rounding average of two planes, SATD, + (lam * mv_bits) >> 8.  Full and
-// half positions read the same plane twice, and (a+a+1)>>1 == a.
-__device__ __forceinline__ uint32_t satd_cost(const SCtx &c, const Lane &L, const SrcRegs &s, int bx,
-                                              int by, int cx, int cy, int ux, int uy, int mvpx,
-                                              int mvpy) {
-  int xi = bx + L.sx + (ux >> 2), yi = by + L.sy + (uy >> 2);
-  uint32_t e = qpel_entry((uint32_t)((uy & 3) * 4 + (ux & 3)));
-  const uint8_t *A = plane_sel(c, e & 3u) +
-                     (size_t)(yi + (int)((e >> 3) & 1u) + PCME_PAD) * c.stride +
-                     (xi + (int)((e >> 2) & 1u) + PCME_PAD);
-  const uint8_t *B = plane_sel(c, (e >> 4) & 3u) +
-                     (size_t)(yi + (int)((e >> 7) & 1u) + PCME_PAD) * c.stride +
-                     (xi + (int)((e >> 6) & 1u) + PCME_PAD);
+// rounding average of two of the staged planes, SATD, + (lam * mv_bits) >> 8.
+// Full and half positions read the same plane twice, and (a+a+1)>>1 == a.
+__device__ __forceinline__ uint32_t satd_cost(const SCtx &c, const Lane &L, const v2s *sp, int bx, int by, int cx,
+                                              int cy, int ux, int uy, int mvpx, int mvpy, const uint32_t *lsp,
+                                              int sx0, int sy0) {
+  const int xi = bx + L.sx + (ux >> 2) - sx0, yi = by + L.sy + (uy >> 2) - sy0;
+  const uint32_t e = qpel_entry((uint32_t)((uy & 3) * 4 + (ux & 3)));
+  const int oa = (int)(e & 3u) * SP_PLANE + (yi + (int)((e >> 3) & 1u)) * SP_W + xi + (int)((e >> 2) & 1u);
+  const int ob = (int)((e >> 4) & 3u) * SP_PLANE + (yi + (int)((e >> 7) & 1u)) * SP_W + xi + (int)((e >> 6) & 1u);
   uint32_t P[8];
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     uint32_t a0, a1, b0, b1;
-    ld8u(A + r * c.stride, a0, a1);
-    ld8u(B + r * c.stride, b0, b1);
+    ld8_at(lsp, oa + r * SP_W, a0, a1);
+    ld8_at(lsp, ob + r * SP_W, b0, b1);
     P[2 * r] = __builtin_amdgcn_lerp(a0, b0, 0x01010101u);
     P[2 * r + 1] = __builtin_amdgcn_lerp(a1, b1, 0x01010101u);
   }
-  uint32_t acc = satd_strip8(s.sp, P);
+  uint32_t acc = satd_strip8(sp, P);
   acc = L.active ? acc : 0u;
   acc = red8(acc);
   return acc + ((c.lam * (se_bits(cx - mvpx) + se_bits(cy - mvpy))) >> 8);
@@ -257,12 +303,48 @@ struct SRes {
   uint32_t cost;
 };
 
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Re-stage the full-pel window around the current best (bxv, byv) when a
+// pass reaching +-R px around it could leave the staged window.
+template <int W, int H>
+__device__ __forceinline__ void ensure_win(const SCtx &c, int bx, int by, int bxv, int byv, int R, int &wx0,
+                                           int &wy0, uint32_t *lw) {
+  const int X = bx + (bxv >> 2), Y = by + (byv >> 2);
+  const bool ok = X - R - wx0 >= 0 && X + R + W + 4 - wx0 <= WIN_W && Y - R - wy0 >= 0 && Y + R + H - wy0 <= WIN_H;
+  if (ok) return;
+  wx0 = clampi((X - WIN_M) & ~3, -PCME_PAD, c.pw + PCME_PAD - WIN_W);
+  wy0 = clampi(Y - WIN_M, -PCME_PAD, c.ph + PCME_PAD - WIN_H);
+  __syncthreads();
+  stage_region<WIN_W, WIN_H>(c.pl[0], c.stride, wx0, wy0, lw);
+  __syncthreads();
+}
+
+// Same for the four sub-pel planes: candidates of a round lie within +-1 px
+// of the current best, reading one more row/column for the half-pel offsets.
+template <int W, int H>
+__device__ __forceinline__ void ensure_sp(const SCtx &c, int bx, int by, int bxv, int byv, int &sx0, int &sy0,
+                                          uint32_t *lsp) {
+  const int X = bx + (bxv >> 2), Y = by + (byv >> 2);
+  const bool ok = X - 1 - sx0 >= 0 && X + W + 6 - sx0 <= SP_W && Y - 1 - sy0 >= 0 && Y + H + 2 - sy0 <= SP_H;
+  if (ok) return;
+  sx0 = clampi((X - 2) & ~3, -PCME_PAD, c.pw + PCME_PAD - SP_W);
+  sy0 = clampi(Y - 2, -PCME_PAD, c.ph + PCME_PAD - SP_H);
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < 4; p++) stage_region<SP_W, SP_H>(c.pl[p], c.stride, sx0, sy0, lsp + p * (SP_PLANE / 4));
+  __syncthreads();
+}
+
 // SliceEnc::search (slice.rs:1590-1723) for a W x H block at picture
 // position (bx, by).  mvp, cands are wave-uniform; returns (best, cost).
+// lds = this wave's LDS_PER_WAVE bytes.
 template <int W, int H>
-__device__ SRes search(const SCtx &c, const SrcRegs &s, const Lane &L, int bx, int by, int mvpx,
-                       int mvpy, int c0x, int c0y, int c1x, int c1y, int ncand) {
-  // slice.rs:1617-1631: rounded predictor, then the candidates.
+__device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const Lane &L, int bx, int by, int mvpx, int mvpy,
+                       int c0x, int c0y, int c1x, int c1y, int ncand, uint32_t *lds) {
+  uint32_t *lw = lds;
+  uint32_t *lsp = lds + WIN_BYTES / 4;
+  // slice.rs:1617-1631: rounded predictor, then the candidates (HBM reads).
   int bxv = (mvpx + 2) & ~3, byv = (mvpy + 2) & ~3;
   if (!mv_ok(c, bx, by, W, H, bxv, byv)) { bxv = 0; byv = 0; }
   int r0x = (c0x + 2) & ~3, r0y = (c0y + 2) & ~3;
@@ -273,7 +355,7 @@ __device__ SRes search(const SCtx &c, const SrcRegs &s, const Lane &L, int bx, i
     int cy = L.slot == 1 ? r0y : (L.slot == 2 ? r1y : byv);
     bool valid = L.slot == 0 || (L.slot <= ncand && mv_ok(c, bx, by, W, H, cx, cy));
     uint64_t vm = __ballot(valid);
-    uint32_t k = sad_cost(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy);
+    uint32_t k = sad_cost<false>(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy, lw, 0, 0);
     bcost = rdlane(k, 0);
     if (ncand >= 1 && ((vm >> 8) & 1u) && !(r0x == bxv && r0y == byv)) {
       uint32_t k1 = rdlane(k, 8);
@@ -284,17 +366,20 @@ __device__ SRes search(const SCtx &c, const SrcRegs &s, const Lane &L, int bx, i
       if (k2 < bcost) { bcost = k2; bxv = r1x; byv = r1y; }
     }
   }
+  // Full-pel window: nothing staged yet (an impossible origin forces it).
+  int wx0 = 1 << 20, wy0 = 1 << 20;
   // slice.rs:1632-1656: hexagon descent clamped to +-range around centre0.
   const int c0xx = bxv, c0yy = byv;
   const int range = c.range;
   const int hdx = hex_dx(L.slot & 7), hdy = hex_dy(L.slot & 7);
   for (int it = 0; it < range; it++) {
+    ensure_win<W, H>(c, bx, by, bxv, byv, 2, wx0, wy0, lw);
     int cx = bxv + hdx * 4, cy = byv + hdy * 4;
     int ddx = cx - c0xx, ddy = cy - c0yy;
     bool valid = L.slot < 6 && ((ddx < 0 ? -ddx : ddx) >> 2) <= range &&
                  ((ddy < 0 ? -ddy : ddy) >> 2) <= range && mv_ok(c, bx, by, W, H, cx, cy);
     uint64_t vm = __ballot(valid);
-    uint32_t k = sad_cost(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy);
+    uint32_t k = sad_cost<true>(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy, lw, wx0, wy0);
     const int ox = bxv, oy = byv;
     bool moved = false;
 #pragma unroll
@@ -314,10 +399,11 @@ __device__ SRes search(const SCtx &c, const SrcRegs &s, const Lane &L, int bx, i
   // slice.rs:1657-1682: two rounds of 8-neighbour integer refinement.
   const int qdx = sq_dx(L.slot), qdy = sq_dy(L.slot);
   for (int it = 0; it < 2; it++) {
+    ensure_win<W, H>(c, bx, by, bxv, byv, 1, wx0, wy0, lw);
     int cx = bxv + qdx * 4, cy = byv + qdy * 4;
     bool valid = mv_ok(c, bx, by, W, H, cx, cy);
     uint64_t vm = __ballot(valid);
-    uint32_t k = sad_cost(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy);
+    uint32_t k = sad_cost<true>(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy, lw, wx0, wy0);
     const int ox = bxv, oy = byv;
 #pragma unroll
     for (int j = 0; j < 8; j++) {
@@ -332,17 +418,24 @@ __device__ SRes search(const SCtx &c, const SrcRegs &s, const Lane &L, int bx, i
     }
     if (bxv == ox && byv == oy) break;
   }
+  // Stage the four planes around the integer best for the sub-pel phase.
+  int sx0 = 1 << 20, sy0 = 1 << 20;
+  ensure_sp<W, H>(c, bx, by, bxv, byv, sx0, sy0, lsp);
+  v2s sp[16];
+  src_pairs(s, sp);
   // slice.rs:1688: SATD cost at the integer best.
-  uint32_t bs = rdlane(satd_cost(c, L, s, bx, by, bxv, byv, bxv, byv, mvpx, mvpy), 0);
+  uint32_t bs = rdlane(satd_cost(c, L, sp, bx, by, bxv, byv, bxv, byv, mvpx, mvpy, lsp, sx0, sy0), 0);
   // slice.rs:1689-1721: half-pel round, then 1 + subpel_rounds quarter-pel.
 #pragma unroll
   for (int step = 2; step >= 1; step--) {
     int rounds = step == 1 ? 1 + c.subpel : 1;
     for (int it = 0; it < rounds; it++) {
+      ensure_sp<W, H>(c, bx, by, bxv, byv, sx0, sy0, lsp);
       int cx = bxv + qdx * step, cy = byv + qdy * step;
       bool valid = mv_ok(c, bx, by, W, H, cx, cy);
       uint64_t vm = __ballot(valid);
-      uint32_t k = satd_cost(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy);
+      uint32_t k = satd_cost(c, L, sp, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy, lsp,
+                             sx0, sy0);
       const int ox = bxv, oy = byv;
 #pragma unroll
       for (int j = 0; j < 8; j++) {
@@ -482,7 +575,7 @@ struct PartRes {
 template <int PART, int P>
 __device__ __forceinline__ void part_step(const SCtx &c, const Nbr &n, Cur &cur, const uint8_t *src,
                                           int sstride, int mbx, int mby, int seedx, int seedy,
-                                          PartRes &res) {
+                                          PartRes &res, uint32_t *lds) {
   if constexpr (P < part_count(PART)) {
     constexpr int X = rect_x(PART, P), Y = rect_y(PART, P);
     constexpr int W = rect_w(PART), H = rect_h(PART);
@@ -492,24 +585,25 @@ __device__ __forceinline__ void part_step(const SCtx &c, const Nbr &n, Cur &cur,
     SrcRegs s;
     load_src(src + Y * sstride + X, sstride, L, s);
     int m0x = P == 0 ? 0 : res.mx[0], m0y = P == 0 ? 0 : res.my[0];
-    SRes r = search<W, H>(c, s, L, mbx + X, mby + Y, mvpx, mvpy, seedx, seedy, m0x, m0y, 2);
+    SRes r = search<W, H>(c, s, L, mbx + X, mby + Y, mvpx, mvpy, seedx, seedy, m0x, m0y, 2, lds);
     res.mx[P] = r.x; res.my[P] = r.y;
     res.px[P] = mvpx; res.py[P] = mvpy;
     res.total += r.cost;
     cur.x[P] = r.x; cur.y[P] = r.y;
-    part_step<PART, P + 1>(c, n, cur, src, sstride, mbx, mby, seedx, seedy, res);
+    part_step<PART, P + 1>(c, n, cur, src, sstride, mbx, mby, seedx, seedy, res, lds);
   }
 }
 
 // search_part (slice.rs:1810-1868), list 0.  src = MB top-left.
 template <int PART>
 __device__ __forceinline__ void search_part(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
-                                            int mbx, int mby, int seedx, int seedy, PartRes &res) {
+                                            int mbx, int mby, int seedx, int seedy, PartRes &res,
+                                            uint32_t *lds) {
   Cur cur = {};
 #pragma unroll
   for (int i = 0; i < 4; i++) { res.mx[i] = res.my[i] = res.px[i] = res.py[i] = 0; }
   res.total = 0;
-  part_step<PART, 0>(c, n, cur, src, sstride, mbx, mby, seedx, seedy, res);
+  part_step<PART, 0>(c, n, cur, src, sstride, mbx, mby, seedx, seedy, res, lds);
 }
 
 __device__ __forceinline__ uint32_t pack_mv(int x, int y) {
@@ -517,6 +611,18 @@ __device__ __forceinline__ uint32_t pack_mv(int x, int y) {
 }
 __device__ __forceinline__ int sel4(int k, int a, int b, int cc, int d) {
   return k == 0 ? a : (k == 1 ? b : (k == 2 ? cc : d));
+}
+
+// dst = take ? src : dst, element by element (keeps the records in registers).
+__device__ __forceinline__ void take_if(PartRes &dst, const PartRes &src, bool take) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    dst.mx[i] = take ? src.mx[i] : dst.mx[i];
+    dst.my[i] = take ? src.my[i] : dst.my[i];
+    dst.px[i] = take ? src.px[i] : dst.px[i];
+    dst.py[i] = take ? src.py[i] : dst.py[i];
+  }
+  dst.total = take ? src.total : dst.total;
 }
 
 struct MbRes {
@@ -530,14 +636,14 @@ struct MbRes {
 // ME-only analysis of one MB (slice.rs:1884-1945 without early skip,
 // intra and RD).
 __device__ __forceinline__ void analyse_mb(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
-                                           int mbx, int mby, int partitions, MbRes &o) {
+                                           int mbx, int mby, int partitions, MbRes &o, uint32_t *lds) {
   int skx, sky;
   pskip(n, skx, sky);
   PartRes r16;
-  search_part<0>(c, n, src, sstride, mbx, mby, skx, sky, r16);
+  search_part<0>(c, n, src, sstride, mbx, mby, skx, sky, r16, lds);
   const uint32_t cost16 = r16.total;
   o.part = 0;
-  o.best = r16;
+  take_if(o.best, r16, true);
   o.best_cost = cost16 + (c.lam >> 8);
   o.cost[0] = cost16;
   o.cost[1] = o.cost[2] = o.cost[3] = 0xFFFFFFFFu;
@@ -545,18 +651,27 @@ __device__ __forceinline__ void analyse_mb(const SCtx &c, const Nbr &n, const ui
   o.m16x = r16.mx[0]; o.m16y = r16.my[0];
   if (partitions && cost16 > 1024u) {
     PartRes r;
-    search_part<1>(c, n, src, sstride, mbx, mby, r16.mx[0], r16.my[0], r);
+    search_part<1>(c, n, src, sstride, mbx, mby, r16.mx[0], r16.my[0], r, lds);
     o.cost[1] = r.total;
     uint32_t cc = r.total + ((c.lam * 3u) >> 8);
-    if (cc < o.best_cost) { o.best_cost = cc; o.part = 1; o.best = r; }
-    search_part<2>(c, n, src, sstride, mbx, mby, r16.mx[0], r16.my[0], r);
+    bool t = cc < o.best_cost;
+    o.best_cost = t ? cc : o.best_cost;
+    o.part = t ? 1 : o.part;
+    take_if(o.best, r, t);
+    search_part<2>(c, n, src, sstride, mbx, mby, r16.mx[0], r16.my[0], r, lds);
     o.cost[2] = r.total;
     cc = r.total + ((c.lam * 3u) >> 8);
-    if (cc < o.best_cost) { o.best_cost = cc; o.part = 2; o.best = r; }
-    search_part<3>(c, n, src, sstride, mbx, mby, r16.mx[0], r16.my[0], r);
+    t = cc < o.best_cost;
+    o.best_cost = t ? cc : o.best_cost;
+    o.part = t ? 2 : o.part;
+    take_if(o.best, r, t);
+    search_part<3>(c, n, src, sstride, mbx, mby, r16.mx[0], r16.my[0], r, lds);
     o.cost[3] = r.total;
     cc = r.total + ((c.lam * 10u) >> 8);
-    if (cc < o.best_cost) { o.best_cost = cc; o.part = 3; o.best = r; }
+    t = cc < o.best_cost;
+    o.best_cost = t ? cc : o.best_cost;
+    o.part = t ? 3 : o.part;
+    take_if(o.best, r, t);
   }
 }
 
@@ -637,6 +752,7 @@ __global__ void __launch_bounds__(64) search_frames_kernel(const DevJob *__restr
                                                            const int2 *__restrict__ tickets, int ntickets,
                                                            uint32_t *ctl, uint32_t tag,
                                                            uint32_t spin_limit) {
+  __shared__ uint32_t s_lds[LDS_PER_WAVE / 4];
   uint32_t *counter = ctl;
   uint32_t *err = ctl + 1;
   const int l = (int)lane_id();
@@ -658,6 +774,7 @@ __global__ void __launch_bounds__(64) search_frames_kernel(const DevJob *__restr
     c.pl[0] = J.pl[0]; c.pl[1] = J.pl[1]; c.pl[2] = J.pl[2]; c.pl[3] = J.pl[3];
     c.stride = stride;
     c.pw = 16 * mbw;
+    c.ph = 16 * mbh;
     {
       // frame.rs:152 with picture.rs:261-265
       int need = 16 * (my + 1) + 96;
@@ -698,7 +815,7 @@ __global__ void __launch_bounds__(64) search_frames_kernel(const DevJob *__restr
       n.TRx = sext12(nxt[0] >> 12); n.TRy = sext12(nxt[0]);
       const uint8_t *src = J.src + (size_t)(16 * my + PCME_PAD) * stride + 16 * mx + PCME_PAD;
       MbRes o;
-      analyse_mb(c, n, src, stride, 16 * mx, 16 * my, J.partitions, o);
+      analyse_mb(c, n, src, stride, 16 * mx, 16 * my, J.partitions, o, s_lds);
       store_out(J.out + (size_t)my * mbw + mx, o);
       // Publish the bottom row (blocks 12..15) for the row below.
       if (l < 4) {
@@ -718,6 +835,7 @@ __global__ void __launch_bounds__(64) search_frames_kernel(const DevJob *__restr
 __global__ void __launch_bounds__(64) search_batch_kernel(const pcme_search_job *__restrict__ jobs,
                                                           int njobs, pcme_mv *out_mv,
                                                           uint32_t *out_cost) {
+  __shared__ uint32_t s_lds[LDS_PER_WAVE / 4];
   const int j = blockIdx.x;
   if (j >= njobs) return;
   const pcme_search_job &J = jobs[j];
@@ -725,6 +843,7 @@ __global__ void __launch_bounds__(64) search_batch_kernel(const pcme_search_job 
   c.pl[0] = J.ref; c.pl[1] = J.ref_h; c.pl[2] = J.ref_v; c.pl[3] = J.ref_c;
   c.stride = 16 * J.mbw + 2 * PCME_PAD;
   c.pw = 16 * J.mbw;
+  c.ph = 16 * J.mbh;
   c.vlim = (int)J.vlim;
   c.lam = J.lam;
   c.range = J.me_range;
@@ -735,16 +854,16 @@ __global__ void __launch_bounds__(64) search_batch_kernel(const pcme_search_job 
   r.x = r.y = 0; r.cost = 0;
   if (J.w == 16 && J.h == 16) {
     Lane L = lane_layout<16, 16>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
-    r = search<16, 16>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands);
+    r = search<16, 16>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, s_lds);
   } else if (J.w == 16 && J.h == 8) {
     Lane L = lane_layout<16, 8>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
-    r = search<16, 8>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands);
+    r = search<16, 8>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, s_lds);
   } else if (J.w == 8 && J.h == 16) {
     Lane L = lane_layout<8, 16>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
-    r = search<8, 16>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands);
+    r = search<8, 16>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, s_lds);
   } else {
     Lane L = lane_layout<8, 8>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
-    r = search<8, 8>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands);
+    r = search<8, 8>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, s_lds);
   }
   if (lane_id() == 0) {
     pcme_mv m;
