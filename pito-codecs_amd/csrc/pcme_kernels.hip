@@ -53,9 +53,11 @@ __device__ __forceinline__ uint32_t rdlane(uint32_t v, int l) {
 }
 
 // Exp-Golomb se(v) length (slice.rs:130-139).
+// With k = 2v-1 (v > 0) or -2v, k+1 = 2|v| + (v <= 0): branch-free.
 __device__ __forceinline__ uint32_t se_bits(int v) {
-  uint32_t k = v > 0 ? 2u * (uint32_t)v - 1u : (uint32_t)(-2 * v);
-  return 2u * (31u - (uint32_t)__builtin_clz(k + 1u)) + 1u;
+  const uint32_t a = (uint32_t)(v < 0 ? -v : v);
+  const uint32_t kp = 2u * a + (v <= 0 ? 1u : 0u);
+  return 2u * (31u - (uint32_t)__builtin_clz(kp)) + 1u;
 }
 
 typedef short v2s __attribute__((ext_vector_type(2)));
@@ -224,14 +226,16 @@ __device__ __forceinline__ void stage_region(const uint8_t *plane, int stride, i
   uint32_t v[IT];
 #pragma unroll
   for (int k = 0; k < IT; k++) {
+    // lanes past the end re-read the last dword (no branch around the load)
     int i = l + 64 * k;
+    i = (64 * k + 63 < N || i < N) ? i : N - 1;
     int r = i / DPR, cc = i - r * DPR;
-    v[k] = i < N ? src[base + r * sdw + cc] : 0u;
+    v[k] = src[base + r * sdw + cc];
   }
 #pragma unroll
   for (int k = 0; k < IT; k++) {
     int i = l + 64 * k;
-    if (i < N) dst[i] = v[k];
+    if (64 * k + 63 < N || i < N) dst[i] = v[k];
   }
 }
 
@@ -290,13 +294,18 @@ __device__ __forceinline__ uint32_t satd_cost(const SCtx &c, const Lane &L, cons
   return acc + ((c.lam * (se_bits(cx - mvpx) + se_bits(cy - mvpy))) >> 8);
 }
 
-// Per-slot candidate offsets.  HEX: (-2,0),(2,0),(-1,-2),(1,-2),(-1,2),(1,2)
-// (slice.rs:1634).  SQ8: (-1,0),(1,0),(0,-1),(0,1),(-1,-1),(1,-1),(-1,1),
-// (1,1) (slice.rs:1660-1669, 1693-1702).
-__device__ __forceinline__ int hex_dx(int s) { return s < 2 ? (s == 0 ? -2 : 2) : ((s & 1) ? 1 : -1); }
-__device__ __forceinline__ int hex_dy(int s) { return s < 2 ? 0 : (s < 4 ? -2 : 2); }
-__device__ __forceinline__ int sq_dx(int s) { return s < 2 ? (s ? 1 : -1) : (s < 4 ? 0 : ((s & 1) ? 1 : -1)); }
-__device__ __forceinline__ int sq_dy(int s) { return s < 2 ? 0 : (s < 4 ? (s == 2 ? -1 : 1) : (s < 6 ? -1 : 1)); }
+// Per-slot candidate offsets, 4-bit fields (value + bias) per slot.
+// HEX: (-2,0),(2,0),(-1,-2),(1,-2),(-1,2),(1,2) (slice.rs:1634).
+// SQ8: (-1,0),(1,0),(0,-1),(0,1),(-1,-1),(1,-1),(-1,1),(1,1)
+// (slice.rs:1660-1669, 1693-1702).
+#define HEX_DX 0x313140u  // dx + 2
+#define HEX_DY 0x440022u  // dy + 2
+#define SQ_DX 0x20201120u  // dx + 1
+#define SQ_DY 0x22002011u  // dy + 1
+__device__ __forceinline__ int hex_dx(int s) { return (int)((HEX_DX >> (4 * s)) & 15u) - 2; }
+__device__ __forceinline__ int hex_dy(int s) { return (int)((HEX_DY >> (4 * s)) & 15u) - 2; }
+__device__ __forceinline__ int sq_dx(int s) { return (int)((SQ_DX >> (4 * s)) & 15u) - 1; }
+__device__ __forceinline__ int sq_dy(int s) { return (int)((SQ_DY >> (4 * s)) & 15u) - 1; }
 
 struct SRes {
   int x, y;
@@ -304,6 +313,28 @@ struct SRes {
 };
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+#define DPP_ROW_MIRROR 0x140
+#define DPP_ROW_BCAST15 0x142
+#define DPP_ROW_BCAST31 0x143
+
+// The reference scans a pass's candidates in slot order and takes each one
+// whose cost is strictly below the running best; the result is the
+// candidate of least (cost, slot) among the valid ones, taken when its cost
+// is below the incumbent.  Every lane of a slot group holds the group's
+// cost; key = cost*8 + slot (invalid: all ones) is min-reduced over the
+// eight groups (row mirror, then the row broadcasts) and read from lane 63.
+// Returns the wave-uniform key.
+__device__ __forceinline__ uint32_t min_key(uint32_t cost, bool valid, int slot) {
+  uint32_t k = valid ? ((cost << 3) | (uint32_t)slot) : 0xFFFFFFFFu;
+  uint32_t t = (uint32_t)__builtin_amdgcn_update_dpp((int)k, (int)k, DPP_ROW_MIRROR, 0xF, 0xF, false);
+  k = t < k ? t : k;
+  t = (uint32_t)__builtin_amdgcn_update_dpp((int)k, (int)k, DPP_ROW_BCAST15, 0xA, 0xF, false);
+  k = t < k ? t : k;
+  t = (uint32_t)__builtin_amdgcn_update_dpp((int)k, (int)k, DPP_ROW_BCAST31, 0xC, 0xF, false);
+  k = t < k ? t : k;
+  return rdlane(k, 63);
+}
 
 // Re-stage the full-pel window around the current best (bxv, byv) when a
 // pass reaching +-R px around it could leave the staged window.
@@ -340,83 +371,65 @@ __device__ __forceinline__ void ensure_sp(const SCtx &c, int bx, int by, int bxv
 // position (bx, by).  mvp, cands are wave-uniform; returns (best, cost).
 // lds = this wave's LDS_PER_WAVE bytes.
 template <int W, int H>
-__device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const Lane &L, int bx, int by, int mvpx, int mvpy,
-                       int c0x, int c0y, int c1x, int c1y, int ncand, uint32_t *lds) {
+__device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const Lane &L, int bx, int by, int mvpx,
+                                       int mvpy, int c0x, int c0y, int c1x, int c1y, int ncand, uint32_t *lds) {
   uint32_t *lw = lds;
   uint32_t *lsp = lds + WIN_BYTES / 4;
-  // slice.rs:1617-1631: rounded predictor, then the candidates (HBM reads).
+  // slice.rs:1617-1631: rounded predictor (slot 0), then the candidates
+  // (slots 1, 2), read from HBM.  A candidate equal to the running best has
+  // the best's cost and cannot be taken, so least (cost, slot) reproduces
+  // the scan including its "c != best" test.
   int bxv = (mvpx + 2) & ~3, byv = (mvpy + 2) & ~3;
   if (!mv_ok(c, bx, by, W, H, bxv, byv)) { bxv = 0; byv = 0; }
-  int r0x = (c0x + 2) & ~3, r0y = (c0y + 2) & ~3;
-  int r1x = (c1x + 2) & ~3, r1y = (c1y + 2) & ~3;
   uint32_t bcost;
   {
-    int cx = L.slot == 1 ? r0x : (L.slot == 2 ? r1x : bxv);
-    int cy = L.slot == 1 ? r0y : (L.slot == 2 ? r1y : byv);
-    bool valid = L.slot == 0 || (L.slot <= ncand && mv_ok(c, bx, by, W, H, cx, cy));
-    uint64_t vm = __ballot(valid);
-    uint32_t k = sad_cost<false>(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy, lw, 0, 0);
-    bcost = rdlane(k, 0);
-    if (ncand >= 1 && ((vm >> 8) & 1u) && !(r0x == bxv && r0y == byv)) {
-      uint32_t k1 = rdlane(k, 8);
-      if (k1 < bcost) { bcost = k1; bxv = r0x; byv = r0y; }
-    }
-    if (ncand >= 2 && ((vm >> 16) & 1u) && !(r1x == bxv && r1y == byv)) {
-      uint32_t k2 = rdlane(k, 16);
-      if (k2 < bcost) { bcost = k2; bxv = r1x; byv = r1y; }
-    }
+    const int r0x = (c0x + 2) & ~3, r0y = (c0y + 2) & ~3;
+    const int r1x = (c1x + 2) & ~3, r1y = (c1y + 2) & ~3;
+    const int cx = L.slot == 1 ? r0x : (L.slot == 2 ? r1x : bxv);
+    const int cy = L.slot == 1 ? r0y : (L.slot == 2 ? r1y : byv);
+    const bool valid = L.slot == 0 || (L.slot <= ncand && mv_ok(c, bx, by, W, H, cx, cy));
+    const uint32_t k = sad_cost<false>(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy,
+                                       lw, 0, 0);
+    const uint32_t key = min_key(k, valid, L.slot);
+    bcost = key >> 3;
+    const int w = (int)(key & 7u);
+    bxv = w == 1 ? r0x : (w == 2 ? r1x : bxv);
+    byv = w == 1 ? r0y : (w == 2 ? r1y : byv);
   }
-  // Full-pel window: nothing staged yet (an impossible origin forces it).
-  int wx0 = 1 << 20, wy0 = 1 << 20;
   // slice.rs:1632-1656: hexagon descent clamped to +-range around centre0.
+  int wx0 = 1 << 20, wy0 = 1 << 20;  // nothing staged yet
   const int c0xx = bxv, c0yy = byv;
   const int range = c.range;
-  const int hdx = hex_dx(L.slot & 7), hdy = hex_dy(L.slot & 7);
+  const int hdx = hex_dx(L.slot & 7) * 4, hdy = hex_dy(L.slot & 7) * 4;
   for (int it = 0; it < range; it++) {
     ensure_win<W, H>(c, bx, by, bxv, byv, 2, wx0, wy0, lw);
-    int cx = bxv + hdx * 4, cy = byv + hdy * 4;
-    int ddx = cx - c0xx, ddy = cy - c0yy;
-    bool valid = L.slot < 6 && ((ddx < 0 ? -ddx : ddx) >> 2) <= range &&
-                 ((ddy < 0 ? -ddy : ddy) >> 2) <= range && mv_ok(c, bx, by, W, H, cx, cy);
-    uint64_t vm = __ballot(valid);
-    uint32_t k = sad_cost<true>(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy, lw, wx0, wy0);
-    const int ox = bxv, oy = byv;
-    bool moved = false;
-#pragma unroll
-    for (int j = 0; j < 6; j++) {
-      if ((vm >> (8 * j)) & 1u) {
-        uint32_t kj = rdlane(k, 8 * j);
-        if (kj < bcost) {
-          bcost = kj;
-          bxv = ox + hex_dx(j) * 4;
-          byv = oy + hex_dy(j) * 4;
-          moved = true;
-        }
-      }
-    }
-    if (!moved) break;
+    const int cx = bxv + hdx, cy = byv + hdy;
+    const int ddx = cx - c0xx, ddy = cy - c0yy;
+    const bool valid = L.slot < 6 && ((ddx < 0 ? -ddx : ddx) >> 2) <= range &&
+                       ((ddy < 0 ? -ddy : ddy) >> 2) <= range && mv_ok(c, bx, by, W, H, cx, cy);
+    const uint32_t k =
+        sad_cost<true>(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy, lw, wx0, wy0);
+    const uint32_t key = min_key(k, valid, L.slot);
+    if ((key >> 3) >= bcost) break;  // also covers "no valid candidate"
+    bcost = key >> 3;
+    const int j = (int)(key & 7u);
+    bxv += hex_dx(j) * 4;
+    byv += hex_dy(j) * 4;
   }
   // slice.rs:1657-1682: two rounds of 8-neighbour integer refinement.
   const int qdx = sq_dx(L.slot), qdy = sq_dy(L.slot);
   for (int it = 0; it < 2; it++) {
     ensure_win<W, H>(c, bx, by, bxv, byv, 1, wx0, wy0, lw);
-    int cx = bxv + qdx * 4, cy = byv + qdy * 4;
-    bool valid = mv_ok(c, bx, by, W, H, cx, cy);
-    uint64_t vm = __ballot(valid);
-    uint32_t k = sad_cost<true>(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy, lw, wx0, wy0);
-    const int ox = bxv, oy = byv;
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      if ((vm >> (8 * j)) & 1u) {
-        uint32_t kj = rdlane(k, 8 * j);
-        if (kj < bcost) {
-          bcost = kj;
-          bxv = ox + sq_dx(j) * 4;
-          byv = oy + sq_dy(j) * 4;
-        }
-      }
-    }
-    if (bxv == ox && byv == oy) break;
+    const int cx = bxv + qdx * 4, cy = byv + qdy * 4;
+    const bool valid = mv_ok(c, bx, by, W, H, cx, cy);
+    const uint32_t k =
+        sad_cost<true>(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy, lw, wx0, wy0);
+    const uint32_t key = min_key(k, valid, L.slot);
+    if ((key >> 3) >= bcost) break;
+    bcost = key >> 3;
+    const int j = (int)(key & 7u);
+    bxv += sq_dx(j) * 4;
+    byv += sq_dy(j) * 4;
   }
   // Stage the four planes around the integer best for the sub-pel phase.
   int sx0 = 1 << 20, sy0 = 1 << 20;
@@ -428,27 +441,19 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
   // slice.rs:1689-1721: half-pel round, then 1 + subpel_rounds quarter-pel.
 #pragma unroll
   for (int step = 2; step >= 1; step--) {
-    int rounds = step == 1 ? 1 + c.subpel : 1;
+    const int rounds = step == 1 ? 1 + c.subpel : 1;
     for (int it = 0; it < rounds; it++) {
       ensure_sp<W, H>(c, bx, by, bxv, byv, sx0, sy0, lsp);
-      int cx = bxv + qdx * step, cy = byv + qdy * step;
-      bool valid = mv_ok(c, bx, by, W, H, cx, cy);
-      uint64_t vm = __ballot(valid);
-      uint32_t k = satd_cost(c, L, sp, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy, lsp,
-                             sx0, sy0);
-      const int ox = bxv, oy = byv;
-#pragma unroll
-      for (int j = 0; j < 8; j++) {
-        if ((vm >> (8 * j)) & 1u) {
-          uint32_t kj = rdlane(k, 8 * j);
-          if (kj < bs) {
-            bs = kj;
-            bxv = ox + sq_dx(j) * step;
-            byv = oy + sq_dy(j) * step;
-          }
-        }
-      }
-      if (bxv == ox && byv == oy) break;
+      const int cx = bxv + qdx * step, cy = byv + qdy * step;
+      const bool valid = mv_ok(c, bx, by, W, H, cx, cy);
+      const uint32_t k = satd_cost(c, L, sp, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy, lsp,
+                                   sx0, sy0);
+      const uint32_t key = min_key(k, valid, L.slot);
+      if ((key >> 3) >= bs) break;
+      bs = key >> 3;
+      const int j = (int)(key & 7u);
+      bxv += sq_dx(j) * step;
+      byv += sq_dy(j) * step;
     }
   }
   SRes r;
@@ -460,25 +465,31 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
 
 // ------------------------------------------------------------ neighbours
 
+// MVs of the scalar state are packed like pcme_mv: x in bits 0-15, y in
+// bits 16-31 (halves the scalar registers the neighbourhood needs).
+__device__ __forceinline__ uint32_t pack_mv(int x, int y) { return ((uint32_t)x & 0xFFFFu) | ((uint32_t)y << 16); }
+__device__ __forceinline__ int mv_x(uint32_t p) { return (int)(int16_t)(p & 0xFFFFu); }
+__device__ __forceinline__ int mv_y(uint32_t p) { return (int)p >> 16; }
+
 // Neighbour state of the MB being analysed in the ME-only field: every
 // coded MB is PInter with ref 0, so an available neighbour has ref 0.
 struct Nbr {
   bool aL, aT, aTR, aTL;
-  int Lx[4], Ly[4];  // left MB, blocks 3, 7, 11, 15
-  int Tx[4], Ty[4];  // top MB, blocks 12..15
-  int TRx, TRy;      // top-right MB, block 12
-  int TLx, TLy;      // top-left MB, block 15
+  uint32_t L[4];  // left MB, blocks 3, 7, 11, 15
+  uint32_t T[4];  // top MB, blocks 12..15
+  uint32_t TR;    // top-right MB, block 12
+  uint32_t TL;    // top-left MB, block 15
 };
 
 struct NbRes {
   bool avail;
   int ref;
-  int x, y;
+  uint32_t mv;
 };
 
 // Partitions already searched in the current mode (search_part's `info`).
 struct Cur {
-  int x[4], y[4];
+  uint32_t mv[4];
 };
 
 __host__ __device__ constexpr int part_count(int part) { return part == 0 ? 1 : (part == 3 ? 4 : 2); }
@@ -495,23 +506,23 @@ __host__ __device__ constexpr int part_of(int part, int r) {
 // (BX, BY) in 4x4 units relative to the MB.
 template <int PART, int P, int BX, int BY>
 __device__ __forceinline__ NbRes nb(const Nbr &n, const Cur &cur) {
-  NbRes r = {false, -1, 0, 0};
+  NbRes r = {false, -1, 0u};
   if constexpr (BX >= 0 && BX < 4 && BY >= 0 && BY < 4) {
     constexpr int k = part_of(PART, BY * 4 + BX);
     if constexpr (k < P) {
-      r.avail = true; r.ref = 0; r.x = cur.x[k]; r.y = cur.y[k];
+      r.avail = true; r.ref = 0; r.mv = cur.mv[k];
     }
   } else if constexpr (BY >= 4 || (BY >= 0 && BX >= 4)) {
   } else if constexpr (BY < 0) {
     if constexpr (BX < 0) {
-      if (n.aTL) { r.avail = true; r.ref = 0; r.x = n.TLx; r.y = n.TLy; }
+      if (n.aTL) { r.avail = true; r.ref = 0; r.mv = n.TL; }
     } else if constexpr (BX < 4) {
-      if (n.aT) { r.avail = true; r.ref = 0; r.x = n.Tx[BX]; r.y = n.Ty[BX]; }
+      if (n.aT) { r.avail = true; r.ref = 0; r.mv = n.T[BX]; }
     } else {
-      if (n.aTR) { r.avail = true; r.ref = 0; r.x = n.TRx; r.y = n.TRy; }
+      if (n.aTR) { r.avail = true; r.ref = 0; r.mv = n.TR; }
     }
   } else {
-    if (n.aL) { r.avail = true; r.ref = 0; r.x = n.Lx[BY]; r.y = n.Ly[BY]; }
+    if (n.aL) { r.avail = true; r.ref = 0; r.mv = n.L[BY]; }
   }
   return r;
 }
@@ -523,42 +534,36 @@ __device__ __forceinline__ int median3(int a, int b, int c) {
 }
 
 // mvp (slice.rs:434-482) with rf = 0.  SHAPE: 0 Normal, 1 Top16x8,
-// 2 Bottom16x8, 3 Left8x16, 4 Right8x16.
+// 2 Bottom16x8, 3 Left8x16, 4 Right8x16.  Returns a packed MV.
 template <int PART, int P, int X, int Y, int WB, int SHAPE>
-__device__ __forceinline__ void mvp(const Nbr &n, const Cur &cur, int &ox, int &oy) {
+__device__ __forceinline__ uint32_t mvp(const Nbr &n, const Cur &cur) {
   NbRes a = nb<PART, P, X - 1, Y>(n, cur);
   NbRes b = nb<PART, P, X, Y - 1>(n, cur);
   NbRes cc = nb<PART, P, X + WB, Y - 1>(n, cur);
   if (!cc.avail) cc = nb<PART, P, X - 1, Y - 1>(n, cur);
   if (!b.avail && !cc.avail && a.avail) { b = a; cc = a; }
-  if (!a.avail) { a.ref = -1; a.x = a.y = 0; }
-  if (!b.avail) { b.ref = -1; b.x = b.y = 0; }
-  if (!cc.avail) { cc.ref = -1; cc.x = cc.y = 0; }
+  if (!a.avail) { a.ref = -1; a.mv = 0u; }
+  if (!b.avail) { b.ref = -1; b.mv = 0u; }
+  if (!cc.avail) { cc.ref = -1; cc.mv = 0u; }
   const int rf = 0;
-  if (SHAPE == 1 && b.ref == rf) { ox = b.x; oy = b.y; return; }
-  if (SHAPE == 2 && a.ref == rf) { ox = a.x; oy = a.y; return; }
-  if (SHAPE == 3 && a.ref == rf) { ox = a.x; oy = a.y; return; }
-  if (SHAPE == 4 && cc.ref == rf) { ox = cc.x; oy = cc.y; return; }
-  int cnt = (a.ref == rf) + (b.ref == rf) + (cc.ref == rf);
-  if (cnt == 1) {
-    if (a.ref == rf) { ox = a.x; oy = a.y; return; }
-    if (b.ref == rf) { ox = b.x; oy = b.y; return; }
-    ox = cc.x; oy = cc.y; return;
-  }
-  ox = median3(a.x, b.x, cc.x);
-  oy = median3(a.y, b.y, cc.y);
+  if (SHAPE == 1 && b.ref == rf) return b.mv;
+  if (SHAPE == 2 && a.ref == rf) return a.mv;
+  if (SHAPE == 3 && a.ref == rf) return a.mv;
+  if (SHAPE == 4 && cc.ref == rf) return cc.mv;
+  const int cnt = (a.ref == rf) + (b.ref == rf) + (cc.ref == rf);
+  if (cnt == 1) return a.ref == rf ? a.mv : (b.ref == rf ? b.mv : cc.mv);
+  return pack_mv(median3(mv_x(a.mv), mv_x(b.mv), mv_x(cc.mv)), median3(mv_y(a.mv), mv_y(b.mv), mv_y(cc.mv)));
 }
 
 // pskip_mv (slice.rs:484-494)
-__device__ __forceinline__ void pskip(const Nbr &n, int &ox, int &oy) {
+__device__ __forceinline__ uint32_t pskip(const Nbr &n) {
   Cur cur = {};
   NbRes a = nb<0, 0, -1, 0>(n, cur);
   NbRes b = nb<0, 0, 0, -1>(n, cur);
-  ox = oy = 0;
-  if (!a.avail || !b.avail) return;
-  if (a.ref == 0 && a.x == 0 && a.y == 0) return;
-  if (b.ref == 0 && b.x == 0 && b.y == 0) return;
-  mvp<0, 0, 0, 0, 4, 0>(n, cur, ox, oy);
+  if (!a.avail || !b.avail) return 0u;
+  if (a.ref == 0 && a.mv == 0u) return 0u;
+  if (b.ref == 0 && b.mv == 0u) return 0u;
+  return mvp<0, 0, 0, 0, 4, 0>(n, cur);
 }
 
 __host__ __device__ constexpr int shape_of(int part, int p) {
@@ -566,50 +571,47 @@ __host__ __device__ constexpr int shape_of(int part, int p) {
 }
 
 struct PartRes {
-  int mx[4], my[4];    // mvs
-  int px[4], py[4];    // mvps
+  uint32_t mv[4];   // mvs
+  uint32_t mvp[4];  // mvps
   uint32_t total;
 };
 
 // One partition of search_part (slice.rs:1826-1866), then the next.
 template <int PART, int P>
 __device__ __forceinline__ void part_step(const SCtx &c, const Nbr &n, Cur &cur, const uint8_t *src,
-                                          int sstride, int mbx, int mby, int seedx, int seedy,
-                                          PartRes &res, uint32_t *lds) {
+                                          int sstride, int mbx, int mby, uint32_t seed, PartRes &res,
+                                          uint32_t *lds) {
   if constexpr (P < part_count(PART)) {
     constexpr int X = rect_x(PART, P), Y = rect_y(PART, P);
     constexpr int W = rect_w(PART), H = rect_h(PART);
-    int mvpx, mvpy;
-    mvp<PART, P, X / 4, Y / 4, W / 4, shape_of(PART, P)>(n, cur, mvpx, mvpy);
+    const uint32_t pv = mvp<PART, P, X / 4, Y / 4, W / 4, shape_of(PART, P)>(n, cur);
     Lane L = lane_layout<W, H>();
     SrcRegs s;
     load_src(src + Y * sstride + X, sstride, L, s);
-    int m0x = P == 0 ? 0 : res.mx[0], m0y = P == 0 ? 0 : res.my[0];
-    SRes r = search<W, H>(c, s, L, mbx + X, mby + Y, mvpx, mvpy, seedx, seedy, m0x, m0y, 2, lds);
-    res.mx[P] = r.x; res.my[P] = r.y;
-    res.px[P] = mvpx; res.py[P] = mvpy;
+    const uint32_t m0 = P == 0 ? 0u : res.mv[0];
+    SRes r = search<W, H>(c, s, L, mbx + X, mby + Y, mv_x(pv), mv_y(pv), mv_x(seed), mv_y(seed), mv_x(m0),
+                          mv_y(m0), 2, lds);
+    const uint32_t m = pack_mv(r.x, r.y);
+    res.mv[P] = m;
+    res.mvp[P] = pv;
     res.total += r.cost;
-    cur.x[P] = r.x; cur.y[P] = r.y;
-    part_step<PART, P + 1>(c, n, cur, src, sstride, mbx, mby, seedx, seedy, res, lds);
+    cur.mv[P] = m;
+    part_step<PART, P + 1>(c, n, cur, src, sstride, mbx, mby, seed, res, lds);
   }
 }
 
 // search_part (slice.rs:1810-1868), list 0.  src = MB top-left.
 template <int PART>
 __device__ __forceinline__ void search_part(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
-                                            int mbx, int mby, int seedx, int seedy, PartRes &res,
-                                            uint32_t *lds) {
+                                            int mbx, int mby, uint32_t seed, PartRes &res, uint32_t *lds) {
   Cur cur = {};
 #pragma unroll
-  for (int i = 0; i < 4; i++) { res.mx[i] = res.my[i] = res.px[i] = res.py[i] = 0; }
+  for (int i = 0; i < 4; i++) { res.mv[i] = res.mvp[i] = 0u; }
   res.total = 0;
-  part_step<PART, 0>(c, n, cur, src, sstride, mbx, mby, seedx, seedy, res, lds);
+  part_step<PART, 0>(c, n, cur, src, sstride, mbx, mby, seed, res, lds);
 }
 
-__device__ __forceinline__ uint32_t pack_mv(int x, int y) {
-  return ((uint32_t)x & 0xFFFFu) | ((uint32_t)y << 16);
-}
-__device__ __forceinline__ int sel4(int k, int a, int b, int cc, int d) {
+__device__ __forceinline__ uint32_t sel4(int k, uint32_t a, uint32_t b, uint32_t cc, uint32_t d) {
   return k == 0 ? a : (k == 1 ? b : (k == 2 ? cc : d));
 }
 
@@ -617,10 +619,8 @@ __device__ __forceinline__ int sel4(int k, int a, int b, int cc, int d) {
 __device__ __forceinline__ void take_if(PartRes &dst, const PartRes &src, bool take) {
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    dst.mx[i] = take ? src.mx[i] : dst.mx[i];
-    dst.my[i] = take ? src.my[i] : dst.my[i];
-    dst.px[i] = take ? src.px[i] : dst.px[i];
-    dst.py[i] = take ? src.py[i] : dst.py[i];
+    dst.mv[i] = take ? src.mv[i] : dst.mv[i];
+    dst.mvp[i] = take ? src.mvp[i] : dst.mvp[i];
   }
   dst.total = take ? src.total : dst.total;
 }
@@ -630,42 +630,41 @@ struct MbRes {
   PartRes best;
   uint32_t cost[4];
   uint32_t best_cost;
-  int skx, sky, m16x, m16y;
+  uint32_t skip, m16;
 };
 
 // ME-only analysis of one MB (slice.rs:1884-1945 without early skip,
 // intra and RD).
 __device__ __forceinline__ void analyse_mb(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
                                            int mbx, int mby, int partitions, MbRes &o, uint32_t *lds) {
-  int skx, sky;
-  pskip(n, skx, sky);
+  const uint32_t skip = pskip(n);
   PartRes r16;
-  search_part<0>(c, n, src, sstride, mbx, mby, skx, sky, r16, lds);
+  search_part<0>(c, n, src, sstride, mbx, mby, skip, r16, lds);
   const uint32_t cost16 = r16.total;
   o.part = 0;
   take_if(o.best, r16, true);
   o.best_cost = cost16 + (c.lam >> 8);
   o.cost[0] = cost16;
   o.cost[1] = o.cost[2] = o.cost[3] = 0xFFFFFFFFu;
-  o.skx = skx; o.sky = sky;
-  o.m16x = r16.mx[0]; o.m16y = r16.my[0];
+  o.skip = skip;
+  o.m16 = r16.mv[0];
   if (partitions && cost16 > 1024u) {
     PartRes r;
-    search_part<1>(c, n, src, sstride, mbx, mby, r16.mx[0], r16.my[0], r, lds);
+    search_part<1>(c, n, src, sstride, mbx, mby, o.m16, r, lds);
     o.cost[1] = r.total;
     uint32_t cc = r.total + ((c.lam * 3u) >> 8);
     bool t = cc < o.best_cost;
     o.best_cost = t ? cc : o.best_cost;
     o.part = t ? 1 : o.part;
     take_if(o.best, r, t);
-    search_part<2>(c, n, src, sstride, mbx, mby, r16.mx[0], r16.my[0], r, lds);
+    search_part<2>(c, n, src, sstride, mbx, mby, o.m16, r, lds);
     o.cost[2] = r.total;
     cc = r.total + ((c.lam * 3u) >> 8);
     t = cc < o.best_cost;
     o.best_cost = t ? cc : o.best_cost;
     o.part = t ? 2 : o.part;
     take_if(o.best, r, t);
-    search_part<3>(c, n, src, sstride, mbx, mby, r16.mx[0], r16.my[0], r, lds);
+    search_part<3>(c, n, src, sstride, mbx, mby, o.m16, r, lds);
     o.cost[3] = r.total;
     cc = r.total + ((c.lam * 10u) >> 8);
     t = cc < o.best_cost;
@@ -675,14 +674,10 @@ __device__ __forceinline__ void analyse_mb(const SCtx &c, const Nbr &n, const ui
   }
 }
 
-// MV of 4x4 block r after cand_inter_info (slice.rs:1751-1785).
-__device__ __forceinline__ int blk_mvx(const MbRes &o, int r) {
+// Packed MV of 4x4 block r after cand_inter_info (slice.rs:1751-1785).
+__device__ __forceinline__ uint32_t blk_mv(const MbRes &o, int r) {
   int k = o.part == 0 ? 0 : o.part == 1 ? (r >> 3) : o.part == 2 ? ((r & 3) >> 1) : (((r >> 3) << 1) | ((r & 3) >> 1));
-  return sel4(k, o.best.mx[0], o.best.mx[1], o.best.mx[2], o.best.mx[3]);
-}
-__device__ __forceinline__ int blk_mvy(const MbRes &o, int r) {
-  int k = o.part == 0 ? 0 : o.part == 1 ? (r >> 3) : o.part == 2 ? ((r & 3) >> 1) : (((r >> 3) << 1) | ((r & 3) >> 1));
-  return sel4(k, o.best.my[0], o.best.my[1], o.best.my[2], o.best.my[3]);
+  return sel4(k, o.best.mv[0], o.best.mv[1], o.best.mv[2], o.best.mv[3]);
 }
 
 // 32 dwords of pcme_mb_out, lane i < 32 stores dword i.
@@ -690,24 +685,19 @@ __device__ __forceinline__ void store_out(pcme_mb_out *out, const MbRes &o) {
   int l = (int)lane_id();
   uint32_t v;
   if (l < 16) {
-    v = pack_mv(blk_mvx(o, l), blk_mvy(o, l));
+    v = blk_mv(o, l);
   } else if (l < 20) {
-    int k = l - 16;
-    v = pack_mv(sel4(k, o.best.mx[0], o.best.mx[1], o.best.mx[2], o.best.mx[3]),
-                sel4(k, o.best.my[0], o.best.my[1], o.best.my[2], o.best.my[3]));
+    v = sel4(l - 16, o.best.mv[0], o.best.mv[1], o.best.mv[2], o.best.mv[3]);
   } else if (l < 24) {
-    int k = l - 20;
-    v = pack_mv(sel4(k, o.best.px[0], o.best.px[1], o.best.px[2], o.best.px[3]),
-                sel4(k, o.best.py[0], o.best.py[1], o.best.py[2], o.best.py[3]));
+    v = sel4(l - 20, o.best.mvp[0], o.best.mvp[1], o.best.mvp[2], o.best.mvp[3]);
   } else if (l < 28) {
-    int k = l - 24;
-    v = (uint32_t)sel4(k, (int)o.cost[0], (int)o.cost[1], (int)o.cost[2], (int)o.cost[3]);
+    v = sel4(l - 24, o.cost[0], o.cost[1], o.cost[2], o.cost[3]);
   } else if (l == 28) {
     v = o.best_cost;
   } else if (l == 29) {
-    v = pack_mv(o.skx, o.sky);
+    v = o.skip;
   } else if (l == 30) {
-    v = pack_mv(o.m16x, o.m16y);
+    v = o.m16;
   } else {
     v = (uint32_t)o.part;
   }
@@ -716,13 +706,16 @@ __device__ __forceinline__ void store_out(pcme_mb_out *out, const MbRes &o) {
 
 // ------------------------------------------------------ frame search kernel
 
+// Exchange word: tag in bits 24-31, x in 23-12, y in 11-0 (12-bit two's
+// complement; legal MVs lie within +-2000).
 __device__ __forceinline__ int sext12(uint32_t v) { return (int)((v & 0xFFFu) ^ 0x800u) - 0x800; }
-__device__ __forceinline__ uint32_t xword(uint32_t tag, int x, int y) {
-  return (tag << 24) | (((uint32_t)x & 0xFFFu) << 12) | ((uint32_t)y & 0xFFFu);
+__device__ __forceinline__ uint32_t xword(uint32_t tag, uint32_t mv) {
+  return (tag << 24) | (((uint32_t)mv_x(mv) & 0xFFFu) << 12) | ((uint32_t)mv_y(mv) & 0xFFFu);
 }
+__device__ __forceinline__ uint32_t xword_mv(uint32_t w) { return pack_mv(sext12(w >> 12), sext12(w)); }
 
-// Wait until the 4 exchange words of one MB carry `tag`; returns them in
-// w[0..3] (uniform).  False on stall or on a raised error flag.
+// Wait until the 4 exchange words of one MB carry `tag`; returns their MVs
+// (packed) in w[0..3] (uniform).  False on stall or on a raised error flag.
 __device__ __forceinline__ bool fetch_words(const uint32_t *xw, uint32_t tag, uint32_t *err,
                                             uint32_t spin_limit, uint32_t w[4]) {
   int l = (int)lane_id();
@@ -743,6 +736,7 @@ __device__ __forceinline__ bool fetch_words(const uint32_t *xw, uint32_t tag, ui
     }
     __builtin_amdgcn_s_sleep(2);
   }
+  v = xword_mv(v);
 #pragma unroll
   for (int i = 0; i < 4; i++) w[i] = rdlane(v, i);
   return true;
@@ -788,20 +782,21 @@ __global__ void __launch_bounds__(64) search_frames_kernel(const DevJob *__restr
     Nbr n;
     n.aL = false;
 #pragma unroll
-    for (int i = 0; i < 4; i++) { n.Lx[i] = n.Ly[i] = 0; n.Tx[i] = n.Ty[i] = 0; }
-    n.TRx = n.TRy = n.TLx = n.TLy = 0;
-    // Rolling cache of the row above: words of MB x-1 (prev), x (cur), x+1 (next).
-    uint32_t prev[4] = {0, 0, 0, 0}, cur[4] = {0, 0, 0, 0}, nxt[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 4; i++) { n.L[i] = 0u; n.T[i] = 0u; }
+    n.TR = n.TL = 0u;
+    // Row above: n.T = MB x, nxt = MB x+1; n.TL = block 15 of MB x-1.
+    uint32_t nxt[4] = {0u, 0u, 0u, 0u};
     bool ok = true;
     if (my > 0) {
-      ok = fetch_words(xup, tag, err, spin_limit, cur);
+      ok = fetch_words(xup, tag, err, spin_limit, n.T);
       if (ok && mbw > 1) ok = fetch_words(xup + 4, tag, err, spin_limit, nxt);
     }
     if (!ok) return;
     for (int mx = 0; mx < mbw; mx++) {
       if (my > 0 && mx > 0) {
+        n.TL = n.T[3];
 #pragma unroll
-        for (int i = 0; i < 4; i++) { prev[i] = cur[i]; cur[i] = nxt[i]; }
+        for (int i = 0; i < 4; i++) n.T[i] = nxt[i];
         if (mx + 1 < mbw) {
           if (!fetch_words(xup + (size_t)(mx + 1) * 4, tag, err, spin_limit, nxt)) return;
         }
@@ -809,23 +804,20 @@ __global__ void __launch_bounds__(64) search_frames_kernel(const DevJob *__restr
       n.aT = my > 0;
       n.aTL = my > 0 && mx > 0;
       n.aTR = my > 0 && mx + 1 < mbw;
-#pragma unroll
-      for (int i = 0; i < 4; i++) { n.Tx[i] = sext12(cur[i] >> 12); n.Ty[i] = sext12(cur[i]); }
-      n.TLx = sext12(prev[3] >> 12); n.TLy = sext12(prev[3]);
-      n.TRx = sext12(nxt[0] >> 12); n.TRy = sext12(nxt[0]);
+      n.TR = nxt[0];
       const uint8_t *src = J.src + (size_t)(16 * my + PCME_PAD) * stride + 16 * mx + PCME_PAD;
       MbRes o;
       analyse_mb(c, n, src, stride, 16 * mx, 16 * my, J.partitions, o, s_lds);
       store_out(J.out + (size_t)my * mbw + mx, o);
       // Publish the bottom row (blocks 12..15) for the row below.
       if (l < 4) {
-        uint32_t wv = xword(tag, blk_mvx(o, 12 + l), blk_mvy(o, 12 + l));
+        uint32_t wv = xword(tag, blk_mv(o, 12 + l));
         __hip_atomic_store(xrow + (size_t)mx * 4 + l, wv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
       // Left column (blocks 3, 7, 11, 15) for the next MB of this row.
       n.aL = true;
 #pragma unroll
-      for (int i = 0; i < 4; i++) { n.Lx[i] = blk_mvx(o, 4 * i + 3); n.Ly[i] = blk_mvy(o, 4 * i + 3); }
+      for (int i = 0; i < 4; i++) n.L[i] = blk_mv(o, 4 * i + 3);
     }
   }
 }
