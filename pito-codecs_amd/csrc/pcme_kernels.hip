@@ -124,13 +124,6 @@ struct SCtx {            // wave-uniform
   int subpel;
 };
 
-// slice.rs:496-507
-__device__ __forceinline__ bool mv_ok(const SCtx &c, int x, int y, int w, int h, int mx, int my) {
-  int x0 = x + (mx >> 2), y0 = y + (my >> 2);
-  int ax = mx < 0 ? -mx : mx, ay = my < 0 ? -my : my;
-  return ax < 2000 && ay < 2000 && x0 >= -44 && y0 >= -44 && x0 + w <= c.pw + 44 &&
-         y0 + h < c.vlim;
-}
 
 // Lane constants of one search: strip origin inside the block and whether
 // the lane's strip lies inside the block.
@@ -327,13 +320,33 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? l
 // Returns the wave-uniform key.
 __device__ __forceinline__ uint32_t min_key(uint32_t cost, bool valid, int slot) {
   uint32_t k = valid ? ((cost << 3) | (uint32_t)slot) : 0xFFFFFFFFu;
-  uint32_t t = (uint32_t)__builtin_amdgcn_update_dpp((int)k, (int)k, DPP_ROW_MIRROR, 0xF, 0xF, false);
+  // old = all ones (the identity of min) for lanes a row mask leaves out
+  uint32_t t = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)k, DPP_ROW_MIRROR, 0xF, 0xF, false);
   k = t < k ? t : k;
-  t = (uint32_t)__builtin_amdgcn_update_dpp((int)k, (int)k, DPP_ROW_BCAST15, 0xA, 0xF, false);
+  t = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)k, DPP_ROW_BCAST15, 0xA, 0xF, false);
   k = t < k ? t : k;
-  t = (uint32_t)__builtin_amdgcn_update_dpp((int)k, (int)k, DPP_ROW_BCAST31, 0xC, 0xF, false);
+  t = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)k, DPP_ROW_BCAST31, 0xC, 0xF, false);
   k = t < k ? t : k;
   return rdlane(k, 63);
+}
+
+// mv_ok (slice.rs:496-507) of a w x h block at (x, y) as a box of legal
+// quarter-pel MVs: (m >> 2) >= k <=> m >= 4k and (m >> 2) <= k <=> m <= 4k+3,
+// so |m| < 2000, x0 >= -44, y0 >= -44, x0 + w <= pw + 44, y0 + h < vlim
+// become lo <= m <= hi per component.
+struct Box {
+  int xlo, xhi, ylo, yhi;
+};
+__device__ __forceinline__ Box legal_box(const SCtx &c, int x, int y, int w, int h) {
+  Box b;
+  b.xlo = max(-1999, 4 * (-44 - x));
+  b.xhi = min(1999, 4 * (c.pw + 44 - w - x) + 3);
+  b.ylo = max(-1999, 4 * (-44 - y));
+  b.yhi = min(1999, 4 * (c.vlim - h - y - 1) + 3);
+  return b;
+}
+__device__ __forceinline__ bool in_box(const Box &b, int mx, int my) {
+  return (mx >= b.xlo) & (mx <= b.xhi) & (my >= b.ylo) & (my <= b.yhi);
 }
 
 // Re-stage the full-pel window around the current best (bxv, byv) when a
@@ -342,7 +355,7 @@ template <int W, int H>
 __device__ __forceinline__ void ensure_win(const SCtx &c, int bx, int by, int bxv, int byv, int R, int &wx0,
                                            int &wy0, uint32_t *lw) {
   const int X = bx + (bxv >> 2), Y = by + (byv >> 2);
-  const bool ok = X - R - wx0 >= 0 && X + R + W + 4 - wx0 <= WIN_W && Y - R - wy0 >= 0 && Y + R + H - wy0 <= WIN_H;
+  const bool ok = (X - R - wx0 >= 0) & (X + R + W + 4 - wx0 <= WIN_W) & (Y - R - wy0 >= 0) & (Y + R + H - wy0 <= WIN_H);
   if (ok) return;
   wx0 = clampi((X - WIN_M) & ~3, -PCME_PAD, c.pw + PCME_PAD - WIN_W);
   wy0 = clampi(Y - WIN_M, -PCME_PAD, c.ph + PCME_PAD - WIN_H);
@@ -357,7 +370,7 @@ template <int W, int H>
 __device__ __forceinline__ void ensure_sp(const SCtx &c, int bx, int by, int bxv, int byv, int &sx0, int &sy0,
                                           uint32_t *lsp) {
   const int X = bx + (bxv >> 2), Y = by + (byv >> 2);
-  const bool ok = X - 1 - sx0 >= 0 && X + W + 6 - sx0 <= SP_W && Y - 1 - sy0 >= 0 && Y + H + 2 - sy0 <= SP_H;
+  const bool ok = (X - 1 - sx0 >= 0) & (X + W + 6 - sx0 <= SP_W) & (Y - 1 - sy0 >= 0) & (Y + H + 2 - sy0 <= SP_H);
   if (ok) return;
   sx0 = clampi((X - 2) & ~3, -PCME_PAD, c.pw + PCME_PAD - SP_W);
   sy0 = clampi(Y - 2, -PCME_PAD, c.ph + PCME_PAD - SP_H);
@@ -375,19 +388,20 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
                                        int mvpy, int c0x, int c0y, int c1x, int c1y, int ncand, uint32_t *lds) {
   uint32_t *lw = lds;
   uint32_t *lsp = lds + WIN_BYTES / 4;
+  const Box ok = legal_box(c, bx, by, W, H);
   // slice.rs:1617-1631: rounded predictor (slot 0), then the candidates
   // (slots 1, 2), read from HBM.  A candidate equal to the running best has
   // the best's cost and cannot be taken, so least (cost, slot) reproduces
   // the scan including its "c != best" test.
   int bxv = (mvpx + 2) & ~3, byv = (mvpy + 2) & ~3;
-  if (!mv_ok(c, bx, by, W, H, bxv, byv)) { bxv = 0; byv = 0; }
+  if (!in_box(ok, bxv, byv)) { bxv = 0; byv = 0; }
   uint32_t bcost;
   {
     const int r0x = (c0x + 2) & ~3, r0y = (c0y + 2) & ~3;
     const int r1x = (c1x + 2) & ~3, r1y = (c1y + 2) & ~3;
     const int cx = L.slot == 1 ? r0x : (L.slot == 2 ? r1x : bxv);
     const int cy = L.slot == 1 ? r0y : (L.slot == 2 ? r1y : byv);
-    const bool valid = L.slot == 0 || (L.slot <= ncand && mv_ok(c, bx, by, W, H, cx, cy));
+    const bool valid = (L.slot == 0) | ((L.slot <= ncand) & in_box(ok, cx, cy));
     const uint32_t k = sad_cost<false>(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy,
                                        lw, 0, 0);
     const uint32_t key = min_key(k, valid, L.slot);
@@ -401,12 +415,17 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
   const int c0xx = bxv, c0yy = byv;
   const int range = c.range;
   const int hdx = hex_dx(L.slot & 7) * 4, hdy = hex_dy(L.slot & 7) * 4;
+  // hex candidates: legal and |m - centre0| >> 2 <= range, one box
+  Box hb = ok;
+  hb.xlo = max(hb.xlo, c0xx - 4 * range - 3);
+  hb.xhi = min(hb.xhi, c0xx + 4 * range + 3);
+  hb.ylo = max(hb.ylo, c0yy - 4 * range - 3);
+  hb.yhi = min(hb.yhi, c0yy + 4 * range + 3);
+  if (L.slot >= 6) hb.xlo = 1 << 20;  // slots 6, 7 carry no hex point
   for (int it = 0; it < range; it++) {
     ensure_win<W, H>(c, bx, by, bxv, byv, 2, wx0, wy0, lw);
     const int cx = bxv + hdx, cy = byv + hdy;
-    const int ddx = cx - c0xx, ddy = cy - c0yy;
-    const bool valid = L.slot < 6 && ((ddx < 0 ? -ddx : ddx) >> 2) <= range &&
-                       ((ddy < 0 ? -ddy : ddy) >> 2) <= range && mv_ok(c, bx, by, W, H, cx, cy);
+    const bool valid = in_box(hb, cx, cy);
     const uint32_t k =
         sad_cost<true>(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy, lw, wx0, wy0);
     const uint32_t key = min_key(k, valid, L.slot);
@@ -421,7 +440,7 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
   for (int it = 0; it < 2; it++) {
     ensure_win<W, H>(c, bx, by, bxv, byv, 1, wx0, wy0, lw);
     const int cx = bxv + qdx * 4, cy = byv + qdy * 4;
-    const bool valid = mv_ok(c, bx, by, W, H, cx, cy);
+    const bool valid = in_box(ok, cx, cy);
     const uint32_t k =
         sad_cost<true>(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy, lw, wx0, wy0);
     const uint32_t key = min_key(k, valid, L.slot);
@@ -445,7 +464,7 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
     for (int it = 0; it < rounds; it++) {
       ensure_sp<W, H>(c, bx, by, bxv, byv, sx0, sy0, lsp);
       const int cx = bxv + qdx * step, cy = byv + qdy * step;
-      const bool valid = mv_ok(c, bx, by, W, H, cx, cy);
+      const bool valid = in_box(ok, cx, cy);
       const uint32_t k = satd_cost(c, L, sp, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy, lsp,
                                    sx0, sy0);
       const uint32_t key = min_key(k, valid, L.slot);
