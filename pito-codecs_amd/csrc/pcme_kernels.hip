@@ -48,6 +48,13 @@ __device__ __forceinline__ uint32_t red8(uint32_t v) {
   return v;
 }
 
+// Sum over each aligned group of 4 lanes (quad).
+__device__ __forceinline__ uint32_t red4(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_QP_1032, 0xF, 0xF, false);
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_QP_2301, 0xF, 0xF, false);
+  return v;
+}
+
 __device__ __forceinline__ uint32_t rdlane(uint32_t v, int l) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
 }
@@ -189,7 +196,8 @@ __device__ __forceinline__ void load_src(const uint8_t *blk, int sstride, const 
 #define SP_H 21
 #define WIN_BYTES (WIN_W * WIN_H)
 #define SP_PLANE (SP_W * SP_H)
-#define LDS_PER_WAVE (((WIN_BYTES + 4 * SP_PLANE) + 15) & ~15)
+#define LDS_PER_SEARCH (((WIN_BYTES + 4 * SP_PLANE) + 15) & ~15)
+#define LDS_PER_WAVE (2 * LDS_PER_SEARCH)  // room for two concurrent searches
 
 typedef const __attribute__((address_space(1))) uint32_t gu32;
 
@@ -235,7 +243,7 @@ __device__ __forceinline__ void stage_region(const uint8_t *plane, int stride, i
 // Integer-pel SAD cost of the lane's slot candidate (slice.rs:1605-1616):
 // SAD over the block + (lam * mv_bits) >> 8.  (ux, uy) is the address MV:
 // the candidate when valid, else a valid fallback (its cost is ignored).
-template <bool LDS>
+template <bool LDS, int LPC = 8>
 __device__ __forceinline__ uint32_t sad_cost(const SCtx &c, const Lane &L, const SrcRegs &s, int bx, int by,
                                              int cx, int cy, int ux, int uy, int mvpx, int mvpy,
                                              const uint32_t *lw, int wx0, int wy0) {
@@ -252,7 +260,7 @@ __device__ __forceinline__ uint32_t sad_cost(const SCtx &c, const Lane &L, const
     acc = __builtin_amdgcn_sad_u8(b, s.raw[2 * r + 1], acc);
   }
   acc = L.active ? acc : 0u;
-  acc = red8(acc);
+  acc = LPC == 8 ? red8(acc) : red4(acc);
   return acc + ((c.lam * (se_bits(cx - mvpx) + se_bits(cy - mvpy))) >> 8);
 }
 
@@ -265,6 +273,7 @@ __device__ __forceinline__ const uint8_t *plane_sel(const SCtx &c, uint32_t p) {
 // Sub-pel cost (slice.rs:1684-1687): mc_luma (picture.rs:415-449) as the
 // rounding average of two of the staged planes, SATD, + (lam * mv_bits) >> 8.
 // Full and half positions read the same plane twice, and (a+a+1)>>1 == a.
+template <int LPC = 8>
 __device__ __forceinline__ uint32_t satd_cost(const SCtx &c, const Lane &L, const v2s *sp, int bx, int by, int cx,
                                               int cy, int ux, int uy, int mvpx, int mvpy, const uint32_t *lsp,
                                               int sx0, int sy0) {
@@ -283,7 +292,7 @@ __device__ __forceinline__ uint32_t satd_cost(const SCtx &c, const Lane &L, cons
   }
   uint32_t acc = satd_strip8(sp, P);
   acc = L.active ? acc : 0u;
-  acc = red8(acc);
+  acc = LPC == 8 ? red8(acc) : red4(acc);
   return acc + ((c.lam * (se_bits(cx - mvpx) + se_bits(cy - mvpy))) >> 8);
 }
 
@@ -482,6 +491,207 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
   return r;
 }
 
+// ---- paired search ---------------------------------------------------------
+// The 16x8 and 8x16 modes are independent of each other (both seeded with
+// mv16 and predicted from the same committed neighbours), so partition p of
+// each runs in one wave: lanes 0-31 search A (16x8 partition p), lanes 32-63
+// search B (8x16 partition p); 4 lanes per candidate, 8 candidates per
+// search per pass.  Each search keeps its own scalar state, LDS region and
+// strict-< scan; a search whose phase ended passes with no valid candidate.
+
+struct PState {          // wave-uniform state of one search of a pair
+  int bx, by;            // block position (picture pixels)
+  int mvpx, mvpy;
+  Box ok, hb;            // legal box, hex box
+  int x, y;              // running best
+  uint32_t cost;
+  int wx0, wy0, sx0, sy0;
+  bool act;
+  int it;
+};
+
+// Per-group keys of a paired pass (lanes 0-31: A, 32-63: B): least
+// cost*8 + slot of each group, read from lanes 31 and 63.
+__device__ __forceinline__ void min_key_pair(uint32_t cost, bool valid, int slot, uint32_t &ka, uint32_t &kb) {
+  uint32_t k = valid ? ((cost << 3) | (uint32_t)slot) : 0xFFFFFFFFu;
+  uint32_t t = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)k, DPP_ROW_HALF_MIRROR, 0xF, 0xF, false);
+  k = t < k ? t : k;
+  t = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)k, DPP_ROW_MIRROR, 0xF, 0xF, false);
+  k = t < k ? t : k;
+  t = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)k, DPP_ROW_BCAST15, 0xA, 0xF, false);
+  k = t < k ? t : k;
+  ka = rdlane(k, 31);
+  kb = rdlane(k, 63);
+}
+
+__device__ __forceinline__ int gsel(bool g, int a, int b) { return g ? b : a; }
+
+// Apply a group's key: take the candidate when its cost is below the
+// running best; returns whether it moved.
+__device__ __forceinline__ bool take_key(PState &S, uint32_t key, int dx, int dy) {
+  if ((key >> 3) >= S.cost) return false;
+  S.cost = key >> 3;
+  S.x += dx;
+  S.y += dy;
+  return true;
+}
+
+// search (slice.rs:1590-1723) for A = 16x8 partition and B = 8x16
+// partition at the same time.  src* = block top-left of each source.
+__device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B, const uint8_t *srcA,
+                                            const uint8_t *srcB, int sstride, int c0x, int c0y, int c1ax,
+                                            int c1ay, int c1bx, int c1by, uint32_t *lds, SRes &ra, SRes &rb) {
+  const uint32_t l = lane_id();
+  const bool g = l >= 32;
+  Lane L;
+  L.slot = (int)((l >> 2) & 7u);
+  L.sub = (int)(l & 3u);
+  L.sx = g ? 0 : (L.sub & 1) * 8;
+  L.sy = g ? L.sub * 4 : (L.sub >> 1) * 4;
+  L.active = true;
+  SrcRegs s;
+  load_src(g ? srcB : srcA, sstride, L, s);
+  uint32_t *lw = lds + (g ? LDS_PER_SEARCH / 4 : 0);
+  uint32_t *lsp = lw + WIN_BYTES / 4;
+  const int bx = gsel(g, A.bx, B.bx), by = gsel(g, A.by, B.by);
+  const int mvpx = gsel(g, A.mvpx, B.mvpx), mvpy = gsel(g, A.mvpy, B.mvpy);
+  A.ok = legal_box(c, A.bx, A.by, 16, 8);
+  B.ok = legal_box(c, B.bx, B.by, 8, 16);
+  // slice.rs:1617-1631 (see search()).
+  int a0x = (A.mvpx + 2) & ~3, a0y = (A.mvpy + 2) & ~3;
+  if (!in_box(A.ok, a0x, a0y)) { a0x = 0; a0y = 0; }
+  int b0x = (B.mvpx + 2) & ~3, b0y = (B.mvpy + 2) & ~3;
+  if (!in_box(B.ok, b0x, b0y)) { b0x = 0; b0y = 0; }
+  {
+    // cands = [seed, mvs[0] of the search's own mode]
+    const int r0x = (c0x + 2) & ~3, r0y = (c0y + 2) & ~3;
+    const int r1ax = (c1ax + 2) & ~3, r1ay = (c1ay + 2) & ~3;
+    const int r1bx = (c1bx + 2) & ~3, r1by = (c1by + 2) & ~3;
+    const int r1x = gsel(g, r1ax, r1bx), r1y = gsel(g, r1ay, r1by);
+    const int px = gsel(g, a0x, b0x), py = gsel(g, a0y, b0y);
+    const int cx = L.slot == 1 ? r0x : (L.slot == 2 ? r1x : px);
+    const int cy = L.slot == 1 ? r0y : (L.slot == 2 ? r1y : py);
+    const Box ok = g ? B.ok : A.ok;
+    const bool valid = (L.slot == 0) | ((L.slot <= 2) & in_box(ok, cx, cy));
+    const uint32_t k = sad_cost<false, 4>(c, L, s, bx, by, cx, cy, valid ? cx : px, valid ? cy : py, mvpx, mvpy,
+                                          lw, 0, 0);
+    uint32_t ka, kb;
+    min_key_pair(k, valid, L.slot, ka, kb);
+    A.cost = ka >> 3;
+    B.cost = kb >> 3;
+    const int wa = (int)(ka & 7u), wb = (int)(kb & 7u);
+    A.x = wa == 1 ? r0x : (wa == 2 ? r1ax : a0x);
+    A.y = wa == 1 ? r0y : (wa == 2 ? r1ay : a0y);
+    B.x = wb == 1 ? r0x : (wb == 2 ? r1bx : b0x);
+    B.y = wb == 1 ? r0y : (wb == 2 ? r1by : b0y);
+  }
+  const int range = c.range;
+  A.wx0 = A.wy0 = B.wx0 = B.wy0 = 1 << 20;
+  A.hb = A.ok;
+  A.hb.xlo = max(A.hb.xlo, A.x - 4 * range - 3);
+  A.hb.xhi = min(A.hb.xhi, A.x + 4 * range + 3);
+  A.hb.ylo = max(A.hb.ylo, A.y - 4 * range - 3);
+  A.hb.yhi = min(A.hb.yhi, A.y + 4 * range + 3);
+  B.hb = B.ok;
+  B.hb.xlo = max(B.hb.xlo, B.x - 4 * range - 3);
+  B.hb.xhi = min(B.hb.xhi, B.x + 4 * range + 3);
+  B.hb.ylo = max(B.hb.ylo, B.y - 4 * range - 3);
+  B.hb.yhi = min(B.hb.yhi, B.y + 4 * range + 3);
+  // slice.rs:1632-1656: hexagon descent of both.
+  const int hdx = hex_dx(L.slot) * 4, hdy = hex_dy(L.slot) * 4;
+  A.act = B.act = range > 0;
+  A.it = B.it = 0;
+  while (A.act | B.act) {
+    if (A.act) ensure_win<16, 8>(c, A.bx, A.by, A.x, A.y, 2, A.wx0, A.wy0, lds);
+    if (B.act) ensure_win<8, 16>(c, B.bx, B.by, B.x, B.y, 2, B.wx0, B.wy0, lds + LDS_PER_SEARCH / 4);
+    const int ox = gsel(g, A.x, B.x), oy = gsel(g, A.y, B.y);
+    const int cx = ox + hdx, cy = oy + hdy;
+    Box hb = g ? B.hb : A.hb;
+    const bool act = g ? B.act : A.act;
+    const bool valid = (L.slot < 6) & act & in_box(hb, cx, cy);
+    const uint32_t k = sad_cost<true, 4>(c, L, s, bx, by, cx, cy, valid ? cx : ox, valid ? cy : oy, mvpx, mvpy, lw,
+                                         gsel(g, A.wx0, B.wx0), gsel(g, A.wy0, B.wy0));
+    uint32_t ka, kb;
+    min_key_pair(k, valid, L.slot, ka, kb);
+    if (A.act) {
+      const int j = (int)(ka & 7u);
+      A.act = take_key(A, ka, hex_dx(j) * 4, hex_dy(j) * 4) && ++A.it < range;
+    }
+    if (B.act) {
+      const int j = (int)(kb & 7u);
+      B.act = take_key(B, kb, hex_dx(j) * 4, hex_dy(j) * 4) && ++B.it < range;
+    }
+  }
+  // slice.rs:1657-1682: two rounds of 8-neighbour refinement of both.
+  const int qdx = sq_dx(L.slot), qdy = sq_dy(L.slot);
+  A.act = B.act = true;
+  A.it = B.it = 0;
+  while (A.act | B.act) {
+    if (A.act) ensure_win<16, 8>(c, A.bx, A.by, A.x, A.y, 1, A.wx0, A.wy0, lds);
+    if (B.act) ensure_win<8, 16>(c, B.bx, B.by, B.x, B.y, 1, B.wx0, B.wy0, lds + LDS_PER_SEARCH / 4);
+    const int ox = gsel(g, A.x, B.x), oy = gsel(g, A.y, B.y);
+    const int cx = ox + qdx * 4, cy = oy + qdy * 4;
+    const Box ok = g ? B.ok : A.ok;
+    const bool act = g ? B.act : A.act;
+    const bool valid = act & in_box(ok, cx, cy);
+    const uint32_t k = sad_cost<true, 4>(c, L, s, bx, by, cx, cy, valid ? cx : ox, valid ? cy : oy, mvpx, mvpy, lw,
+                                         gsel(g, A.wx0, B.wx0), gsel(g, A.wy0, B.wy0));
+    uint32_t ka, kb;
+    min_key_pair(k, valid, L.slot, ka, kb);
+    if (A.act) {
+      const int j = (int)(ka & 7u);
+      A.act = take_key(A, ka, sq_dx(j) * 4, sq_dy(j) * 4) && ++A.it < 2;
+    }
+    if (B.act) {
+      const int j = (int)(kb & 7u);
+      B.act = take_key(B, kb, sq_dx(j) * 4, sq_dy(j) * 4) && ++B.it < 2;
+    }
+  }
+  // Sub-pel planes of both, then the SATD cost at each integer best.
+  A.sx0 = A.sy0 = B.sx0 = B.sy0 = 1 << 20;
+  ensure_sp<16, 8>(c, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
+  ensure_sp<8, 16>(c, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
+  v2s sp[16];
+  src_pairs(s, sp);
+  {
+    const int ox = gsel(g, A.x, B.x), oy = gsel(g, A.y, B.y);
+    const uint32_t k = satd_cost<4>(c, L, sp, bx, by, ox, oy, ox, oy, mvpx, mvpy, lsp, gsel(g, A.sx0, B.sx0),
+                                    gsel(g, A.sy0, B.sy0));
+    A.cost = rdlane(k, 0);
+    B.cost = rdlane(k, 32);
+  }
+  // slice.rs:1689-1721: half-pel round, then 1 + subpel_rounds quarter-pel.
+#pragma unroll
+  for (int step = 2; step >= 1; step--) {
+    const int rounds = step == 1 ? 1 + c.subpel : 1;
+    A.act = B.act = true;
+    A.it = B.it = 0;
+    while (A.act | B.act) {
+      if (A.act) ensure_sp<16, 8>(c, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
+      if (B.act) ensure_sp<8, 16>(c, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
+      const int ox = gsel(g, A.x, B.x), oy = gsel(g, A.y, B.y);
+      const int cx = ox + qdx * step, cy = oy + qdy * step;
+      const Box ok = g ? B.ok : A.ok;
+      const bool act = g ? B.act : A.act;
+      const bool valid = act & in_box(ok, cx, cy);
+      const uint32_t k = satd_cost<4>(c, L, sp, bx, by, cx, cy, valid ? cx : ox, valid ? cy : oy, mvpx, mvpy, lsp,
+                                      gsel(g, A.sx0, B.sx0), gsel(g, A.sy0, B.sy0));
+      uint32_t ka, kb;
+      min_key_pair(k, valid, L.slot, ka, kb);
+      if (A.act) {
+        const int j = (int)(ka & 7u);
+        A.act = take_key(A, ka, sq_dx(j) * step, sq_dy(j) * step) && ++A.it < rounds;
+      }
+      if (B.act) {
+        const int j = (int)(kb & 7u);
+        B.act = take_key(B, kb, sq_dx(j) * step, sq_dy(j) * step) && ++B.it < rounds;
+      }
+    }
+  }
+  ra.x = A.x; ra.y = A.y; ra.cost = A.cost;
+  rb.x = B.x; rb.y = B.y; rb.cost = B.cost;
+}
+
 // ------------------------------------------------------------ neighbours
 
 // MVs of the scalar state are packed like pcme_mv: x in bits 0-15, y in
@@ -630,6 +840,34 @@ __device__ __forceinline__ void search_part(const SCtx &c, const Nbr &n, const u
   part_step<PART, 0>(c, n, cur, src, sstride, mbx, mby, seed, res, lds);
 }
 
+// search_part (slice.rs:1810-1868) of P16x8 (ra) and P8x16 (rb) together:
+// partition p of both modes in one paired search.
+__device__ __forceinline__ void search_part_12(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
+                                               int mbx, int mby, uint32_t seed, PartRes &ra, PartRes &rb,
+                                               uint32_t *lds) {
+  Cur ca = {}, cb = {};
+#pragma unroll
+  for (int i = 0; i < 4; i++) { ra.mv[i] = ra.mvp[i] = rb.mv[i] = rb.mvp[i] = 0u; }
+  PState A, B;
+  SRes xa, xb;
+  // partition 0: 16x8 top (x 0, y 0, w 4 blocks), 8x16 left (x 0, y 0, w 2)
+  uint32_t pa = mvp<1, 0, 0, 0, 4, 1>(n, ca), pb = mvp<2, 0, 0, 0, 2, 3>(n, cb);
+  A.bx = mbx; A.by = mby; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
+  B.bx = mbx; B.by = mby; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
+  search_pair(c, A, B, src, src, sstride, mv_x(seed), mv_y(seed), 0, 0, 0, 0, lds, xa, xb);
+  ra.mv[0] = ca.mv[0] = pack_mv(xa.x, xa.y); ra.mvp[0] = pa; ra.total = xa.cost;
+  rb.mv[0] = cb.mv[0] = pack_mv(xb.x, xb.y); rb.mvp[0] = pb; rb.total = xb.cost;
+  // partition 1: 16x8 bottom (y 2 blocks), 8x16 right (x 2 blocks)
+  pa = mvp<1, 1, 0, 2, 4, 2>(n, ca);
+  pb = mvp<2, 1, 2, 0, 2, 4>(n, cb);
+  A.bx = mbx; A.by = mby + 8; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
+  B.bx = mbx + 8; B.by = mby; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
+  search_pair(c, A, B, src + 8 * sstride, src + 8, sstride, mv_x(seed), mv_y(seed), mv_x(ra.mv[0]),
+              mv_y(ra.mv[0]), mv_x(rb.mv[0]), mv_y(rb.mv[0]), lds, xa, xb);
+  ra.mv[1] = pack_mv(xa.x, xa.y); ra.mvp[1] = pa; ra.total += xa.cost;
+  rb.mv[1] = pack_mv(xb.x, xb.y); rb.mvp[1] = pb; rb.total += xb.cost;
+}
+
 __device__ __forceinline__ uint32_t sel4(int k, uint32_t a, uint32_t b, uint32_t cc, uint32_t d) {
   return k == 0 ? a : (k == 1 ? b : (k == 2 ? cc : d));
 }
@@ -668,21 +906,20 @@ __device__ __forceinline__ void analyse_mb(const SCtx &c, const Nbr &n, const ui
   o.skip = skip;
   o.m16 = r16.mv[0];
   if (partitions && cost16 > 1024u) {
-    PartRes r;
-    search_part<1>(c, n, src, sstride, mbx, mby, o.m16, r, lds);
+    PartRes r, r2;
+    search_part_12(c, n, src, sstride, mbx, mby, o.m16, r, r2, lds);
     o.cost[1] = r.total;
     uint32_t cc = r.total + ((c.lam * 3u) >> 8);
     bool t = cc < o.best_cost;
     o.best_cost = t ? cc : o.best_cost;
     o.part = t ? 1 : o.part;
     take_if(o.best, r, t);
-    search_part<2>(c, n, src, sstride, mbx, mby, o.m16, r, lds);
-    o.cost[2] = r.total;
-    cc = r.total + ((c.lam * 3u) >> 8);
+    o.cost[2] = r2.total;
+    cc = r2.total + ((c.lam * 3u) >> 8);
     t = cc < o.best_cost;
     o.best_cost = t ? cc : o.best_cost;
     o.part = t ? 2 : o.part;
-    take_if(o.best, r, t);
+    take_if(o.best, r2, t);
     search_part<3>(c, n, src, sstride, mbx, mby, o.m16, r, lds);
     o.cost[3] = r.total;
     cc = r.total + ((c.lam * 10u) >> 8);
