@@ -1,15 +1,21 @@
 """Benchmark: H.264 P-frame motion search (ME-only field) on one MI355X.
 
-Workload (one step): a seeded synthetic 1920x1080 (coded 1920x1088) clip of
-60 frames IPPP, QP 27, me_range 32, subpel_rounds 1, all four partition
-shapes; 12 rectangles of 64-400 px moving at up to 8 px/frame over a panned
-blurred-noise background.  The raw I420 luma frames are resident in HBM when
-the timed region starts; a step imports/pads the 60 frames, builds the
-half-pel planes of the 59 reference frames and searches every macroblock of
-the 59 P frames (481,440 MBs) -- the whole hot path of the reference's
+Default workload (one step): a seeded synthetic 1920x1080 (coded 1920x1088)
+clip of 60 frames IPPP, QP 27, me_range 32, subpel_rounds 1, all four
+partition shapes; 12 rectangles of 64-400 px moving at up to 8 px/frame over
+a panned blurred-noise background.  The raw luma frames are resident in HBM
+when the timed region starts; a step imports/pads the frames, builds the
+half-pel planes of the reference frames and searches every macroblock of
+the P frames (59 x 8,160 = 481,440 MBs) -- the hot path of the reference's
 search (src/h264/enc/slice.rs:1590-1945, picture.rs:318-449).
 
-Prints one JSON line (see the field list at the end of main()).
+Other workloads (--workload): "batch" = 64 clips of uneven size (16 each
+of 352x288, 640x360, 1280x720, 1920x1080), 30 frames each, QP per clip from
+{22,27,32,37}, me_range 32, all in one persistent launch; "4k" = 3840x2160,
+24 frames, me_range 64, QP 27, 8 rects of 128-900 px at up to 16 px/frame,
+0.2%/frame zoom.
+
+Prints one JSON line (fields at the end of main()).
 """
 from __future__ import annotations
 
@@ -28,8 +34,9 @@ METRIC = "1080p H.264 ME Mblocks/s: plain GPU vs CNN-pruned at <=1% BD-rate; GB/
 HBM_PEAK_GBS = 8000.0
 # Integer VALU peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3 T lane-op/s
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
-
-W, H, FRAMES, QP, RANGE, SUBPEL = 1920, 1080, 60, 27, 32, 1
+# Algorithmic HBM bytes of the search per MB: 256 B source luma, 4 x 256 B
+# reference planes read once, 128 B record, 32 B exchange words.
+BYTES_PER_MB = 256 + 4 * 256 + 128 + 32
 
 
 def parse():
@@ -37,13 +44,33 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-frames", type=int, default=16,
-                    help="P frames of the CPU-baseline sample")
+    ap.add_argument("--workload", choices=("1080p", "batch", "4k"), default="1080p")
+    ap.add_argument("--cpu-frames", type=int, default=16, help="P frames of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known "
                          "traffic, for scaling the FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+def clip_specs(pcme, workload):
+    """[(ClipSpec, qp, me_range)] of the workload."""
+    if workload == "1080p":
+        return [(pcme.ClipSpec(1920, 1080, 60, nrect=12, rect_lo=64, rect_hi=400, vmax=8.0, pan_max=1.0,
+                               seed=0x5EED0000), 27, 32)]
+    if workload == "4k":
+        return [(pcme.ClipSpec(3840, 2160, 24, nrect=8, rect_lo=128, rect_hi=900, vmax=16.0, pan_max=1.0,
+                               zoom=0.002, seed=0x5EED0000), 27, 64)]
+    out = []
+    sizes = [(352, 288, 8, 120, 6.0), (640, 360, 8, 180, 6.0), (1280, 720, 10, 300, 8.0), (1920, 1080, 12, 400, 8.0)]
+    qps = (22, 27, 32, 37)
+    for i in range(64):
+        w, h, nrect, hi, v = sizes[i // 16]
+        out.append((pcme.ClipSpec(w, h, 30, nrect=nrect, rect_lo=24 if w < 1000 else 64, rect_hi=hi, vmax=v,
+                                  pan_max=1.0, seed=0x5EED0000 + i), qps[(i * 7 + 3) % 4], 32))
+    return out
 
 
 def main():
@@ -66,28 +93,47 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    spec = pcme.ClipSpec(W, H, FRAMES, nrect=12, rect_lo=64, rect_hi=400, vmax=8.0, pan_max=1.0,
-                         zoom=0.0, noise_sigma=2.0, seed=0x5EED0000)
-    t0 = time.time()
-    y, _, _ = pcme.synth_clip(spec, threads=min(16, os.cpu_count() or 1))
-    print(f"[bench] generated {FRAMES} frames in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-    mbw, mbh = pcme.mb_dims(W, H)
-    clip = pcme.DeviceClip(y, mbw, mbh)
-    nP = FRAMES - 1
-    mbs_per_step = nP * mbw * mbh
-    out = torch.empty((nP, mbw * mbh * pcme.MB_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
-    jobs = [pcme.frame_job(clip, t, t - 1, out[t - 1].data_ptr(), RANGE, SUBPEL, True, QP) for t in range(1, FRAMES)]
-    stream = torch.cuda.current_stream()
 
-    ev = []  # (start, end) events around each search launch
+    specs = clip_specs(pcme, args.workload)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.time()
+    clips, outs, jobs, host = [], [], [], []
+    for spec, qp, rng in specs:
+        y, _, _ = pcme.synth_clip(spec, threads=threads)
+        mbw, mbh = pcme.mb_dims(spec.w, spec.h)
+        dc = pcme.DeviceClip(y, mbw, mbh)
+        o = torch.empty((spec.frames - 1, mbw * mbh * pcme.MB_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+        jobs += [pcme.frame_job(dc, t, t - 1, o[t - 1].data_ptr(), rng, 1, True, qp) for t in range(1, spec.frames)]
+        clips.append(dc)
+        outs.append(o)
+        host.append((y, mbw, mbh, qp, rng))
+    print(f"[bench] generated {sum(s.frames for s, _, _ in specs)} frames in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    mbs_per_step = sum((dc.n - 1) * dc.mbw * dc.mbh for dc in clips)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    def prepare():
+        for dc in clips:
+            dc.pad(sp)
+            dc.hpel(0, dc.n - 1, sp)
+
+    plan = pcme.Plan(jobs)
+    prepare()
+    tune = None
+    if not args.no_autotune:
+        waves, tms = plan.autotune(reps=2, stream=sp)
+        tune = {"waves": waves, "search_ms": round(tms, 3)}
+        print(f"[bench] autotune: {waves} waves, {tms:.2f} ms", file=sys.stderr, flush=True)
+
+    ev = []  # (start, end) events around each search launch, on the launch stream
 
     def step(record: bool):
-        clip.pad(stream.cuda_stream)
-        clip.hpel(0, nP, stream.cuda_stream)
+        prepare()
         if record:
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             a.record(stream)
-        pcme.search_frames(jobs, stream.cuda_stream, sync=False)
+        plan.launch(sp)
         if record:
             b.record(stream)
             ev.append((a, b))
@@ -100,55 +146,82 @@ def main():
         step(True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    plan.check(sp)
     ms_step = 1e3 * dt / args.steps
     value = mbs_per_step * args.steps / dt / 1e6
     search_ms = sum(a.elapsed_time(b) for a, b in ev) / len(ev)
     print(f"[bench] {ms_step:.2f} ms/step, search kernel {search_ms:.2f} ms", file=sys.stderr, flush=True)
 
-    # Stall check + results of the last step.
-    pcme.search_frames(jobs, stream.cuda_stream, sync=True)
-    gpu_res = out.cpu().numpy().view(pcme.MB_DTYPE).reshape(nP, mbh, mbw)
+    graph = None
+    if not args.no_graph:
+        # The same step captured once as a hipGraph and replayed.
+        g = torch.cuda.CUDAGraph()
+        s2 = torch.cuda.Stream()
+        s2.wait_stream(stream)
+        with torch.cuda.stream(s2):
+            with torch.cuda.graph(g, stream=s2, capture_error_mode="thread_local"):
+                for dc in clips:
+                    dc.pad(s2.cuda_stream)
+                    dc.hpel(0, dc.n - 1, s2.cuda_stream)
+                plan.launch(s2.cuda_stream)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            g.replay()
+        torch.cuda.synchronize()
+        gdt = time.perf_counter() - t1
+        plan.check(sp)
+        graph = {"ms_per_step": round(1e3 * gdt / args.steps, 3),
+                 "mblocks_per_s": round(mbs_per_step * args.steps / gdt / 1e6, 4)}
+        print(f"[bench] graph replay {graph['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
 
-    # Algorithmic bytes of one search launch (per MB: 256 B source luma,
-    # 4 x 256 B reference planes read once, 128 B record, 32 B exchange).
-    bytes_per_mb = 256 + 4 * 256 + 128 + 32
-    traffic_bytes = bytes_per_mb * mbs_per_step
-    achieved_gbs = traffic_bytes / (search_ms * 1e-3) / 1e9
+    achieved_gbs = BYTES_PER_MB * mbs_per_step / (search_ms * 1e-3) / 1e9
+    wl = {"1080p": "plain GPU ME-only search, 1920x1080 60 frames IPPP, 1 ref, QP 27, me_range 32 + qpel, "
+                   "partitions 16x16/16x8/8x16/8x8, 12 rects 64-400 px, v in [-8,8] px/frame",
+          "batch": "plain GPU ME-only search, 64 clips (16 each of 352x288, 640x360, 1280x720, 1920x1080), "
+                   "30 frames each IPPP, QP per clip from {22,27,32,37}, me_range 32, one persistent launch",
+          "4k": "plain GPU ME-only search, 3840x2160 24 frames IPPP, 1 ref, QP 27, me_range 64 + qpel, "
+                "8 rects 128-900 px, v in [-16,16] px/frame, 0.2%/frame zoom"}[args.workload]
     line = {
         "metric": METRIC, "value": round(value, 4), "unit": "Mblocks/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
         "data": "synthetic (seeded generator, include/pcme_synth.h)",
-        "config": {"workload": "plain GPU ME-only search, 1920x1080 60 frames IPPP, 1 ref, QP 27, "
-                               "me_range 32 + qpel, partitions 16x16/16x8/8x16/8x8, 12 rects 64-400 px, "
-                               "v in [-8,8] px/frame",
-                   "mbs_per_step": mbs_per_step, "p_frames": nP, "parallelism": "single GPU"},
+        "config": {"workload": wl, "mbs_per_step": mbs_per_step, "p_frames": len(jobs),
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "search_frames_kernel", "achieved": round(achieved_gbs, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
-                     "traffic": None, "algorithmic_bytes_per_mb": bytes_per_mb,
+                     "traffic": None, "algorithmic_bytes_per_mb": BYTES_PER_MB,
                      "kernel_ms": round(search_ms, 3)},
+        "autotune": tune,
+        "graph": graph,
     }
 
     if not args.no_cpu:
         from oracle import oracle as O
+        y, mbw, mbh, qp, rng = host[-1]
+        nP = y.shape[0] - 1
         k = min(args.cpu_frames, nP)
-        threads = max(1, min(16, os.cpu_count() or 1))
         src = [O.pad_luma(y[t], mbw, mbh) for t in range(1, k + 1)]
         ref = [O.pad_luma(y[t - 1], mbw, mbh) for t in range(1, k + 1)]
-        res, secs, ctr = O.search_jobs_mt(src, ref, mbw, mbh, RANGE, SUBPEL, True, QP, threads)
+        res, secs, ctr = O.search_jobs_mt(src, ref, mbw, mbh, rng, 1, True, qp, threads)
         cpu_val = k * mbw * mbh / secs / 1e6
+        gpu_res = outs[-1].cpu().numpy().view(pcme.MB_DTYPE).reshape(nP, mbh, mbw)
         parity = all(np.array_equal(np.ascontiguousarray(gpu_res[i]).view(np.uint8),
                                     np.ascontiguousarray(res[i]).view(np.uint8)) for i in range(k))
         # Lane-op count of the same work (the GPU replays the oracle's path):
-        # SAD 0.25/px, MC+SATD 3.75/px, half-pel build 9/px of reference.
+        # SAD 0.25/px, MC+SATD 3.75/px.
         per_mb_ops = (0.25 * ctr.sad_px + 3.75 * ctr.satd_px) / max(ctr.mbs, 1)
         valu_tops = per_mb_ops * mbs_per_step / (search_ms * 1e-3) / 1e12
         line["valu_roofline"] = {"bound": "valu", "achieved": round(valu_tops, 3), "peak": round(VALU_PEAK_TOPS, 2),
                                  "unit": "T lane-op/s", "frac": round(valu_tops / VALU_PEAK_TOPS, 4),
                                  "lane_ops_per_mb": round(per_mb_ops, 1)}
         line["cpu_baseline"] = {"value": round(cpu_val, 5), "unit": "Mblocks/s", "cores": threads, "kind": "port",
-                                "sample": f"{k} P frames of the same 1080p clip (frames 1..{k}), half-pel build + "
-                                          f"search, one frame per thread, C oracle -O3 -march=x86-64-v3",
+                                "sample": f"{k} P frames of the last clip of the workload ({y.shape[2]}x{y.shape[1]}), "
+                                          f"half-pel build + search, one frame per thread, C oracle -O3 "
+                                          f"-march=x86-64-v3",
                                 "seconds": round(secs, 3)}
         line["parity_sample"] = bool(parity)
     print(json.dumps(line), flush=True)

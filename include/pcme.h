@@ -142,6 +142,40 @@ int pcme_search_frames(const pcme_frame_job *jobs, int njobs, void *stream);
 int pcme_search_frames_sync(const pcme_frame_job *jobs, int njobs,
                             void *stream);
 
+/* Plans: the same ME-only field search, split into a set-up step and a
+ * launch step, for callers that replay one frame-step layout many times
+ * (a frame pipeline, hipGraph capture).  Same computation and results as
+ * pcme_search_frames (which keeps one such plan cached per device).
+ *
+ * pcme_plan_create copies the job table (host array) to the device and
+ * allocates the plan's own workspace (synchronous).  The device buffers the
+ * jobs point to must stay valid while the plan is used; their contents may
+ * change between launches.
+ * pcme_plan_launch enqueues kernels only -- no host synchronisation, copies
+ * or memsets -- so a stream capture of it replays correctly.  Launches of one
+ * plan must not overlap (they share the plan's workspace); different plans
+ * may run concurrently.
+ * pcme_plan_check waits for `stream` and returns PCME_ESTALL if a
+ * dependency wait of the last launch exceeded its bound.
+ * pcme_plan_set_grid sets the persistent grid (waves; 0 = every resident
+ * wave) and returns the value in effect.  pcme_plan_autotune times the plan
+ * over a set of grid sizes (`reps` launches each, on `stream`) and keeps the
+ * fastest; it writes the chosen wave count and its time. */
+typedef struct pcme_plan pcme_plan;
+int pcme_plan_create(const pcme_frame_job *jobs, int njobs, pcme_plan **out);
+int pcme_plan_launch(pcme_plan *plan, void *stream);
+int pcme_plan_check(pcme_plan *plan, void *stream);
+int pcme_plan_set_grid(pcme_plan *plan, int32_t waves);
+int pcme_plan_autotune(pcme_plan *plan, void *stream, int32_t reps,
+                       int32_t *best_waves, float *best_ms);
+void pcme_plan_destroy(pcme_plan *plan);
+
+/* Test hook: bound of every wavefront dependency wait, in polls (default
+ * 2^22; 0 restores it) for launches made after the call.  A tiny bound lets
+ * a test drive the stall path (every wave exits, PCME_ESTALL) without any
+ * fault.  Returns the previous bound.  No counterpart in the reference. */
+uint32_t pcme_set_spin_limit(uint32_t polls);
+
 /* Per-call form of SliceEnc::search (slice.rs:1590-1723), batched: job i
  * searches one w x h block at (bx, by) of a reference with predictor mvp and
  * candidates cands[0..ncands) (ncands <= 2, as search_part passes

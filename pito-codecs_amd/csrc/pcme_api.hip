@@ -33,30 +33,22 @@ static int fail(int code, const char *fmt, ...) {
       return fail(PCME_EHIP, "%s failed: %s", #call, hipGetErrorString(e_));         \
   } while (0)
 
-// Per-device state: workspaces of the search, grown on demand.
+struct Plan;
+
+// Per-device state: the per-call search workspace, the upload staging and
+// the plan cached by pcme_search_frames.
 struct Dev {
   int id = -1;
   std::mutex mu;
   int resident_waves = 0;
-  uint32_t *ctl = nullptr;          // [0] ticket counter, [1] stall flag
-  DevJob *djobs = nullptr;
-  size_t djobs_cap = 0;
-  int2 *dtickets = nullptr;
-  size_t dtickets_cap = 0;
-  uint32_t *xchg = nullptr;
-  size_t xchg_cap = 0;              // words
-  bool xchg_dirty = true;
   pcme_search_job *dsj = nullptr;
   size_t dsj_cap = 0;
   unsigned char *hstage = nullptr;  // pinned upload staging
   size_t hstage_cap = 0;
   hipEvent_t stage_ev = nullptr;
   bool stage_pending = false;
-  hipEvent_t done_ev = nullptr;     // end of the last search launch
-  bool done_recorded = false;
-  uint32_t tag = 0;
-  std::vector<DevJob> last_jobs;
-  std::vector<int2> last_tickets;
+  Plan *cached = nullptr;           // plan of the last pcme_search_frames call
+  std::vector<pcme_frame_job> cached_jobs;
 };
 
 static std::mutex g_mu;
@@ -68,19 +60,21 @@ static bool is_gfx950(int d) {
   return strncmp(p.gcnArchName, "gfx950", 6) == 0;
 }
 
+// The current device's state; the first call per device validates it.
+// Later calls only read the current device id, so entry points stay cheap
+// and safe inside a stream capture.
 static int get_dev(Dev **out) {
-  int d = 0;
-  int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
-    return fail(PCME_ENODEV, "no HIP device visible");
-  HIPCHK(hipGetDevice(&d));
-  if (!is_gfx950(d)) return fail(PCME_ENODEV, "device %d is not a gfx950 (MI355X)", d);
+  int d = -1;
+  if (hipGetDevice(&d) != hipSuccess || d < 0) return fail(PCME_ENODEV, "no HIP device visible");
   std::lock_guard<std::mutex> g(g_mu);
   auto it = g_devs.find(d);
   if (it != g_devs.end()) {
     *out = it->second;
     return PCME_OK;
   }
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(PCME_ENODEV, "no HIP device visible");
+  if (!is_gfx950(d)) return fail(PCME_ENODEV, "device %d is not a gfx950 (MI355X)", d);
   Dev *dv = new Dev();
   dv->id = d;
   hipDeviceProp_t p;
@@ -89,10 +83,7 @@ static int get_dev(Dev **out) {
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, search_frames_kernel, 64, 0));
   if (per_cu < 1) per_cu = 1;
   dv->resident_waves = per_cu * p.multiProcessorCount;
-  HIPCHK(hipMalloc(&dv->ctl, 64));
-  HIPCHK(hipMemset(dv->ctl, 0, 64));
   HIPCHK(hipEventCreateWithFlags(&dv->stage_ev, hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&dv->done_ev, hipEventDisableTiming));
   g_devs[d] = dv;
   *out = dv;
   return PCME_OK;
@@ -153,11 +144,28 @@ static int check_params(const pcme_params &p) {
   return PCME_OK;
 }
 
-static const uint32_t kSpinLimit = 1u << 22;
+// Bound of every dependency wait (polls of ~1 us); a test hook lowers it.
+static uint32_t kSpinLimit = 1u << 22;
 
-static int search_frames_impl(const pcme_frame_job *jobs, int njobs, hipStream_t stream, Dev **dvo) {
-  if (njobs < 0 || (njobs > 0 && !jobs)) return fail(PCME_EINVAL, "bad job array");
-  if (njobs == 0) return PCME_OK;
+// A frame-search plan: the device copy of a job table, its tickets and its
+// own exchange words and control words.  Launching a plan issues kernels
+// only (no host synchronisation, no copies, no memsets), so it can be
+// captured in a hipGraph and replayed.
+struct Plan {
+  int dev = -1;
+  int njobs = 0;
+  DevJob *djobs = nullptr;
+  int2 *dtickets = nullptr;
+  int ntickets = 0;
+  uint32_t *xchg = nullptr;
+  size_t nwords = 0;
+  uint32_t *ctl = nullptr;  // [0] tickets taken, [1] stall flag, [2] epoch, [3] clear request
+  int resident = 0;         // resident waves of the search kernel on this device
+  int grid = 0;             // waves launched (persistent grid)
+};
+
+static int check_jobs(const pcme_frame_job *jobs, int njobs) {
+  if (njobs < 1 || !jobs) return fail(PCME_EINVAL, "bad job array");
   int rc;
   for (int i = 0; i < njobs; i++) {
     if ((rc = check_params(jobs[i].prm))) return rc;
@@ -167,10 +175,23 @@ static int search_frames_impl(const pcme_frame_job *jobs, int njobs, hipStream_t
           (uintptr_t)jobs[i].ref_c) & 3u) != 0)
       return fail(PCME_EINVAL, "job %d: plane pointers must be 4-byte aligned", i);
   }
-  Dev *dv = nullptr;
-  if ((rc = get_dev(&dv))) return rc;
-  *dvo = dv;
-  std::lock_guard<std::mutex> g(dv->mu);
+  return PCME_OK;
+}
+
+static void plan_free(Plan *p) {
+  if (!p) return;
+  if (p->djobs) (void)hipFree(p->djobs);
+  if (p->dtickets) (void)hipFree(p->dtickets);
+  if (p->xchg) (void)hipFree(p->xchg);
+  if (p->ctl) (void)hipFree(p->ctl);
+  delete p;
+}
+
+static int plan_create(Dev *dv, const pcme_frame_job *jobs, int njobs, Plan **out) {
+  Plan *p = new Plan();
+  p->dev = dv->id;
+  p->njobs = njobs;
+  p->resident = dv->resident_waves;
   std::vector<DevJob> dj((size_t)njobs);
   size_t words = 0;
   int maxh = 0;
@@ -192,16 +213,6 @@ static int search_frames_impl(const pcme_frame_job *jobs, int njobs, hipStream_t
     words += (size_t)J.prm.mbw * J.prm.mbh * 4;
     maxh = std::max(maxh, J.prm.mbh);
   }
-  {
-    uint32_t *old = dv->xchg;
-    if ((rc = grow(&dv->xchg, &dv->xchg_cap, words))) return rc;
-    if (dv->xchg != old) dv->xchg_dirty = true;  // fresh memory: clear before use
-  }
-  size_t off = 0;
-  for (int i = 0; i < njobs; i++) {
-    dj[(size_t)i].xchg = dv->xchg + off;
-    off += (size_t)dj[(size_t)i].mbw * dj[(size_t)i].mbh * 4;
-  }
   // Tickets in (row, job) order: a row's dependency (row-1 of the same job)
   // always has a smaller ticket.
   std::vector<int2> tk;
@@ -209,42 +220,72 @@ static int search_frames_impl(const pcme_frame_job *jobs, int njobs, hipStream_t
   for (int y = 0; y < maxh; y++)
     for (int i = 0; i < njobs; i++)
       if (y < dj[(size_t)i].mbh) tk.push_back(make_int2(i, y));
-  const bool same = dv->last_jobs.size() == dj.size() && dv->last_tickets.size() == tk.size() &&
-                    memcmp(dv->last_jobs.data(), dj.data(), dj.size() * sizeof(DevJob)) == 0 &&
-                    memcmp(dv->last_tickets.data(), tk.data(), tk.size() * sizeof(int2)) == 0;
-  if (!same) {
-    if ((rc = grow(&dv->djobs, &dv->djobs_cap, dj.size()))) return rc;
-    if ((rc = grow(&dv->dtickets, &dv->dtickets_cap, tk.size()))) return rc;
-    if ((rc = upload(dv, dv->djobs, dj.data(), dj.size() * sizeof(DevJob), stream))) return rc;
-    // second upload waits for the first staged copy to finish
-    if ((rc = upload(dv, dv->dtickets, tk.data(), tk.size() * sizeof(int2), stream))) return rc;
-    dv->last_jobs = dj;
-    dv->last_tickets = tk;
+  p->ntickets = (int)tk.size();
+  p->nwords = words;
+  p->grid = std::min(p->ntickets, p->resident);
+  if (hipMalloc((void **)&p->xchg, words * 4) != hipSuccess || hipMalloc((void **)&p->ctl, 64) != hipSuccess ||
+      hipMalloc((void **)&p->djobs, dj.size() * sizeof(DevJob)) != hipSuccess ||
+      hipMalloc((void **)&p->dtickets, tk.size() * sizeof(int2)) != hipSuccess) {
+    plan_free(p);
+    return fail(PCME_ENOMEM, "plan allocation of %zu exchange words failed", words);
   }
-  dv->tag++;
-  if (dv->tag > 255u) {
-    dv->tag = 1;
-    dv->xchg_dirty = true;
+  size_t off = 0;
+  for (int i = 0; i < njobs; i++) {
+    dj[(size_t)i].xchg = p->xchg + off;
+    off += (size_t)dj[(size_t)i].mbw * dj[(size_t)i].mbh * 4;
   }
-  if (dv->xchg_dirty) dv->tag = 1;
-  // Launches share the ticket counter and the exchange words: order this one
-  // after the previous one even when they come on different streams.
-  if (dv->done_recorded) HIPCHK(hipStreamWaitEvent(stream, dv->done_ev, 0));
-  {
-    size_t nw = dv->xchg_dirty ? dv->xchg_cap : 0;
-    int blocks = dv->xchg_dirty ? 1024 : 1;
-    hipLaunchKernelGGL(prep_kernel, dim3(blocks), dim3(256), 0, stream, dv->ctl,
-                       dv->xchg_dirty ? dv->xchg : nullptr, nw);
-    HIPCHK(hipGetLastError());
-    dv->xchg_dirty = false;
+  // ctl[2] = 0: the first launch takes tag 1 and clears the exchange words.
+  if (hipMemset(p->ctl, 0, 64) != hipSuccess ||
+      hipMemcpy(p->djobs, dj.data(), dj.size() * sizeof(DevJob), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p->dtickets, tk.data(), tk.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) {
+    plan_free(p);
+    return fail(PCME_EHIP, "plan upload failed");
   }
-  int grid = (int)std::min<size_t>(tk.size(), (size_t)dv->resident_waves);
-  hipLaunchKernelGGL(search_frames_kernel, dim3(grid), dim3(64), 0, stream, dv->djobs, dv->dtickets,
-                     (int)tk.size(), dv->ctl, dv->tag, kSpinLimit);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(dv->done_ev, stream));
-  dv->done_recorded = true;
+  *out = p;
   return PCME_OK;
+}
+
+static int plan_launch(Plan *p, hipStream_t stream) {
+  hipLaunchKernelGGL(prep_kernel, dim3(1), dim3(64), 0, stream, p->ctl);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(clear_kernel, dim3(512), dim3(256), 0, stream, p->ctl, p->xchg, p->nwords);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(search_frames_kernel, dim3((unsigned)p->grid), dim3(64), 0, stream, p->djobs, p->dtickets,
+                     p->ntickets, p->ctl, kSpinLimit);
+  HIPCHK(hipGetLastError());
+  return PCME_OK;
+}
+
+static int plan_check(Plan *p, hipStream_t stream) {
+  HIPCHK(hipStreamSynchronize(stream));
+  uint32_t flag = 0;
+  HIPCHK(hipMemcpy(&flag, p->ctl + 1, 4, hipMemcpyDeviceToHost));
+  if (flag) return fail(PCME_ESTALL, "a wavefront dependency wait exceeded its bound");
+  return PCME_OK;
+}
+
+static bool same_jobs(const std::vector<pcme_frame_job> &a, const pcme_frame_job *b, int n) {
+  return a.size() == (size_t)n && memcmp(a.data(), b, (size_t)n * sizeof(pcme_frame_job)) == 0;
+}
+
+static int search_frames_impl(const pcme_frame_job *jobs, int njobs, hipStream_t stream, bool sync) {
+  if (njobs == 0) return PCME_OK;
+  int rc;
+  if ((rc = check_jobs(jobs, njobs))) return rc;
+  Dev *dv = nullptr;
+  if ((rc = get_dev(&dv))) return rc;
+  std::lock_guard<std::mutex> g(dv->mu);
+  if (!dv->cached || !same_jobs(dv->cached_jobs, jobs, njobs)) {
+    if (dv->cached) {
+      HIPCHK(hipDeviceSynchronize());
+      plan_free(dv->cached);
+      dv->cached = nullptr;
+    }
+    if ((rc = plan_create(dv, jobs, njobs, &dv->cached))) return rc;
+    dv->cached_jobs.assign(jobs, jobs + njobs);
+  }
+  if ((rc = plan_launch(dv->cached, stream))) return rc;
+  return sync ? plan_check(dv->cached, stream) : PCME_OK;
 }
 
 }  // namespace pcme
@@ -315,22 +356,94 @@ int pcme_hpel(const uint8_t *ref, uint8_t *h, uint8_t *v, uint8_t *c, int64_t pi
 }
 
 int pcme_search_frames(const pcme_frame_job *jobs, int njobs, void *stream) {
-  Dev *dv = nullptr;
-  return search_frames_impl(jobs, njobs, (hipStream_t)stream, &dv);
+  return search_frames_impl(jobs, njobs, (hipStream_t)stream, false);
 }
 
 int pcme_search_frames_sync(const pcme_frame_job *jobs, int njobs, void *stream) {
+  return search_frames_impl(jobs, njobs, (hipStream_t)stream, true);
+}
+
+int pcme_plan_create(const pcme_frame_job *jobs, int njobs, pcme_plan **out) {
+  if (!out) return fail(PCME_EINVAL, "pcme_plan_create: null out");
+  *out = nullptr;
+  int rc;
+  if ((rc = check_jobs(jobs, njobs))) return rc;
   Dev *dv = nullptr;
-  int rc = search_frames_impl(jobs, njobs, (hipStream_t)stream, &dv);
-  if (rc || !dv) return rc;
-  HIPCHK(hipStreamSynchronize((hipStream_t)stream));
-  uint32_t flag = 0;
-  {
-    std::lock_guard<std::mutex> g(dv->mu);
-    HIPCHK(hipMemcpy(&flag, dv->ctl + 1, 4, hipMemcpyDeviceToHost));
-  }
-  if (flag) return fail(PCME_ESTALL, "a wavefront dependency wait exceeded its bound");
+  if ((rc = get_dev(&dv))) return rc;
+  Plan *p = nullptr;
+  if ((rc = plan_create(dv, jobs, njobs, &p))) return rc;
+  *out = (pcme_plan *)p;
   return PCME_OK;
+}
+
+int pcme_plan_launch(pcme_plan *plan, void *stream) {
+  if (!plan) return fail(PCME_EINVAL, "pcme_plan_launch: null plan");
+  return plan_launch((Plan *)plan, (hipStream_t)stream);
+}
+
+int pcme_plan_check(pcme_plan *plan, void *stream) {
+  if (!plan) return fail(PCME_EINVAL, "pcme_plan_check: null plan");
+  return plan_check((Plan *)plan, (hipStream_t)stream);
+}
+
+int pcme_plan_set_grid(pcme_plan *plan, int32_t waves) {
+  if (!plan || waves < 0) return fail(PCME_EINVAL, "pcme_plan_set_grid: bad arguments");
+  Plan *p = (Plan *)plan;
+  int w = waves == 0 ? p->resident : waves;
+  p->grid = std::max(1, std::min(w, p->ntickets));
+  return p->grid;
+}
+
+int pcme_plan_autotune(pcme_plan *plan, void *stream, int32_t reps, int32_t *best_waves, float *best_ms) {
+  if (!plan || reps < 1) return fail(PCME_EINVAL, "pcme_plan_autotune: bad arguments");
+  Plan *p = (Plan *)plan;
+  hipStream_t st = (hipStream_t)stream;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  // Candidate persistent grids: fractions of the resident capacity.  Fewer
+  // waves per SIMD give each row walker more issue slots (shorter critical
+  // path); more waves hide latency and keep more rows in flight.
+  const int fr[] = {16, 14, 12, 10, 8, 6};
+  int best = p->grid;
+  float bms = 1e30f;
+  int rc = PCME_OK;
+  for (int f : fr) {
+    int w = std::max(1, std::min(p->ntickets, p->resident * f / 16));
+    p->grid = w;
+    if ((rc = plan_launch(p, st))) break;  // warm-up
+    HIPCHK(hipEventRecord(e0, st));
+    for (int r = 0; r < reps && !rc; r++) rc = plan_launch(p, st);
+    if (rc) break;
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    if ((rc = plan_check(p, st))) break;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    ms /= (float)reps;
+    if (ms < bms) {
+      bms = ms;
+      best = w;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  p->grid = best;
+  if (best_waves) *best_waves = best;
+  if (best_ms) *best_ms = bms;
+  return rc;
+}
+
+uint32_t pcme_set_spin_limit(uint32_t polls) {
+  uint32_t old = kSpinLimit;
+  kSpinLimit = polls == 0 ? (1u << 22) : polls;
+  return old;
+}
+
+void pcme_plan_destroy(pcme_plan *plan) {
+  if (!plan) return;
+  (void)hipDeviceSynchronize();
+  plan_free((Plan *)plan);
 }
 
 int pcme_search_batch(const pcme_search_job *jobs, int njobs, pcme_mv *out_mv, uint32_t *out_cost,

@@ -26,7 +26,7 @@ struct DevJob {
 
 #ifdef __HIP__
 __global__ void search_frames_kernel(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
-                                     int ntickets, uint32_t *ctl, uint32_t tag, uint32_t spin_limit);
+                                     int ntickets, uint32_t *ctl, uint32_t spin_limit);
 __global__ void search_batch_kernel(const pcme_search_job *__restrict__ jobs, int njobs,
                                     pcme_mv *out_mv, uint32_t *out_cost);
 __global__ void pad_planes_kernel(const uint8_t *__restrict__ src, int64_t src_pitch, int sstride, int w,
@@ -41,7 +41,8 @@ __global__ void pred_frame_kernel(const uint8_t *__restrict__ pl0, const uint8_t
                                   const pcme_mb_out *__restrict__ mbs, int mbw, int mbh,
                                   uint8_t *__restrict__ oy, uint8_t *__restrict__ ou,
                                   uint8_t *__restrict__ ov);
-__global__ void prep_kernel(uint32_t *ctl, uint32_t *xchg, size_t nwords);
+__global__ void prep_kernel(uint32_t *ctl);
+__global__ void clear_kernel(const uint32_t *ctl, uint32_t *xchg, size_t nwords);
 #endif
 
 // Tile of the half-pel kernel.

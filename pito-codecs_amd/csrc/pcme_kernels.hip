@@ -1000,11 +1000,11 @@ __device__ __forceinline__ bool fetch_words(const uint32_t *xw, uint32_t tag, ui
 
 __global__ void __launch_bounds__(64) search_frames_kernel(const DevJob *__restrict__ jobs,
                                                            const int2 *__restrict__ tickets, int ntickets,
-                                                           uint32_t *ctl, uint32_t tag,
-                                                           uint32_t spin_limit) {
+                                                           uint32_t *ctl, uint32_t spin_limit) {
   __shared__ uint32_t s_lds[LDS_PER_WAVE / 4];
   uint32_t *counter = ctl;
   uint32_t *err = ctl + 1;
+  const uint32_t tag = ctl[2] & 0xFFu;  // this launch's epoch (set by prep_kernel)
   const int l = (int)lane_id();
   for (;;) {
     uint32_t t = 0;
@@ -1271,12 +1271,26 @@ __global__ void __launch_bounds__(256) pred_frame_kernel(const uint8_t *__restri
 
 // Reset the ticket counter and the error flag; optionally clear the
 // exchange words (on tag wrap-around).
-__global__ void __launch_bounds__(256) prep_kernel(uint32_t *ctl, uint32_t *xchg, size_t nwords) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 2) ctl[i] = 0;
-  if (xchg) {
-    for (; i < nwords; i += (size_t)gridDim.x * blockDim.x) xchg[i] = 0;
+// Per launch: reset the ticket counter and the stall flag and advance the
+// launch epoch ctl[2], whose low byte tags the exchange words.  Tag 0 is
+// skipped; when the tag wraps to 1, ctl[3] asks clear_kernel to zero the
+// exchange words so no word of 255 launches ago can pass for a fresh one.
+// Everything lives in device memory, so a captured graph replays correctly.
+__global__ void prep_kernel(uint32_t *ctl) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    uint32_t e = ctl[2] + 1u;
+    if ((e & 0xFFu) == 0u) e += 1u;
+    ctl[0] = 0u;
+    ctl[1] = 0u;
+    ctl[2] = e;
+    ctl[3] = (e & 0xFFu) == 1u ? 1u : 0u;
   }
+}
+
+__global__ void __launch_bounds__(256) clear_kernel(const uint32_t *ctl, uint32_t *xchg, size_t nwords) {
+  if (ctl[3] == 0u) return;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += (size_t)gridDim.x * blockDim.x)
+    xchg[i] = 0u;
 }
 
 }  // namespace pcme

@@ -93,7 +93,8 @@ EXPORTS = (
     "pcme_version", "pcme_last_error", "pcme_device_count", "pcme_lambda",
     "pcme_qpel_table", "pcme_pad_planes", "pcme_hpel", "pcme_search_frames",
     "pcme_search_frames_sync", "pcme_search_batch", "pcme_pred_frame",
-    "pcme_synth_clip",
+    "pcme_synth_clip", "pcme_plan_create", "pcme_plan_launch", "pcme_plan_check",
+    "pcme_plan_set_grid", "pcme_plan_autotune", "pcme_plan_destroy", "pcme_set_spin_limit",
 )
 
 _lib = None
@@ -128,6 +129,13 @@ def load() -> ctypes.CDLL:
         "pcme_search_batch": (c.c_int, [c.POINTER(SearchJob), c.c_int, vp, vp, vp]),
         "pcme_pred_frame": (c.c_int, [vp, vp, vp, vp, vp, vp, vp, c.c_int32, c.c_int32, vp, vp, vp, vp]),
         "pcme_synth_clip": (c.c_int, [c.POINTER(SynthCfg), vp, vp, vp, c.c_int32]),
+        "pcme_plan_create": (c.c_int, [c.POINTER(FrameJob), c.c_int, c.POINTER(vp)]),
+        "pcme_plan_launch": (c.c_int, [vp, vp]),
+        "pcme_plan_check": (c.c_int, [vp, vp]),
+        "pcme_plan_set_grid": (c.c_int, [vp, c.c_int32]),
+        "pcme_plan_autotune": (c.c_int, [vp, vp, c.c_int32, c.POINTER(c.c_int32), c.POINTER(c.c_float)]),
+        "pcme_plan_destroy": (None, [vp]),
+        "pcme_set_spin_limit": (c.c_uint32, [c.c_uint32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -284,6 +292,48 @@ def search_frames(jobs: list[FrameJob], stream=None, sync: bool = True) -> None:
     s = _stream_ptr(stream)
     fn = lib.pcme_search_frames_sync if sync else lib.pcme_search_frames
     _check(fn(arr, len(jobs), s))
+
+
+class Plan:
+    """A reusable frame-search plan (pcme_plan_*): set up once, launch many
+    times; launches are graph-capturable."""
+
+    def __init__(self, jobs: list[FrameJob]):
+        lib = load()
+        self._jobs = (FrameJob * len(jobs))(*jobs)
+        h = ctypes.c_void_p()
+        _check(lib.pcme_plan_create(self._jobs, len(jobs), ctypes.byref(h)))
+        self.handle = h
+
+    def launch(self, stream=None) -> None:
+        _check(load().pcme_plan_launch(self.handle, _stream_ptr(stream)))
+
+    def check(self, stream=None) -> None:
+        _check(load().pcme_plan_check(self.handle, _stream_ptr(stream)))
+
+    def set_grid(self, waves: int) -> int:
+        rc = load().pcme_plan_set_grid(self.handle, int(waves))
+        if rc < 0:
+            _check(rc)
+        return rc
+
+    def autotune(self, reps: int = 3, stream=None) -> tuple[int, float]:
+        w = ctypes.c_int32()
+        ms = ctypes.c_float()
+        _check(load().pcme_plan_autotune(self.handle, _stream_ptr(stream), int(reps), ctypes.byref(w),
+                                         ctypes.byref(ms)))
+        return int(w.value), float(ms.value)
+
+    def close(self) -> None:
+        if self.handle:
+            load().pcme_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def search_batch(jobs: list[SearchJob], out_mv_ptr: int, out_cost_ptr: int, stream=None) -> None:

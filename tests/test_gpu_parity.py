@@ -252,3 +252,142 @@ def test_pred_frame_matches_oracle(gpu, oracle):
             np.testing.assert_array_equal(gy[my, mx], wy, err_msg=f"luma MB {mx},{my}")
             np.testing.assert_array_equal(guv[0][my, mx], wu, err_msg=f"U MB {mx},{my}")
             np.testing.assert_array_equal(guv[1][my, mx], wv, err_msg=f"V MB {mx},{my}")
+
+
+# ------------------------------------------------------------------ plans
+
+def _jobs_for(gpu, dc, outs, rng, sub, parts, qp):
+    return [gpu.frame_job(dc, t, t - 1, outs[t - 1].data_ptr(), rng, sub, parts, qp) for t in range(1, dc.n)]
+
+
+def _field(gpu, outs, n, mbh, mbw):
+    return outs[:n].cpu().numpy().view(gpu.MB_DTYPE).reshape(n, mbh, mbw)
+
+
+def test_plan_graph_replay_matches_oracle(gpu, oracle):
+    """A captured hipGraph of a plan launch, replayed over changed inputs
+    (eager pad/hpel of a new clip between replays), equals the oracle on
+    every replay: the launch tag lives in device memory."""
+    import torch
+    say("plan graph replay")
+    w, h, n = 160, 96, 4
+    mbw, mbh = gpu.mb_dims(w, h)
+    clips = [_clip(gpu, w, h, n, seed=0x5EED0000 + k, nrect=3, rect_lo=16, rect_hi=48, vmax=4.0)[0]
+             for k in range(3)]
+    dc = gpu.DeviceClip(clips[0], mbw, mbh)
+    outs = torch.zeros((n - 1, mbw * mbh * gpu.MB_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+    plan = gpu.Plan(_jobs_for(gpu, dc, outs, 16, 1, True, 28))
+    dc.pad()
+    dc.hpel(0, n - 1)
+    torch.cuda.synchronize()
+    plan.launch()  # eager launch before capture
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s, capture_error_mode="thread_local"):
+            plan.launch(s.cuda_stream)
+    for k in (1, 2, 0, 1):
+        dc.raw.copy_(torch.as_tensor(clips[k]).cuda())
+        dc.pad()
+        dc.hpel(0, n - 1)
+        outs.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        plan.check()
+        want = oracle.search_clip_me(clips[k], 16, 1, True, 28)
+        assert_same_field(_field(gpu, outs, n - 1, mbh, mbw), want, f"replay of clip {k}")
+    plan.close()
+
+
+def test_batch_of_uneven_clips_in_one_launch(gpu, oracle):
+    """Several clips of different sizes and parameters searched by one plan
+    (one persistent launch), each equal to the oracle."""
+    import torch
+    say("uneven batch")
+    specs = [(176, 144, 3, 32, 1, True, 22), (320, 240, 3, 16, 1, True, 37), (96, 64, 4, 8, 0, False, 27),
+             (352, 288, 2, 32, 2, True, 32)]
+    dcs, outs, jobs, ys = [], [], [], []
+    for k, (w, h, n, rng, sub, parts, qp) in enumerate(specs):
+        y, _, _ = _clip(gpu, w, h, n, seed=0x5EED0000 + 17 * k, nrect=4, rect_lo=16, rect_hi=80, vmax=6.0)
+        mbw, mbh = gpu.mb_dims(w, h)
+        dc = gpu.DeviceClip(y, mbw, mbh)
+        dc.pad()
+        dc.hpel(0, n - 1)
+        o = torch.zeros((n - 1, mbw * mbh * gpu.MB_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+        jobs += _jobs_for(gpu, dc, o, rng, sub, parts, qp)
+        dcs.append(dc)
+        outs.append(o)
+        ys.append(y)
+    torch.cuda.synchronize()
+    plan = gpu.Plan(jobs)
+    plan.launch()
+    plan.check()
+    for k, (w, h, n, rng, sub, parts, qp) in enumerate(specs):
+        mbw, mbh = gpu.mb_dims(w, h)
+        want = oracle.search_clip_me(ys[k], rng, sub, parts, qp)
+        assert_same_field(_field(gpu, outs[k], n - 1, mbh, mbw), want, f"clip {k} {w}x{h}")
+    plan.close()
+
+
+def test_autotune_keeps_results(gpu, oracle):
+    import torch
+    say("autotune")
+    w, h, n = 320, 240, 5
+    y, _, _ = _clip(gpu, w, h, n)
+    mbw, mbh = gpu.mb_dims(w, h)
+    dc = gpu.DeviceClip(y, mbw, mbh)
+    dc.pad()
+    dc.hpel(0, n - 1)
+    outs = torch.zeros((n - 1, mbw * mbh * gpu.MB_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+    plan = gpu.Plan(_jobs_for(gpu, dc, outs, 16, 1, True, 28))
+    waves, ms = plan.autotune(reps=2)
+    assert waves >= 1 and ms > 0
+    for grid in (1, 7, 0):
+        plan.set_grid(grid)
+        outs.zero_()
+        plan.launch()
+        plan.check()
+        assert_same_field(_field(gpu, outs, n - 1, mbh, mbw), oracle.search_clip_me(y, 16, 1, True, 28),
+                          f"grid {grid}")
+    plan.close()
+
+
+def test_stall_path_reports_estall(gpu):
+    """With the dependency-wait bound forced to one poll, rows that must wait
+    give up, raise the flag, every wave exits, and the call fails with
+    PCME_ESTALL instead of hanging; the default bound then works again."""
+    import torch
+    say("stall path")
+    lib = gpu.load()
+    w, h, n = 640, 360, 3
+    y, _, _ = _clip(gpu, w, h, n)
+    mbw, mbh = gpu.mb_dims(w, h)
+    dc = gpu.DeviceClip(y, mbw, mbh)
+    dc.pad()
+    dc.hpel(0, n - 1)
+    outs = torch.zeros((n - 1, mbw * mbh * gpu.MB_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+    plan = gpu.Plan(_jobs_for(gpu, dc, outs, 16, 1, True, 28))
+    old = lib.pcme_set_spin_limit(1)
+    try:
+        plan.launch()
+        with pytest.raises(gpu.PcmeError) as e:
+            plan.check()
+        assert e.value.code == gpu.PCME_ESTALL
+    finally:
+        lib.pcme_set_spin_limit(old)
+    plan.launch()
+    plan.check()
+    plan.close()
+
+
+def test_search_frames_4k_me_range_64(gpu, oracle):
+    """3840x2160, me_range 64, QP 27: one P frame against the oracle."""
+    say("4k")
+    y, _, _ = _clip(gpu, 3840, 2160, 2, nrect=8, rect_lo=128, rect_hi=900, vmax=16.0, pan_max=1.0, zoom=0.002)
+    mbw, mbh = gpu.mb_dims(3840, 2160)
+    dc = gpu.DeviceClip(y, mbw, mbh)
+    got = gpu.search_clip_me(dc, 64, 1, True, 27)
+    want = oracle.search_clip_me(y, 64, 1, True, 27)
+    assert_same_field(got, want, "4k")
