@@ -77,6 +77,24 @@ typedef struct pcme_params {
   int32_t qp;              /* lambda = pcme_lambda(qp) */
 } pcme_params;
 
+/* Per-macroblock search hint of the learned pruner (include/pcme_net.h).
+ * No counterpart in the reference.  When `prune` is 1 the macroblock's
+ * search runs pruned: every partition search starts from the rounded
+ * predictor, its two search_part candidates and the hint's candidates
+ * instead of the hexagon descent, and only P16x16 plus the hinted mode are
+ * searched.  When `prune` is 0 the macroblock runs the full search.
+ * 40 bytes. */
+typedef struct pcme_hint {
+  pcme_mv cand[8];         /* ranked candidate MVs, quarter-pel units on
+                              the integer grid */
+  uint16_t conf_part;      /* probability of `part` x 65535 */
+  uint16_t conf_mv;        /* probability mass of the candidates x 65535 */
+  uint8_t part;            /* hinted partition mode (PCME_P*) */
+  uint8_t ncand;           /* candidates used, 0..8 */
+  uint8_t prune;           /* 1: pruned search; 0: full search */
+  uint8_t reserved;
+} pcme_hint;
+
 /* One P frame of the batched ME-only field search.  All pointers (device). */
 typedef struct pcme_frame_job {
   const uint8_t *src;      /* padded source luma of the frame */
@@ -85,6 +103,8 @@ typedef struct pcme_frame_job {
   const uint8_t *ref_v;
   const uint8_t *ref_c;
   pcme_mb_out *out;        /* mbw*mbh records, raster order */
+  const pcme_hint *hint;   /* NULL: full search of every MB; else mbw*mbh
+                              hints, raster order (pruned search) */
   pcme_params prm;
 } pcme_frame_job;
 

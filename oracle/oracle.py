@@ -57,6 +57,12 @@ OMB_DTYPE = np.dtype([
     ("mv16", "<i2", (2,)), ("part", "u1"), ("reserved", "u1", (3,)),
 ])
 
+# ohint has the layout of pcme_hint
+OHINT_DTYPE = np.dtype([
+    ("cand", "<i2", (8, 2)), ("conf_part", "<u2"), ("conf_mv", "<u2"),
+    ("part", "u1"), ("ncand", "u1"), ("prune", "u1"), ("reserved", "u1"),
+])
+
 _lib = None
 
 
@@ -119,6 +125,13 @@ def load() -> ctypes.CDLL:
     lib.o_analyse_mb.restype = None
     lib.o_analyse_mb.argtypes = [c.POINTER(oparams), P, c.c_int, c.c_int, c.POINTER(oref), P, c.c_int,
                                  c.c_int, c.c_uint32, P, c.POINTER(ocounters)]
+    lib.o_search_ex.restype = omv
+    lib.o_search_ex.argtypes = [c.POINTER(oparams), c.POINTER(oref), c.c_int, c.c_int, c.c_int, c.c_int, P,
+                                c.c_int, omv, c.POINTER(omv), c.c_int, c.POINTER(omv), c.c_int, c.c_uint32,
+                                c.POINTER(c.c_uint32), c.POINTER(ocounters)]
+    lib.o_search_frame_ex.restype = None
+    lib.o_search_frame_ex.argtypes = [c.c_int, c.c_int, c.c_int, c.c_int, c.c_int, c.c_uint32, P, c.c_int,
+                                      c.c_int, c.POINTER(oref), P, P, c.POINTER(ocounters)]
     lib.o_search_frame.restype = None
     lib.o_search_frame.argtypes = [c.c_int, c.c_int, c.c_int, c.c_int, c.c_int, c.c_uint32, P, c.c_int,
                                    c.c_int, c.POINTER(oref), P, c.POINTER(ocounters)]
@@ -182,13 +195,20 @@ class Ref:
 
 
 def search_frame(src_padded: np.ndarray, ref: Ref, mbw: int, mbh: int, me_range: int,
-                 subpel_rounds: int, partitions: bool, qp: int, counters: ocounters | None = None):
+                 subpel_rounds: int, partitions: bool, qp: int, counters: ocounters | None = None,
+                 hints: np.ndarray | None = None):
+    """ME-only field of one frame; with `hints` (mbw*mbh OHINT_DTYPE records)
+    the pruned search of the learned pruner."""
     lib = load()
     out = np.zeros(mbw * mbh, OMB_DTYPE)
     s = np.ascontiguousarray(src_padded)
-    lib.o_search_frame(mbw, mbh, me_range, subpel_rounds, 1 if partitions else 0, lib.o_lambda(qp),
-                       _p(s), s.shape[1], LUMA_PAD, ctypes.byref(ref.view), _p(out),
-                       ctypes.byref(counters) if counters is not None else None)
+    h = None
+    if hints is not None:
+        h = np.ascontiguousarray(hints.reshape(-1).view(OHINT_DTYPE))
+        assert h.size == mbw * mbh
+    lib.o_search_frame_ex(mbw, mbh, me_range, subpel_rounds, 1 if partitions else 0, lib.o_lambda(qp),
+                          _p(s), s.shape[1], LUMA_PAD, ctypes.byref(ref.view), _p(h) if h is not None else None,
+                          _p(out), ctypes.byref(counters) if counters is not None else None)
     return out.reshape(mbh, mbw)
 
 

@@ -477,6 +477,15 @@ static const int SQ8[8][2] = {{-1, 0}, {1, 0},  {0, -1}, {0, 1},
 omv o_search(const oparams *p, const oref *r, int bx, int by, int w, int h,
              const uint8_t *src, int sstride, omv mvp, const omv *cands,
              int ncands, uint32_t lam, uint32_t *cost_out, ocounters *ctr) {
+  return o_search_ex(p, r, bx, by, w, h, src, sstride, mvp, cands, ncands,
+                     NULL, -1, lam, cost_out, ctr);
+}
+
+/* slice.rs:1590-1723; with nh >= 0 the hint scan replaces lines 1632-1656 */
+omv o_search_ex(const oparams *p, const oref *r, int bx, int by, int w, int h,
+                const uint8_t *src, int sstride, omv mvp, const omv *cands,
+                int ncands, const omv *hc, int nh, uint32_t lam,
+                uint32_t *cost_out, ocounters *ctr) {
   sctx s = {p, r, bx, by, w, h, src, sstride, mvp, lam, ctr};
   if (ctr) ctr->searches++;
   omv best = mvnew(((int32_t)mvp.x + 2) & ~3, ((int32_t)mvp.y + 2) & ~3);
@@ -492,7 +501,17 @@ omv o_search(const oparams *p, const oref *r, int bx, int by, int w, int h,
       }
     }
   }
-  int range = p->me_range;
+  for (int i = 0; i < nh; i++) {
+    omv c = mvnew(((int32_t)hc[i].x + 2) & ~3, ((int32_t)hc[i].y + 2) & ~3);
+    if (!mveq(c, best) && o_mv_ok(p, bx, by, w, h, c)) {
+      uint32_t k = fcost(&s, c);
+      if (k < bcost) {
+        bcost = k;
+        best = c;
+      }
+    }
+  }
+  int range = nh < 0 ? p->me_range : 0;
   omv centre0 = best;
   static const int HEX[6][2] = {{-2, 0}, {2, 0}, {-1, -2}, {1, -2}, {-1, 2}, {1, 2}};
   for (int it = 0; it < range; it++) {
@@ -581,11 +600,26 @@ static int shape_of(int part, int q) {
   return 0;
 }
 
+static uint32_t search_part_ex(const oparams *p, const ombinfo *mbs, int mx,
+                               int my, const oref *r, int part,
+                               const uint8_t *src16, uint32_t lam, omv seed,
+                               const omv *hc, int nh, omv mvs[4],
+                               omv mvps[4], ocounters *ctr);
+
 /* slice.rs:1810-1868 (list 0) */
 uint32_t o_search_part(const oparams *p, const ombinfo *mbs, int mx, int my,
                        const oref *r, int part, const uint8_t *src16,
                        uint32_t lam, omv seed, omv mvs[4], omv mvps[4],
                        ocounters *ctr) {
+  return search_part_ex(p, mbs, mx, my, r, part, src16, lam, seed, NULL, -1,
+                        mvs, mvps, ctr);
+}
+
+static uint32_t search_part_ex(const oparams *p, const ombinfo *mbs, int mx,
+                               int my, const oref *r, int part,
+                               const uint8_t *src16, uint32_t lam, omv seed,
+                               const omv *hc, int nh, omv mvs[4],
+                               omv mvps[4], ocounters *ctr) {
   int px = mx * 16, py = my * 16;
   ombinfo info;
   memset(&info, 0, sizeof info);
@@ -601,8 +635,8 @@ uint32_t o_search_part(const oparams *p, const ombinfo *mbs, int mx, int my,
                     shape_of(part, q));
     omv cands[2] = {seed, mvs[0]};
     uint32_t cost;
-    omv mv = o_search(p, r, px + x, py + y, w, h, src16 + y * 16 + x, 16, mvp,
-                      cands, 2, lam, &cost, ctr);
+    omv mv = o_search_ex(p, r, px + x, py + y, w, h, src16 + y * 16 + x, 16,
+                         mvp, cands, 2, hc, nh, lam, &cost, ctr);
     mvs[q] = mv;
     mvps[q] = mvp;
     total += cost;
@@ -620,6 +654,19 @@ uint32_t o_search_part(const oparams *p, const ombinfo *mbs, int mx, int my,
 void o_analyse_mb(const oparams *p, ombinfo *mbs, int mx, int my,
                   const oref *r, const uint8_t *src_plane, int sstride,
                   int spad, uint32_t lam, omb_out *out, ocounters *ctr) {
+  o_analyse_mb_ex(p, mbs, mx, my, r, src_plane, sstride, spad, lam, NULL, out,
+                  ctr);
+}
+
+/* o_analyse_mb; a pruned MB (h->prune) passes the hint candidates to every
+ * search and searches only the hinted modes. */
+void o_analyse_mb_ex(const oparams *p, ombinfo *mbs, int mx, int my,
+                     const oref *r, const uint8_t *src_plane, int sstride,
+                     int spad, uint32_t lam, const ohint *h, omb_out *out,
+                     ocounters *ctr) {
+  const int pr = h != NULL && h->prune != 0;
+  const omv *hc = pr ? h->cand : NULL;
+  const int nh = pr ? (h->ncand > 8 ? 8 : (int)h->ncand) : -1;
   uint8_t src16[256];
   int px = mx * 16, py = my * 16;
   for (int rr = 0; rr < 16; rr++)
@@ -627,8 +674,8 @@ void o_analyse_mb(const oparams *p, ombinfo *mbs, int mx, int my,
            src_plane + (size_t)(py + rr + spad) * sstride + px + spad, 16);
   omv skip = o_pskip_mv(p, mbs, mx, my);
   omv mv16[4], mvp16[4];
-  uint32_t cost16 = o_search_part(p, mbs, mx, my, r, OP_16x16, src16, lam,
-                                  skip, mv16, mvp16, ctr);
+  uint32_t cost16 = search_part_ex(p, mbs, mx, my, r, OP_16x16, src16, lam,
+                                   skip, hc, nh, mv16, mvp16, ctr);
   int best_part = OP_16x16;
   omv best_mvs[4], best_mvps[4];
   memcpy(best_mvs, mv16, sizeof mv16);
@@ -639,9 +686,12 @@ void o_analyse_mb(const oparams *p, ombinfo *mbs, int mx, int my,
   if (p->partitions && (uint64_t)cost16 > 256u * 4u) {
     static const int parts[3] = {OP_16x8, OP_8x16, OP_8x8};
     for (int k = 0; k < 3; k++) {
+      if (pr && (h->part == OP_8x8 ? parts[k] != OP_8x8
+                                   : (h->part == OP_16x16 || parts[k] == OP_8x8)))
+        continue;
       omv m[4], pp[4];
-      uint32_t c = o_search_part(p, mbs, mx, my, r, parts[k], src16, lam,
-                                 mv16[0], m, pp, ctr);
+      uint32_t c = search_part_ex(p, mbs, mx, my, r, parts[k], src16, lam,
+                                  mv16[0], hc, nh, m, pp, ctr);
       out->cost[parts[k]] = c;
       uint32_t extra = parts[k] == OP_8x8 ? 10 : 3;
       uint64_t cc = (uint64_t)c + (uint64_t)((lam * extra) >> 8);
@@ -680,13 +730,22 @@ void o_search_frame(int mbw, int mbh, int me_range, int subpel_rounds,
                     int partitions, uint32_t lam, const uint8_t *src,
                     int sstride, int spad, const oref *r, omb_out *out,
                     ocounters *ctr) {
+  o_search_frame_ex(mbw, mbh, me_range, subpel_rounds, partitions, lam, src,
+                    sstride, spad, r, NULL, out, ctr);
+}
+
+void o_search_frame_ex(int mbw, int mbh, int me_range, int subpel_rounds,
+                       int partitions, uint32_t lam, const uint8_t *src,
+                       int sstride, int spad, const oref *r,
+                       const ohint *hints, omb_out *out, ocounters *ctr) {
   ombinfo *mbs = (ombinfo *)calloc((size_t)mbw * mbh, sizeof(ombinfo));
   oparams p = {mbw, mbh, me_range, subpel_rounds, partitions, 0};
   for (int my = 0; my < mbh; my++) {
     p.vlim = o_vlim(my, mbh);
     for (int mx = 0; mx < mbw; mx++)
-      o_analyse_mb(&p, mbs, mx, my, r, src, sstride, spad, lam,
-                   &out[my * mbw + mx], ctr);
+      o_analyse_mb_ex(&p, mbs, mx, my, r, src, sstride, spad, lam,
+                      hints ? &hints[my * mbw + mx] : NULL,
+                      &out[my * mbw + mx], ctr);
   }
   free(mbs);
 }

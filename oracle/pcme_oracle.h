@@ -84,6 +84,15 @@ typedef struct {
   uint8_t pad_[3];
 } omb_out;
 
+/* Learned-pruner hint (layout of pcme_hint, include/pcme.h).  No
+ * counterpart in the reference: the pruned search is this build's own
+ * variant, restated here so its field can be checked bit for bit. */
+typedef struct {
+  omv cand[8];
+  uint16_t conf_part, conf_mv;
+  uint8_t part, ncand, prune, reserved;
+} ohint;
+
 /* ---- dsp.rs ---- */
 uint32_t o_sad(const uint8_t *a, int sa, const uint8_t *b, int sb, int w, int h);
 uint32_t o_sad16(const uint8_t *a, int sa, const uint8_t *b, int sb, int h);
@@ -152,6 +161,29 @@ void o_search_frame(int mbw, int mbh, int me_range, int subpel_rounds,
                     int partitions, uint32_t lam, const uint8_t *src,
                     int sstride, int spad, const oref *r, omb_out *out,
                     ocounters *ctr);
+
+/* ---- pruned search (no reference counterpart) ----
+ * o_search_ex: o_search when nh < 0.  With nh >= 0 the hexagon descent is
+ * replaced by a scan of the hint candidates hc[0..nh) (rounded like the
+ * search_part candidates, skipped when illegal) after the predictor and the
+ * search_part candidates; the square and sub-pel rounds are unchanged. */
+omv o_search_ex(const oparams *p, const oref *r, int bx, int by, int w, int h,
+                const uint8_t *src, int sstride, omv mvp, const omv *cands,
+                int ncands, const omv *hc, int nh, uint32_t lam,
+                uint32_t *cost, ocounters *ctr);
+/* o_analyse_mb with an optional hint: h == NULL or h->prune == 0 is the full
+ * analysis.  A pruned MB searches P16x16 and, when partitions and
+ * cost16 > 1024, the hinted mode only (P16x8 or P8x16 hints search both of
+ * those, P8x8 hints search P8x8), all with o_search_ex(nh = h->ncand). */
+void o_analyse_mb_ex(const oparams *p, ombinfo *mbs, int mx, int my,
+                     const oref *r, const uint8_t *src_plane, int sstride,
+                     int spad, uint32_t lam, const ohint *h, omb_out *out,
+                     ocounters *ctr);
+/* o_search_frame with per-MB hints (mbw*mbh, raster; NULL = none). */
+void o_search_frame_ex(int mbw, int mbh, int me_range, int subpel_rounds,
+                       int partitions, uint32_t lam, const uint8_t *src,
+                       int sstride, int spad, const oref *r,
+                       const ohint *hints, omb_out *out, ocounters *ctr);
 
 /* Multi-frame CPU driver (frame threads, as enc/mod.rs:189-263): for each
  * job, build the reference's half-pel planes then search the frame.  All

@@ -26,6 +26,8 @@ static int fail(int code, const char *fmt, ...) {
   return code;
 }
 
+void set_error(const char *msg) { snprintf(g_err, sizeof g_err, "%s", msg); }
+
 #define HIPCHK(call)                                                                 \
   do {                                                                               \
     hipError_t e_ = (call);                                                          \
@@ -40,7 +42,7 @@ struct Plan;
 struct Dev {
   int id = -1;
   std::mutex mu;
-  int resident_waves = 0;
+  int resident_waves[2] = {0, 0};  // full-search / hinted kernel
   pcme_search_job *dsj = nullptr;
   size_t dsj_cap = 0;
   unsigned char *hstage = nullptr;  // pinned upload staging
@@ -79,10 +81,13 @@ static int get_dev(Dev **out) {
   dv->id = d;
   hipDeviceProp_t p;
   HIPCHK(hipGetDeviceProperties(&p, d));
-  int per_cu = 0;
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, search_frames_kernel, 64, 0));
-  if (per_cu < 1) per_cu = 1;
-  dv->resident_waves = per_cu * p.multiProcessorCount;
+  for (int v = 0; v < 2; v++) {
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, v ? search_frames_kernel<true> : search_frames_kernel<false>, 64, 0));
+    if (per_cu < 1) per_cu = 1;
+    dv->resident_waves[v] = per_cu * p.multiProcessorCount;
+  }
   HIPCHK(hipEventCreateWithFlags(&dv->stage_ev, hipEventDisableTiming));
   g_devs[d] = dv;
   *out = dv;
@@ -161,6 +166,7 @@ struct Plan {
   size_t nwords = 0;
   uint32_t *ctl = nullptr;  // [0] tickets taken, [1] stall flag, [2] epoch, [3] clear request
   int resident = 0;         // resident waves of the search kernel on this device
+  bool hinted = false;      // some job carries pruner hints: the hinted kernel
   int grid = 0;             // waves launched (persistent grid)
 };
 
@@ -174,6 +180,7 @@ static int check_jobs(const pcme_frame_job *jobs, int njobs) {
     if ((((uintptr_t)jobs[i].src | (uintptr_t)jobs[i].ref | (uintptr_t)jobs[i].ref_h | (uintptr_t)jobs[i].ref_v |
           (uintptr_t)jobs[i].ref_c) & 3u) != 0)
       return fail(PCME_EINVAL, "job %d: plane pointers must be 4-byte aligned", i);
+    if (((uintptr_t)jobs[i].hint & 3u) != 0) return fail(PCME_EINVAL, "job %d: hints must be 4-byte aligned", i);
   }
   return PCME_OK;
 }
@@ -191,7 +198,8 @@ static int plan_create(Dev *dv, const pcme_frame_job *jobs, int njobs, Plan **ou
   Plan *p = new Plan();
   p->dev = dv->id;
   p->njobs = njobs;
-  p->resident = dv->resident_waves;
+  for (int i = 0; i < njobs; i++) p->hinted |= jobs[i].hint != nullptr;
+  p->resident = dv->resident_waves[p->hinted ? 1 : 0];
   std::vector<DevJob> dj((size_t)njobs);
   size_t words = 0;
   int maxh = 0;
@@ -201,6 +209,7 @@ static int plan_create(Dev *dv, const pcme_frame_job *jobs, int njobs, Plan **ou
     d.src = J.src;
     d.pl[0] = J.ref; d.pl[1] = J.ref_h; d.pl[2] = J.ref_v; d.pl[3] = J.ref_c;
     d.out = J.out;
+    d.hint = J.hint;
     d.xchg = nullptr;
     d.mbw = J.prm.mbw;
     d.mbh = J.prm.mbh;
@@ -250,7 +259,8 @@ static int plan_launch(Plan *p, hipStream_t stream) {
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(clear_kernel, dim3(512), dim3(256), 0, stream, p->ctl, p->xchg, p->nwords);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(search_frames_kernel, dim3((unsigned)p->grid), dim3(64), 0, stream, p->djobs, p->dtickets,
+  hipLaunchKernelGGL(p->hinted ? search_frames_kernel<true> : search_frames_kernel<false>, dim3((unsigned)p->grid),
+                     dim3(64), 0, stream, p->djobs, p->dtickets,
                      p->ntickets, p->ctl, kSpinLimit);
   HIPCHK(hipGetLastError());
   return PCME_OK;

@@ -17,6 +17,7 @@ struct DevJob {
   const uint8_t *src;    // padded source luma
   const uint8_t *pl[4];  // reference planes: full, H, V, C
   pcme_mb_out *out;      // mbw*mbh records
+  const pcme_hint *hint; // nullptr: full search; else mbw*mbh pruner hints
   uint32_t *xchg;        // mbw*mbh*4 exchange words (bottom-row MVs)
   int mbw, mbh, stride;
   int me_range, subpel_rounds, partitions;
@@ -25,6 +26,7 @@ struct DevJob {
 };
 
 #ifdef __HIP__
+template <bool HINTS>
 __global__ void search_frames_kernel(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
                                      int ntickets, uint32_t *ctl, uint32_t spin_limit);
 __global__ void search_batch_kernel(const pcme_search_job *__restrict__ jobs, int njobs,
@@ -44,6 +46,9 @@ __global__ void pred_frame_kernel(const uint8_t *__restrict__ pl0, const uint8_t
 __global__ void prep_kernel(uint32_t *ctl);
 __global__ void clear_kernel(const uint32_t *ctl, uint32_t *xchg, size_t nwords);
 #endif
+
+// Message returned by pcme_last_error() on this thread (host side).
+void set_error(const char *msg);
 
 // Tile of the half-pel kernel.
 #define HT_W 64

@@ -389,12 +389,26 @@ __device__ __forceinline__ void ensure_sp(const SCtx &c, int bx, int by, int bxv
   __syncthreads();
 }
 
+// Hint candidate i (lane-varying) of the packed list hc (global memory,
+// wave-uniform pointer), rounded like the search_part candidates
+// (slice.rs:1622); lanes with i outside [0, nh) read nothing.
+__device__ __forceinline__ void hint_cand(const uint32_t *hc, int i, int nh, int &x, int &y) {
+  uint32_t v = 0u;
+  if ((i >= 0) & (i < nh)) v = hc[i];
+  x = ((int)(int16_t)(v & 0xFFFFu) + 2) & ~3;
+  y = (((int)v >> 16) + 2) & ~3;
+}
+
 // SliceEnc::search (slice.rs:1590-1723) for a W x H block at picture
 // position (bx, by).  mvp, cands are wave-uniform; returns (best, cost).
-// lds = this wave's LDS_PER_WAVE bytes.
+// lds = this wave's LDS_PER_WAVE bytes.  nh >= 0 selects the pruned search:
+// the hint candidates hc[0..nh) are scanned after the search_part
+// candidates (slots 3-7, then slots 0-2 of a second pass) and the hexagon
+// descent is skipped.
 template <int W, int H>
 __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const Lane &L, int bx, int by, int mvpx,
-                                       int mvpy, int c0x, int c0y, int c1x, int c1y, int ncand, uint32_t *lds) {
+                                       int mvpy, int c0x, int c0y, int c1x, int c1y, int ncand, const uint32_t *hc,
+                                       int nh, uint32_t *lds) {
   uint32_t *lw = lds;
   uint32_t *lsp = lds + WIN_BYTES / 4;
   const Box ok = legal_box(c, bx, by, W, H);
@@ -419,6 +433,21 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
     bxv = w == 1 ? r0x : (w == 2 ? r1x : bxv);
     byv = w == 1 ? r0y : (w == 2 ? r1y : byv);
   }
+  if (nh > 0) {
+    // pruned search: the hint candidates continue the scan (one per slot)
+    int hx, hy;
+    hint_cand(hc, L.slot, nh, hx, hy);
+    const bool valid = (L.slot < nh) & in_box(ok, hx, hy);
+    const uint32_t k = sad_cost<false>(c, L, s, bx, by, hx, hy, valid ? hx : bxv, valid ? hy : byv, mvpx, mvpy,
+                                       lw, 0, 0);
+    const uint32_t key = min_key(k, valid, L.slot);
+    if ((key >> 3) < bcost) {
+      bcost = key >> 3;
+      const int w = (int)(key & 7u);
+      bxv = rdlane((uint32_t)hx, 8 * w);
+      byv = rdlane((uint32_t)hy, 8 * w);
+    }
+  }
   // slice.rs:1632-1656: hexagon descent clamped to +-range around centre0.
   int wx0 = 1 << 20, wy0 = 1 << 20;  // nothing staged yet
   const int c0xx = bxv, c0yy = byv;
@@ -431,7 +460,7 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
   hb.ylo = max(hb.ylo, c0yy - 4 * range - 3);
   hb.yhi = min(hb.yhi, c0yy + 4 * range + 3);
   if (L.slot >= 6) hb.xlo = 1 << 20;  // slots 6, 7 carry no hex point
-  for (int it = 0; it < range; it++) {
+  for (int it = 0; it < (nh < 0 ? range : 0); it++) {
     ensure_win<W, H>(c, bx, by, bxv, byv, 2, wx0, wy0, lw);
     const int cx = bxv + hdx, cy = byv + hdy;
     const bool valid = in_box(hb, cx, cy);
@@ -538,9 +567,11 @@ __device__ __forceinline__ bool take_key(PState &S, uint32_t key, int dx, int dy
 
 // search (slice.rs:1590-1723) for A = 16x8 partition and B = 8x16
 // partition at the same time.  src* = block top-left of each source.
+// nh >= 0: pruned search with hint candidates hc[0..nh) (see search()).
 __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B, const uint8_t *srcA,
                                             const uint8_t *srcB, int sstride, int c0x, int c0y, int c1ax,
-                                            int c1ay, int c1bx, int c1by, uint32_t *lds, SRes &ra, SRes &rb) {
+                                            int c1ay, int c1bx, int c1by, const uint32_t *hc, int nh,
+                                            uint32_t *lds, SRes &ra, SRes &rb) {
   const uint32_t l = lane_id();
   const bool g = l >= 32;
   Lane L;
@@ -585,6 +616,30 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
     B.x = wb == 1 ? r0x : (wb == 2 ? r1bx : b0x);
     B.y = wb == 1 ? r0y : (wb == 2 ? r1by : b0y);
   }
+  if (nh > 0) {
+    // pruned search: the hint candidates continue each scan (one per slot)
+    int hx, hy;
+    hint_cand(hc, L.slot, nh, hx, hy);
+    const Box ok = g ? B.ok : A.ok;
+    const int ox = gsel(g, A.x, B.x), oy = gsel(g, A.y, B.y);
+    const bool valid = (L.slot < nh) & in_box(ok, hx, hy);
+    const uint32_t k = sad_cost<false, 4>(c, L, s, bx, by, hx, hy, valid ? hx : ox, valid ? hy : oy, mvpx, mvpy,
+                                          lw, 0, 0);
+    uint32_t ka, kb;
+    min_key_pair(k, valid, L.slot, ka, kb);
+    if ((ka >> 3) < A.cost) {
+      const int wa = (int)(ka & 7u);
+      A.cost = ka >> 3;
+      A.x = (int)rdlane((uint32_t)hx, 4 * wa);
+      A.y = (int)rdlane((uint32_t)hy, 4 * wa);
+    }
+    if ((kb >> 3) < B.cost) {
+      const int wb = (int)(kb & 7u);
+      B.cost = kb >> 3;
+      B.x = (int)rdlane((uint32_t)hx, 32 + 4 * wb);
+      B.y = (int)rdlane((uint32_t)hy, 32 + 4 * wb);
+    }
+  }
   const int range = c.range;
   A.wx0 = A.wy0 = B.wx0 = B.wy0 = 1 << 20;
   A.hb = A.ok;
@@ -599,7 +654,7 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
   B.hb.yhi = min(B.hb.yhi, B.y + 4 * range + 3);
   // slice.rs:1632-1656: hexagon descent of both.
   const int hdx = hex_dx(L.slot) * 4, hdy = hex_dy(L.slot) * 4;
-  A.act = B.act = range > 0;
+  A.act = B.act = range > 0 && nh < 0;
   A.it = B.it = 0;
   while (A.act | B.act) {
     if (A.act) ensure_win<16, 8>(c, A.bx, A.by, A.x, A.y, 2, A.wx0, A.wy0, lds);
@@ -808,8 +863,8 @@ struct PartRes {
 // One partition of search_part (slice.rs:1826-1866), then the next.
 template <int PART, int P>
 __device__ __forceinline__ void part_step(const SCtx &c, const Nbr &n, Cur &cur, const uint8_t *src,
-                                          int sstride, int mbx, int mby, uint32_t seed, PartRes &res,
-                                          uint32_t *lds) {
+                                          int sstride, int mbx, int mby, uint32_t seed, const uint32_t *hc,
+                                          int nh, PartRes &res, uint32_t *lds) {
   if constexpr (P < part_count(PART)) {
     constexpr int X = rect_x(PART, P), Y = rect_y(PART, P);
     constexpr int W = rect_w(PART), H = rect_h(PART);
@@ -819,32 +874,33 @@ __device__ __forceinline__ void part_step(const SCtx &c, const Nbr &n, Cur &cur,
     load_src(src + Y * sstride + X, sstride, L, s);
     const uint32_t m0 = P == 0 ? 0u : res.mv[0];
     SRes r = search<W, H>(c, s, L, mbx + X, mby + Y, mv_x(pv), mv_y(pv), mv_x(seed), mv_y(seed), mv_x(m0),
-                          mv_y(m0), 2, lds);
+                          mv_y(m0), 2, hc, nh, lds);
     const uint32_t m = pack_mv(r.x, r.y);
     res.mv[P] = m;
     res.mvp[P] = pv;
     res.total += r.cost;
     cur.mv[P] = m;
-    part_step<PART, P + 1>(c, n, cur, src, sstride, mbx, mby, seed, res, lds);
+    part_step<PART, P + 1>(c, n, cur, src, sstride, mbx, mby, seed, hc, nh, res, lds);
   }
 }
 
 // search_part (slice.rs:1810-1868), list 0.  src = MB top-left.
 template <int PART>
 __device__ __forceinline__ void search_part(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
-                                            int mbx, int mby, uint32_t seed, PartRes &res, uint32_t *lds) {
+                                            int mbx, int mby, uint32_t seed, const uint32_t *hc, int nh,
+                                            PartRes &res, uint32_t *lds) {
   Cur cur = {};
 #pragma unroll
   for (int i = 0; i < 4; i++) { res.mv[i] = res.mvp[i] = 0u; }
   res.total = 0;
-  part_step<PART, 0>(c, n, cur, src, sstride, mbx, mby, seed, res, lds);
+  part_step<PART, 0>(c, n, cur, src, sstride, mbx, mby, seed, hc, nh, res, lds);
 }
 
 // search_part (slice.rs:1810-1868) of P16x8 (ra) and P8x16 (rb) together:
 // partition p of both modes in one paired search.
 __device__ __forceinline__ void search_part_12(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
-                                               int mbx, int mby, uint32_t seed, PartRes &ra, PartRes &rb,
-                                               uint32_t *lds) {
+                                               int mbx, int mby, uint32_t seed, const uint32_t *hc, int nh,
+                                               PartRes &ra, PartRes &rb, uint32_t *lds) {
   Cur ca = {}, cb = {};
 #pragma unroll
   for (int i = 0; i < 4; i++) { ra.mv[i] = ra.mvp[i] = rb.mv[i] = rb.mvp[i] = 0u; }
@@ -854,7 +910,7 @@ __device__ __forceinline__ void search_part_12(const SCtx &c, const Nbr &n, cons
   uint32_t pa = mvp<1, 0, 0, 0, 4, 1>(n, ca), pb = mvp<2, 0, 0, 0, 2, 3>(n, cb);
   A.bx = mbx; A.by = mby; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
   B.bx = mbx; B.by = mby; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
-  search_pair(c, A, B, src, src, sstride, mv_x(seed), mv_y(seed), 0, 0, 0, 0, lds, xa, xb);
+  search_pair(c, A, B, src, src, sstride, mv_x(seed), mv_y(seed), 0, 0, 0, 0, hc, nh, lds, xa, xb);
   ra.mv[0] = ca.mv[0] = pack_mv(xa.x, xa.y); ra.mvp[0] = pa; ra.total = xa.cost;
   rb.mv[0] = cb.mv[0] = pack_mv(xb.x, xb.y); rb.mvp[0] = pb; rb.total = xb.cost;
   // partition 1: 16x8 bottom (y 2 blocks), 8x16 right (x 2 blocks)
@@ -863,7 +919,7 @@ __device__ __forceinline__ void search_part_12(const SCtx &c, const Nbr &n, cons
   A.bx = mbx; A.by = mby + 8; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
   B.bx = mbx + 8; B.by = mby; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
   search_pair(c, A, B, src + 8 * sstride, src + 8, sstride, mv_x(seed), mv_y(seed), mv_x(ra.mv[0]),
-              mv_y(ra.mv[0]), mv_x(rb.mv[0]), mv_y(rb.mv[0]), lds, xa, xb);
+              mv_y(ra.mv[0]), mv_x(rb.mv[0]), mv_y(rb.mv[0]), hc, nh, lds, xa, xb);
   ra.mv[1] = pack_mv(xa.x, xa.y); ra.mvp[1] = pa; ra.total += xa.cost;
   rb.mv[1] = pack_mv(xb.x, xb.y); rb.mvp[1] = pb; rb.total += xb.cost;
 }
@@ -891,12 +947,16 @@ struct MbRes {
 };
 
 // ME-only analysis of one MB (slice.rs:1884-1945 without early skip,
-// intra and RD).
+// intra and RD).  nh >= 0: pruned MB (hint candidates hc[0..nh), hinted
+// mode hpart): P16x16, then only P16x8 + P8x16 (hpart 1 or 2) or only P8x8
+// (hpart 3), with the same strict-< choice.
 __device__ __forceinline__ void analyse_mb(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
-                                           int mbx, int mby, int partitions, MbRes &o, uint32_t *lds) {
+                                           int mbx, int mby, int partitions, const uint32_t *hc, int nh,
+                                           int hpart, MbRes &o, uint32_t *lds) {
   const uint32_t skip = pskip(n);
+  const bool pr = nh >= 0;
   PartRes r16;
-  search_part<0>(c, n, src, sstride, mbx, mby, skip, r16, lds);
+  search_part<0>(c, n, src, sstride, mbx, mby, skip, hc, nh, r16, lds);
   const uint32_t cost16 = r16.total;
   o.part = 0;
   take_if(o.best, r16, true);
@@ -905,9 +965,10 @@ __device__ __forceinline__ void analyse_mb(const SCtx &c, const Nbr &n, const ui
   o.cost[1] = o.cost[2] = o.cost[3] = 0xFFFFFFFFu;
   o.skip = skip;
   o.m16 = r16.mv[0];
-  if (partitions && cost16 > 1024u) {
+  if (partitions && cost16 > 1024u && (!pr || hpart != 0)) {
     PartRes r, r2;
-    search_part_12(c, n, src, sstride, mbx, mby, o.m16, r, r2, lds);
+    if (!pr || hpart == 1 || hpart == 2) {
+    search_part_12(c, n, src, sstride, mbx, mby, o.m16, hc, nh, r, r2, lds);
     o.cost[1] = r.total;
     uint32_t cc = r.total + ((c.lam * 3u) >> 8);
     bool t = cc < o.best_cost;
@@ -920,13 +981,16 @@ __device__ __forceinline__ void analyse_mb(const SCtx &c, const Nbr &n, const ui
     o.best_cost = t ? cc : o.best_cost;
     o.part = t ? 2 : o.part;
     take_if(o.best, r2, t);
-    search_part<3>(c, n, src, sstride, mbx, mby, o.m16, r, lds);
+    }
+    if (!pr || hpart == 3) {
+    search_part<3>(c, n, src, sstride, mbx, mby, o.m16, hc, nh, r, lds);
     o.cost[3] = r.total;
-    cc = r.total + ((c.lam * 10u) >> 8);
-    t = cc < o.best_cost;
+    const uint32_t cc = r.total + ((c.lam * 10u) >> 8);
+    const bool t = cc < o.best_cost;
     o.best_cost = t ? cc : o.best_cost;
     o.part = t ? 3 : o.part;
     take_if(o.best, r, t);
+    }
   }
 }
 
@@ -998,6 +1062,7 @@ __device__ __forceinline__ bool fetch_words(const uint32_t *xw, uint32_t tag, ui
   return true;
 }
 
+template <bool HINTS>
 __global__ void __launch_bounds__(64) search_frames_kernel(const DevJob *__restrict__ jobs,
                                                            const int2 *__restrict__ tickets, int ntickets,
                                                            uint32_t *ctl, uint32_t spin_limit) {
@@ -1062,8 +1127,19 @@ __global__ void __launch_bounds__(64) search_frames_kernel(const DevJob *__restr
       n.aTR = my > 0 && mx + 1 < mbw;
       n.TR = nxt[0];
       const uint8_t *src = J.src + (size_t)(16 * my + PCME_PAD) * stride + 16 * mx + PCME_PAD;
+      // Pruner hint of this MB (wave-uniform); a full search without one.
+      const uint32_t *hc = nullptr;
+      int nh = -1, hpart = 0;
+      if (HINTS && J.hint) {
+        hc = (const uint32_t *)(J.hint + (size_t)my * mbw + mx);  // cand[0..8) = words 0-7
+        const uint32_t w9 = hc[9];  // part | ncand << 8 | prune << 16
+        if ((w9 >> 16) & 0xFFu) {
+          nh = min((int)((w9 >> 8) & 0xFFu), 8);
+          hpart = (int)(w9 & 3u);
+        }
+      }
       MbRes o;
-      analyse_mb(c, n, src, stride, 16 * mx, 16 * my, J.partitions, o, s_lds);
+      analyse_mb(c, n, src, stride, 16 * mx, 16 * my, J.partitions, hc, nh, hpart, o, s_lds);
       store_out(J.out + (size_t)my * mbw + mx, o);
       // Publish the bottom row (blocks 12..15) for the row below.
       if (l < 4) {
@@ -1077,6 +1153,13 @@ __global__ void __launch_bounds__(64) search_frames_kernel(const DevJob *__restr
     }
   }
 }
+
+// The full-search variant carries no hint code (fewer registers, higher
+// occupancy); the hinted variant runs jobs with and without hints.
+template __global__ void search_frames_kernel<false>(const DevJob *__restrict__, const int2 *__restrict__, int,
+                                                     uint32_t *, uint32_t);
+template __global__ void search_frames_kernel<true>(const DevJob *__restrict__, const int2 *__restrict__, int,
+                                                    uint32_t *, uint32_t);
 
 // -------------------------------------------------- per-call search batch
 
@@ -1100,18 +1183,19 @@ __global__ void __launch_bounds__(64) search_batch_kernel(const pcme_search_job 
   int c1x = J.ncands > 1 ? J.cands[1].x : 0, c1y = J.ncands > 1 ? J.cands[1].y : 0;
   SRes r;
   r.x = r.y = 0; r.cost = 0;
+  const uint32_t *hc = nullptr;  // full search: no hints
   if (J.w == 16 && J.h == 16) {
     Lane L = lane_layout<16, 16>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
-    r = search<16, 16>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, s_lds);
+    r = search<16, 16>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hc, -1, s_lds);
   } else if (J.w == 16 && J.h == 8) {
     Lane L = lane_layout<16, 8>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
-    r = search<16, 8>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, s_lds);
+    r = search<16, 8>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hc, -1, s_lds);
   } else if (J.w == 8 && J.h == 16) {
     Lane L = lane_layout<8, 16>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
-    r = search<8, 16>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, s_lds);
+    r = search<8, 16>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hc, -1, s_lds);
   } else {
     Lane L = lane_layout<8, 8>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
-    r = search<8, 8>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, s_lds);
+    r = search<8, 8>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hc, -1, s_lds);
   }
   if (lane_id() == 0) {
     pcme_mv m;

@@ -56,7 +56,7 @@ class FrameJob(ctypes.Structure):
     _fields_ = [("src", ctypes.c_void_p), ("ref", ctypes.c_void_p),
                 ("ref_h", ctypes.c_void_p), ("ref_v", ctypes.c_void_p),
                 ("ref_c", ctypes.c_void_p), ("out", ctypes.c_void_p),
-                ("prm", Params)]
+                ("hint", ctypes.c_void_p), ("prm", Params)]
 
 
 class SearchJob(ctypes.Structure):
@@ -88,6 +88,13 @@ MB_DTYPE = np.dtype([
 ])
 assert MB_DTYPE.itemsize == 128
 
+# pcme_hint as a numpy record (40 bytes)
+HINT_DTYPE = np.dtype([
+    ("cand", "<i2", (8, 2)), ("conf_part", "<u2"), ("conf_mv", "<u2"),
+    ("part", "u1"), ("ncand", "u1"), ("prune", "u1"), ("reserved", "u1"),
+])
+assert HINT_DTYPE.itemsize == 40
+
 # Every symbol include/pcme.h and include/pcme_synth.h declare.
 EXPORTS = (
     "pcme_version", "pcme_last_error", "pcme_device_count", "pcme_lambda",
@@ -95,6 +102,8 @@ EXPORTS = (
     "pcme_search_frames_sync", "pcme_search_batch", "pcme_pred_frame",
     "pcme_synth_clip", "pcme_plan_create", "pcme_plan_launch", "pcme_plan_check",
     "pcme_plan_set_grid", "pcme_plan_autotune", "pcme_plan_destroy", "pcme_set_spin_limit",
+    "pcme_net_create", "pcme_net_destroy", "pcme_net_get_params", "pcme_net_set_params",
+    "pcme_net_inputs", "pcme_net_forward", "pcme_net_grad", "pcme_net_train_step", "pcme_net_hints",
 )
 
 _lib = None
@@ -278,10 +287,10 @@ class DeviceClip:
 
 
 def frame_job(clip: DeviceClip, cur: int, ref: int, out_ptr: int, me_range: int, subpel_rounds: int,
-              partitions: bool, qp: int, ref_clip: DeviceClip | None = None) -> FrameJob:
+              partitions: bool, qp: int, ref_clip: DeviceClip | None = None, hint_ptr: int | None = None) -> FrameJob:
     rc = clip if ref_clip is None else ref_clip
     return FrameJob(clip.plane_ptr(0, cur), rc.plane_ptr(0, ref), rc.plane_ptr(1, ref),
-                    rc.plane_ptr(2, ref), rc.plane_ptr(3, ref), out_ptr,
+                    rc.plane_ptr(2, ref), rc.plane_ptr(3, ref), out_ptr, hint_ptr,
                     Params(clip.mbw, clip.mbh, me_range, subpel_rounds, 1 if partitions else 0, qp))
 
 

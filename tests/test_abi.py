@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     names = set()
-    for hdr in ("pcme.h", "pcme_synth.h"):
+    for hdr in ("pcme.h", "pcme_synth.h", "pcme_net.h"):
         text = open(os.path.join(ROOT, "include", hdr)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"\b(pcme_\w+)\s*\(", text):
@@ -115,3 +115,13 @@ def test_search_frames_rejects_bad_params(pcme_lib):
     arr = (pcme_lib.FrameJob * 1)(job)
     rc = lib.pcme_search_frames(arr, 1, None)
     assert rc in (pcme_lib.PCME_EINVAL, pcme_lib.PCME_ENODEV)
+
+
+def test_pruner_without_device_fails_loudly(pcme_lib):
+    """pcme_net_create reports PCME_ENODEV where no gfx950 device is visible."""
+    if pcme_lib.device_count() > 0:
+        pytest.skip("a device is visible")
+    import pcme.pruner as P
+    with pytest.raises(pcme_lib.PcmeError) as e:
+        P.Net(seed=1, max_batch=64)
+    assert e.value.code == pcme_lib.PCME_ENODEV

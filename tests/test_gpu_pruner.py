@@ -1,0 +1,202 @@
+"""GPU tests of the learned search pruner (include/pcme_net.h): its HIP
+training and inference kernels against the numpy oracle, its hints against
+its own forward pass, and the pruned search against the C oracle's
+restatement of it, bit for bit.
+
+Each case prints what it runs before the GPU step so a failing run names
+its case."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from _util import assert_same_field
+from oracle import cnn_oracle as C
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _dclip(pcme, w, h, n, seed, **kw):
+    spec = pcme.ClipSpec(w, h, n, seed=seed, **kw)
+    y, _, _ = pcme.synth_clip(spec)
+    mbw, mbh = pcme.mb_dims(w, h)
+    dc = pcme.DeviceClip(y, mbw, mbh)
+    dc.pad()
+    dc.hpel(0, n - 1)
+    return y, dc
+
+
+def _bf16_bits(v: np.ndarray) -> np.ndarray:
+    return (np.asarray(v, np.float32).view(np.uint32) >> 16).astype(np.uint16)
+
+
+def _inputs_all(P, dc, t):
+    import torch
+    nmb = dc.mbw * dc.mbh
+    x = torch.empty((nmb, 1152), dtype=torch.int16, device="cuda")
+    mbs = torch.arange(nmb, dtype=torch.int32, device="cuda")
+    P.inputs(dc, t, t - 1, mbs, x)
+    torch.cuda.synchronize()
+    return x
+
+
+def test_inputs_kernel_bit_exact(gpu):
+    import pcme.pruner as P
+    say("net inputs 100x70")
+    _, dc = _dclip(gpu, 100, 70, 2, 0x5EED0200, nrect=3, rect_lo=16, rect_hi=40)
+    x = _inputs_all(P, dc, 1).cpu().numpy().view(np.uint16).reshape(-1, 24, 24, 2)
+    planes = dc.planes[0].cpu().numpy()
+    want = C.inputs(planes[1], planes[0], dc.mbw, dc.mbh, range(dc.mbw * dc.mbh))
+    np.testing.assert_array_equal(x, _bf16_bits(want))
+
+
+def _net(P, seed=11):
+    return P.Net(seed=seed, max_batch=256)
+
+
+def test_forward_matches_oracle(gpu):
+    import torch
+    import pcme.pruner as P
+    say("net forward 64 MBs")
+    _, dc = _dclip(gpu, 128, 64, 2, 0x5EED0201, nrect=3, rect_lo=16, rect_hi=40)
+    net = _net(P)
+    x = _inputs_all(P, dc, 1)[:32 * ((dc.mbw * dc.mbh) // 32)]
+    n = x.shape[0]
+    logits = torch.empty((n, 293), dtype=torch.float32, device="cuda")
+    net.forward(x, n, logits)
+    torch.cuda.synchronize()
+    got = logits.cpu().numpy().astype(np.float64)
+    xs = x.cpu().numpy().view(np.uint16).astype(np.uint32) << 16
+    xin = xs.view(np.float32).astype(np.float64).reshape(n, 24, 24, 2)
+    want = C.forward(xin, C.unpack(net.get_params()), emulate=True)["logits"]
+    err = np.linalg.norm(got - want) / np.linalg.norm(want)
+    # bf16 activations: fp32 vs fp64 accumulation may flip a rounding
+    assert err < 1e-2, err
+    assert np.mean(np.argmax(got[:, :289], 1) == np.argmax(want[:, :289], 1)) > 0.9
+
+
+def test_gradient_matches_oracle(gpu):
+    """The HIP backward pass (bf16 MFMA products, fp32 accumulation) against
+    the oracle's float64 backpropagation of the same bf16-rounded forward."""
+    import torch
+    import pcme.pruner as P
+    say("net gradient 64 MBs")
+    _, dc = _dclip(gpu, 128, 64, 2, 0x5EED0202, nrect=3, rect_lo=16, rect_hi=40)
+    net = _net(P, seed=12)
+    x = _inputs_all(P, dc, 1)
+    n = 32 * (x.shape[0] // 32)
+    rng = np.random.default_rng(3)
+    mv = torch.as_tensor(rng.integers(0, 289, x.shape[0]).astype(np.int32), device="cuda")
+    part = torch.as_tensor(rng.integers(0, 4, x.shape[0]).astype(np.int32), device="cuda")
+    idx = torch.as_tensor(rng.permutation(x.shape[0])[:n].astype(np.int32), device="cuda")
+    loss = torch.zeros(3, dtype=torch.float32, device="cuda")
+    grad = torch.zeros(P.NPARAM, dtype=torch.float32, device="cuda")
+    net.grad(x, mv, part, idx, n, loss, grad)
+    torch.cuda.synchronize()
+    g = grad.cpu().numpy().astype(np.float64)
+    ii = idx.cpu().numpy()
+    xs = x.cpu().numpy().view(np.uint16)[ii].astype(np.uint32) << 16
+    xin = xs.view(np.float32).astype(np.float64).reshape(n, 24, 24, 2)
+    lt, lm, lp, want = C.loss_and_grad(xin, C.unpack(net.get_params()), mv.cpu().numpy()[ii],
+                                       part.cpu().numpy()[ii], emulate=True)
+    lv = loss.cpu().numpy()
+    assert abs(lv[0] - lt) < 1e-3 * abs(lt) and abs(lv[1] - lm) < 1e-3 * abs(lm)
+    gu, wu = C.unpack(g), C.unpack(want)
+    for name, _ in C.LAYOUT:
+        a, b = gu[name].ravel(), wu[name].ravel()
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        cos = float(a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30))
+        assert rel < 3e-2 and cos > 0.999, (name, rel, cos)
+
+
+def test_training_reduces_loss_and_is_deterministic(gpu):
+    import torch
+    import pcme.pruner as P
+    say("net training 320x240 x 5 frames, 2 nets")
+    _, dc = _dclip(gpu, 320, 240, 6, 0x5EED0203, nrect=4, rect_lo=24, rect_hi=80)
+    data, _ = P.label_clip(dc, 16, 28)
+    hs, ps = [], []
+    for _ in range(2):
+        net = P.Net(seed=21, max_batch=256)
+        hs.append(P.train(net, data, epochs=8, batch=256, lr=3e-3, seed=5))
+        ps.append(net.get_params())
+        net.close()
+    first, last = hs[0][0][1], hs[0][-1][1]
+    assert last < 0.7 * first, hs[0]
+    np.testing.assert_array_equal(ps[0], ps[1])
+
+
+def _topk(z, k):
+    order = np.lexsort((np.arange(z.size), -z))  # logit desc, class asc
+    return order[:k]
+
+
+def test_hints_match_forward_logits(gpu):
+    """The fused inference kernel's logits equal pcme_net_forward's bit for
+    bit, so its top-k classes and partition argmax are the same."""
+    import torch
+    import pcme
+    import pcme.pruner as P
+    say("net hints vs forward 192x96")
+    _, dc = _dclip(gpu, 192, 96, 2, 0x5EED0204, nrect=3, rect_lo=16, rect_hi=48)
+    net = _net(P, seed=13)
+    nmb = dc.mbw * dc.mbh  # 72
+    x = _inputs_all(P, dc, 1)
+    n = 32 * ((nmb + 31) // 32)
+    xp = torch.cat([x, x[: n - nmb]]) if n > nmb else x
+    logits = torch.empty((n, 293), dtype=torch.float32, device="cuda")
+    net.forward(xp, n, logits)
+    hints = torch.zeros(nmb * pcme.HINT_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    out = torch.empty(nmb * pcme.MB_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    job = pcme.frame_job(dc, 1, 0, out.data_ptr(), 16, 1, True, 28, hint_ptr=hints.data_ptr())
+    net.hints([job], ncand=8, thr_part=0.5, thr_mv=0.3)
+    torch.cuda.synchronize()
+    z = logits.cpu().numpy()[:nmb].astype(np.float64)
+    h = hints.cpu().numpy().view(pcme.HINT_DTYPE)
+    for i in range(nmb):
+        top = _topk(z[i, :289], 8)
+        np.testing.assert_array_equal(h["cand"][i], C.class_mv(top), err_msg=f"mb {i}")
+        pz = z[i, 289:]
+        assert h["part"][i] == int(np.argmax(pz))
+        pp = np.exp(pz - pz.max())
+        cp = pp.max() / pp.sum()
+        e = np.exp(z[i, :289] - z[i, :289].max())
+        cm = e[top].sum() / e.sum()
+        assert abs(h["conf_part"][i] / 65535 - cp) < 2e-4
+        assert abs(h["conf_mv"][i] / 65535 - cm) < 2e-4
+        if abs(cp - 0.5) > 1e-4 and abs(cm - 0.3) > 1e-4:
+            assert h["prune"][i] == int(cp >= 0.5 and cm >= 0.3)
+        assert h["ncand"][i] == 8
+
+
+@pytest.mark.parametrize("ncand,thr", [(8, 0.0), (3, 0.0), (0, 0.0), (8, 0.45), (5, 0.3)])
+def test_pruned_search_matches_oracle(gpu, oracle, ncand, thr):
+    """GPU pruned field == the oracle's pruned search on the same hints."""
+    import pcme.pruner as P
+    say("pruned search 320x240 x 3 frames", ncand, thr)
+    y, dc = _dclip(gpu, 320, 240, 4, 0x5EED0205, nrect=4, rect_lo=24, rect_hi=80)
+    net = _net(P, seed=14)
+    field, hints = P.search_clip(dc, 16, 28, net=net, ncand=ncand, thr_part=thr, thr_mv=thr, prepared=True)
+    mbw, mbh = dc.mbw, dc.mbh
+    for t in range(1, 4):
+        src = oracle.pad_luma(y[t], mbw, mbh)
+        ref = oracle.Ref(oracle.pad_luma(y[t - 1], mbw, mbh))
+        want = oracle.search_frame(src, ref, mbw, mbh, 16, 1, True, 28, hints=hints[t - 1].reshape(-1))
+        assert_same_field(field[t - 1], want, f"frame {t}")
+    if thr == 0.0:
+        assert np.all(hints["prune"] == 1)
+
+
+def test_pruned_search_without_confidence_is_the_full_search(gpu):
+    import pcme.pruner as P
+    say("pruned search, thresholds above 1")
+    _, dc = _dclip(gpu, 320, 240, 3, 0x5EED0206, nrect=4, rect_lo=24, rect_hi=80)
+    net = _net(P, seed=15)
+    plain, _ = P.search_clip(dc, 16, 28, prepared=True)
+    field, hints = P.search_clip(dc, 16, 28, net=net, thr_part=1.5, thr_mv=1.5, prepared=True)
+    assert np.all(hints["prune"] == 0)
+    np.testing.assert_array_equal(field, plain)
