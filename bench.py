@@ -37,6 +37,11 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 # Algorithmic HBM bytes of the search per MB: 256 B source luma, 4 x 256 B
 # reference planes read once, 128 B record, 32 B exchange words.
 BYTES_PER_MB = 256 + 4 * 256 + 128 + 32
+# Pruner network multiply-adds per MB (include/pcme_net.h): conv1 144 px x
+# 32 x 18, conv2 36 x 32 x 288, conv3 36 x 32 x 288, conv4 9 x 32 x 288,
+# fc 293 x 32.
+NET_MACS_PER_MB = 144 * 32 * 18 + 36 * 32 * 288 * 2 + 9 * 32 * 288 + 293 * 32
+MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
 def parse():
@@ -49,6 +54,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-pruned", action="store_true", help="skip the CNN-pruned leg")
+    ap.add_argument("--ncand", type=int, default=8, help="pruner candidates per MB")
+    ap.add_argument("--thr", type=float, default=0.6, help="pruner confidence threshold")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known "
                          "traffic, for scaling the FETCH_SIZE/WRITE_SIZE counters)")
@@ -199,6 +207,9 @@ def main():
         "graph": graph,
     }
 
+    if not args.no_pruned:
+        line["pruned"] = pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, ms_step)
+
     if not args.no_cpu:
         from oracle import oracle as O
         y, mbw, mbh, qp, rng = host[-1]
@@ -225,6 +236,100 @@ def main():
                                 "seconds": round(secs, 3)}
         line["parity_sample"] = bool(parity)
     print(json.dumps(line), flush=True)
+
+
+def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plain_ms_step):
+    """The same frames through the CNN-pruned search: a step = pad + half-pel
+    planes + the fused network inference (hints) + the pruned search; the
+    plain search of the step above judges the result."""
+    import numpy as np
+    import pcme.pruner as P
+    if not os.path.exists(P.WEIGHTS):
+        return {"skipped": "no trained weights (pcme/pruner_weights.npz)"}
+    sp = stream.cuda_stream
+    nets, jobs, pouts, hints, groups = {}, [], [], [], {}
+    for dc, (spec, qp, rng) in zip(clips, specs):
+        if qp not in nets:
+            nets[qp] = P.Net(seed=1, max_batch=32, weights=P.WEIGHTS, qp=qp)
+        nb, nmb = dc.n - 1, dc.mbw * dc.mbh
+        o = torch.empty((nb, nmb * pcme.MB_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+        h = torch.zeros((nb, nmb * pcme.HINT_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+        js = [pcme.frame_job(dc, t, t - 1, o[t - 1].data_ptr(), rng, 1, True, qp, hint_ptr=h[t - 1].data_ptr())
+              for t in range(1, dc.n)]
+        groups.setdefault(qp, []).extend(js)
+        jobs += js
+        pouts.append(o)
+        hints.append(h)
+    plan = pcme.Plan(jobs)
+
+    def prepare():
+        for dc in clips:
+            dc.pad(sp)
+            dc.hpel(0, dc.n - 1, sp)
+
+    def infer():
+        for qp, js in groups.items():
+            nets[qp].hints(js, args.ncand, args.thr, args.thr, sp)
+
+    prepare()
+    infer()
+    tune = None
+    if not args.no_autotune:
+        waves, tms = plan.autotune(reps=2, stream=sp)
+        tune = {"waves": waves, "search_ms": round(tms, 3)}
+    ev_h, ev_s = [], []
+
+    def step(record):
+        prepare()
+        if record:
+            a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            a.record(stream)
+        infer()
+        if record:
+            b.record(stream)
+        plan.launch(sp)
+        if record:
+            c.record(stream)
+            ev_h.append((a, b))
+            ev_s.append((b, c))
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    plan.check(sp)
+    hints_ms = sum(a.elapsed_time(b) for a, b in ev_h) / len(ev_h)
+    search_ms = sum(a.elapsed_time(b) for a, b in ev_s) / len(ev_s)
+    ms_step = 1e3 * dt / args.steps
+    plain = np.concatenate([o.cpu().numpy().view(pcme.MB_DTYPE).reshape(-1) for o in outs])
+    pruned = np.concatenate([o.cpu().numpy().view(pcme.MB_DTYPE).reshape(-1) for o in pouts])
+    hs = np.concatenate([h.cpu().numpy().view(pcme.HINT_DTYPE).reshape(-1) for h in hints])
+    j = P.judge(plain[None], pruned[None])
+    tflops = 2.0 * NET_MACS_PER_MB * mbs_per_step / (hints_ms * 1e-3) / 1e12
+    for n in nets.values():
+        n.close()
+    print(f"[bench] pruned: {ms_step:.2f} ms/step (hints {hints_ms:.2f} ms, search {search_ms:.2f} ms), "
+          f"agreement {j['agreement']:.4f}, J-cost +{100 * j['jcost_increase']:.3f}%", file=sys.stderr, flush=True)
+    return {
+        "mblocks_per_s": round(mbs_per_step / (ms_step * 1e-3) / 1e6, 4),
+        "ms_per_step": round(ms_step, 3),
+        "speedup_vs_plain": round(plain_ms_step / ms_step, 3),
+        "hints_ms": round(hints_ms, 3), "search_ms": round(search_ms, 3),
+        "decision_agreement": round(j["agreement"], 5),
+        "part_agreement": round(j["part_agreement"], 5),
+        "jcost_increase": round(j["jcost_increase"], 6),
+        "pruned_fraction": round(float((hs["prune"] != 0).mean()), 5),
+        "ncand": args.ncand, "confidence_threshold": args.thr,
+        "autotune": tune,
+        "net_roofline": {"bound": "mfma", "kernel": "hints_kernel", "achieved": round(tflops, 3),
+                         "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5), "macs_per_mb": NET_MACS_PER_MB},
+        "bd_rate": None,
+    }
 
 
 if __name__ == "__main__":

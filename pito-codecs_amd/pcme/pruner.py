@@ -63,14 +63,15 @@ def mv_class(mv16: np.ndarray) -> np.ndarray:
 class Net:
     """A pruner network on the current device (pcme_net_*)."""
 
-    def __init__(self, seed: int = 1, max_batch: int = 1024, weights: str | np.ndarray | None = None):
+    def __init__(self, seed: int = 1, max_batch: int = 1024, weights: str | np.ndarray | None = None,
+                 qp: int = 27):
         lib = _lib()
         h = ctypes.c_void_p()
         _check(lib.pcme_net_create(int(seed), int(max_batch), ctypes.byref(h)))
         self.handle = h
         self.max_batch = max_batch
         if weights is not None:
-            self.set_params(load_weights(weights) if isinstance(weights, str) else weights)
+            self.set_params(load_weights(weights, qp) if isinstance(weights, str) else weights)
 
     def close(self) -> None:
         if getattr(self, "handle", None):
@@ -120,13 +121,24 @@ def inputs(clip: DeviceClip, cur: int, ref: int, mbs, out, stream=None) -> None:
                                   mbs.data_ptr(), int(mbs.numel()), out.data_ptr(), _stream_ptr(stream)))
 
 
-def load_weights(path: str = WEIGHTS) -> np.ndarray:
+def load_weights(path: str = WEIGHTS, qp: int = 27) -> np.ndarray:
+    """Parameters of the network trained for the QP nearest to `qp`."""
     with np.load(path, allow_pickle=False) as z:
-        return np.asarray(z["params"], np.float32)
+        if "params" in z.files:
+            return np.asarray(z["params"], np.float32)
+        qps = [int(k[len("params_qp"):]) for k in z.files if k.startswith("params_qp")]
+        best = min(qps, key=lambda q: (abs(q - qp), q))
+        return np.asarray(z[f"params_qp{best}"], np.float32)
 
 
 def save_weights(p: np.ndarray, path: str = WEIGHTS, **meta) -> None:
     np.savez_compressed(path, params=np.asarray(p, np.float32),
+                        **{k: np.asarray(v) for k, v in meta.items()})
+
+
+def save_weights_qps(params: dict, path: str = WEIGHTS, **meta) -> None:
+    """One parameter vector per QP (keys params_qp<QP>)."""
+    np.savez_compressed(path, **{f"params_qp{int(q)}": np.asarray(v, np.float32) for q, v in params.items()},
                         **{k: np.asarray(v) for k, v in meta.items()})
 
 
