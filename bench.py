@@ -56,7 +56,9 @@ def parse():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-pruned", action="store_true", help="skip the CNN-pruned leg")
     ap.add_argument("--ncand", type=int, default=8, help="pruner candidates per MB")
-    ap.add_argument("--thr", type=float, default=0.6, help="pruner confidence threshold")
+    ap.add_argument("--thr", type=float, default=0.6, help="pruner confidence threshold (partition head)")
+    ap.add_argument("--thr-mv", type=float, default=None,
+                    help="threshold on the motion head's candidate mass (default: --thr)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known "
                          "traffic, for scaling the FETCH_SIZE/WRITE_SIZE counters)")
@@ -247,6 +249,7 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
     if not os.path.exists(P.WEIGHTS):
         return {"skipped": "no trained weights (pcme/pruner_weights.npz)"}
     sp = stream.cuda_stream
+    thr_mv = args.thr if args.thr_mv is None else args.thr_mv
     nets, jobs, pouts, hints, groups = {}, [], [], [], {}
     for dc, (spec, qp, rng) in zip(clips, specs):
         if qp not in nets:
@@ -269,7 +272,7 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
 
     def infer():
         for qp, js in groups.items():
-            nets[qp].hints(js, args.ncand, args.thr, args.thr, sp)
+            nets[qp].hints(js, args.ncand, args.thr, thr_mv, sp)
 
     prepare()
     infer()
@@ -323,7 +326,7 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
         "part_agreement": round(j["part_agreement"], 5),
         "jcost_increase": round(j["jcost_increase"], 6),
         "pruned_fraction": round(float((hs["prune"] != 0).mean()), 5),
-        "ncand": args.ncand, "confidence_threshold": args.thr,
+        "ncand": args.ncand, "confidence_threshold": args.thr, "mv_mass_threshold": thr_mv,
         "autotune": tune,
         "net_roofline": {"bound": "mfma", "kernel": "hints_kernel", "achieved": round(tflops, 3),
                          "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",

@@ -84,7 +84,7 @@ static int get_dev(Dev **out) {
   for (int v = 0; v < 2; v++) {
     int per_cu = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, v ? search_frames_kernel<true> : search_frames_kernel<false>, 64, 0));
+        &per_cu, v ? search_frames_hinted_kernel : search_frames_kernel, 64, 0));
     if (per_cu < 1) per_cu = 1;
     dv->resident_waves[v] = per_cu * p.multiProcessorCount;
   }
@@ -259,7 +259,7 @@ static int plan_launch(Plan *p, hipStream_t stream) {
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(clear_kernel, dim3(512), dim3(256), 0, stream, p->ctl, p->xchg, p->nwords);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(p->hinted ? search_frames_kernel<true> : search_frames_kernel<false>, dim3((unsigned)p->grid),
+  hipLaunchKernelGGL(p->hinted ? search_frames_hinted_kernel : search_frames_kernel, dim3((unsigned)p->grid),
                      dim3(64), 0, stream, p->djobs, p->dtickets,
                      p->ntickets, p->ctl, kSpinLimit);
   HIPCHK(hipGetLastError());

@@ -193,7 +193,7 @@ __device__ __forceinline__ void load_src(const uint8_t *blk, int sstride, const 
 #define WIN_W 44   // 16 + 2*WIN_M + 8 bytes of alignment / over-read slack
 #define WIN_H 37   // 16 + 2*WIN_M + 1
 #define SP_W 32
-#define SP_H 21
+#define SP_H 23   // 16 + 2*3 + 1: room for a pruned search's square rounds
 #define WIN_BYTES (WIN_W * WIN_H)
 #define SP_PLANE (SP_W * SP_H)
 #define LDS_PER_SEARCH (((WIN_BYTES + 4 * SP_PLANE) + 15) & ~15)
@@ -243,7 +243,7 @@ __device__ __forceinline__ void stage_region(const uint8_t *plane, int stride, i
 // Integer-pel SAD cost of the lane's slot candidate (slice.rs:1605-1616):
 // SAD over the block + (lam * mv_bits) >> 8.  (ux, uy) is the address MV:
 // the candidate when valid, else a valid fallback (its cost is ignored).
-template <bool LDS, int LPC = 8>
+template <bool LDS, int LPC = 8, int PITCH = WIN_W>
 __device__ __forceinline__ uint32_t sad_cost(const SCtx &c, const Lane &L, const SrcRegs &s, int bx, int by,
                                              int cx, int cy, int ux, int uy, int mvpx, int mvpy,
                                              const uint32_t *lw, int wx0, int wy0) {
@@ -253,7 +253,7 @@ __device__ __forceinline__ uint32_t sad_cost(const SCtx &c, const Lane &L, const
   for (int r = 0; r < 4; r++) {
     uint32_t a, b;
     if (LDS)
-      ld8_at(lw, (y - wy0 + r) * WIN_W + (x - wx0), a, b);
+      ld8_at(lw, (y - wy0 + r) * PITCH + (x - wx0), a, b);
     else
       ld8_at(gplane(c.pl[0]), (y + r + PCME_PAD) * c.stride + x + PCME_PAD, a, b);
     acc = __builtin_amdgcn_sad_u8(a, s.raw[2 * r], acc);
@@ -375,14 +375,17 @@ __device__ __forceinline__ void ensure_win(const SCtx &c, int bx, int by, int bx
 
 // Same for the four sub-pel planes: candidates of a round lie within +-1 px
 // of the current best, reading one more row/column for the half-pel offsets.
-template <int W, int H>
+// A restage centres the region with RM pixels of margin (a pruned search
+// stages once with RM = 3 before its square rounds, which then read the
+// full-pel plane of the region).
+template <int W, int H, int RM = 2>
 __device__ __forceinline__ void ensure_sp(const SCtx &c, int bx, int by, int bxv, int byv, int &sx0, int &sy0,
                                           uint32_t *lsp) {
   const int X = bx + (bxv >> 2), Y = by + (byv >> 2);
   const bool ok = (X - 1 - sx0 >= 0) & (X + W + 6 - sx0 <= SP_W) & (Y - 1 - sy0 >= 0) & (Y + H + 2 - sy0 <= SP_H);
   if (ok) return;
-  sx0 = clampi((X - 2) & ~3, -PCME_PAD, c.pw + PCME_PAD - SP_W);
-  sy0 = clampi(Y - 2, -PCME_PAD, c.ph + PCME_PAD - SP_H);
+  sx0 = clampi((X - RM) & ~3, -PCME_PAD, c.pw + PCME_PAD - SP_W);
+  sy0 = clampi(Y - RM, -PCME_PAD, c.ph + PCME_PAD - SP_H);
   __syncthreads();
 #pragma unroll
   for (int p = 0; p < 4; p++) stage_region<SP_W, SP_H>(c.pl[p], c.stride, sx0, sy0, lsp + p * (SP_PLANE / 4));
@@ -405,7 +408,7 @@ __device__ __forceinline__ void hint_cand(const uint32_t *hc, int i, int nh, int
 // the hint candidates hc[0..nh) are scanned after the search_part
 // candidates (slots 3-7, then slots 0-2 of a second pass) and the hexagon
 // descent is skipped.
-template <int W, int H>
+template <int W, int H, bool PR>
 __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const Lane &L, int bx, int by, int mvpx,
                                        int mvpy, int c0x, int c0y, int c1x, int c1y, int ncand, const uint32_t *hc,
                                        int nh, uint32_t *lds) {
@@ -419,7 +422,7 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
   int bxv = (mvpx + 2) & ~3, byv = (mvpy + 2) & ~3;
   if (!in_box(ok, bxv, byv)) { bxv = 0; byv = 0; }
   uint32_t bcost;
-  {
+  if constexpr (!PR) {
     const int r0x = (c0x + 2) & ~3, r0y = (c0y + 2) & ~3;
     const int r1x = (c1x + 2) & ~3, r1y = (c1y + 2) & ~3;
     const int cx = L.slot == 1 ? r0x : (L.slot == 2 ? r1x : bxv);
@@ -432,12 +435,30 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
     const int w = (int)(key & 7u);
     bxv = w == 1 ? r0x : (w == 2 ? r1x : bxv);
     byv = w == 1 ? r0y : (w == 2 ? r1y : byv);
-  }
-  if (nh > 0) {
-    // pruned search: the hint candidates continue the scan (one per slot)
+  } else {
+    // Pruned search: the hint candidates 0..4 continue the scan in slots
+    // 3-7 of the same pass.
+    const int r0x = (c0x + 2) & ~3, r0y = (c0y + 2) & ~3;
+    const int r1x = (c1x + 2) & ~3, r1y = (c1y + 2) & ~3;
     int hx, hy;
-    hint_cand(hc, L.slot, nh, hx, hy);
-    const bool valid = (L.slot < nh) & in_box(ok, hx, hy);
+    hint_cand(hc, L.slot - 3, nh, hx, hy);
+    const bool hs = L.slot >= 3;
+    const int cx = L.slot == 1 ? r0x : (L.slot == 2 ? r1x : (hs ? hx : bxv));
+    const int cy = L.slot == 1 ? r0y : (L.slot == 2 ? r1y : (hs ? hy : byv));
+    const bool valid = (L.slot == 0) | (((L.slot <= ncand) | (hs & (L.slot - 3 < nh))) & in_box(ok, cx, cy));
+    const uint32_t k = sad_cost<false>(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy,
+                                       lw, 0, 0);
+    const uint32_t key = min_key(k, valid, L.slot);
+    bcost = key >> 3;
+    const int w = (int)(key & 7u);
+    bxv = (int)rdlane((uint32_t)cx, 8 * w);
+    byv = (int)rdlane((uint32_t)cy, 8 * w);
+  }
+  if (PR && nh > 5) {
+    // hint candidates 5..7 continue the scan (slots 0-2)
+    int hx, hy;
+    hint_cand(hc, L.slot + 5, nh, hx, hy);
+    const bool valid = (L.slot + 5 < nh) & in_box(ok, hx, hy);
     const uint32_t k = sad_cost<false>(c, L, s, bx, by, hx, hy, valid ? hx : bxv, valid ? hy : byv, mvpx, mvpy,
                                        lw, 0, 0);
     const uint32_t key = min_key(k, valid, L.slot);
@@ -460,7 +481,7 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
   hb.ylo = max(hb.ylo, c0yy - 4 * range - 3);
   hb.yhi = min(hb.yhi, c0yy + 4 * range + 3);
   if (L.slot >= 6) hb.xlo = 1 << 20;  // slots 6, 7 carry no hex point
-  for (int it = 0; it < (nh < 0 ? range : 0); it++) {
+  for (int it = 0; it < (PR ? 0 : range); it++) {
     ensure_win<W, H>(c, bx, by, bxv, byv, 2, wx0, wy0, lw);
     const int cx = bxv + hdx, cy = byv + hdy;
     const bool valid = in_box(hb, cx, cy);
@@ -473,14 +494,22 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
     bxv += hex_dx(j) * 4;
     byv += hex_dy(j) * 4;
   }
-  // slice.rs:1657-1682: two rounds of 8-neighbour integer refinement.
+  // slice.rs:1657-1682: two rounds of 8-neighbour integer refinement.  The
+  // full search reads the full-pel window; a pruned search stages the four
+  // sub-pel planes once (margin 3) and reads the full-pel plane among them.
   const int qdx = sq_dx(L.slot), qdy = sq_dy(L.slot);
+  int sx0 = 1 << 20, sy0 = 1 << 20;
   for (int it = 0; it < 2; it++) {
-    ensure_win<W, H>(c, bx, by, bxv, byv, 1, wx0, wy0, lw);
+    if constexpr (PR)
+      ensure_sp<W, H, 3>(c, bx, by, bxv, byv, sx0, sy0, lsp);
+    else
+      ensure_win<W, H>(c, bx, by, bxv, byv, 1, wx0, wy0, lw);
     const int cx = bxv + qdx * 4, cy = byv + qdy * 4;
     const bool valid = in_box(ok, cx, cy);
     const uint32_t k =
-        sad_cost<true>(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy, lw, wx0, wy0);
+        PR ? sad_cost<true, 8, SP_W>(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy, lsp,
+                                     sx0, sy0)
+           : sad_cost<true>(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy, lw, wx0, wy0);
     const uint32_t key = min_key(k, valid, L.slot);
     if ((key >> 3) >= bcost) break;
     bcost = key >> 3;
@@ -489,7 +518,6 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
     byv += sq_dy(j) * 4;
   }
   // Stage the four planes around the integer best for the sub-pel phase.
-  int sx0 = 1 << 20, sy0 = 1 << 20;
   ensure_sp<W, H>(c, bx, by, bxv, byv, sx0, sy0, lsp);
   v2s sp[16];
   src_pairs(s, sp);
@@ -568,6 +596,7 @@ __device__ __forceinline__ bool take_key(PState &S, uint32_t key, int dx, int dy
 // search (slice.rs:1590-1723) for A = 16x8 partition and B = 8x16
 // partition at the same time.  src* = block top-left of each source.
 // nh >= 0: pruned search with hint candidates hc[0..nh) (see search()).
+template <bool PR>
 __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B, const uint8_t *srcA,
                                             const uint8_t *srcB, int sstride, int c0x, int c0y, int c1ax,
                                             int c1ay, int c1bx, int c1by, const uint32_t *hc, int nh,
@@ -600,10 +629,14 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
     const int r1bx = (c1bx + 2) & ~3, r1by = (c1by + 2) & ~3;
     const int r1x = gsel(g, r1ax, r1bx), r1y = gsel(g, r1ay, r1by);
     const int px = gsel(g, a0x, b0x), py = gsel(g, a0y, b0y);
-    const int cx = L.slot == 1 ? r0x : (L.slot == 2 ? r1x : px);
-    const int cy = L.slot == 1 ? r0y : (L.slot == 2 ? r1y : py);
+    // a pruned search scans hint candidates 0..4 in slots 3-7 of this pass
+    int hx = 0, hy = 0;
+    if constexpr (PR) hint_cand(hc, L.slot - 3, nh, hx, hy);
+    const bool hs = PR && L.slot >= 3;
+    const int cx = L.slot == 1 ? r0x : (L.slot == 2 ? r1x : (hs ? hx : px));
+    const int cy = L.slot == 1 ? r0y : (L.slot == 2 ? r1y : (hs ? hy : py));
     const Box ok = g ? B.ok : A.ok;
-    const bool valid = (L.slot == 0) | ((L.slot <= 2) & in_box(ok, cx, cy));
+    const bool valid = (L.slot == 0) | (((L.slot <= 2) | (hs & (L.slot - 3 < nh))) & in_box(ok, cx, cy));
     const uint32_t k = sad_cost<false, 4>(c, L, s, bx, by, cx, cy, valid ? cx : px, valid ? cy : py, mvpx, mvpy,
                                           lw, 0, 0);
     uint32_t ka, kb;
@@ -611,18 +644,25 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
     A.cost = ka >> 3;
     B.cost = kb >> 3;
     const int wa = (int)(ka & 7u), wb = (int)(kb & 7u);
-    A.x = wa == 1 ? r0x : (wa == 2 ? r1ax : a0x);
-    A.y = wa == 1 ? r0y : (wa == 2 ? r1ay : a0y);
-    B.x = wb == 1 ? r0x : (wb == 2 ? r1bx : b0x);
-    B.y = wb == 1 ? r0y : (wb == 2 ? r1by : b0y);
+    if constexpr (PR) {
+      A.x = (int)rdlane((uint32_t)cx, 4 * wa);
+      A.y = (int)rdlane((uint32_t)cy, 4 * wa);
+      B.x = (int)rdlane((uint32_t)cx, 32 + 4 * wb);
+      B.y = (int)rdlane((uint32_t)cy, 32 + 4 * wb);
+    } else {
+      A.x = wa == 1 ? r0x : (wa == 2 ? r1ax : a0x);
+      A.y = wa == 1 ? r0y : (wa == 2 ? r1ay : a0y);
+      B.x = wb == 1 ? r0x : (wb == 2 ? r1bx : b0x);
+      B.y = wb == 1 ? r0y : (wb == 2 ? r1by : b0y);
+    }
   }
-  if (nh > 0) {
-    // pruned search: the hint candidates continue each scan (one per slot)
+  if (PR && nh > 5) {
+    // hint candidates 5..7 continue each scan (slots 0-2)
     int hx, hy;
-    hint_cand(hc, L.slot, nh, hx, hy);
+    hint_cand(hc, L.slot + 5, nh, hx, hy);
     const Box ok = g ? B.ok : A.ok;
     const int ox = gsel(g, A.x, B.x), oy = gsel(g, A.y, B.y);
-    const bool valid = (L.slot < nh) & in_box(ok, hx, hy);
+    const bool valid = (L.slot + 5 < nh) & in_box(ok, hx, hy);
     const uint32_t k = sad_cost<false, 4>(c, L, s, bx, by, hx, hy, valid ? hx : ox, valid ? hy : oy, mvpx, mvpy,
                                           lw, 0, 0);
     uint32_t ka, kb;
@@ -654,7 +694,7 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
   B.hb.yhi = min(B.hb.yhi, B.y + 4 * range + 3);
   // slice.rs:1632-1656: hexagon descent of both.
   const int hdx = hex_dx(L.slot) * 4, hdy = hex_dy(L.slot) * 4;
-  A.act = B.act = range > 0 && nh < 0;
+  A.act = B.act = range > 0 && !PR;
   A.it = B.it = 0;
   while (A.act | B.act) {
     if (A.act) ensure_win<16, 8>(c, A.bx, A.by, A.x, A.y, 2, A.wx0, A.wy0, lds);
@@ -677,20 +717,32 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
       B.act = take_key(B, kb, hex_dx(j) * 4, hex_dy(j) * 4) && ++B.it < range;
     }
   }
-  // slice.rs:1657-1682: two rounds of 8-neighbour refinement of both.
+  // slice.rs:1657-1682: two rounds of 8-neighbour refinement of both (a
+  // pruned search reads the full-pel plane of its sub-pel region, staged
+  // once with margin 3).
   const int qdx = sq_dx(L.slot), qdy = sq_dy(L.slot);
+  A.sx0 = A.sy0 = B.sx0 = B.sy0 = 1 << 20;
   A.act = B.act = true;
   A.it = B.it = 0;
   while (A.act | B.act) {
-    if (A.act) ensure_win<16, 8>(c, A.bx, A.by, A.x, A.y, 1, A.wx0, A.wy0, lds);
-    if (B.act) ensure_win<8, 16>(c, B.bx, B.by, B.x, B.y, 1, B.wx0, B.wy0, lds + LDS_PER_SEARCH / 4);
+    if constexpr (PR) {
+      if (A.act) ensure_sp<16, 8, 3>(c, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
+      if (B.act)
+        ensure_sp<8, 16, 3>(c, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
+    } else {
+      if (A.act) ensure_win<16, 8>(c, A.bx, A.by, A.x, A.y, 1, A.wx0, A.wy0, lds);
+      if (B.act) ensure_win<8, 16>(c, B.bx, B.by, B.x, B.y, 1, B.wx0, B.wy0, lds + LDS_PER_SEARCH / 4);
+    }
     const int ox = gsel(g, A.x, B.x), oy = gsel(g, A.y, B.y);
     const int cx = ox + qdx * 4, cy = oy + qdy * 4;
     const Box ok = g ? B.ok : A.ok;
     const bool act = g ? B.act : A.act;
     const bool valid = act & in_box(ok, cx, cy);
-    const uint32_t k = sad_cost<true, 4>(c, L, s, bx, by, cx, cy, valid ? cx : ox, valid ? cy : oy, mvpx, mvpy, lw,
-                                         gsel(g, A.wx0, B.wx0), gsel(g, A.wy0, B.wy0));
+    const uint32_t k =
+        PR ? sad_cost<true, 4, SP_W>(c, L, s, bx, by, cx, cy, valid ? cx : ox, valid ? cy : oy, mvpx, mvpy, lsp,
+                                     gsel(g, A.sx0, B.sx0), gsel(g, A.sy0, B.sy0))
+           : sad_cost<true, 4>(c, L, s, bx, by, cx, cy, valid ? cx : ox, valid ? cy : oy, mvpx, mvpy, lw,
+                               gsel(g, A.wx0, B.wx0), gsel(g, A.wy0, B.wy0));
     uint32_t ka, kb;
     min_key_pair(k, valid, L.slot, ka, kb);
     if (A.act) {
@@ -703,7 +755,6 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
     }
   }
   // Sub-pel planes of both, then the SATD cost at each integer best.
-  A.sx0 = A.sy0 = B.sx0 = B.sy0 = 1 << 20;
   ensure_sp<16, 8>(c, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
   ensure_sp<8, 16>(c, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
   v2s sp[16];
@@ -861,7 +912,7 @@ struct PartRes {
 };
 
 // One partition of search_part (slice.rs:1826-1866), then the next.
-template <int PART, int P>
+template <int PART, int P, bool PR>
 __device__ __forceinline__ void part_step(const SCtx &c, const Nbr &n, Cur &cur, const uint8_t *src,
                                           int sstride, int mbx, int mby, uint32_t seed, const uint32_t *hc,
                                           int nh, PartRes &res, uint32_t *lds) {
@@ -873,19 +924,19 @@ __device__ __forceinline__ void part_step(const SCtx &c, const Nbr &n, Cur &cur,
     SrcRegs s;
     load_src(src + Y * sstride + X, sstride, L, s);
     const uint32_t m0 = P == 0 ? 0u : res.mv[0];
-    SRes r = search<W, H>(c, s, L, mbx + X, mby + Y, mv_x(pv), mv_y(pv), mv_x(seed), mv_y(seed), mv_x(m0),
+    SRes r = search<W, H, PR>(c, s, L, mbx + X, mby + Y, mv_x(pv), mv_y(pv), mv_x(seed), mv_y(seed), mv_x(m0),
                           mv_y(m0), 2, hc, nh, lds);
     const uint32_t m = pack_mv(r.x, r.y);
     res.mv[P] = m;
     res.mvp[P] = pv;
     res.total += r.cost;
     cur.mv[P] = m;
-    part_step<PART, P + 1>(c, n, cur, src, sstride, mbx, mby, seed, hc, nh, res, lds);
+    part_step<PART, P + 1, PR>(c, n, cur, src, sstride, mbx, mby, seed, hc, nh, res, lds);
   }
 }
 
 // search_part (slice.rs:1810-1868), list 0.  src = MB top-left.
-template <int PART>
+template <int PART, bool PR>
 __device__ __forceinline__ void search_part(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
                                             int mbx, int mby, uint32_t seed, const uint32_t *hc, int nh,
                                             PartRes &res, uint32_t *lds) {
@@ -893,11 +944,12 @@ __device__ __forceinline__ void search_part(const SCtx &c, const Nbr &n, const u
 #pragma unroll
   for (int i = 0; i < 4; i++) { res.mv[i] = res.mvp[i] = 0u; }
   res.total = 0;
-  part_step<PART, 0>(c, n, cur, src, sstride, mbx, mby, seed, hc, nh, res, lds);
+  part_step<PART, 0, PR>(c, n, cur, src, sstride, mbx, mby, seed, hc, nh, res, lds);
 }
 
 // search_part (slice.rs:1810-1868) of P16x8 (ra) and P8x16 (rb) together:
 // partition p of both modes in one paired search.
+template <bool PR>
 __device__ __forceinline__ void search_part_12(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
                                                int mbx, int mby, uint32_t seed, const uint32_t *hc, int nh,
                                                PartRes &ra, PartRes &rb, uint32_t *lds) {
@@ -910,7 +962,7 @@ __device__ __forceinline__ void search_part_12(const SCtx &c, const Nbr &n, cons
   uint32_t pa = mvp<1, 0, 0, 0, 4, 1>(n, ca), pb = mvp<2, 0, 0, 0, 2, 3>(n, cb);
   A.bx = mbx; A.by = mby; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
   B.bx = mbx; B.by = mby; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
-  search_pair(c, A, B, src, src, sstride, mv_x(seed), mv_y(seed), 0, 0, 0, 0, hc, nh, lds, xa, xb);
+  search_pair<PR>(c, A, B, src, src, sstride, mv_x(seed), mv_y(seed), 0, 0, 0, 0, hc, nh, lds, xa, xb);
   ra.mv[0] = ca.mv[0] = pack_mv(xa.x, xa.y); ra.mvp[0] = pa; ra.total = xa.cost;
   rb.mv[0] = cb.mv[0] = pack_mv(xb.x, xb.y); rb.mvp[0] = pb; rb.total = xb.cost;
   // partition 1: 16x8 bottom (y 2 blocks), 8x16 right (x 2 blocks)
@@ -918,7 +970,7 @@ __device__ __forceinline__ void search_part_12(const SCtx &c, const Nbr &n, cons
   pb = mvp<2, 1, 2, 0, 2, 4>(n, cb);
   A.bx = mbx; A.by = mby + 8; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
   B.bx = mbx + 8; B.by = mby; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
-  search_pair(c, A, B, src + 8 * sstride, src + 8, sstride, mv_x(seed), mv_y(seed), mv_x(ra.mv[0]),
+  search_pair<PR>(c, A, B, src + 8 * sstride, src + 8, sstride, mv_x(seed), mv_y(seed), mv_x(ra.mv[0]),
               mv_y(ra.mv[0]), mv_x(rb.mv[0]), mv_y(rb.mv[0]), hc, nh, lds, xa, xb);
   ra.mv[1] = pack_mv(xa.x, xa.y); ra.mvp[1] = pa; ra.total += xa.cost;
   rb.mv[1] = pack_mv(xb.x, xb.y); rb.mvp[1] = pb; rb.total += xb.cost;
@@ -946,17 +998,52 @@ struct MbRes {
   uint32_t skip, m16;
 };
 
+// Modes P16x8 and P8x16 (searched together), then the strict-< choice
+// against the running best (slice.rs:1925-1945).
+template <bool PR>
+__device__ __forceinline__ void mode_12(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride, int mbx,
+                                        int mby, const uint32_t *hc, int nh, MbRes &o, uint32_t *lds) {
+  PartRes r, r2;
+  search_part_12<PR>(c, n, src, sstride, mbx, mby, o.m16, hc, nh, r, r2, lds);
+  o.cost[1] = r.total;
+  uint32_t cc = r.total + ((c.lam * 3u) >> 8);
+  bool t = cc < o.best_cost;
+  o.best_cost = t ? cc : o.best_cost;
+  o.part = t ? 1 : o.part;
+  take_if(o.best, r, t);
+  o.cost[2] = r2.total;
+  cc = r2.total + ((c.lam * 3u) >> 8);
+  t = cc < o.best_cost;
+  o.best_cost = t ? cc : o.best_cost;
+  o.part = t ? 2 : o.part;
+  take_if(o.best, r2, t);
+}
+
+// Mode P8x8 and its choice.
+template <bool PR>
+__device__ __forceinline__ void mode_3(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride, int mbx,
+                                       int mby, const uint32_t *hc, int nh, MbRes &o, uint32_t *lds) {
+  PartRes r;
+  search_part<3, PR>(c, n, src, sstride, mbx, mby, o.m16, hc, nh, r, lds);
+  o.cost[3] = r.total;
+  const uint32_t cc = r.total + ((c.lam * 10u) >> 8);
+  const bool t = cc < o.best_cost;
+  o.best_cost = t ? cc : o.best_cost;
+  o.part = t ? 3 : o.part;
+  take_if(o.best, r, t);
+}
+
 // ME-only analysis of one MB (slice.rs:1884-1945 without early skip,
 // intra and RD).  nh >= 0: pruned MB (hint candidates hc[0..nh), hinted
 // mode hpart): P16x16, then only P16x8 + P8x16 (hpart 1 or 2) or only P8x8
 // (hpart 3), with the same strict-< choice.
+template <bool PR>
 __device__ __forceinline__ void analyse_mb(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
                                            int mbx, int mby, int partitions, const uint32_t *hc, int nh,
                                            int hpart, MbRes &o, uint32_t *lds) {
   const uint32_t skip = pskip(n);
-  const bool pr = nh >= 0;
   PartRes r16;
-  search_part<0>(c, n, src, sstride, mbx, mby, skip, hc, nh, r16, lds);
+  search_part<0, PR>(c, n, src, sstride, mbx, mby, skip, hc, nh, r16, lds);
   const uint32_t cost16 = r16.total;
   o.part = 0;
   take_if(o.best, r16, true);
@@ -965,32 +1052,16 @@ __device__ __forceinline__ void analyse_mb(const SCtx &c, const Nbr &n, const ui
   o.cost[1] = o.cost[2] = o.cost[3] = 0xFFFFFFFFu;
   o.skip = skip;
   o.m16 = r16.mv[0];
-  if (partitions && cost16 > 1024u && (!pr || hpart != 0)) {
-    PartRes r, r2;
-    if (!pr || hpart == 1 || hpart == 2) {
-    search_part_12(c, n, src, sstride, mbx, mby, o.m16, hc, nh, r, r2, lds);
-    o.cost[1] = r.total;
-    uint32_t cc = r.total + ((c.lam * 3u) >> 8);
-    bool t = cc < o.best_cost;
-    o.best_cost = t ? cc : o.best_cost;
-    o.part = t ? 1 : o.part;
-    take_if(o.best, r, t);
-    o.cost[2] = r2.total;
-    cc = r2.total + ((c.lam * 3u) >> 8);
-    t = cc < o.best_cost;
-    o.best_cost = t ? cc : o.best_cost;
-    o.part = t ? 2 : o.part;
-    take_if(o.best, r2, t);
-    }
-    if (!pr || hpart == 3) {
-    search_part<3>(c, n, src, sstride, mbx, mby, o.m16, hc, nh, r, lds);
-    o.cost[3] = r.total;
-    const uint32_t cc = r.total + ((c.lam * 10u) >> 8);
-    const bool t = cc < o.best_cost;
-    o.best_cost = t ? cc : o.best_cost;
-    o.part = t ? 3 : o.part;
-    take_if(o.best, r, t);
-    }
+  const bool split = partitions && cost16 > 1024u;
+  if (PR) {
+    // A pruned MB searches the hinted mode only: P16x8 with P8x16, or P8x8.
+    if (split & ((hpart == 1) | (hpart == 2)))
+      mode_12<PR>(c, n, src, sstride, mbx, mby, hc, nh, o, lds);
+    else if (split & (hpart == 3))
+      mode_3<PR>(c, n, src, sstride, mbx, mby, hc, nh, o, lds);
+  } else if (split) {
+    mode_12<PR>(c, n, src, sstride, mbx, mby, hc, nh, o, lds);
+    mode_3<PR>(c, n, src, sstride, mbx, mby, hc, nh, o, lds);
   }
 }
 
@@ -1063,10 +1134,8 @@ __device__ __forceinline__ bool fetch_words(const uint32_t *xw, uint32_t tag, ui
 }
 
 template <bool HINTS>
-__global__ void __launch_bounds__(64) search_frames_kernel(const DevJob *__restrict__ jobs,
-                                                           const int2 *__restrict__ tickets, int ntickets,
-                                                           uint32_t *ctl, uint32_t spin_limit) {
-  __shared__ uint32_t s_lds[LDS_PER_WAVE / 4];
+__device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
+                                            int ntickets, uint32_t *ctl, uint32_t spin_limit, uint32_t *s_lds) {
   uint32_t *counter = ctl;
   uint32_t *err = ctl + 1;
   const uint32_t tag = ctl[2] & 0xFFu;  // this launch's epoch (set by prep_kernel)
@@ -1139,7 +1208,10 @@ __global__ void __launch_bounds__(64) search_frames_kernel(const DevJob *__restr
         }
       }
       MbRes o;
-      analyse_mb(c, n, src, stride, 16 * mx, 16 * my, J.partitions, hc, nh, hpart, o, s_lds);
+      if (HINTS && nh >= 0)
+        analyse_mb<true>(c, n, src, stride, 16 * mx, 16 * my, J.partitions, hc, nh, hpart, o, s_lds);
+      else
+        analyse_mb<false>(c, n, src, stride, 16 * mx, 16 * my, J.partitions, hc, nh, hpart, o, s_lds);
       store_out(J.out + (size_t)my * mbw + mx, o);
       // Publish the bottom row (blocks 12..15) for the row below.
       if (l < 4) {
@@ -1154,12 +1226,23 @@ __global__ void __launch_bounds__(64) search_frames_kernel(const DevJob *__restr
   }
 }
 
-// The full-search variant carries no hint code (fewer registers, higher
-// occupancy); the hinted variant runs jobs with and without hints.
-template __global__ void search_frames_kernel<false>(const DevJob *__restrict__, const int2 *__restrict__, int,
-                                                     uint32_t *, uint32_t);
-template __global__ void search_frames_kernel<true>(const DevJob *__restrict__, const int2 *__restrict__, int,
-                                                    uint32_t *, uint32_t);
+// The full-search kernel carries no hint code (165 VGPRs, 3 waves per
+// SIMD); the hinted kernel runs jobs with and without hints (its two MB
+// paths together need 2 waves per SIMD's registers; forcing 3 spills and
+// runs slower).
+__global__ void __launch_bounds__(64) search_frames_kernel(const DevJob *__restrict__ jobs,
+                                                           const int2 *__restrict__ tickets, int ntickets,
+                                                           uint32_t *ctl, uint32_t spin_limit) {
+  __shared__ uint32_t s_lds[LDS_PER_WAVE / 4];
+  frames_body<false>(jobs, tickets, ntickets, ctl, spin_limit, s_lds);
+}
+
+__global__ void __launch_bounds__(64)
+search_frames_hinted_kernel(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets, int ntickets,
+                            uint32_t *ctl, uint32_t spin_limit) {
+  __shared__ uint32_t s_lds[LDS_PER_WAVE / 4];
+  frames_body<true>(jobs, tickets, ntickets, ctl, spin_limit, s_lds);
+}
 
 // -------------------------------------------------- per-call search batch
 
@@ -1186,16 +1269,16 @@ __global__ void __launch_bounds__(64) search_batch_kernel(const pcme_search_job 
   const uint32_t *hc = nullptr;  // full search: no hints
   if (J.w == 16 && J.h == 16) {
     Lane L = lane_layout<16, 16>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
-    r = search<16, 16>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hc, -1, s_lds);
+    r = search<16, 16, false>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hc, -1, s_lds);
   } else if (J.w == 16 && J.h == 8) {
     Lane L = lane_layout<16, 8>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
-    r = search<16, 8>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hc, -1, s_lds);
+    r = search<16, 8, false>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hc, -1, s_lds);
   } else if (J.w == 8 && J.h == 16) {
     Lane L = lane_layout<8, 16>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
-    r = search<8, 16>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hc, -1, s_lds);
+    r = search<8, 16, false>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hc, -1, s_lds);
   } else {
     Lane L = lane_layout<8, 8>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
-    r = search<8, 8>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hc, -1, s_lds);
+    r = search<8, 8, false>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hc, -1, s_lds);
   }
   if (lane_id() == 0) {
     pcme_mv m;

@@ -94,15 +94,28 @@ __device__ __forceinline__ uint32_t input_px(const uint8_t *src_, const uint8_t 
   return lo | (hi << 16);
 }
 
+// Position of 16-byte channel chunk q (channels 8q..8q+7) inside pixel p of
+// a [pixel][32] bf16 map.  A 16-byte LDS read of a wave is served 32 lanes
+// (512 bytes of banks) at a time; the B-fragment lanes of one half-wave read
+// chunks q and q^1 of 16 pixels, so pixels p and p+8 -- 512 bytes apart --
+// would share banks.  The inference kernel therefore stores the chunks of
+// pixels with bit 3 set in the other half of the pixel (q ^ 2); the training
+// kernels keep the plain order.
+template <bool SW>
+__device__ __forceinline__ int chunk_at(int p, int q) {
+  return SW ? (q ^ (((p >> 3) & 1) << 1)) : q;
+}
+
 // B fragment of a forward conv: 8 K values of output pixel (oy, ox) for
 // K chunk `chunk` and lane group g.  x = one MB's input [HIN*HIN][CIN].
-template <int CIN, int HIN, int S>
+template <int CIN, int HIN, int S, bool SW = false>
 __device__ __forceinline__ u4 bfrag_fwd(const uint16_t *x, int oy, int ox, int chunk, int g) {
   u4 v = {0u, 0u, 0u, 0u};
   if constexpr (CIN == CH) {
     const int ty = chunk / 3, tx = chunk - 3 * ty;
     const int iy = oy * S + ty - 1, ix = ox * S + tx - 1;
-    if ((iy >= 0) & (iy < HIN) & (ix >= 0) & (ix < HIN)) v = *(const u4 *)(x + (iy * HIN + ix) * CH + 8 * g);
+    const int pi = iy * HIN + ix;
+    if ((iy >= 0) & (iy < HIN) & (ix >= 0) & (ix < HIN)) v = *(const u4 *)(x + pi * CH + 8 * chunk_at<SW>(pi, g));
   } else {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -117,10 +130,10 @@ __device__ __forceinline__ u4 bfrag_fwd(const uint16_t *x, int oy, int ox, int c
   return v;
 }
 
-// A fragment of the weights [32][KP] (bf16) for M tile mt.
-template <int KP>
+// A fragment of the weights [32][KP] (bf16, rows PITCH apart) for M tile mt.
+template <int KP, int PITCH = KP>
 __device__ __forceinline__ u4 afrag(const uint16_t *w, int mt, int chunk, int l) {
-  return *(const u4 *)(w + (mt * 16 + (l & 15)) * KP + chunk * 32 + 8 * (l >> 4));
+  return *(const u4 *)(w + (mt * 16 + (l & 15)) * PITCH + chunk * 32 + 8 * (l >> 4));
 }
 
 __device__ __forceinline__ f4 bias4(const float *b, int mt, int g) {
@@ -129,7 +142,7 @@ __device__ __forceinline__ f4 bias4(const float *b, int mt, int g) {
 
 // Output pixel tile of a forward conv for one MB: accumulators of both M
 // tiles for the lane's pixel (oy, ox).
-template <int CIN, int HIN, int S>
+template <int CIN, int HIN, int S, bool SW = false, int PITCH = (CIN == CH ? KC : K1)>
 __device__ __forceinline__ void conv_tile(const uint16_t *x, const uint16_t *w, const float *b, int oy, int ox,
                                           int l, f4 &acc0, f4 &acc1) {
   constexpr int KP = CIN == CH ? KC : K1, NCH = CIN == CH ? 9 : 1;
@@ -138,17 +151,21 @@ __device__ __forceinline__ void conv_tile(const uint16_t *x, const uint16_t *w, 
   acc1 = bias4(b, 1, g);
 #pragma unroll
   for (int c = 0; c < NCH; c++) {
-    const u4 B = bfrag_fwd<CIN, HIN, S>(x, oy, ox, c, g);
-    acc0 = mfma(afrag<KP>(w, 0, c, l), B, acc0);
-    acc1 = mfma(afrag<KP>(w, 1, c, l), B, acc1);
+    const u4 B = bfrag_fwd<CIN, HIN, S, SW>(x, oy, ox, c, g);
+    acc0 = mfma(afrag<KP, PITCH>(w, 0, c, l), B, acc0);
+    acc1 = mfma(afrag<KP, PITCH>(w, 1, c, l), B, acc1);
   }
 }
 
-__device__ __forceinline__ void store_bf(uint16_t *y, int g, f4 a0, f4 a1) {
+// ReLU + bf16 of the lane's 8 outputs (channels 4g..4g+3 and 16+4g..) of
+// pixel p; y = the map's base.
+template <bool SW = false>
+__device__ __forceinline__ void store_bf(uint16_t *y, int p, int g, f4 a0, f4 a1) {
   a0 = relu4(a0);
   a1 = relu4(a1);
-  *(u2 *)(y + 4 * g) = u2{pack2(a0[0], a0[1]), pack2(a0[2], a0[3])};
-  *(u2 *)(y + 16 + 4 * g) = u2{pack2(a1[0], a1[1]), pack2(a1[2], a1[3])};
+  uint16_t *px = y + p * CH;
+  *(u2 *)(px + 8 * chunk_at<SW>(p, g >> 1) + 4 * (g & 1)) = u2{pack2(a0[0], a0[1]), pack2(a0[2], a0[3])};
+  *(u2 *)(px + 8 * chunk_at<SW>(p, 2 + (g >> 1)) + 4 * (g & 1)) = u2{pack2(a1[0], a1[1]), pack2(a1[2], a1[3])};
 }
 
 // ------------------------------------------------------------- training
@@ -200,7 +217,7 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(const uint16_t *__restric
     *(f4 *)(yo + 4 * g) = relu4(a0);
     *(f4 *)(yo + 16 + 4 * g) = relu4(a1);
   } else {
-    store_bf((uint16_t *)y + (size_t)gp * CH, g, a0, a1);
+    store_bf((uint16_t *)y + (size_t)gp * CH, 0, g, a0, a1);
   }
 }
 
@@ -213,11 +230,18 @@ __device__ __forceinline__ float pool9(const float *a4, int co) {
   return s * (1.0f / 9.0f);
 }
 
-// fc logit of class c: fp32 fma chain over k = 0..31 from the bias.
-__device__ __forceinline__ float fc_logit(const float *gv, const uint16_t *wfrow, int kstride, float bias) {
+// fc logit of class c: fp32 fma chain over k = 0..31 from the bias; the
+// row's 32 bf16 weights are read as 16 dwords (scalar loads where the row is
+// wave-uniform).
+__device__ __forceinline__ float fc_logit(const float *gv, const uint16_t *wfrow, float bias) {
+  const uint32_t *w = (const uint32_t *)wfrow;
   float acc = bias;
 #pragma unroll
-  for (int k = 0; k < CH; k++) acc = __builtin_fmaf(gv[k], bf2f(wfrow[k * kstride]), acc);
+  for (int k = 0; k < CH; k += 2) {
+    const uint32_t d = w[k >> 1];
+    acc = __builtin_fmaf(gv[k], __builtin_bit_cast(float, d << 16), acc);
+    acc = __builtin_fmaf(gv[k + 1], __builtin_bit_cast(float, d & 0xFFFF0000u), acc);
+  }
   return acc;
 }
 
@@ -233,7 +257,7 @@ __global__ void __launch_bounds__(256) fc_kernel(const float *__restrict__ gv, c
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= n * NCLS) return;
   const int i = t / NCLS, c = t - i * NCLS;
-  logits[t] = fc_logit(gv + (size_t)i * CH, wfb + (size_t)c * CH, 1, bf[c]);
+  logits[t] = fc_logit(gv + (size_t)i * CH, wfb + (size_t)c * CH, bf[c]);
 }
 
 // Softmax cross-entropy of both heads, one thread per MB: loss terms and
@@ -474,48 +498,32 @@ struct NetJob {
   int64_t mb0;  // first global MB index of the job
 };
 
-// LDS of the fused kernel: the weights (shared by the block) and per wave
-// two activation regions (R0: input / conv2 out / conv4 out; R1: conv1 out /
-// conv3 out), the pooled vector and the logits.
-constexpr int HW = 4;  // waves per block
+// Inference runs in two kernels.  conv_kernel: each wave takes one MB at
+// a time through conv1-4 and the pool (8 waves per block share the conv
+// weights in LDS; each wave owns two activation regions, R0: input / conv2
+// out / conv4 out, R1: conv1 out / conv3 out) and writes the 32 pooled
+// values.  heads_kernel: one lane per MB runs the fc layer (the fc weights
+// are wave-uniform: scalar loads), an online softmax of each head and a
+// running top-k of the motion logits, and writes the hint.
+constexpr int CW = 8;                           // waves per conv block
+constexpr int KCP = KC + 16;                    // LDS pitch of a conv2-4 weight row: 608 bytes put the
+                                                // 16 rows of an A fragment on distinct 32-byte bank slots
 constexpr int L_W1 = 0;                         // u16 [32][32]
-constexpr int L_W2 = L_W1 + CH * K1 * 2;        // u16 [32][288]
-constexpr int L_W3 = L_W2 + CH * KC * 2;
-constexpr int L_W4 = L_W3 + CH * KC * 2;
-constexpr int L_WF = L_W4 + CH * KC * 2;        // u16 [32][304] (k-major)
-constexpr int L_B = L_WF + CH * NCLS_PAD * 2;   // f32 b1..b4 [4][32], bf [304]
-constexpr int L_WAVE = L_B + (4 * CH + NCLS_PAD) * 4;
+constexpr int L_W2 = L_W1 + CH * K1 * 2;        // u16 [32][KCP]
+constexpr int L_W3 = L_W2 + CH * KCP * 2;
+constexpr int L_W4 = L_W3 + CH * KCP * 2;
+constexpr int L_B = L_W4 + CH * KCP * 2;        // f32 b1..b4 [4][32]
+constexpr int L_ZERO = L_B + 4 * CH * 4;         // 1 KB of zeros: out-of-bounds taps (one slot per lane)
+constexpr int L_WAVE = L_ZERO + 1024;
 constexpr int R0_B = IN * IN * 2 * 2;           // 2304 (>= 36*32*2 and 9*32*4)
 constexpr int R1_B = 144 * CH * 2;              // 9216
-constexpr int G_B = CH * 4;
-constexpr int Z_B = NCLS_PAD * 4;
-constexpr int WAVE_B = R0_B + R1_B + G_B + Z_B;
-constexpr int HINTS_LDS = L_WAVE + HW * WAVE_B;
-static_assert(HINTS_LDS <= 160 * 1024, "LDS budget");
+constexpr int WAVE_B = R0_B + R1_B;
+constexpr int CONV_LDS = L_WAVE + CW * WAVE_B;
+static_assert(CONV_LDS <= 160 * 1024, "LDS budget");
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
-}
-
-// Monotonic u32 key of a float (larger float -> larger key).
-__device__ __forceinline__ uint32_t fkey(float f) {
-  const uint32_t u = __builtin_bit_cast(uint32_t, f);
-  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-
-__device__ __forceinline__ uint64_t wave_max64(uint64_t v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) {
-    const uint64_t t = __shfl_xor(v, o, 64);
-    v = t > v ? t : v;
-  }
-  return v;
-}
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
 }
 
 __device__ __forceinline__ uint16_t q16(float p) {
@@ -523,10 +531,96 @@ __device__ __forceinline__ uint16_t q16(float p) {
   return (uint16_t)(s >= 65535.f ? 65535u : (s <= 0.f ? 0u : (uint32_t)s));
 }
 
-// conv2 / conv3 of one MB in the fused kernel: 36 output pixels in three
-// tiles, X -> Y (both [pixel][32] bf16 in this wave's LDS).
-template <int HIN, int S>
-__device__ __forceinline__ void conv_mid(const uint16_t *X, uint16_t *Y, const uint16_t *W, const float *b, int l) {
+// bf16 bits of (v - 128) / 64 for a byte v held in bits [8k, 8k+8) .. of a
+// 16-bit lane: exact (7 significant bits), the low half of the fp32 is zero.
+__device__ __forceinline__ uint32_t ds_bf(uint32_t v) {
+  return __builtin_bit_cast(uint32_t, (float)((int)v - 128) * 0.015625f) >> 16;
+}
+
+// Two input pixels (2c, 2c+1) of input row r from two dwords of each of two
+// rows: out = ((a0 + a1 + b0 + b1 + 2) >> 2 per byte pair).
+__device__ __forceinline__ uint32_t ds_pair(uint32_t a, uint32_t b) {
+  const uint32_t s = (a & 0x00FF00FFu) + ((a >> 8) & 0x00FF00FFu) + (b & 0x00FF00FFu) + ((b >> 8) & 0x00FF00FFu) +
+                     0x00020002u;
+  return ((s >> 2) & 0x3FFFu) | (((s >> 18) & 0x3FFFu) << 16);
+}
+
+// Byte offset of chunk q of pixel k (k = pixel index of the stored order).
+__device__ __forceinline__ int map_at(int k, int q) { return k * 64 + 16 * chunk_at<true>(k, q); }
+
+// Per-lane read table of the fused convolutions: the LDS byte offset / 4 of
+// every B-fragment read of one MB (the same for every MB a wave runs), two
+// u16 entries per dword.  Out-of-bounds taps and unused pixels read zeros at
+// L_ZERO, each lane its own slot (reads of one address by many lanes
+// conflict).  Entries: conv1 (tile t, tap group q) at 4t+q, conv2 (t, chunk
+// c) at T_C2 + 9t + c, conv3 at T_C3 + 9t + c, conv4 (c) at T_C4 + c.
+constexpr int T_C2 = 36, T_C3 = 63, T_C4 = 90, T_N = 99, T_W = (T_N + 1) / 2;
+
+__device__ __forceinline__ uint32_t tab(const uint32_t *tb, int i) {
+  return ((tb[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu) << 2;
+}
+
+// Entry of a B-fragment read: input pixel (iy, ix) of a HIN x HIN map at
+// `base` (raster order).
+__device__ __forceinline__ uint32_t tab_entry(int base, int hin, int iy, int ix, bool ok, int l) {
+  const int g = l >> 4;
+  ok = ok & (iy >= 0) & (iy < hin) & (ix >= 0) & (ix < hin);
+  const int k = iy * hin + ix;
+  return (uint32_t)(ok ? base + map_at(k, g) : L_ZERO + map_at(l & 15, g)) >> 2;
+}
+
+__device__ __forceinline__ void build_table(int l, int r0, int r1, uint32_t *tb) {
+  const int g = l >> 4;
+  uint32_t e[T_W * 2];
+#pragma unroll
+  for (int t = 0; t < 9; t++) {
+    const int p = t * 16 + (l & 15), oy = p / 12, ox = p - 12 * (p / 12);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int tap = 4 * g + q, ty = tap / 3, tx = tap - 3 * (tap / 3);
+      const int iy = 2 * oy + ty - 1, ix = 2 * ox + tx - 1;
+      const bool ok = (tap < 9) & (iy >= 0) & (iy < IN) & (ix >= 0) & (ix < IN);
+      e[4 * t + q] = (uint32_t)(ok ? r0 + 4 * (iy * IN + ix) : L_ZERO + 4 * l) >> 2;
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 3; t++) {
+    const int p = t * 16 + (l & 15), oy = p / 6, ox = p - 6 * (p / 6);
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+      const int ty = c / 3, tx = c - 3 * (c / 3);
+      e[T_C2 + 9 * t + c] = tab_entry(r1, 12, 2 * oy + ty - 1, 2 * ox + tx - 1, p < 36, l);
+      e[T_C3 + 9 * t + c] = tab_entry(r0, 6, oy + ty - 1, ox + tx - 1, p < 36, l);
+    }
+  }
+  {
+    const int p = l & 15, oy = p / 3, ox = p - 3 * (p / 3);
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+      const int ty = c / 3, tx = c - 3 * (c / 3);
+      e[T_C4 + c] = tab_entry(r1, 6, 2 * oy + ty - 1, 2 * ox + tx - 1, p < 9, l);
+    }
+  }
+  e[T_N] = 0u;
+#pragma unroll
+  for (int i = 0; i < T_W; i++) tb[i] = e[2 * i] | (e[2 * i + 1] << 16);
+}
+
+// ReLU + bf16 of the lane's 8 outputs (channels 4g..4g+3 and 16+4g..) of a
+// pixel stored at index k of a fused-kernel map (swizzled chunks).
+__device__ __forceinline__ void store_map(uint16_t *y, int k, int g, f4 a0, f4 a1) {
+  a0 = relu4(a0);
+  a1 = relu4(a1);
+  uint8_t *b = (uint8_t *)y;
+  *(u2 *)(b + map_at(k, g >> 1) + 8 * (g & 1)) = u2{pack2(a0[0], a0[1]), pack2(a0[2], a0[3])};
+  *(u2 *)(b + map_at(k, 2 + (g >> 1)) + 8 * (g & 1)) = u2{pack2(a1[0], a1[1]), pack2(a1[2], a1[3])};
+}
+
+// conv2 / conv3 of one MB: 36 output pixels in three tiles, reads through
+// the table from entry T0, writes Y ([pixel][32] bf16, swizzled).
+template <int T0>
+__device__ __forceinline__ void conv_mid(const uint8_t *lds, const uint32_t *tb, uint16_t *Y, const uint16_t *W,
+                                         const float *b, int l) {
   const int g = l >> 4;
   f4 acc[3][2];
 #pragma unroll
@@ -534,60 +628,97 @@ __device__ __forceinline__ void conv_mid(const uint16_t *X, uint16_t *Y, const u
     acc[t][0] = bias4(b, 0, g);
     acc[t][1] = bias4(b, 1, g);
   }
-#pragma unroll 1
+#pragma unroll
   for (int c = 0; c < 9; c++) {
-    const u4 A0 = afrag<KC>(W, 0, c, l), A1 = afrag<KC>(W, 1, c, l);
+    const u4 A0 = afrag<KC, KCP>(W, 0, c, l), A1 = afrag<KC, KCP>(W, 1, c, l);
 #pragma unroll
     for (int t = 0; t < 3; t++) {
-      const int p = t * 16 + (l & 15), oy = p / 6, ox = p - oy * 6;
-      u4 B = {0u, 0u, 0u, 0u};
-      if (p < 36) B = bfrag_fwd<CH, HIN, S>(X, oy, ox, c, g);
+      const u4 B = *(const u4 *)(lds + tab(tb, T0 + 9 * t + c));
       acc[t][0] = mfma(A0, B, acc[t][0]);
       acc[t][1] = mfma(A1, B, acc[t][1]);
     }
   }
-  wave_sync();  // every read of X precedes the writes of Y
+  wave_sync();  // every read of the input precedes the writes of Y
 #pragma unroll
   for (int t = 0; t < 3; t++) {
     const int p = t * 16 + (l & 15);
-    if (p < 36) store_bf(Y + p * CH, g, acc[t][0], acc[t][1]);
+    if (p < 36) store_map(Y, p, g, acc[t][0], acc[t][1]);
   }
   wave_sync();
 }
 
-__global__ void __launch_bounds__(256) hints_kernel(const NetJob *__restrict__ jobs, int njobs, int64_t total,
-                                                    const uint16_t *__restrict__ w1b, const uint16_t *__restrict__ w2b,
-                                                    const uint16_t *__restrict__ w3b, const uint16_t *__restrict__ w4b,
-                                                    const uint16_t *__restrict__ wfb, const float *__restrict__ prm,
-                                                    int ncand, float thr_part, float thr_mv) {
+typedef const __attribute__((address_space(1))) uint32_t gu32c;
+
+// Raw bytes of one MB's network input: item (r, c) of the lane = input row r
+// (0..23), dword c (0..11) of the 48x48 reference window (input pixels 2c,
+// 2c+1), rows 2r and 2r+1; the source MB likewise where the item lies in
+// rows/cols 8..15.
+struct InputRegs {
+  uint32_t ra[5], rb[5], sa[5], sb[5];
+};
+
+__device__ __forceinline__ void load_input(const NetJob &J, int li, int l, InputRegs &in) {
+  const int my = li / J.mbw, mx = li - my * J.mbw;
+  const int sdw = (16 * J.mbw + 2 * PCME_PAD) >> 2;  // stride in dwords
+  gu32c *ref = (gu32c *)J.ref;
+  gu32c *src = (gu32c *)J.src;
+  const int64_t rb = (int64_t)(16 * my - 16 + PCME_PAD) * sdw + ((16 * mx - 16 + PCME_PAD) >> 2);
+  const int64_t sb = (int64_t)(16 * my + PCME_PAD) * sdw + ((16 * mx + PCME_PAD) >> 2);
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const int it = min(l + 64 * q, 287);
+    const int r = it / 12, c = it - 12 * (it / 12);
+    in.ra[q] = ref[rb + (int64_t)(2 * r) * sdw + c];
+    in.rb[q] = ref[rb + (int64_t)(2 * r + 1) * sdw + c];
+    const int rs = min(max(r - 8, 0), 7), cs = min(max(c - 4, 0), 3);
+    in.sa[q] = src[sb + (int64_t)(2 * rs) * sdw + cs];
+    in.sb[q] = src[sb + (int64_t)(2 * rs + 1) * sdw + cs];
+  }
+}
+
+// The 24x24x2 input map (one dword per pixel: channel 0 | channel 1 << 16).
+__device__ __forceinline__ void store_input(const InputRegs &in, int l, uint16_t *R0) {
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const int it = l + 64 * q;
+    if (it < 288) {
+      const int r = it / 12, c = it - 12 * (it / 12);
+      const uint32_t d = ds_pair(in.ra[q], in.rb[q]);
+      uint32_t e0 = 0u, e1 = 0u;
+      if ((r >= 8) & (r < 16) & (c >= 4) & (c < 8)) {
+        const uint32_t e = ds_pair(in.sa[q], in.sb[q]);
+        e0 = ds_bf(e & 0xFFFFu) << 16;
+        e1 = ds_bf(e >> 16) << 16;
+      }
+      *(u2 *)(R0 + (r * IN + 2 * c) * 2) = u2{ds_bf(d & 0xFFFFu) | e0, ds_bf(d >> 16) | e1};
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64 * CW) conv_kernel(const NetJob *__restrict__ jobs, int njobs, int64_t total,
+                                                       const uint16_t *__restrict__ w1b,
+                                                       const uint16_t *__restrict__ w2b,
+                                                       const uint16_t *__restrict__ w3b,
+                                                       const uint16_t *__restrict__ w4b,
+                                                       const float *__restrict__ prm, float *__restrict__ gout) {
   extern __shared__ __attribute__((aligned(16))) uint8_t lds[];
   const int tid = (int)threadIdx.x;
-  // Stage the weights (all waves), then each wave walks its MBs.
   {
     const u4 *s;
     u4 *d;
     s = (const u4 *)w1b; d = (u4 *)(lds + L_W1);
-    for (int i = tid; i < CH * K1 * 2 / 16; i += 256) d[i] = s[i];
-    s = (const u4 *)w2b; d = (u4 *)(lds + L_W2);
-    for (int i = tid; i < CH * KC * 2 / 16; i += 256) d[i] = s[i];
-    s = (const u4 *)w3b; d = (u4 *)(lds + L_W3);
-    for (int i = tid; i < CH * KC * 2 / 16; i += 256) d[i] = s[i];
-    s = (const u4 *)w4b; d = (u4 *)(lds + L_W4);
-    for (int i = tid; i < CH * KC * 2 / 16; i += 256) d[i] = s[i];
-    uint16_t *wf = (uint16_t *)(lds + L_WF);
-    for (int i = tid; i < CH * NCLS_PAD; i += 256) {
-      const int k = i / NCLS_PAD, c = i - k * NCLS_PAD;
-      wf[i] = c < NCLS ? wfb[c * CH + k] : (uint16_t)0;
+    for (int i = tid; i < CH * K1 * 2 / 16; i += 64 * CW) d[i] = s[i];
+    // conv2-4 rows: 36 u4 of weights each, placed KCP apart
+    constexpr int RU = KC * 2 / 16, PU = KCP * 2 / 16;
+    for (int i = tid; i < 3 * CH * RU; i += 64 * CW) {
+      const int k = i / (CH * RU), r = (i / RU) % CH, j = i % RU;
+      const u4 *w = (const u4 *)(k == 0 ? w2b : (k == 1 ? w3b : w4b));
+      ((u4 *)(lds + L_W2))[k * CH * PU + r * PU + j] = w[r * RU + j];
     }
     float *bb = (float *)(lds + L_B);
-    for (int i = tid; i < 4 * CH + NCLS_PAD; i += 256) {
-      float v = 0.f;
-      if (i < CH) v = prm[OB1 + i];
-      else if (i < 2 * CH) v = prm[OB2 + i - CH];
-      else if (i < 3 * CH) v = prm[OB3 + i - 2 * CH];
-      else if (i < 4 * CH) v = prm[OB4 + i - 3 * CH];
-      else if (i - 4 * CH < NCLS) v = prm[OBF + i - 4 * CH];
-      bb[i] = v;
+    for (int i = tid; i < 4 * CH; i += 64 * CW) {
+      const int k = i / CH;
+      bb[i] = prm[(k == 0 ? OB1 : k == 1 ? OB2 : k == 2 ? OB3 : OB4) + (i - k * CH)];
     }
   }
   __syncthreads();
@@ -595,114 +726,143 @@ __global__ void __launch_bounds__(256) hints_kernel(const NetJob *__restrict__ j
   const uint16_t *W2 = (const uint16_t *)(lds + L_W2);
   const uint16_t *W3 = (const uint16_t *)(lds + L_W3);
   const uint16_t *W4 = (const uint16_t *)(lds + L_W4);
-  const uint16_t *WF = (const uint16_t *)(lds + L_WF);
   const float *BB = (const float *)(lds + L_B);
   const int wv = tid >> 6, l = tid & 63, g = l >> 4;
   uint8_t *wbase = lds + L_WAVE + wv * WAVE_B;
   uint16_t *R0 = (uint16_t *)wbase;
   uint16_t *R1 = (uint16_t *)(wbase + R0_B);
-  float *GV = (float *)(wbase + R0_B + R1_B);
-  float *Z = (float *)(wbase + R0_B + R1_B + G_B);
-  const int64_t nw = (int64_t)gridDim.x * HW;
+  if (tid < 256) ((uint32_t *)(lds + L_ZERO))[tid] = 0u;
+  uint32_t tb[T_W];
+  build_table(l, L_WAVE + wv * WAVE_B, L_WAVE + wv * WAVE_B + R0_B, tb);
+  __syncthreads();  // the zero block
+  const int64_t nw = (int64_t)gridDim.x * CW;
+  // The input loads of the next MB are issued before the current MB's
+  // convolutions (software pipelining over the wave's MB sequence).
+  InputRegs in;
   int jb = 0;
-  for (int64_t m = (int64_t)blockIdx.x * HW + wv; m < total; m += nw) {
+  int64_t m = (int64_t)blockIdx.x * CW + wv;
+  if (m < total) {
     while (jb + 1 < njobs && m >= jobs[jb + 1].mb0) jb++;
-    const NetJob &J = jobs[jb];
-    const int li = (int)(m - J.mb0);
-    const int my = li / J.mbw, mx = li - my * J.mbw;
-    const int stride = 16 * J.mbw + 2 * PCME_PAD;
-    // input -> R0 [24*24][2]
-    for (int i = l; i < IN * IN; i += 64)
-      ((uint32_t *)R0)[i] = input_px(J.src, J.ref, stride, 16 * mx, 16 * my, i / IN, i % IN);
-    wave_sync();
-    // conv1: R0 -> R1 [144][32]
-#pragma unroll 1
-    for (int t = 0; t < 9; t++) {
-      const int p = t * 16 + (l & 15), oy = p / 12, ox = p - oy * 12;
-      f4 a0, a1;
-      conv_tile<2, 24, 2>(R0, W1, BB, oy, ox, l, a0, a1);
-      store_bf(R1 + p * CH, g, a0, a1);
+    load_input(jobs[jb], (int)(m - jobs[jb].mb0), l, in);
+  }
+  for (; m < total; m += nw) {
+    store_input(in, l, R0);
+    const int64_t mn = m + nw;
+    if (mn < total) {
+      while (jb + 1 < njobs && mn >= jobs[jb + 1].mb0) jb++;
+      load_input(jobs[jb], (int)(mn - jobs[jb].mb0), l, in);
     }
     wave_sync();
-    // conv2: R1 -> R0 [36][32]; conv3: R0 -> R1.
-    conv_mid<12, 2>(R1, R0, W2, BB + CH, l);
-    conv_mid<6, 1>(R0, R1, W3, BB + 2 * CH, l);
-    // conv4: R1 -> fp32 [9][32] in R0
+    // conv1: R0 -> R1 [144][32]
     {
-      const int p = l & 15, oy = p / 3, ox = p - oy * 3;
-      f4 a0, a1;
-      conv_tile<CH, 6, 2>(R1, W4, BB + 3 * CH, p < 9 ? oy : 0, p < 9 ? ox : 0, l, a0, a1);
-      wave_sync();
+      const u4 A0 = afrag<K1>(W1, 0, 0, l), A1 = afrag<K1>(W1, 1, 0, l);
+#pragma unroll
+      for (int t = 0; t < 9; t++) {
+        const int p = t * 16 + (l & 15);
+        u4 B;
+#pragma unroll
+        for (int q = 0; q < 4; q++) B[q] = *(const uint32_t *)(lds + tab(tb, 4 * t + q));
+        const f4 a0 = mfma(A0, B, bias4(BB, 0, g));
+        const f4 a1 = mfma(A1, B, bias4(BB, 1, g));
+        store_map(R1, p, g, a0, a1);
+      }
+    }
+    wave_sync();
+    conv_mid<T_C2>(lds, tb, R0, W2, BB + CH, l);
+    conv_mid<T_C3>(lds, tb, R1, W3, BB + 2 * CH, l);
+    // conv4: R1 -> fp32 [9][32] in R0, then the pool
+    {
+      const int p = l & 15;
+      f4 a0 = bias4(BB + 3 * CH, 0, g), a1 = bias4(BB + 3 * CH, 1, g);
+#pragma unroll
+      for (int c = 0; c < 9; c++) {
+        const u4 B = *(const u4 *)(lds + tab(tb, T_C4 + c));
+        a0 = mfma(afrag<KC, KCP>(W4, 0, c, l), B, a0);
+        a1 = mfma(afrag<KC, KCP>(W4, 1, c, l), B, a1);
+      }
       float *A4 = (float *)R0;
       if (p < 9) {
         *(f4 *)(A4 + p * CH + 4 * g) = relu4(a0);
         *(f4 *)(A4 + p * CH + 16 + 4 * g) = relu4(a1);
       }
       wave_sync();
-      if (l < CH) GV[l] = pool9(A4, l);
+      if (l < CH) gout[m * CH + l] = pool9(A4, l);
       wave_sync();
     }
-    // fc: lane handles classes l + 64j
-    float z[5];
-#pragma unroll
-    for (int j = 0; j < 5; j++) {
-      const int c = l + 64 * j;
-      z[j] = c < NCLS ? fc_logit(GV, WF + c, NCLS_PAD, BB[4 * CH + c]) : 0.f;
-    }
-    // motion head: max, softmax mass, top-k (logit desc, class asc)
-#pragma unroll
-    for (int j = 0; j < 5; j++)
-      if (l + 64 * j < NCLS_PAD) Z[l + 64 * j] = z[j];
-    wave_sync();
-    uint64_t kv[5];
-#pragma unroll
-    for (int j = 0; j < 5; j++) {
-      const int c = l + 64 * j;
-      kv[j] = c < MVC ? (((uint64_t)fkey(z[j]) << 32) | (uint32_t)(0xFFFFFFFFu - (uint32_t)c)) : 0ull;
-    }
-    uint64_t lm = kv[0];
-#pragma unroll
-    for (int j = 1; j < 5; j++) lm = kv[j] > lm ? kv[j] : lm;
-    const uint64_t top = wave_max64(lm);
-    const int ctop = (int)(0xFFFFFFFFu - (uint32_t)top);
-    const float zmax = Z[ctop];
-    float es = 0.f;
-#pragma unroll
-    for (int j = 0; j < 5; j++)
-      if (l + 64 * j < MVC) es += expf(z[j] - zmax);
-    const float ssum = wave_sum(es);
-    uint32_t cand = 0u;  // lane k < ncand: packed MV of the k-th class
-    float mass = 0.f;
-    for (int k = 0; k < ncand; k++) {
-      uint64_t b = kv[0];
-#pragma unroll
-      for (int j = 1; j < 5; j++) b = kv[j] > b ? kv[j] : b;
-      const uint64_t w = wave_max64(b);
-      const int c = (int)(0xFFFFFFFFu - (uint32_t)w);
-      const float zc = Z[c];
-      mass += expf(zc - zmax);
-#pragma unroll
-      for (int j = 0; j < 5; j++)
-        if (c == l + 64 * j) kv[j] = 0ull;
-      const int cy = c / PCME_NET_MV_SIDE, cx = c - cy * PCME_NET_MV_SIDE;
-      if (l == k) cand = ((uint32_t)(8 * cx - 64) & 0xFFFFu) | ((uint32_t)(8 * cy - 64) << 16);
-    }
-    const float conf_mv = mass / ssum;
-    // partition head (every lane computes the same)
-    const float q0 = Z[MVC], q1 = Z[MVC + 1], q2 = Z[MVC + 2], q3 = Z[MVC + 3];
-    int pa = 0;
-    float qm = q0;
-    if (q1 > qm) { qm = q1; pa = 1; }
-    if (q2 > qm) { qm = q2; pa = 2; }
-    if (q3 > qm) { qm = q3; pa = 3; }
-    const float conf_part = 1.f / (expf(q0 - qm) + expf(q1 - qm) + expf(q2 - qm) + expf(q3 - qm));
-    const bool prune = (conf_part >= thr_part) & (conf_mv >= thr_mv);
-    __attribute__((address_space(1))) uint32_t *hw = (__attribute__((address_space(1))) uint32_t *)(J.hint + li);
-    if (l < 8) hw[l] = l < ncand ? cand : 0u;
-    if (l == 8) hw[8] = (uint32_t)q16(conf_part) | ((uint32_t)q16(conf_mv) << 16);
-    if (l == 9) hw[9] = (uint32_t)pa | ((uint32_t)ncand << 8) | ((prune ? 1u : 0u) << 16);
-    wave_sync();
   }
+}
+
+// The fc layer, both heads and the hint of one MB per lane.  z is computed
+// class by class with the same fma chain as fc_kernel; the motion head keeps
+// a running max / sum of exp and the k best (logit desc, class asc: classes
+// arrive in ascending order and only a strictly larger logit moves ahead).
+__global__ void __launch_bounds__(256) heads_kernel(const NetJob *__restrict__ jobs, int njobs, int64_t total,
+                                                    const float *__restrict__ gin, const uint16_t *__restrict__ wfb,
+                                                    const float *__restrict__ prm, int ncand, float thr_part,
+                                                    float thr_mv) {
+  const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (m >= total) return;
+  int jb = 0;
+  while (jb + 1 < njobs && m >= jobs[jb + 1].mb0) jb++;
+  float gv[CH];
+#pragma unroll
+  for (int k = 0; k < CH; k += 4) {
+    const f4 v = *(const f4 *)(gin + m * CH + k);
+    gv[k] = v[0]; gv[k + 1] = v[1]; gv[k + 2] = v[2]; gv[k + 3] = v[3];
+  }
+  const float *bf = prm + OBF;
+  float zmax = -INFINITY, ssum = 0.f;
+  float tz[8];
+  int tc[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) { tz[i] = -INFINITY; tc[i] = 0; }
+  for (int c = 0; c < MVC; c++) {
+    const float z = fc_logit(gv, wfb + c * CH, bf[c]);
+    // online softmax: ssum = sum exp(z_i - zmax)
+    if (z > zmax) {
+      ssum = ssum * expf(zmax - z) + 1.f;
+      zmax = z;
+    } else {
+      ssum += expf(z - zmax);
+    }
+#pragma unroll
+    for (int i = 7; i >= 1; i--) {
+      const bool above = z > tz[i - 1];
+      const bool here = !above && z > tz[i];
+      tz[i] = above ? tz[i - 1] : (here ? z : tz[i]);
+      tc[i] = above ? tc[i - 1] : (here ? c : tc[i]);
+    }
+    const bool h0 = z > tz[0];
+    tz[0] = h0 ? z : tz[0];
+    tc[0] = h0 ? c : tc[0];
+  }
+  float q[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) q[i] = fc_logit(gv, wfb + (MVC + i) * CH, bf[MVC + i]);
+  int pa = 0;
+  float qm = q[0];
+  if (q[1] > qm) { qm = q[1]; pa = 1; }
+  if (q[2] > qm) { qm = q[2]; pa = 2; }
+  if (q[3] > qm) { qm = q[3]; pa = 3; }
+  const float conf_part = 1.f / (expf(q[0] - qm) + expf(q[1] - qm) + expf(q[2] - qm) + expf(q[3] - qm));
+  float mass = 0.f;
+  uint32_t cw[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const bool on = i < ncand;
+    mass += on ? expf(tz[i] - zmax) : 0.f;
+    const int cy = tc[i] / PCME_NET_MV_SIDE, cx = tc[i] - cy * PCME_NET_MV_SIDE;
+    cw[i] = on ? (((uint32_t)(8 * cx - 64) & 0xFFFFu) | ((uint32_t)(8 * cy - 64) << 16)) : 0u;
+  }
+  const float conf_mv = mass / ssum;
+  const bool prune = (conf_part >= thr_part) & (conf_mv >= thr_mv);
+  const NetJob &J = jobs[jb];
+  __attribute__((address_space(1))) uint32_t *hw =
+      (__attribute__((address_space(1))) uint32_t *)(J.hint + (m - J.mb0));
+#pragma unroll
+  for (int i = 0; i < 8; i++) hw[i] = cw[i];
+  hw[8] = (uint32_t)q16(conf_part) | ((uint32_t)q16(conf_mv) << 16);
+  hw[9] = (uint32_t)pa | ((uint32_t)ncand << 8) | ((prune ? 1u : 0u) << 16);
 }
 
 }  // namespace cnn
@@ -759,7 +919,9 @@ struct pcme_net {
   float *part1 = nullptr, *part2 = nullptr, *part3 = nullptr, *part4 = nullptr;
   NetJob *djobs = nullptr;
   std::vector<NetJob> hjobs;
-  int hints_blocks = 0;
+  int conv_blocks = 0;
+  float *gbuf = nullptr;  // pooled vectors of the last inference (fp32 [MBs][32])
+  int64_t gcap = 0;
   std::vector<void *> allocs;
 };
 
@@ -779,6 +941,7 @@ void net_free(pcme_net *n) {
   if (!n) return;
   for (void *p : n->allocs) (void)hipFree(p);
   if (n->djobs) (void)hipFree(n->djobs);
+  if (n->gbuf) (void)hipFree(n->gbuf);
   delete n;
 }
 
@@ -928,18 +1091,18 @@ int pcme_net_create(uint64_t seed, int32_t max_batch, pcme_net **out) {
     return rc;
   }
   int per_cu = 0;
-  if (hipFuncSetAttribute((const void *)hints_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, HINTS_LDS) !=
+  if (hipFuncSetAttribute((const void *)conv_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, CONV_LDS) !=
           hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, hints_kernel, 256, HINTS_LDS) != hipSuccess) {
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, conv_kernel, 64 * CW, CONV_LDS) != hipSuccess) {
     net_free(n);
-    return nfail(PCME_EHIP, "hints kernel set-up failed");
+    return nfail(PCME_EHIP, "conv kernel set-up failed");
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, n->dev) != hipSuccess) {
     net_free(n);
     return nfail(PCME_EHIP, "hipGetDeviceProperties failed");
   }
-  n->hints_blocks = std::max(1, per_cu) * prop.multiProcessorCount;
+  n->conv_blocks = std::max(1, per_cu) * prop.multiProcessorCount;
   *out = n;
   return PCME_OK;
 }
@@ -1050,10 +1213,21 @@ int pcme_net_hints(pcme_net *net, const pcme_frame_job *jobs, int32_t n, int32_t
     NCHK(hipMemcpy(net->djobs, hj.data(), hj.size() * sizeof(NetJob), hipMemcpyHostToDevice));
     net->hjobs = hj;
   }
-  const int64_t need = (total + HW - 1) / HW;
-  const unsigned grid = (unsigned)std::min<int64_t>(need, net->hints_blocks);
-  hipLaunchKernelGGL(hints_kernel, dim3(grid), dim3(256), HINTS_LDS, s, net->djobs, n, total, net->w1b, net->w2b,
-                     net->w3b, net->w4b, net->wfb, net->master, ncand, thr_part, thr_mv);
+  if (total > net->gcap) {
+    NCHK(hipStreamSynchronize(s));
+    if (net->gbuf) NCHK(hipFree(net->gbuf));
+    net->gbuf = nullptr;
+    net->gcap = 0;
+    if (hipMalloc((void **)&net->gbuf, (size_t)total * CH * sizeof(float)) != hipSuccess)
+      return nfail(PCME_ENOMEM, "pooled-vector buffer of %lld MBs failed", (long long)total);
+    net->gcap = total;
+  }
+  const int64_t need = (total + CW - 1) / CW;
+  const unsigned grid = (unsigned)std::min<int64_t>(need, net->conv_blocks);
+  hipLaunchKernelGGL(conv_kernel, dim3(grid), dim3(64 * CW), CONV_LDS, s, net->djobs, n, total, net->w1b, net->w2b,
+                     net->w3b, net->w4b, net->master, net->gbuf);
+  hipLaunchKernelGGL(heads_kernel, dim3(blocks(total, 256)), dim3(256), 0, s, net->djobs, n, total, net->gbuf,
+                     net->wfb, net->master, ncand, thr_part, thr_mv);
   NCHK(hipGetLastError());
   return PCME_OK;
 }
