@@ -55,6 +55,14 @@ __device__ __forceinline__ uint32_t red4(uint32_t v) {
   return v;
 }
 
+// Sums of a paired search whose lower half runs 4 lanes per candidate and
+// upper half 2 (a 16x8 or 8x16 search beside an 8x8 one).
+__device__ __forceinline__ uint32_t red42(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_QP_1032, 0xF, 0xF, false);
+  const uint32_t w = v + (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_QP_2301, 0xF, 0xF, false);
+  return __lane_id() < 32 ? w : v;
+}
+
 __device__ __forceinline__ uint32_t rdlane(uint32_t v, int l) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
 }
@@ -260,7 +268,7 @@ __device__ __forceinline__ uint32_t sad_cost(const SCtx &c, const Lane &L, const
     acc = __builtin_amdgcn_sad_u8(b, s.raw[2 * r + 1], acc);
   }
   acc = L.active ? acc : 0u;
-  acc = LPC == 8 ? red8(acc) : red4(acc);
+  acc = LPC == 8 ? red8(acc) : (LPC == 4 ? red4(acc) : red42(acc));
   return acc + ((c.lam * (se_bits(cx - mvpx) + se_bits(cy - mvpy))) >> 8);
 }
 
@@ -292,7 +300,7 @@ __device__ __forceinline__ uint32_t satd_cost(const SCtx &c, const Lane &L, cons
   }
   uint32_t acc = satd_strip8(sp, P);
   acc = L.active ? acc : 0u;
-  acc = LPC == 8 ? red8(acc) : red4(acc);
+  acc = LPC == 8 ? red8(acc) : (LPC == 4 ? red4(acc) : red42(acc));
   return acc + ((c.lam * (se_bits(cx - mvpx) + se_bits(cy - mvpy))) >> 8);
 }
 
@@ -549,12 +557,15 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
 }
 
 // ---- paired search ---------------------------------------------------------
-// The 16x8 and 8x16 modes are independent of each other (both seeded with
-// mv16 and predicted from the same committed neighbours), so partition p of
-// each runs in one wave: lanes 0-31 search A (16x8 partition p), lanes 32-63
-// search B (8x16 partition p); 4 lanes per candidate, 8 candidates per
-// search per pass.  Each search keeps its own scalar state, LDS region and
-// strict-< scan; a search whose phase ended passes with no valid candidate.
+// The 16x8, 8x16 and 8x8 modes are independent of each other (all seeded
+// with mv16 and predicted from the same committed neighbours plus their own
+// earlier partitions), so two partition searches of different modes run in
+// one wave: lanes 0-31 search A (a 16x8 or 8x16 partition, 4 lanes per
+// candidate), lanes 32-63 search B (an 8x16 partition, 4 lanes per
+// candidate, or an 8x8 partition, 2 lanes per candidate on lanes 32-47).
+// 8 candidates per search per pass.  Each search keeps its own scalar state,
+// LDS region and strict-< scan; a search whose phase ended passes with no
+// valid candidate.
 
 struct PState {          // wave-uniform state of one search of a pair
   int bx, by;            // block position (picture pixels)
@@ -568,10 +579,18 @@ struct PState {          // wave-uniform state of one search of a pair
 };
 
 // Per-group keys of a paired pass (lanes 0-31: A, 32-63: B): least
-// cost*8 + slot of each group, read from lanes 31 and 63.
+// cost*8 + slot of each group, read from lanes 31 and 63.  The mirror steps
+// reach every slot when a slot spans 4 lanes; with 2-lane slots (TWO) a quad
+// swap first gives each lane the least of its quad's two slots.
+template <bool TWO = false>
 __device__ __forceinline__ void min_key_pair(uint32_t cost, bool valid, int slot, uint32_t &ka, uint32_t &kb) {
   uint32_t k = valid ? ((cost << 3) | (uint32_t)slot) : 0xFFFFFFFFu;
-  uint32_t t = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)k, DPP_ROW_HALF_MIRROR, 0xF, 0xF, false);
+  uint32_t t;
+  if constexpr (TWO) {
+    t = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)k, DPP_QP_2301, 0xF, 0xF, false);
+    k = t < k ? t : k;
+  }
+  t = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)k, DPP_ROW_HALF_MIRROR, 0xF, 0xF, false);
   k = t < k ? t : k;
   t = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)k, DPP_ROW_MIRROR, 0xF, 0xF, false);
   k = t < k ? t : k;
@@ -593,30 +612,42 @@ __device__ __forceinline__ bool take_key(PState &S, uint32_t key, int dx, int dy
   return true;
 }
 
-// search (slice.rs:1590-1723) for A = 16x8 partition and B = 8x16
-// partition at the same time.  src* = block top-left of each source.
+// search (slice.rs:1590-1723) for A (an AW x AH partition) and B (a BW x BH
+// partition) at the same time.  src* = block top-left of each source.
 // nh >= 0: pruned search with hint candidates hc[0..nh) (see search()).
-template <bool PR>
+template <int AW, int AH, int BW, int BH, bool PR>
 __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B, const uint8_t *srcA,
                                             const uint8_t *srcB, int sstride, int c0x, int c0y, int c1ax,
                                             int c1ay, int c1bx, int c1by, const uint32_t *hc, int nh,
                                             uint32_t *lds, SRes &ra, SRes &rb) {
+  static_assert(AW * AH == 128 && BW == 8, "pair shapes");
+  constexpr bool B8 = BH == 8;            // B is an 8x8 partition: 2 lanes per candidate
+  constexpr int LPC = B8 ? 2 : 4;         // reduction code of the cost helpers
+  constexpr int BLS = B8 ? 2 : 4;         // lanes per B candidate
   const uint32_t l = lane_id();
   const bool g = l >= 32;
   Lane L;
-  L.slot = (int)((l >> 2) & 7u);
-  L.sub = (int)(l & 3u);
-  L.sx = g ? 0 : (L.sub & 1) * 8;
-  L.sy = g ? L.sub * 4 : (L.sub >> 1) * 4;
-  L.active = true;
+  if (!g) {
+    L.slot = (int)((l >> 2) & 7u);
+    L.sub = (int)(l & 3u);
+    L.sx = AW == 16 ? (L.sub & 1) * 8 : 0;
+    L.sy = AW == 16 ? (L.sub >> 1) * 4 : L.sub * 4;
+    L.active = true;
+  } else {
+    L.slot = (int)(((l - 32) / BLS) & 7u);
+    L.sub = (int)((l - 32) % BLS);
+    L.sx = 0;
+    L.sy = L.sub * 4;
+    L.active = !B8 || l < 48;
+  }
   SrcRegs s;
   load_src(g ? srcB : srcA, sstride, L, s);
   uint32_t *lw = lds + (g ? LDS_PER_SEARCH / 4 : 0);
   uint32_t *lsp = lw + WIN_BYTES / 4;
   const int bx = gsel(g, A.bx, B.bx), by = gsel(g, A.by, B.by);
   const int mvpx = gsel(g, A.mvpx, B.mvpx), mvpy = gsel(g, A.mvpy, B.mvpy);
-  A.ok = legal_box(c, A.bx, A.by, 16, 8);
-  B.ok = legal_box(c, B.bx, B.by, 8, 16);
+  A.ok = legal_box(c, A.bx, A.by, AW, AH);
+  B.ok = legal_box(c, B.bx, B.by, BW, BH);
   // slice.rs:1617-1631 (see search()).
   int a0x = (A.mvpx + 2) & ~3, a0y = (A.mvpy + 2) & ~3;
   if (!in_box(A.ok, a0x, a0y)) { a0x = 0; a0y = 0; }
@@ -636,19 +667,20 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
     const int cx = L.slot == 1 ? r0x : (L.slot == 2 ? r1x : (hs ? hx : px));
     const int cy = L.slot == 1 ? r0y : (L.slot == 2 ? r1y : (hs ? hy : py));
     const Box ok = g ? B.ok : A.ok;
-    const bool valid = (L.slot == 0) | (((L.slot <= 2) | (hs & (L.slot - 3 < nh))) & in_box(ok, cx, cy));
-    const uint32_t k = sad_cost<false, 4>(c, L, s, bx, by, cx, cy, valid ? cx : px, valid ? cy : py, mvpx, mvpy,
-                                          lw, 0, 0);
+    const bool valid =
+        L.active & ((L.slot == 0) | (((L.slot <= 2) | (hs & (L.slot - 3 < nh))) & in_box(ok, cx, cy)));
+    const uint32_t k = sad_cost<false, LPC>(c, L, s, bx, by, cx, cy, valid ? cx : px, valid ? cy : py, mvpx, mvpy,
+                                            lw, 0, 0);
     uint32_t ka, kb;
-    min_key_pair(k, valid, L.slot, ka, kb);
+    min_key_pair<B8>(k, valid, L.slot, ka, kb);
     A.cost = ka >> 3;
     B.cost = kb >> 3;
     const int wa = (int)(ka & 7u), wb = (int)(kb & 7u);
     if constexpr (PR) {
       A.x = (int)rdlane((uint32_t)cx, 4 * wa);
       A.y = (int)rdlane((uint32_t)cy, 4 * wa);
-      B.x = (int)rdlane((uint32_t)cx, 32 + 4 * wb);
-      B.y = (int)rdlane((uint32_t)cy, 32 + 4 * wb);
+      B.x = (int)rdlane((uint32_t)cx, 32 + BLS * wb);
+      B.y = (int)rdlane((uint32_t)cy, 32 + BLS * wb);
     } else {
       A.x = wa == 1 ? r0x : (wa == 2 ? r1ax : a0x);
       A.y = wa == 1 ? r0y : (wa == 2 ? r1ay : a0y);
@@ -662,11 +694,11 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
     hint_cand(hc, L.slot + 5, nh, hx, hy);
     const Box ok = g ? B.ok : A.ok;
     const int ox = gsel(g, A.x, B.x), oy = gsel(g, A.y, B.y);
-    const bool valid = (L.slot + 5 < nh) & in_box(ok, hx, hy);
-    const uint32_t k = sad_cost<false, 4>(c, L, s, bx, by, hx, hy, valid ? hx : ox, valid ? hy : oy, mvpx, mvpy,
-                                          lw, 0, 0);
+    const bool valid = L.active & (L.slot + 5 < nh) & in_box(ok, hx, hy);
+    const uint32_t k = sad_cost<false, LPC>(c, L, s, bx, by, hx, hy, valid ? hx : ox, valid ? hy : oy, mvpx, mvpy,
+                                            lw, 0, 0);
     uint32_t ka, kb;
-    min_key_pair(k, valid, L.slot, ka, kb);
+    min_key_pair<B8>(k, valid, L.slot, ka, kb);
     if ((ka >> 3) < A.cost) {
       const int wa = (int)(ka & 7u);
       A.cost = ka >> 3;
@@ -676,8 +708,8 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
     if ((kb >> 3) < B.cost) {
       const int wb = (int)(kb & 7u);
       B.cost = kb >> 3;
-      B.x = (int)rdlane((uint32_t)hx, 32 + 4 * wb);
-      B.y = (int)rdlane((uint32_t)hy, 32 + 4 * wb);
+      B.x = (int)rdlane((uint32_t)hx, 32 + BLS * wb);
+      B.y = (int)rdlane((uint32_t)hy, 32 + BLS * wb);
     }
   }
   const int range = c.range;
@@ -697,17 +729,17 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
   A.act = B.act = range > 0 && !PR;
   A.it = B.it = 0;
   while (A.act | B.act) {
-    if (A.act) ensure_win<16, 8>(c, A.bx, A.by, A.x, A.y, 2, A.wx0, A.wy0, lds);
-    if (B.act) ensure_win<8, 16>(c, B.bx, B.by, B.x, B.y, 2, B.wx0, B.wy0, lds + LDS_PER_SEARCH / 4);
+    if (A.act) ensure_win<AW, AH>(c, A.bx, A.by, A.x, A.y, 2, A.wx0, A.wy0, lds);
+    if (B.act) ensure_win<BW, BH>(c, B.bx, B.by, B.x, B.y, 2, B.wx0, B.wy0, lds + LDS_PER_SEARCH / 4);
     const int ox = gsel(g, A.x, B.x), oy = gsel(g, A.y, B.y);
     const int cx = ox + hdx, cy = oy + hdy;
     Box hb = g ? B.hb : A.hb;
     const bool act = g ? B.act : A.act;
-    const bool valid = (L.slot < 6) & act & in_box(hb, cx, cy);
-    const uint32_t k = sad_cost<true, 4>(c, L, s, bx, by, cx, cy, valid ? cx : ox, valid ? cy : oy, mvpx, mvpy, lw,
-                                         gsel(g, A.wx0, B.wx0), gsel(g, A.wy0, B.wy0));
+    const bool valid = L.active & (L.slot < 6) & act & in_box(hb, cx, cy);
+    const uint32_t k = sad_cost<true, LPC>(c, L, s, bx, by, cx, cy, valid ? cx : ox, valid ? cy : oy, mvpx, mvpy,
+                                           lw, gsel(g, A.wx0, B.wx0), gsel(g, A.wy0, B.wy0));
     uint32_t ka, kb;
-    min_key_pair(k, valid, L.slot, ka, kb);
+    min_key_pair<B8>(k, valid, L.slot, ka, kb);
     if (A.act) {
       const int j = (int)(ka & 7u);
       A.act = take_key(A, ka, hex_dx(j) * 4, hex_dy(j) * 4) && ++A.it < range;
@@ -726,25 +758,25 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
   A.it = B.it = 0;
   while (A.act | B.act) {
     if constexpr (PR) {
-      if (A.act) ensure_sp<16, 8, 3>(c, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
+      if (A.act) ensure_sp<AW, AH, 3>(c, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
       if (B.act)
-        ensure_sp<8, 16, 3>(c, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
+        ensure_sp<BW, BH, 3>(c, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
     } else {
-      if (A.act) ensure_win<16, 8>(c, A.bx, A.by, A.x, A.y, 1, A.wx0, A.wy0, lds);
-      if (B.act) ensure_win<8, 16>(c, B.bx, B.by, B.x, B.y, 1, B.wx0, B.wy0, lds + LDS_PER_SEARCH / 4);
+      if (A.act) ensure_win<AW, AH>(c, A.bx, A.by, A.x, A.y, 1, A.wx0, A.wy0, lds);
+      if (B.act) ensure_win<BW, BH>(c, B.bx, B.by, B.x, B.y, 1, B.wx0, B.wy0, lds + LDS_PER_SEARCH / 4);
     }
     const int ox = gsel(g, A.x, B.x), oy = gsel(g, A.y, B.y);
     const int cx = ox + qdx * 4, cy = oy + qdy * 4;
     const Box ok = g ? B.ok : A.ok;
     const bool act = g ? B.act : A.act;
-    const bool valid = act & in_box(ok, cx, cy);
+    const bool valid = L.active & act & in_box(ok, cx, cy);
     const uint32_t k =
-        PR ? sad_cost<true, 4, SP_W>(c, L, s, bx, by, cx, cy, valid ? cx : ox, valid ? cy : oy, mvpx, mvpy, lsp,
-                                     gsel(g, A.sx0, B.sx0), gsel(g, A.sy0, B.sy0))
-           : sad_cost<true, 4>(c, L, s, bx, by, cx, cy, valid ? cx : ox, valid ? cy : oy, mvpx, mvpy, lw,
-                               gsel(g, A.wx0, B.wx0), gsel(g, A.wy0, B.wy0));
+        PR ? sad_cost<true, LPC, SP_W>(c, L, s, bx, by, cx, cy, valid ? cx : ox, valid ? cy : oy, mvpx, mvpy, lsp,
+                                       gsel(g, A.sx0, B.sx0), gsel(g, A.sy0, B.sy0))
+           : sad_cost<true, LPC>(c, L, s, bx, by, cx, cy, valid ? cx : ox, valid ? cy : oy, mvpx, mvpy, lw,
+                                 gsel(g, A.wx0, B.wx0), gsel(g, A.wy0, B.wy0));
     uint32_t ka, kb;
-    min_key_pair(k, valid, L.slot, ka, kb);
+    min_key_pair<B8>(k, valid, L.slot, ka, kb);
     if (A.act) {
       const int j = (int)(ka & 7u);
       A.act = take_key(A, ka, sq_dx(j) * 4, sq_dy(j) * 4) && ++A.it < 2;
@@ -755,14 +787,14 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
     }
   }
   // Sub-pel planes of both, then the SATD cost at each integer best.
-  ensure_sp<16, 8>(c, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
-  ensure_sp<8, 16>(c, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
+  ensure_sp<AW, AH>(c, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
+  ensure_sp<BW, BH>(c, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
   v2s sp[16];
   src_pairs(s, sp);
   {
     const int ox = gsel(g, A.x, B.x), oy = gsel(g, A.y, B.y);
-    const uint32_t k = satd_cost<4>(c, L, sp, bx, by, ox, oy, ox, oy, mvpx, mvpy, lsp, gsel(g, A.sx0, B.sx0),
-                                    gsel(g, A.sy0, B.sy0));
+    const uint32_t k = satd_cost<LPC>(c, L, sp, bx, by, ox, oy, ox, oy, mvpx, mvpy, lsp, gsel(g, A.sx0, B.sx0),
+                                      gsel(g, A.sy0, B.sy0));
     A.cost = rdlane(k, 0);
     B.cost = rdlane(k, 32);
   }
@@ -773,17 +805,17 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
     A.act = B.act = true;
     A.it = B.it = 0;
     while (A.act | B.act) {
-      if (A.act) ensure_sp<16, 8>(c, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
-      if (B.act) ensure_sp<8, 16>(c, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
+      if (A.act) ensure_sp<AW, AH>(c, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
+      if (B.act) ensure_sp<BW, BH>(c, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
       const int ox = gsel(g, A.x, B.x), oy = gsel(g, A.y, B.y);
       const int cx = ox + qdx * step, cy = oy + qdy * step;
       const Box ok = g ? B.ok : A.ok;
       const bool act = g ? B.act : A.act;
-      const bool valid = act & in_box(ok, cx, cy);
-      const uint32_t k = satd_cost<4>(c, L, sp, bx, by, cx, cy, valid ? cx : ox, valid ? cy : oy, mvpx, mvpy, lsp,
-                                      gsel(g, A.sx0, B.sx0), gsel(g, A.sy0, B.sy0));
+      const bool valid = L.active & act & in_box(ok, cx, cy);
+      const uint32_t k = satd_cost<LPC>(c, L, sp, bx, by, cx, cy, valid ? cx : ox, valid ? cy : oy, mvpx, mvpy,
+                                        lsp, gsel(g, A.sx0, B.sx0), gsel(g, A.sy0, B.sy0));
       uint32_t ka, kb;
-      min_key_pair(k, valid, L.slot, ka, kb);
+      min_key_pair<B8>(k, valid, L.slot, ka, kb);
       if (A.act) {
         const int j = (int)(ka & 7u);
         A.act = take_key(A, ka, sq_dx(j) * step, sq_dy(j) * step) && ++A.it < rounds;
@@ -962,7 +994,8 @@ __device__ __forceinline__ void search_part_12(const SCtx &c, const Nbr &n, cons
   uint32_t pa = mvp<1, 0, 0, 0, 4, 1>(n, ca), pb = mvp<2, 0, 0, 0, 2, 3>(n, cb);
   A.bx = mbx; A.by = mby; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
   B.bx = mbx; B.by = mby; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
-  search_pair<PR>(c, A, B, src, src, sstride, mv_x(seed), mv_y(seed), 0, 0, 0, 0, hc, nh, lds, xa, xb);
+  search_pair<16, 8, 8, 16, PR>(c, A, B, src, src, sstride, mv_x(seed), mv_y(seed), 0, 0, 0, 0, hc, nh, lds, xa,
+                                xb);
   ra.mv[0] = ca.mv[0] = pack_mv(xa.x, xa.y); ra.mvp[0] = pa; ra.total = xa.cost;
   rb.mv[0] = cb.mv[0] = pack_mv(xb.x, xb.y); rb.mvp[0] = pb; rb.total = xb.cost;
   // partition 1: 16x8 bottom (y 2 blocks), 8x16 right (x 2 blocks)
@@ -970,10 +1003,62 @@ __device__ __forceinline__ void search_part_12(const SCtx &c, const Nbr &n, cons
   pb = mvp<2, 1, 2, 0, 2, 4>(n, cb);
   A.bx = mbx; A.by = mby + 8; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
   B.bx = mbx + 8; B.by = mby; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
-  search_pair<PR>(c, A, B, src + 8 * sstride, src + 8, sstride, mv_x(seed), mv_y(seed), mv_x(ra.mv[0]),
+  search_pair<16, 8, 8, 16, PR>(c, A, B, src + 8 * sstride, src + 8, sstride, mv_x(seed), mv_y(seed), mv_x(ra.mv[0]),
               mv_y(ra.mv[0]), mv_x(rb.mv[0]), mv_y(rb.mv[0]), hc, nh, lds, xa, xb);
   ra.mv[1] = pack_mv(xa.x, xa.y); ra.mvp[1] = pa; ra.total += xa.cost;
   rb.mv[1] = pack_mv(xb.x, xb.y); rb.mvp[1] = pb; rb.total += xb.cost;
+}
+
+// search_part (slice.rs:1810-1868) of P16x8 (r1), P8x16 (r2) and P8x8 (r3)
+// in four paired steps: (16x8 p0, 8x8 p0), (8x16 p0, 8x8 p1), (16x8 p1,
+// 8x8 p2), (8x16 p1, 8x8 p3).  Each partition still follows the earlier
+// partitions of its own mode, with the same predictor and candidates as the
+// one-at-a-time order, so the results are the same.
+__device__ __forceinline__ void search_part_123(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
+                                                int mbx, int mby, uint32_t seed, PartRes &r1, PartRes &r2,
+                                                PartRes &r3, uint32_t *lds) {
+  Cur c1 = {}, c2 = {}, c3 = {};
+#pragma unroll
+  for (int i = 0; i < 4; i++) { r1.mv[i] = r1.mvp[i] = r2.mv[i] = r2.mvp[i] = r3.mv[i] = r3.mvp[i] = 0u; }
+  PState A, B;
+  SRes xa, xb;
+  const uint32_t *nohint = nullptr;
+  const int sx = mv_x(seed), sy = mv_y(seed);
+  // step 1: 16x8 top (x 0, y 0, w 4 blocks) with 8x8 top-left (x 0, y 0, w 2)
+  uint32_t pa = mvp<1, 0, 0, 0, 4, 1>(n, c1), pb = mvp<3, 0, 0, 0, 2, 0>(n, c3);
+  A.bx = mbx; A.by = mby; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
+  B.bx = mbx; B.by = mby; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
+  search_pair<16, 8, 8, 8, false>(c, A, B, src, src, sstride, sx, sy, 0, 0, 0, 0, nohint, -1, lds, xa, xb);
+  r1.mv[0] = c1.mv[0] = pack_mv(xa.x, xa.y); r1.mvp[0] = pa; r1.total = xa.cost;
+  r3.mv[0] = c3.mv[0] = pack_mv(xb.x, xb.y); r3.mvp[0] = pb; r3.total = xb.cost;
+  // step 2: 8x16 left (x 0, y 0, w 2) with 8x8 top-right (x 2, y 0)
+  pa = mvp<2, 0, 0, 0, 2, 3>(n, c2);
+  pb = mvp<3, 1, 2, 0, 2, 0>(n, c3);
+  A.bx = mbx; A.by = mby; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
+  B.bx = mbx + 8; B.by = mby; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
+  search_pair<8, 16, 8, 8, false>(c, A, B, src, src + 8, sstride, sx, sy, 0, 0, mv_x(r3.mv[0]), mv_y(r3.mv[0]),
+                                  nohint, -1, lds, xa, xb);
+  r2.mv[0] = c2.mv[0] = pack_mv(xa.x, xa.y); r2.mvp[0] = pa; r2.total = xa.cost;
+  r3.mv[1] = c3.mv[1] = pack_mv(xb.x, xb.y); r3.mvp[1] = pb; r3.total += xb.cost;
+  // step 3: 16x8 bottom (y 2 blocks) with 8x8 bottom-left (x 0, y 2)
+  pa = mvp<1, 1, 0, 2, 4, 2>(n, c1);
+  pb = mvp<3, 2, 0, 2, 2, 0>(n, c3);
+  A.bx = mbx; A.by = mby + 8; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
+  B.bx = mbx; B.by = mby + 8; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
+  search_pair<16, 8, 8, 8, false>(c, A, B, src + 8 * sstride, src + 8 * sstride, sstride, sx, sy,
+                                  mv_x(r1.mv[0]), mv_y(r1.mv[0]), mv_x(r3.mv[0]), mv_y(r3.mv[0]), nohint, -1, lds,
+                                  xa, xb);
+  r1.mv[1] = pack_mv(xa.x, xa.y); r1.mvp[1] = pa; r1.total += xa.cost;
+  r3.mv[2] = c3.mv[2] = pack_mv(xb.x, xb.y); r3.mvp[2] = pb; r3.total += xb.cost;
+  // step 4: 8x16 right (x 2 blocks) with 8x8 bottom-right (x 2, y 2)
+  pa = mvp<2, 1, 2, 0, 2, 4>(n, c2);
+  pb = mvp<3, 3, 2, 2, 2, 0>(n, c3);
+  A.bx = mbx + 8; A.by = mby; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
+  B.bx = mbx + 8; B.by = mby + 8; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
+  search_pair<8, 16, 8, 8, false>(c, A, B, src + 8, src + 8 * sstride + 8, sstride, sx, sy, mv_x(r2.mv[0]),
+                                  mv_y(r2.mv[0]), mv_x(r3.mv[0]), mv_y(r3.mv[0]), nohint, -1, lds, xa, xb);
+  r2.mv[1] = pack_mv(xa.x, xa.y); r2.mvp[1] = pa; r2.total += xa.cost;
+  r3.mv[3] = pack_mv(xb.x, xb.y); r3.mvp[3] = pb; r3.total += xb.cost;
 }
 
 __device__ __forceinline__ uint32_t sel4(int k, uint32_t a, uint32_t b, uint32_t cc, uint32_t d) {
@@ -997,6 +1082,17 @@ struct MbRes {
   uint32_t best_cost;
   uint32_t skip, m16;
 };
+
+// The strict-< choice of mode `part` (total r, penalty pen) against the
+// running best (slice.rs:1925-1945).
+__device__ __forceinline__ void take_mode(MbRes &o, const PartRes &r, int part, uint32_t pen) {
+  o.cost[part] = r.total;
+  const uint32_t cc = r.total + pen;
+  const bool t = cc < o.best_cost;
+  o.best_cost = t ? cc : o.best_cost;
+  o.part = t ? part : o.part;
+  take_if(o.best, r, t);
+}
 
 // Modes P16x8 and P8x16 (searched together), then the strict-< choice
 // against the running best (slice.rs:1925-1945).
@@ -1060,8 +1156,11 @@ __device__ __forceinline__ void analyse_mb(const SCtx &c, const Nbr &n, const ui
     else if (split & (hpart == 3))
       mode_3<PR>(c, n, src, sstride, mbx, mby, hc, nh, o, lds);
   } else if (split) {
-    mode_12<PR>(c, n, src, sstride, mbx, mby, hc, nh, o, lds);
-    mode_3<PR>(c, n, src, sstride, mbx, mby, hc, nh, o, lds);
+    PartRes r1, r2, r3;
+    search_part_123(c, n, src, sstride, mbx, mby, o.m16, r1, r2, r3, lds);
+    take_mode(o, r1, 1, (c.lam * 3u) >> 8);
+    take_mode(o, r2, 2, (c.lam * 3u) >> 8);
+    take_mode(o, r3, 3, (c.lam * 10u) >> 8);
   }
 }
 
