@@ -196,16 +196,20 @@ __device__ __forceinline__ void load_src(const uint8_t *blk, int sstride, const 
 // wave-uniform check re-stages the region when the pass's candidates could
 // leave it; a region centred on the current best always covers every legal
 // candidate of the next pass (mv_ok keeps blocks 44 px inside the 64 px pad),
-// so all hex, square and sub-pel passes read LDS.
-#define WIN_M 10
-#define WIN_W 44   // 16 + 2*WIN_M + 8 bytes of alignment / over-read slack
-#define WIN_H 37   // 16 + 2*WIN_M + 1
+// so all hex, square and sub-pel passes read LDS.  Regions go HBM -> LDS by
+// 16-byte LDS-DMA (no VGPR round trip); a search issues its first region
+// before the candidate pass, so the two HBM round trips overlap.
+#define WIN_MX 12  // horizontal margin of the full-pel window
+#define WIN_MY 13  // vertical margin
+#define WIN_W 48   // 16 + 2*WIN_MX + 8 bytes of alignment / over-read slack (3 x 16 B)
+#define WIN_H 42   // 16 + 2*WIN_MY
 #define SP_W 32
 #define SP_H 23   // 16 + 2*3 + 1: room for a pruned search's square rounds
 #define WIN_BYTES (WIN_W * WIN_H)
 #define SP_PLANE (SP_W * SP_H)
 #define LDS_PER_SEARCH (((WIN_BYTES + 4 * SP_PLANE) + 15) & ~15)
 #define LDS_PER_WAVE (2 * LDS_PER_SEARCH)  // room for two concurrent searches
+static_assert(WIN_BYTES % 16 == 0 && SP_PLANE % 16 == 0, "LDS-DMA destinations are 16-byte aligned");
 
 typedef const __attribute__((address_space(1))) uint32_t gu32;
 
@@ -222,31 +226,8 @@ __device__ __forceinline__ void ld8_at(P base, int off, uint32_t &lo, uint32_t &
 
 __device__ __forceinline__ gu32 *gplane(const uint8_t *p) { return (gu32 *)p; }
 
-// Copy a DW x DH byte region (DW multiple of 4, origin ox 4-aligned, picture
-// coordinates) of a padded plane into LDS.  All loads are issued before the
-// stores (fully unrolled).
-template <int DW, int DH>
-__device__ __forceinline__ void stage_region(const uint8_t *plane, int stride, int ox, int oy, uint32_t *dst) {
-  constexpr int DPR = DW / 4, N = DPR * DH, IT = (N + 63) / 64;
-  const int l = (int)lane_id();
-  gu32 *src = gplane(plane);
-  const int base = ((oy + PCME_PAD) * stride + ox + PCME_PAD) >> 2;
-  const int sdw = stride >> 2;
-  uint32_t v[IT];
-#pragma unroll
-  for (int k = 0; k < IT; k++) {
-    // lanes past the end re-read the last dword (no branch around the load)
-    int i = l + 64 * k;
-    i = (64 * k + 63 < N || i < N) ? i : N - 1;
-    int r = i / DPR, cc = i - r * DPR;
-    v[k] = src[base + r * sdw + cc];
-  }
-#pragma unroll
-  for (int k = 0; k < IT; k++) {
-    int i = l + 64 * k;
-    if (64 * k + 63 < N || i < N) dst[i] = v[k];
-  }
-}
+// Regions of a padded plane (DW bytes wide, DW a multiple of 4, origin ox
+// 4-aligned, picture coordinates) are staged into LDS by dma_region below.
 
 // Integer-pel SAD cost of the lane's slot candidate (slice.rs:1605-1616):
 // SAD over the block + (lam * mv_bits) >> 8.  (ux, uy) is the address MV:
@@ -366,6 +347,57 @@ __device__ __forceinline__ bool in_box(const Box &b, int mx, int my) {
   return (mx >= b.xlo) & (mx <= b.xhi) & (my >= b.ylo) & (my <= b.yhi);
 }
 
+typedef const __attribute__((address_space(1))) void *gvptr;
+typedef __attribute__((address_space(3))) void *lvptr;
+
+// The same copy with LDS-DMA loads (global_load_lds_dword: lane l of
+// instruction k writes dword 64k + l of dst), issued without waiting, so a
+// region can be staged while other work runs; no data passes through VGPRs.
+// dma_wait() completes every load issued before it.
+// Stage a DW x DH byte region (DW a multiple of 16, origin 4-aligned) into
+// LDS with 16-byte LDS-DMA loads (global_load_lds_dwordx4: lane l of an
+// instruction writes its 16 bytes at the instruction's LDS base + 16 l;
+// masked lanes write nothing).  Chunk d = 64k + lane of instruction k is
+// region row d / CPR, 16-byte column d % CPR: the lane's (row, col) of
+// instruction 0 plus a compile-time step with one carry.  The empty asm
+// keeps the compiler from hoisting every instruction's per-lane address
+// out of the search loops (a VGPR each).
+template <int DW, int DH>
+__device__ __forceinline__ void dma_region(const uint8_t *plane, int stride, int ox, int oy, uint32_t *dst) {
+  static_assert(DW % 16 == 0, "regions are staged in 16-byte chunks");
+  constexpr int CPR = DW / 16, N = CPR * DH, IT = (N + 63) / 64;
+  const int l = (int)lane_id();
+  int r0 = l / CPR, c0 = l - (l / CPR) * CPR;
+  asm volatile("" : "+v"(r0), "+v"(c0));
+  const uint8_t *base = plane + (size_t)((oy + PCME_PAD) * stride + ox + PCME_PAD);
+#pragma unroll
+  for (int k = 0; k < IT; k++) {
+    const int q = (64 * k) / CPR, m = (64 * k) % CPR;
+    const int t = c0 + m;
+    const bool cy = t >= CPR;
+    const int row = r0 + q + (cy ? 1 : 0), col = cy ? t - CPR : t;
+    if (64 * k + 63 < N || l + 64 * k < N)
+      __builtin_amdgcn_global_load_lds((gvptr)(base + row * stride + 16 * col), (lvptr)(dst + 256 * k), 16, 0, 0);
+  }
+}
+__device__ __forceinline__ void dma_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// Origin of a full-pel window centred on the block at MV (bxv, byv).
+__device__ __forceinline__ void win_origin(const SCtx &c, int bx, int by, int bxv, int byv, int &wx0, int &wy0) {
+  const int X = bx + (bxv >> 2), Y = by + (byv >> 2);
+  wx0 = clampi((X - WIN_MX) & ~3, -PCME_PAD, c.pw + PCME_PAD - WIN_W);
+  wy0 = clampi(Y - WIN_MY, -PCME_PAD, c.ph + PCME_PAD - WIN_H);
+}
+
+// Start staging the full-pel window around (bxv, byv); the caller waits.
+template <int W, int H>
+__device__ __forceinline__ void prefetch_win(const SCtx &c, int bx, int by, int bxv, int byv, int &wx0, int &wy0,
+                                             uint32_t *lw) {
+  win_origin(c, bx, by, bxv, byv, wx0, wy0);
+  __syncthreads();
+  dma_region<WIN_W, WIN_H>(c.pl[0], c.stride, wx0, wy0, lw);
+}
+
 // Re-stage the full-pel window around the current best (bxv, byv) when a
 // pass reaching +-R px around it could leave the staged window.
 template <int W, int H>
@@ -374,29 +406,40 @@ __device__ __forceinline__ void ensure_win(const SCtx &c, int bx, int by, int bx
   const int X = bx + (bxv >> 2), Y = by + (byv >> 2);
   const bool ok = (X - R - wx0 >= 0) & (X + R + W + 4 - wx0 <= WIN_W) & (Y - R - wy0 >= 0) & (Y + R + H - wy0 <= WIN_H);
   if (ok) return;
-  wx0 = clampi((X - WIN_M) & ~3, -PCME_PAD, c.pw + PCME_PAD - WIN_W);
-  wy0 = clampi(Y - WIN_M, -PCME_PAD, c.ph + PCME_PAD - WIN_H);
+  win_origin(c, bx, by, bxv, byv, wx0, wy0);
   __syncthreads();
-  stage_region<WIN_W, WIN_H>(c.pl[0], c.stride, wx0, wy0, lw);
+  dma_region<WIN_W, WIN_H>(c.pl[0], c.stride, wx0, wy0, lw);
+  dma_wait();
   __syncthreads();
 }
 
 // Same for the four sub-pel planes: candidates of a round lie within +-1 px
 // of the current best, reading one more row/column for the half-pel offsets.
-// A restage centres the region with RM pixels of margin (a pruned search
-// stages once with RM = 3 before its square rounds, which then read the
-// full-pel plane of the region).
-template <int W, int H, int RM = 2>
-__device__ __forceinline__ void ensure_sp(const SCtx &c, int bx, int by, int bxv, int byv, int &sx0, int &sy0,
-                                          uint32_t *lsp) {
+// Start staging the four planes around the block at (bxv, byv) with RM
+// pixels of margin (a search prefetches them while its integer passes run;
+// a pruned search's square rounds read the full-pel plane of the region).
+template <int W, int H, int RM>
+__device__ __forceinline__ void prefetch_sp(const SCtx &c, int bx, int by, int bxv, int byv, int &sx0, int &sy0,
+                                            uint32_t *lsp) {
   const int X = bx + (bxv >> 2), Y = by + (byv >> 2);
-  const bool ok = (X - 1 - sx0 >= 0) & (X + W + 6 - sx0 <= SP_W) & (Y - 1 - sy0 >= 0) & (Y + H + 2 - sy0 <= SP_H);
-  if (ok) return;
   sx0 = clampi((X - RM) & ~3, -PCME_PAD, c.pw + PCME_PAD - SP_W);
   sy0 = clampi(Y - RM, -PCME_PAD, c.ph + PCME_PAD - SP_H);
   __syncthreads();
 #pragma unroll
-  for (int p = 0; p < 4; p++) stage_region<SP_W, SP_H>(c.pl[p], c.stride, sx0, sy0, lsp + p * (SP_PLANE / 4));
+  for (int p = 0; p < 4; p++) dma_region<SP_W, SP_H>(c.pl[p], c.stride, sx0, sy0, lsp + p * (SP_PLANE / 4));
+}
+
+// Complete any staging in flight, then re-stage the four planes (centred
+// with RM pixels of margin) when the next sub-pel round could leave them.
+template <int W, int H, int RM = 2>
+__device__ __forceinline__ void ensure_sp(const SCtx &c, int bx, int by, int bxv, int byv, int &sx0, int &sy0,
+                                          uint32_t *lsp) {
+  dma_wait();
+  const int X = bx + (bxv >> 2), Y = by + (byv >> 2);
+  const bool ok = (X - 1 - sx0 >= 0) & (X + W + 6 - sx0 <= SP_W) & (Y - 1 - sy0 >= 0) & (Y + H + 2 - sy0 <= SP_H);
+  if (ok) return;
+  prefetch_sp<W, H, RM>(c, bx, by, bxv, byv, sx0, sy0, lsp);
+  dma_wait();
   __syncthreads();
 }
 
@@ -429,6 +472,14 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
   // the scan including its "c != best" test.
   int bxv = (mvpx + 2) & ~3, byv = (mvpy + 2) & ~3;
   if (!in_box(ok, bxv, byv)) { bxv = 0; byv = 0; }
+  // Staging runs beside the candidate pass (its HBM round trip): the full
+  // window around the rounded predictor (the usual winner), or for a pruned
+  // search the sub-pel planes its square rounds read.
+  int wx0 = 1 << 20, wy0 = 1 << 20, sx0 = 1 << 20, sy0 = 1 << 20;
+  if constexpr (PR)
+    prefetch_sp<W, H, 3>(c, bx, by, bxv, byv, sx0, sy0, lsp);
+  else
+    prefetch_win<W, H>(c, bx, by, bxv, byv, wx0, wy0, lw);
   uint32_t bcost;
   if constexpr (!PR) {
     const int r0x = (c0x + 2) & ~3, r0y = (c0y + 2) & ~3;
@@ -477,8 +528,8 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
       byv = rdlane((uint32_t)hy, 8 * w);
     }
   }
+  dma_wait();
   // slice.rs:1632-1656: hexagon descent clamped to +-range around centre0.
-  int wx0 = 1 << 20, wy0 = 1 << 20;  // nothing staged yet
   const int c0xx = bxv, c0yy = byv;
   const int range = c.range;
   const int hdx = hex_dx(L.slot & 7) * 4, hdy = hex_dy(L.slot & 7) * 4;
@@ -506,7 +557,6 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
   // full search reads the full-pel window; a pruned search stages the four
   // sub-pel planes once (margin 3) and reads the full-pel plane among them.
   const int qdx = sq_dx(L.slot), qdy = sq_dy(L.slot);
-  int sx0 = 1 << 20, sy0 = 1 << 20;
   for (int it = 0; it < 2; it++) {
     if constexpr (PR)
       ensure_sp<W, H, 3>(c, bx, by, bxv, byv, sx0, sy0, lsp);
@@ -653,6 +703,16 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
   if (!in_box(A.ok, a0x, a0y)) { a0x = 0; a0y = 0; }
   int b0x = (B.mvpx + 2) & ~3, b0y = (B.mvpy + 2) & ~3;
   if (!in_box(B.ok, b0x, b0y)) { b0x = 0; b0y = 0; }
+  // Staging beside the candidate pass (see search()).
+  A.wx0 = A.wy0 = B.wx0 = B.wy0 = 1 << 20;
+  A.sx0 = A.sy0 = B.sx0 = B.sy0 = 1 << 20;
+  if constexpr (PR) {
+    prefetch_sp<AW, AH, 3>(c, A.bx, A.by, a0x, a0y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
+    prefetch_sp<BW, BH, 3>(c, B.bx, B.by, b0x, b0y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
+  } else {
+    prefetch_win<AW, AH>(c, A.bx, A.by, a0x, a0y, A.wx0, A.wy0, lds);
+    prefetch_win<BW, BH>(c, B.bx, B.by, b0x, b0y, B.wx0, B.wy0, lds + LDS_PER_SEARCH / 4);
+  }
   {
     // cands = [seed, mvs[0] of the search's own mode]
     const int r0x = (c0x + 2) & ~3, r0y = (c0y + 2) & ~3;
@@ -712,8 +772,8 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
       B.y = (int)rdlane((uint32_t)hy, 32 + BLS * wb);
     }
   }
+  dma_wait();
   const int range = c.range;
-  A.wx0 = A.wy0 = B.wx0 = B.wy0 = 1 << 20;
   A.hb = A.ok;
   A.hb.xlo = max(A.hb.xlo, A.x - 4 * range - 3);
   A.hb.xhi = min(A.hb.xhi, A.x + 4 * range + 3);
@@ -753,7 +813,6 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
   // pruned search reads the full-pel plane of its sub-pel region, staged
   // once with margin 3).
   const int qdx = sq_dx(L.slot), qdy = sq_dy(L.slot);
-  A.sx0 = A.sy0 = B.sx0 = B.sy0 = 1 << 20;
   A.act = B.act = true;
   A.it = B.it = 0;
   while (A.act | B.act) {
@@ -1332,14 +1391,14 @@ __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, con
 __global__ void __launch_bounds__(64) search_frames_kernel(const DevJob *__restrict__ jobs,
                                                            const int2 *__restrict__ tickets, int ntickets,
                                                            uint32_t *ctl, uint32_t spin_limit) {
-  __shared__ uint32_t s_lds[LDS_PER_WAVE / 4];
+  __shared__ __attribute__((aligned(16))) uint32_t s_lds[LDS_PER_WAVE / 4];
   frames_body<false>(jobs, tickets, ntickets, ctl, spin_limit, s_lds);
 }
 
 __global__ void __launch_bounds__(64)
 search_frames_hinted_kernel(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets, int ntickets,
                             uint32_t *ctl, uint32_t spin_limit) {
-  __shared__ uint32_t s_lds[LDS_PER_WAVE / 4];
+  __shared__ __attribute__((aligned(16))) uint32_t s_lds[LDS_PER_WAVE / 4];
   frames_body<true>(jobs, tickets, ntickets, ctl, spin_limit, s_lds);
 }
 
@@ -1348,7 +1407,7 @@ search_frames_hinted_kernel(const DevJob *__restrict__ jobs, const int2 *__restr
 __global__ void __launch_bounds__(64) search_batch_kernel(const pcme_search_job *__restrict__ jobs,
                                                           int njobs, pcme_mv *out_mv,
                                                           uint32_t *out_cost) {
-  __shared__ uint32_t s_lds[LDS_PER_WAVE / 4];
+  __shared__ __attribute__((aligned(16))) uint32_t s_lds[LDS_PER_WAVE / 4];
   const int j = blockIdx.x;
   if (j >= njobs) return;
   const pcme_search_job &J = jobs[j];
