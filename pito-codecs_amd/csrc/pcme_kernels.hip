@@ -261,8 +261,9 @@ __device__ __forceinline__ const uint8_t *plane_sel(const SCtx &c, uint32_t p) {
 
 // Sub-pel cost (slice.rs:1684-1687): mc_luma (picture.rs:415-449) as the
 // rounding average of two of the staged planes, SATD, + (lam * mv_bits) >> 8.
-// Full and half positions read the same plane twice, and (a+a+1)>>1 == a.
-template <int LPC = 8>
+// Full and half positions read the same plane twice, and (a+a+1)>>1 == a:
+// HALF (every candidate of the pass at a full or half position) reads it once.
+template <int LPC = 8, bool HALF = false>
 __device__ __forceinline__ uint32_t satd_cost(const SCtx &c, const Lane &L, const v2s *sp, int bx, int by, int cx,
                                               int cy, int ux, int uy, int mvpx, int mvpy, const uint32_t *lsp,
                                               int sx0, int sy0) {
@@ -273,11 +274,15 @@ __device__ __forceinline__ uint32_t satd_cost(const SCtx &c, const Lane &L, cons
   uint32_t P[8];
 #pragma unroll
   for (int r = 0; r < 4; r++) {
-    uint32_t a0, a1, b0, b1;
-    ld8_at(lsp, oa + r * SP_W, a0, a1);
-    ld8_at(lsp, ob + r * SP_W, b0, b1);
-    P[2 * r] = __builtin_amdgcn_lerp(a0, b0, 0x01010101u);
-    P[2 * r + 1] = __builtin_amdgcn_lerp(a1, b1, 0x01010101u);
+    if constexpr (HALF) {
+      ld8_at(lsp, oa + r * SP_W, P[2 * r], P[2 * r + 1]);
+    } else {
+      uint32_t a0, a1, b0, b1;
+      ld8_at(lsp, oa + r * SP_W, a0, a1);
+      ld8_at(lsp, ob + r * SP_W, b0, b1);
+      P[2 * r] = __builtin_amdgcn_lerp(a0, b0, 0x01010101u);
+      P[2 * r + 1] = __builtin_amdgcn_lerp(a1, b1, 0x01010101u);
+    }
   }
   uint32_t acc = satd_strip8(sp, P);
   acc = L.active ? acc : 0u;
@@ -443,12 +448,13 @@ __device__ __forceinline__ void ensure_sp(const SCtx &c, int bx, int by, int bxv
   __syncthreads();
 }
 
-// Hint candidate i (lane-varying) of the packed list hc (global memory,
-// wave-uniform pointer), rounded like the search_part candidates
-// (slice.rs:1622); lanes with i outside [0, nh) read nothing.
-__device__ __forceinline__ void hint_cand(const uint32_t *hc, int i, int nh, int &x, int &y) {
-  uint32_t v = 0u;
-  if ((i >= 0) & (i < nh)) v = hc[i];
+// Hint candidate i (lane-varying) of the MB's hint record hv (lane j < 10
+// holds dword j of its pcme_hint; cand[k] is dword k), rounded like the
+// search_part candidates (slice.rs:1622); i outside [0, nh) reads as 0.
+__device__ __forceinline__ void hint_cand(uint32_t hv, int i, int nh, int &x, int &y) {
+  const bool in = (i >= 0) & (i < nh);
+  uint32_t v = (uint32_t)__builtin_amdgcn_ds_bpermute((in ? i : 0) << 2, (int)hv);
+  v = in ? v : 0u;
   x = ((int)(int16_t)(v & 0xFFFFu) + 2) & ~3;
   y = (((int)v >> 16) + 2) & ~3;
 }
@@ -456,12 +462,12 @@ __device__ __forceinline__ void hint_cand(const uint32_t *hc, int i, int nh, int
 // SliceEnc::search (slice.rs:1590-1723) for a W x H block at picture
 // position (bx, by).  mvp, cands are wave-uniform; returns (best, cost).
 // lds = this wave's LDS_PER_WAVE bytes.  nh >= 0 selects the pruned search:
-// the hint candidates hc[0..nh) are scanned after the search_part
+// the hint candidates 0..nh-1 of hv are scanned after the search_part
 // candidates (slots 3-7, then slots 0-2 of a second pass) and the hexagon
 // descent is skipped.
 template <int W, int H, bool PR>
 __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const Lane &L, int bx, int by, int mvpx,
-                                       int mvpy, int c0x, int c0y, int c1x, int c1y, int ncand, const uint32_t *hc,
+                                       int mvpy, int c0x, int c0y, int c1x, int c1y, int ncand, uint32_t hv,
                                        int nh, uint32_t *lds) {
   uint32_t *lw = lds;
   uint32_t *lsp = lds + WIN_BYTES / 4;
@@ -500,7 +506,7 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
     const int r0x = (c0x + 2) & ~3, r0y = (c0y + 2) & ~3;
     const int r1x = (c1x + 2) & ~3, r1y = (c1y + 2) & ~3;
     int hx, hy;
-    hint_cand(hc, L.slot - 3, nh, hx, hy);
+    hint_cand(hv, L.slot - 3, nh, hx, hy);
     const bool hs = L.slot >= 3;
     const int cx = L.slot == 1 ? r0x : (L.slot == 2 ? r1x : (hs ? hx : bxv));
     const int cy = L.slot == 1 ? r0y : (L.slot == 2 ? r1y : (hs ? hy : byv));
@@ -516,7 +522,7 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
   if (PR && nh > 5) {
     // hint candidates 5..7 continue the scan (slots 0-2)
     int hx, hy;
-    hint_cand(hc, L.slot + 5, nh, hx, hy);
+    hint_cand(hv, L.slot + 5, nh, hx, hy);
     const bool valid = (L.slot + 5 < nh) & in_box(ok, hx, hy);
     const uint32_t k = sad_cost<false>(c, L, s, bx, by, hx, hy, valid ? hx : bxv, valid ? hy : byv, mvpx, mvpy,
                                        lw, 0, 0);
@@ -556,6 +562,9 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
   // slice.rs:1657-1682: two rounds of 8-neighbour integer refinement.  The
   // full search reads the full-pel window; a pruned search stages the four
   // sub-pel planes once (margin 3) and reads the full-pel plane among them.
+  // The sub-pel planes stage while the square rounds run: two rounds move
+  // the best by at most 2 px, which margin 3 covers.
+  if constexpr (!PR) prefetch_sp<W, H, 3>(c, bx, by, bxv, byv, sx0, sy0, lsp);
   const int qdx = sq_dx(L.slot), qdy = sq_dy(L.slot);
   for (int it = 0; it < 2; it++) {
     if constexpr (PR)
@@ -580,7 +589,7 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
   v2s sp[16];
   src_pairs(s, sp);
   // slice.rs:1688: SATD cost at the integer best.
-  uint32_t bs = rdlane(satd_cost(c, L, sp, bx, by, bxv, byv, bxv, byv, mvpx, mvpy, lsp, sx0, sy0), 0);
+  uint32_t bs = rdlane(satd_cost<8, true>(c, L, sp, bx, by, bxv, byv, bxv, byv, mvpx, mvpy, lsp, sx0, sy0), 0);
   // slice.rs:1689-1721: half-pel round, then 1 + subpel_rounds quarter-pel.
 #pragma unroll
   for (int step = 2; step >= 1; step--) {
@@ -589,8 +598,11 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
       ensure_sp<W, H>(c, bx, by, bxv, byv, sx0, sy0, lsp);
       const int cx = bxv + qdx * step, cy = byv + qdy * step;
       const bool valid = in_box(ok, cx, cy);
-      const uint32_t k = satd_cost(c, L, sp, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy, lsp,
-                                   sx0, sy0);
+      // the half-pel round's candidates and fallback sit at half positions
+      const uint32_t k = step == 2 ? satd_cost<8, true>(c, L, sp, bx, by, cx, cy, valid ? cx : bxv,
+                                                        valid ? cy : byv, mvpx, mvpy, lsp, sx0, sy0)
+                                   : satd_cost<8, false>(c, L, sp, bx, by, cx, cy, valid ? cx : bxv,
+                                                         valid ? cy : byv, mvpx, mvpy, lsp, sx0, sy0);
       const uint32_t key = min_key(k, valid, L.slot);
       if ((key >> 3) >= bs) break;
       bs = key >> 3;
@@ -664,11 +676,11 @@ __device__ __forceinline__ bool take_key(PState &S, uint32_t key, int dx, int dy
 
 // search (slice.rs:1590-1723) for A (an AW x AH partition) and B (a BW x BH
 // partition) at the same time.  src* = block top-left of each source.
-// nh >= 0: pruned search with hint candidates hc[0..nh) (see search()).
+// nh >= 0: pruned search with hint candidates 0..nh-1 of hv (see search()).
 template <int AW, int AH, int BW, int BH, bool PR>
 __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B, const uint8_t *srcA,
                                             const uint8_t *srcB, int sstride, int c0x, int c0y, int c1ax,
-                                            int c1ay, int c1bx, int c1by, const uint32_t *hc, int nh,
+                                            int c1ay, int c1bx, int c1by, uint32_t hv, int nh,
                                             uint32_t *lds, SRes &ra, SRes &rb) {
   static_assert(AW * AH == 128 && BW == 8, "pair shapes");
   constexpr bool B8 = BH == 8;            // B is an 8x8 partition: 2 lanes per candidate
@@ -722,7 +734,7 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
     const int px = gsel(g, a0x, b0x), py = gsel(g, a0y, b0y);
     // a pruned search scans hint candidates 0..4 in slots 3-7 of this pass
     int hx = 0, hy = 0;
-    if constexpr (PR) hint_cand(hc, L.slot - 3, nh, hx, hy);
+    if constexpr (PR) hint_cand(hv, L.slot - 3, nh, hx, hy);
     const bool hs = PR && L.slot >= 3;
     const int cx = L.slot == 1 ? r0x : (L.slot == 2 ? r1x : (hs ? hx : px));
     const int cy = L.slot == 1 ? r0y : (L.slot == 2 ? r1y : (hs ? hy : py));
@@ -751,7 +763,7 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
   if (PR && nh > 5) {
     // hint candidates 5..7 continue each scan (slots 0-2)
     int hx, hy;
-    hint_cand(hc, L.slot + 5, nh, hx, hy);
+    hint_cand(hv, L.slot + 5, nh, hx, hy);
     const Box ok = g ? B.ok : A.ok;
     const int ox = gsel(g, A.x, B.x), oy = gsel(g, A.y, B.y);
     const bool valid = L.active & (L.slot + 5 < nh) & in_box(ok, hx, hy);
@@ -812,6 +824,10 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
   // slice.rs:1657-1682: two rounds of 8-neighbour refinement of both (a
   // pruned search reads the full-pel plane of its sub-pel region, staged
   // once with margin 3).
+  if constexpr (!PR) {
+    prefetch_sp<AW, AH, 3>(c, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
+    prefetch_sp<BW, BH, 3>(c, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
+  }
   const int qdx = sq_dx(L.slot), qdy = sq_dy(L.slot);
   A.act = B.act = true;
   A.it = B.it = 0;
@@ -852,8 +868,8 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
   src_pairs(s, sp);
   {
     const int ox = gsel(g, A.x, B.x), oy = gsel(g, A.y, B.y);
-    const uint32_t k = satd_cost<LPC>(c, L, sp, bx, by, ox, oy, ox, oy, mvpx, mvpy, lsp, gsel(g, A.sx0, B.sx0),
-                                      gsel(g, A.sy0, B.sy0));
+    const uint32_t k = satd_cost<LPC, true>(c, L, sp, bx, by, ox, oy, ox, oy, mvpx, mvpy, lsp,
+                                            gsel(g, A.sx0, B.sx0), gsel(g, A.sy0, B.sy0));
     A.cost = rdlane(k, 0);
     B.cost = rdlane(k, 32);
   }
@@ -871,8 +887,11 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
       const Box ok = g ? B.ok : A.ok;
       const bool act = g ? B.act : A.act;
       const bool valid = L.active & act & in_box(ok, cx, cy);
-      const uint32_t k = satd_cost<LPC>(c, L, sp, bx, by, cx, cy, valid ? cx : ox, valid ? cy : oy, mvpx, mvpy,
-                                        lsp, gsel(g, A.sx0, B.sx0), gsel(g, A.sy0, B.sy0));
+      const uint32_t k =
+          step == 2 ? satd_cost<LPC, true>(c, L, sp, bx, by, cx, cy, valid ? cx : ox, valid ? cy : oy, mvpx, mvpy,
+                                           lsp, gsel(g, A.sx0, B.sx0), gsel(g, A.sy0, B.sy0))
+                    : satd_cost<LPC, false>(c, L, sp, bx, by, cx, cy, valid ? cx : ox, valid ? cy : oy, mvpx, mvpy,
+                                            lsp, gsel(g, A.sx0, B.sx0), gsel(g, A.sy0, B.sy0));
       uint32_t ka, kb;
       min_key_pair<B8>(k, valid, L.slot, ka, kb);
       if (A.act) {
@@ -1005,7 +1024,7 @@ struct PartRes {
 // One partition of search_part (slice.rs:1826-1866), then the next.
 template <int PART, int P, bool PR>
 __device__ __forceinline__ void part_step(const SCtx &c, const Nbr &n, Cur &cur, const uint8_t *src,
-                                          int sstride, int mbx, int mby, uint32_t seed, const uint32_t *hc,
+                                          int sstride, int mbx, int mby, uint32_t seed, uint32_t hv,
                                           int nh, PartRes &res, uint32_t *lds) {
   if constexpr (P < part_count(PART)) {
     constexpr int X = rect_x(PART, P), Y = rect_y(PART, P);
@@ -1016,33 +1035,33 @@ __device__ __forceinline__ void part_step(const SCtx &c, const Nbr &n, Cur &cur,
     load_src(src + Y * sstride + X, sstride, L, s);
     const uint32_t m0 = P == 0 ? 0u : res.mv[0];
     SRes r = search<W, H, PR>(c, s, L, mbx + X, mby + Y, mv_x(pv), mv_y(pv), mv_x(seed), mv_y(seed), mv_x(m0),
-                          mv_y(m0), 2, hc, nh, lds);
+                          mv_y(m0), 2, hv, nh, lds);
     const uint32_t m = pack_mv(r.x, r.y);
     res.mv[P] = m;
     res.mvp[P] = pv;
     res.total += r.cost;
     cur.mv[P] = m;
-    part_step<PART, P + 1, PR>(c, n, cur, src, sstride, mbx, mby, seed, hc, nh, res, lds);
+    part_step<PART, P + 1, PR>(c, n, cur, src, sstride, mbx, mby, seed, hv, nh, res, lds);
   }
 }
 
 // search_part (slice.rs:1810-1868), list 0.  src = MB top-left.
 template <int PART, bool PR>
 __device__ __forceinline__ void search_part(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
-                                            int mbx, int mby, uint32_t seed, const uint32_t *hc, int nh,
+                                            int mbx, int mby, uint32_t seed, uint32_t hv, int nh,
                                             PartRes &res, uint32_t *lds) {
   Cur cur = {};
 #pragma unroll
   for (int i = 0; i < 4; i++) { res.mv[i] = res.mvp[i] = 0u; }
   res.total = 0;
-  part_step<PART, 0, PR>(c, n, cur, src, sstride, mbx, mby, seed, hc, nh, res, lds);
+  part_step<PART, 0, PR>(c, n, cur, src, sstride, mbx, mby, seed, hv, nh, res, lds);
 }
 
 // search_part (slice.rs:1810-1868) of P16x8 (ra) and P8x16 (rb) together:
 // partition p of both modes in one paired search.
 template <bool PR>
 __device__ __forceinline__ void search_part_12(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
-                                               int mbx, int mby, uint32_t seed, const uint32_t *hc, int nh,
+                                               int mbx, int mby, uint32_t seed, uint32_t hv, int nh,
                                                PartRes &ra, PartRes &rb, uint32_t *lds) {
   Cur ca = {}, cb = {};
 #pragma unroll
@@ -1053,7 +1072,7 @@ __device__ __forceinline__ void search_part_12(const SCtx &c, const Nbr &n, cons
   uint32_t pa = mvp<1, 0, 0, 0, 4, 1>(n, ca), pb = mvp<2, 0, 0, 0, 2, 3>(n, cb);
   A.bx = mbx; A.by = mby; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
   B.bx = mbx; B.by = mby; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
-  search_pair<16, 8, 8, 16, PR>(c, A, B, src, src, sstride, mv_x(seed), mv_y(seed), 0, 0, 0, 0, hc, nh, lds, xa,
+  search_pair<16, 8, 8, 16, PR>(c, A, B, src, src, sstride, mv_x(seed), mv_y(seed), 0, 0, 0, 0, hv, nh, lds, xa,
                                 xb);
   ra.mv[0] = ca.mv[0] = pack_mv(xa.x, xa.y); ra.mvp[0] = pa; ra.total = xa.cost;
   rb.mv[0] = cb.mv[0] = pack_mv(xb.x, xb.y); rb.mvp[0] = pb; rb.total = xb.cost;
@@ -1063,7 +1082,7 @@ __device__ __forceinline__ void search_part_12(const SCtx &c, const Nbr &n, cons
   A.bx = mbx; A.by = mby + 8; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
   B.bx = mbx + 8; B.by = mby; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
   search_pair<16, 8, 8, 16, PR>(c, A, B, src + 8 * sstride, src + 8, sstride, mv_x(seed), mv_y(seed), mv_x(ra.mv[0]),
-              mv_y(ra.mv[0]), mv_x(rb.mv[0]), mv_y(rb.mv[0]), hc, nh, lds, xa, xb);
+              mv_y(ra.mv[0]), mv_x(rb.mv[0]), mv_y(rb.mv[0]), hv, nh, lds, xa, xb);
   ra.mv[1] = pack_mv(xa.x, xa.y); ra.mvp[1] = pa; ra.total += xa.cost;
   rb.mv[1] = pack_mv(xb.x, xb.y); rb.mvp[1] = pb; rb.total += xb.cost;
 }
@@ -1081,7 +1100,7 @@ __device__ __forceinline__ void search_part_123(const SCtx &c, const Nbr &n, con
   for (int i = 0; i < 4; i++) { r1.mv[i] = r1.mvp[i] = r2.mv[i] = r2.mvp[i] = r3.mv[i] = r3.mvp[i] = 0u; }
   PState A, B;
   SRes xa, xb;
-  const uint32_t *nohint = nullptr;
+  const uint32_t nohint = 0u;
   const int sx = mv_x(seed), sy = mv_y(seed);
   // step 1: 16x8 top (x 0, y 0, w 4 blocks) with 8x8 top-left (x 0, y 0, w 2)
   uint32_t pa = mvp<1, 0, 0, 0, 4, 1>(n, c1), pb = mvp<3, 0, 0, 0, 2, 0>(n, c3);
@@ -1157,9 +1176,9 @@ __device__ __forceinline__ void take_mode(MbRes &o, const PartRes &r, int part, 
 // against the running best (slice.rs:1925-1945).
 template <bool PR>
 __device__ __forceinline__ void mode_12(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride, int mbx,
-                                        int mby, const uint32_t *hc, int nh, MbRes &o, uint32_t *lds) {
+                                        int mby, uint32_t hv, int nh, MbRes &o, uint32_t *lds) {
   PartRes r, r2;
-  search_part_12<PR>(c, n, src, sstride, mbx, mby, o.m16, hc, nh, r, r2, lds);
+  search_part_12<PR>(c, n, src, sstride, mbx, mby, o.m16, hv, nh, r, r2, lds);
   o.cost[1] = r.total;
   uint32_t cc = r.total + ((c.lam * 3u) >> 8);
   bool t = cc < o.best_cost;
@@ -1177,9 +1196,9 @@ __device__ __forceinline__ void mode_12(const SCtx &c, const Nbr &n, const uint8
 // Mode P8x8 and its choice.
 template <bool PR>
 __device__ __forceinline__ void mode_3(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride, int mbx,
-                                       int mby, const uint32_t *hc, int nh, MbRes &o, uint32_t *lds) {
+                                       int mby, uint32_t hv, int nh, MbRes &o, uint32_t *lds) {
   PartRes r;
-  search_part<3, PR>(c, n, src, sstride, mbx, mby, o.m16, hc, nh, r, lds);
+  search_part<3, PR>(c, n, src, sstride, mbx, mby, o.m16, hv, nh, r, lds);
   o.cost[3] = r.total;
   const uint32_t cc = r.total + ((c.lam * 10u) >> 8);
   const bool t = cc < o.best_cost;
@@ -1189,16 +1208,16 @@ __device__ __forceinline__ void mode_3(const SCtx &c, const Nbr &n, const uint8_
 }
 
 // ME-only analysis of one MB (slice.rs:1884-1945 without early skip,
-// intra and RD).  nh >= 0: pruned MB (hint candidates hc[0..nh), hinted
+// intra and RD).  nh >= 0: pruned MB (hint candidates 0..nh-1 of hv, hinted
 // mode hpart): P16x16, then only P16x8 + P8x16 (hpart 1 or 2) or only P8x8
 // (hpart 3), with the same strict-< choice.
 template <bool PR>
 __device__ __forceinline__ void analyse_mb(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
-                                           int mbx, int mby, int partitions, const uint32_t *hc, int nh,
+                                           int mbx, int mby, int partitions, uint32_t hv, int nh,
                                            int hpart, MbRes &o, uint32_t *lds) {
   const uint32_t skip = pskip(n);
   PartRes r16;
-  search_part<0, PR>(c, n, src, sstride, mbx, mby, skip, hc, nh, r16, lds);
+  search_part<0, PR>(c, n, src, sstride, mbx, mby, skip, hv, nh, r16, lds);
   const uint32_t cost16 = r16.total;
   o.part = 0;
   take_if(o.best, r16, true);
@@ -1211,9 +1230,9 @@ __device__ __forceinline__ void analyse_mb(const SCtx &c, const Nbr &n, const ui
   if (PR) {
     // A pruned MB searches the hinted mode only: P16x8 with P8x16, or P8x8.
     if (split & ((hpart == 1) | (hpart == 2)))
-      mode_12<PR>(c, n, src, sstride, mbx, mby, hc, nh, o, lds);
+      mode_12<PR>(c, n, src, sstride, mbx, mby, hv, nh, o, lds);
     else if (split & (hpart == 3))
-      mode_3<PR>(c, n, src, sstride, mbx, mby, hc, nh, o, lds);
+      mode_3<PR>(c, n, src, sstride, mbx, mby, hv, nh, o, lds);
   } else if (split) {
     PartRes r1, r2, r3;
     search_part_123(c, n, src, sstride, mbx, mby, o.m16, r1, r2, r3, lds);
@@ -1231,25 +1250,17 @@ __device__ __forceinline__ uint32_t blk_mv(const MbRes &o, int r) {
 
 // 32 dwords of pcme_mb_out, lane i < 32 stores dword i.
 __device__ __forceinline__ void store_out(pcme_mb_out *out, const MbRes &o) {
-  int l = (int)lane_id();
-  uint32_t v;
-  if (l < 16) {
-    v = blk_mv(o, l);
-  } else if (l < 20) {
-    v = sel4(l - 16, o.best.mv[0], o.best.mv[1], o.best.mv[2], o.best.mv[3]);
-  } else if (l < 24) {
-    v = sel4(l - 20, o.best.mvp[0], o.best.mvp[1], o.best.mvp[2], o.best.mvp[3]);
-  } else if (l < 28) {
-    v = sel4(l - 24, o.cost[0], o.cost[1], o.cost[2], o.cost[3]);
-  } else if (l == 28) {
-    v = o.best_cost;
-  } else if (l == 29) {
-    v = o.skip;
-  } else if (l == 30) {
-    v = o.m16;
-  } else {
-    v = (uint32_t)o.part;
-  }
+  // Branch-free: every value is wave-uniform, so each lane picks its dword
+  // with selects (lane-range branches cost more than the whole record).
+  const int l = (int)lane_id();
+  const int q = l & 3;
+  const int pt = o.part;
+  const int kb = pt == 0 ? 0 : pt == 1 ? (l >> 3) : pt == 2 ? (q >> 1) : (((l >> 3) << 1) | (q >> 1));
+  const uint32_t vm = sel4(l < 16 ? kb : q, o.best.mv[0], o.best.mv[1], o.best.mv[2], o.best.mv[3]);
+  const uint32_t vp = sel4(q, o.best.mvp[0], o.best.mvp[1], o.best.mvp[2], o.best.mvp[3]);
+  const uint32_t vc = sel4(q, o.cost[0], o.cost[1], o.cost[2], o.cost[3]);
+  const uint32_t vx = sel4(q, o.best_cost, o.skip, o.m16, (uint32_t)pt);
+  const uint32_t v = l < 20 ? vm : (l < 24 ? vp : (l < 28 ? vc : vx));
   if (l < 32) ((uint32_t *)out)[l] = v;
 }
 
@@ -1289,6 +1300,15 @@ __device__ __forceinline__ bool fetch_words(const uint32_t *xw, uint32_t tag, ui
 #pragma unroll
   for (int i = 0; i < 4; i++) w[i] = rdlane(v, i);
   return true;
+}
+
+static_assert(sizeof(pcme_hint) == 40 && offsetof(pcme_hint, conf_part) == 32, "pcme_hint: 10 dwords, cand[] first");
+// The 10 dwords of a pcme_hint, dword i in lane i (lanes >= 10 read 0).
+__device__ __forceinline__ uint32_t load_hint(const pcme_hint *h) {
+  const int l = (int)lane_id();
+  uint32_t v = 0u;
+  if (l < 10) v = ((const uint32_t *)h)[l];
+  return v;
 }
 
 template <bool HINTS>
@@ -1340,6 +1360,11 @@ __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, con
       if (ok && mbw > 1) ok = fetch_words(xup + 4, tag, err, spin_limit, nxt);
     }
     if (!ok) return;
+    // Hint records travel one MB ahead of their use (a vector load whose
+    // round trip hides under the previous MB's search).
+    const bool hinted = HINTS && J.hint != nullptr;
+    uint32_t hnext = 0u;
+    if (hinted) hnext = load_hint(J.hint + (size_t)my * mbw);
     for (int mx = 0; mx < mbw; mx++) {
       if (my > 0 && mx > 0) {
         n.TL = n.T[3];
@@ -1354,12 +1379,13 @@ __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, con
       n.aTR = my > 0 && mx + 1 < mbw;
       n.TR = nxt[0];
       const uint8_t *src = J.src + (size_t)(16 * my + PCME_PAD) * stride + 16 * mx + PCME_PAD;
-      // Pruner hint of this MB (wave-uniform); a full search without one.
-      const uint32_t *hc = nullptr;
+      // Pruner hint of this MB; a full search without one.
+      uint32_t hv = 0u;
       int nh = -1, hpart = 0;
-      if (HINTS && J.hint) {
-        hc = (const uint32_t *)(J.hint + (size_t)my * mbw + mx);  // cand[0..8) = words 0-7
-        const uint32_t w9 = hc[9];  // part | ncand << 8 | prune << 16
+      if (hinted) {
+        hv = hnext;
+        if (mx + 1 < mbw) hnext = load_hint(J.hint + (size_t)my * mbw + mx + 1);
+        const uint32_t w9 = rdlane(hv, 9);  // part | ncand << 8 | prune << 16
         if ((w9 >> 16) & 0xFFu) {
           nh = min((int)((w9 >> 8) & 0xFFu), 8);
           hpart = (int)(w9 & 3u);
@@ -1367,9 +1393,9 @@ __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, con
       }
       MbRes o;
       if (HINTS && nh >= 0)
-        analyse_mb<true>(c, n, src, stride, 16 * mx, 16 * my, J.partitions, hc, nh, hpart, o, s_lds);
+        analyse_mb<true>(c, n, src, stride, 16 * mx, 16 * my, J.partitions, hv, nh, hpart, o, s_lds);
       else
-        analyse_mb<false>(c, n, src, stride, 16 * mx, 16 * my, J.partitions, hc, nh, hpart, o, s_lds);
+        analyse_mb<false>(c, n, src, stride, 16 * mx, 16 * my, J.partitions, hv, nh, hpart, o, s_lds);
       store_out(J.out + (size_t)my * mbw + mx, o);
       // Publish the bottom row (blocks 12..15) for the row below.
       if (l < 4) {
@@ -1424,19 +1450,19 @@ __global__ void __launch_bounds__(64) search_batch_kernel(const pcme_search_job 
   int c1x = J.ncands > 1 ? J.cands[1].x : 0, c1y = J.ncands > 1 ? J.cands[1].y : 0;
   SRes r;
   r.x = r.y = 0; r.cost = 0;
-  const uint32_t *hc = nullptr;  // full search: no hints
+  const uint32_t hv = 0u;  // full search: no hints
   if (J.w == 16 && J.h == 16) {
     Lane L = lane_layout<16, 16>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
-    r = search<16, 16, false>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hc, -1, s_lds);
+    r = search<16, 16, false>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hv, -1, s_lds);
   } else if (J.w == 16 && J.h == 8) {
     Lane L = lane_layout<16, 8>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
-    r = search<16, 8, false>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hc, -1, s_lds);
+    r = search<16, 8, false>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hv, -1, s_lds);
   } else if (J.w == 8 && J.h == 16) {
     Lane L = lane_layout<8, 16>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
-    r = search<8, 16, false>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hc, -1, s_lds);
+    r = search<8, 16, false>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hv, -1, s_lds);
   } else {
     Lane L = lane_layout<8, 8>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
-    r = search<8, 8, false>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hc, -1, s_lds);
+    r = search<8, 8, false>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hv, -1, s_lds);
   }
   if (lane_id() == 0) {
     pcme_mv m;
