@@ -125,8 +125,7 @@ def main():
 
     def prepare():
         for dc in clips:
-            dc.pad(sp)
-            dc.hpel(0, dc.n - 1, sp)
+            dc.prepare(sp)
 
     plan = pcme.Plan(jobs)
     prepare()
@@ -171,8 +170,7 @@ def main():
         with torch.cuda.stream(s2):
             with torch.cuda.graph(g, stream=s2, capture_error_mode="thread_local"):
                 for dc in clips:
-                    dc.pad(s2.cuda_stream)
-                    dc.hpel(0, dc.n - 1, s2.cuda_stream)
+                    dc.prepare(s2.cuda_stream)
                 plan.launch(s2.cuda_stream)
         torch.cuda.synchronize()
         g.replay()
@@ -267,8 +265,7 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
 
     def prepare():
         for dc in clips:
-            dc.pad(sp)
-            dc.hpel(0, dc.n - 1, sp)
+            dc.prepare(sp)
 
     def infer():
         for qp, js in groups.items():

@@ -141,6 +141,16 @@ int pcme_pad_planes(const uint8_t *src, int64_t src_pitch, int32_t sstride,
 int pcme_hpel(const uint8_t *ref, uint8_t *h, uint8_t *v, uint8_t *c,
               int64_t pitch, int32_t n, int32_t mbw, int32_t mbh,
               void *stream);
+/* pcme_pad_planes (luma: pad 64, aw = 16*mbw, dstride = aw + 128) followed
+ * by pcme_hpel, in one pass from the unpadded source: the same four planes
+ * bit for bit (mod.rs:842-873 import, frame.rs:58-122 extend + half-pel
+ * planes, picture.rs:318-380).  Plane i of src (device) is w x h at
+ * src + i*src_pitch with row stride sstride; its full, H, V and C planes are
+ * written at full/hp/vp/cp + i*pitch (device, 4-byte aligned). */
+int pcme_luma_planes(const uint8_t *src, int64_t src_pitch, int32_t sstride,
+                     int32_t w, int32_t h, uint8_t *full, uint8_t *hp,
+                     uint8_t *vp, uint8_t *cp, int64_t pitch, int32_t n,
+                     int32_t mbw, int32_t mbh, void *stream);
 
 /* ---- the search ----------------------------------------------------- */
 /* ME-only field over a batch of P frames: for every macroblock of every

@@ -365,6 +365,27 @@ int pcme_hpel(const uint8_t *ref, uint8_t *h, uint8_t *v, uint8_t *c, int64_t pi
   return PCME_OK;
 }
 
+int pcme_luma_planes(const uint8_t *src, int64_t src_pitch, int32_t sstride, int32_t w, int32_t h,
+                     uint8_t *full, uint8_t *hp, uint8_t *vp, uint8_t *cp, int64_t pitch, int32_t n, int32_t mbw,
+                     int32_t mbh, void *stream) {
+  if (n < 0 || mbw < 1 || mbh < 1 || w < 1 || h < 1 || w > 16 * mbw || h > 16 * mbh || sstride < w || !src ||
+      !full || !hp || !vp || !cp)
+    return fail(PCME_EINVAL, "pcme_luma_planes: bad geometry");
+  const int stride = 16 * mbw + 2 * PCME_PAD, rows = 16 * mbh + 2 * PCME_PAD;
+  if (pitch < (int64_t)stride * rows || (pitch & 3) ||
+      (((uintptr_t)full | (uintptr_t)hp | (uintptr_t)vp | (uintptr_t)cp) & 3))
+    return fail(PCME_EINVAL, "pcme_luma_planes: pitch smaller than a plane or not 4-byte aligned");
+  if (n == 0) return PCME_OK;
+  Dev *dv;
+  int rc = get_dev(&dv);
+  if (rc) return rc;
+  dim3 grid((unsigned)((stride / 4 + 255) / 256), (unsigned)((rows + LP_TH - 1) / LP_TH), (unsigned)n);
+  hipLaunchKernelGGL(luma_planes_kernel, grid, dim3(256), 0, (hipStream_t)stream, src, src_pitch, sstride, w, h,
+                     full, hp, vp, cp, pitch, stride, rows);
+  HIPCHK(hipGetLastError());
+  return PCME_OK;
+}
+
 int pcme_search_frames(const pcme_frame_job *jobs, int njobs, void *stream) {
   return search_frames_impl(jobs, njobs, (hipStream_t)stream, false);
 }

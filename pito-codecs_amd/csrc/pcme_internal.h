@@ -38,6 +38,11 @@ __global__ void pad_planes_kernel(const uint8_t *__restrict__ src, int64_t src_p
 __global__ void hpel_kernel(const uint8_t *__restrict__ ref, uint8_t *__restrict__ ho,
                             uint8_t *__restrict__ vo, uint8_t *__restrict__ co, int64_t pitch,
                             int stride, int rows);
+__global__ void luma_planes_kernel(const uint8_t *__restrict__ src, int64_t src_pitch, int sstride, int w,
+                                   int h, uint8_t *__restrict__ full, uint8_t *__restrict__ hp,
+                                   uint8_t *__restrict__ vp, uint8_t *__restrict__ cp, int64_t pitch, int stride,
+                                   int rows);
+#define LP_TH 32  // rows of a luma_planes_kernel band
 __global__ void pred_frame_kernel(const uint8_t *__restrict__ pl0, const uint8_t *__restrict__ pl1,
                                   const uint8_t *__restrict__ pl2, const uint8_t *__restrict__ pl3,
                                   const uint8_t *__restrict__ pu, const uint8_t *__restrict__ pv,

@@ -1578,6 +1578,139 @@ __global__ void __launch_bounds__(256) hpel_kernel(const uint8_t *__restrict__ r
   }
 }
 
+// ------------------------------------------------------ luma planes kernel
+
+// Import + edge extension + the three half-pel planes of a luma frame in one
+// pass from the unpadded source (pcme_pad_planes then pcme_hpel, fused).
+// Clamping a tap to the padded plane and then the padded pixel to the
+// picture is clamping the tap to the picture, so every output pixel reads
+// the source directly: full(y, x) = S(cy(y), cx(x)), H/V = the 6-tap filters
+// (1, -5, 20, 20, -5, 1) along x / y rounded (+16) >> 5, C = the same filter
+// along y over the unrounded horizontal sums, rounded (+512) >> 10, each
+// clipped to 0..255 (picture.rs:318-380), with cy, cx the clamps into the
+// picture.  A thread owns 4 output columns of an LP_TH-row band and slides
+// a 6-row window down it: per source row 8 byte pairs (packed i16), the
+// horizontal sums of its 4 columns; per output row the H, V and C dwords.
+
+// Byte pair (b[k], b[k+1]) of the 12 source bytes b[-4..7] held in D0 (b[-4..-1]),
+// D1 (b[0..3]), D2 (b[4..7]) as two zero-extended 16-bit lanes.
+__device__ __forceinline__ v2s lp_pair(uint32_t D0, uint32_t D1, uint32_t D2, int k) {
+  switch (k) {
+    case -2: return as_v2s(__builtin_amdgcn_perm(0u, D0, 0x0C030C02u));
+    case -1: return as_v2s(__builtin_amdgcn_perm(D1, D0, 0x0C040C03u));
+    case 0: return as_v2s(__builtin_amdgcn_perm(0u, D1, 0x0C010C00u));
+    case 1: return as_v2s(__builtin_amdgcn_perm(0u, D1, 0x0C020C01u));
+    case 2: return as_v2s(__builtin_amdgcn_perm(0u, D1, 0x0C030C02u));
+    case 3: return as_v2s(__builtin_amdgcn_perm(D2, D1, 0x0C040C03u));
+    case 4: return as_v2s(__builtin_amdgcn_perm(0u, D2, 0x0C010C00u));
+    default: return as_v2s(__builtin_amdgcn_perm(0u, D2, 0x0C020C01u));
+  }
+}
+
+// The 6-tap filter of packed i16 lanes: a + f - 5 (b + e) + 20 (c + d).
+__device__ __forceinline__ v2s lp_tap6(v2s a, v2s b, v2s c, v2s d, v2s e, v2s f) {
+  return (a + f) - (v2s)5 * (b + e) + (v2s)20 * (c + d);
+}
+
+// Four bytes of two rounded, clipped i16 pairs (lo = bytes 0-1, hi = 2-3).
+__device__ __forceinline__ uint32_t lp_bytes(v2s lo, v2s hi) {
+  const v2s z = (v2s)0, m = (v2s)255;
+  lo = __builtin_elementwise_min(__builtin_elementwise_max(lo, z), m);
+  hi = __builtin_elementwise_min(__builtin_elementwise_max(hi, z), m);
+  return __builtin_amdgcn_perm(__builtin_bit_cast(uint32_t, hi), __builtin_bit_cast(uint32_t, lo), 0x06040200u);
+}
+
+// Column j (0 or 1 = low or high lane) of the 2-D filter over six rows of
+// horizontal sums (i32: the sum reaches 42 * 10710).
+__device__ __forceinline__ int lp_c(v2s h0, v2s h1, v2s h2, v2s h3, v2s h4, v2s h5, int j) {
+  const uint32_t sel = j ? 0x07060302u : 0x05040100u;
+  const auto pr = [sel](v2s a, v2s b) {
+    return as_v2s(__builtin_amdgcn_perm(__builtin_bit_cast(uint32_t, b), __builtin_bit_cast(uint32_t, a), sel));
+  };
+  int acc = __builtin_amdgcn_sdot2(pr(h0, h5), (v2s)1, 0, false);
+  acc = __builtin_amdgcn_sdot2(pr(h1, h4), (v2s)-5, acc, false);
+  return __builtin_amdgcn_sdot2(pr(h2, h3), (v2s)20, acc, false);
+}
+
+__device__ __forceinline__ uint32_t lp_cbyte(int v) {
+  int b = (v + 512) >> 10;
+  b = b < 0 ? 0 : (b > 255 ? 255 : b);
+  asm volatile("" : "+v"(b));  // see hpel_kernel: no fused byte packing
+  return (uint32_t)b;
+}
+
+__global__ void __launch_bounds__(256) luma_planes_kernel(const uint8_t *__restrict__ src, int64_t src_pitch,
+                                                          int sstride, int w, int h, uint8_t *__restrict__ full,
+                                                          uint8_t *__restrict__ hp, uint8_t *__restrict__ vp,
+                                                          uint8_t *__restrict__ cp, int64_t pitch, int stride,
+                                                          int rows) {
+  const int x0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+  if (x0 >= stride) return;  // stride is a multiple of 16
+  const int y0 = blockIdx.y * LP_TH;
+  const uint8_t *s = src + (size_t)blockIdx.z * src_pitch;
+  const size_t po = (size_t)blockIdx.z * pitch;
+  const int sx0 = x0 - PCME_PAD;
+  // Interior lanes read three aligned dwords of a row; lanes near the
+  // picture's left/right edges gather clamped bytes.
+  const bool fast = (sx0 >= 4) & (sx0 + 8 <= w) & ((((uintptr_t)s | (uintptr_t)sstride) & 3u) == 0u);
+  v2s q01[6], q23[6], h01[6], h23[6];
+#pragma unroll
+  for (int i = 0; i < LP_TH + 5; i++) {
+    const int yy = y0 - 2 + i;  // padded row of source row i of the window
+    int sy = yy - PCME_PAD;
+    sy = sy < 0 ? 0 : (sy > h - 1 ? h - 1 : sy);
+    const uint8_t *row = s + (size_t)sy * sstride;
+    uint32_t D0, D1, D2;
+    if (fast) {
+      const uint32_t *r32 = (const uint32_t *)(row + sx0);
+      D0 = r32[-1];
+      D1 = r32[0];
+      D2 = r32[1];
+    } else {
+      uint32_t b[9];
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        int x = sx0 - 2 + k;
+        x = x < 0 ? 0 : (x > w - 1 ? w - 1 : x);
+        b[k] = row[x];
+      }
+      D0 = (b[0] << 16) | (b[1] << 24);
+      D1 = b[2] | (b[3] << 8) | (b[4] << 16) | (b[5] << 24);
+      D2 = b[6] | (b[7] << 8) | (b[8] << 16);
+    }
+    const int j = i % 6;
+    const v2s pm2 = lp_pair(D0, D1, D2, -2), pm1 = lp_pair(D0, D1, D2, -1), p0 = lp_pair(D0, D1, D2, 0),
+              p1 = lp_pair(D0, D1, D2, 1), p2 = lp_pair(D0, D1, D2, 2), p3 = lp_pair(D0, D1, D2, 3),
+              p4 = lp_pair(D0, D1, D2, 4), p5 = lp_pair(D0, D1, D2, 5);
+    q01[j] = p0;
+    q23[j] = p2;
+    h01[j] = lp_tap6(pm2, pm1, p0, p1, p2, p3);
+    h23[j] = lp_tap6(p0, p1, p2, p3, p4, p5);
+    if (i < 5) continue;
+    // Output row y = yy - 3 uses window rows i-5 .. i (source rows y-2 .. y+3).
+    const int y = yy - 3;
+    if (y >= rows) continue;
+    const int a0 = (i - 5) % 6, a1 = (i - 4) % 6, a2 = (i - 3) % 6, a3 = (i - 2) % 6, a4 = (i - 1) % 6, a5 = i % 6;
+    const size_t o = po + (size_t)y * stride + x0;
+    const uint32_t fv = __builtin_amdgcn_perm(__builtin_bit_cast(uint32_t, q23[a2]),
+                                              __builtin_bit_cast(uint32_t, q01[a2]), 0x06040200u);
+    const uint32_t hv = lp_bytes((h01[a2] + (v2s)16) >> (v2s)5, (h23[a2] + (v2s)16) >> (v2s)5);
+    const v2s v01 = lp_tap6(q01[a0], q01[a1], q01[a2], q01[a3], q01[a4], q01[a5]);
+    const v2s v23 = lp_tap6(q23[a0], q23[a1], q23[a2], q23[a3], q23[a4], q23[a5]);
+    const uint32_t vv = lp_bytes((v01 + (v2s)16) >> (v2s)5, (v23 + (v2s)16) >> (v2s)5);
+    const uint32_t c0 = lp_cbyte(lp_c(h01[a0], h01[a1], h01[a2], h01[a3], h01[a4], h01[a5], 0));
+    const uint32_t c1 = lp_cbyte(lp_c(h01[a0], h01[a1], h01[a2], h01[a3], h01[a4], h01[a5], 1));
+    const uint32_t c2 = lp_cbyte(lp_c(h23[a0], h23[a1], h23[a2], h23[a3], h23[a4], h23[a5], 0));
+    const uint32_t c3 = lp_cbyte(lp_c(h23[a0], h23[a1], h23[a2], h23[a3], h23[a4], h23[a5], 1));
+    const uint32_t cv = __builtin_amdgcn_perm(__builtin_amdgcn_perm(c3, c2, 0x0C0C0400u),
+                                              __builtin_amdgcn_perm(c1, c0, 0x0C0C0400u), 0x05040100u);
+    *(uint32_t *)(full + o) = fv;
+    *(uint32_t *)(hp + o) = hv;
+    *(uint32_t *)(vp + o) = vv;
+    *(uint32_t *)(cp + o) = cv;
+  }
+}
+
 // ------------------------------------------------------------ pred kernel
 
 // One 256-thread block per MB: threads make the 16x16 luma prediction

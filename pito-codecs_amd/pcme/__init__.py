@@ -98,7 +98,7 @@ assert HINT_DTYPE.itemsize == 40
 # Every symbol include/pcme.h and include/pcme_synth.h declare.
 EXPORTS = (
     "pcme_version", "pcme_last_error", "pcme_device_count", "pcme_lambda",
-    "pcme_qpel_table", "pcme_pad_planes", "pcme_hpel", "pcme_search_frames",
+    "pcme_qpel_table", "pcme_pad_planes", "pcme_hpel", "pcme_luma_planes", "pcme_search_frames",
     "pcme_search_frames_sync", "pcme_search_batch", "pcme_pred_frame",
     "pcme_synth_clip", "pcme_plan_create", "pcme_plan_launch", "pcme_plan_check",
     "pcme_plan_set_grid", "pcme_plan_autotune", "pcme_plan_destroy", "pcme_set_spin_limit",
@@ -133,6 +133,8 @@ def load() -> ctypes.CDLL:
         "pcme_pad_planes": (c.c_int, [vp, c.c_int64, c.c_int32, c.c_int32, c.c_int32, vp, c.c_int64,
                                       c.c_int32, c.c_int32, c.c_int32, c.c_int32, c.c_int32, vp]),
         "pcme_hpel": (c.c_int, [vp, vp, vp, vp, c.c_int64, c.c_int32, c.c_int32, c.c_int32, vp]),
+        "pcme_luma_planes": (c.c_int, [vp, c.c_int64, c.c_int32, c.c_int32, c.c_int32, vp, vp, vp, vp, c.c_int64,
+                                       c.c_int32, c.c_int32, c.c_int32, vp]),
         "pcme_search_frames": (c.c_int, [c.POINTER(FrameJob), c.c_int, vp]),
         "pcme_search_frames_sync": (c.c_int, [c.POINTER(FrameJob), c.c_int, vp]),
         "pcme_search_batch": (c.c_int, [c.POINTER(SearchJob), c.c_int, vp, vp, vp]),
@@ -268,6 +270,21 @@ class DeviceClip:
         _check(lib.pcme_pad_planes(self.raw.data_ptr(), self.h * self.w, self.w, self.w, self.h,
                                    self.planes[0].data_ptr(), self.pitch, self.stride,
                                    16 * self.mbw, 16 * self.mbh, LUMA_PAD, self.n, s))
+        if self.has_chroma:
+            ch, cw = self.h // 2, self.w // 2
+            for k in range(2):
+                _check(lib.pcme_pad_planes(self.craw[k].data_ptr(), ch * cw, cw, cw, ch,
+                                           self.cplanes[k].data_ptr(), self.crows * self.cstride,
+                                           self.cstride, 8 * self.mbw, 8 * self.mbh, CHROMA_PAD,
+                                           self.n, s))
+
+    def prepare(self, stream=None) -> None:
+        """pad() and hpel() of every frame: the luma planes in one fused pass
+        (pcme_luma_planes), the chroma planes padded."""
+        lib, s = load(), _stream_ptr(stream)
+        _check(lib.pcme_luma_planes(self.raw.data_ptr(), self.h * self.w, self.w, self.w, self.h,
+                                    self.plane_ptr(0, 0), self.plane_ptr(1, 0), self.plane_ptr(2, 0),
+                                    self.plane_ptr(3, 0), self.pitch, self.n, self.mbw, self.mbh, s))
         if self.has_chroma:
             ch, cw = self.h // 2, self.w // 2
             for k in range(2):

@@ -61,6 +61,32 @@ def test_hpel_matches_oracle(gpu, oracle, w, h):
             np.testing.assert_array_equal(planes[1 + k, i], want[k], err_msg=f"plane {'HVC'[k]} frame {i}")
 
 
+@pytest.mark.parametrize("w,h", [(1920, 1080), (320, 240), (100, 70), (16, 16), (34, 18), (52, 200)])
+def test_fused_luma_planes_match_pad_and_hpel(gpu, oracle, w, h):
+    """pcme_luma_planes (one pass from the unpadded source) writes the four
+    planes of pcme_pad_planes + pcme_hpel bit for bit, on widths that are and
+    are not multiples of 4 (the clamped-byte path) and tiny pictures where
+    every lane is an edge lane."""
+    import torch
+    say("luma planes", w, h)
+    y, u, v = _clip(gpu, w, h, 3)
+    mbw, mbh = gpu.mb_dims(w, h)
+    a = gpu.DeviceClip(y, mbw, mbh, u, v)
+    a.pad()
+    a.hpel()
+    b = gpu.DeviceClip(y, mbw, mbh, u, v)
+    b.planes.fill_(0xA5)
+    b.prepare()
+    torch.cuda.synchronize()
+    pa, pb = a.planes.cpu().numpy(), b.planes.cpu().numpy()
+    for k in range(4):
+        np.testing.assert_array_equal(pb[k], pa[k], err_msg=f"plane {'FHVC'[k]}")
+    np.testing.assert_array_equal(b.cplanes.cpu().numpy(), a.cplanes.cpu().numpy())
+    want = oracle.hpel(oracle.pad_luma(y[1], mbw, mbh))
+    for k in range(3):
+        np.testing.assert_array_equal(pb[1 + k, 1], want[k], err_msg=f"oracle plane {'HVC'[k]}")
+
+
 CASES = [
     # (w, h, frames, me_range, subpel, partitions, qp, generator kwargs)
     (320, 240, 8, 16, 1, True, 28, dict(nrect=4, rect_lo=24, rect_hi=80, vmax=3.0, pan_max=1.0)),
