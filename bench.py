@@ -50,19 +50,36 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=("1080p", "batch", "4k"), default="1080p")
-    ap.add_argument("--cpu-frames", type=int, default=16, help="P frames of the CPU-baseline sample")
+    ap.add_argument("--cpu-frames", type=int, default=59, help="P frames of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-pruned", action="store_true", help="skip the CNN-pruned leg")
-    ap.add_argument("--ncand", type=int, default=8, help="pruner candidates per MB")
+    ap.add_argument("--ncand", type=int, default=5, help="pruner candidates per MB")
     ap.add_argument("--thr", type=float, default=0.6, help="pruner confidence threshold (partition head)")
-    ap.add_argument("--thr-mv", type=float, default=None,
-                    help="threshold on the motion head's candidate mass (default: --thr)")
+    ap.add_argument("--thr-mv", type=float, default=0.0,
+                    help="threshold on the motion head's candidate mass (0: the partition head decides)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known "
                          "traffic, for scaling the FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "kernels_1080p_traffic.md")
+
+
+def measured_traffic(workload):
+    """HBM-side bytes per launch of the search kernel on the 1080p workload,
+    from the FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (scaled by
+    a 1 GiB copy measured in the same runs); None for other workloads or
+    when the summary is absent."""
+    if workload != "1080p" or not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    with open(TRAFFIC_SUMMARY) as f:
+        for ln in f:
+            if ln.startswith("`search_frames_kernel`: bytes_per_launch="):
+                return int(ln.split("=", 1)[1])
+    return None
 
 
 def clip_specs(pcme, workload):
@@ -201,7 +218,10 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "search_frames_kernel", "achieved": round(achieved_gbs, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
-                     "traffic": None, "algorithmic_bytes_per_mb": BYTES_PER_MB,
+                     "traffic": measured_traffic(args.workload),
+                     "traffic_source": "profiles/kernels_1080p_traffic.md (FETCH_SIZE/WRITE_SIZE runs)"
+                     if measured_traffic(args.workload) is not None else None,
+                     "algorithmic_bytes_per_mb": BYTES_PER_MB,
                      "kernel_ms": round(search_ms, 3)},
         "autotune": tune,
         "graph": graph,

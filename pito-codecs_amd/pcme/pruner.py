@@ -225,9 +225,11 @@ def train(net: Net, data: TrainSet, epochs: int = 3, batch: int = 1024, lr: floa
 # ------------------------------------------------------- pruned search
 
 def search_clip(clip: DeviceClip, me_range: int, qp: int, net: Net | None = None, ncand: int = 8,
-                thr_part: float = 0.6, thr_mv: float = 0.6, stream=None, prepared: bool = False):
+                thr_part: float = 0.6, thr_mv: float = 0.6, stream=None, prepared: bool = False,
+                hints_in: np.ndarray | None = None):
     """ME-only field of every P frame of a device clip: the full search
-    (net None) or the pruned search with the network's hints.  Returns
+    (net None), the pruned search with the network's hints, or with given
+    hint records `hints_in` ([P frames, mbh, mbw] of HINT_DTYPE).  Returns
     (field, hints) as numpy records (hints None for the full search)."""
     from . import search_frames
     torch = clip.torch
@@ -240,12 +242,16 @@ def search_clip(clip: DeviceClip, me_range: int, qp: int, net: Net | None = None
     out = torch.empty((nb, nmb * MB_DTYPE.itemsize), dtype=torch.uint8, device=dev)
     hints = None
     hp = [None] * nb
-    if net is not None:
+    if hints_in is not None:
+        raw = np.ascontiguousarray(np.asarray(hints_in, HINT_DTYPE).reshape(nb, nmb)).view(np.uint8)
+        hints = torch.as_tensor(raw.reshape(nb, -1)).to(dev)
+        hp = [hints[t].data_ptr() for t in range(nb)]
+    elif net is not None:
         hints = torch.zeros((nb, nmb * HINT_DTYPE.itemsize), dtype=torch.uint8, device=dev)
         hp = [hints[t].data_ptr() for t in range(nb)]
     jobs = [frame_job(clip, t, t - 1, out[t - 1].data_ptr(), me_range, 1, True, qp, hint_ptr=hp[t - 1])
             for t in range(1, clip.n)]
-    if net is not None:
+    if net is not None and hints_in is None:
         net.hints(jobs, ncand, thr_part, thr_mv, stream)
     search_frames(jobs, stream, sync=True)
     field = out.cpu().numpy().view(MB_DTYPE).reshape(nb, clip.mbh, clip.mbw)

@@ -198,3 +198,54 @@ def test_pruned_search_invariants(oracle):
     for i in range(mbw * mbh):
         pt = int(got["part"][i])
         assert int(got["best_cost"][i]) == int(got["cost"][i][pt]) + pen[pt]
+
+
+# ------------------------------------------------- what the network buys
+
+def _gain_fields(oracle, w, h, nf, seed, qp=27):
+    """Plain, CNN-pruned and network-free pruned fields of a held-out clip
+    (the oracle's exact searches; hints from the oracle network on the
+    committed weights), with the oracle's work counters."""
+    import pcme
+    import pcme.pruner as P
+    spec = pcme.ClipSpec(w, h, nf, nrect=8, rect_lo=24, rect_hi=120, vmax=6.0, pan_max=1.0, seed=seed)
+    y, _, _ = pcme.synth_clip(spec)
+    mbw, mbh = pcme.mb_dims(w, h)
+    pads = [oracle.pad_luma(f, mbw, mbh) for f in y]
+    prm = C.unpack(P.load_weights(P.WEIGHTS, qp).astype(np.float64))
+    out = {k: [] for k in ("plain", "cnn", "none")}
+    ctr = {k: oracle.ocounters() for k in out}
+    for t in range(1, nf):
+        z = C.forward(C.inputs(pads[t], pads[t - 1], mbw, mbh, range(mbw * mbh)), prm, emulate=True)["logits"]
+        zm, zp = z[:, :C.MVC], z[:, C.MVC:]
+        top = np.argsort(-zm, axis=1, kind="stable")[:, :8]
+        pp = np.exp(zp - zp.max(1, keepdims=True))
+        pp /= pp.sum(1, keepdims=True)
+        hc = np.zeros(mbw * mbh, oracle.OHINT_DTYPE)
+        hc["cand"] = C.class_mv(top).astype(np.int16)
+        hc["part"] = np.argmax(zp, 1)
+        hc["ncand"] = 5
+        hc["prune"] = pp.max(1) >= 0.6
+        h0 = np.zeros(mbw * mbh, oracle.OHINT_DTYPE)  # prune every MB, no candidates, P16x16 only
+        h0["prune"] = 1
+        ref = oracle.Ref(pads[t - 1])
+        for k, hh in (("plain", None), ("cnn", hc), ("none", h0)):
+            out[k].append(oracle.search_frame(pads[t], ref, mbw, mbh, 32, 1, True, qp, counters=ctr[k], hints=hh))
+    work = {k: c.sad_px + 4 * c.satd_px for k, c in ctr.items()}
+    return {k: np.stack(v) for k, v in out.items()}, work
+
+
+def test_network_gain_over_the_search_without_it(oracle):
+    """On a held-out 640x368 clip the network's hints (5 candidates, partition
+    head at confidence 0.6, fallback below) keep the J-cost within 1% of the
+    plain search at well under half its work, and lose several times less
+    than the same pruned search given no hints at all (the method without the
+    network: predictor and search_part candidates, P16x16 only)."""
+    import pcme.pruner as P
+    f, work = _gain_fields(oracle, 640, 368, 4, 0x5EED9101)
+    j_cnn = P.judge(f["plain"], f["cnn"])
+    j_none = P.judge(f["plain"], f["none"])
+    assert work["cnn"] < 0.5 * work["plain"], work
+    assert j_cnn["jcost_increase"] < 0.01, j_cnn
+    assert j_cnn["jcost_increase"] * 4 < j_none["jcost_increase"], (j_cnn, j_none)
+    assert j_cnn["agreement"] > j_none["agreement"] and j_cnn["part_agreement"] > j_none["part_agreement"]

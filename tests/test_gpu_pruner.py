@@ -200,3 +200,23 @@ def test_pruned_search_without_confidence_is_the_full_search(gpu):
     field, hints = P.search_clip(dc, 16, 28, net=net, thr_part=1.5, thr_mv=1.5, prepared=True)
     assert np.all(hints["prune"] == 0)
     np.testing.assert_array_equal(field, plain)
+
+
+def test_network_gain_over_the_search_without_it(gpu):
+    """The product path on a held-out 640x368 clip with the committed
+    weights: the network's hints keep the J-cost within 1% of the plain
+    search and lose several times less than the same pruned search given no
+    hints (every MB pruned, no candidates, P16x16 only)."""
+    import pcme.pruner as P
+    say("pruner gain 640x368 x 3 P frames, committed weights")
+    _, dc = _dclip(gpu, 640, 368, 4, 0x5EED9101, nrect=8, rect_lo=24, rect_hi=120, vmax=6.0, pan_max=1.0)
+    net = P.Net(seed=1, max_batch=64, weights=P.WEIGHTS, qp=27)
+    plain, _ = P.search_clip(dc, 32, 27, prepared=True)
+    cnn, hints = P.search_clip(dc, 32, 27, net=net, ncand=5, thr_part=0.6, thr_mv=0.0, prepared=True)
+    none = np.zeros(hints.shape, gpu.HINT_DTYPE)
+    none["prune"] = 1
+    bare, _ = P.search_clip(dc, 32, 27, prepared=True, hints_in=none)
+    j_cnn, j_none = P.judge(plain, cnn), P.judge(plain, bare)
+    assert np.mean(hints["prune"]) > 0.8
+    assert j_cnn["jcost_increase"] < 0.01, j_cnn
+    assert j_cnn["jcost_increase"] * 4 < j_none["jcost_increase"], (j_cnn, j_none)
