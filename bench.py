@@ -57,8 +57,8 @@ def parse():
     ap.add_argument("--no-pruned", action="store_true", help="skip the CNN-pruned leg")
     ap.add_argument("--ncand", type=int, default=5, help="pruner candidates per MB")
     ap.add_argument("--thr", type=float, default=0.6, help="pruner confidence threshold (partition head)")
-    ap.add_argument("--thr-mv", type=float, default=0.0,
-                    help="threshold on the motion head's candidate mass (0: the partition head decides)")
+    ap.add_argument("--thr-mv", type=float, default=None,
+                    help="threshold on the motion head's candidate mass (default: --thr)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known "
                          "traffic, for scaling the FETCH_SIZE/WRITE_SIZE counters)")
@@ -79,6 +79,29 @@ def measured_traffic(workload):
         for ln in f:
             if ln.startswith("`search_frames_kernel`: bytes_per_launch="):
                 return int(ln.split("=", 1)[1])
+    return None
+
+
+BDRATE_SUMMARY = os.path.join(ROOT, "profiles", "pruner_bdrate_1080p.md")
+
+
+def measured_bd_rate(ncand, thr_part, thr_mv):
+    """The pruned search's BD-rate against the plain search under the P-frame
+    coding model (pcme/rdmodel.py, QP 22/27/32/37, held-out 1080p clip), from
+    the evaluation summarised in profiles/ for the same confidence rule;
+    None for other rules or without the summary."""
+    if not os.path.exists(BDRATE_SUMMARY):
+        return None
+    key = f"`thr_part {thr_part:g}, thr_mv {thr_mv:g}`: "
+    if ncand != 5:
+        return None
+    with open(BDRATE_SUMMARY) as f:
+        for ln in f:
+            if ln.startswith(key):
+                kv = dict(x.split("=") for x in ln[len(key):].split())
+                return {"cubic_pct": float(kv["bd_rate_cubic_pct"]), "pchip_pct": float(kv["bd_rate_pchip_pct"]),
+                        "model": "pcme/rdmodel.py P-frame coding model, QP 22/27/32/37",
+                        "source": "profiles/pruner_bdrate_1080p.md"}
     return None
 
 
@@ -348,7 +371,7 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
         "net_roofline": {"bound": "mfma", "kernel": "hints_kernel", "achieved": round(tflops, 3),
                          "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5), "macs_per_mb": NET_MACS_PER_MB},
-        "bd_rate": None,
+        "bd_rate": measured_bd_rate(args.ncand, args.thr, thr_mv),
     }
 
 

@@ -1,0 +1,116 @@
+"""BD-rate of the learned search pruner under the P-frame coding model of
+pcme.rdmodel, on a held-out 1080p clip at QP 22, 27, 32 and 37.
+
+    python -m pcme.eval_pruner [--frames 16] [--seed 0x5EED9200] [--out report.json]
+
+For each QP the same frames are searched three ways: the plain search, the
+pruned search with that QP's network (5 candidates; full search where
+either head's confidence is below 0.6) and the same pruned search with empty hints (the
+method without the network).  Every field is then coded by the model --
+motion-compensated prediction of its chosen partitions (pcme_pred_frame),
+luma residual through the H.264 4x4 transform and quantiser, run/level and
+MV-difference bits -- and the (rate, PSNR) points of each method give its
+BD-rate against the plain search.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+
+import numpy as np
+
+from . import DeviceClip, HINT_DTYPE, MB_DTYPE, ClipSpec, mb_dims, pred_frame, synth_clip, LUMA_PAD
+from . import pruner as P
+from . import rdmodel as R
+
+QPS = (22, 27, 32, 37)
+
+
+def clip_spec(seed: int, frames: int, w: int = 1920, h: int = 1080) -> ClipSpec:
+    """The 1080p workload's generator (12 rects of 64-400 px at up to 8 px/frame) at w x h."""
+    return ClipSpec(w, h, frames, nrect=12, rect_lo=64, rect_hi=400, vmax=8.0, pan_max=1.0, seed=seed)
+
+
+def source_mbs(dc: DeviceClip, t: int) -> np.ndarray:
+    """[mbw*mbh, 16, 16] luma of frame t (coded area of its padded plane)."""
+    p = dc.planes[0, t].cpu().numpy()
+    a = p[LUMA_PAD:LUMA_PAD + 16 * dc.mbh, LUMA_PAD:LUMA_PAD + 16 * dc.mbw]
+    return a.reshape(dc.mbh, 16, dc.mbw, 16).transpose(0, 2, 1, 3).reshape(-1, 16, 16)
+
+
+def rd_point(dc: DeviceClip, field: np.ndarray, qp: int) -> tuple[float, float]:
+    """(bits, PSNR) of coding every P frame of `field` under the model."""
+    torch = dc.torch
+    nmb = dc.mbw * dc.mbh
+    bits = sse = 0.0
+    py = torch.empty(nmb * 256, dtype=torch.uint8, device=dc.raw.device)
+    pc = torch.empty(nmb * 64, dtype=torch.uint8, device=dc.raw.device)
+    for t in range(1, dc.n):
+        rec = np.ascontiguousarray(field[t - 1].reshape(-1).view(MB_DTYPE))
+        mb = torch.as_tensor(rec.view(np.uint8)).to(dc.raw.device)
+        pred_frame(dc, t - 1, mb.data_ptr(), py.data_ptr(), pc.data_ptr(), pc.data_ptr())
+        pred = py.cpu().numpy().reshape(nmb, 16, 16)
+        b, s = R.code_frame(source_mbs(dc, t), pred, rec, qp)
+        bits += b
+        sse += s
+    npix = (dc.n - 1) * nmb * 256
+    return bits, R.psnr(sse, npix)
+
+
+def evaluate(frames: int, seed: int, ncand: int = 5, thr_part: float = 0.6, thr_mv: float = 0.6,
+             log=print, w: int = 1920, h: int = 1080) -> dict:
+    y, u, v = synth_clip(clip_spec(seed, frames + 1, w, h))
+    mbw, mbh = mb_dims(w, h)
+    dc = DeviceClip(y, mbw, mbh, u, v)
+    dc.prepare()
+    pts = {"plain": [], "pruned": [], "no_network": []}
+    judge = {}
+    for qp in QPS:
+        net = P.Net(seed=1, max_batch=64, weights=P.WEIGHTS, qp=qp)
+        plain, _ = P.search_clip(dc, 32, qp, prepared=True)
+        pruned, hints = P.search_clip(dc, 32, qp, net=net, ncand=ncand, thr_part=thr_part, thr_mv=thr_mv,
+                                      prepared=True)
+        none = np.zeros(hints.shape, HINT_DTYPE)
+        none["prune"] = 1
+        bare, _ = P.search_clip(dc, 32, qp, prepared=True, hints_in=none)
+        net.close()
+        for k, f in (("plain", plain), ("pruned", pruned), ("no_network", bare)):
+            pts[k].append(rd_point(dc, f, qp))
+        judge[qp] = {"pruned": P.judge(plain, pruned), "no_network": P.judge(plain, bare),
+                     "pruned_fraction": float(np.mean(hints["prune"] != 0))}
+        log(f"qp {qp}: " + ", ".join(f"{k} {pts[k][-1][0] / 1e3:.0f} kbit {pts[k][-1][1]:.3f} dB" for k in pts))
+    return {
+        "frames": frames, "seed": seed, "size": [w, h], "qps": list(QPS), "ncand": ncand, "thr_part": thr_part, "thr_mv": thr_mv,
+        "points": {k: [[round(r, 1), round(q, 4)] for r, q in v] for k, v in pts.items()},
+        "bd_rate_pruned_pct": round(R.bd_rate(pts["plain"], pts["pruned"]), 4),
+        "bd_rate_no_network_pct": round(R.bd_rate(pts["plain"], pts["no_network"]), 4),
+        "bd_rate_pruned_pchip_pct": round(R.bd_rate_pchip(pts["plain"], pts["pruned"]), 4),
+        "bd_rate_no_network_pchip_pct": round(R.bd_rate_pchip(pts["plain"], pts["no_network"]), 4),
+        "judge": {str(q): j for q, j in judge.items()},
+    }
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--frames", type=int, default=16, help="P frames per QP")
+    ap.add_argument("--seed", type=lambda s: int(s, 0), default=0x5EED9200)
+    ap.add_argument("--ncand", type=int, default=5)
+    ap.add_argument("--thr", type=float, default=0.6, help="partition-head confidence threshold")
+    ap.add_argument("--thr-mv", type=float, default=0.6, help="motion-head candidate-mass threshold")
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    import torch
+    if not torch.cuda.is_available():
+        print("the evaluation needs the GPU", file=sys.stderr)
+        return 2
+    rep = evaluate(a.frames, a.seed, a.ncand, a.thr, a.thr_mv, log=lambda s: print(s, file=sys.stderr, flush=True))
+    print(json.dumps(rep))
+    if a.out:
+        with open(a.out, "w") as f:
+            json.dump(rep, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

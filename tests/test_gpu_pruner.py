@@ -212,11 +212,28 @@ def test_network_gain_over_the_search_without_it(gpu):
     _, dc = _dclip(gpu, 640, 368, 4, 0x5EED9101, nrect=8, rect_lo=24, rect_hi=120, vmax=6.0, pan_max=1.0)
     net = P.Net(seed=1, max_batch=64, weights=P.WEIGHTS, qp=27)
     plain, _ = P.search_clip(dc, 32, 27, prepared=True)
-    cnn, hints = P.search_clip(dc, 32, 27, net=net, ncand=5, thr_part=0.6, thr_mv=0.0, prepared=True)
+    cnn, hints = P.search_clip(dc, 32, 27, net=net, ncand=5, thr_part=0.6, thr_mv=0.6, prepared=True)
     none = np.zeros(hints.shape, gpu.HINT_DTYPE)
     none["prune"] = 1
     bare, _ = P.search_clip(dc, 32, 27, prepared=True, hints_in=none)
     j_cnn, j_none = P.judge(plain, cnn), P.judge(plain, bare)
-    assert np.mean(hints["prune"]) > 0.8
+    assert np.mean(hints["prune"]) > 0.5
     assert j_cnn["jcost_increase"] < 0.01, j_cnn
     assert j_cnn["jcost_increase"] * 4 < j_none["jcost_increase"], (j_cnn, j_none)
+
+
+def test_bd_rate_model_runs_on_product_fields(gpu):
+    """pcme.eval_pruner end to end on a small held-out clip: four QPs, three
+    fields each, finite (rate, PSNR) points rising with rate, and the plain
+    field's BD-rate against itself zero."""
+    from pcme import eval_pruner as E, rdmodel as R
+    say("bd-rate model 320x240 x 2 P frames, 4 QPs")
+    rep = E.evaluate(2, 0x5EED9300, log=lambda s: None, w=320, h=240)
+    pts = rep["points"]
+    for k in ("plain", "pruned", "no_network"):
+        r = [p[0] for p in pts[k]]
+        q = [p[1] for p in pts[k]]
+        assert all(np.isfinite(r)) and all(np.isfinite(q))
+        assert r == sorted(r, reverse=True) and q == sorted(q, reverse=True), (k, pts[k])
+    assert abs(R.bd_rate([tuple(p) for p in pts["plain"]], [tuple(p) for p in pts["plain"]])) < 1e-9
+    assert np.isfinite(rep["bd_rate_pruned_pct"]) and np.isfinite(rep["bd_rate_no_network_pct"])
