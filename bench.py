@@ -4,15 +4,15 @@ Default workload (one step): a seeded synthetic 1920x1080 (coded 1920x1088)
 clip of 60 frames IPPP, QP 27, me_range 32, subpel_rounds 1, all four
 partition shapes; 12 rectangles of 64-400 px moving at up to 8 px/frame over
 a panned blurred-noise background.  The raw luma frames are resident in HBM
-when the timed region starts; a step imports/pads the frames, builds the
-half-pel planes of the reference frames and searches every macroblock of
-the P frames (59 x 8,160 = 481,440 MBs) -- the hot path of the reference's
+when the timed region starts; a step builds the four padded planes of every
+frame (import, edge extension and half-pel planes in one pass) and searches
+every macroblock of the P frames (59 x 8,160 = 481,440 MBs) -- the hot path of the reference's
 search (src/h264/enc/slice.rs:1590-1945, picture.rs:318-449).
 
 Other workloads (--workload): "batch" = 64 clips of uneven size (16 each
 of 352x288, 640x360, 1280x720, 1920x1080), 30 frames each, QP per clip from
 {22,27,32,37}, me_range 32, all in one persistent launch; "4k" = 3840x2160,
-24 frames, me_range 64, QP 27, 8 rects of 128-900 px at up to 16 px/frame,
+120 frames, me_range 64, QP 27, 8 rects of 128-900 px at up to 16 px/frame,
 0.2%/frame zoom.
 
 Prints one JSON line (fields at the end of main()).
@@ -56,6 +56,7 @@ def parse():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-pruned", action="store_true", help="skip the CNN-pruned leg")
     ap.add_argument("--ncand", type=int, default=5, help="pruner candidates per MB")
+    ap.add_argument("--weights", default=None, help="pruner weights (.npz; default: the committed ones)")
     ap.add_argument("--thr", type=float, default=0.6, help="pruner confidence threshold (partition head)")
     ap.add_argument("--thr-mv", type=float, default=None,
                     help="threshold on the motion head's candidate mass (default: --thr)")
@@ -111,7 +112,7 @@ def clip_specs(pcme, workload):
         return [(pcme.ClipSpec(1920, 1080, 60, nrect=12, rect_lo=64, rect_hi=400, vmax=8.0, pan_max=1.0,
                                seed=0x5EED0000), 27, 32)]
     if workload == "4k":
-        return [(pcme.ClipSpec(3840, 2160, 24, nrect=8, rect_lo=128, rect_hi=900, vmax=16.0, pan_max=1.0,
+        return [(pcme.ClipSpec(3840, 2160, 120, nrect=8, rect_lo=128, rect_hi=900, vmax=16.0, pan_max=1.0,
                                zoom=0.002, seed=0x5EED0000), 27, 64)]
     out = []
     sizes = [(352, 288, 8, 120, 6.0), (640, 360, 8, 180, 6.0), (1280, 720, 10, 300, 8.0), (1920, 1080, 12, 400, 8.0)]
@@ -230,7 +231,7 @@ def main():
                    "partitions 16x16/16x8/8x16/8x8, 12 rects 64-400 px, v in [-8,8] px/frame",
           "batch": "plain GPU ME-only search, 64 clips (16 each of 352x288, 640x360, 1280x720, 1920x1080), "
                    "30 frames each IPPP, QP per clip from {22,27,32,37}, me_range 32, one persistent launch",
-          "4k": "plain GPU ME-only search, 3840x2160 24 frames IPPP, 1 ref, QP 27, me_range 64 + qpel, "
+          "4k": "plain GPU ME-only search, 3840x2160 120 frames IPPP, 1 ref, QP 27, me_range 64 + qpel, "
                 "8 rects 128-900 px, v in [-16,16] px/frame, 0.2%/frame zoom"}[args.workload]
     line = {
         "metric": METRIC, "value": round(value, 4), "unit": "Mblocks/s", "n_gpus": 1,
@@ -287,14 +288,15 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
     plain search of the step above judges the result."""
     import numpy as np
     import pcme.pruner as P
-    if not os.path.exists(P.WEIGHTS):
+    wpath = args.weights or P.WEIGHTS
+    if not os.path.exists(wpath):
         return {"skipped": "no trained weights (pcme/pruner_weights.npz)"}
     sp = stream.cuda_stream
     thr_mv = args.thr if args.thr_mv is None else args.thr_mv
     nets, jobs, pouts, hints, groups = {}, [], [], [], {}
     for dc, (spec, qp, rng) in zip(clips, specs):
         if qp not in nets:
-            nets[qp] = P.Net(seed=1, max_batch=32, weights=P.WEIGHTS, qp=qp)
+            nets[qp] = P.Net(seed=1, max_batch=32, weights=wpath, qp=qp)
         nb, nmb = dc.n - 1, dc.mbw * dc.mbh
         o = torch.empty((nb, nmb * pcme.MB_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
         h = torch.zeros((nb, nmb * pcme.HINT_DTYPE.itemsize), dtype=torch.uint8, device="cuda")

@@ -59,7 +59,7 @@ def rd_point(dc: DeviceClip, field: np.ndarray, qp: int) -> tuple[float, float]:
 
 
 def evaluate(frames: int, seed: int, ncand: int = 5, thr_part: float = 0.6, thr_mv: float = 0.6,
-             log=print, w: int = 1920, h: int = 1080) -> dict:
+             log=print, w: int = 1920, h: int = 1080, weights: str | None = None) -> dict:
     y, u, v = synth_clip(clip_spec(seed, frames + 1, w, h))
     mbw, mbh = mb_dims(w, h)
     dc = DeviceClip(y, mbw, mbh, u, v)
@@ -67,7 +67,7 @@ def evaluate(frames: int, seed: int, ncand: int = 5, thr_part: float = 0.6, thr_
     pts = {"plain": [], "pruned": [], "no_network": []}
     judge = {}
     for qp in QPS:
-        net = P.Net(seed=1, max_batch=64, weights=P.WEIGHTS, qp=qp)
+        net = P.Net(seed=1, max_batch=64, weights=weights or P.WEIGHTS, qp=qp)
         plain, _ = P.search_clip(dc, 32, qp, prepared=True)
         pruned, hints = P.search_clip(dc, 32, qp, net=net, ncand=ncand, thr_part=thr_part, thr_mv=thr_mv,
                                       prepared=True)
@@ -98,13 +98,15 @@ def main(argv=None) -> int:
     ap.add_argument("--ncand", type=int, default=5)
     ap.add_argument("--thr", type=float, default=0.6, help="partition-head confidence threshold")
     ap.add_argument("--thr-mv", type=float, default=0.6, help="motion-head candidate-mass threshold")
+    ap.add_argument("--weights", default=None, help="pruner weights (.npz; default: the committed ones)")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     import torch
     if not torch.cuda.is_available():
         print("the evaluation needs the GPU", file=sys.stderr)
         return 2
-    rep = evaluate(a.frames, a.seed, a.ncand, a.thr, a.thr_mv, log=lambda s: print(s, file=sys.stderr, flush=True))
+    rep = evaluate(a.frames, a.seed, a.ncand, a.thr, a.thr_mv, log=lambda s: print(s, file=sys.stderr, flush=True),
+                   weights=a.weights)
     print(json.dumps(rep))
     if a.out:
         with open(a.out, "w") as f:
