@@ -516,10 +516,13 @@ constexpr int L_B = L_W4 + CH * KCP * 2;        // f32 b1..b4 [4][32]
 constexpr int L_ZERO = L_B + 4 * CH * 4;         // 1 KB of zeros: out-of-bounds taps (one slot per lane)
 constexpr int L_WAVE = L_ZERO + 1024;
 constexpr int R0_B = IN * IN * 2 * 2;           // 2304 (>= 36*32*2 and 9*32*4)
-constexpr int R1_B = 144 * CH * 2;              // 9216
+constexpr int R1_B = 12 * (12 * 64 + 64);       // 9984: conv1's map with padded rows (map_off<MAP_A>)
 constexpr int WAVE_B = R0_B + R1_B;
 constexpr int CONV_LDS = L_WAVE + CW * WAVE_B;
 static_assert(CONV_LDS <= 160 * 1024, "LDS budget");
+// Bank slots (32 bytes of a 256-byte line) are computed from offsets within
+// 256-aligned regions.
+static_assert(L_ZERO % 256 == 0 && L_WAVE % 256 == 0 && R0_B % 256 == 0 && WAVE_B % 256 == 0, "256-aligned regions");
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -545,39 +548,86 @@ __device__ __forceinline__ uint32_t ds_pair(uint32_t a, uint32_t b) {
   return ((s >> 2) & 0x3FFFu) | (((s >> 18) & 0x3FFFu) << 16);
 }
 
-// Byte offset of chunk q of pixel k (k = pixel index of the stored order).
-__device__ __forceinline__ int map_at(int k, int q) { return k * 64 + 16 * chunk_at<true>(k, q); }
+// Placement of the fused kernel's activation maps in LDS.  A map pixel is
+// 32 bf16 channels in four 16-byte chunks; a B-fragment read takes one chunk
+// of 16 pixels per lane group, 8 lanes per LDS cycle, and 8 lanes conflict
+// unless they hit 8 distinct 32-byte slots of a 256-byte line.  So each map
+// has the placement its reader needs: conv1's 12x12 output (read at stride
+// 2) and conv3's 6x6 output (stride 2) store each row's even columns, then
+// its odd ones, rows padded by 64 bytes, chunks XOR-swapped on bit 3 of the
+// position; conv2's 6x6 output (read at stride 1) is raster with the chunk
+// swap on bit 2.  Out-of-bounds taps read zeros in the slots their lane
+// group leaves free (tab_group).
+template <int HIN, bool SPLIT, int SWB, int RP>
+__device__ __forceinline__ int mpos(int iy, int ix, int q) {
+  const int x = SPLIT ? ((ix & 1) ? (HIN + 1) / 2 + (ix >> 1) : (ix >> 1)) : ix;
+  const int m = iy * HIN + x;
+  return iy * (HIN * 64 + RP) + x * 64 + 16 * (q ^ (((m >> SWB) & 1) << 1));
+}
+enum { MAP_A = 0, MAP_B = 1, MAP_C = 2 };  // conv1 out (R1), conv2 out (R0), conv3 out (R1)
+template <int MAP>
+__device__ __forceinline__ int map_off(int k, int q) {
+  if constexpr (MAP == MAP_A) return mpos<12, true, 3, 64>(k / 12, k % 12, q);
+  else if constexpr (MAP == MAP_B) return mpos<6, false, 2, 0>(k / 6, k % 6, q);
+  else return mpos<6, true, 3, 64>(k / 6, k % 6, q);
+}
 
 // Per-lane read table of the fused convolutions: the LDS byte offset / 4 of
 // every B-fragment read of one MB (the same for every MB a wave runs), two
-// u16 entries per dword.  Out-of-bounds taps and unused pixels read zeros at
-// L_ZERO, each lane its own slot (reads of one address by many lanes
-// conflict).  Entries: conv1 (tile t, tap group q) at 4t+q, conv2 (t, chunk
-// c) at T_C2 + 9t + c, conv3 at T_C3 + 9t + c, conv4 (c) at T_C4 + c.
+// u16 entries per dword.  Entries: conv1 (tile t, tap group q) at 4t+q,
+// conv2 (t, chunk c) at T_C2 + 9t + c, conv3 at T_C3 + 9t + c, conv4 (c) at
+// T_C4 + c.
 constexpr int T_C2 = 36, T_C3 = 63, T_C4 = 90, T_N = 99, T_W = (T_N + 1) / 2;
 
 __device__ __forceinline__ uint32_t tab(const uint32_t *tb, int i) {
   return ((tb[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu) << 2;
 }
 
-// Entry of a B-fragment read: input pixel (iy, ix) of a HIN x HIN map at
-// `base` (raster order).
-__device__ __forceinline__ uint32_t tab_entry(int base, int hin, int iy, int ix, bool ok, int l) {
-  const int g = l >> 4;
-  ok = ok & (iy >= 0) & (iy < hin) & (ix >= 0) & (ix < hin);
-  const int k = iy * hin + ix;
-  return (uint32_t)(ok ? base + map_at(k, g) : L_ZERO + map_at(l & 15, g)) >> 2;
+// Entry of lane l for the read of tap (ty, tx) by output pixel tile t of a
+// HOUT x HOUT output (NPIX pixels) at stride S from the map MAP (HIN x HIN)
+// at `base`.  The lane's 8-lane group is evaluated whole: an out-of-bounds
+// lane takes the r-th 32-byte slot of the zero line that no in-bounds lane of
+// the group uses, r = its rank among the group's out-of-bounds lanes (there
+// are at least as many free slots as out-of-bounds lanes).
+template <int MAP, int HIN, int HOUT, int S, int NPIX>
+__device__ __forceinline__ uint32_t tab_group(int base, int t, int ty, int tx, int l) {
+  const int g = l >> 4, l0 = l & ~7;
+  uint32_t used = 0u;
+  int rank = 0, mine = 0;
+  bool oob = false;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int p = t * 16 + ((l0 + j) & 15), oy = p / HOUT, ox = p - HOUT * (p / HOUT);
+    const int iy = S * oy + ty - 1, ix = S * ox + tx - 1;
+    const bool ok = (p < NPIX) & (iy >= 0) & (iy < HIN) & (ix >= 0) & (ix < HIN);
+    const int a = ok ? base + map_off<MAP>(iy * HIN + ix, g) : 0;
+    if (ok) used |= 1u << ((a & 255) >> 5);
+    if (l0 + j == l) {
+      mine = a;
+      oob = !ok;
+    } else if (!ok && l0 + j < l) {
+      rank++;
+    }
+  }
+  if (!oob) return (uint32_t)mine >> 2;
+  int slot = 0;
+#pragma unroll
+  for (int s = 0, n = 0; s < 8; s++) {
+    const bool f = ((used >> s) & 1u) == 0u;
+    if (f && n == rank) slot = s;
+    n += f ? 1 : 0;
+  }
+  return (uint32_t)(L_ZERO + 32 * slot) >> 2;
 }
 
 __device__ __forceinline__ void build_table(int l, int r0, int r1, uint32_t *tb) {
-  const int g = l >> 4;
   uint32_t e[T_W * 2];
 #pragma unroll
   for (int t = 0; t < 9; t++) {
     const int p = t * 16 + (l & 15), oy = p / 12, ox = p - 12 * (p / 12);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const int tap = 4 * g + q, ty = tap / 3, tx = tap - 3 * (tap / 3);
+      const int tap = 4 * (l >> 4) + q, ty = tap / 3, tx = tap - 3 * (tap / 3);
       const int iy = 2 * oy + ty - 1, ix = 2 * ox + tx - 1;
       const bool ok = (tap < 9) & (iy >= 0) & (iy < IN) & (ix >= 0) & (ix < IN);
       e[4 * t + q] = (uint32_t)(ok ? r0 + 4 * (iy * IN + ix) : L_ZERO + 4 * l) >> 2;
@@ -585,21 +635,17 @@ __device__ __forceinline__ void build_table(int l, int r0, int r1, uint32_t *tb)
   }
 #pragma unroll
   for (int t = 0; t < 3; t++) {
-    const int p = t * 16 + (l & 15), oy = p / 6, ox = p - 6 * (p / 6);
 #pragma unroll
     for (int c = 0; c < 9; c++) {
       const int ty = c / 3, tx = c - 3 * (c / 3);
-      e[T_C2 + 9 * t + c] = tab_entry(r1, 12, 2 * oy + ty - 1, 2 * ox + tx - 1, p < 36, l);
-      e[T_C3 + 9 * t + c] = tab_entry(r0, 6, oy + ty - 1, ox + tx - 1, p < 36, l);
+      e[T_C2 + 9 * t + c] = tab_group<MAP_A, 12, 6, 2, 36>(r1, t, ty, tx, l);
+      e[T_C3 + 9 * t + c] = tab_group<MAP_B, 6, 6, 1, 36>(r0, t, ty, tx, l);
     }
   }
-  {
-    const int p = l & 15, oy = p / 3, ox = p - 3 * (p / 3);
 #pragma unroll
-    for (int c = 0; c < 9; c++) {
-      const int ty = c / 3, tx = c - 3 * (c / 3);
-      e[T_C4 + c] = tab_entry(r1, 6, 2 * oy + ty - 1, 2 * ox + tx - 1, p < 9, l);
-    }
+  for (int c = 0; c < 9; c++) {
+    const int ty = c / 3, tx = c - 3 * (c / 3);
+    e[T_C4 + c] = tab_group<MAP_C, 6, 3, 2, 9>(r1, 0, ty, tx, l);
   }
   e[T_N] = 0u;
 #pragma unroll
@@ -608,17 +654,18 @@ __device__ __forceinline__ void build_table(int l, int r0, int r1, uint32_t *tb)
 
 // ReLU + bf16 of the lane's 8 outputs (channels 4g..4g+3 and 16+4g..) of a
 // pixel stored at index k of a fused-kernel map (swizzled chunks).
+template <int MAP>
 __device__ __forceinline__ void store_map(uint16_t *y, int k, int g, f4 a0, f4 a1) {
   a0 = relu4(a0);
   a1 = relu4(a1);
   uint8_t *b = (uint8_t *)y;
-  *(u2 *)(b + map_at(k, g >> 1) + 8 * (g & 1)) = u2{pack2(a0[0], a0[1]), pack2(a0[2], a0[3])};
-  *(u2 *)(b + map_at(k, 2 + (g >> 1)) + 8 * (g & 1)) = u2{pack2(a1[0], a1[1]), pack2(a1[2], a1[3])};
+  *(u2 *)(b + map_off<MAP>(k, g >> 1) + 8 * (g & 1)) = u2{pack2(a0[0], a0[1]), pack2(a0[2], a0[3])};
+  *(u2 *)(b + map_off<MAP>(k, 2 + (g >> 1)) + 8 * (g & 1)) = u2{pack2(a1[0], a1[1]), pack2(a1[2], a1[3])};
 }
 
 // conv2 / conv3 of one MB: 36 output pixels in three tiles, reads through
 // the table from entry T0, writes Y ([pixel][32] bf16, swizzled).
-template <int T0>
+template <int T0, int MAP>
 __device__ __forceinline__ void conv_mid(const uint8_t *lds, const uint32_t *tb, uint16_t *Y, const uint16_t *W,
                                          const float *b, int l) {
   const int g = l >> 4;
@@ -642,7 +689,7 @@ __device__ __forceinline__ void conv_mid(const uint8_t *lds, const uint32_t *tb,
 #pragma unroll
   for (int t = 0; t < 3; t++) {
     const int p = t * 16 + (l & 15);
-    if (p < 36) store_map(Y, p, g, acc[t][0], acc[t][1]);
+    if (p < 36) store_map<MAP>(Y, p, g, acc[t][0], acc[t][1]);
   }
   wave_sync();
 }
@@ -764,12 +811,12 @@ __global__ void __launch_bounds__(64 * CW) conv_kernel(const NetJob *__restrict_
         for (int q = 0; q < 4; q++) B[q] = *(const uint32_t *)(lds + tab(tb, 4 * t + q));
         const f4 a0 = mfma(A0, B, bias4(BB, 0, g));
         const f4 a1 = mfma(A1, B, bias4(BB, 1, g));
-        store_map(R1, p, g, a0, a1);
+        store_map<MAP_A>(R1, p, g, a0, a1);
       }
     }
     wave_sync();
-    conv_mid<T_C2>(lds, tb, R0, W2, BB + CH, l);
-    conv_mid<T_C3>(lds, tb, R1, W3, BB + 2 * CH, l);
+    conv_mid<T_C2, MAP_B>(lds, tb, R0, W2, BB + CH, l);
+    conv_mid<T_C3, MAP_C>(lds, tb, R1, W3, BB + 2 * CH, l);
     // conv4: R1 -> fp32 [9][32] in R0, then the pool
     {
       const int p = l & 15;
