@@ -243,6 +243,70 @@ int pcme_pred_frame(const uint8_t *ref, const uint8_t *ref_h,
                     uint8_t *pred_y, uint8_t *pred_u, uint8_t *pred_v,
                     void *stream);
 
+
+/* ---- lowres lookahead: scenecut complexity and macroblock-tree costs ----
+ * A lowres frame is the w x h = (16*mbw/2) x (16*mbh/2) 2x2 average of a
+ * coded luma plane, rows w bytes apart, 4-byte aligned; w and h are
+ * multiples of 8 (one 8x8 block per macroblock). */
+
+/* LowRes::new (lookahead.rs:10-33): lowres frames of n padded luma planes.
+ * Plane i (device) is at planes + i*plane_pitch with row stride `stride` and
+ * `pad` pixels of padding (picture (0,0) at pad*stride + pad); w x h is its
+ * coded area (w, h multiples of 16).  Frame i is written at out + i*out_pitch
+ * (device), (w/2) x (h/2), rows w/2 apart.  planes, plane_pitch, stride and
+ * pad 4-byte aligned; out and out_pitch too. */
+int pcme_lowres(const uint8_t *planes, int64_t plane_pitch, int32_t stride,
+                int32_t pad, int32_t w, int32_t h, uint8_t *out,
+                int64_t out_pitch, int32_t n, void *stream);
+
+/* Per 8x8 block of complexity(): intra cost (mbtree.rs:21-49), the
+ * contribution to Complexity.inter (min(SATD at the diamond's best + 8,
+ * intra), or intra without a previous frame) and the diamond's integer MV
+ * (lowres pixels; zero without a previous frame).  12 bytes. */
+typedef struct pcme_cplx_block {
+  uint32_t intra, inter;
+  pcme_mv mv;
+} pcme_cplx_block;
+
+/* One complexity() call (lookahead.rs:52-108): cur and prev (device, NULL =
+ * no previous frame) are w x h lowres frames.  sums (device, 8-byte
+ * aligned) receives Complexity {intra, inter}; blocks (device, may be NULL)
+ * (w/8)*(h/8) records in raster order.  40 bytes. */
+typedef struct pcme_cplx_job {
+  const uint8_t *cur, *prev;
+  pcme_cplx_block *blocks;
+  uint64_t *sums;
+  int32_t w, h;
+} pcme_cplx_job;
+
+/* lookahead::complexity for n frames at once (the scenecut input,
+ * mod.rs:656-657).  Bit-identical to the reference. */
+int pcme_complexity(const pcme_cplx_job *jobs, int32_t n, void *stream);
+
+/* mbtree.rs:10-15 Block: intra cost, the best of intra / list 0 / list 1 /
+ * bi (`list` 0..3) and its cost, and the MVs as analyse records them (a
+ * list-0 win leaves mv[1] = 0).  20 bytes. */
+typedef struct pcme_tree_block {
+  uint32_t intra, inter;
+  pcme_mv mv[2];
+  uint8_t list;
+  uint8_t reserved[3];
+} pcme_tree_block;
+
+/* One analyse() call (mbtree.rs:113-159): cur, ref0, ref1 (device; a NULL
+ * reference skips its list) are w x h lowres frames; blocks (device)
+ * receives (w/8)*(h/8) records in raster order.  40 bytes. */
+typedef struct pcme_tree_job {
+  const uint8_t *cur, *ref0, *ref1;
+  pcme_tree_block *blocks;
+  int32_t w, h;
+} pcme_tree_job;
+
+/* mbtree::analyse for n frames at once (mod.rs:760-790, the lookahead's
+ * per-frame costs of the macroblock tree).  Bit-identical to the
+ * reference. */
+int pcme_mbtree_analyse(const pcme_tree_job *jobs, int32_t n, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

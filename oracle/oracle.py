@@ -57,6 +57,10 @@ OMB_DTYPE = np.dtype([
     ("mv16", "<i2", (2,)), ("part", "u1"), ("reserved", "u1", (3,)),
 ])
 
+# otree_block has the layout of pcme_tree_block (mbtree.rs Block)
+OTREE_DTYPE = np.dtype([("intra", "<u4"), ("inter", "<u4"), ("mv", "<i2", (2, 2)), ("list", "u1"),
+                        ("reserved", "u1", (3,))])
+
 # ohint has the layout of pcme_hint
 OHINT_DTYPE = np.dtype([
     ("cand", "<i2", (8, 2)), ("conf_part", "<u2"), ("conf_mv", "<u2"),
@@ -137,6 +141,18 @@ def load() -> ctypes.CDLL:
                                    c.c_int, c.POINTER(oref), P, c.POINTER(ocounters)]
     lib.o_search_jobs_mt.restype = c.c_double
     lib.o_search_jobs_mt.argtypes = [c.POINTER(ojob), c.c_int, c.c_int, c.POINTER(ocounters)]
+    lib.o_lowres.restype = None
+    lib.o_lowres.argtypes = [P, c.c_int, c.c_int, c.c_int, c.c_int, P]
+    lib.o_lowres_intra.restype = c.c_uint32
+    lib.o_lowres_intra.argtypes = [P, c.c_int, c.c_int, c.c_int]
+    lib.o_complexity.restype = None
+    lib.o_complexity.argtypes = [P, P, c.c_int, c.c_int, c.POINTER(c.c_uint64), c.POINTER(c.c_uint64), P, P, P]
+    lib.o_tree_search.restype = omv
+    lib.o_tree_search.argtypes = [P, P, c.c_int, c.c_int, c.c_int, c.c_int, omv]
+    lib.o_tree_analyse.restype = None
+    lib.o_tree_analyse.argtypes = [P, P, P, c.c_int, c.c_int, P]
+    lib.o_lookahead_mt.restype = c.c_double
+    lib.o_lookahead_mt.argtypes = [P, c.c_int, c.c_int, c.c_int, c.c_int, P, P]
     _lib = lib
     return lib
 
@@ -239,3 +255,69 @@ def search_jobs_mt(srcs, refs, mbw, mbh, me_range, subpel_rounds, partitions, qp
     ctr = ocounters()
     secs = lib.o_search_jobs_mt(jobs, n, threads, ctypes.byref(ctr))
     return [o.reshape(mbh, mbw) for o in outs], secs, ctr
+
+
+# ------------------------------------------------ lookahead (lookahead.rs, mbtree.rs)
+
+def lowres(plane_padded: np.ndarray, pw: int, ph: int, pad: int = LUMA_PAD) -> np.ndarray:
+    """LowRes::new of a padded luma plane whose coded area is pw x ph."""
+    p = np.ascontiguousarray(plane_padded, np.uint8)
+    out = np.empty((ph // 2, pw // 2), np.uint8)
+    load().o_lowres(_p(p), p.shape[1], pad, pw, ph, _p(out))
+    return out
+
+
+def lowres_intra(cur: np.ndarray, x0: int, y0: int) -> int:
+    """mbtree::intra_cost of the 8x8 block at (x0, y0) of a lowres frame."""
+    c = np.ascontiguousarray(cur, np.uint8)
+    return int(load().o_lowres_intra(_p(c), c.shape[1], x0, y0))
+
+
+def complexity(cur: np.ndarray, prev: np.ndarray | None):
+    """lookahead::complexity -> (intra, inter, per-block intra, per-block
+    inter, per-block MV [bh, bw, 2])."""
+    c = np.ascontiguousarray(cur, np.uint8)
+    h, w = c.shape
+    nb = (w // 8) * (h // 8)
+    bi = np.zeros(nb, np.uint32)
+    bp = np.zeros(nb, np.uint32)
+    mv = np.zeros((nb, 2), np.int16)
+    a, b = ctypes.c_uint64(), ctypes.c_uint64()
+    pp = None
+    if prev is not None:
+        pp = np.ascontiguousarray(prev, np.uint8)
+        assert pp.shape == c.shape
+    load().o_complexity(_p(c), _p(pp) if pp is not None else None, w, h, ctypes.byref(a), ctypes.byref(b),
+                        _p(bi), _p(bp), _p(mv))
+    return a.value, b.value, bi, bp, mv.reshape(h // 8, w // 8, 2)
+
+
+def tree_search(cur: np.ndarray, ref: np.ndarray, x0: int, y0: int, seed=(0, 0)) -> tuple[int, int]:
+    c = np.ascontiguousarray(cur, np.uint8)
+    r = np.ascontiguousarray(ref, np.uint8)
+    m = load().o_tree_search(_p(c), _p(r), c.shape[1], c.shape[0], x0, y0, omv(*seed))
+    return m.x, m.y
+
+
+def tree_analyse(cur: np.ndarray, l0: np.ndarray | None, l1: np.ndarray | None) -> np.ndarray:
+    """mbtree::analyse -> [bh*bw] OTREE_DTYPE records."""
+    c = np.ascontiguousarray(cur, np.uint8)
+    h, w = c.shape
+    out = np.zeros((w // 8) * (h // 8), OTREE_DTYPE)
+    r0 = np.ascontiguousarray(l0, np.uint8) if l0 is not None else None
+    r1 = np.ascontiguousarray(l1, np.uint8) if l1 is not None else None
+    load().o_tree_analyse(_p(c), _p(r0) if r0 is not None else None, _p(r1) if r1 is not None else None, w, h,
+                          _p(out))
+    return out
+
+
+def lookahead_mt(lr: np.ndarray, threads: int):
+    """complexity and analyse (list 0 = previous frame) of frames 1..n-1 of
+    [n, h, w] lowres frames on `threads` threads -> (seconds, sums [n, 2],
+    blocks [n, bh*bw])."""
+    a = np.ascontiguousarray(lr, np.uint8)
+    n, h, w = a.shape
+    sums = np.zeros((n, 2), np.uint64)
+    blocks = np.zeros((n, (w // 8) * (h // 8)), OTREE_DTYPE)
+    sec = load().o_lookahead_mt(_p(a), w, h, n, threads, _p(sums), _p(blocks))
+    return sec, sums, blocks

@@ -20,6 +20,11 @@
  *   src/h264/enc/frame.rs:13,58-97,152-156  VMAX, extend, the ref view row
  *                                     limit; picture.rs:211-268 (lim)
  *   src/h264/enc/mod.rs:842-873       import (alignment replication)
+ *   src/h264/enc/lookahead.rs:10-33   LowRes::new (o_lowres)
+ *   src/h264/enc/lookahead.rs:52-108  complexity (o_complexity)
+ *   src/h264/enc/mbtree.rs:21-49      intra_cost (o_lowres_intra)
+ *   src/h264/enc/mbtree.rs:51-159     search, fetch, analyse (o_tree_search,
+ *                                     o_tree_analyse)
  *
  * How each part is pinned (tests/):
  *   o_sad/o_sad16            closed form (numpy |a-b| sum), random + extremes
@@ -40,6 +45,18 @@
  *                            cost == SATD(mc(best)) + lambda*bits, mv_ok(best),
  *                            brute-force recovery of a pure integer shift
  *   o_import_plane/o_extend  numpy edge replication
+ *   o_lowres                 constant and 2x2-constant frames, the
+ *                            rounding of (s + 2) >> 2 on planted blocks
+ *   o_lowres_intra           predictions that a block matches exactly
+ *                            (flat 128, left column, top row: cost 16),
+ *                            a constant residual over a mode's prediction
+ *                            (32|d| + 16), and the minimum of textbook
+ *                            Hadamard SATDs over the three predictions
+ *   o_complexity/            identical frames (every inter cost 8), no
+ *   o_tree_analyse           previous frame (inter == intra), pure shifts
+ *                            recovered exactly, an exact bi-average win
+ *                            (cost 12), and a second, independently written
+ *                            Python restatement on small frames
  */
 #include "pcme_oracle.h"
 
@@ -809,5 +826,238 @@ double o_search_jobs_mt(const ojob *jobs, int njobs, int threads,
   free(th);
   pthread_mutex_destroy(&pl.mu);
   if (ctr) *ctr = pl.total;
+  return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
+
+/* ------------------------------------------- lookahead complexity (C6) */
+
+/* LowRes::new (lookahead.rs:10-33): the coded luma plane (pw x ph inside a
+ * plane of the given stride and pad) 2x2-averaged with rounding. */
+void o_lowres(const uint8_t *plane, int stride, int pad, int pw, int ph, uint8_t *out) {
+  const int w = pw / 2, h = ph / 2;
+  for (int y = 0; y < h; y++) {
+    const uint8_t *r0 = plane + (size_t)(2 * y + pad) * stride + pad;
+    const uint8_t *r1 = r0 + stride;
+    for (int x = 0; x < w; x++) {
+      const uint32_t s = (uint32_t)r0[2 * x] + r0[2 * x + 1] + r1[2 * x] + r1[2 * x + 1];
+      out[(size_t)y * w + x] = (uint8_t)((s + 2) >> 2);
+    }
+  }
+}
+
+/* mbtree.rs:21-49: least SATD of the 8x8 block at (x0, y0) against the
+ * average of its top row and left column (128 at the picture's top or left
+ * edge), its left column and its top row (the last two only where they
+ * exist), plus 16. */
+uint32_t o_lowres_intra(const uint8_t *cur, int w, int x0, int y0) {
+  uint8_t blk[64];
+  uint32_t best = 0xFFFFFFFFu;
+  const uint8_t *base = cur + (size_t)y0 * w + x0;
+  for (int mode = 0; mode < 3; mode++) {
+    if ((mode == 1 && x0 == 0) || (mode == 2 && y0 == 0)) continue;
+    for (int y = 0; y < 8; y++)
+      for (int x = 0; x < 8; x++) {
+        uint8_t v;
+        if (mode == 0)
+          v = (x0 > 0 && y0 > 0)
+                  ? (uint8_t)(((uint32_t)cur[(size_t)(y0 - 1) * w + x0 + x] + cur[(size_t)(y0 + y) * w + x0 - 1]) / 2)
+                  : 128;
+        else if (mode == 1)
+          v = cur[(size_t)(y0 + y) * w + x0 - 1];
+        else
+          v = cur[(size_t)(y0 - 1) * w + x0 + x];
+        blk[y * 8 + x] = v;
+      }
+    const uint32_t s = o_satd(base, w, blk, 8, 8, 8);
+    if (s < best) best = s;
+  }
+  return best + 16;
+}
+
+/* lookahead.rs:52-108 (complexity): for every 8x8 block of the w x h lowres
+ * frame cur, the intra cost; with a previous lowres frame, a diamond SAD
+ * search from (0, 0) -- up to 8 rounds of (-1,0),(1,0),(0,-1),(0,1),(-2,0),
+ * (2,0),(0,-2),(0,2) around the round's centre, strict-< in that order,
+ * candidates outside the frame never taken, stop when the best stays -- then
+ * SATD at the best + 8, capped by the intra cost.  Sums into *intra and
+ * *inter; per-block intra, inter and MV (x, y) when the arrays are given. */
+void o_complexity(const uint8_t *cur, const uint8_t *prev, int w, int h, uint64_t *intra, uint64_t *inter,
+                  uint32_t *blk_intra, uint32_t *blk_inter, int16_t *blk_mv) {
+  static const int DX[8] = {-1, 1, 0, 0, -2, 2, 0, 0};
+  static const int DY[8] = {0, 0, -1, 1, 0, 0, -2, 2};
+  const int bw = w / 8, bh = h / 8;
+  uint64_t ci = 0, cp = 0;
+  for (int by = 0; by < bh; by++)
+    for (int bx = 0; bx < bw; bx++) {
+      const int x0 = 8 * bx, y0 = 8 * by, b = by * bw + bx;
+      const uint8_t *base = cur + (size_t)y0 * w + x0;
+      const uint32_t ic = o_lowres_intra(cur, w, x0, y0);
+      ci += ic;
+      uint32_t pc = ic;
+      int mx = 0, my = 0;
+      if (prev) {
+        uint32_t bc = o_sad(base, w, prev + (size_t)y0 * w + x0, w, 8, 8);
+        for (int it = 0; it < 8; it++) {
+          const int cx = mx, cy = my;
+          for (int k = 0; k < 8; k++) {
+            const int x = x0 + cx + DX[k], y = y0 + cy + DY[k];
+            if (x < 0 || y < 0 || x + 8 > w || y + 8 > h) continue;
+            const uint32_t c = o_sad(base, w, prev + (size_t)y * w + x, w, 8, 8);
+            if (c < bc) {
+              bc = c;
+              mx = cx + DX[k];
+              my = cy + DY[k];
+            }
+          }
+          if (mx == cx && my == cy) break;
+        }
+        const uint32_t sc = o_satd(base, w, prev + (size_t)(y0 + my) * w + x0 + mx, w, 8, 8) + 8;
+        pc = sc < ic ? sc : ic;
+      }
+      cp += pc;
+      if (blk_intra) blk_intra[b] = ic;
+      if (blk_inter) blk_inter[b] = pc;
+      if (blk_mv) {
+        blk_mv[2 * b] = (int16_t)mx;
+        blk_mv[2 * b + 1] = (int16_t)my;
+      }
+    }
+  *intra = ci;
+  *inter = cp;
+}
+
+/* mbtree.rs:51-102 (search): SAD + |mx| + |my| from (0, 0), then the seed
+ * when strictly cheaper, then up to 16 rounds of (-1,0),(1,0),(0,-1),(0,1),
+ * (-2,-2),(2,-2),(-2,2),(2,2) around the round's centre, strict-< in that
+ * order, MVs whose block leaves the frame never taken. */
+static uint32_t o_tree_cost(const uint8_t *cur, const uint8_t *r, int w, int h, int x0, int y0, int mx, int my) {
+  const int x = x0 + mx, y = y0 + my;
+  if (x < 0 || y < 0 || x > w - 8 || y > h - 8) return 0xFFFFFFFFu;
+  const uint32_t mvc = (uint32_t)(abs(mx) + abs(my));
+  return o_sad(cur + (size_t)y0 * w + x0, w, r + (size_t)y * w + x, w, 8, 8) + mvc;
+}
+
+omv o_tree_search(const uint8_t *cur, const uint8_t *r, int w, int h, int x0, int y0, omv seed) {
+  static const int DX[8] = {-1, 1, 0, 0, -2, 2, -2, 2};
+  static const int DY[8] = {0, 0, -1, 1, -2, -2, 2, 2};
+  int bx = 0, by = 0;
+  uint32_t bc = o_tree_cost(cur, r, w, h, x0, y0, 0, 0);
+  const uint32_t sc = o_tree_cost(cur, r, w, h, x0, y0, seed.x, seed.y);
+  if (sc < bc) {
+    bc = sc;
+    bx = seed.x;
+    by = seed.y;
+  }
+  for (int it = 0; it < 16; it++) {
+    const int cx = bx, cy = by;
+    for (int k = 0; k < 8; k++) {
+      const uint32_t c = o_tree_cost(cur, r, w, h, x0, y0, cx + DX[k], cy + DY[k]);
+      if (c < bc) {
+        bc = c;
+        bx = cx + DX[k];
+        by = cy + DY[k];
+      }
+    }
+    if (bx == cx && by == cy) break;
+  }
+  omv m = {(int16_t)bx, (int16_t)by};
+  return m;
+}
+
+/* mbtree.rs:104-111 (fetch) */
+static void o_tree_fetch(const uint8_t *r, int w, int x0, int y0, omv m, uint8_t out[64]) {
+  for (int row = 0; row < 8; row++) memcpy(out + 8 * row, r + (size_t)(y0 + m.y + row) * w + x0 + m.x, 8);
+}
+
+/* mbtree.rs:113-159 (analyse): per 8x8 block of the w x h lowres frame cur,
+ * the intra cost and the best of list 0, list 1 (each the search above,
+ * seeded with the left block's MV of that list, then SATD + 8) and their
+ * rounded average (SATD + 12).  A list whose reference is NULL is skipped.
+ * out[b] gets {intra, inter, list (0 intra, 1 L0, 2 L1, 3 bi), mv[2]}, with
+ * mv as the reference records it: at a list-0 win mv[1] is still (0, 0). */
+void o_tree_analyse(const uint8_t *cur, const uint8_t *l0, const uint8_t *l1, int w, int h, otree_block *out) {
+  const int bw = w / 8, bh = h / 8;
+  const uint8_t *refs[2] = {l0, l1};
+  for (int by = 0; by < bh; by++) {
+    omv left[2] = {{0, 0}, {0, 0}};
+    for (int bx = 0; bx < bw; bx++) {
+      const int x0 = 8 * bx, y0 = 8 * by;
+      const uint8_t *base = cur + (size_t)y0 * w + x0;
+      const uint32_t intra = o_lowres_intra(cur, w, x0, y0);
+      uint32_t best = intra;
+      uint8_t list = 0;
+      omv bmv[2] = {{0, 0}, {0, 0}}, mvs[2] = {{0, 0}, {0, 0}};
+      uint8_t p[2][64], bi[64];
+      for (int l = 0; l < 2; l++) {
+        if (!refs[l]) continue;
+        const omv m = o_tree_search(cur, refs[l], w, h, x0, y0, left[l]);
+        mvs[l] = m;
+        o_tree_fetch(refs[l], w, x0, y0, m, p[l]);
+        const uint32_t c = o_satd(base, w, p[l], 8, 8, 8) + 8;
+        if (c < best) {
+          best = c;
+          list = (uint8_t)(1 + l);
+          bmv[0] = mvs[0];
+          bmv[1] = mvs[1];
+        }
+      }
+      if (l0 && l1) {
+        for (int i = 0; i < 64; i++) bi[i] = (uint8_t)(((uint32_t)p[0][i] + p[1][i] + 1) >> 1);
+        const uint32_t c = o_satd(base, w, bi, 8, 8, 8) + 12;
+        if (c < best) {
+          best = c;
+          list = 3;
+          bmv[0] = mvs[0];
+          bmv[1] = mvs[1];
+        }
+      }
+      left[0] = mvs[0];
+      left[1] = mvs[1];
+      otree_block *o = &out[by * bw + bx];
+      o->intra = intra;
+      o->inter = best;
+      o->mv[0] = bmv[0];
+      o->mv[1] = bmv[1];
+      o->list = list;
+      o->reserved[0] = o->reserved[1] = o->reserved[2] = 0;
+    }
+  }
+}
+
+/* The CPU baseline of the lookahead: complexity (against the previous
+ * frame) and analyse (list 0 = the previous frame) of frames 1..n-1 of n
+ * w x h lowres frames, frames split over `threads` threads.  Returns
+ * seconds of wall time. */
+typedef struct {
+  const uint8_t *lr;
+  int w, h, n, threads, t;
+  uint64_t *sums;
+  otree_block *blocks;
+} o_la_arg;
+
+static void *o_la_worker(void *p) {
+  o_la_arg *a = (o_la_arg *)p;
+  const size_t fs = (size_t)a->w * a->h, nb = (size_t)(a->w / 8) * (a->h / 8);
+  for (int f = 1 + a->t; f < a->n; f += a->threads) {
+    o_complexity(a->lr + f * fs, a->lr + (f - 1) * fs, a->w, a->h, &a->sums[2 * f], &a->sums[2 * f + 1], NULL, NULL,
+                 NULL);
+    o_tree_analyse(a->lr + f * fs, a->lr + (f - 1) * fs, NULL, a->w, a->h, a->blocks + (size_t)f * nb);
+  }
+  return NULL;
+}
+
+double o_lookahead_mt(const uint8_t *lr, int w, int h, int n, int threads, uint64_t *sums, otree_block *blocks) {
+  struct timespec t0, t1;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  if (threads < 1) threads = 1;
+  pthread_t th[64];
+  o_la_arg args[64];
+  if (threads > 64) threads = 64;
+  for (int t = 0; t < threads; t++) {
+    args[t] = (o_la_arg){lr, w, h, n, threads, t, sums, blocks};
+    pthread_create(&th[t], NULL, o_la_worker, &args[t]);
+  }
+  for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+  clock_gettime(CLOCK_MONOTONIC, &t1);
   return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 }

@@ -205,3 +205,20 @@ int o_version(void);
 }
 #endif
 #endif
+
+/* ---- lookahead.rs / mbtree.rs: the scenecut complexity of lowres frames ---- */
+void o_lowres(const uint8_t *plane, int stride, int pad, int pw, int ph, uint8_t *out);
+uint32_t o_lowres_intra(const uint8_t *cur, int w, int x0, int y0);
+void o_complexity(const uint8_t *cur, const uint8_t *prev, int w, int h, uint64_t *intra, uint64_t *inter,
+                  uint32_t *blk_intra, uint32_t *blk_inter, int16_t *blk_mv);
+
+/* mbtree.rs:10-15 (Block): lowres block costs of the macroblock tree */
+typedef struct {
+  uint32_t intra, inter;
+  omv mv[2];
+  uint8_t list; /* 0 intra, 1 list 0, 2 list 1, 3 bi */
+  uint8_t reserved[3];
+} otree_block;
+omv o_tree_search(const uint8_t *cur, const uint8_t *r, int w, int h, int x0, int y0, omv seed);
+void o_tree_analyse(const uint8_t *cur, const uint8_t *l0, const uint8_t *l1, int w, int h, otree_block *out);
+double o_lookahead_mt(const uint8_t *lr, int w, int h, int n, int threads, uint64_t *sums, otree_block *blocks);

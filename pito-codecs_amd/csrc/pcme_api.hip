@@ -529,4 +529,106 @@ int pcme_pred_frame(const uint8_t *ref, const uint8_t *ref_h, const uint8_t *ref
   return PCME_OK;
 }
 
+
+// ------------------------------------------------------------- lookahead
+
+static bool a4(const void *p) { return ((uintptr_t)p & 3u) == 0; }
+
+static int check_lowres_frame(const char *fn, int i, const void *p, bool may_be_null) {
+  if (!p) return may_be_null ? PCME_OK : fail(PCME_EINVAL, "%s: job %d: null frame", fn, i);
+  if (!a4(p)) return fail(PCME_EINVAL, "%s: job %d: frame not 4-byte aligned", fn, i);
+  return PCME_OK;
+}
+
+static int check_lowres_dims(const char *fn, int i, int w, int h) {
+  if (w < 8 || h < 8 || (w & 7) || (h & 7))
+    return fail(PCME_EINVAL, "%s: job %d: lowres size %dx%d is not a positive multiple of 8", fn, i, w, h);
+  return PCME_OK;
+}
+
+int pcme_lowres(const uint8_t *planes, int64_t plane_pitch, int32_t stride, int32_t pad, int32_t w, int32_t h,
+                uint8_t *out, int64_t out_pitch, int32_t n, void *stream) {
+  if (n < 0 || w < 16 || h < 16 || (w & 15) || (h & 15) || pad < 0 || stride < w + pad || !planes || !out)
+    return fail(PCME_EINVAL, "pcme_lowres: bad geometry (w, h multiples of 16, stride >= w + pad)");
+  if (n > 1 && (plane_pitch < (int64_t)(h + pad) * stride || out_pitch < (int64_t)(w / 2) * (h / 2)))
+    return fail(PCME_EINVAL, "pcme_lowres: plane or lowres pitch smaller than a frame");
+  if (!a4(planes) || !a4(out) || ((plane_pitch | out_pitch) & 3) || ((stride | pad) & 3))
+    return fail(PCME_EINVAL, "pcme_lowres: planes, out, pitches, stride and pad must be 4-byte aligned");
+  if (n == 0) return PCME_OK;
+  Dev *dv;
+  int rc = get_dev(&dv);
+  if (rc) return rc;
+  const int w2 = w / 2;
+  dim3 grid((unsigned)((w2 / 4 + 255) / 256), (unsigned)(h / 2), (unsigned)n);
+  hipLaunchKernelGGL(lowres_kernel, grid, dim3(256), 0, (hipStream_t)stream, planes, plane_pitch, stride, pad, w2,
+                     out, out_pitch);
+  HIPCHK(hipGetLastError());
+  return PCME_OK;
+}
+
+int pcme_complexity(const pcme_cplx_job *jobs, int32_t n, void *stream) {
+  static const char *fn = "pcme_complexity";
+  if (n < 0 || (n > 0 && !jobs)) return fail(PCME_EINVAL, "%s: bad job list", fn);
+  int rc;
+  for (int i = 0; i < n; i++) {
+    const pcme_cplx_job &j = jobs[i];
+    if ((rc = check_lowres_dims(fn, i, j.w, j.h))) return rc;
+    if ((rc = check_lowres_frame(fn, i, j.cur, false))) return rc;
+    if ((rc = check_lowres_frame(fn, i, j.prev, true))) return rc;
+    if (!j.sums || ((uintptr_t)j.sums & 7u)) return fail(PCME_EINVAL, "%s: job %d: sums null or not 8-byte aligned", fn, i);
+    if (!a4(j.blocks)) return fail(PCME_EINVAL, "%s: job %d: blocks not 4-byte aligned", fn, i);
+  }
+  if (n == 0) return PCME_OK;
+  Dev *dv;
+  if ((rc = get_dev(&dv))) return rc;
+  for (int c0 = 0; c0 < n; c0 += LA_CHUNK) {
+    const int m = std::min(LA_CHUNK, n - c0);
+    CplxArgs A;
+    memset(&A, 0, sizeof A);
+    int maxnb = 0;
+    for (int i = 0; i < m; i++) {
+      A.j[i] = jobs[c0 + i];
+      maxnb = std::max(maxnb, (A.j[i].w / 8) * (A.j[i].h / 8));
+    }
+    hipLaunchKernelGGL(complexity_clear_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, A, m);
+    HIPCHK(hipGetLastError());
+    const int per_wg = 4 * CPLX_BPW;
+    hipLaunchKernelGGL(complexity_kernel, dim3((unsigned)((maxnb + per_wg - 1) / per_wg), (unsigned)m), dim3(256), 0,
+                       (hipStream_t)stream, A);
+    HIPCHK(hipGetLastError());
+  }
+  return PCME_OK;
+}
+
+int pcme_mbtree_analyse(const pcme_tree_job *jobs, int32_t n, void *stream) {
+  static const char *fn = "pcme_mbtree_analyse";
+  if (n < 0 || (n > 0 && !jobs)) return fail(PCME_EINVAL, "%s: bad job list", fn);
+  int rc;
+  for (int i = 0; i < n; i++) {
+    const pcme_tree_job &j = jobs[i];
+    if ((rc = check_lowres_dims(fn, i, j.w, j.h))) return rc;
+    if ((rc = check_lowres_frame(fn, i, j.cur, false))) return rc;
+    if ((rc = check_lowres_frame(fn, i, j.ref0, true))) return rc;
+    if ((rc = check_lowres_frame(fn, i, j.ref1, true))) return rc;
+    if (!j.blocks || !a4(j.blocks)) return fail(PCME_EINVAL, "%s: job %d: blocks null or not 4-byte aligned", fn, i);
+  }
+  if (n == 0) return PCME_OK;
+  Dev *dv;
+  if ((rc = get_dev(&dv))) return rc;
+  for (int c0 = 0; c0 < n; c0 += LA_CHUNK) {
+    const int m = std::min(LA_CHUNK, n - c0);
+    TreeArgs A;
+    memset(&A, 0, sizeof A);
+    int maxbh = 0;
+    for (int i = 0; i < m; i++) {
+      A.j[i] = jobs[c0 + i];
+      maxbh = std::max(maxbh, A.j[i].h / 8);
+    }
+    hipLaunchKernelGGL(tree_kernel, dim3((unsigned)((maxbh + 3) / 4), (unsigned)m), dim3(256), 0,
+                       (hipStream_t)stream, A);
+    HIPCHK(hipGetLastError());
+  }
+  return PCME_OK;
+}
+
 }  // extern "C"

@@ -25,6 +25,16 @@ struct DevJob {
   int pad_;
 };
 
+// Lookahead jobs travel in the kernel arguments, LA_CHUNK per launch.
+#define LA_CHUNK 32
+#define CPLX_BPW 2  // complexity_kernel: 8x8 blocks per wave
+struct CplxArgs {
+  pcme_cplx_job j[LA_CHUNK];
+};
+struct TreeArgs {
+  pcme_tree_job j[LA_CHUNK];
+};
+
 #ifdef __HIP__
 __global__ void search_frames_kernel(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
                                      int ntickets, uint32_t *ctl, uint32_t spin_limit);
@@ -50,7 +60,12 @@ __global__ void pred_frame_kernel(const uint8_t *__restrict__ pl0, const uint8_t
                                   uint8_t *__restrict__ oy, uint8_t *__restrict__ ou,
                                   uint8_t *__restrict__ ov);
 __global__ void prep_kernel(uint32_t *ctl);
+__global__ void lowres_kernel(const uint8_t *__restrict__ planes, int64_t plane_pitch, int stride, int pad, int w2,
+                              uint8_t *__restrict__ out, int64_t out_pitch);
 __global__ void clear_kernel(const uint32_t *ctl, uint32_t *xchg, size_t nwords);
+__global__ void complexity_kernel(const CplxArgs A);
+__global__ void complexity_clear_kernel(const CplxArgs A, int n);
+__global__ void tree_kernel(const TreeArgs A);
 #endif
 
 // Message returned by pcme_last_error() on this thread (host side).

@@ -1,0 +1,367 @@
+// pcme_lookahead.hip -- gfx950 kernels of the lowres lookahead: the 2x2
+// downsample (lookahead.rs:10-33), the scenecut complexity
+// (lookahead.rs:52-108) and the macroblock-tree block costs
+// (mbtree.rs:21-159).
+//
+// Kernels:
+//   lowres_kernel      one thread per 4 output bytes; HBM-bound (reads 1 B,
+//                      writes 1/4 B per input pixel).
+//   complexity_kernel  one wave per 8x8 block (blocks are independent):
+//                      intra cost, diamond SAD search from (0, 0), SATD at
+//                      its best; the frame sums are reduced per workgroup
+//                      and added with one 64-bit atomic each (integer adds:
+//                      the sums do not depend on the order).
+//   tree_kernel        one wave per block row: a row is sequential in the
+//                      reference (each block's search is seeded with its
+//                      left neighbour's MVs), rows and frames are not.
+//
+// Lane layouts (wave64):
+//   SAD passes   lane = 8*slot + row: slot = candidate (8 per pass), row =
+//                one 8-byte row of the block; the 8 row SADs of a slot are
+//                summed with DPP and the strict-< scan over the slots is a
+//                min over (cost, slot) keys (min_key).
+//   SATD passes  lane & 1 = strip (rows 0-3 or 4-7 of the block), bits 1-2
+//                select what the lane prices (an intra mode, or list 0 /
+//                list 1 / bi); the strip pair is summed with DPP and the
+//                per-kind totals read from lanes 0, 2, 4.  Lanes 8-63 repeat
+//                lanes 0-7: the wave stays converged and the extra lanes
+//                cost no time.
+//
+// Memory: lowres frames are read through L2 (a 1080p lowres frame is
+// 510 KiB); unaligned 8-byte rows are assembled from aligned dwords with
+// v_alignbyte.  A frame's base and width are 4-byte multiples, so every
+// dword that holds a needed byte lies inside the frame.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pcme_internal.h"
+#include "pcme_dev.h"
+
+namespace pcme {
+
+// ------------------------------------------------------------------ lowres
+
+// 2x2 sums of bytes (0,1) and (2,3) of rows x and y, rounded and halved
+// twice, in bytes 0 and 2 (16-bit lanes: at most 4*255+2, no carry out).
+__device__ __forceinline__ uint32_t lr_pair(uint32_t x, uint32_t y) {
+  const uint32_t s = (x & 0x00FF00FFu) + ((x >> 8) & 0x00FF00FFu) + (y & 0x00FF00FFu) +
+                     ((y >> 8) & 0x00FF00FFu) + 0x00020002u;
+  return (s >> 2) & 0x00FF00FFu;
+}
+
+__global__ void __launch_bounds__(256) lowres_kernel(const uint8_t *__restrict__ planes, int64_t plane_pitch,
+                                                     int stride, int pad, int w2, uint8_t *__restrict__ out,
+                                                     int64_t out_pitch) {
+  const int q = (int)(blockIdx.x * 256 + threadIdx.x);
+  if (4 * q >= w2) return;
+  const int y = (int)blockIdx.y, f = (int)blockIdx.z;
+  const uint8_t *r0 = planes + (int64_t)f * plane_pitch + (int64_t)(2 * y + pad) * stride + pad + 8 * q;
+  const uint32_t *a = (const uint32_t *)r0;
+  const uint32_t *b = (const uint32_t *)(r0 + stride);
+  const uint32_t lo = lr_pair(a[0], b[0]), hi = lr_pair(a[1], b[1]);
+  *(uint32_t *)(out + (int64_t)f * out_pitch + (int64_t)y * w2 + 4 * q) = __builtin_amdgcn_perm(hi, lo, 0x06040200u);
+}
+
+// ------------------------------------------------------- block primitives
+
+__device__ __forceinline__ uint32_t ld32(const uint8_t *p) { return *(const uint32_t *)p; }
+
+// The 8 bytes of row y of a lowres frame from column x (any alignment).
+__device__ __forceinline__ void row8(const uint8_t *frame, int w, int x, int y, uint32_t &lo, uint32_t &hi) {
+  const uint32_t *p = (const uint32_t *)(frame + (size_t)y * w + (x & ~3));
+  const uint32_t sh = (uint32_t)x & 3u;
+  const uint32_t a = p[0], b = p[1], c = p[sh ? 2 : 1];
+  lo = __builtin_amdgcn_alignbyte(b, a, sh);
+  hi = __builtin_amdgcn_alignbyte(c, b, sh);
+}
+
+// SATD of one 4-row x 8-column strip (dsp.rs:35-88, satd_strip<8, 4>):
+// S, P = source and prediction rows, two dwords (columns 0-3, 4-7) per row.
+__device__ __forceinline__ uint32_t strip_satd(const uint32_t *S, const uint32_t *P) {
+  v2s sp[16];
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) sp[r * 4 + c] = as_v2s(__builtin_amdgcn_perm(S[2 * r + 1], S[2 * r], SELC(c)));
+  return satd_strip8(sp, P);
+}
+
+// Sum of lane pairs (2k, 2k+1): the two strips of one 8x8 SATD.
+__device__ __forceinline__ uint32_t pair_sum(uint32_t v) {
+  return v + (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_QP_1032, 0xF, 0xF, false);
+}
+
+// Per-byte floor((a + b) / 2) and (a + b + 1) >> 1.
+__device__ __forceinline__ uint32_t avg_floor(uint32_t a, uint32_t b) {
+  return (a & b) + (((a ^ b) >> 1) & 0x7F7F7F7Fu);
+}
+__device__ __forceinline__ uint32_t avg_round(uint32_t a, uint32_t b) {
+  return (a | b) - (((a ^ b) >> 1) & 0x7F7F7F7Fu);
+}
+
+// Source registers of one block for one lane: its strip (S, 4 rows) and
+// its SAD row (s0, s1).
+struct BlkSrc {
+  uint32_t S[8];
+  uint32_t s0, s1;
+};
+
+__device__ __forceinline__ void load_blk(const uint8_t *cur, int w, int x0, int y0, BlkSrc &b) {
+  const int lane = (int)__lane_id();
+  const int strip = lane & 1;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const uint8_t *p = cur + (size_t)(y0 + 4 * strip + r) * w + x0;
+    b.S[2 * r] = ld32(p);
+    b.S[2 * r + 1] = ld32(p + 4);
+  }
+  const uint8_t *p = cur + (size_t)(y0 + (lane & 7)) * w + x0;
+  b.s0 = ld32(p);
+  b.s1 = ld32(p + 4);
+}
+
+// mbtree.rs:21-49 intra_cost: lane kind (lane >> 1) & 3 = mode 0 (average
+// of top row and left column, 128 at the top or left edge), 1 (left
+// column, needs x0 > 0), 2 (top row, needs y0 > 0), 3 (unused).  The least
+// SATD + 16; wave-uniform.
+__device__ __forceinline__ uint32_t intra_cost(const uint8_t *cur, int w, int x0, int y0, const BlkSrc &b) {
+  const int lane = (int)__lane_id();
+  const int mode = (lane >> 1) & 3, strip = lane & 1;
+  const bool hasL = x0 > 0, hasT = y0 > 0;
+  const uint8_t *tp = cur + (size_t)(hasT ? y0 - 1 : y0) * w + x0;
+  const uint32_t t0 = ld32(tp), t1 = ld32(tp + 4);
+  uint32_t P[8];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    // left neighbour = byte 3 of the dword ending at x0 - 1
+    const uint32_t l = hasL ? (ld32(cur + (size_t)(y0 + 4 * strip + r) * w + x0 - 4) >> 24) * 0x01010101u : 0u;
+    uint32_t p0, p1;
+    if (mode == 0) {
+      p0 = (hasL && hasT) ? avg_floor(t0, l) : 0x80808080u;
+      p1 = (hasL && hasT) ? avg_floor(t1, l) : 0x80808080u;
+    } else if (mode == 1) {
+      p0 = p1 = l;
+    } else {
+      p0 = t0;
+      p1 = t1;
+    }
+    P[2 * r] = p0;
+    P[2 * r + 1] = p1;
+  }
+  const uint32_t s = pair_sum(strip_satd(b.S, P));
+  const uint32_t m0 = rdlane(s, 0);
+  const uint32_t m1 = hasL ? rdlane(s, 2) : 0xFFFFFFFFu;
+  const uint32_t m2 = hasT ? rdlane(s, 4) : 0xFFFFFFFFu;
+  return min(m0, min(m1, m2)) + 16u;
+}
+
+// SAD of the block against ref at (x, y), summed over each slot's 8 lanes.
+__device__ __forceinline__ uint32_t slot_sad(const uint8_t *ref, int w, int x, int y, const BlkSrc &b) {
+  uint32_t lo, hi;
+  row8(ref, w, x, y + (int)(__lane_id() & 7u), lo, hi);
+  return red8(__builtin_amdgcn_sad_u8(b.s0, lo, __builtin_amdgcn_sad_u8(b.s1, hi, 0u)));
+}
+
+__device__ __forceinline__ int nib(uint32_t table, int k) { return (int)((table >> (4 * k)) & 15u) - 2; }
+
+// lookahead.rs:76-98: (-1,0),(1,0),(0,-1),(0,1),(-2,0),(2,0),(0,-2),(0,2), +2
+#define DIA_DX 0x22402231u
+#define DIA_DY 0x40223122u
+// mbtree.rs:79-88: (-1,0),(1,0),(0,-1),(0,1),(-2,-2),(2,-2),(-2,2),(2,2), +2
+#define TREE_DX 0x40402231u
+#define TREE_DY 0x44003122u
+
+// ------------------------------------------------------------- complexity
+
+__global__ void __launch_bounds__(256) complexity_kernel(const CplxArgs A) {
+  const pcme_cplx_job &J = A.j[blockIdx.y];
+  const int w = J.w, h = J.h, bw = w >> 3, nb = bw * (h >> 3);
+  const int wave = (int)(threadIdx.x >> 6), lane = (int)__lane_id();
+  const int first = (int)blockIdx.x * 4 * CPLX_BPW;
+  if (first >= nb) return;  // whole workgroup
+  const uint8_t *cur = J.cur, *prev = J.prev;
+  const int g = lane >> 3;
+  const int gdx = nib(DIA_DX, g), gdy = nib(DIA_DY, g);
+  uint64_t si = 0, sp = 0;
+  for (int i = 0; i < CPLX_BPW; i++) {
+    const int blk = first + wave * CPLX_BPW + i;
+    if (blk >= nb) break;
+    const int x0 = 8 * (blk % bw), y0 = 8 * (blk / bw);
+    BlkSrc b;
+    load_blk(cur, w, x0, y0, b);
+    const uint32_t ic = intra_cost(cur, w, x0, y0, b);
+    uint32_t pc = ic;
+    int mx = 0, my = 0;
+    if (prev) {
+      uint32_t bc = rdlane(slot_sad(prev, w, x0, y0, b), 0);
+      for (int it = 0; it < 8; it++) {
+        const int cx = mx + gdx, cy = my + gdy;
+        const bool ok = x0 + cx >= 0 && y0 + cy >= 0 && x0 + cx + 8 <= w && y0 + cy + 8 <= h;
+        const uint32_t c = slot_sad(prev, w, x0 + (ok ? cx : mx), y0 + (ok ? cy : my), b);
+        const uint32_t key = min_key(c, ok, g);
+        if (key == 0xFFFFFFFFu || (key >> 3) >= bc) break;
+        bc = key >> 3;
+        mx += nib(DIA_DX, (int)(key & 7u));
+        my += nib(DIA_DY, (int)(key & 7u));
+      }
+      uint32_t P[8];
+      const int strip = lane & 1;
+#pragma unroll
+      for (int r = 0; r < 4; r++) row8(prev, w, x0 + mx, y0 + my + 4 * strip + r, P[2 * r], P[2 * r + 1]);
+      const uint32_t sc = rdlane(pair_sum(strip_satd(b.S, P)), 0) + 8u;
+      pc = min(sc, ic);
+    }
+    si += ic;
+    sp += pc;
+    if (J.blocks && lane == 0) {
+      pcme_cplx_block o;
+      o.intra = ic;
+      o.inter = pc;
+      o.mv.x = (int16_t)mx;
+      o.mv.y = (int16_t)my;
+      J.blocks[blk] = o;
+    }
+  }
+  __shared__ unsigned long long red[2][4];
+  if (lane == 0) {
+    red[0][wave] = si;
+    red[1][wave] = sp;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd((unsigned long long *)&J.sums[0], red[0][0] + red[0][1] + red[0][2] + red[0][3]);
+    atomicAdd((unsigned long long *)&J.sums[1], red[1][0] + red[1][1] + red[1][2] + red[1][3]);
+  }
+}
+
+__global__ void __launch_bounds__(64) complexity_clear_kernel(const CplxArgs A, int n) {
+  const int t = (int)threadIdx.x;
+  if (t < 2 * n) A.j[t >> 1].sums[t & 1] = 0;
+}
+
+// ------------------------------------------------------------ mbtree costs
+
+// mbtree.rs:51-102 search: SAD + |mx| + |my|; (0, 0), then the seed, then
+// up to 16 rounds of the pattern.  Group 0 prices (0, 0) and group 1 the
+// seed in one pass.
+__device__ __forceinline__ void tree_search(const uint8_t *ref, int w, int h, int x0, int y0, const BlkSrc &b, int sx,
+                                            int sy, int &mx, int &my) {
+  const int lane = (int)__lane_id(), g = lane >> 3;
+  const bool seed_ok = x0 + sx >= 0 && y0 + sy >= 0 && x0 + sx <= w - 8 && y0 + sy <= h - 8;
+  const bool use_seed = g == 1 && seed_ok;
+  const uint32_t c0 = slot_sad(ref, w, x0 + (use_seed ? sx : 0), y0 + (use_seed ? sy : 0), b);
+  uint32_t bc = rdlane(c0, 0);
+  mx = 0;
+  my = 0;
+  if (seed_ok) {
+    const uint32_t cs = rdlane(c0, 8) + (uint32_t)(abs(sx) + abs(sy));
+    if (cs < bc) {
+      bc = cs;
+      mx = sx;
+      my = sy;
+    }
+  }
+  const int gdx = nib(TREE_DX, g), gdy = nib(TREE_DY, g);
+  for (int it = 0; it < 16; it++) {
+    const int cx = mx + gdx, cy = my + gdy;
+    const bool ok = x0 + cx >= 0 && y0 + cy >= 0 && x0 + cx <= w - 8 && y0 + cy <= h - 8;
+    const uint32_t c = slot_sad(ref, w, x0 + (ok ? cx : mx), y0 + (ok ? cy : my), b) + (uint32_t)(abs(cx) + abs(cy));
+    const uint32_t key = min_key(c, ok, g);
+    if (key == 0xFFFFFFFFu || (key >> 3) >= bc) break;
+    bc = key >> 3;
+    mx += nib(TREE_DX, (int)(key & 7u));
+    my += nib(TREE_DY, (int)(key & 7u));
+  }
+}
+
+__global__ void __launch_bounds__(256) tree_kernel(const TreeArgs A) {
+  const pcme_tree_job &J = A.j[blockIdx.y];
+  const int w = J.w, h = J.h, bw = w >> 3, bh = h >> 3;
+  const int by = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  if (by >= bh) return;
+  const int lane = (int)__lane_id();
+  const uint8_t *cur = J.cur, *r0 = J.ref0, *r1 = J.ref1;
+  const int kind = (lane >> 1) & 3, strip = lane & 1;
+  int l0x = 0, l0y = 0, l1x = 0, l1y = 0;  // left neighbour's MVs (mbtree.rs:124, 153)
+  const int y0 = 8 * by;
+  for (int bx = 0; bx < bw; bx++) {
+    const int x0 = 8 * bx;
+    BlkSrc b;
+    load_blk(cur, w, x0, y0, b);
+    const uint32_t intra = intra_cost(cur, w, x0, y0, b);
+    int m0x = 0, m0y = 0, m1x = 0, m1y = 0;
+    if (r0) tree_search(r0, w, h, x0, y0, b, l0x, l0y, m0x, m0y);
+    if (r1) tree_search(r1, w, h, x0, y0, b, l1x, l1y, m1x, m1y);
+    // kinds: 0 = list 0, 1 = list 1, 2 = bi, 3 = unused (a missing list
+    // reads cur, its cost is never used)
+    const uint8_t *ra = (kind == 1) ? (r1 ? r1 : cur) : (r0 ? r0 : cur);
+    const int ax = (kind == 1) ? (r1 ? m1x : 0) : (r0 ? m0x : 0);
+    const int ay = (kind == 1) ? (r1 ? m1y : 0) : (r0 ? m0y : 0);
+    uint32_t P[8];
+#pragma unroll
+    for (int r = 0; r < 4; r++) row8(ra, w, x0 + ax, y0 + ay + 4 * strip + r, P[2 * r], P[2 * r + 1]);
+    if (kind == 2) {
+      const uint8_t *rb = r1 ? r1 : cur;
+      const int bxm = r1 ? m1x : 0, bym = r1 ? m1y : 0;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        uint32_t q0, q1;
+        row8(rb, w, x0 + bxm, y0 + bym + 4 * strip + r, q0, q1);
+        P[2 * r] = avg_round(P[2 * r], q0);
+        P[2 * r + 1] = avg_round(P[2 * r + 1], q1);
+      }
+    }
+    const uint32_t s = pair_sum(strip_satd(b.S, P));
+    uint32_t best = intra, list = 0;
+    int o0x = 0, o0y = 0, o1x = 0, o1y = 0;
+    if (r0) {
+      const uint32_t c = rdlane(s, 0) + 8u;
+      if (c < best) {
+        best = c;
+        list = 1;
+        o0x = m0x;
+        o0y = m0y;
+      }
+    }
+    if (r1) {
+      const uint32_t c = rdlane(s, 2) + 8u;
+      if (c < best) {
+        best = c;
+        list = 2;
+        o0x = m0x;
+        o0y = m0y;
+        o1x = m1x;
+        o1y = m1y;
+      }
+    }
+    if (r0 && r1) {
+      const uint32_t c = rdlane(s, 4) + 12u;
+      if (c < best) {
+        best = c;
+        list = 3;
+        o0x = m0x;
+        o0y = m0y;
+        o1x = m1x;
+        o1y = m1y;
+      }
+    }
+    l0x = m0x;
+    l0y = m0y;
+    l1x = m1x;
+    l1y = m1y;
+    if (lane == 0) {
+      pcme_tree_block o;
+      o.intra = intra;
+      o.inter = best;
+      o.mv[0].x = (int16_t)o0x;
+      o.mv[0].y = (int16_t)o0y;
+      o.mv[1].x = (int16_t)o1x;
+      o.mv[1].y = (int16_t)o1y;
+      o.list = (uint8_t)list;
+      o.reserved[0] = o.reserved[1] = o.reserved[2] = 0;
+      J.blocks[by * bw + bx] = o;
+    }
+  }
+}
+
+}  // namespace pcme

@@ -80,6 +80,23 @@ class SynthCfg(ctypes.Structure):
                 ("seed", ctypes.c_uint64)]
 
 
+class CplxJob(ctypes.Structure):
+    _fields_ = [("cur", ctypes.c_void_p), ("prev", ctypes.c_void_p), ("blocks", ctypes.c_void_p),
+                ("sums", ctypes.c_void_p), ("w", ctypes.c_int32), ("h", ctypes.c_int32)]
+
+
+class TreeJob(ctypes.Structure):
+    _fields_ = [("cur", ctypes.c_void_p), ("ref0", ctypes.c_void_p), ("ref1", ctypes.c_void_p),
+                ("blocks", ctypes.c_void_p), ("w", ctypes.c_int32), ("h", ctypes.c_int32)]
+
+
+# pcme_cplx_block (12 bytes) and pcme_tree_block (20 bytes) as numpy records
+CPLX_BLOCK_DTYPE = np.dtype([("intra", "<u4"), ("inter", "<u4"), ("mv", "<i2", (2,))])
+assert CPLX_BLOCK_DTYPE.itemsize == 12
+TREE_BLOCK_DTYPE = np.dtype([("intra", "<u4"), ("inter", "<u4"), ("mv", "<i2", (2, 2)), ("list", "u1"),
+                             ("reserved", "u1", (3,))])
+assert TREE_BLOCK_DTYPE.itemsize == 20
+
 # pcme_mb_out as a numpy record (128 bytes)
 MB_DTYPE = np.dtype([
     ("mv", "<i2", (16, 2)), ("mvs", "<i2", (4, 2)), ("mvps", "<i2", (4, 2)),
@@ -104,6 +121,7 @@ EXPORTS = (
     "pcme_plan_set_grid", "pcme_plan_autotune", "pcme_plan_destroy", "pcme_set_spin_limit",
     "pcme_net_create", "pcme_net_destroy", "pcme_net_get_params", "pcme_net_set_params",
     "pcme_net_inputs", "pcme_net_forward", "pcme_net_grad", "pcme_net_train_step", "pcme_net_hints",
+    "pcme_lowres", "pcme_complexity", "pcme_mbtree_analyse",
 )
 
 _lib = None
@@ -147,6 +165,10 @@ def load() -> ctypes.CDLL:
         "pcme_plan_autotune": (c.c_int, [vp, vp, c.c_int32, c.POINTER(c.c_int32), c.POINTER(c.c_float)]),
         "pcme_plan_destroy": (None, [vp]),
         "pcme_set_spin_limit": (c.c_uint32, [c.c_uint32]),
+        "pcme_lowres": (c.c_int, [vp, c.c_int64, c.c_int32, c.c_int32, c.c_int32, c.c_int32, vp, c.c_int64,
+                                  c.c_int32, vp]),
+        "pcme_complexity": (c.c_int, [c.POINTER(CplxJob), c.c_int32, vp]),
+        "pcme_mbtree_analyse": (c.c_int, [c.POINTER(TreeJob), c.c_int32, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -1,0 +1,164 @@
+"""The encoder's lowres lookahead on the GPU, behind libpcme's C ABI.
+
+Mirrors the reference's interface (pito-codecs src/h264/enc/):
+
+* ``LowRes.new(frame)``       lookahead.rs:10-33, the 2x2-averaged luma
+* ``complexity(cur, prev)``   lookahead.rs:52-108, the scenecut input
+                              (``Complexity {intra, inter}``)
+* ``analyse(cur, l0, l1)``    mbtree.rs:113-159, the per-block costs of the
+                              macroblock tree (``Costs.blocks``)
+
+plus batched forms that take many frames in one launch.  Results are
+bit-identical to the reference.  There is no CPU path: without libpcme or a
+gfx950 device every call raises ``PcmeError``.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import (CPLX_BLOCK_DTYPE, TREE_BLOCK_DTYPE, CplxJob, DeviceClip, LUMA_PAD, PcmeError, PCME_EINVAL,
+               TreeJob, _check, _stream_ptr, _torch, load)
+
+
+@dataclass
+class Complexity:
+    """lookahead.rs:35-39"""
+    intra: int
+    inter: int
+
+
+class LowRes:
+    """A lowres frame resident in HBM (lookahead.rs:4-8): ``data`` is a
+    [h, w] uint8 tensor on the GPU."""
+
+    def __init__(self, data):
+        self.data = data
+        self.h, self.w = (int(s) for s in data.shape)
+
+    @staticmethod
+    def new(clip: DeviceClip, i: int, stream=None) -> "LowRes":
+        """LowRes::new of frame i of a prepared clip."""
+        return LowRes(lowres_frames(clip, i, 1, stream)[0])
+
+
+def lowres_frames(clip: DeviceClip, first: int = 0, count: int | None = None, stream=None):
+    """LowRes::new of frames [first, first+count) of a clip whose luma planes
+    are padded: a [count, 8*mbh, 8*mbw] uint8 tensor."""
+    torch = clip.torch
+    count = clip.n - first if count is None else count
+    w, h = 16 * clip.mbw, 16 * clip.mbh
+    out = torch.empty((max(count, 0), h // 2, w // 2), dtype=torch.uint8, device=clip.raw.device)
+    if count > 0:
+        _check(load().pcme_lowres(clip.plane_ptr(0, first), clip.pitch, clip.stride, LUMA_PAD, w, h,
+                                  out.data_ptr(), (h // 2) * (w // 2), count, _stream_ptr(stream)))
+    return out
+
+
+def _ptr(x) -> int | None:
+    if x is None:
+        return None
+    return (x.data if isinstance(x, LowRes) else x).data_ptr()
+
+
+def _dims(x) -> tuple[int, int]:
+    d = x.data if isinstance(x, LowRes) else x
+    return int(d.shape[-1]), int(d.shape[-2])
+
+
+def complexity_jobs(frames, pairs, sums, blocks=None) -> list[CplxJob]:
+    """Jobs of pcme_complexity: pairs = [(cur, prev or None)] indices into
+    frames [n, h, w]; sums [len(pairs), 2] int64; blocks [len(pairs),
+    nblocks * 12] uint8 or None."""
+    n, h, w = (int(s) for s in frames.shape)
+    jobs = []
+    for k, (c, p) in enumerate(pairs):
+        if not (0 <= c < n) or (p is not None and not (0 <= p < n)):
+            raise PcmeError(PCME_EINVAL, f"complexity job {k}: frame index out of range")
+        jobs.append(CplxJob(frames[c].data_ptr(), None if p is None else frames[p].data_ptr(),
+                            None if blocks is None else blocks[k].data_ptr(), sums[k].data_ptr(), w, h))
+    return jobs
+
+
+def run_complexity(jobs: list[CplxJob], stream=None) -> None:
+    arr = (CplxJob * len(jobs))(*jobs)
+    _check(load().pcme_complexity(arr, len(jobs), _stream_ptr(stream)))
+
+
+def complexity_batch(frames, pairs, with_blocks: bool = False, stream=None):
+    """complexity() of every (cur, prev) index pair over frames [n, h, w]
+    (device).  Returns (sums [k, 2] int64 tensor, blocks [k, bh, bw]
+    CPLX_BLOCK_DTYPE array or None); synchronises when blocks are asked for."""
+    torch = _torch()
+    _, h, w = (int(s) for s in frames.shape)
+    nb = (w // 8) * (h // 8)
+    sums = torch.empty((len(pairs), 2), dtype=torch.int64, device=frames.device)
+    blocks = torch.empty((len(pairs), nb * CPLX_BLOCK_DTYPE.itemsize), dtype=torch.uint8,
+                         device=frames.device) if with_blocks else None
+    run_complexity(complexity_jobs(frames, pairs, sums, blocks), stream)
+    if blocks is None:
+        return sums, None
+    return sums, blocks.cpu().numpy().view(CPLX_BLOCK_DTYPE).reshape(len(pairs), h // 8, w // 8)
+
+
+def complexity(cur, prev=None, stream=None) -> Complexity:
+    """lookahead::complexity of one lowres frame against the previous one."""
+    torch = _torch()
+    w, h = _dims(cur)
+    if prev is not None and _dims(prev) != (w, h):
+        raise PcmeError(PCME_EINVAL, "complexity: cur and prev differ in size")
+    sums = torch.empty(2, dtype=torch.int64, device=(cur.data if isinstance(cur, LowRes) else cur).device)
+    run_complexity([CplxJob(_ptr(cur), _ptr(prev), None, sums.data_ptr(), w, h)], stream)
+    a, b = sums.cpu().tolist()
+    return Complexity(int(a), int(b))
+
+
+def tree_jobs(frames, triples, blocks) -> list[TreeJob]:
+    """Jobs of pcme_mbtree_analyse: triples = [(cur, l0 or None, l1 or None)]
+    indices into frames [n, h, w]; blocks [len(triples), nblocks * 20]."""
+    n, h, w = (int(s) for s in frames.shape)
+    jobs = []
+    for k, (c, a, b) in enumerate(triples):
+        for i in (c, a, b):
+            if i is not None and not (0 <= i < n):
+                raise PcmeError(PCME_EINVAL, f"analyse job {k}: frame index out of range")
+        jobs.append(TreeJob(frames[c].data_ptr(), None if a is None else frames[a].data_ptr(),
+                            None if b is None else frames[b].data_ptr(), blocks[k].data_ptr(), w, h))
+    return jobs
+
+
+def run_analyse(jobs: list[TreeJob], stream=None) -> None:
+    arr = (TreeJob * len(jobs))(*jobs)
+    _check(load().pcme_mbtree_analyse(arr, len(jobs), _stream_ptr(stream)))
+
+
+def analyse_batch(frames, triples, stream=None):
+    """mbtree::analyse of every (cur, l0, l1) over frames [n, h, w] (device):
+    the blocks as a [k, nblocks * 20] uint8 device tensor (view it with
+    TREE_BLOCK_DTYPE)."""
+    torch = _torch()
+    _, h, w = (int(s) for s in frames.shape)
+    nb = (w // 8) * (h // 8)
+    blocks = torch.empty((len(triples), nb * TREE_BLOCK_DTYPE.itemsize), dtype=torch.uint8, device=frames.device)
+    run_analyse(tree_jobs(frames, triples, blocks), stream)
+    return blocks
+
+
+def analyse(cur, l0=None, l1=None, stream=None) -> np.ndarray:
+    """mbtree::analyse of one lowres frame: its Costs.blocks as a
+    [bh*bw] TREE_BLOCK_DTYPE array."""
+    torch = _torch()
+    w, h = _dims(cur)
+    for r in (l0, l1):
+        if r is not None and _dims(r) != (w, h):
+            raise PcmeError(PCME_EINVAL, "analyse: reference size differs from cur")
+    nb = (w // 8) * (h // 8)
+    dev = (cur.data if isinstance(cur, LowRes) else cur).device
+    blocks = torch.empty(nb * TREE_BLOCK_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    run_analyse([TreeJob(_ptr(cur), _ptr(l0), _ptr(l1), blocks.data_ptr(), w, h)], stream)
+    return blocks.cpu().numpy().view(TREE_BLOCK_DTYPE)
+
+
+__all__ = ["Complexity", "LowRes", "lowres_frames", "complexity", "complexity_batch", "complexity_jobs",
+           "run_complexity", "analyse", "analyse_batch", "tree_jobs", "run_analyse"]

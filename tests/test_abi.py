@@ -125,3 +125,27 @@ def test_pruner_without_device_fails_loudly(pcme_lib):
     with pytest.raises(pcme_lib.PcmeError) as e:
         P.Net(seed=1, max_batch=64)
     assert e.value.code == pcme_lib.PCME_ENODEV
+
+
+def test_lookahead_entry_points_check_arguments(pcme_lib):
+    """Sizes that are not multiples of 8, misaligned frames and missing
+    outputs are refused before any device work; well-formed calls without
+    a gfx950 device report PCME_ENODEV."""
+    lib = pcme_lib.load()
+    C, T = pcme_lib.CplxJob, pcme_lib.TreeJob
+    bad = [C(64, None, None, 64, 12, 8), C(66, None, None, 64, 8, 8), C(64, None, None, 0, 8, 8),
+           C(64, None, None, 68, 8, 8), C(64, 64, 66, 64, 8, 8)]
+    for j in bad:
+        assert lib.pcme_complexity((C * 1)(j), 1, None) == pcme_lib.PCME_EINVAL
+        assert b"pcme_complexity" in lib.pcme_last_error()
+    assert lib.pcme_complexity(None, 1, None) == pcme_lib.PCME_EINVAL
+    assert lib.pcme_complexity(None, 0, None) == pcme_lib.PCME_OK
+    for j in [T(64, None, None, 0, 8, 8), T(64, 65, None, 64, 8, 8), T(64, None, None, 64, 8, 20)]:
+        assert lib.pcme_mbtree_analyse((T * 1)(j), 1, None) == pcme_lib.PCME_EINVAL
+    assert lib.pcme_lowres(64, 0, 64, 0, 24, 16, 64, 0, 1, None) == pcme_lib.PCME_EINVAL  # w % 16
+    assert lib.pcme_lowres(66, 0, 64, 0, 16, 16, 64, 0, 1, None) == pcme_lib.PCME_EINVAL  # misaligned
+    assert lib.pcme_lowres(64, 100, 64, 0, 16, 16, 64, 64, 2, None) == pcme_lib.PCME_EINVAL  # pitch < frame
+    if pcme_lib.device_count() == 0:
+        assert lib.pcme_complexity((C * 1)(C(64, None, None, 64, 8, 8)), 1, None) == pcme_lib.PCME_ENODEV
+        assert lib.pcme_mbtree_analyse((T * 1)(T(64, None, None, 64, 8, 8)), 1, None) == pcme_lib.PCME_ENODEV
+        assert lib.pcme_lowres(64, 0, 64, 0, 16, 16, 64, 0, 1, None) == pcme_lib.PCME_ENODEV
