@@ -15,6 +15,11 @@ of 352x288, 640x360, 1280x720, 1920x1080), 30 frames each, QP per clip from
 120 frames, me_range 64, QP 27, 8 rects of 128-900 px at up to 16 px/frame,
 0.2%/frame zoom.
 
+The line also carries the CNN-pruned search of the same frames ("pruned")
+and the lowres lookahead of the same frames ("lookahead": LowRes::new,
+scenecut complexity and macroblock-tree costs, lookahead.rs / mbtree.rs),
+each with its own timing.
+
 Prints one JSON line (fields at the end of main()).
 """
 from __future__ import annotations
@@ -55,6 +60,7 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-pruned", action="store_true", help="skip the CNN-pruned leg")
+    ap.add_argument("--no-lookahead", action="store_true", help="skip the lowres-lookahead leg")
     ap.add_argument("--ncand", type=int, default=5, help="pruner candidates per MB")
     ap.add_argument("--weights", default=None, help="pruner weights (.npz; default: the committed ones)")
     ap.add_argument("--thr", type=float, default=0.6, help="pruner confidence threshold (partition head)")
@@ -254,6 +260,9 @@ def main():
     if not args.no_pruned:
         line["pruned"] = pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, ms_step)
 
+    if not args.no_lookahead:
+        line["lookahead"] = lookahead_leg(args, pcme, torch, clips, stream, threads)
+
     if not args.no_cpu:
         from oracle import oracle as O
         y, mbw, mbh, qp, rng = host[-1]
@@ -280,6 +289,99 @@ def main():
                                 "seconds": round(secs, 3)}
         line["parity_sample"] = bool(parity)
     print(json.dumps(line), flush=True)
+
+
+def lookahead_leg(args, pcme, torch, clips, stream, threads):
+    """The lowres lookahead of the same frames (lookahead.rs, mbtree.rs): a
+    step = LowRes::new of every frame + complexity of every frame against
+    its predecessor (the first alone) + the macroblock-tree costs of every
+    frame with its predecessor as list 0.  The C oracle times the same work
+    on the last clip's frames on the host."""
+    import numpy as np
+    from pcme import lookahead as L
+    sp = stream.cuda_stream
+    lrs, cjobs, tjobs, sums, tblocks = [], [], [], [], []
+    blocks_per_step = 0
+    for dc in clips:
+        lr = L.lowres_frames(dc, stream=sp)
+        n = dc.n
+        s = torch.zeros((n, 2), dtype=torch.int64, device="cuda")
+        nb = dc.mbw * dc.mbh
+        tb = torch.empty((n, nb * pcme.TREE_BLOCK_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+        cjobs += L.complexity_jobs(lr, [(0, None)] + [(t, t - 1) for t in range(1, n)], s)
+        tjobs += L.tree_jobs(lr, [(0, None, None)] + [(t, t - 1, None) for t in range(1, n)], tb)
+        lrs.append(lr)
+        sums.append(s)
+        tblocks.append(tb)
+        blocks_per_step += n * nb
+    ev = []
+
+    def lowres_all():
+        for dc, lr in zip(clips, lrs):
+            w, h = 16 * dc.mbw, 16 * dc.mbh
+            pcme._check(pcme.load().pcme_lowres(dc.plane_ptr(0, 0), dc.pitch, dc.stride, pcme.LUMA_PAD, w, h,
+                                                lr.data_ptr(), (w // 2) * (h // 2), dc.n, sp))
+
+    def step(record):
+        if record:
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+            e[0].record(stream)
+        lowres_all()
+        if record:
+            e[1].record(stream)
+        L.run_complexity(cjobs, sp)
+        if record:
+            e[2].record(stream)
+        L.run_analyse(tjobs, sp)
+        if record:
+            e[3].record(stream)
+            ev.append(e)
+
+    for dc in clips:
+        dc.prepare(sp)
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms = [sum(e[i].elapsed_time(e[i + 1]) for e in ev) / len(ev) for i in range(3)]
+    ms_step = 1e3 * dt / args.steps
+    in_bytes = sum(dc.n * 256 * dc.mbw * dc.mbh for dc in clips)  # coded luma read once
+    lowres_gbs = 1.25 * in_bytes / (ms[0] * 1e-3) / 1e9            # + 1/4 written
+    out = {
+        "workload": "LowRes::new + complexity (vs previous frame) + mbtree analyse (list 0 = previous frame) "
+                    "of every frame",
+        "frames": sum(dc.n for dc in clips), "blocks_per_step": blocks_per_step,
+        "ms_per_step": round(ms_step, 3), "mblocks_per_s": round(blocks_per_step / (ms_step * 1e-3) / 1e6, 4),
+        "lowres_ms": round(ms[0], 4), "complexity_ms": round(ms[1], 4), "analyse_ms": round(ms[2], 4),
+        "lowres_roofline": {"bound": "hbm", "kernel": "lowres_kernel", "achieved": round(lowres_gbs, 1),
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(lowres_gbs / HBM_PEAK_GBS, 4),
+                            "algorithmic_bytes_per_block": 320},
+    }
+    print(f"[bench] lookahead: {ms_step:.3f} ms/step (lowres {ms[0]:.3f}, complexity {ms[1]:.3f}, "
+          f"analyse {ms[2]:.3f} ms)", file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        from oracle import oracle as O
+        dc, lr = clips[-1], lrs[-1]
+        k = min(args.cpu_frames + 1, dc.n)
+        frames = lr[:k].cpu().numpy()
+        secs, csums, cblocks = O.lookahead_mt(frames, threads)
+        nb = dc.mbw * dc.mbh
+        cpu_val = (k - 1) * nb / secs / 1e6
+        gs = sums[-1][:k].cpu().numpy().astype(np.uint64)
+        gb = tblocks[-1][:k].cpu().numpy().view(pcme.TREE_BLOCK_DTYPE).reshape(k, nb)
+        parity = bool(np.array_equal(gs[1:], csums[1:]) and
+                      np.array_equal(gb[1:].view(np.uint8), np.ascontiguousarray(cblocks[1:]).view(np.uint8)))
+        out["cpu_baseline"] = {"value": round(cpu_val, 5), "unit": "Mblocks/s", "cores": threads, "kind": "port",
+                               "sample": f"complexity + mbtree analyse of {k - 1} lowres frames "
+                                         f"({8 * dc.mbw}x{8 * dc.mbh}) of the last clip, one frame per thread, "
+                                         f"C oracle -O3 -march=x86-64-v3",
+                               "seconds": round(secs, 3)}
+        out["parity_sample"] = parity
+    return out
 
 
 def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plain_ms_step):

@@ -590,7 +590,7 @@ int pcme_complexity(const pcme_cplx_job *jobs, int32_t n, void *stream) {
       A.j[i] = jobs[c0 + i];
       maxnb = std::max(maxnb, (A.j[i].w / 8) * (A.j[i].h / 8));
     }
-    hipLaunchKernelGGL(complexity_clear_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, A, m);
+    hipLaunchKernelGGL(complexity_clear_kernel, dim3(1), dim3(2 * LA_CHUNK), 0, (hipStream_t)stream, A, m);
     HIPCHK(hipGetLastError());
     const int per_wg = 4 * CPLX_BPW;
     hipLaunchKernelGGL(complexity_kernel, dim3((unsigned)((maxnb + per_wg - 1) / per_wg), (unsigned)m), dim3(256), 0,

@@ -25,8 +25,9 @@ struct DevJob {
   int pad_;
 };
 
-// Lookahead jobs travel in the kernel arguments, LA_CHUNK per launch.
-#define LA_CHUNK 32
+// Lookahead jobs travel in the kernel arguments, LA_CHUNK per launch
+// (64 x 40 B of the 4 KiB kernel-argument space).
+#define LA_CHUNK 64
 #define CPLX_BPW 2  // complexity_kernel: 8x8 blocks per wave
 struct CplxArgs {
   pcme_cplx_job j[LA_CHUNK];
@@ -34,6 +35,7 @@ struct CplxArgs {
 struct TreeArgs {
   pcme_tree_job j[LA_CHUNK];
 };
+static_assert(sizeof(CplxArgs) <= 4000 && sizeof(TreeArgs) <= 4000, "lookahead jobs exceed the kernel arguments");
 
 #ifdef __HIP__
 __global__ void search_frames_kernel(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,

@@ -171,13 +171,24 @@ __device__ __forceinline__ int nib(uint32_t table, int k) { return (int)((table 
 #define TREE_DX 0x40402231u
 #define TREE_DY 0x44003122u
 
+// Workgroups are handed to the 8 XCDs round-robin in launch order.  This
+// remap gives XCD k the k-th contiguous eighth of the launch instead, so
+// the frames one XCD works on stay few and in its own L2.  A bijection of
+// [0, n) for any n.
+__device__ __forceinline__ int xcd_remap(int l, int n) {
+  const int xcd = l & 7, r = l >> 3, q = n >> 3, rem = n & 7;
+  return xcd < rem ? xcd * (q + 1) + r : rem * (q + 1) + (xcd - rem) * q + r;
+}
+
 // ------------------------------------------------------------- complexity
 
 __global__ void __launch_bounds__(256) complexity_kernel(const CplxArgs A) {
-  const pcme_cplx_job &J = A.j[blockIdx.y];
+  const int gx = (int)gridDim.x;
+  const int wg = xcd_remap((int)(blockIdx.x + blockIdx.y * gridDim.x), gx * (int)gridDim.y);
+  const pcme_cplx_job &J = A.j[wg / gx];
   const int w = J.w, h = J.h, bw = w >> 3, nb = bw * (h >> 3);
   const int wave = (int)(threadIdx.x >> 6), lane = (int)__lane_id();
-  const int first = (int)blockIdx.x * 4 * CPLX_BPW;
+  const int first = (wg % gx) * 4 * CPLX_BPW;
   if (first >= nb) return;  // whole workgroup
   const uint8_t *cur = J.cur, *prev = J.prev;
   const int g = lane >> 3;
@@ -234,7 +245,8 @@ __global__ void __launch_bounds__(256) complexity_kernel(const CplxArgs A) {
   }
 }
 
-__global__ void __launch_bounds__(64) complexity_clear_kernel(const CplxArgs A, int n) {
+// One thread per sum: launched with 2 * LA_CHUNK threads.
+__global__ void __launch_bounds__(2 * LA_CHUNK) complexity_clear_kernel(const CplxArgs A, int n) {
   const int t = (int)threadIdx.x;
   if (t < 2 * n) A.j[t >> 1].sums[t & 1] = 0;
 }
@@ -275,9 +287,11 @@ __device__ __forceinline__ void tree_search(const uint8_t *ref, int w, int h, in
 }
 
 __global__ void __launch_bounds__(256) tree_kernel(const TreeArgs A) {
-  const pcme_tree_job &J = A.j[blockIdx.y];
+  const int gx = (int)gridDim.x;
+  const int wg = xcd_remap((int)(blockIdx.x + blockIdx.y * gridDim.x), gx * (int)gridDim.y);
+  const pcme_tree_job &J = A.j[wg / gx];
   const int w = J.w, h = J.h, bw = w >> 3, bh = h >> 3;
-  const int by = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  const int by = (wg % gx) * 4 + (int)(threadIdx.x >> 6);
   if (by >= bh) return;
   const int lane = (int)__lane_id();
   const uint8_t *cur = J.cur, *r0 = J.ref0, *r1 = J.ref1;

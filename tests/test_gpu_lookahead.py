@@ -98,7 +98,7 @@ def test_complexity_edge_content(gpu, oracle):
               rng.integers(0, 256, (32, 48)).astype(np.uint8), rng.integers(0, 256, (32, 48)).astype(np.uint8)]
     fr = torch.as_tensor(np.stack(frames)).cuda()
     pairs = [(i, j) for i in range(4) for j in range(4)] + [(i, None) for i in range(4)]
-    pairs = pairs * 2  # 40 jobs: two launches
+    pairs = pairs * 4  # 80 jobs: two launches
     sums, blocks = L.complexity_batch(fr, pairs, with_blocks=True)
     torch.cuda.synchronize()
     sums = sums.cpu().numpy()
@@ -160,7 +160,7 @@ def test_mbtree_analyse_edge_content(gpu, oracle):
     frames = [np.zeros((24, 40), np.uint8), np.full((24, 40), 255, np.uint8),
               rng.integers(0, 256, (24, 40)).astype(np.uint8), rng.integers(0, 256, (24, 40)).astype(np.uint8)]
     fr = torch.as_tensor(np.stack(frames)).cuda()
-    triples = [(i, j, k) for i in range(4) for j in (None, 0, 2) for k in (None, 1, 3)]  # 36 jobs
+    triples = [(i, j, k) for i in range(4) for j in (None, 0, 2) for k in (None, 1, 3)] * 2  # 72 jobs: two launches
     got = L.analyse_batch(fr, triples).cpu().numpy().view(TREE_BLOCK_DTYPE)
     torch.cuda.synchronize()
     for n_, (c, a, b) in enumerate(triples):
