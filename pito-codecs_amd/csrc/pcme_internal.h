@@ -28,7 +28,7 @@ struct DevJob {
 // Lookahead jobs travel in the kernel arguments, LA_CHUNK per launch
 // (64 x 40 B of the 4 KiB kernel-argument space).
 #define LA_CHUNK 64
-#define CPLX_BPW 2  // complexity_kernel: 8x8 blocks per wave
+#define CPLX_BPW 8  // complexity_kernel: 8x8 blocks per wave (one per 8-lane group)
 struct CplxArgs {
   pcme_cplx_job j[LA_CHUNK];
 };

@@ -99,35 +99,23 @@ __device__ __forceinline__ uint32_t avg_round(uint32_t a, uint32_t b) {
   return (a | b) - (((a ^ b) >> 1) & 0x7F7F7F7Fu);
 }
 
-// Source registers of one block for one lane: its strip (S, 4 rows) and
-// its SAD row (s0, s1).
-struct BlkSrc {
-  uint32_t S[8];
-  uint32_t s0, s1;
-};
-
-__device__ __forceinline__ void load_blk(const uint8_t *cur, int w, int x0, int y0, BlkSrc &b) {
-  const int lane = (int)__lane_id();
-  const int strip = lane & 1;
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const uint8_t *p = cur + (size_t)(y0 + 4 * strip + r) * w + x0;
-    b.S[2 * r] = ld32(p);
-    b.S[2 * r + 1] = ld32(p + 4);
-  }
-  const uint8_t *p = cur + (size_t)(y0 + (lane & 7)) * w + x0;
-  b.s0 = ld32(p);
-  b.s1 = ld32(p + 4);
-}
-
-// mbtree.rs:21-49 intra_cost: lane kind (lane >> 1) & 3 = mode 0 (average
-// of top row and left column, 128 at the top or left edge), 1 (left
-// column, needs x0 > 0), 2 (top row, needs y0 > 0), 3 (unused).  The least
-// SATD + 16; wave-uniform.
-__device__ __forceinline__ uint32_t intra_cost(const uint8_t *cur, int w, int x0, int y0, const BlkSrc &b) {
+// Intra costs of eight blocks at once (mbtree.rs:21-49 intra_cost): lane
+// group g = lane >> 3 prices the block at (x0, y0) of its lanes, lane kind
+// (lane >> 1) & 3 = mode 0 (average of top row and left column, 128 at the
+// top or left edge), 1 (left column, needs x0 > 0), 2 (top row, needs
+// y0 > 0), 3 (unused); lane & 1 = strip.  Loads the lane's strip of the
+// source into S (kept for the batch's final SATD pass) and returns the
+// block's least SATD + 16 in every lane of the group.
+__device__ __forceinline__ uint32_t intra8(const uint8_t *cur, int w, int x0, int y0, uint32_t *S) {
   const int lane = (int)__lane_id();
   const int mode = (lane >> 1) & 3, strip = lane & 1;
   const bool hasL = x0 > 0, hasT = y0 > 0;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const uint8_t *p = cur + (size_t)(y0 + 4 * strip + r) * w + x0;
+    S[2 * r] = ld32(p);
+    S[2 * r + 1] = ld32(p + 4);
+  }
   const uint8_t *tp = cur + (size_t)(hasT ? y0 - 1 : y0) * w + x0;
   const uint32_t t0 = ld32(tp), t1 = ld32(tp + 4);
   uint32_t P[8];
@@ -148,18 +136,29 @@ __device__ __forceinline__ uint32_t intra_cost(const uint8_t *cur, int w, int x0
     P[2 * r] = p0;
     P[2 * r + 1] = p1;
   }
-  const uint32_t s = pair_sum(strip_satd(b.S, P));
-  const uint32_t m0 = rdlane(s, 0);
-  const uint32_t m1 = hasL ? rdlane(s, 2) : 0xFFFFFFFFu;
-  const uint32_t m2 = hasT ? rdlane(s, 4) : 0xFFFFFFFFu;
-  return min(m0, min(m1, m2)) + 16u;
+  uint32_t v = pair_sum(strip_satd(S, P));
+  const bool ok = mode == 0 || (mode == 1 && hasL) || (mode == 2 && hasT);
+  v = ok ? v : 0xFFFFFFFFu;
+  // min over the group: lanes (0,2),(1,3),(4,6),(5,7), then i <-> 7-i
+  uint32_t t = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, DPP_QP_2301, 0xF, 0xF, false);
+  v = min(v, t);
+  t = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, DPP_ROW_HALF_MIRROR, 0xF, 0xF, false);
+  v = min(v, t);
+  return v + 16u;
+}
+
+// The row (lane & 7) of the 8x8 block at (x0, y0), as the SAD passes use it.
+__device__ __forceinline__ void load_row(const uint8_t *cur, int w, int x0, int y0, uint32_t &s0, uint32_t &s1) {
+  const uint8_t *p = cur + (size_t)(y0 + (int)(__lane_id() & 7u)) * w + x0;
+  s0 = ld32(p);
+  s1 = ld32(p + 4);
 }
 
 // SAD of the block against ref at (x, y), summed over each slot's 8 lanes.
-__device__ __forceinline__ uint32_t slot_sad(const uint8_t *ref, int w, int x, int y, const BlkSrc &b) {
+__device__ __forceinline__ uint32_t slot_sad(const uint8_t *ref, int w, int x, int y, uint32_t s0, uint32_t s1) {
   uint32_t lo, hi;
   row8(ref, w, x, y + (int)(__lane_id() & 7u), lo, hi);
-  return red8(__builtin_amdgcn_sad_u8(b.s0, lo, __builtin_amdgcn_sad_u8(b.s1, hi, 0u)));
+  return red8(__builtin_amdgcn_sad_u8(s0, lo, __builtin_amdgcn_sad_u8(s1, hi, 0u)));
 }
 
 __device__ __forceinline__ int nib(uint32_t table, int k) { return (int)((table >> (4 * k)) & 15u) - 2; }
@@ -182,6 +181,8 @@ __device__ __forceinline__ int xcd_remap(int l, int n) {
 
 // ------------------------------------------------------------- complexity
 
+// One wave = 8 consecutive blocks (raster order): one intra pass for all
+// eight, the eight diamond searches in turn, one SATD pass for all eight.
 __global__ void __launch_bounds__(256) complexity_kernel(const CplxArgs A) {
   const int gx = (int)gridDim.x;
   const int wg = xcd_remap((int)(blockIdx.x + blockIdx.y * gridDim.x), gx * (int)gridDim.y);
@@ -192,45 +193,55 @@ __global__ void __launch_bounds__(256) complexity_kernel(const CplxArgs A) {
   if (first >= nb) return;  // whole workgroup
   const uint8_t *cur = J.cur, *prev = J.prev;
   const int g = lane >> 3;
-  const int gdx = nib(DIA_DX, g), gdy = nib(DIA_DY, g);
+  const int b0 = first + wave * CPLX_BPW;
+  const int nblk = max(0, min(CPLX_BPW, nb - b0));
   uint64_t si = 0, sp = 0;
-  for (int i = 0; i < CPLX_BPW; i++) {
-    const int blk = first + wave * CPLX_BPW + i;
-    if (blk >= nb) break;
-    const int x0 = 8 * (blk % bw), y0 = 8 * (blk / bw);
-    BlkSrc b;
-    load_blk(cur, w, x0, y0, b);
-    const uint32_t ic = intra_cost(cur, w, x0, y0, b);
+  if (nblk > 0) {
+    // the lane's block (lanes past the last block repeat block b0)
+    const int lb = g < nblk ? b0 + g : b0;
+    const int lx0 = 8 * (lb % bw), ly0 = 8 * (lb / bw);
+    uint32_t S[8];
+    const uint32_t ic = intra8(cur, w, lx0, ly0, S);
     uint32_t pc = ic;
-    int mx = 0, my = 0;
+    int lmx = 0, lmy = 0;  // the lane's block's MV
     if (prev) {
-      uint32_t bc = rdlane(slot_sad(prev, w, x0, y0, b), 0);
-      for (int it = 0; it < 8; it++) {
-        const int cx = mx + gdx, cy = my + gdy;
-        const bool ok = x0 + cx >= 0 && y0 + cy >= 0 && x0 + cx + 8 <= w && y0 + cy + 8 <= h;
-        const uint32_t c = slot_sad(prev, w, x0 + (ok ? cx : mx), y0 + (ok ? cy : my), b);
-        const uint32_t key = min_key(c, ok, g);
-        if (key == 0xFFFFFFFFu || (key >> 3) >= bc) break;
-        bc = key >> 3;
-        mx += nib(DIA_DX, (int)(key & 7u));
-        my += nib(DIA_DY, (int)(key & 7u));
+      const int gdx = nib(DIA_DX, g), gdy = nib(DIA_DY, g);
+      for (int b = 0; b < nblk; b++) {
+        const int x0 = 8 * ((b0 + b) % bw), y0 = 8 * ((b0 + b) / bw);
+        uint32_t s0, s1;
+        load_row(cur, w, x0, y0, s0, s1);
+        int mx = 0, my = 0;
+        uint32_t bc = rdlane(slot_sad(prev, w, x0, y0, s0, s1), 0);
+        for (int it = 0; it < 8; it++) {
+          const int cx = mx + gdx, cy = my + gdy;
+          const bool ok = x0 + cx >= 0 && y0 + cy >= 0 && x0 + cx + 8 <= w && y0 + cy + 8 <= h;
+          const uint32_t c = slot_sad(prev, w, x0 + (ok ? cx : mx), y0 + (ok ? cy : my), s0, s1);
+          const uint32_t key = min_key(c, ok, g);
+          if (key == 0xFFFFFFFFu || (key >> 3) >= bc) break;
+          bc = key >> 3;
+          mx += nib(DIA_DX, (int)(key & 7u));
+          my += nib(DIA_DY, (int)(key & 7u));
+        }
+        lmx = g == b ? mx : lmx;
+        lmy = g == b ? my : lmy;
       }
       uint32_t P[8];
       const int strip = lane & 1;
 #pragma unroll
-      for (int r = 0; r < 4; r++) row8(prev, w, x0 + mx, y0 + my + 4 * strip + r, P[2 * r], P[2 * r + 1]);
-      const uint32_t sc = rdlane(pair_sum(strip_satd(b.S, P)), 0) + 8u;
-      pc = min(sc, ic);
+      for (int r = 0; r < 4; r++) row8(prev, w, lx0 + lmx, ly0 + lmy + 4 * strip + r, P[2 * r], P[2 * r + 1]);
+      pc = min(pair_sum(strip_satd(S, P)) + 8u, ic);
     }
-    si += ic;
-    sp += pc;
-    if (J.blocks && lane == 0) {
+    for (int b = 0; b < nblk; b++) {
+      si += rdlane(ic, 8 * b);
+      sp += rdlane(pc, 8 * b);
+    }
+    if (J.blocks && (lane & 7) == 0 && g < nblk) {
       pcme_cplx_block o;
       o.intra = ic;
       o.inter = pc;
-      o.mv.x = (int16_t)mx;
-      o.mv.y = (int16_t)my;
-      J.blocks[blk] = o;
+      o.mv.x = (int16_t)lmx;
+      o.mv.y = (int16_t)lmy;
+      J.blocks[b0 + g] = o;
     }
   }
   __shared__ unsigned long long red[2][4];
@@ -256,12 +267,12 @@ __global__ void __launch_bounds__(2 * LA_CHUNK) complexity_clear_kernel(const Cp
 // mbtree.rs:51-102 search: SAD + |mx| + |my|; (0, 0), then the seed, then
 // up to 16 rounds of the pattern.  Group 0 prices (0, 0) and group 1 the
 // seed in one pass.
-__device__ __forceinline__ void tree_search(const uint8_t *ref, int w, int h, int x0, int y0, const BlkSrc &b, int sx,
-                                            int sy, int &mx, int &my) {
+__device__ __forceinline__ void tree_search(const uint8_t *ref, int w, int h, int x0, int y0, uint32_t s0, uint32_t s1,
+                                            int sx, int sy, int &mx, int &my) {
   const int lane = (int)__lane_id(), g = lane >> 3;
   const bool seed_ok = x0 + sx >= 0 && y0 + sy >= 0 && x0 + sx <= w - 8 && y0 + sy <= h - 8;
   const bool use_seed = g == 1 && seed_ok;
-  const uint32_t c0 = slot_sad(ref, w, x0 + (use_seed ? sx : 0), y0 + (use_seed ? sy : 0), b);
+  const uint32_t c0 = slot_sad(ref, w, x0 + (use_seed ? sx : 0), y0 + (use_seed ? sy : 0), s0, s1);
   uint32_t bc = rdlane(c0, 0);
   mx = 0;
   my = 0;
@@ -277,7 +288,7 @@ __device__ __forceinline__ void tree_search(const uint8_t *ref, int w, int h, in
   for (int it = 0; it < 16; it++) {
     const int cx = mx + gdx, cy = my + gdy;
     const bool ok = x0 + cx >= 0 && y0 + cy >= 0 && x0 + cx <= w - 8 && y0 + cy <= h - 8;
-    const uint32_t c = slot_sad(ref, w, x0 + (ok ? cx : mx), y0 + (ok ? cy : my), b) + (uint32_t)(abs(cx) + abs(cy));
+    const uint32_t c = slot_sad(ref, w, x0 + (ok ? cx : mx), y0 + (ok ? cy : my), s0, s1) + (uint32_t)(abs(cx) + abs(cy));
     const uint32_t key = min_key(c, ok, g);
     if (key == 0xFFFFFFFFu || (key >> 3) >= bc) break;
     bc = key >> 3;
@@ -286,6 +297,9 @@ __device__ __forceinline__ void tree_search(const uint8_t *ref, int w, int h, in
   }
 }
 
+// One wave = one block row, in batches of 8 blocks: one intra pass, the
+// searches block by block (each seeded with its left neighbour's MVs),
+// one SATD pass pricing list 0, list 1 and bi for the 8 blocks.
 __global__ void __launch_bounds__(256) tree_kernel(const TreeArgs A) {
   const int gx = (int)gridDim.x;
   const int wg = xcd_remap((int)(blockIdx.x + blockIdx.y * gridDim.x), gx * (int)gridDim.y);
@@ -293,87 +307,91 @@ __global__ void __launch_bounds__(256) tree_kernel(const TreeArgs A) {
   const int w = J.w, h = J.h, bw = w >> 3, bh = h >> 3;
   const int by = (wg % gx) * 4 + (int)(threadIdx.x >> 6);
   if (by >= bh) return;
-  const int lane = (int)__lane_id();
+  const int lane = (int)__lane_id(), g = lane >> 3;
   const uint8_t *cur = J.cur, *r0 = J.ref0, *r1 = J.ref1;
   const int kind = (lane >> 1) & 3, strip = lane & 1;
   int l0x = 0, l0y = 0, l1x = 0, l1y = 0;  // left neighbour's MVs (mbtree.rs:124, 153)
   const int y0 = 8 * by;
-  for (int bx = 0; bx < bw; bx++) {
-    const int x0 = 8 * bx;
-    BlkSrc b;
-    load_blk(cur, w, x0, y0, b);
-    const uint32_t intra = intra_cost(cur, w, x0, y0, b);
-    int m0x = 0, m0y = 0, m1x = 0, m1y = 0;
-    if (r0) tree_search(r0, w, h, x0, y0, b, l0x, l0y, m0x, m0y);
-    if (r1) tree_search(r1, w, h, x0, y0, b, l1x, l1y, m1x, m1y);
+  for (int bx0 = 0; bx0 < bw; bx0 += 8) {
+    const int nblk = min(8, bw - bx0);
+    const int lx0 = 8 * (bx0 + (g < nblk ? g : 0));
+    uint32_t S[8];
+    const uint32_t intra = intra8(cur, w, lx0, y0, S);
+    int m0x = 0, m0y = 0, m1x = 0, m1y = 0;  // the lane's block's MVs
+    for (int b = 0; b < nblk; b++) {
+      const int x0 = 8 * (bx0 + b);
+      uint32_t s0, s1;
+      load_row(cur, w, x0, y0, s0, s1);
+      int ax = 0, ay = 0, cx = 0, cy = 0;
+      if (r0) tree_search(r0, w, h, x0, y0, s0, s1, l0x, l0y, ax, ay);
+      if (r1) tree_search(r1, w, h, x0, y0, s0, s1, l1x, l1y, cx, cy);
+      l0x = ax;
+      l0y = ay;
+      l1x = cx;
+      l1y = cy;
+      const bool mine = g == b;
+      m0x = mine ? ax : m0x;
+      m0y = mine ? ay : m0y;
+      m1x = mine ? cx : m1x;
+      m1y = mine ? cy : m1y;
+    }
     // kinds: 0 = list 0, 1 = list 1, 2 = bi, 3 = unused (a missing list
-    // reads cur, its cost is never used)
+    // reads cur at the block, its cost is never used)
     const uint8_t *ra = (kind == 1) ? (r1 ? r1 : cur) : (r0 ? r0 : cur);
-    const int ax = (kind == 1) ? (r1 ? m1x : 0) : (r0 ? m0x : 0);
-    const int ay = (kind == 1) ? (r1 ? m1y : 0) : (r0 ? m0y : 0);
+    const int ax = (kind == 1) ? m1x : m0x, ay = (kind == 1) ? m1y : m0y;
     uint32_t P[8];
 #pragma unroll
-    for (int r = 0; r < 4; r++) row8(ra, w, x0 + ax, y0 + ay + 4 * strip + r, P[2 * r], P[2 * r + 1]);
+    for (int r = 0; r < 4; r++) row8(ra, w, lx0 + ax, y0 + ay + 4 * strip + r, P[2 * r], P[2 * r + 1]);
     if (kind == 2) {
       const uint8_t *rb = r1 ? r1 : cur;
-      const int bxm = r1 ? m1x : 0, bym = r1 ? m1y : 0;
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         uint32_t q0, q1;
-        row8(rb, w, x0 + bxm, y0 + bym + 4 * strip + r, q0, q1);
+        row8(rb, w, lx0 + m1x, y0 + m1y + 4 * strip + r, q0, q1);
         P[2 * r] = avg_round(P[2 * r], q0);
         P[2 * r + 1] = avg_round(P[2 * r + 1], q1);
       }
     }
-    const uint32_t s = pair_sum(strip_satd(b.S, P));
-    uint32_t best = intra, list = 0;
-    int o0x = 0, o0y = 0, o1x = 0, o1y = 0;
-    if (r0) {
-      const uint32_t c = rdlane(s, 0) + 8u;
-      if (c < best) {
-        best = c;
-        list = 1;
-        o0x = m0x;
-        o0y = m0y;
+    const uint32_t s = pair_sum(strip_satd(S, P));
+    for (int b = 0; b < nblk; b++) {
+      uint32_t best = rdlane(intra, 8 * b), list = 0;
+      const uint32_t bm0 = rdlane(((uint32_t)m0x & 0xFFFFu) | ((uint32_t)m0y << 16), 8 * b);
+      const uint32_t bm1 = rdlane(((uint32_t)m1x & 0xFFFFu) | ((uint32_t)m1y << 16), 8 * b);
+      uint32_t o0 = 0, o1 = 0;
+      if (r0) {
+        const uint32_t c = rdlane(s, 8 * b) + 8u;
+        if (c < best) {
+          best = c;
+          list = 1;
+          o0 = bm0;
+        }
       }
-    }
-    if (r1) {
-      const uint32_t c = rdlane(s, 2) + 8u;
-      if (c < best) {
-        best = c;
-        list = 2;
-        o0x = m0x;
-        o0y = m0y;
-        o1x = m1x;
-        o1y = m1y;
+      if (r1) {
+        const uint32_t c = rdlane(s, 8 * b + 2) + 8u;
+        if (c < best) {
+          best = c;
+          list = 2;
+          o0 = bm0;
+          o1 = bm1;
+        }
       }
-    }
-    if (r0 && r1) {
-      const uint32_t c = rdlane(s, 4) + 12u;
-      if (c < best) {
-        best = c;
-        list = 3;
-        o0x = m0x;
-        o0y = m0y;
-        o1x = m1x;
-        o1y = m1y;
+      if (r0 && r1) {
+        const uint32_t c = rdlane(s, 8 * b + 4) + 12u;
+        if (c < best) {
+          best = c;
+          list = 3;
+          o0 = bm0;
+          o1 = bm1;
+        }
       }
-    }
-    l0x = m0x;
-    l0y = m0y;
-    l1x = m1x;
-    l1y = m1y;
-    if (lane == 0) {
-      pcme_tree_block o;
-      o.intra = intra;
-      o.inter = best;
-      o.mv[0].x = (int16_t)o0x;
-      o.mv[0].y = (int16_t)o0y;
-      o.mv[1].x = (int16_t)o1x;
-      o.mv[1].y = (int16_t)o1y;
-      o.list = (uint8_t)list;
-      o.reserved[0] = o.reserved[1] = o.reserved[2] = 0;
-      J.blocks[by * bw + bx] = o;
+      if (lane == 0) {
+        uint32_t *o = (uint32_t *)&J.blocks[by * bw + bx0 + b];
+        o[0] = rdlane(intra, 8 * b);
+        o[1] = best;
+        o[2] = o0;
+        o[3] = o1;
+        o[4] = list;
+      }
     }
   }
 }
