@@ -230,6 +230,55 @@ typedef struct pcme_search_job {
 int pcme_search_batch(const pcme_search_job *jobs, int njobs,
                       pcme_mv *out_mv, uint32_t *out_cost, void *stream);
 
+/* One neighbour 4x4 block as nb_part reads it (slice.rs:392-432) for the
+ * searched list: ref -2 = not available (outside the picture, or the
+ * top-right MB past the right edge), -1 = available without a reference of
+ * that list (an intra MB reads as ref -1, MV 0), >= 0 = reference index,
+ * with its MV.  8 bytes. */
+typedef struct pcme_nb_block {
+  pcme_mv mv;
+  int8_t ref;
+  uint8_t reserved[3];
+} pcme_nb_block;
+
+/* One search_part call (slice.rs:1810-1868): the partitions of `part` of
+ * the MB at (mx, my) searched in order, each with its predictor from the
+ * neighbours and the partitions already searched, candidates [seed,
+ * mvs[0]].  The neighbour blocks are those of the list being searched, so
+ * the same call serves list 0 and list 1 (analyse_b, slice.rs:2277-2289,
+ * b_partition slice.rs:2100-2102).  168 bytes. */
+typedef struct pcme_part_job {
+  const uint8_t *src;      /* (device) top-left of the 16x16 source MB */
+  int32_t sstride;
+  int32_t part;            /* PCME_P16x16 .. PCME_P8x8 */
+  const uint8_t *ref;      /* (device) padded planes full, H, V, C */
+  const uint8_t *ref_h;
+  const uint8_t *ref_v;
+  const uint8_t *ref_c;
+  int32_t mbw, mbh, mx, my;
+  pcme_mv seed;
+  uint32_t lam;
+  int32_t me_range, subpel_rounds;
+  int64_t vlim;            /* SliceEnc::vlim for the MB's row */
+  pcme_nb_block left[4];   /* left MB, 4x4 blocks 3, 7, 11, 15 */
+  pcme_nb_block top[4];    /* top MB, blocks 12, 13, 14, 15 */
+  pcme_nb_block top_right; /* top-right MB, block 12 */
+  pcme_nb_block top_left;  /* top-left MB, block 15 */
+} pcme_part_job;
+
+/* search_part's result: ([Mv; 4] mvs, [Mv; 4] mvps, u32 total), unused
+ * partitions zero.  36 bytes. */
+typedef struct pcme_part_out {
+  pcme_mv mvs[4];
+  pcme_mv mvps[4];
+  uint32_t total;
+} pcme_part_out;
+
+/* Independent search_part calls, one wave each; out[i] (device) gets the
+ * result of jobs[i] (host array). */
+int pcme_search_part_batch(const pcme_part_job *jobs, int njobs,
+                           pcme_part_out *out, void *stream);
+
 /* Motion-compensated prediction of the chosen partitions (pred_block,
  * slice.rs:1725-1749 = mc_luma picture.rs:415-449 + mc_chroma
  * picture.rs:451-487) for every MB of a searched frame.  ref planes and

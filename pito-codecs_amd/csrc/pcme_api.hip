@@ -45,6 +45,8 @@ struct Dev {
   int resident_waves[2] = {0, 0};  // full-search / hinted kernel
   pcme_search_job *dsj = nullptr;
   size_t dsj_cap = 0;
+  pcme_part_job *dpj = nullptr;     // per-call search_part jobs
+  size_t dpj_cap = 0;
   unsigned char *hstage = nullptr;  // pinned upload staging
   size_t hstage_cap = 0;
   hipEvent_t stage_ev = nullptr;
@@ -510,6 +512,46 @@ int pcme_search_batch(const pcme_search_job *jobs, int njobs, pcme_mv *out_mv, u
   if ((rc = upload(dv, dv->dsj, jobs, sizeof(pcme_search_job) * (size_t)njobs, (hipStream_t)stream))) return rc;
   hipLaunchKernelGGL(search_batch_kernel, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream, dv->dsj, njobs,
                      out_mv, out_cost);
+  HIPCHK(hipGetLastError());
+  return PCME_OK;
+}
+
+static bool nb_ok(const pcme_nb_block &b) {
+  const int ax = b.mv.x < 0 ? -b.mv.x : b.mv.x, ay = b.mv.y < 0 ? -b.mv.y : b.mv.y;
+  return b.ref >= -2 && ax < 4096 && ay < 4096;
+}
+
+int pcme_search_part_batch(const pcme_part_job *jobs, int njobs, pcme_part_out *out, void *stream) {
+  if (njobs < 0 || (njobs > 0 && (!jobs || !out)) || ((uintptr_t)out & 3u))
+    return fail(PCME_EINVAL, "pcme_search_part_batch: bad arguments");
+  for (int i = 0; i < njobs; i++) {
+    const pcme_part_job &J = jobs[i];
+    if (J.part < PCME_P16x16 || J.part > PCME_P8x8) return fail(PCME_EINVAL, "job %d: part %d", i, J.part);
+    if (!J.src || !J.ref || !J.ref_h || !J.ref_v || !J.ref_c) return fail(PCME_EINVAL, "job %d has a null pointer", i);
+    if ((((uintptr_t)J.ref | (uintptr_t)J.ref_h | (uintptr_t)J.ref_v | (uintptr_t)J.ref_c) & 3u) != 0)
+      return fail(PCME_EINVAL, "job %d: reference planes must be 4-byte aligned", i);
+    if (((uintptr_t)J.src & 3u) || (J.sstride & 3) || J.sstride < 16)
+      return fail(PCME_EINVAL, "job %d: src must be 4-byte aligned with a stride multiple of 4", i);
+    if (J.mbw < 1 || J.mbh < 1 || J.mx < 0 || J.my < 0 || J.mx >= J.mbw || J.my >= J.mbh)
+      return fail(PCME_EINVAL, "job %d: MB outside the picture", i);
+    if (J.me_range < 0 || J.me_range > 1024 || J.subpel_rounds < 0 || J.subpel_rounds > 8)
+      return fail(PCME_EINVAL, "job %d: me_range/subpel_rounds out of range", i);
+    if (J.vlim < 16LL * J.my + 17 || J.vlim > 16LL * J.mbh + 44)
+      return fail(PCME_EINVAL, "job %d: vlim %lld outside [16*my+17, 16*mbh+44]", i, (long long)J.vlim);
+    pcme_nb_block sb = {J.seed, 0, {0, 0, 0}};
+    bool ok = nb_ok(sb) && nb_ok(J.top_right) && nb_ok(J.top_left);
+    for (int k = 0; k < 4; k++) ok = ok && nb_ok(J.left[k]) && nb_ok(J.top[k]);
+    if (!ok) return fail(PCME_EINVAL, "job %d: a neighbour ref < -2 or an MV component beyond +-4095", i);
+  }
+  if (njobs == 0) return PCME_OK;
+  Dev *dv;
+  int rc = get_dev(&dv);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(dv->mu);
+  if ((rc = grow(&dv->dpj, &dv->dpj_cap, (size_t)njobs))) return rc;
+  if ((rc = upload(dv, dv->dpj, jobs, sizeof(pcme_part_job) * (size_t)njobs, (hipStream_t)stream))) return rc;
+  hipLaunchKernelGGL(search_part_batch_kernel, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream, dv->dpj,
+                     njobs, out);
   HIPCHK(hipGetLastError());
   return PCME_OK;
 }

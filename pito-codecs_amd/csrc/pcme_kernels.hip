@@ -828,6 +828,15 @@ struct NbRes {
   uint32_t mv;
 };
 
+// Neighbour state given explicitly, block by block (the per-call
+// search_part): ref -2 = unavailable, -1 = available without a reference
+// of the searched list (an intra neighbour reads as (-1, 0)), >= 0 the
+// reference index, with its MV.
+struct NbrG {
+  int rL[4], rT[4], rTR, rTL;
+  uint32_t L[4], T[4], TR, TL;
+};
+
 // Partitions already searched in the current mode (search_part's `info`).
 struct Cur {
   uint32_t mv[4];
@@ -868,6 +877,29 @@ __device__ __forceinline__ NbRes nb(const Nbr &n, const Cur &cur) {
   return r;
 }
 
+template <int PART, int P, int BX, int BY>
+__device__ __forceinline__ NbRes nb(const NbrG &n, const Cur &cur) {
+  NbRes r = {false, -1, 0u};
+  if constexpr (BX >= 0 && BX < 4 && BY >= 0 && BY < 4) {
+    constexpr int k = part_of(PART, BY * 4 + BX);
+    if constexpr (k < P) {
+      r.avail = true; r.ref = 0; r.mv = cur.mv[k];
+    }
+  } else if constexpr (BY >= 4 || (BY >= 0 && BX >= 4)) {
+  } else if constexpr (BY < 0) {
+    if constexpr (BX < 0) {
+      r.avail = n.rTL >= -1; r.ref = n.rTL; r.mv = n.TL;
+    } else if constexpr (BX < 4) {
+      r.avail = n.rT[BX] >= -1; r.ref = n.rT[BX]; r.mv = n.T[BX];
+    } else {
+      r.avail = n.rTR >= -1; r.ref = n.rTR; r.mv = n.TR;
+    }
+  } else {
+    r.avail = n.rL[BY] >= -1; r.ref = n.rL[BY]; r.mv = n.L[BY];
+  }
+  return r;
+}
+
 __device__ __forceinline__ int median3(int a, int b, int c) {
   int mx = a > b ? a : b, mn = a < b ? a : b;
   int t = mn > c ? mn : c;
@@ -876,8 +908,8 @@ __device__ __forceinline__ int median3(int a, int b, int c) {
 
 // mvp (slice.rs:434-482) with rf = 0.  SHAPE: 0 Normal, 1 Top16x8,
 // 2 Bottom16x8, 3 Left8x16, 4 Right8x16.  Returns a packed MV.
-template <int PART, int P, int X, int Y, int WB, int SHAPE>
-__device__ __forceinline__ uint32_t mvp(const Nbr &n, const Cur &cur) {
+template <int PART, int P, int X, int Y, int WB, int SHAPE, class N>
+__device__ __forceinline__ uint32_t mvp(const N &n, const Cur &cur) {
   NbRes a = nb<PART, P, X - 1, Y>(n, cur);
   NbRes b = nb<PART, P, X, Y - 1>(n, cur);
   NbRes cc = nb<PART, P, X + WB, Y - 1>(n, cur);
@@ -918,8 +950,8 @@ struct PartRes {
 };
 
 // One partition of search_part (slice.rs:1826-1866), then the next.
-template <int PART, int P, bool PR>
-__device__ __forceinline__ void part_step(const SCtx &c, const Nbr &n, Cur &cur, const uint8_t *src,
+template <int PART, int P, bool PR, class N>
+__device__ __forceinline__ void part_step(const SCtx &c, const N &n, Cur &cur, const uint8_t *src,
                                           int sstride, int mbx, int mby, uint32_t seed, uint32_t hv,
                                           int nh, PartRes &res, uint32_t *lds) {
   if constexpr (P < part_count(PART)) {
@@ -942,8 +974,8 @@ __device__ __forceinline__ void part_step(const SCtx &c, const Nbr &n, Cur &cur,
 }
 
 // search_part (slice.rs:1810-1868), list 0.  src = MB top-left.
-template <int PART, bool PR>
-__device__ __forceinline__ void search_part(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
+template <int PART, bool PR, class N>
+__device__ __forceinline__ void search_part(const SCtx &c, const N &n, const uint8_t *src, int sstride,
                                             int mbx, int mby, uint32_t seed, uint32_t hv, int nh,
                                             PartRes &res, uint32_t *lds) {
   Cur cur = {};
@@ -1322,6 +1354,57 @@ search_frames_hinted_kernel(const DevJob *__restrict__ jobs, const int2 *__restr
                             uint32_t *ctl, uint32_t spin_limit) {
   __shared__ __attribute__((aligned(16))) uint32_t s_lds[LDS_PER_WAVE / 4];
   frames_body<true>(jobs, tickets, ntickets, ctl, spin_limit, s_lds);
+}
+
+// -------------------------------------------------- per-call search_part
+
+static_assert(sizeof(pcme_nb_block) == 8 && sizeof(pcme_part_job) == 168 && sizeof(pcme_part_out) == 36,
+              "pcme_part_job / pcme_part_out layout");
+
+__device__ __forceinline__ uint32_t nb_mv(const pcme_nb_block &b) { return pack_mv(b.mv.x, b.mv.y); }
+
+__global__ void __launch_bounds__(64) search_part_batch_kernel(const pcme_part_job *__restrict__ jobs,
+                                                               int njobs, pcme_part_out *out) {
+  __shared__ __attribute__((aligned(16))) uint32_t s_lds[LDS_PER_WAVE / 4];
+  const int j = blockIdx.x;
+  if (j >= njobs) return;
+  const pcme_part_job &J = jobs[j];
+  SCtx c;
+  c.pl[0] = J.ref; c.pl[1] = J.ref_h; c.pl[2] = J.ref_v; c.pl[3] = J.ref_c;
+  c.stride = 16 * J.mbw + 2 * PCME_PAD;
+  c.pw = 16 * J.mbw;
+  c.ph = 16 * J.mbh;
+  c.vlim = (int)J.vlim;
+  c.lam = J.lam;
+  c.range = J.me_range;
+  c.subpel = J.subpel_rounds;
+  NbrG n;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    n.rL[i] = J.left[i].ref; n.L[i] = nb_mv(J.left[i]);
+    n.rT[i] = J.top[i].ref; n.T[i] = nb_mv(J.top[i]);
+  }
+  n.rTR = J.top_right.ref; n.TR = nb_mv(J.top_right);
+  n.rTL = J.top_left.ref; n.TL = nb_mv(J.top_left);
+  const uint32_t seed = pack_mv(J.seed.x, J.seed.y);
+  const int mbx = 16 * J.mx, mby = 16 * J.my;
+  PartRes r;
+  switch (J.part) {
+    case 0: search_part<0, false>(c, n, J.src, J.sstride, mbx, mby, seed, 0u, -1, r, s_lds); break;
+    case 1: search_part<1, false>(c, n, J.src, J.sstride, mbx, mby, seed, 0u, -1, r, s_lds); break;
+    case 2: search_part<2, false>(c, n, J.src, J.sstride, mbx, mby, seed, 0u, -1, r, s_lds); break;
+    default: search_part<3, false>(c, n, J.src, J.sstride, mbx, mby, seed, 0u, -1, r, s_lds); break;
+  }
+  const int lane = (int)lane_id();
+  if (lane < 9) {
+    uint32_t v = r.total;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      v = lane == i ? r.mv[i] : v;
+      v = lane == 4 + i ? r.mvp[i] : v;
+    }
+    ((uint32_t *)&out[j])[lane] = v;
+  }
 }
 
 // -------------------------------------------------- per-call search batch

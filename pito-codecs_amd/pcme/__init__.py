@@ -71,6 +71,24 @@ class SearchJob(ctypes.Structure):
                 ("subpel_rounds", ctypes.c_int32), ("vlim", ctypes.c_int64)]
 
 
+class NbBlock(ctypes.Structure):
+    """pcme_nb_block: ref -2 unavailable, -1 no reference of the list, >= 0 index."""
+    _fields_ = [("mv", Mv), ("ref", ctypes.c_int8), ("reserved", ctypes.c_uint8 * 3)]
+
+
+class PartJob(ctypes.Structure):
+    _fields_ = [("src", ctypes.c_void_p), ("sstride", ctypes.c_int32), ("part", ctypes.c_int32),
+                ("ref", ctypes.c_void_p), ("ref_h", ctypes.c_void_p),
+                ("ref_v", ctypes.c_void_p), ("ref_c", ctypes.c_void_p),
+                ("mbw", ctypes.c_int32), ("mbh", ctypes.c_int32), ("mx", ctypes.c_int32), ("my", ctypes.c_int32),
+                ("seed", Mv), ("lam", ctypes.c_uint32), ("me_range", ctypes.c_int32),
+                ("subpel_rounds", ctypes.c_int32), ("vlim", ctypes.c_int64),
+                ("left", NbBlock * 4), ("top", NbBlock * 4), ("top_right", NbBlock), ("top_left", NbBlock)]
+
+
+assert ctypes.sizeof(NbBlock) == 8 and ctypes.sizeof(PartJob) == 168
+
+
 class SynthCfg(ctypes.Structure):
     _fields_ = [("w", ctypes.c_int32), ("h", ctypes.c_int32),
                 ("frames", ctypes.c_int32), ("nrect", ctypes.c_int32),
@@ -89,6 +107,10 @@ class TreeJob(ctypes.Structure):
     _fields_ = [("cur", ctypes.c_void_p), ("ref0", ctypes.c_void_p), ("ref1", ctypes.c_void_p),
                 ("blocks", ctypes.c_void_p), ("w", ctypes.c_int32), ("h", ctypes.c_int32)]
 
+
+# pcme_part_out (36 bytes): search_part's (mvs, mvps, total)
+PART_OUT_DTYPE = np.dtype([("mvs", "<i2", (4, 2)), ("mvps", "<i2", (4, 2)), ("total", "<u4")])
+assert PART_OUT_DTYPE.itemsize == 36
 
 # pcme_cplx_block (12 bytes) and pcme_tree_block (20 bytes) as numpy records
 CPLX_BLOCK_DTYPE = np.dtype([("intra", "<u4"), ("inter", "<u4"), ("mv", "<i2", (2,))])
@@ -121,7 +143,7 @@ EXPORTS = (
     "pcme_plan_set_grid", "pcme_plan_autotune", "pcme_plan_destroy", "pcme_set_spin_limit",
     "pcme_net_create", "pcme_net_destroy", "pcme_net_get_params", "pcme_net_set_params",
     "pcme_net_inputs", "pcme_net_forward", "pcme_net_grad", "pcme_net_train_step", "pcme_net_hints",
-    "pcme_lowres", "pcme_complexity", "pcme_mbtree_analyse",
+    "pcme_lowres", "pcme_complexity", "pcme_mbtree_analyse", "pcme_search_part_batch",
 )
 
 _lib = None
@@ -156,6 +178,7 @@ def load() -> ctypes.CDLL:
         "pcme_search_frames": (c.c_int, [c.POINTER(FrameJob), c.c_int, vp]),
         "pcme_search_frames_sync": (c.c_int, [c.POINTER(FrameJob), c.c_int, vp]),
         "pcme_search_batch": (c.c_int, [c.POINTER(SearchJob), c.c_int, vp, vp, vp]),
+        "pcme_search_part_batch": (c.c_int, [c.POINTER(PartJob), c.c_int, vp, vp]),
         "pcme_pred_frame": (c.c_int, [vp, vp, vp, vp, vp, vp, vp, c.c_int32, c.c_int32, vp, vp, vp, vp]),
         "pcme_synth_clip": (c.c_int, [c.POINTER(SynthCfg), vp, vp, vp, c.c_int32]),
         "pcme_plan_create": (c.c_int, [c.POINTER(FrameJob), c.c_int, c.POINTER(vp)]),
@@ -388,6 +411,14 @@ def search_batch(jobs: list[SearchJob], out_mv_ptr: int, out_cost_ptr: int, stre
     lib = load()
     arr = (SearchJob * len(jobs))(*jobs)
     _check(lib.pcme_search_batch(arr, len(jobs), out_mv_ptr, out_cost_ptr, _stream_ptr(stream)))
+
+
+def search_part_batch(jobs: list[PartJob], out_ptr: int, stream=None) -> None:
+    """pcme_search_part_batch: out_ptr (device) receives len(jobs)
+    PART_OUT_DTYPE records."""
+    lib = load()
+    arr = (PartJob * len(jobs))(*jobs)
+    _check(lib.pcme_search_part_batch(arr, len(jobs), out_ptr, _stream_ptr(stream)))
 
 
 def pred_frame(clip: DeviceClip, ref: int, mb_ptr: int, pred_y_ptr: int, pred_u_ptr: int,

@@ -149,3 +149,26 @@ def test_lookahead_entry_points_check_arguments(pcme_lib):
         assert lib.pcme_complexity((C * 1)(C(64, None, None, 64, 8, 8)), 1, None) == pcme_lib.PCME_ENODEV
         assert lib.pcme_mbtree_analyse((T * 1)(T(64, None, None, 64, 8, 8)), 1, None) == pcme_lib.PCME_ENODEV
         assert lib.pcme_lowres(64, 0, 64, 0, 16, 16, 64, 0, 1, None) == pcme_lib.PCME_ENODEV
+
+
+def test_search_part_batch_checks_arguments(pcme_lib):
+    lib = pcme_lib.load()
+
+    def job(**kw):
+        j = pcme_lib.PartJob()
+        j.src, j.ref, j.ref_h, j.ref_v, j.ref_c = 64, 64, 64, 64, 64
+        j.sstride, j.mbw, j.mbh, j.me_range, j.subpel_rounds, j.vlim = 144, 4, 4, 16, 1, 4 * 16 + 44
+        for k, v in kw.items():
+            setattr(j, k, v)
+        return (pcme_lib.PartJob * 1)(j)
+
+    out = 64
+    for bad in (job(part=4), job(src=None), job(vlim=16), job(vlim=4 * 16 + 45), job(mx=4), job(sstride=146),
+                job(seed=pcme_lib.Mv(4096, 0))):
+        assert lib.pcme_search_part_batch(bad, 1, out, None) == pcme_lib.PCME_EINVAL
+    j = job()
+    j[0].left[2].ref = -3
+    assert lib.pcme_search_part_batch(j, 1, out, None) == pcme_lib.PCME_EINVAL
+    assert lib.pcme_search_part_batch(job(), 1, 66, None) == pcme_lib.PCME_EINVAL  # misaligned out
+    if pcme_lib.device_count() == 0:
+        assert lib.pcme_search_part_batch(job(), 1, out, None) == pcme_lib.PCME_ENODEV
