@@ -279,6 +279,56 @@ typedef struct pcme_part_out {
 int pcme_search_part_batch(const pcme_part_job *jobs, int njobs,
                            pcme_part_out *out, void *stream);
 
+/* The motion search of one B macroblock (analyse_b, slice.rs:2189-2377,
+ * and b_partition, slice.rs:2086-2187) without the parts that need the
+ * residual coder or RD (BSkip, rd_cost, intra).  nb[l] holds the
+ * neighbour blocks of list l in the order left[4], top[4], top_right,
+ * top_left (as pcme_part_job); col_* describe the co-located MB of the
+ * list-1 picture for direct_spatial (slice.rs:2057-2071).  328 bytes. */
+typedef struct pcme_b_job {
+  const uint8_t *src;      /* (device) top-left of the 16x16 source MB */
+  int32_t sstride;
+  int32_t reserved0;
+  const uint8_t *ref0[4];  /* (device) list-0 picture: full, H, V, C planes */
+  const uint8_t *ref1[4];  /* (device) list-1 picture */
+  int32_t mbw, mbh, mx, my;
+  uint32_t lam;
+  int32_t me_range, subpel_rounds, reserved1;
+  int64_t vlim;
+  pcme_nb_block nb[2][10];
+  pcme_mv col_mv[2][4];    /* co-located MB: MVs of its 4x4 blocks 0, 3, 12, 15 per list */
+  int8_t col_ref[2][4];    /* its ref_idx per 8x8 per list */
+  int32_t col_intra;       /* nonzero: the co-located MB is intra */
+  int32_t reserved2;
+} pcme_b_job;
+
+/* Result of one B macroblock's motion search.  direct: direct_spatial's
+ * refs and MVs per 8x8, whether every used MV is legal, and SATD of the
+ * direct prediction + (lam*2)>>8 (all ones when not legal).  16x16: the
+ * list-0/list-1 search_part results seeded with the direct MVs, the
+ * direction costs (list costs + (lam*3)>>8; bi SATD + lam*(bits+5)>>8) and
+ * the first least one.  part_*: b_partition of 16x8 [0] and 8x16 [1]
+ * seeded with the 16x16 MVs: MVs [mode][list][partition], per-partition
+ * directions and the total (lam*8)>>8 + sum of the best.  136 bytes. */
+typedef struct pcme_b_out {
+  pcme_mv direct_mv[2][4];
+  int8_t direct_ref[2][4];
+  uint32_t direct_ok, direct_cost;
+  pcme_mv mv16[2], mvp16[2];
+  uint32_t cost16[2];
+  uint32_t cost[3];
+  uint32_t dir;
+  pcme_mv part_mv[2][2][2];
+  uint32_t part_cost[2];
+  uint8_t part_dir[2][2];
+  uint8_t reserved[4];
+} pcme_b_out;
+
+/* Independent B macroblocks, one wave each; out[i] (device) gets the
+ * result of jobs[i] (host array). */
+int pcme_b_analyse_batch(const pcme_b_job *jobs, int njobs, pcme_b_out *out,
+                         void *stream);
+
 /* Motion-compensated prediction of the chosen partitions (pred_block,
  * slice.rs:1725-1749 = mc_luma picture.rs:415-449 + mc_chroma
  * picture.rs:451-487) for every MB of a searched frame.  ref planes and

@@ -29,6 +29,10 @@ class ombinfo(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int8), ("ref_idx", ctypes.c_int8 * 4), ("mv", omv * 16)]
 
 
+class ombinfo2(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int8), ("ref_idx", (ctypes.c_int8 * 4) * 2), ("mv", (omv * 16) * 2)]
+
+
 class oref(ctypes.Structure):
     _fields_ = [("pl", ctypes.c_void_p * 4), ("stride", ctypes.c_int), ("pad", ctypes.c_int)]
 
@@ -60,6 +64,15 @@ OMB_DTYPE = np.dtype([
 # otree_block has the layout of pcme_tree_block (mbtree.rs Block)
 OTREE_DTYPE = np.dtype([("intra", "<u4"), ("inter", "<u4"), ("mv", "<i2", (2, 2)), ("list", "u1"),
                         ("reserved", "u1", (3,))])
+
+# ob_out has the layout of pcme_b_out
+OB_DTYPE = np.dtype([
+    ("direct_mv", "<i2", (2, 4, 2)), ("direct_ref", "i1", (2, 4)), ("direct_ok", "<u4"), ("direct_cost", "<u4"),
+    ("mv16", "<i2", (2, 2)), ("mvp16", "<i2", (2, 2)), ("cost16", "<u4", (2,)), ("cost", "<u4", (3,)),
+    ("dir", "<u4"), ("part_mv", "<i2", (2, 2, 2, 2)), ("part_cost", "<u4", (2,)), ("part_dir", "u1", (2, 2)),
+    ("reserved", "u1", (4,)),
+])
+assert OB_DTYPE.itemsize == 136
 
 # ohint has the layout of pcme_hint
 OHINT_DTYPE = np.dtype([
@@ -151,6 +164,11 @@ def load() -> ctypes.CDLL:
     lib.o_tree_search.argtypes = [P, P, c.c_int, c.c_int, c.c_int, c.c_int, omv]
     lib.o_tree_analyse.restype = None
     lib.o_tree_analyse.argtypes = [P, P, P, c.c_int, c.c_int, P]
+    lib.o_direct_spatial.restype = None
+    lib.o_direct_spatial.argtypes = [c.POINTER(oparams), P, P, c.c_int, c.c_int, P, P]
+    lib.o_b_analyse.restype = None
+    lib.o_b_analyse.argtypes = [c.POINTER(oparams), P, P, c.c_int, c.c_int, c.POINTER(oref), c.POINTER(oref), P,
+                                c.c_uint32, P]
     lib.o_lookahead_mt.restype = c.c_double
     lib.o_lookahead_mt.argtypes = [P, c.c_int, c.c_int, c.c_int, c.c_int, P, P]
     _lib = lib
@@ -321,3 +339,24 @@ def lookahead_mt(lr: np.ndarray, threads: int):
     blocks = np.zeros((n, (w // 8) * (h // 8)), OTREE_DTYPE)
     sec = load().o_lookahead_mt(_p(a), w, h, n, threads, _p(sums), _p(blocks))
     return sec, sums, blocks
+
+
+# ------------------------------------------------------- B macroblocks (slice.rs)
+
+def direct_spatial(prm: oparams, mbs, col: ombinfo2, mx: int, my: int):
+    """direct_spatial -> (refs int8[2, 4], mvs int16[2, 4, 2])."""
+    refs = np.zeros((2, 4), np.int8)
+    mvs = np.zeros((2, 4, 2), np.int16)
+    load().o_direct_spatial(ctypes.byref(prm), ctypes.cast(mbs, ctypes.c_void_p), ctypes.addressof(col), mx, my,
+                            _p(refs), _p(mvs))
+    return refs, mvs
+
+
+def b_analyse(prm: oparams, mbs, col: ombinfo2, mx: int, my: int, r0: Ref, r1: Ref, src16: np.ndarray,
+              lam: int) -> np.ndarray:
+    """The motion search of analyse_b for one MB -> one OB_DTYPE record."""
+    out = np.zeros(1, OB_DTYPE)
+    s = np.ascontiguousarray(src16, np.uint8)
+    load().o_b_analyse(ctypes.byref(prm), ctypes.cast(mbs, ctypes.c_void_p), ctypes.addressof(col), mx, my,
+                       ctypes.byref(r0.view), ctypes.byref(r1.view), _p(s), lam, _p(out))
+    return out[0]

@@ -1061,3 +1061,174 @@ double o_lookahead_mt(const uint8_t *lr, int w, int h, int n, int threads, uint6
   clock_gettime(CLOCK_MONOTONIC, &t1);
   return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 }
+
+/* ------------------------------------------------------ B macroblocks (C7) */
+
+/* The list-l view of a two-list MbInfo array: nb_part (slice.rs:392-432)
+ * reads only kind, ref_idx[l] and mv[l], so the list-0 functions above run
+ * any list on its projection. */
+static void project_list(const ombinfo2 *mbs, int n, int l, ombinfo *out) {
+  for (int i = 0; i < n; i++) {
+    out[i].kind = mbs[i].kind;
+    memcpy(out[i].ref_idx, mbs[i].ref_idx[l], 4);
+    memcpy(out[i].mv, mbs[i].mv[l], sizeof out[i].mv);
+  }
+}
+
+static int8_t minp(int8_t x, int8_t y) { return (x >= 0 && y >= 0) ? (x < y ? x : y) : (x > y ? x : y); }
+
+/* slice.rs:2030-2084 direct_spatial; col = the list-1 picture's MbInfo at
+ * (mx, my). */
+void o_direct_spatial(const oparams *p, const ombinfo2 *mbs, const ombinfo2 *col, int mx, int my,
+                      int8_t out_ref[2][4], omv out_mv[2][4]) {
+  const int n = p->mbw * p->mbh;
+  ombinfo *proj = (ombinfo *)malloc(sizeof(ombinfo) * (size_t)n);
+  ombinfo cur;
+  memset(&cur, 0, sizeof cur);
+  for (int i = 0; i < 4; i++) cur.ref_idx[i] = -1;
+  int8_t refs[2] = {-1, -1};
+  omv mvp[2] = {{0, 0}, {0, 0}};
+  for (int l = 0; l < 2; l++) {
+    project_list(mbs, n, l, proj);
+    int8_t a = -1, b = -1, c = -1, r;
+    omv m;
+    if (o_nb_part(p, proj, &cur, 0, mx, my, -1, 0, &r, &m)) a = r;
+    if (o_nb_part(p, proj, &cur, 0, mx, my, 0, -1, &r, &m)) b = r;
+    if (o_nb_part(p, proj, &cur, 0, mx, my, 4, -1, &r, &m)) c = r;
+    else if (o_nb_part(p, proj, &cur, 0, mx, my, -1, -1, &r, &m)) c = r;
+    refs[l] = minp(a, minp(b, c));
+  }
+  const int zero = refs[0] < 0 && refs[1] < 0;
+  if (zero) {
+    refs[0] = refs[1] = 0;
+  } else {
+    for (int l = 0; l < 2; l++)
+      if (refs[l] >= 0) {
+        project_list(mbs, n, l, proj);
+        mvp[l] = o_mvp(p, proj, &cur, 0, mx, my, 0, 0, 4, refs[l], 0);
+      }
+  }
+  free(proj);
+  static const int corner[4] = {0, 3, 12, 15};
+  for (int q = 0; q < 4; q++) {
+    int col_zero = 0;
+    if (!is_intra(col->kind)) {
+      int8_t rc;
+      omv mvc;
+      if (col->ref_idx[0][q] >= 0) {
+        rc = col->ref_idx[0][q];
+        mvc = col->mv[0][corner[q]];
+      } else {
+        rc = col->ref_idx[1][q];
+        mvc = col->mv[1][corner[q]];
+      }
+      col_zero = rc == 0 && abs(mvc.x) <= 1 && abs(mvc.y) <= 1;
+    }
+    for (int l = 0; l < 2; l++) {
+      out_ref[l][q] = refs[l];
+      out_mv[l][q] = mvnew(0, 0);
+      if (refs[l] >= 0) out_mv[l][q] = (zero || (refs[l] == 0 && col_zero)) ? mvnew(0, 0) : mvp[l];
+    }
+  }
+}
+
+/* picture.rs:491-495 avg_into */
+static void avg_into(uint8_t *d, const uint8_t *s, int n) {
+  for (int i = 0; i < n; i++) d[i] = (uint8_t)(((uint32_t)d[i] + s[i] + 1) >> 1);
+}
+
+/* The motion search of analyse_b (slice.rs:2189-2377) and b_partition
+ * (slice.rs:2086-2187) for the B MB at (mx, my), without the parts that
+ * need the residual coder or RD (BSkip, rd_cost, intra): direct_spatial
+ * and the direct prediction's cost, the list-0/list-1 16x16 search_part
+ * seeded with the direct MVs, the three direction costs, and the 16x8 and
+ * 8x16 b_partition searches (seeded with the 16x16 MVs) with their
+ * per-partition directions and costs.  The caller's decision picks among
+ * them (direct, 16x16 dir, partitions when best.cost > 1024). */
+void o_b_analyse(const oparams *p, const ombinfo2 *mbs, const ombinfo2 *col, int mx, int my, const oref *r0,
+                 const oref *r1, const uint8_t *src16, uint32_t lam, ob_out *o) {
+  memset(o, 0, sizeof *o);
+  const int n = p->mbw * p->mbh, px = 16 * mx, py = 16 * my;
+  const oref *rp[2] = {r0, r1};
+  o_direct_spatial(p, mbs, col, mx, my, o->direct_ref, o->direct_mv);
+  int ok = 1;
+  for (int q = 0; q < 4; q++)
+    for (int l = 0; l < 2; l++)
+      if (o->direct_ref[l][q] >= 0 && !o_mv_ok(p, px + 8 * (q & 1), py + 8 * (q >> 1), 8, 8, o->direct_mv[l][q]))
+        ok = 0;
+  o->direct_ok = (uint32_t)ok;
+  o->direct_cost = 0xFFFFFFFFu;
+  if (ok) {
+    uint8_t y[256], t[256];
+    for (int q = 0; q < 4; q++) {
+      const int x = 8 * (q & 1), yy = 8 * (q >> 1);
+      const int u0 = o->direct_ref[0][q] >= 0, u1 = o->direct_ref[1][q] >= 0;
+      if (u0) o_mc_luma(r0, px + x, py + yy, o->direct_mv[0][q], 8, 8, y + yy * 16 + x, 16);
+      if (u1) {
+        if (u0) {
+          o_mc_luma(r1, px + x, py + yy, o->direct_mv[1][q], 8, 8, t + yy * 16 + x, 16);
+          for (int rr = 0; rr < 8; rr++) avg_into(y + (yy + rr) * 16 + x, t + (yy + rr) * 16 + x, 8);
+        } else {
+          o_mc_luma(r1, px + x, py + yy, o->direct_mv[1][q], 8, 8, y + yy * 16 + x, 16);
+        }
+      }
+    }
+    o->direct_cost = o_satd(src16, 16, y, 16, 16, 16) + ((lam * 2) >> 8);
+  }
+  ombinfo *proj = (ombinfo *)malloc(sizeof(ombinfo) * (size_t)n);
+  omv mvs[4], mvps[4];
+  for (int l = 0; l < 2; l++) {
+    project_list(mbs, n, l, proj);
+    o->cost16[l] = o_search_part(p, proj, mx, my, rp[l], OP_16x16, src16, lam, o->direct_mv[l][0], mvs, mvps, NULL);
+    o->mv16[l] = mvs[0];
+    o->mvp16[l] = mvps[0];
+  }
+  uint8_t pr[2][256], bi[256];
+  for (int l = 0; l < 2; l++) o_mc_luma(rp[l], px, py, o->mv16[l], 16, 16, pr[l], 16);
+  memcpy(bi, pr[0], 256);
+  avg_into(bi, pr[1], 256);
+  const uint32_t bits = o_mv_bits(o->mv16[0], o->mvp16[0]) + o_mv_bits(o->mv16[1], o->mvp16[1]);
+  o->cost[0] = o->cost16[0] + ((lam * 3) >> 8);
+  o->cost[1] = o->cost16[1] + ((lam * 3) >> 8);
+  o->cost[2] = o_satd(src16, 16, bi, 16, 16, 16) + ((lam * (bits + 5)) >> 8);
+  o->dir = 0;
+  for (int d = 1; d < 3; d++)
+    if (o->cost[d] < o->cost[o->dir]) o->dir = (uint32_t)d;
+  for (int k = 0; k < 2; k++) {
+    const int part = k == 0 ? OP_16x8 : OP_8x16;
+    for (int l = 0; l < 2; l++) {
+      project_list(mbs, n, l, proj);
+      o_search_part(p, proj, mx, my, rp[l], part, src16, lam, o->mv16[l], mvs, mvps, NULL);
+      o->part_mv[k][l][0] = mvs[0];
+      o->part_mv[k][l][1] = mvs[1];
+    }
+    for (int l = 0; l < 2; l++)
+      for (int pi = 0; pi < 2; pi++) {
+        int x, y, w, h;
+        part_rect(part, pi, &x, &y, &w, &h);
+        o_mc_luma(rp[l], px + x, py + y, o->part_mv[k][l][pi], w, h, pr[l] + y * 16 + x, 16);
+      }
+    memcpy(bi, pr[0], 256);
+    avg_into(bi, pr[1], 256);
+    const uint8_t *srcs[3] = {pr[0], pr[1], bi};
+    uint32_t cost = (lam * 8) >> 8;
+    for (int pi = 0; pi < 2; pi++) {
+      int x, y, w, h;
+      part_rect(part, pi, &x, &y, &w, &h);
+      uint32_t bc = 0xFFFFFFFFu;
+      int bd = 0;
+      for (int d = 0; d < 3; d++) {
+        const uint32_t b = d == 2 ? 14 : 7;
+        const uint32_t c = o_satd(src16 + y * 16 + x, 16, srcs[d] + y * 16 + x, 16, w, h) + ((lam * b) >> 8);
+        if (c < bc) {
+          bc = c;
+          bd = d;
+        }
+      }
+      cost += bc;
+      o->part_dir[k][pi] = (uint8_t)bd;
+    }
+    o->part_cost[k] = cost;
+  }
+  free(proj);
+}

@@ -222,3 +222,31 @@ typedef struct {
 omv o_tree_search(const uint8_t *cur, const uint8_t *r, int w, int h, int x0, int y0, omv seed);
 void o_tree_analyse(const uint8_t *cur, const uint8_t *l0, const uint8_t *l1, int w, int h, otree_block *out);
 double o_lookahead_mt(const uint8_t *lr, int w, int h, int n, int threads, uint64_t *sums, otree_block *blocks);
+
+/* ---- B macroblocks: direct_spatial and the motion search of analyse_b ---- */
+/* mbinfo.rs:74-107 with both lists */
+typedef struct {
+  int8_t kind;
+  int8_t ref_idx[2][4];
+  omv mv[2][16];
+} ombinfo2;
+
+/* Layout of pcme_b_out (include/pcme.h). */
+typedef struct {
+  omv direct_mv[2][4];
+  int8_t direct_ref[2][4];
+  uint32_t direct_ok, direct_cost;
+  omv mv16[2], mvp16[2];
+  uint32_t cost16[2];
+  uint32_t cost[3];
+  uint32_t dir;
+  omv part_mv[2][2][2];
+  uint32_t part_cost[2];
+  uint8_t part_dir[2][2];
+  uint8_t reserved[4];
+} ob_out;
+
+void o_direct_spatial(const oparams *p, const ombinfo2 *mbs, const ombinfo2 *col, int mx, int my,
+                      int8_t out_ref[2][4], omv out_mv[2][4]);
+void o_b_analyse(const oparams *p, const ombinfo2 *mbs, const ombinfo2 *col, int mx, int my, const oref *r0,
+                 const oref *r1, const uint8_t *src16, uint32_t lam, ob_out *o);

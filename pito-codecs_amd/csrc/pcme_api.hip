@@ -47,6 +47,8 @@ struct Dev {
   size_t dsj_cap = 0;
   pcme_part_job *dpj = nullptr;     // per-call search_part jobs
   size_t dpj_cap = 0;
+  pcme_b_job *dbj = nullptr;        // B macroblock jobs
+  size_t dbj_cap = 0;
   unsigned char *hstage = nullptr;  // pinned upload staging
   size_t hstage_cap = 0;
   hipEvent_t stage_ev = nullptr;
@@ -551,6 +553,47 @@ int pcme_search_part_batch(const pcme_part_job *jobs, int njobs, pcme_part_out *
   if ((rc = grow(&dv->dpj, &dv->dpj_cap, (size_t)njobs))) return rc;
   if ((rc = upload(dv, dv->dpj, jobs, sizeof(pcme_part_job) * (size_t)njobs, (hipStream_t)stream))) return rc;
   hipLaunchKernelGGL(search_part_batch_kernel, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream, dv->dpj,
+                     njobs, out);
+  HIPCHK(hipGetLastError());
+  return PCME_OK;
+}
+
+int pcme_b_analyse_batch(const pcme_b_job *jobs, int njobs, pcme_b_out *out, void *stream) {
+  if (njobs < 0 || (njobs > 0 && (!jobs || !out)) || ((uintptr_t)out & 3u))
+    return fail(PCME_EINVAL, "pcme_b_analyse_batch: bad arguments");
+  for (int i = 0; i < njobs; i++) {
+    const pcme_b_job &J = jobs[i];
+    if (!J.src || ((uintptr_t)J.src & 3u) || (J.sstride & 3) || J.sstride < 16)
+      return fail(PCME_EINVAL, "job %d: src null, not 4-byte aligned or stride not a multiple of 4", i);
+    for (int l = 0; l < 2; l++)
+      for (int k = 0; k < 4; k++) {
+        const uint8_t *pl = l ? J.ref1[k] : J.ref0[k];
+        if (!pl || ((uintptr_t)pl & 3u)) return fail(PCME_EINVAL, "job %d: reference plane null or misaligned", i);
+      }
+    if (J.mbw < 1 || J.mbh < 1 || J.mx < 0 || J.my < 0 || J.mx >= J.mbw || J.my >= J.mbh)
+      return fail(PCME_EINVAL, "job %d: MB outside the picture", i);
+    if (J.me_range < 0 || J.me_range > 1024 || J.subpel_rounds < 0 || J.subpel_rounds > 8)
+      return fail(PCME_EINVAL, "job %d: me_range/subpel_rounds out of range", i);
+    if (J.vlim < 16LL * J.my + 17 || J.vlim > 16LL * J.mbh + 44)
+      return fail(PCME_EINVAL, "job %d: vlim %lld outside [16*my+17, 16*mbh+44]", i, (long long)J.vlim);
+    bool ok = true;
+    for (int l = 0; l < 2; l++) {
+      for (int k = 0; k < 10; k++) ok = ok && nb_ok(J.nb[l][k]);
+      for (int q = 0; q < 4; q++) {
+        pcme_nb_block cb = {J.col_mv[l][q], J.col_ref[l][q], {0, 0, 0}};
+        ok = ok && nb_ok(cb) && J.col_ref[l][q] >= -1;
+      }
+    }
+    if (!ok) return fail(PCME_EINVAL, "job %d: a neighbour or co-located ref/MV out of range", i);
+  }
+  if (njobs == 0) return PCME_OK;
+  Dev *dv;
+  int rc = get_dev(&dv);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(dv->mu);
+  if ((rc = grow(&dv->dbj, &dv->dbj_cap, (size_t)njobs))) return rc;
+  if ((rc = upload(dv, dv->dbj, jobs, sizeof(pcme_b_job) * (size_t)njobs, (hipStream_t)stream))) return rc;
+  hipLaunchKernelGGL(b_analyse_batch_kernel, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream, dv->dbj,
                      njobs, out);
   HIPCHK(hipGetLastError());
   return PCME_OK;

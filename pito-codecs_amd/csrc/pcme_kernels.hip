@@ -1361,7 +1361,21 @@ search_frames_hinted_kernel(const DevJob *__restrict__ jobs, const int2 *__restr
 static_assert(sizeof(pcme_nb_block) == 8 && sizeof(pcme_part_job) == 168 && sizeof(pcme_part_out) == 36,
               "pcme_part_job / pcme_part_out layout");
 
-__device__ __forceinline__ uint32_t nb_mv(const pcme_nb_block &b) { return pack_mv(b.mv.x, b.mv.y); }
+// nb_part gives MV zero with a negative reference (slice.rs:429-430).
+__device__ __forceinline__ uint32_t nb_mv(const pcme_nb_block &b) { return b.ref >= 0 ? pack_mv(b.mv.x, b.mv.y) : 0u; }
+
+// Neighbour blocks in the order left[4], top[4], top_right, top_left.
+__device__ __forceinline__ NbrG nbr_of(const pcme_nb_block *b) {
+  NbrG n;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    n.rL[i] = b[i].ref; n.L[i] = nb_mv(b[i]);
+    n.rT[i] = b[4 + i].ref; n.T[i] = nb_mv(b[4 + i]);
+  }
+  n.rTR = b[8].ref; n.TR = nb_mv(b[8]);
+  n.rTL = b[9].ref; n.TL = nb_mv(b[9]);
+  return n;
+}
 
 __global__ void __launch_bounds__(64) search_part_batch_kernel(const pcme_part_job *__restrict__ jobs,
                                                                int njobs, pcme_part_out *out) {
@@ -1378,14 +1392,11 @@ __global__ void __launch_bounds__(64) search_part_batch_kernel(const pcme_part_j
   c.lam = J.lam;
   c.range = J.me_range;
   c.subpel = J.subpel_rounds;
-  NbrG n;
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    n.rL[i] = J.left[i].ref; n.L[i] = nb_mv(J.left[i]);
-    n.rT[i] = J.top[i].ref; n.T[i] = nb_mv(J.top[i]);
-  }
-  n.rTR = J.top_right.ref; n.TR = nb_mv(J.top_right);
-  n.rTL = J.top_left.ref; n.TL = nb_mv(J.top_left);
+  static_assert(offsetof(pcme_part_job, top) == offsetof(pcme_part_job, left) + 32 &&
+                offsetof(pcme_part_job, top_right) == offsetof(pcme_part_job, top) + 32 &&
+                offsetof(pcme_part_job, top_left) == offsetof(pcme_part_job, top_right) + 8,
+                "pcme_part_job: neighbour blocks contiguous");
+  const NbrG n = nbr_of(J.left);
   const uint32_t seed = pack_mv(J.seed.x, J.seed.y);
   const int mbx = 16 * J.mx, mby = 16 * J.my;
   PartRes r;
@@ -1405,6 +1416,281 @@ __global__ void __launch_bounds__(64) search_part_batch_kernel(const pcme_part_j
     }
     ((uint32_t *)&out[j])[lane] = v;
   }
+}
+
+// ---------------------------------------------------------- B macroblocks
+
+static_assert(sizeof(pcme_b_job) == 328 && sizeof(pcme_b_out) == 136, "pcme_b_job / pcme_b_out layout");
+
+__device__ __forceinline__ int minp(int x, int y) { return (x >= 0 && y >= 0) ? min(x, y) : max(x, y); }
+
+// mvp (slice.rs:434-482) of a 16x16 partition (Normal shape) for reference
+// index rf, from explicit neighbours.
+__device__ __forceinline__ uint32_t mvp16_rf(const NbrG &n, int rf) {
+  bool aa = n.rL[0] >= -1, ab = n.rT[0] >= -1, ac = n.rTR >= -1;
+  int ra = n.rL[0], rb = n.rT[0], rc = n.rTR;
+  uint32_t ma = n.L[0], mb = n.T[0], mc = n.TR;
+  if (!ac) {
+    ac = n.rTL >= -1; rc = n.rTL; mc = n.TL;
+  }
+  if (!ab && !ac && aa) {
+    ab = ac = true; rb = rc = ra; mb = mc = ma;
+  }
+  if (!aa) { ra = -1; ma = 0u; }
+  if (!ab) { rb = -1; mb = 0u; }
+  if (!ac) { rc = -1; mc = 0u; }
+  const int cnt = (ra == rf) + (rb == rf) + (rc == rf);
+  if (cnt == 1) return ra == rf ? ma : (rb == rf ? mb : mc);
+  return pack_mv(median3(mv_x(ma), mv_x(mb), mv_x(mc)), median3(mv_y(ma), mv_y(mb), mv_y(mc)));
+}
+
+// One quarter-pel luma sample (mc_luma, picture.rs:415-449) of picture
+// pixel (x, y) displaced by the packed MV m.
+__device__ __forceinline__ uint32_t mc_px(const SCtx &c, int x, int y, uint32_t m) {
+  const int mxv = mv_x(m), myv = mv_y(m);
+  const int xi = x + (mxv >> 2) + PCME_PAD, yi = y + (myv >> 2) + PCME_PAD;
+  const uint32_t e = qpel_entry((uint32_t)((myv & 3) * 4 + (mxv & 3)));
+  const uint32_t a = plane_sel(c, e & 3u)[(size_t)(yi + ((e >> 3) & 1)) * c.stride + xi + ((e >> 2) & 1)];
+  const uint32_t b = plane_sel(c, (e >> 4) & 3u)[(size_t)(yi + ((e >> 7) & 1)) * c.stride + xi + ((e >> 6) & 1)];
+  return (a + b + 1u) >> 1;
+}
+
+// SATD of the W x H rect at (x, y) of the source MB against a 16x16
+// prediction in LDS (row stride 16): one 4x8 strip per lane, wave-uniform.
+template <int W, int H>
+__device__ __forceinline__ uint32_t satd_rect(const uint8_t *src, int sstride, const uint8_t *pred, int x, int y) {
+  constexpr int NS = (W / 8) * (H / 4);
+  const int lane = (int)lane_id();
+  const int s = lane < NS ? lane : 0;
+  const int sx = x + 8 * (s % (W / 8)), sy = y + 4 * (s / (W / 8));
+  uint32_t S[8], P[8];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const uint8_t *q = src + (size_t)(sy + r) * sstride + sx;
+    S[2 * r] = *(const uint32_t *)q;
+    S[2 * r + 1] = *(const uint32_t *)(q + 4);
+    P[2 * r] = *(const uint32_t *)(pred + (sy + r) * 16 + sx);
+    P[2 * r + 1] = *(const uint32_t *)(pred + (sy + r) * 16 + sx + 4);
+  }
+  v2s sp[16];
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int cc = 0; cc < 4; cc++) sp[r * 4 + cc] = as_v2s(__builtin_amdgcn_perm(S[2 * r + 1], S[2 * r], SELC(cc)));
+  const uint32_t v = lane < NS ? satd_strip8(sp, P) : 0u;
+  return rdlane(red8(v), 0);
+}
+
+// Prediction of the rect (x, y, w, h) of the MB at picture (px, py) with MV
+// m, into the 16x16 LDS buffer; lanes cover the rect's pixels.
+__device__ __forceinline__ void mc_rect(const SCtx &c, int px, int py, int x, int y, int w, int h, uint32_t m,
+                                        uint8_t *buf) {
+  for (int i = (int)lane_id(); i < w * h; i += 64) {
+    const int xx = x + i % w, yy = y + i / w;
+    buf[yy * 16 + xx] = (uint8_t)mc_px(c, px + xx, py + yy, m);
+  }
+}
+
+__device__ __forceinline__ void avg_buf(uint8_t *d, const uint8_t *a, const uint8_t *b) {
+  for (int i = (int)lane_id(); i < 256; i += 64) d[i] = (uint8_t)(((uint32_t)a[i] + b[i] + 1u) >> 1);
+}
+
+// analyse_b's motion search (slice.rs:2189-2377 without BSkip, RD and
+// intra) and b_partition (slice.rs:2086-2187) for one MB per wave.
+__global__ void __launch_bounds__(64) b_analyse_batch_kernel(const pcme_b_job *__restrict__ jobs, int njobs,
+                                                             pcme_b_out *out) {
+  __shared__ __attribute__((aligned(16))) uint32_t s_lds[LDS_PER_WAVE / 4];
+  __shared__ __attribute__((aligned(16))) uint8_t s_pred[3][256];
+  const int j = blockIdx.x;
+  if (j >= njobs) return;
+  const pcme_b_job &J = jobs[j];
+  const int lane = (int)lane_id();
+  SCtx c[2];
+#pragma unroll
+  for (int l = 0; l < 2; l++) {
+#pragma unroll
+    for (int p = 0; p < 4; p++) c[l].pl[p] = l ? J.ref1[p] : J.ref0[p];
+    c[l].stride = 16 * J.mbw + 2 * PCME_PAD;
+    c[l].pw = 16 * J.mbw;
+    c[l].ph = 16 * J.mbh;
+    c[l].vlim = (int)J.vlim;
+    c[l].lam = J.lam;
+    c[l].range = J.me_range;
+    c[l].subpel = J.subpel_rounds;
+  }
+  const NbrG n[2] = {nbr_of(J.nb[0]), nbr_of(J.nb[1])};
+  const int px = 16 * J.mx, py = 16 * J.my;
+  const uint32_t lam = J.lam;
+  // direct_spatial (slice.rs:2030-2084)
+  int refs[2];
+  uint32_t dmvp[2] = {0u, 0u};
+#pragma unroll
+  for (int l = 0; l < 2; l++) {
+    const int a = n[l].rL[0] >= -1 ? n[l].rL[0] : -1;
+    const int b = n[l].rT[0] >= -1 ? n[l].rT[0] : -1;
+    const int cc = n[l].rTR >= -1 ? n[l].rTR : (n[l].rTL >= -1 ? n[l].rTL : -1);
+    refs[l] = minp(a, minp(b, cc));
+  }
+  const bool zero = refs[0] < 0 && refs[1] < 0;
+  if (zero) {
+    refs[0] = refs[1] = 0;
+  } else {
+#pragma unroll
+    for (int l = 0; l < 2; l++)
+      if (refs[l] >= 0) dmvp[l] = mvp16_rf(n[l], refs[l]);
+  }
+  uint32_t dmv[2][4];
+  bool direct_ok = true;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    bool col_zero = false;
+    if (!J.col_intra) {
+      const int l = J.col_ref[0][q] >= 0 ? 0 : 1;
+      const pcme_mv m = J.col_mv[l][q];
+      col_zero = J.col_ref[l][q] == 0 && m.x >= -1 && m.x <= 1 && m.y >= -1 && m.y <= 1;
+    }
+#pragma unroll
+    for (int l = 0; l < 2; l++) {
+      dmv[l][q] = (refs[l] >= 0 && !(zero || (refs[l] == 0 && col_zero))) ? dmvp[l] : 0u;
+      if (refs[l] >= 0) {
+        const Box bx = legal_box(c[l], px + 8 * (q & 1), py + 8 * (q >> 1), 8, 8);
+        direct_ok = direct_ok && in_box(bx, mv_x(dmv[l][q]), mv_y(dmv[l][q]));
+      }
+    }
+  }
+  uint32_t direct_cost = 0xFFFFFFFFu;
+  if (direct_ok) {
+    for (int i = lane; i < 256; i += 64) {
+      const int x = i & 15, y = i >> 4, q = ((y >> 3) << 1) | (x >> 3);
+      uint32_t m0 = dmv[0][0], m1 = dmv[1][0];
+#pragma unroll
+      for (int k = 1; k < 4; k++) {
+        m0 = q == k ? dmv[0][k] : m0;
+        m1 = q == k ? dmv[1][k] : m1;
+      }
+      uint32_t v;
+      if (refs[0] >= 0 && refs[1] >= 0)
+        v = (mc_px(c[0], px + x, py + y, m0) + mc_px(c[1], px + x, py + y, m1) + 1u) >> 1;
+      else if (refs[0] >= 0)
+        v = mc_px(c[0], px + x, py + y, m0);
+      else
+        v = mc_px(c[1], px + x, py + y, m1);
+      s_pred[2][i] = (uint8_t)v;
+    }
+    __syncthreads();
+    direct_cost = satd_rect<16, 16>(J.src, J.sstride, s_pred[2], 0, 0) + ((lam * 2u) >> 8);
+    __syncthreads();
+  }
+  // 16x16 per list, seeded with the direct MVs (slice.rs:2232-2244)
+  PartRes r16[2];
+  search_part<0, false>(c[0], n[0], J.src, J.sstride, px, py, dmv[0][0], 0u, -1, r16[0], s_lds);
+  search_part<0, false>(c[1], n[1], J.src, J.sstride, px, py, dmv[1][0], 0u, -1, r16[1], s_lds);
+  __syncthreads();
+  mc_rect(c[0], px, py, 0, 0, 16, 16, r16[0].mv[0], s_pred[0]);
+  mc_rect(c[1], px, py, 0, 0, 16, 16, r16[1].mv[0], s_pred[1]);
+  __syncthreads();
+  avg_buf(s_pred[2], s_pred[0], s_pred[1]);
+  __syncthreads();
+  uint32_t bits = 0;
+#pragma unroll
+  for (int l = 0; l < 2; l++)
+    bits += se_bits(mv_x(r16[l].mv[0]) - mv_x(r16[l].mvp[0])) + se_bits(mv_y(r16[l].mv[0]) - mv_y(r16[l].mvp[0]));
+  uint32_t cost[3];
+  cost[0] = r16[0].total + ((lam * 3u) >> 8);
+  cost[1] = r16[1].total + ((lam * 3u) >> 8);
+  cost[2] = satd_rect<16, 16>(J.src, J.sstride, s_pred[2], 0, 0) + ((lam * (bits + 5u)) >> 8);
+  const uint32_t dir = cost[1] < cost[0] ? (cost[2] < cost[1] ? 2u : 1u) : (cost[2] < cost[0] ? 2u : 0u);
+  // b_partition 16x8 and 8x16 (slice.rs:2086-2187)
+  uint32_t pmv[2][2][2], pcost[2], pdir[2][2];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    PartRes rp[2];
+#pragma unroll
+    for (int l = 0; l < 2; l++) {
+      if (k == 0)
+        search_part<1, false>(c[l], n[l], J.src, J.sstride, px, py, r16[l].mv[0], 0u, -1, rp[l], s_lds);
+      else
+        search_part<2, false>(c[l], n[l], J.src, J.sstride, px, py, r16[l].mv[0], 0u, -1, rp[l], s_lds);
+      pmv[k][l][0] = rp[l].mv[0];
+      pmv[k][l][1] = rp[l].mv[1];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < 2; l++)
+#pragma unroll
+      for (int pi = 0; pi < 2; pi++) {
+        if (k == 0)
+          mc_rect(c[l], px, py, 0, 8 * pi, 16, 8, rp[l].mv[pi], s_pred[l]);
+        else
+          mc_rect(c[l], px, py, 8 * pi, 0, 8, 16, rp[l].mv[pi], s_pred[l]);
+      }
+    __syncthreads();
+    avg_buf(s_pred[2], s_pred[0], s_pred[1]);
+    __syncthreads();
+    uint32_t tot = (lam * 8u) >> 8;
+#pragma unroll
+    for (int pi = 0; pi < 2; pi++) {
+      uint32_t best = 0xFFFFFFFFu, bd = 0;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const uint32_t sd = k == 0 ? satd_rect<16, 8>(J.src, J.sstride, s_pred[d], 0, 8 * pi)
+                                   : satd_rect<8, 16>(J.src, J.sstride, s_pred[d], 8 * pi, 0);
+        const uint32_t cd = sd + ((lam * (d == 2 ? 14u : 7u)) >> 8);
+        if (cd < best) {
+          best = cd;
+          bd = (uint32_t)d;
+        }
+      }
+      tot += best;
+      pdir[k][pi] = bd;
+    }
+    pcost[k] = tot;
+    __syncthreads();
+  }
+  // the record: 34 dwords, one per lane
+  uint32_t w = 0;
+  if (lane < 8) {
+    const int l = lane >> 2, q = lane & 3;
+    uint32_t v = dmv[0][0];
+#pragma unroll
+    for (int ll = 0; ll < 2; ll++)
+#pragma unroll
+      for (int qq = 0; qq < 4; qq++) v = (l == ll && q == qq) ? dmv[ll][qq] : v;
+    w = v;
+  } else if (lane == 8 || lane == 9) {
+    const int l = lane - 8;
+    const uint32_t r = (uint32_t)(uint8_t)(int8_t)refs[l] * 0x01010101u;
+    w = r;
+  } else if (lane == 10) {
+    w = direct_ok ? 1u : 0u;
+  } else if (lane == 11) {
+    w = direct_cost;
+  } else if (lane == 12 || lane == 13) {
+    w = r16[lane - 12].mv[0];
+  } else if (lane == 14 || lane == 15) {
+    w = r16[lane - 14].mvp[0];
+  } else if (lane == 16 || lane == 17) {
+    w = r16[lane - 16].total;
+  } else if (lane >= 18 && lane <= 20) {
+    w = cost[lane - 18];
+  } else if (lane == 21) {
+    w = dir;
+  } else if (lane >= 22 && lane <= 29) {
+    const int i = lane - 22, k = i >> 2, l = (i >> 1) & 1, pi = i & 1;
+    uint32_t v = pmv[0][0][0];
+#pragma unroll
+    for (int kk = 0; kk < 2; kk++)
+#pragma unroll
+      for (int ll = 0; ll < 2; ll++)
+#pragma unroll
+        for (int pp = 0; pp < 2; pp++) v = (k == kk && l == ll && pi == pp) ? pmv[kk][ll][pp] : v;
+    w = v;
+  } else if (lane == 30 || lane == 31) {
+    w = pcost[lane - 30];
+  } else if (lane == 32) {
+    w = pdir[0][0] | (pdir[0][1] << 8) | (pdir[1][0] << 16) | (pdir[1][1] << 24);
+  }
+  if (lane < 34) ((uint32_t *)&out[j])[lane] = w;
 }
 
 // -------------------------------------------------- per-call search batch

@@ -89,6 +89,19 @@ class PartJob(ctypes.Structure):
 assert ctypes.sizeof(NbBlock) == 8 and ctypes.sizeof(PartJob) == 168
 
 
+class BJob(ctypes.Structure):
+    _fields_ = [("src", ctypes.c_void_p), ("sstride", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+                ("ref0", ctypes.c_void_p * 4), ("ref1", ctypes.c_void_p * 4),
+                ("mbw", ctypes.c_int32), ("mbh", ctypes.c_int32), ("mx", ctypes.c_int32), ("my", ctypes.c_int32),
+                ("lam", ctypes.c_uint32), ("me_range", ctypes.c_int32), ("subpel_rounds", ctypes.c_int32),
+                ("reserved1", ctypes.c_int32), ("vlim", ctypes.c_int64),
+                ("nb", (NbBlock * 10) * 2), ("col_mv", (Mv * 4) * 2), ("col_ref", (ctypes.c_int8 * 4) * 2),
+                ("col_intra", ctypes.c_int32), ("reserved2", ctypes.c_int32)]
+
+
+assert ctypes.sizeof(BJob) == 328
+
+
 class SynthCfg(ctypes.Structure):
     _fields_ = [("w", ctypes.c_int32), ("h", ctypes.c_int32),
                 ("frames", ctypes.c_int32), ("nrect", ctypes.c_int32),
@@ -111,6 +124,15 @@ class TreeJob(ctypes.Structure):
 # pcme_part_out (36 bytes): search_part's (mvs, mvps, total)
 PART_OUT_DTYPE = np.dtype([("mvs", "<i2", (4, 2)), ("mvps", "<i2", (4, 2)), ("total", "<u4")])
 assert PART_OUT_DTYPE.itemsize == 36
+
+# pcme_b_out (136 bytes): one B macroblock's motion search
+B_OUT_DTYPE = np.dtype([
+    ("direct_mv", "<i2", (2, 4, 2)), ("direct_ref", "i1", (2, 4)), ("direct_ok", "<u4"), ("direct_cost", "<u4"),
+    ("mv16", "<i2", (2, 2)), ("mvp16", "<i2", (2, 2)), ("cost16", "<u4", (2,)), ("cost", "<u4", (3,)),
+    ("dir", "<u4"), ("part_mv", "<i2", (2, 2, 2, 2)), ("part_cost", "<u4", (2,)), ("part_dir", "u1", (2, 2)),
+    ("reserved", "u1", (4,)),
+])
+assert B_OUT_DTYPE.itemsize == 136
 
 # pcme_cplx_block (12 bytes) and pcme_tree_block (20 bytes) as numpy records
 CPLX_BLOCK_DTYPE = np.dtype([("intra", "<u4"), ("inter", "<u4"), ("mv", "<i2", (2,))])
@@ -143,7 +165,7 @@ EXPORTS = (
     "pcme_plan_set_grid", "pcme_plan_autotune", "pcme_plan_destroy", "pcme_set_spin_limit",
     "pcme_net_create", "pcme_net_destroy", "pcme_net_get_params", "pcme_net_set_params",
     "pcme_net_inputs", "pcme_net_forward", "pcme_net_grad", "pcme_net_train_step", "pcme_net_hints",
-    "pcme_lowres", "pcme_complexity", "pcme_mbtree_analyse", "pcme_search_part_batch",
+    "pcme_lowres", "pcme_complexity", "pcme_mbtree_analyse", "pcme_search_part_batch", "pcme_b_analyse_batch",
 )
 
 _lib = None
@@ -179,6 +201,7 @@ def load() -> ctypes.CDLL:
         "pcme_search_frames_sync": (c.c_int, [c.POINTER(FrameJob), c.c_int, vp]),
         "pcme_search_batch": (c.c_int, [c.POINTER(SearchJob), c.c_int, vp, vp, vp]),
         "pcme_search_part_batch": (c.c_int, [c.POINTER(PartJob), c.c_int, vp, vp]),
+        "pcme_b_analyse_batch": (c.c_int, [c.POINTER(BJob), c.c_int, vp, vp]),
         "pcme_pred_frame": (c.c_int, [vp, vp, vp, vp, vp, vp, vp, c.c_int32, c.c_int32, vp, vp, vp, vp]),
         "pcme_synth_clip": (c.c_int, [c.POINTER(SynthCfg), vp, vp, vp, c.c_int32]),
         "pcme_plan_create": (c.c_int, [c.POINTER(FrameJob), c.c_int, c.POINTER(vp)]),
@@ -419,6 +442,14 @@ def search_part_batch(jobs: list[PartJob], out_ptr: int, stream=None) -> None:
     lib = load()
     arr = (PartJob * len(jobs))(*jobs)
     _check(lib.pcme_search_part_batch(arr, len(jobs), out_ptr, _stream_ptr(stream)))
+
+
+def b_analyse_batch(jobs: list[BJob], out_ptr: int, stream=None) -> None:
+    """pcme_b_analyse_batch: out_ptr (device) receives len(jobs)
+    B_OUT_DTYPE records."""
+    lib = load()
+    arr = (BJob * len(jobs))(*jobs)
+    _check(lib.pcme_b_analyse_batch(arr, len(jobs), out_ptr, _stream_ptr(stream)))
 
 
 def pred_frame(clip: DeviceClip, ref: int, mb_ptr: int, pred_y_ptr: int, pred_u_ptr: int,

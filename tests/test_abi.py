@@ -172,3 +172,31 @@ def test_search_part_batch_checks_arguments(pcme_lib):
     assert lib.pcme_search_part_batch(job(), 1, 66, None) == pcme_lib.PCME_EINVAL  # misaligned out
     if pcme_lib.device_count() == 0:
         assert lib.pcme_search_part_batch(job(), 1, out, None) == pcme_lib.PCME_ENODEV
+
+
+def test_b_analyse_batch_checks_arguments(pcme_lib):
+    lib = pcme_lib.load()
+
+    def job(**kw):
+        j = pcme_lib.BJob()
+        j.src, j.sstride, j.mbw, j.mbh, j.me_range, j.subpel_rounds, j.vlim = 64, 144, 4, 4, 16, 1, 4 * 16 + 44
+        for k in range(4):
+            j.ref0[k] = 64
+            j.ref1[k] = 128
+        for k, v in kw.items():
+            setattr(j, k, v)
+        return (pcme_lib.BJob * 1)(j)
+
+    for bad in (job(src=None), job(sstride=18), job(my=4), job(vlim=10), job(me_range=-1)):
+        assert lib.pcme_b_analyse_batch(bad, 1, 64, None) == pcme_lib.PCME_EINVAL
+    j = job()
+    j[0].ref1[2] = None
+    assert lib.pcme_b_analyse_batch(j, 1, 64, None) == pcme_lib.PCME_EINVAL
+    j = job()
+    j[0].col_ref[1][3] = -2
+    assert lib.pcme_b_analyse_batch(j, 1, 64, None) == pcme_lib.PCME_EINVAL
+    j = job()
+    j[0].nb[1][9].ref = -5
+    assert lib.pcme_b_analyse_batch(j, 1, 64, None) == pcme_lib.PCME_EINVAL
+    if pcme_lib.device_count() == 0:
+        assert lib.pcme_b_analyse_batch(job(), 1, 64, None) == pcme_lib.PCME_ENODEV
