@@ -61,6 +61,7 @@ def parse():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-pruned", action="store_true", help="skip the CNN-pruned leg")
     ap.add_argument("--no-lookahead", action="store_true", help="skip the lowres-lookahead leg")
+    ap.add_argument("--no-b", action="store_true", help="skip the B-picture search leg")
     ap.add_argument("--ncand", type=int, default=5, help="pruner candidates per MB")
     ap.add_argument("--weights", default=None, help="pruner weights (.npz; default: the committed ones)")
     ap.add_argument("--thr", type=float, default=0.6, help="pruner confidence threshold (partition head)")
@@ -262,6 +263,8 @@ def main():
 
     if not args.no_lookahead:
         line["lookahead"] = lookahead_leg(args, pcme, torch, clips, stream, threads)
+    if not args.no_b and args.workload == "1080p":
+        line["b_search"] = b_leg(args, pcme, torch, clips[0], outs[0], specs[0][1], specs[0][2], stream, threads)
 
     if not args.no_cpu:
         from oracle import oracle as O
@@ -382,6 +385,75 @@ def lookahead_leg(args, pcme, torch, clips, stream, threads):
                                "seconds": round(secs, 3)}
         out["parity_sample"] = parity
     return out
+
+
+def b_leg(args, pcme, torch, dc, out, qp, me_range, stream, threads):
+    """The B-picture search of frame 1 of the clip between frames 0 (list 0)
+    and 2 (list 1), all 8,160 MBs in one pcme_b_analyse_batch call (job
+    upload included).  The causal neighbours' motion is the plain P field of
+    frame 1 for list 0 and its mirror for list 1 (every MB inter, reference
+    0); the co-located MBs are the P field of frame 2.  The C oracle runs the
+    same MBs on the host (o_b_frame_mt) and the records are compared."""
+    import numpy as np
+    from pcme import bsearch as B
+    from oracle import oracle as O
+    mbw, mbh = dc.mbw, dc.mbh
+    f = out.cpu().numpy().view(pcme.MB_DTYPE).reshape(-1, mbh, mbw)
+    mv0 = f[0]["mv"].astype(np.int16)
+    mv = np.stack([mv0, -mv0])
+    ref = np.zeros((2, mbh, mbw, 4), np.int8)
+    intra = np.zeros((mbh, mbw), bool)
+    col_mv = np.zeros((mbh, mbw, 2, 16, 2), np.int16)
+    col_mv[:, :, 0] = f[1]["mv"]
+    col_ref = np.zeros((mbh, mbw, 2, 4), np.int8)
+    col_ref[:, :, 1] = -1
+    jobs = B.b_jobs(dc, 1, (dc, 0), (dc, 2), mv, ref, intra, col_mv, col_ref, intra, qp, me_range, 1)
+    nmb = mbw * mbh
+    res = torch.empty(nmb * pcme.B_OUT_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    sp = stream.cuda_stream
+    for _ in range(max(1, args.warmup)):
+        B.b_analyse(jobs, res.data_ptr(), sp)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = max(3, args.steps)
+    a.record(stream)
+    for _ in range(reps):
+        B.b_analyse(jobs, res.data_ptr(), sp)
+    b.record(stream)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    got = res.cpu().numpy().view(pcme.B_OUT_DTYPE)
+    o = {"workload": f"B picture {16 * mbw}x{16 * mbh} (frame 1 between 0 and 2), {nmb} MBs: direct_spatial + "
+                     "direct cost, list-0/list-1 16x16 search_part, bi cost, b_partition 16x8/8x16",
+         "mbs": nmb, "ms_per_picture": round(ms, 3), "mblocks_per_s": round(nmb / (ms * 1e-3) / 1e6, 4),
+         "search_part_calls_per_mb": 6, "searches_per_mb": 10}
+    print(f"[bench] B picture: {ms:.3f} ms ({nmb} MBs)", file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        n = nmb
+        mbs = np.zeros(n, [("kind", "i1"), ("ref_idx", "i1", (2, 4)), ("pad", "u1"), ("mv", "<i2", (2, 16, 2))])
+        mbs["kind"] = O.OK_PINTER
+        mbs["ref_idx"] = ref.transpose(1, 2, 0, 3).reshape(n, 2, 4)
+        mbs["mv"] = mv.transpose(1, 2, 0, 3, 4).reshape(n, 2, 16, 2)
+        cols = np.zeros_like(mbs)
+        cols["kind"] = O.OK_PINTER
+        cols["ref_idx"] = col_ref.reshape(n, 2, 4)
+        cols["mv"] = col_mv.reshape(n, 2, 16, 2)
+        assert mbs.dtype.itemsize == ctypes_sizeof_ombinfo2(O)
+        planes = dc.planes[0, :3].cpu().numpy()
+        r0, r1 = O.Ref(planes[0]), O.Ref(planes[2])
+        prm = O.oparams(mbw, mbh, me_range, 1, 1, 0)
+        secs, want = O.b_frame_mt(prm, (O.ombinfo2 * n).from_buffer(mbs), (O.ombinfo2 * n).from_buffer(cols), r0,
+                                  r1, planes[1], pcme.lam(qp), threads)
+        o["cpu_baseline"] = {"value": round(n / secs / 1e6, 5), "unit": "Mblocks/s", "cores": threads, "kind": "port",
+                             "sample": f"the same {n} MBs, MB rows over threads, C oracle -O3 -march=x86-64-v3",
+                             "seconds": round(secs, 3)}
+        o["parity_sample"] = bool(got.tobytes() == want.tobytes())
+    return o
+
+
+def ctypes_sizeof_ombinfo2(O):
+    import ctypes
+    return ctypes.sizeof(O.ombinfo2)
 
 
 def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plain_ms_step):

@@ -169,6 +169,9 @@ def load() -> ctypes.CDLL:
     lib.o_b_analyse.restype = None
     lib.o_b_analyse.argtypes = [c.POINTER(oparams), P, P, c.c_int, c.c_int, c.POINTER(oref), c.POINTER(oref), P,
                                 c.c_uint32, P]
+    lib.o_b_frame_mt.restype = c.c_double
+    lib.o_b_frame_mt.argtypes = [c.POINTER(oparams), P, P, c.POINTER(oref), c.POINTER(oref), P, c.c_int, c.c_int,
+                                 c.c_uint32, c.c_int, P]
     lib.o_lookahead_mt.restype = c.c_double
     lib.o_lookahead_mt.argtypes = [P, c.c_int, c.c_int, c.c_int, c.c_int, P, P]
     _lib = lib
@@ -360,3 +363,14 @@ def b_analyse(prm: oparams, mbs, col: ombinfo2, mx: int, my: int, r0: Ref, r1: R
     load().o_b_analyse(ctypes.byref(prm), ctypes.cast(mbs, ctypes.c_void_p), ctypes.addressof(col), mx, my,
                        ctypes.byref(r0.view), ctypes.byref(r1.view), _p(s), lam, _p(out))
     return out[0]
+
+
+def b_frame_mt(prm: oparams, mbs, cols, r0: Ref, r1: Ref, src_padded: np.ndarray, lam: int, threads: int):
+    """o_b_analyse of every MB of a frame on `threads` threads -> (seconds,
+    [mbh*mbw] OB_DTYPE records)."""
+    out = np.zeros(prm.mbw * prm.mbh, OB_DTYPE)
+    s = np.ascontiguousarray(src_padded, np.uint8)
+    sec = load().o_b_frame_mt(ctypes.byref(prm), ctypes.cast(mbs, ctypes.c_void_p), ctypes.cast(cols, ctypes.c_void_p),
+                              ctypes.byref(r0.view), ctypes.byref(r1.view), _p(s), s.shape[1], LUMA_PAD, lam,
+                              threads, _p(out))
+    return sec, out

@@ -1079,17 +1079,15 @@ static int8_t minp(int8_t x, int8_t y) { return (x >= 0 && y >= 0) ? (x < y ? x 
 
 /* slice.rs:2030-2084 direct_spatial; col = the list-1 picture's MbInfo at
  * (mx, my). */
-void o_direct_spatial(const oparams *p, const ombinfo2 *mbs, const ombinfo2 *col, int mx, int my,
-                      int8_t out_ref[2][4], omv out_mv[2][4]) {
-  const int n = p->mbw * p->mbh;
-  ombinfo *proj = (ombinfo *)malloc(sizeof(ombinfo) * (size_t)n);
+static void direct_spatial_proj(const oparams *p, const ombinfo *const proj2[2], const ombinfo2 *col, int mx,
+                                int my, int8_t out_ref[2][4], omv out_mv[2][4]) {
   ombinfo cur;
   memset(&cur, 0, sizeof cur);
   for (int i = 0; i < 4; i++) cur.ref_idx[i] = -1;
   int8_t refs[2] = {-1, -1};
   omv mvp[2] = {{0, 0}, {0, 0}};
   for (int l = 0; l < 2; l++) {
-    project_list(mbs, n, l, proj);
+    const ombinfo *proj = proj2[l];
     int8_t a = -1, b = -1, c = -1, r;
     omv m;
     if (o_nb_part(p, proj, &cur, 0, mx, my, -1, 0, &r, &m)) a = r;
@@ -1103,12 +1101,8 @@ void o_direct_spatial(const oparams *p, const ombinfo2 *mbs, const ombinfo2 *col
     refs[0] = refs[1] = 0;
   } else {
     for (int l = 0; l < 2; l++)
-      if (refs[l] >= 0) {
-        project_list(mbs, n, l, proj);
-        mvp[l] = o_mvp(p, proj, &cur, 0, mx, my, 0, 0, 4, refs[l], 0);
-      }
+      if (refs[l] >= 0) mvp[l] = o_mvp(p, proj2[l], &cur, 0, mx, my, 0, 0, 4, refs[l], 0);
   }
-  free(proj);
   static const int corner[4] = {0, 3, 12, 15};
   for (int q = 0; q < 4; q++) {
     int col_zero = 0;
@@ -1132,6 +1126,17 @@ void o_direct_spatial(const oparams *p, const ombinfo2 *mbs, const ombinfo2 *col
   }
 }
 
+void o_direct_spatial(const oparams *p, const ombinfo2 *mbs, const ombinfo2 *col, int mx, int my,
+                      int8_t out_ref[2][4], omv out_mv[2][4]) {
+  const int n = p->mbw * p->mbh;
+  ombinfo *proj = (ombinfo *)malloc(sizeof(ombinfo) * 2 * (size_t)n);
+  project_list(mbs, n, 0, proj);
+  project_list(mbs, n, 1, proj + n);
+  const ombinfo *pl[2] = {proj, proj + n};
+  direct_spatial_proj(p, pl, col, mx, my, out_ref, out_mv);
+  free(proj);
+}
+
 /* picture.rs:491-495 avg_into */
 static void avg_into(uint8_t *d, const uint8_t *s, int n) {
   for (int i = 0; i < n; i++) d[i] = (uint8_t)(((uint32_t)d[i] + s[i] + 1) >> 1);
@@ -1145,12 +1150,12 @@ static void avg_into(uint8_t *d, const uint8_t *s, int n) {
  * 8x16 b_partition searches (seeded with the 16x16 MVs) with their
  * per-partition directions and costs.  The caller's decision picks among
  * them (direct, 16x16 dir, partitions when best.cost > 1024). */
-void o_b_analyse(const oparams *p, const ombinfo2 *mbs, const ombinfo2 *col, int mx, int my, const oref *r0,
-                 const oref *r1, const uint8_t *src16, uint32_t lam, ob_out *o) {
+static void b_analyse_proj(const oparams *p, const ombinfo *const proj2[2], const ombinfo2 *col, int mx, int my,
+                           const oref *r0, const oref *r1, const uint8_t *src16, uint32_t lam, ob_out *o) {
   memset(o, 0, sizeof *o);
-  const int n = p->mbw * p->mbh, px = 16 * mx, py = 16 * my;
+  const int px = 16 * mx, py = 16 * my;
   const oref *rp[2] = {r0, r1};
-  o_direct_spatial(p, mbs, col, mx, my, o->direct_ref, o->direct_mv);
+  direct_spatial_proj(p, proj2, col, mx, my, o->direct_ref, o->direct_mv);
   int ok = 1;
   for (int q = 0; q < 4; q++)
     for (int l = 0; l < 2; l++)
@@ -1175,11 +1180,10 @@ void o_b_analyse(const oparams *p, const ombinfo2 *mbs, const ombinfo2 *col, int
     }
     o->direct_cost = o_satd(src16, 16, y, 16, 16, 16) + ((lam * 2) >> 8);
   }
-  ombinfo *proj = (ombinfo *)malloc(sizeof(ombinfo) * (size_t)n);
   omv mvs[4], mvps[4];
   for (int l = 0; l < 2; l++) {
-    project_list(mbs, n, l, proj);
-    o->cost16[l] = o_search_part(p, proj, mx, my, rp[l], OP_16x16, src16, lam, o->direct_mv[l][0], mvs, mvps, NULL);
+    o->cost16[l] = o_search_part(p, proj2[l], mx, my, rp[l], OP_16x16, src16, lam, o->direct_mv[l][0], mvs, mvps,
+                                 NULL);
     o->mv16[l] = mvs[0];
     o->mvp16[l] = mvps[0];
   }
@@ -1197,8 +1201,7 @@ void o_b_analyse(const oparams *p, const ombinfo2 *mbs, const ombinfo2 *col, int
   for (int k = 0; k < 2; k++) {
     const int part = k == 0 ? OP_16x8 : OP_8x16;
     for (int l = 0; l < 2; l++) {
-      project_list(mbs, n, l, proj);
-      o_search_part(p, proj, mx, my, rp[l], part, src16, lam, o->mv16[l], mvs, mvps, NULL);
+      o_search_part(p, proj2[l], mx, my, rp[l], part, src16, lam, o->mv16[l], mvs, mvps, NULL);
       o->part_mv[k][l][0] = mvs[0];
       o->part_mv[k][l][1] = mvs[1];
     }
@@ -1230,5 +1233,67 @@ void o_b_analyse(const oparams *p, const ombinfo2 *mbs, const ombinfo2 *col, int
     }
     o->part_cost[k] = cost;
   }
+}
+
+void o_b_analyse(const oparams *p, const ombinfo2 *mbs, const ombinfo2 *col, int mx, int my, const oref *r0,
+                 const oref *r1, const uint8_t *src16, uint32_t lam, ob_out *o) {
+  const int n = p->mbw * p->mbh;
+  ombinfo *proj = (ombinfo *)malloc(sizeof(ombinfo) * 2 * (size_t)n);
+  project_list(mbs, n, 0, proj);
+  project_list(mbs, n, 1, proj + n);
+  const ombinfo *pl[2] = {proj, proj + n};
+  b_analyse_proj(p, pl, col, mx, my, r0, r1, src16, lam, o);
   free(proj);
+}
+
+/* The CPU baseline of the B search: o_b_analyse of every MB of a frame
+ * (source plane padded like the references; cols[mb] the co-located MBs),
+ * MB rows split over `threads` threads.  Returns seconds of wall time. */
+typedef struct {
+  const oparams *p;
+  const ombinfo *proj[2];
+  const ombinfo2 *cols;
+  const oref *r0, *r1;
+  const uint8_t *src;
+  int sstride, pad, threads, t;
+  uint32_t lam;
+  ob_out *out;
+} o_b_arg;
+
+static void *o_b_worker(void *a_) {
+  o_b_arg *a = (o_b_arg *)a_;
+  for (int my = a->t; my < a->p->mbh; my += a->threads) {
+    oparams q = *a->p;
+    q.vlim = o_vlim(my, q.mbh);
+    for (int mx = 0; mx < q.mbw; mx++) {
+      uint8_t s16[256];
+      for (int r = 0; r < 16; r++)
+        memcpy(s16 + 16 * r, a->src + (size_t)(a->pad + 16 * my + r) * a->sstride + a->pad + 16 * mx, 16);
+      b_analyse_proj(&q, a->proj, &a->cols[my * q.mbw + mx], mx, my, a->r0, a->r1, s16, a->lam,
+                     &a->out[my * q.mbw + mx]);
+    }
+  }
+  return NULL;
+}
+
+double o_b_frame_mt(const oparams *p, const ombinfo2 *mbs, const ombinfo2 *cols, const oref *r0, const oref *r1,
+                    const uint8_t *src, int sstride, int pad, uint32_t lam, int threads, ob_out *out) {
+  struct timespec t0, t1;
+  const int n = p->mbw * p->mbh;
+  ombinfo *proj = (ombinfo *)malloc(sizeof(ombinfo) * 2 * (size_t)n);
+  project_list(mbs, n, 0, proj);
+  project_list(mbs, n, 1, proj + n);
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  if (threads < 1) threads = 1;
+  if (threads > 64) threads = 64;
+  pthread_t th[64];
+  o_b_arg args[64];
+  for (int t = 0; t < threads; t++) {
+    args[t] = (o_b_arg){p, {proj, proj + n}, cols, r0, r1, src, sstride, pad, threads, t, lam, out};
+    pthread_create(&th[t], NULL, o_b_worker, &args[t]);
+  }
+  for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  free(proj);
+  return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 }

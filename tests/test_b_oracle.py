@@ -197,3 +197,30 @@ def test_b_analyse_costs_match_the_definitions(oracle, seed):
                     cost += min(cs)
                 assert r["part_cost"][k] == cost
     assert n_direct > 0
+
+
+def test_b_frame_driver_equals_per_mb_calls(oracle):
+    """o_b_frame_mt (one projection per frame, rows over threads) gives the
+    per-MB o_b_analyse records."""
+    O = oracle
+    rs = np.random.default_rng(5)
+    w, h = 64, 48
+    mbw, mbh = w // 16, h // 16
+    import pcme
+    y, _, _ = pcme.synth_clip(pcme.ClipSpec(w, h, 3, seed=0x5EED1900, nrect=2, rect_lo=16, rect_hi=32, vmax=3.0))
+    src = O.pad_luma(y[1], mbw, mbh)
+    refs = [O.Ref(O.pad_luma(y[0], mbw, mbh)), O.Ref(O.pad_luma(y[2], mbw, mbh))]
+    lam = O.load().o_lambda(LAM_QP)
+    mbs = (O.ombinfo2 * (mbw * mbh))()
+    cols = (O.ombinfo2 * (mbw * mbh))()
+    for i in range(mbw * mbh):
+        _rand_mbinfo2(O, mbs[i], rs)
+        _rand_mbinfo2(O, cols[i], rs)
+    prm = O.oparams(mbw, mbh, 16, 1, 1, 0)
+    _, got = O.b_frame_mt(prm, mbs, cols, refs[0], refs[1], src, lam, 3)
+    for my in range(mbh):
+        p1 = O.oparams(mbw, mbh, 16, 1, 1, O.load().o_vlim(my, mbh))
+        for mx in range(mbw):
+            s16 = np.ascontiguousarray(src[64 + 16 * my:80 + 16 * my, 64 + 16 * mx:80 + 16 * mx])
+            want = O.b_analyse(p1, mbs, cols[my * mbw + mx], mx, my, refs[0], refs[1], s16, lam)
+            assert got[my * mbw + mx].tobytes() == want.tobytes(), (mx, my)
