@@ -371,9 +371,14 @@ def lookahead_leg(args, pcme, torch, clips, stream, threads):
         dc, lr = clips[-1], lrs[-1]
         k = min(args.cpu_frames + 1, dc.n)
         frames = lr[:k].cpu().numpy()
-        secs, csums, cblocks = O.lookahead_mt(frames, threads)
+        # repeated until about a second of wall time (~16 s of CPU work)
+        secs, reps = 0.0, 0
+        while secs < 1.0 and reps < 50:
+            s1, csums, cblocks = O.lookahead_mt(frames, threads)
+            secs += s1
+            reps += 1
         nb = dc.mbw * dc.mbh
-        cpu_val = (k - 1) * nb / secs / 1e6
+        cpu_val = reps * (k - 1) * nb / secs / 1e6
         gs = sums[-1][:k].cpu().numpy().astype(np.uint64)
         gb = tblocks[-1][:k].cpu().numpy().view(pcme.TREE_BLOCK_DTYPE).reshape(k, nb)
         parity = bool(np.array_equal(gs[1:], csums[1:]) and
@@ -381,7 +386,7 @@ def lookahead_leg(args, pcme, torch, clips, stream, threads):
         out["cpu_baseline"] = {"value": round(cpu_val, 5), "unit": "Mblocks/s", "cores": threads, "kind": "port",
                                "sample": f"complexity + mbtree analyse of {k - 1} lowres frames "
                                          f"({8 * dc.mbw}x{8 * dc.mbh}) of the last clip, one frame per thread, "
-                                         f"C oracle -O3 -march=x86-64-v3",
+                                         f"C oracle -O3 -march=x86-64-v3, {reps} repetitions",
                                "seconds": round(secs, 3)}
         out["parity_sample"] = parity
     return out
@@ -442,10 +447,16 @@ def b_leg(args, pcme, torch, dc, out, qp, me_range, stream, threads):
         planes = dc.planes[0, :3].cpu().numpy()
         r0, r1 = O.Ref(planes[0]), O.Ref(planes[2])
         prm = O.oparams(mbw, mbh, me_range, 1, 1, 0)
-        secs, want = O.b_frame_mt(prm, (O.ombinfo2 * n).from_buffer(mbs), (O.ombinfo2 * n).from_buffer(cols), r0,
-                                  r1, planes[1], pcme.lam(qp), threads)
-        o["cpu_baseline"] = {"value": round(n / secs / 1e6, 5), "unit": "Mblocks/s", "cores": threads, "kind": "port",
-                             "sample": f"the same {n} MBs, MB rows over threads, C oracle -O3 -march=x86-64-v3",
+        secs, reps = 0.0, 0
+        while secs < 1.0 and reps < 50:
+            s1, want = O.b_frame_mt(prm, (O.ombinfo2 * n).from_buffer(mbs), (O.ombinfo2 * n).from_buffer(cols), r0,
+                                    r1, planes[1], pcme.lam(qp), threads)
+            secs += s1
+            reps += 1
+        o["cpu_baseline"] = {"value": round(reps * n / secs / 1e6, 5), "unit": "Mblocks/s", "cores": threads,
+                             "kind": "port",
+                             "sample": f"the same {n} MBs, MB rows over threads, C oracle -O3 -march=x86-64-v3, "
+                                       f"{reps} repetitions",
                              "seconds": round(secs, 3)}
         o["parity_sample"] = bool(got.tobytes() == want.tobytes())
     return o

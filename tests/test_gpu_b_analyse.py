@@ -99,3 +99,46 @@ def test_b_analyse_batch_matches_oracle(gpu, oracle, seed):
     for k, (g, wnt) in enumerate(zip(got, want)):
         for f in pcme.B_OUT_DTYPE.names:
             assert np.array_equal(g[f], wnt[f]), (k, divmod(k, mbw)[::-1], f, g[f], wnt[f])
+
+
+def test_b_picture_through_the_job_builder(gpu, oracle):
+    """A whole 352x288 B picture: pcme.bsearch.b_jobs from random committed
+    fields (intra MBs, refs -1/0/1 per list, MVs within +-16 px) and
+    co-located MBs, against the oracle's o_b_frame_mt on the same fields."""
+    import torch
+    from pcme import bsearch as B
+    pcme, O = gpu, oracle
+    w, h, qp = 352, 288, 32
+    mbw, mbh = pcme.mb_dims(w, h)
+    say("b picture 352x288")
+    y, _, _ = pcme.synth_clip(pcme.ClipSpec(w, h, 3, seed=0x5EED1A00, nrect=5, vmax=6.0))
+    dc = pcme.DeviceClip(y, mbw, mbh)
+    dc.pad()
+    dc.hpel()
+    rs = np.random.default_rng(77)
+    mv = rs.integers(-64, 65, (2, mbh, mbw, 16, 2)).astype(np.int16)
+    ref = rs.choice([-1, 0, 0, 0, 1], (2, mbh, mbw, 4)).astype(np.int8)
+    intra = rs.random((mbh, mbw)) < 0.1
+    col_mv = rs.integers(-2, 3, (mbh, mbw, 2, 16, 2)).astype(np.int16)
+    col_ref = rs.choice([-1, 0, 1], (mbh, mbw, 2, 4)).astype(np.int8)
+    col_intra = rs.random((mbh, mbw)) < 0.1
+    jobs = B.b_jobs(dc, 1, (dc, 0), (dc, 2), mv, ref, intra, col_mv, col_ref, col_intra, qp, 24, 1)
+    n = mbw * mbh
+    out = torch.zeros(n * pcme.B_OUT_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    B.b_analyse(jobs, out.data_ptr())
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().view(pcme.B_OUT_DTYPE)
+    mbs = np.zeros(n, [("kind", "i1"), ("ref_idx", "i1", (2, 4)), ("pad", "u1"), ("mv", "<i2", (2, 16, 2))])
+    mbs["kind"] = np.where(intra.reshape(n), O.OK_I16x16, O.OK_PINTER)
+    mbs["ref_idx"] = ref.transpose(1, 2, 0, 3).reshape(n, 2, 4)
+    mbs["mv"] = mv.transpose(1, 2, 0, 3, 4).reshape(n, 2, 16, 2)
+    cols = np.zeros_like(mbs)
+    cols["kind"] = np.where(col_intra.reshape(n), O.OK_I16x16, O.OK_PINTER)
+    cols["ref_idx"] = col_ref.reshape(n, 2, 4)
+    cols["mv"] = col_mv.reshape(n, 2, 16, 2)
+    planes = dc.planes[0, :3].cpu().numpy()
+    prm = O.oparams(mbw, mbh, 24, 1, 1, 0)
+    _, want = O.b_frame_mt(prm, (O.ombinfo2 * n).from_buffer(mbs), (O.ombinfo2 * n).from_buffer(cols),
+                           O.Ref(planes[0]), O.Ref(planes[2]), planes[1], O.load().o_lambda(qp), 4)
+    for k in range(n):
+        assert got[k].tobytes() == want[k].tobytes(), (k, divmod(k, mbw)[::-1])
