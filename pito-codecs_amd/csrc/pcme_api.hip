@@ -593,7 +593,13 @@ int pcme_b_analyse_batch(const pcme_b_job *jobs, int njobs, pcme_b_out *out, voi
   std::lock_guard<std::mutex> g(dv->mu);
   if ((rc = grow(&dv->dbj, &dv->dbj_cap, (size_t)njobs))) return rc;
   if ((rc = upload(dv, dv->dbj, jobs, sizeof(pcme_b_job) * (size_t)njobs, (hipStream_t)stream))) return rc;
-  hipLaunchKernelGGL(b_analyse_batch_kernel, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream, dv->dbj,
+  hipLaunchKernelGGL(b_analyse_16_kernel, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream, dv->dbj, njobs,
+                     out);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(b_analyse_part_kernel<0>, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream, dv->dbj,
+                     njobs, out);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(b_analyse_part_kernel<1>, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream, dv->dbj,
                      njobs, out);
   HIPCHK(hipGetLastError());
   return PCME_OK;

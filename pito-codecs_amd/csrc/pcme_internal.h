@@ -45,7 +45,9 @@ __global__ void search_frames_hinted_kernel(const DevJob *__restrict__ jobs, con
 __global__ void search_batch_kernel(const pcme_search_job *__restrict__ jobs, int njobs,
                                     pcme_mv *out_mv, uint32_t *out_cost);
 __global__ void search_part_batch_kernel(const pcme_part_job *__restrict__ jobs, int njobs, pcme_part_out *out);
-__global__ void b_analyse_batch_kernel(const pcme_b_job *__restrict__ jobs, int njobs, pcme_b_out *out);
+__global__ void b_analyse_16_kernel(const pcme_b_job *__restrict__ jobs, int njobs, pcme_b_out *out);
+template <int K>
+__global__ void b_analyse_part_kernel(const pcme_b_job *__restrict__ jobs, int njobs, pcme_b_out *out);
 __global__ void pad_planes_kernel(const uint8_t *__restrict__ src, int64_t src_pitch, int sstride, int w,
                                   int h, uint8_t *__restrict__ dst, int64_t dst_pitch, int dstride,
                                   int rows, int pad);

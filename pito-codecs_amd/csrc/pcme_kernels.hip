@@ -1495,17 +1495,8 @@ __device__ __forceinline__ void avg_buf(uint8_t *d, const uint8_t *a, const uint
   for (int i = (int)lane_id(); i < 256; i += 64) d[i] = (uint8_t)(((uint32_t)a[i] + b[i] + 1u) >> 1);
 }
 
-// analyse_b's motion search (slice.rs:2189-2377 without BSkip, RD and
-// intra) and b_partition (slice.rs:2086-2187) for one MB per wave.
-__global__ void __launch_bounds__(64) b_analyse_batch_kernel(const pcme_b_job *__restrict__ jobs, int njobs,
-                                                             pcme_b_out *out) {
-  __shared__ __attribute__((aligned(16))) uint32_t s_lds[LDS_PER_WAVE / 4];
-  __shared__ __attribute__((aligned(16))) uint8_t s_pred[3][256];
-  const int j = blockIdx.x;
-  if (j >= njobs) return;
-  const pcme_b_job &J = jobs[j];
-  const int lane = (int)lane_id();
-  SCtx c[2];
+// Scalar context of one B job.
+__device__ __forceinline__ void b_ctx(const pcme_b_job &J, SCtx *c) {
 #pragma unroll
   for (int l = 0; l < 2; l++) {
 #pragma unroll
@@ -1518,6 +1509,22 @@ __global__ void __launch_bounds__(64) b_analyse_batch_kernel(const pcme_b_job *_
     c[l].range = J.me_range;
     c[l].subpel = J.subpel_rounds;
   }
+}
+
+// analyse_b's motion search (slice.rs:2189-2377 without BSkip, RD and
+// intra), first half: direct_spatial and the direct prediction's cost, the
+// list-0/list-1 16x16 search_part and the direction costs.  Writes dwords
+// 0-21 of the record.  One MB per wave.
+__global__ void __launch_bounds__(64) b_analyse_16_kernel(const pcme_b_job *__restrict__ jobs, int njobs,
+                                                          pcme_b_out *out) {
+  __shared__ __attribute__((aligned(16))) uint32_t s_lds[LDS_PER_WAVE / 4];
+  __shared__ __attribute__((aligned(16))) uint8_t s_pred[3][256];
+  const int j = blockIdx.x;
+  if (j >= njobs) return;
+  const pcme_b_job &J = jobs[j];
+  const int lane = (int)lane_id();
+  SCtx c[2];
+  b_ctx(J, c);
   const NbrG n[2] = {nbr_of(J.nb[0]), nbr_of(J.nb[1])};
   const int px = 16 * J.mx, py = 16 * J.my;
   const uint32_t lam = J.lam;
@@ -1600,54 +1607,7 @@ __global__ void __launch_bounds__(64) b_analyse_batch_kernel(const pcme_b_job *_
   cost[1] = r16[1].total + ((lam * 3u) >> 8);
   cost[2] = satd_rect<16, 16>(J.src, J.sstride, s_pred[2], 0, 0) + ((lam * (bits + 5u)) >> 8);
   const uint32_t dir = cost[1] < cost[0] ? (cost[2] < cost[1] ? 2u : 1u) : (cost[2] < cost[0] ? 2u : 0u);
-  // b_partition 16x8 and 8x16 (slice.rs:2086-2187)
-  uint32_t pmv[2][2][2], pcost[2], pdir[2][2];
-#pragma unroll
-  for (int k = 0; k < 2; k++) {
-    PartRes rp[2];
-#pragma unroll
-    for (int l = 0; l < 2; l++) {
-      if (k == 0)
-        search_part<1, false>(c[l], n[l], J.src, J.sstride, px, py, r16[l].mv[0], 0u, -1, rp[l], s_lds);
-      else
-        search_part<2, false>(c[l], n[l], J.src, J.sstride, px, py, r16[l].mv[0], 0u, -1, rp[l], s_lds);
-      pmv[k][l][0] = rp[l].mv[0];
-      pmv[k][l][1] = rp[l].mv[1];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int l = 0; l < 2; l++)
-#pragma unroll
-      for (int pi = 0; pi < 2; pi++) {
-        if (k == 0)
-          mc_rect(c[l], px, py, 0, 8 * pi, 16, 8, rp[l].mv[pi], s_pred[l]);
-        else
-          mc_rect(c[l], px, py, 8 * pi, 0, 8, 16, rp[l].mv[pi], s_pred[l]);
-      }
-    __syncthreads();
-    avg_buf(s_pred[2], s_pred[0], s_pred[1]);
-    __syncthreads();
-    uint32_t tot = (lam * 8u) >> 8;
-#pragma unroll
-    for (int pi = 0; pi < 2; pi++) {
-      uint32_t best = 0xFFFFFFFFu, bd = 0;
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        const uint32_t sd = k == 0 ? satd_rect<16, 8>(J.src, J.sstride, s_pred[d], 0, 8 * pi)
-                                   : satd_rect<8, 16>(J.src, J.sstride, s_pred[d], 8 * pi, 0);
-        const uint32_t cd = sd + ((lam * (d == 2 ? 14u : 7u)) >> 8);
-        if (cd < best) {
-          best = cd;
-          bd = (uint32_t)d;
-        }
-      }
-      tot += best;
-      pdir[k][pi] = bd;
-    }
-    pcost[k] = tot;
-    __syncthreads();
-  }
-  // the record: 34 dwords, one per lane
+  // dwords 0-21 of the record, one per lane
   uint32_t w = 0;
   if (lane < 8) {
     const int l = lane >> 2, q = lane & 3;
@@ -1658,40 +1618,93 @@ __global__ void __launch_bounds__(64) b_analyse_batch_kernel(const pcme_b_job *_
       for (int qq = 0; qq < 4; qq++) v = (l == ll && q == qq) ? dmv[ll][qq] : v;
     w = v;
   } else if (lane == 8 || lane == 9) {
-    const int l = lane - 8;
-    const uint32_t r = (uint32_t)(uint8_t)(int8_t)refs[l] * 0x01010101u;
-    w = r;
+    w = (uint32_t)(uint8_t)(int8_t)(lane == 8 ? refs[0] : refs[1]) * 0x01010101u;
   } else if (lane == 10) {
     w = direct_ok ? 1u : 0u;
   } else if (lane == 11) {
     w = direct_cost;
   } else if (lane == 12 || lane == 13) {
-    w = r16[lane - 12].mv[0];
+    w = lane == 12 ? r16[0].mv[0] : r16[1].mv[0];
   } else if (lane == 14 || lane == 15) {
-    w = r16[lane - 14].mvp[0];
+    w = lane == 14 ? r16[0].mvp[0] : r16[1].mvp[0];
   } else if (lane == 16 || lane == 17) {
-    w = r16[lane - 16].total;
+    w = lane == 16 ? r16[0].total : r16[1].total;
   } else if (lane >= 18 && lane <= 20) {
-    w = cost[lane - 18];
+    w = lane == 18 ? cost[0] : (lane == 19 ? cost[1] : cost[2]);
   } else if (lane == 21) {
     w = dir;
-  } else if (lane >= 22 && lane <= 29) {
-    const int i = lane - 22, k = i >> 2, l = (i >> 1) & 1, pi = i & 1;
-    uint32_t v = pmv[0][0][0];
-#pragma unroll
-    for (int kk = 0; kk < 2; kk++)
-#pragma unroll
-      for (int ll = 0; ll < 2; ll++)
-#pragma unroll
-        for (int pp = 0; pp < 2; pp++) v = (k == kk && l == ll && pi == pp) ? pmv[kk][ll][pp] : v;
-    w = v;
-  } else if (lane == 30 || lane == 31) {
-    w = pcost[lane - 30];
-  } else if (lane == 32) {
-    w = pdir[0][0] | (pdir[0][1] << 8) | (pdir[1][0] << 16) | (pdir[1][1] << 24);
   }
-  if (lane < 34) ((uint32_t *)&out[j])[lane] = w;
+  if (lane < 22 || lane == 33) ((uint32_t *)&out[j])[lane] = w;  // dword 33: reserved, zero
 }
+
+// Second half: b_partition (slice.rs:2086-2187) of 16x8 (K = 0) or 8x16
+// (K = 1), seeded with the 16x16 MVs the first half wrote (dwords 12-13 of
+// the record).  Writes part_mv[K], part_cost[K] and part_dir[K].  One MB
+// per wave.
+template <int K>
+__global__ void __launch_bounds__(64) b_analyse_part_kernel(const pcme_b_job *__restrict__ jobs, int njobs,
+                                                            pcme_b_out *out) {
+  __shared__ __attribute__((aligned(16))) uint32_t s_lds[LDS_PER_WAVE / 4];
+  __shared__ __attribute__((aligned(16))) uint8_t s_pred[3][256];
+  const int j = blockIdx.x;
+  if (j >= njobs) return;
+  const pcme_b_job &J = jobs[j];
+  const int lane = (int)lane_id();
+  SCtx c[2];
+  b_ctx(J, c);
+  const NbrG n[2] = {nbr_of(J.nb[0]), nbr_of(J.nb[1])};
+  const int px = 16 * J.mx, py = 16 * J.my;
+  const uint32_t lam = J.lam;
+  const uint32_t *rec = (const uint32_t *)&out[j];
+  const uint32_t mv16[2] = {rdlane(rec[12 + (lane & 1)], 0), rdlane(rec[12 + (lane & 1)], 1)};
+  uint32_t pmv[2][2];
+  PartRes rp[2];
+#pragma unroll
+  for (int l = 0; l < 2; l++) {
+    search_part<K + 1, false>(c[l], n[l], J.src, J.sstride, px, py, mv16[l], 0u, -1, rp[l], s_lds);
+    pmv[l][0] = rp[l].mv[0];
+    pmv[l][1] = rp[l].mv[1];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int l = 0; l < 2; l++)
+#pragma unroll
+    for (int pi = 0; pi < 2; pi++) {
+      if (K == 0)
+        mc_rect(c[l], px, py, 0, 8 * pi, 16, 8, rp[l].mv[pi], s_pred[l]);
+      else
+        mc_rect(c[l], px, py, 8 * pi, 0, 8, 16, rp[l].mv[pi], s_pred[l]);
+    }
+  __syncthreads();
+  avg_buf(s_pred[2], s_pred[0], s_pred[1]);
+  __syncthreads();
+  uint32_t tot = (lam * 8u) >> 8, pdir[2];
+#pragma unroll
+  for (int pi = 0; pi < 2; pi++) {
+    uint32_t best = 0xFFFFFFFFu, bd = 0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const uint32_t sd = K == 0 ? satd_rect<16, 8>(J.src, J.sstride, s_pred[d], 0, 8 * pi)
+                                 : satd_rect<8, 16>(J.src, J.sstride, s_pred[d], 8 * pi, 0);
+      const uint32_t cd = sd + ((lam * (d == 2 ? 14u : 7u)) >> 8);
+      if (cd < best) {
+        best = cd;
+        bd = (uint32_t)d;
+      }
+    }
+    tot += best;
+    pdir[pi] = bd;
+  }
+  uint32_t *o = (uint32_t *)&out[j];
+  // part_mv[K][l][pi] = dword 22 + 4K + 2l + pi
+  const uint32_t v = lane == 0 ? pmv[0][0] : (lane == 1 ? pmv[0][1] : (lane == 2 ? pmv[1][0] : pmv[1][1]));
+  if (lane < 4) o[22 + 4 * K + lane] = v;
+  if (lane == 4) o[30 + K] = tot;
+  if (lane == 5 || lane == 6) ((uint8_t *)o)[128 + 2 * K + (lane - 5)] = (uint8_t)pdir[lane - 5];
+}
+
+template __global__ void b_analyse_part_kernel<0>(const pcme_b_job *__restrict__, int, pcme_b_out *);
+template __global__ void b_analyse_part_kernel<1>(const pcme_b_job *__restrict__, int, pcme_b_out *);
 
 // -------------------------------------------------- per-call search batch
 
