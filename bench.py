@@ -233,6 +233,32 @@ def main():
                  "mblocks_per_s": round(mbs_per_step * args.steps / gdt / 1e6, 4)}
         print(f"[bench] graph replay {graph['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
 
+    # The wavefront's dependency bound: one frame alone (one walker per MB
+    # row, nothing else on the GPU) runs its critical path of
+    # mbw + 2*(mbh-1) MB analyses back to back; every frame of the step has
+    # the same path, so that time is the step's lower bound with unlimited
+    # parallel hardware.
+    crit = None
+    if args.workload in ("1080p", "4k"):
+        p1 = pcme.Plan(jobs[:1])
+        p1.launch(sp)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(5):
+            p1.launch(sp)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        p1.check(sp)
+        one_ms = e0.elapsed_time(e1) / 5
+        dc0 = clips[0]
+        path = dc0.mbw + 2 * (dc0.mbh - 1)
+        crit = {"single_frame_ms": round(one_ms, 3), "path_mbs": path,
+                "mb_latency_us": round(1e3 * one_ms / path, 2), "frames_per_step": len(jobs),
+                "search_ms": round(search_ms, 3), "frac": round(one_ms / search_ms, 4)}
+        print(f"[bench] one frame alone {one_ms:.2f} ms ({1e3 * one_ms / path:.1f} us per MB on the critical path)",
+              file=sys.stderr, flush=True)
+        p1.close()
     achieved_gbs = BYTES_PER_MB * mbs_per_step / (search_ms * 1e-3) / 1e9
     wl = {"1080p": "plain GPU ME-only search, 1920x1080 60 frames IPPP, 1 ref, QP 27, me_range 32 + qpel, "
                    "partitions 16x16/16x8/8x16/8x8, 12 rects 64-400 px, v in [-8,8] px/frame",
@@ -256,6 +282,7 @@ def main():
                      "kernel_ms": round(search_ms, 3)},
         "autotune": tune,
         "graph": graph,
+        "critical_path": crit,
     }
 
     if not args.no_pruned:
