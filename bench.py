@@ -64,9 +64,9 @@ def parse():
     ap.add_argument("--no-b", action="store_true", help="skip the B-picture search leg")
     ap.add_argument("--ncand", type=int, default=5, help="pruner candidates per MB")
     ap.add_argument("--weights", default=None, help="pruner weights (.npz; default: the committed ones)")
-    ap.add_argument("--thr", type=float, default=0.6, help="pruner confidence threshold (partition head)")
-    ap.add_argument("--thr-mv", type=float, default=None,
-                    help="threshold on the motion head's candidate mass (default: --thr)")
+    ap.add_argument("--thr", type=float, default=0.7, help="pruner confidence threshold (partition head)")
+    ap.add_argument("--thr-mv", type=float, default=0.0,
+                    help="threshold on the motion head's candidate mass (0: the partition head decides)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known "
                          "traffic, for scaling the FETCH_SIZE/WRITE_SIZE counters)")
@@ -504,7 +504,7 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
     if not os.path.exists(wpath):
         return {"skipped": "no trained weights (pcme/pruner_weights.npz)"}
     sp = stream.cuda_stream
-    thr_mv = args.thr if args.thr_mv is None else args.thr_mv
+    thr_mv = args.thr_mv
     nets, jobs, pouts, hints, groups = {}, [], [], [], {}
     for dc, (spec, qp, rng) in zip(clips, specs):
         if qp not in nets:

@@ -5,7 +5,7 @@ pcme.rdmodel, on a held-out 1080p clip at QP 22, 27, 32 and 37.
 
 For each QP the same frames are searched three ways: the plain search, the
 pruned search with that QP's network (5 candidates; full search where
-either head's confidence is below 0.6) and the same pruned search with empty hints (the
+the partition head's confidence is below 0.7) and the same pruned search with empty hints (the
 method without the network).  Every field is then coded by the model --
 motion-compensated prediction of its chosen partitions (pcme_pred_frame),
 luma residual through the H.264 4x4 transform and quantiser, run/level and
@@ -58,7 +58,7 @@ def rd_point(dc: DeviceClip, field: np.ndarray, qp: int) -> tuple[float, float]:
     return bits, R.psnr(sse, npix)
 
 
-def evaluate(frames: int, seed: int, ncand: int = 5, thr_part: float = 0.6, thr_mv: float = 0.6,
+def evaluate(frames: int, seed: int, ncand: int = 5, thr_part: float = 0.7, thr_mv: float = 0.0,
              log=print, w: int = 1920, h: int = 1080, weights: str | None = None) -> dict:
     y, u, v = synth_clip(clip_spec(seed, frames + 1, w, h))
     mbw, mbh = mb_dims(w, h)
@@ -96,8 +96,8 @@ def main(argv=None) -> int:
     ap.add_argument("--frames", type=int, default=16, help="P frames per QP")
     ap.add_argument("--seed", type=lambda s: int(s, 0), default=0x5EED9200)
     ap.add_argument("--ncand", type=int, default=5)
-    ap.add_argument("--thr", type=float, default=0.6, help="partition-head confidence threshold")
-    ap.add_argument("--thr-mv", type=float, default=0.6, help="motion-head candidate-mass threshold")
+    ap.add_argument("--thr", type=float, default=0.7, help="partition-head confidence threshold")
+    ap.add_argument("--thr-mv", type=float, default=0.0, help="motion-head candidate-mass threshold")
     ap.add_argument("--weights", default=None, help="pruner weights (.npz; default: the committed ones)")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)

@@ -107,7 +107,7 @@ class Net:
         _check(_lib().pcme_net_train_step(self.handle, x.data_ptr(), mv.data_ptr(), part.data_ptr(),
                                           idx.data_ptr(), int(n), float(lr), loss.data_ptr(), _stream_ptr(stream)))
 
-    def hints(self, jobs: list[FrameJob], ncand: int = 8, thr_part: float = 0.6, thr_mv: float = 0.6,
+    def hints(self, jobs: list[FrameJob], ncand: int = 8, thr_part: float = 0.7, thr_mv: float = 0.0,
               stream=None) -> None:
         arr = (FrameJob * len(jobs))(*jobs)
         _check(_lib().pcme_net_hints(self.handle, arr, len(jobs), int(ncand), float(thr_part), float(thr_mv),
@@ -225,7 +225,7 @@ def train(net: Net, data: TrainSet, epochs: int = 3, batch: int = 1024, lr: floa
 # ------------------------------------------------------- pruned search
 
 def search_clip(clip: DeviceClip, me_range: int, qp: int, net: Net | None = None, ncand: int = 8,
-                thr_part: float = 0.6, thr_mv: float = 0.6, stream=None, prepared: bool = False,
+                thr_part: float = 0.7, thr_mv: float = 0.0, stream=None, prepared: bool = False,
                 hints_in: np.ndarray | None = None):
     """ME-only field of every P frame of a device clip: the full search
     (net None), the pruned search with the network's hints, or with given

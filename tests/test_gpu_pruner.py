@@ -202,17 +202,19 @@ def test_pruned_search_without_confidence_is_the_full_search(gpu):
     np.testing.assert_array_equal(field, plain)
 
 
-def test_network_gain_over_the_search_without_it(gpu):
+@pytest.mark.parametrize("thr_part,thr_mv", [(0.7, 0.0), (0.6, 0.6)])
+def test_network_gain_over_the_search_without_it(gpu, thr_part, thr_mv):
     """The product path on a held-out 640x368 clip with the committed
-    weights: the network's hints keep the J-cost within 1% of the plain
+    weights, under the default rule (partition head at 0.7) and under both
+    heads at 0.6: the network's hints keep the J-cost within 1% of the plain
     search and lose several times less than the same pruned search given no
     hints (every MB pruned, no candidates, P16x16 only)."""
     import pcme.pruner as P
-    say("pruner gain 640x368 x 3 P frames, committed weights")
+    say("pruner gain 640x368 x 3 P frames, committed weights", thr_part, thr_mv)
     _, dc = _dclip(gpu, 640, 368, 4, 0x5EED9101, nrect=8, rect_lo=24, rect_hi=120, vmax=6.0, pan_max=1.0)
     net = P.Net(seed=1, max_batch=64, weights=P.WEIGHTS, qp=27)
     plain, _ = P.search_clip(dc, 32, 27, prepared=True)
-    cnn, hints = P.search_clip(dc, 32, 27, net=net, ncand=5, thr_part=0.6, thr_mv=0.6, prepared=True)
+    cnn, hints = P.search_clip(dc, 32, 27, net=net, ncand=5, thr_part=thr_part, thr_mv=thr_mv, prepared=True)
     none = np.zeros(hints.shape, gpu.HINT_DTYPE)
     none["prune"] = 1
     bare, _ = P.search_clip(dc, 32, 27, prepared=True, hints_in=none)
