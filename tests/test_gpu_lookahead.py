@@ -166,3 +166,46 @@ def test_mbtree_analyse_edge_content(gpu, oracle):
     for n_, (c, a, b) in enumerate(triples):
         want = oracle.tree_analyse(frames[c], None if a is None else frames[a], None if b is None else frames[b])
         _check_tree(got[n_], want, (c, a, b))
+
+
+def test_lookahead_replays_in_a_hip_graph(gpu, oracle):
+    """pcme_lowres + pcme_complexity + pcme_mbtree_analyse captured once in a
+    graph and replayed three times, with eager work and synchronisation
+    between the replays; every replay gives the eager (and oracle) result,
+    including the complexity sums the clear kernel resets."""
+    import torch
+    from pcme import TREE_BLOCK_DTYPE
+    from pcme import lookahead as L
+    say("lookahead graph replay")
+    dc = _device_clip(gpu, 320, 240, 4, 0x5EED1B00, nrect=4, vmax=5.0)
+    lr = L.lowres_frames(dc)
+    pairs = [(0, None)] + [(t, t - 1) for t in range(1, 4)]
+    triples = [(0, None, None)] + [(t, t - 1, None) for t in range(1, 4)]
+    sums = torch.zeros((4, 2), dtype=torch.int64, device="cuda")
+    blocks = torch.zeros((4, 20 * 15 * TREE_BLOCK_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+    cj = L.complexity_jobs(lr, pairs, sums)
+    tj = L.tree_jobs(lr, triples, blocks)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s, capture_error_mode="thread_local"):
+            import pcme
+            pcme._check(pcme.load().pcme_lowres(dc.plane_ptr(0, 0), dc.pitch, dc.stride, pcme.LUMA_PAD, 320, 240,
+                                                lr.data_ptr(), 160 * 120, 4, s.cuda_stream))
+            L.run_complexity(cj, s.cuda_stream)
+            L.run_analyse(tj, s.cuda_stream)
+    torch.cuda.synchronize()
+    lrn = lr.cpu().numpy()
+    want_sums = np.array([oracle.complexity(lrn[c], None if p is None else lrn[p])[:2] for c, p in pairs])
+    want_blocks = [oracle.tree_analyse(lrn[c], None if a is None else lrn[a], None) for c, a, _ in triples]
+    for rep in range(3):
+        sums.fill_(-1)
+        blocks.fill_(0xAB)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(sums.cpu().numpy().astype(np.uint64), want_sums.astype(np.uint64)), rep
+        got = blocks.cpu().numpy().view(TREE_BLOCK_DTYPE)
+        for k in range(4):
+            _check_tree(got[k], want_blocks[k], (rep, k))
