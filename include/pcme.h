@@ -367,6 +367,18 @@ typedef struct pcme_cplx_block {
   pcme_mv mv;
 } pcme_cplx_block;
 
+/* lookahead.rs:116-139 aq_offsets (called at mod.rs:658-662 when the rate
+ * control is not constant-QP): per MB, strength * (log2(var_y/256 +
+ * 0.5*(var_u/64 + var_v/64) + 1) - the picture's mean), clamped to +-8,
+ * with the variances of dsp.rs:104-114.  y, u, v (device, 4-byte aligned)
+ * point at picture (0, 0) of the padded source planes, strides multiples
+ * of 4; out (device) receives mbw*mbh floats in raster order.  Float
+ * results: within 1e-5 of the reference (the log2 of the C library and of
+ * the device may differ in the last place); the variances are exact. */
+int pcme_aq_offsets(const uint8_t *y, int32_t ystride, const uint8_t *u,
+                    const uint8_t *v, int32_t cstride, int32_t mbw,
+                    int32_t mbh, float strength, float *out, void *stream);
+
 /* One complexity() call (lookahead.rs:52-108): cur and prev (device, NULL =
  * no previous frame) are w x h lowres frames.  sums (device, 8-byte
  * aligned) receives Complexity {intra, inter}; blocks (device, may be NULL)

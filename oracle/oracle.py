@@ -172,6 +172,10 @@ def load() -> ctypes.CDLL:
     lib.o_b_frame_mt.restype = c.c_double
     lib.o_b_frame_mt.argtypes = [c.POINTER(oparams), P, P, c.POINTER(oref), c.POINTER(oref), P, c.c_int, c.c_int,
                                  c.c_uint32, c.c_int, P]
+    lib.o_block_var.restype = c.c_uint32
+    lib.o_block_var.argtypes = [P, c.c_int, c.c_int, c.c_int]
+    lib.o_aq_offsets.restype = None
+    lib.o_aq_offsets.argtypes = [P, c.c_int, P, P, c.c_int, c.c_int, c.c_int, c.c_float, P]
     lib.o_lookahead_mt.restype = c.c_double
     lib.o_lookahead_mt.argtypes = [P, c.c_int, c.c_int, c.c_int, c.c_int, P, P]
     _lib = lib
@@ -374,3 +378,21 @@ def b_frame_mt(prm: oparams, mbs, cols, r0: Ref, r1: Ref, src_padded: np.ndarray
                               ctypes.byref(r0.view), ctypes.byref(r1.view), _p(s), s.shape[1], LUMA_PAD, lam,
                               threads, _p(out))
     return sec, out
+
+
+# -------------------------------------------------------- AQ (lookahead.rs)
+
+def block_var(a: np.ndarray) -> int:
+    b = np.ascontiguousarray(a, np.uint8)
+    return int(load().o_block_var(_p(b), b.shape[1], b.shape[1], b.shape[0]))
+
+
+def aq_offsets(y: np.ndarray, u: np.ndarray, v: np.ndarray, mbw: int, mbh: int, strength: float) -> np.ndarray:
+    """aq_offsets of a picture given as its coded-area planes (y 16*mbh x
+    16*mbw, u and v 8*mbh x 8*mbw) -> [mbh, mbw] float32."""
+    yy = np.ascontiguousarray(y, np.uint8)
+    uu = np.ascontiguousarray(u, np.uint8)
+    vv = np.ascontiguousarray(v, np.uint8)
+    out = np.zeros(mbw * mbh, np.float32)
+    load().o_aq_offsets(_p(yy), yy.shape[1], _p(uu), _p(vv), uu.shape[1], mbw, mbh, strength, _p(out))
+    return out.reshape(mbh, mbw)

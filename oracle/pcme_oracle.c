@@ -1297,3 +1297,42 @@ double o_b_frame_mt(const oparams *p, const ombinfo2 *mbs, const ombinfo2 *cols,
   free(proj);
   return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 }
+
+/* ------------------------------------------------------------- AQ (C8) */
+
+/* dsp.rs:104-114 block_var: sum of squares minus square of sum over w*h,
+ * the quotient in 64 bits. */
+uint32_t o_block_var(const uint8_t *a, int sa, int w, int h) {
+  uint32_t s = 0, ss = 0;
+  for (int r = 0; r < h; r++)
+    for (int x = 0; x < w; x++) {
+      const uint32_t v = a[(size_t)r * sa + x];
+      s += v;
+      ss += v * v;
+    }
+  return ss - (uint32_t)(((uint64_t)s * (uint64_t)s) / (uint64_t)(w * h));
+}
+
+/* lookahead.rs:116-139 aq_offsets: per MB log2(var_y/256 + 0.5*(var_u/64 +
+ * var_v/64) + 1) in f32, minus the mean over the picture (summed in MB
+ * order), times strength, clamped to +-8.  y, u, v point at picture (0, 0)
+ * of planes with strides ys, cs.  Built with -ffp-contract=off: every f32
+ * operation rounds on its own, as in the reference. */
+void o_aq_offsets(const uint8_t *y, int ys, const uint8_t *u, const uint8_t *v, int cs, int mbw, int mbh,
+                  float strength, float *out) {
+  const int n = mbw * mbh;
+  for (int my = 0; my < mbh; my++)
+    for (int mx = 0; mx < mbw; mx++) {
+      const float vy = (float)o_block_var(y + (size_t)(16 * my) * ys + 16 * mx, ys, 16, 16) / 256.0f;
+      const float vu = (float)o_block_var(u + (size_t)(8 * my) * cs + 8 * mx, cs, 8, 8) / 64.0f;
+      const float vv = (float)o_block_var(v + (size_t)(8 * my) * cs + 8 * mx, cs, 8, 8) / 64.0f;
+      out[my * mbw + mx] = log2f(vy + 0.5f * (vu + vv) + 1.0f);
+    }
+  float sum = 0.0f;
+  for (int i = 0; i < n; i++) sum += out[i];
+  const float mean = sum / (float)n;
+  for (int i = 0; i < n; i++) {
+    float o = strength * (out[i] - mean);
+    out[i] = o < -8.0f ? -8.0f : (o > 8.0f ? 8.0f : o);
+  }
+}

@@ -691,6 +691,24 @@ int pcme_complexity(const pcme_cplx_job *jobs, int32_t n, void *stream) {
   return PCME_OK;
 }
 
+int pcme_aq_offsets(const uint8_t *y, int32_t ystride, const uint8_t *u, const uint8_t *v, int32_t cstride,
+                    int32_t mbw, int32_t mbh, float strength, float *out, void *stream) {
+  if (!y || !u || !v || !out || mbw < 1 || mbh < 1 || ystride < 16 * mbw || cstride < 8 * mbw)
+    return fail(PCME_EINVAL, "pcme_aq_offsets: bad arguments");
+  if (!a4(y) || !a4(u) || !a4(v) || !a4(out) || (ystride & 3) || (cstride & 3))
+    return fail(PCME_EINVAL, "pcme_aq_offsets: planes, strides and out must be 4-byte aligned");
+  Dev *dv;
+  int rc = get_dev(&dv);
+  if (rc) return rc;
+  const int n = mbw * mbh;
+  hipLaunchKernelGGL(aq_var_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, (hipStream_t)stream, y, ystride,
+                     u, v, cstride, mbw, mbh, out);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(aq_mean_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, out, n, strength);
+  HIPCHK(hipGetLastError());
+  return PCME_OK;
+}
+
 int pcme_mbtree_analyse(const pcme_tree_job *jobs, int32_t n, void *stream) {
   static const char *fn = "pcme_mbtree_analyse";
   if (n < 0 || (n > 0 && !jobs)) return fail(PCME_EINVAL, "%s: bad job list", fn);

@@ -72,6 +72,9 @@ __global__ void clear_kernel(const uint32_t *ctl, uint32_t *xchg, size_t nwords)
 __global__ void complexity_kernel(const CplxArgs A);
 __global__ void complexity_clear_kernel(const CplxArgs A, int n);
 __global__ void tree_kernel(const TreeArgs A);
+__global__ void aq_var_kernel(const uint8_t *__restrict__ y, int ys, const uint8_t *__restrict__ u,
+                              const uint8_t *__restrict__ v, int cs, int mbw, int mbh, float *__restrict__ e);
+__global__ void aq_mean_kernel(float *__restrict__ e, int n, float strength);
 #endif
 
 // Message returned by pcme_last_error() on this thread (host side).

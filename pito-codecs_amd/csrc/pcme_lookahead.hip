@@ -396,4 +396,66 @@ __global__ void __launch_bounds__(256) tree_kernel(const TreeArgs A) {
   }
 }
 
+
+// ----------------------------------------------------------- AQ offsets
+
+__device__ __forceinline__ uint32_t bsum4(uint32_t w) {
+  return (w & 0xFFu) + ((w >> 8) & 0xFFu) + ((w >> 16) & 0xFFu) + (w >> 24);
+}
+__device__ __forceinline__ uint32_t bsq4(uint32_t w) {
+  const uint32_t a = w & 0xFFu, b = (w >> 8) & 0xFFu, c = (w >> 16) & 0xFFu, d = w >> 24;
+  return a * a + b * b + c * c + d * d;
+}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o);
+  return v;
+}
+
+// lookahead.rs:116-130: per MB, log2 of the luma and chroma block
+// variances (dsp.rs:104-114 block_var).  One wave per MB: lane l holds luma
+// dword (row l >> 2, bytes 4*(l & 3)..), lanes 0-15 a u dword and 16-31 a v
+// dword (row (l & 15) >> 1, bytes 4*(l & 1)..).  Every f32 operation rounds
+// on its own (no contraction), as in the reference.
+__global__ void __launch_bounds__(256) aq_var_kernel(const uint8_t *__restrict__ y, int ys,
+                                                     const uint8_t *__restrict__ u, const uint8_t *__restrict__ v,
+                                                     int cs, int mbw, int mbh, float *__restrict__ e) {
+#pragma clang fp contract(off)
+  const int mb = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  if (mb >= mbw * mbh) return;
+  const int lane = (int)__lane_id(), mx = mb % mbw, my = mb / mbw;
+  const uint32_t wy = ld32(y + (size_t)(16 * my + (lane >> 2)) * ys + 16 * mx + 4 * (lane & 3));
+  const int cl = lane & 15;
+  const uint8_t *cp = lane < 16 ? u : v;
+  const uint32_t wc = lane < 32 ? ld32(cp + (size_t)(8 * my + (cl >> 1)) * cs + 8 * mx + 4 * (cl & 1)) : 0u;
+  const uint32_t sy = wave_sum(bsum4(wy)), qy = wave_sum(bsq4(wy));
+  const uint32_t su = wave_sum(lane < 16 ? bsum4(wc) : 0u), qu = wave_sum(lane < 16 ? bsq4(wc) : 0u);
+  const uint32_t sv = wave_sum(lane >= 16 ? bsum4(wc) : 0u), qv = wave_sum(lane >= 16 ? bsq4(wc) : 0u);
+  if (lane == 0) {
+    const uint32_t var_y = qy - (uint32_t)(((uint64_t)sy * sy) / 256u);
+    const uint32_t var_u = qu - (uint32_t)(((uint64_t)su * su) / 64u);
+    const uint32_t var_v = qv - (uint32_t)(((uint64_t)sv * sv) / 64u);
+    const float vy = (float)var_y / 256.0f, vu = (float)var_u / 64.0f, vv = (float)var_v / 64.0f;
+    e[mb] = log2f(vy + 0.5f * (vu + vv) + 1.0f);
+  }
+}
+
+// lookahead.rs:134-138: the mean of e summed in MB order (one wave), then
+// clamp(strength * (e - mean), -8, 8) in place.
+__global__ void __launch_bounds__(64) aq_mean_kernel(float *__restrict__ e, int n, float strength) {
+#pragma clang fp contract(off)
+  const int lane = (int)__lane_id();
+  float sum = 0.0f;
+  for (int b = 0; b < n; b += 64) {
+    const float x = b + lane < n ? e[b + lane] : 0.0f;
+    const int m = min(64, n - b);
+    for (int j = 0; j < m; j++) sum += __builtin_bit_cast(float, rdlane(__builtin_bit_cast(uint32_t, x), j));
+  }
+  const float mean = sum / (float)n;
+  for (int i = lane; i < n; i += 64) {
+    const float o = strength * (e[i] - mean);
+    e[i] = o < -8.0f ? -8.0f : (o > 8.0f ? 8.0f : o);
+  }
+}
+
 }  // namespace pcme

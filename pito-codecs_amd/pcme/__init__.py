@@ -166,6 +166,7 @@ EXPORTS = (
     "pcme_net_create", "pcme_net_destroy", "pcme_net_get_params", "pcme_net_set_params",
     "pcme_net_inputs", "pcme_net_forward", "pcme_net_grad", "pcme_net_train_step", "pcme_net_hints",
     "pcme_lowres", "pcme_complexity", "pcme_mbtree_analyse", "pcme_search_part_batch", "pcme_b_analyse_batch",
+    "pcme_aq_offsets",
 )
 
 _lib = None
@@ -214,6 +215,7 @@ def load() -> ctypes.CDLL:
         "pcme_lowres": (c.c_int, [vp, c.c_int64, c.c_int32, c.c_int32, c.c_int32, c.c_int32, vp, c.c_int64,
                                   c.c_int32, vp]),
         "pcme_complexity": (c.c_int, [c.POINTER(CplxJob), c.c_int32, vp]),
+        "pcme_aq_offsets": (c.c_int, [vp, c.c_int32, vp, vp, c.c_int32, c.c_int32, c.c_int32, c.c_float, vp, vp]),
         "pcme_mbtree_analyse": (c.c_int, [c.POINTER(TreeJob), c.c_int32, vp]),
     }
     for name, (res, args) in sig.items():

@@ -7,9 +7,11 @@ Mirrors the reference's interface (pito-codecs src/h264/enc/):
                               (``Complexity {intra, inter}``)
 * ``analyse(cur, l0, l1)``    mbtree.rs:113-159, the per-block costs of the
                               macroblock tree (``Costs.blocks``)
+* ``aq_offsets(frame)``       lookahead.rs:116-139, the per-MB AQ offsets
 
 plus batched forms that take many frames in one launch.  Results are
-bit-identical to the reference.  There is no CPU path: without libpcme or a
+bit-identical to the reference (the AQ offsets, float results, within
+1e-5).  There is no CPU path: without libpcme or a
 gfx950 device every call raises ``PcmeError``.
 """
 from __future__ import annotations
@@ -160,5 +162,24 @@ def analyse(cur, l0=None, l1=None, stream=None) -> np.ndarray:
     return blocks.cpu().numpy().view(TREE_BLOCK_DTYPE)
 
 
-__all__ = ["Complexity", "LowRes", "lowres_frames", "complexity", "complexity_batch", "complexity_jobs",
+AQ_STRENGTH = 0.6  # mod.rs:109
+
+
+def aq_offsets(clip: DeviceClip, t: int, strength: float = AQ_STRENGTH, stream=None):
+    """lookahead::aq_offsets of frame t of a padded clip with chroma: a
+    [mbh, mbw] float32 tensor on the GPU."""
+    from . import CHROMA_PAD
+    if not clip.has_chroma:
+        raise PcmeError(PCME_EINVAL, "aq_offsets needs the clip's chroma planes")
+    torch = clip.torch
+    out = torch.empty((clip.mbh, clip.mbw), dtype=torch.float32, device=clip.raw.device)
+    y = clip.plane_ptr(0, t) + LUMA_PAD * clip.stride + LUMA_PAD
+    u = clip.cplanes[0, t].data_ptr() + CHROMA_PAD * clip.cstride + CHROMA_PAD
+    v = clip.cplanes[1, t].data_ptr() + CHROMA_PAD * clip.cstride + CHROMA_PAD
+    _check(load().pcme_aq_offsets(y, clip.stride, u, v, clip.cstride, clip.mbw, clip.mbh, float(strength),
+                                  out.data_ptr(), _stream_ptr(stream)))
+    return out
+
+
+__all__ = ["Complexity", "LowRes", "aq_offsets", "AQ_STRENGTH", "lowres_frames", "complexity", "complexity_batch", "complexity_jobs",
            "run_complexity", "analyse", "analyse_batch", "tree_jobs", "run_analyse"]

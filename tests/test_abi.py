@@ -200,3 +200,12 @@ def test_b_analyse_batch_checks_arguments(pcme_lib):
     assert lib.pcme_b_analyse_batch(j, 1, 64, None) == pcme_lib.PCME_EINVAL
     if pcme_lib.device_count() == 0:
         assert lib.pcme_b_analyse_batch(job(), 1, 64, None) == pcme_lib.PCME_ENODEV
+
+
+def test_aq_offsets_checks_arguments(pcme_lib):
+    lib = pcme_lib.load()
+    if pcme_lib.device_count() == 0:
+        assert lib.pcme_aq_offsets(64, 128, 64, 64, 64, 4, 4, 0.6, 64, None) == pcme_lib.PCME_ENODEV
+    assert lib.pcme_aq_offsets(66, 128, 64, 64, 64, 4, 4, 0.6, 64, None) == pcme_lib.PCME_EINVAL  # misaligned y
+    assert lib.pcme_aq_offsets(64, 32, 64, 64, 64, 4, 4, 0.6, 64, None) == pcme_lib.PCME_EINVAL  # stride < 16*mbw
+    assert lib.pcme_aq_offsets(64, 128, 64, None, 64, 4, 4, 0.6, 64, None) == pcme_lib.PCME_EINVAL

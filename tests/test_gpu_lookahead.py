@@ -209,3 +209,27 @@ def test_lookahead_replays_in_a_hip_graph(gpu, oracle):
         got = blocks.cpu().numpy().view(TREE_BLOCK_DTYPE)
         for k in range(4):
             _check_tree(got[k], want_blocks[k], (rep, k))
+
+
+@pytest.mark.parametrize("w,h", [(1920, 1080), (352, 288)])
+def test_aq_offsets_match_oracle(gpu, oracle, w, h):
+    """pcme_aq_offsets against the oracle's aq_offsets on the coded-area
+    planes, within 1e-5 (float32 results; the device's and the C library's
+    log2 may differ in the last place)."""
+    import torch
+    from pcme import lookahead as L
+    say("aq offsets", w, h)
+    y, u, v = gpu.synth_clip(gpu.ClipSpec(w, h, 2, seed=0x5EED1D00 + w, nrect=6, vmax=4.0))
+    mbw, mbh = gpu.mb_dims(w, h)
+    dc = gpu.DeviceClip(y, mbw, mbh, u, v)
+    dc.pad()
+    got = L.aq_offsets(dc, 1)
+    torch.cuda.synchronize()
+    pl = dc.planes[0, 1].cpu().numpy()
+    cp = dc.cplanes[:, 1].cpu().numpy()
+    P, C = gpu.LUMA_PAD, gpu.CHROMA_PAD
+    want = oracle.aq_offsets(pl[P:P + 16 * mbh, P:P + 16 * mbw], cp[0, C:C + 8 * mbh, C:C + 8 * mbw],
+                             cp[1, C:C + 8 * mbh, C:C + 8 * mbw], mbw, mbh, 0.6)
+    g = got.cpu().numpy()
+    np.testing.assert_allclose(g, want, atol=1e-5, rtol=0)
+    assert np.abs(g).max() > 0.1  # a textured clip: offsets are not all zero
