@@ -25,6 +25,11 @@
  *   src/h264/enc/mbtree.rs:21-49      intra_cost (o_lowres_intra)
  *   src/h264/enc/mbtree.rs:51-159     search, fetch, analyse (o_tree_search,
  *                                     o_tree_analyse)
+ *   src/h264/enc/slice.rs:2030-2084   direct_spatial (o_direct_spatial)
+ *   src/h264/enc/slice.rs:2086-2377   b_partition, analyse_b's search
+ *                                     (o_b_analyse; without BSkip, RD, intra)
+ *   src/h264/enc/dsp.rs:104-114       block_var (o_block_var)
+ *   src/h264/enc/lookahead.rs:116-139 aq_offsets (o_aq_offsets)
  *
  * How each part is pinned (tests/):
  *   o_sad/o_sad16            closed form (numpy |a-b| sum), random + extremes
@@ -57,6 +62,13 @@
  *                            recovered exactly, an exact bi-average win
  *                            (cost 12), and a second, independently written
  *                            Python restatement on small frames
+ *   o_direct_spatial         spatial direct restated in the reference
+ *                            decoder's structure (dec/slicedec.rs:718-800)
+ *   o_b_analyse              every cost recomputed from its MVs with the
+ *                            H.264 quarter-sample equations and textbook
+ *                            Hadamard; first-minimum directions
+ *   o_block_var/o_aq_offsets closed forms (flat, one textured MB, clamp)
+ *                            and a float32 numpy restatement (1e-5)
  */
 #include "pcme_oracle.h"
 
