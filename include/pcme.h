@@ -418,6 +418,28 @@ typedef struct pcme_tree_job {
  * reference. */
 int pcme_mbtree_analyse(const pcme_tree_job *jobs, int32_t n, void *stream);
 
+/* mbtree.rs:162-165 Node: the node's analyse() blocks (device, bw*bh
+ * records) and the node index each list refers to, -1 for none (Option
+ * None).  A reference at or after the node itself feeds nothing.  16 bytes. */
+typedef struct pcme_tree_node {
+  const pcme_tree_block *blocks;
+  int32_t ref[2];
+} pcme_tree_node;
+
+/* mbtree::propagate (mbtree.rs:167-223; the encoder's tree_offsets,
+ * mod.rs:819-840, runs it over the lookahead queue): prop (device, n*bw*bh
+ * floats, node-major, raster order) receives every node's propagated cost.
+ * Bit-identical to the reference.  Nodes run last to first, one launch
+ * each, on `stream`. */
+int pcme_mbtree_propagate(const pcme_tree_node *nodes, int32_t n, int32_t bw, int32_t bh, float *prop,
+                          void *stream);
+
+/* mbtree::offsets (mbtree.rs:225-231): out[k] = -2 log2((max(intra, 1) +
+ * prop[k]) / max(intra, 1)) over nblocks blocks (all device).  Within 1e-5
+ * of the reference (the GPU's log2). */
+int pcme_mbtree_offsets(const pcme_tree_block *blocks, const float *prop, int32_t nblocks, float *out,
+                        void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -172,6 +172,10 @@ def load() -> ctypes.CDLL:
     lib.o_b_frame_mt.restype = c.c_double
     lib.o_b_frame_mt.argtypes = [c.POINTER(oparams), P, P, c.POINTER(oref), c.POINTER(oref), P, c.c_int, c.c_int,
                                  c.c_uint32, c.c_int, P]
+    lib.o_mbtree_propagate.restype = None
+    lib.o_mbtree_propagate.argtypes = [c.POINTER(c.c_void_p), P, c.c_int, c.c_int, c.c_int, P]
+    lib.o_mbtree_offsets.restype = None
+    lib.o_mbtree_offsets.argtypes = [P, P, c.c_int, P]
     lib.o_block_var.restype = c.c_uint32
     lib.o_block_var.argtypes = [P, c.c_int, c.c_int, c.c_int]
     lib.o_aq_offsets.restype = None
@@ -396,3 +400,26 @@ def aq_offsets(y: np.ndarray, u: np.ndarray, v: np.ndarray, mbw: int, mbh: int, 
     out = np.zeros(mbw * mbh, np.float32)
     load().o_aq_offsets(_p(yy), yy.shape[1], _p(uu), _p(vv), uu.shape[1], mbw, mbh, strength, _p(out))
     return out.reshape(mbh, mbw)
+
+
+# ------------------------------------------------ macroblock tree (mbtree.rs)
+
+def mbtree_propagate(blocks: list[np.ndarray], refs: list[tuple[int, int]], bw: int, bh: int) -> np.ndarray:
+    """propagate over nodes whose Costs.blocks are OTREE_DTYPE arrays
+    (bh*bw each) and refs [(ref0, ref1)] (node index or -1) -> [n, bh*bw]
+    float32."""
+    n = len(blocks)
+    arrs = [np.ascontiguousarray(b, OTREE_DTYPE) for b in blocks]
+    ptrs = (ctypes.c_void_p * n)(*[_p(a) for a in arrs])
+    r = np.array(refs, np.int32).reshape(n, 2)
+    prop = np.zeros((n, bw * bh), np.float32)
+    load().o_mbtree_propagate(ptrs, _p(r), n, bw, bh, _p(prop))
+    return prop
+
+
+def mbtree_offsets(blocks: np.ndarray, prop: np.ndarray) -> np.ndarray:
+    b = np.ascontiguousarray(blocks, OTREE_DTYPE)
+    p = np.ascontiguousarray(prop, np.float32)
+    out = np.zeros(b.size, np.float32)
+    load().o_mbtree_offsets(_p(b), _p(p), b.size, _p(out))
+    return out

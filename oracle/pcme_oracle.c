@@ -30,6 +30,8 @@
  *                                     (o_b_analyse; without BSkip, RD, intra)
  *   src/h264/enc/dsp.rs:104-114       block_var (o_block_var)
  *   src/h264/enc/lookahead.rs:116-139 aq_offsets (o_aq_offsets)
+ *   src/h264/enc/mbtree.rs:162-231    Node, propagate, offsets
+ *                                     (o_mbtree_propagate, o_mbtree_offsets)
  *
  * How each part is pinned (tests/):
  *   o_sad/o_sad16            closed form (numpy |a-b| sum), random + extremes
@@ -69,6 +71,15 @@
  *                            Hadamard; first-minimum directions
  *   o_block_var/o_aq_offsets closed forms (flat, one textured MB, clamp)
  *                            and a float32 numpy restatement (1e-5)
+ *   o_mbtree_propagate       closed forms: a zero-MV edge moves intra -
+ *                            min(inter, intra), whole-cell shifts, the
+ *                            four-cell area split of asymmetric and negative
+ *                            fractional MVs (floor, not truncation), the
+ *                            half split of bi blocks, a three-node chain,
+ *                            skipped references; mass conservation on random
+ *                            nodes; a float64 scatter (np.add.at) on random
+ *                            multi-node inputs
+ *   o_mbtree_offsets         closed forms (prop 0, prop = intra, intra 0)
  */
 #include "pcme_oracle.h"
 
@@ -1346,5 +1357,59 @@ void o_aq_offsets(const uint8_t *y, int ys, const uint8_t *u, const uint8_t *v, 
   for (int i = 0; i < n; i++) {
     float o = strength * (out[i] - mean);
     out[i] = o < -8.0f ? -8.0f : (o > 8.0f ? 8.0f : o);
+  }
+}
+
+/* ------------------------------------------------- macroblock tree (C9) */
+
+static int div_floor(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+/* mbtree.rs:162-223 propagate: nodes n-1..0; node i's blocks move
+ * (intra + prop_i) * (intra - min(inter, intra)) / intra of their cost into
+ * the 8x8 cells their MV lands on in the earlier nodes they reference,
+ * split by area (list 3: half to each list).  prop receives n * bw * bh
+ * floats (zeroed first).  refs[2*i + l] = node index or -1. */
+void o_mbtree_propagate(const otree_block *const *blocks, const int *refs, int n, int bw, int bh, float *prop) {
+  const int nb = bw * bh;
+  memset(prop, 0, sizeof(float) * (size_t)n * (size_t)nb);
+  for (int i = n - 1; i >= 0; i--) {
+    const float *mine = prop + (size_t)i * nb;
+    for (int by = 0; by < bh; by++)
+      for (int bx = 0; bx < bw; bx++) {
+        const int k = by * bw + bx;
+        const otree_block *b = &blocks[i][k];
+        if (b->list == 0 || b->intra == 0) continue;
+        const float intra = (float)b->intra;
+        const float inter0 = (float)b->inter;
+        const float inter = inter0 < intra ? inter0 : intra;
+        const float amount = (intra + mine[k]) * (intra - inter) / intra;
+        if (amount <= 0.0f) continue;
+        const int nl = b->list == 3 ? 2 : 1;
+        for (int j = 0; j < nl; j++) {
+          const int l = b->list == 3 ? j : b->list - 1;
+          const float w = b->list == 3 ? 0.5f : 1.0f;
+          const int r = refs[2 * i + l];
+          if (r < 0 || r >= i) continue;
+          float *dst = prop + (size_t)r * nb;
+          const int x = bx * 8 + b->mv[l].x, y = by * 8 + b->mv[l].y;
+          const int cx = div_floor(x, 8), cy = div_floor(y, 8);
+          const int fx = x - 8 * cx, fy = y - 8 * cy;
+          const float a = amount * w / 64.0f;
+          const int area[4] = {(8 - fx) * (8 - fy), fx * (8 - fy), (8 - fx) * fy, fx * fy};
+          for (int c = 0; c < 4; c++) {
+            const int tx = cx + (c & 1), ty = cy + (c >> 1);
+            if (area[c] == 0 || tx < 0 || ty < 0 || tx >= bw || ty >= bh) continue;
+            dst[ty * bw + tx] += a * (float)area[c];
+          }
+        }
+      }
+  }
+}
+
+/* mbtree.rs:225-231 offsets: -2 * log2((max(intra, 1) + prop) / max(intra, 1)). */
+void o_mbtree_offsets(const otree_block *blocks, const float *prop, int nb, float *out) {
+  for (int k = 0; k < nb; k++) {
+    const float intra = (float)(blocks[k].intra > 1 ? blocks[k].intra : 1);
+    out[k] = -2.0f * log2f((intra + prop[k]) / intra);
   }
 }

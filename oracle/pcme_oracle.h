@@ -250,6 +250,9 @@ void o_direct_spatial(const oparams *p, const ombinfo2 *mbs, const ombinfo2 *col
                       int8_t out_ref[2][4], omv out_mv[2][4]);
 void o_b_analyse(const oparams *p, const ombinfo2 *mbs, const ombinfo2 *col, int mx, int my, const oref *r0,
                  const oref *r1, const uint8_t *src16, uint32_t lam, ob_out *o);
+/* mbtree.rs:162-231 */
+void o_mbtree_propagate(const otree_block *const *blocks, const int *refs, int n, int bw, int bh, float *prop);
+void o_mbtree_offsets(const otree_block *blocks, const float *prop, int nb, float *out);
 /* dsp.rs:104-114, lookahead.rs:116-139 */
 uint32_t o_block_var(const uint8_t *a, int sa, int w, int h);
 void o_aq_offsets(const uint8_t *y, int ys, const uint8_t *u, const uint8_t *v, int cs, int mbw, int mbh,

@@ -49,6 +49,8 @@ struct Dev {
   size_t dpj_cap = 0;
   pcme_b_job *dbj = nullptr;        // B macroblock jobs
   size_t dbj_cap = 0;
+  int4 *dwin = nullptr;             // macroblock-tree landing windows, one per node
+  size_t dwin_cap = 0;
   unsigned char *hstage = nullptr;  // pinned upload staging
   size_t hstage_cap = 0;
   hipEvent_t stage_ev = nullptr;
@@ -737,6 +739,78 @@ int pcme_mbtree_analyse(const pcme_tree_job *jobs, int32_t n, void *stream) {
                        (hipStream_t)stream, A);
     HIPCHK(hipGetLastError());
   }
+  return PCME_OK;
+}
+
+int pcme_mbtree_propagate(const pcme_tree_node *nodes, int32_t n, int32_t bw, int32_t bh, float *prop,
+                          void *stream) {
+  static const char *fn = "pcme_mbtree_propagate";
+  if (n < 0 || (n > 0 && !nodes)) return fail(PCME_EINVAL, "%s: bad node list", fn);
+  if (bw < 1 || bh < 1 || bw > 65535 || bh > 65535 || (int64_t)bw * bh > (1 << 24))
+    return fail(PCME_EINVAL, "%s: bad block grid %dx%d", fn, bw, bh);
+  for (int i = 0; i < n; i++) {
+    if (!nodes[i].blocks || !a4(nodes[i].blocks))
+      return fail(PCME_EINVAL, "%s: node %d: blocks null or not 4-byte aligned", fn, i);
+    if (nodes[i].ref[0] < -1 || nodes[i].ref[1] < -1)
+      return fail(PCME_EINVAL, "%s: node %d: reference index below -1", fn, i);
+  }
+  if (n > 0 && (!prop || !a4(prop))) return fail(PCME_EINVAL, "%s: prop null or not 4-byte aligned", fn);
+  if (n == 0) return PCME_OK;
+  Dev *dv;
+  int rc = get_dev(&dv);
+  if (rc) return rc;
+  if ((rc = grow(&dv->dwin, &dv->dwin_cap, (size_t)n))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = bw * bh;
+  const size_t total = (size_t)n * (size_t)nb;
+  hipLaunchKernelGGL(tree_zero_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, prop, total);
+  HIPCHK(hipGetLastError());
+  for (int c0 = 0; c0 < n; c0 += LA_CHUNK) {
+    const int m = std::min(LA_CHUNK, n - c0);
+    TreeNodeArgs A;
+    memset(&A, 0, sizeof A);
+    for (int i = 0; i < m; i++) A.b[i] = nodes[c0 + i].blocks;
+    hipLaunchKernelGGL(tree_window_kernel, dim3((unsigned)m), dim3(256), 0, st, A, nb, dv->dwin + c0);
+    HIPCHK(hipGetLastError());
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    // The destinations of node i: a list feeds node ref[l] when that is an
+    // earlier node (mbtree.rs:191-194); two lists on one node share a thread.
+    float *dst[2] = {nullptr, nullptr};
+    int mask[2] = {0, 0}, nd = 0;
+    for (int l = 0; l < 2; l++) {
+      const int r = nodes[i].ref[l];
+      if (r < 0 || r >= i) continue;
+      float *p = prop + (size_t)r * nb;
+      if (nd == 1 && dst[0] == p) {
+        mask[0] |= 1 << l;
+        continue;
+      }
+      dst[nd] = p;
+      mask[nd++] = 1 << l;
+    }
+    if (!nd) continue;
+    const unsigned threads = (unsigned)(nd * nb);
+    hipLaunchKernelGGL(tree_prop_kernel, dim3((threads + 255) / 256), dim3(256), 0, st, nodes[i].blocks,
+                       prop + (size_t)i * nb, dst[0], dst[1], mask[0], mask[1], dv->dwin + i, bw, bh);
+    HIPCHK(hipGetLastError());
+  }
+  return PCME_OK;
+}
+
+int pcme_mbtree_offsets(const pcme_tree_block *blocks, const float *prop, int32_t nblocks, float *out,
+                        void *stream) {
+  if (nblocks < 0 || (nblocks > 0 && (!blocks || !prop || !out)))
+    return fail(PCME_EINVAL, "pcme_mbtree_offsets: bad arguments");
+  if (nblocks > 0 && (!a4(blocks) || !a4(prop) || !a4(out)))
+    return fail(PCME_EINVAL, "pcme_mbtree_offsets: blocks, prop and out must be 4-byte aligned");
+  if (nblocks == 0) return PCME_OK;
+  Dev *dv;
+  int rc = get_dev(&dv);
+  if (rc) return rc;
+  hipLaunchKernelGGL(tree_offsets_kernel, dim3((unsigned)((nblocks + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, blocks, prop, nblocks, out);
+  HIPCHK(hipGetLastError());
   return PCME_OK;
 }
 

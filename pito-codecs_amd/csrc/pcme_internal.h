@@ -35,6 +35,10 @@ struct CplxArgs {
 struct TreeArgs {
   pcme_tree_job j[LA_CHUNK];
 };
+struct TreeNodeArgs {
+  const pcme_tree_block *b[LA_CHUNK];
+};
+#define TREE_WIN_EMPTY (1 << 29)  // tree_window_kernel's bound of a node no block of which feeds a cell
 static_assert(sizeof(CplxArgs) <= 4000 && sizeof(TreeArgs) <= 4000, "lookahead jobs exceed the kernel arguments");
 
 #ifdef __HIP__
@@ -75,6 +79,13 @@ __global__ void tree_kernel(const TreeArgs A);
 __global__ void aq_var_kernel(const uint8_t *__restrict__ y, int ys, const uint8_t *__restrict__ u,
                               const uint8_t *__restrict__ v, int cs, int mbw, int mbh, float *__restrict__ e);
 __global__ void aq_mean_kernel(float *__restrict__ e, int n, float strength);
+__global__ void tree_window_kernel(const TreeNodeArgs A, int nb, int4 *__restrict__ win);
+__global__ void tree_zero_kernel(float *__restrict__ p, size_t n);
+__global__ void tree_prop_kernel(const pcme_tree_block *__restrict__ src, const float *__restrict__ mine,
+                                 float *dst0, float *dst1, int mask0, int mask1, const int4 *__restrict__ win,
+                                 int bw, int bh);
+__global__ void tree_offsets_kernel(const pcme_tree_block *__restrict__ b, const float *__restrict__ prop, int n,
+                                    float *__restrict__ out);
 #endif
 
 // Message returned by pcme_last_error() on this thread (host side).

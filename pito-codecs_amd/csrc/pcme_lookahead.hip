@@ -458,4 +458,123 @@ __global__ void __launch_bounds__(64) aq_mean_kernel(float *__restrict__ e, int 
   }
 }
 
+// ------------------------------------------------------ macroblock tree
+
+// mbtree.rs:167-223 feeds a node's blocks into the 8x8 cells their MVs land
+// on: block (bx, by) with MV m lands at cell (bx + floor(mx/8), by +
+// floor(my/8)) and its right / lower neighbours.  For each node this kernel
+// (one workgroup per node) finds the range of floor(m/8) over the blocks
+// that can feed anything (list != 0, intra != 0, the lists list uses):
+// win = {qx_min, qx_max, qy_min, qy_max}, an empty range when none can.
+__global__ void __launch_bounds__(256) tree_window_kernel(const TreeNodeArgs A, int nb, int4 *__restrict__ win) {
+  __shared__ int red[4][4];
+  const pcme_tree_block *b = A.b[blockIdx.x];
+  int v[4] = {TREE_WIN_EMPTY, -TREE_WIN_EMPTY, TREE_WIN_EMPTY, -TREE_WIN_EMPTY};
+  for (int k = (int)threadIdx.x; k < nb; k += 256) {
+    const pcme_tree_block q = b[k];
+    if (q.list == 0 || q.intra == 0) continue;
+#pragma unroll
+    for (int l = 0; l < 2; l++) {
+      if (!((q.list == 3) || (q.list == 1 + l))) continue;
+      const int qx = (int)q.mv[l].x >> 3, qy = (int)q.mv[l].y >> 3;
+      v[0] = min(v[0], qx);
+      v[1] = max(v[1], qx);
+      v[2] = min(v[2], qy);
+      v[3] = max(v[3], qy);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    v[0] = min(v[0], __shfl_xor(v[0], o));
+    v[1] = max(v[1], __shfl_xor(v[1], o));
+    v[2] = min(v[2], __shfl_xor(v[2], o));
+    v[3] = max(v[3], __shfl_xor(v[3], o));
+  }
+  const int w = (int)(threadIdx.x >> 6);
+  if (__lane_id() == 0)
+    for (int c = 0; c < 4; c++) red[w][c] = v[c];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int4 r = make_int4(red[0][0], red[0][1], red[0][2], red[0][3]);
+    for (int j = 1; j < 4; j++) {
+      r.x = min(r.x, red[j][0]);
+      r.y = max(r.y, red[j][1]);
+      r.z = min(r.z, red[j][2]);
+      r.w = max(r.w, red[j][3]);
+    }
+    win[blockIdx.x] = r;
+  }
+}
+
+// Zero n floats (the propagated costs start at 0).
+__global__ void __launch_bounds__(256) tree_zero_kernel(float *__restrict__ p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0.0f;
+}
+
+// One node of mbtree.rs:169-222.  Thread (d, cell) owns cell `cell` of
+// destination d (dst[d], fed by the lists in mask[d]) and gathers what the
+// node's blocks move into it: the sources in the window `win` allows, in
+// raster order, list 0 before list 1 -- the order the reference adds them
+// into that cell -- each `+=` rounded on its own, so the cell's sum is the
+// reference's to the bit.  mine holds the node's own propagated costs,
+// complete because the nodes after it ran first.
+__global__ void __launch_bounds__(256) tree_prop_kernel(const pcme_tree_block *__restrict__ src,
+                                                        const float *__restrict__ mine, float *dst0, float *dst1,
+                                                        int mask0, int mask1, const int4 *__restrict__ win, int bw,
+                                                        int bh) {
+#pragma clang fp contract(off)
+  const int nb = bw * bh;
+  const int t = (int)(blockIdx.x * 256 + threadIdx.x);
+  const int d = t >= nb;
+  const int cell = t - d * nb;
+  if (d && !dst1) return;
+  if (cell >= nb) return;
+  float *dst = d ? dst1 : dst0;
+  const int mask = d ? mask1 : mask0;
+  const int tx = cell % bw, ty = cell / bw;
+  const int4 w = *win;
+  const int x0 = max(0, tx - w.y - 1), x1 = min(bw - 1, tx - w.x);
+  const int y0 = max(0, ty - w.w - 1), y1 = min(bh - 1, ty - w.z);
+  float acc = dst[cell];
+  for (int sy = y0; sy <= y1; sy++)
+    for (int sx = x0; sx <= x1; sx++) {
+      const int k = sy * bw + sx;
+      const pcme_tree_block q = src[k];
+      if (q.list == 0 || q.intra == 0) continue;
+      const int lm = (q.list == 3 ? 3 : (q.list == 1 ? 1 : 2)) & mask;
+      if (!lm) continue;
+      const float intra = (float)q.intra;
+      const float inter0 = (float)q.inter;
+      const float inter = inter0 < intra ? inter0 : intra;
+      const float amount = (intra + mine[k]) * (intra - inter) / intra;
+      if (amount <= 0.0f) continue;
+      const float wt = q.list == 3 ? 0.5f : 1.0f;
+#pragma unroll
+      for (int l = 0; l < 2; l++) {
+        if (!((lm >> l) & 1)) continue;
+        const int mx = q.mv[l].x, my = q.mv[l].y;
+        const int dx = tx - (sx + (mx >> 3)), dy = ty - (sy + (my >> 3));
+        if ((unsigned)dx > 1u || (unsigned)dy > 1u) continue;
+        const int fx = mx & 7, fy = my & 7;
+        const int area = (dx ? fx : 8 - fx) * (dy ? fy : 8 - fy);
+        if (area == 0) continue;
+        const float a = amount * wt / 64.0f;
+        acc += a * (float)area;
+      }
+    }
+  dst[cell] = acc;
+}
+
+// mbtree.rs:225-231: -2 log2((max(intra, 1) + prop) / max(intra, 1)).
+__global__ void __launch_bounds__(256) tree_offsets_kernel(const pcme_tree_block *__restrict__ b,
+                                                           const float *__restrict__ prop, int n,
+                                                           float *__restrict__ out) {
+#pragma clang fp contract(off)
+  const int i = (int)(blockIdx.x * 256 + threadIdx.x);
+  if (i >= n) return;
+  const float intra = (float)(b[i].intra > 1u ? b[i].intra : 1u);
+  out[i] = -2.0f * log2f((intra + prop[i]) / intra);
+}
+
 }  // namespace pcme

@@ -121,6 +121,12 @@ class TreeJob(ctypes.Structure):
                 ("blocks", ctypes.c_void_p), ("w", ctypes.c_int32), ("h", ctypes.c_int32)]
 
 
+class TreeNode(ctypes.Structure):
+    """pcme_tree_node: a node's blocks (device) and its lists' reference
+    node indices (-1: none)."""
+    _fields_ = [("blocks", ctypes.c_void_p), ("ref", ctypes.c_int32 * 2)]
+
+
 # pcme_part_out (36 bytes): search_part's (mvs, mvps, total)
 PART_OUT_DTYPE = np.dtype([("mvs", "<i2", (4, 2)), ("mvps", "<i2", (4, 2)), ("total", "<u4")])
 assert PART_OUT_DTYPE.itemsize == 36
@@ -166,7 +172,7 @@ EXPORTS = (
     "pcme_net_create", "pcme_net_destroy", "pcme_net_get_params", "pcme_net_set_params",
     "pcme_net_inputs", "pcme_net_forward", "pcme_net_grad", "pcme_net_train_step", "pcme_net_hints",
     "pcme_lowres", "pcme_complexity", "pcme_mbtree_analyse", "pcme_search_part_batch", "pcme_b_analyse_batch",
-    "pcme_aq_offsets",
+    "pcme_aq_offsets", "pcme_mbtree_propagate", "pcme_mbtree_offsets",
 )
 
 _lib = None
@@ -217,6 +223,8 @@ def load() -> ctypes.CDLL:
         "pcme_complexity": (c.c_int, [c.POINTER(CplxJob), c.c_int32, vp]),
         "pcme_aq_offsets": (c.c_int, [vp, c.c_int32, vp, vp, c.c_int32, c.c_int32, c.c_int32, c.c_float, vp, vp]),
         "pcme_mbtree_analyse": (c.c_int, [c.POINTER(TreeJob), c.c_int32, vp]),
+        "pcme_mbtree_propagate": (c.c_int, [c.POINTER(TreeNode), c.c_int32, c.c_int32, c.c_int32, vp, vp]),
+        "pcme_mbtree_offsets": (c.c_int, [vp, vp, c.c_int32, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

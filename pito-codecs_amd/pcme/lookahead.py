@@ -8,10 +8,13 @@ Mirrors the reference's interface (pito-codecs src/h264/enc/):
 * ``analyse(cur, l0, l1)``    mbtree.rs:113-159, the per-block costs of the
                               macroblock tree (``Costs.blocks``)
 * ``aq_offsets(frame)``       lookahead.rs:116-139, the per-MB AQ offsets
+* ``propagate(nodes, bw, bh)`` mbtree.rs:167-223, the propagated cost of
+                              every lookahead node
+* ``offsets(costs, prop)``    mbtree.rs:225-231, the tree's QP offsets
 
 plus batched forms that take many frames in one launch.  Results are
-bit-identical to the reference (the AQ offsets, float results, within
-1e-5).  There is no CPU path: without libpcme or a
+bit-identical to the reference, the float propagation included (the AQ
+and tree offsets, which end in the GPU's log2, within 1e-5).  There is no CPU path: without libpcme or a
 gfx950 device every call raises ``PcmeError``.
 """
 from __future__ import annotations
@@ -21,7 +24,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import (CPLX_BLOCK_DTYPE, TREE_BLOCK_DTYPE, CplxJob, DeviceClip, LUMA_PAD, PcmeError, PCME_EINVAL,
-               TreeJob, _check, _stream_ptr, _torch, load)
+               TreeJob, TreeNode, _check, _stream_ptr, _torch, load)
 
 
 @dataclass
@@ -162,6 +165,52 @@ def analyse(cur, l0=None, l1=None, stream=None) -> np.ndarray:
     return blocks.cpu().numpy().view(TREE_BLOCK_DTYPE)
 
 
+@dataclass
+class Node:
+    """mbtree.rs:162-165: ``costs`` is a node's analyse() output resident on
+    the GPU (a uint8 tensor of bw*bh TREE_BLOCK_DTYPE records, e.g. a row of
+    analyse_batch), ``refs`` the node index each list refers to (None: no
+    reference; one at or after the node feeds nothing)."""
+    costs: object
+    refs: tuple = (None, None)
+
+
+def propagate(nodes: list[Node], bw: int, bh: int, out=None, stream=None):
+    """mbtree::propagate: the propagated cost of every node as an [n, bh*bw]
+    float32 tensor on the GPU (written into `out` when given)."""
+    torch = _torch()
+    n = len(nodes)
+    nb = bw * bh
+    for k, nd in enumerate(nodes):
+        if int(nd.costs.numel()) < nb * TREE_BLOCK_DTYPE.itemsize:
+            raise PcmeError(PCME_EINVAL, f"propagate: node {k} holds fewer than {nb} blocks")
+    if out is None:
+        dev = nodes[0].costs.device if n else "cuda"
+        out = torch.empty((n, nb), dtype=torch.float32, device=dev)
+    arr = (TreeNode * max(n, 1))()
+    for k, nd in enumerate(nodes):
+        arr[k].blocks = nd.costs.data_ptr()
+        for l in (0, 1):
+            r = nd.refs[l]
+            arr[k].ref[l] = -1 if r is None else int(r)
+    _check(load().pcme_mbtree_propagate(arr, n, bw, bh, out.data_ptr() if n else None, _stream_ptr(stream)))
+    return out
+
+
+def offsets(costs, prop, stream=None):
+    """mbtree::offsets of one node: its costs (device TREE_BLOCK_DTYPE
+    records) and propagated cost (device float32, one per block) -> a
+    float32 tensor of the node's per-block QP offsets."""
+    torch = _torch()
+    n = int(prop.numel())
+    if int(costs.numel()) < n * TREE_BLOCK_DTYPE.itemsize:
+        raise PcmeError(PCME_EINVAL, "offsets: fewer blocks than propagated costs")
+    prop = prop.contiguous()
+    out = torch.empty(n, dtype=torch.float32, device=prop.device)
+    _check(load().pcme_mbtree_offsets(costs.data_ptr(), prop.data_ptr(), n, out.data_ptr(), _stream_ptr(stream)))
+    return out
+
+
 AQ_STRENGTH = 0.6  # mod.rs:109
 
 
@@ -182,4 +231,4 @@ def aq_offsets(clip: DeviceClip, t: int, strength: float = AQ_STRENGTH, stream=N
 
 
 __all__ = ["Complexity", "LowRes", "aq_offsets", "AQ_STRENGTH", "lowres_frames", "complexity", "complexity_batch", "complexity_jobs",
-           "run_complexity", "analyse", "analyse_batch", "tree_jobs", "run_analyse"]
+           "run_complexity", "analyse", "analyse_batch", "tree_jobs", "run_analyse", "Node", "propagate", "offsets"]

@@ -209,3 +209,32 @@ def test_aq_offsets_checks_arguments(pcme_lib):
     assert lib.pcme_aq_offsets(66, 128, 64, 64, 64, 4, 4, 0.6, 64, None) == pcme_lib.PCME_EINVAL  # misaligned y
     assert lib.pcme_aq_offsets(64, 32, 64, 64, 64, 4, 4, 0.6, 64, None) == pcme_lib.PCME_EINVAL  # stride < 16*mbw
     assert lib.pcme_aq_offsets(64, 128, 64, None, 64, 4, 4, 0.6, 64, None) == pcme_lib.PCME_EINVAL
+
+
+def test_mbtree_propagate_and_offsets_check_arguments(pcme_lib):
+    lib = pcme_lib.load()
+    TN = pcme_lib.TreeNode
+    assert ctypes.sizeof(TN) == 16
+    nodes = (TN * 2)()
+    nodes[0].blocks, nodes[1].blocks = 64, 128
+    nodes[0].ref[:] = [-1, -1]
+    nodes[1].ref[:] = [0, -1]
+    if pcme_lib.device_count() == 0:
+        assert lib.pcme_mbtree_propagate(nodes, 2, 4, 4, 256, None) == pcme_lib.PCME_ENODEV
+        assert lib.pcme_mbtree_offsets(64, 256, 16, 512, None) == pcme_lib.PCME_ENODEV
+    assert lib.pcme_mbtree_propagate(nodes, 0, 4, 4, None, None) == pcme_lib.PCME_OK
+    assert lib.pcme_mbtree_propagate(None, 2, 4, 4, 256, None) == pcme_lib.PCME_EINVAL
+    assert lib.pcme_mbtree_propagate(nodes, -1, 4, 4, 256, None) == pcme_lib.PCME_EINVAL
+    assert lib.pcme_mbtree_propagate(nodes, 2, 0, 4, 256, None) == pcme_lib.PCME_EINVAL    # empty grid
+    assert lib.pcme_mbtree_propagate(nodes, 2, 4, 4, None, None) == pcme_lib.PCME_EINVAL   # no prop
+    assert lib.pcme_mbtree_propagate(nodes, 2, 4, 4, 258, None) == pcme_lib.PCME_EINVAL    # misaligned prop
+    nodes[1].ref[1] = -2
+    assert lib.pcme_mbtree_propagate(nodes, 2, 4, 4, 256, None) == pcme_lib.PCME_EINVAL    # ref below -1
+    nodes[1].ref[1] = -1
+    nodes[1].blocks = None
+    assert lib.pcme_mbtree_propagate(nodes, 2, 4, 4, 256, None) == pcme_lib.PCME_EINVAL    # null blocks
+    assert b"node 1" in lib.pcme_last_error()
+    assert lib.pcme_mbtree_offsets(64, 256, 0, 512, None) == pcme_lib.PCME_OK
+    assert lib.pcme_mbtree_offsets(None, 256, 16, 512, None) == pcme_lib.PCME_EINVAL
+    assert lib.pcme_mbtree_offsets(64, 256, -1, 512, None) == pcme_lib.PCME_EINVAL
+    assert lib.pcme_mbtree_offsets(64, 257, 16, 512, None) == pcme_lib.PCME_EINVAL
