@@ -325,12 +325,14 @@ def lookahead_leg(args, pcme, torch, clips, stream, threads):
     """The lowres lookahead of the same frames (lookahead.rs, mbtree.rs): a
     step = LowRes::new of every frame + complexity of every frame against
     its predecessor (the first alone) + the macroblock-tree costs of every
-    frame with its predecessor as list 0.  The C oracle times the same work
-    on the last clip's frames on the host."""
+    frame with its predecessor as list 0 + one tree_offsets (mod.rs:819-840)
+    with the clip's frames as the queue: propagate over all of them, then
+    the first frame's offsets.  The C oracle times the same work on the last
+    clip's frames on the host."""
     import numpy as np
     from pcme import lookahead as L
     sp = stream.cuda_stream
-    lrs, cjobs, tjobs, sums, tblocks = [], [], [], [], []
+    lrs, cjobs, tjobs, sums, tblocks, tnodes, props, toffs = [], [], [], [], [], [], [], []
     blocks_per_step = 0
     for dc in clips:
         lr = L.lowres_frames(dc, stream=sp)
@@ -343,6 +345,9 @@ def lookahead_leg(args, pcme, torch, clips, stream, threads):
         lrs.append(lr)
         sums.append(s)
         tblocks.append(tb)
+        tnodes.append([L.Node(tb[t], (t - 1 if t else None, None)) for t in range(n)])
+        props.append(torch.empty((n, nb), dtype=torch.float32, device="cuda"))
+        toffs.append(torch.empty(nb, dtype=torch.float32, device="cuda"))
         blocks_per_step += n * nb
     ev = []
 
@@ -354,7 +359,7 @@ def lookahead_leg(args, pcme, torch, clips, stream, threads):
 
     def step(record):
         if record:
-            e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
             e[0].record(stream)
         lowres_all()
         if record:
@@ -365,6 +370,12 @@ def lookahead_leg(args, pcme, torch, clips, stream, threads):
         L.run_analyse(tjobs, sp)
         if record:
             e[3].record(stream)
+        for dc, nodes, pr, of, tb in zip(clips, tnodes, props, toffs, tblocks):
+            L.propagate(nodes, dc.mbw, dc.mbh, out=pr, stream=sp)
+            pcme._check(pcme.load().pcme_mbtree_offsets(tb[0].data_ptr(), pr[0].data_ptr(), dc.mbw * dc.mbh,
+                                                        of.data_ptr(), sp))
+        if record:
+            e[4].record(stream)
             ev.append(e)
 
     for dc in clips:
@@ -377,22 +388,23 @@ def lookahead_leg(args, pcme, torch, clips, stream, threads):
         step(True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    ms = [sum(e[i].elapsed_time(e[i + 1]) for e in ev) / len(ev) for i in range(3)]
+    ms = [sum(e[i].elapsed_time(e[i + 1]) for e in ev) / len(ev) for i in range(4)]
     ms_step = 1e3 * dt / args.steps
     in_bytes = sum(dc.n * 256 * dc.mbw * dc.mbh for dc in clips)  # coded luma read once
     lowres_gbs = 1.25 * in_bytes / (ms[0] * 1e-3) / 1e9            # + 1/4 written
     out = {
         "workload": "LowRes::new + complexity (vs previous frame) + mbtree analyse (list 0 = previous frame) "
-                    "of every frame",
+                    "of every frame + mbtree propagate over the frames as one queue + offsets of the first",
         "frames": sum(dc.n for dc in clips), "blocks_per_step": blocks_per_step,
         "ms_per_step": round(ms_step, 3), "mblocks_per_s": round(blocks_per_step / (ms_step * 1e-3) / 1e6, 4),
         "lowres_ms": round(ms[0], 4), "complexity_ms": round(ms[1], 4), "analyse_ms": round(ms[2], 4),
+        "propagate_ms": round(ms[3], 4),
         "lowres_roofline": {"bound": "hbm", "kernel": "lowres_kernel", "achieved": round(lowres_gbs, 1),
                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(lowres_gbs / HBM_PEAK_GBS, 4),
                             "algorithmic_bytes_per_block": 320},
     }
     print(f"[bench] lookahead: {ms_step:.3f} ms/step (lowres {ms[0]:.3f}, complexity {ms[1]:.3f}, "
-          f"analyse {ms[2]:.3f} ms)", file=sys.stderr, flush=True)
+          f"analyse {ms[2]:.3f}, propagate {ms[3]:.3f} ms)", file=sys.stderr, flush=True)
     if not args.no_cpu:
         from oracle import oracle as O
         dc, lr = clips[-1], lrs[-1]
@@ -415,6 +427,23 @@ def lookahead_leg(args, pcme, torch, clips, stream, threads):
                                          f"({8 * dc.mbw}x{8 * dc.mbh}) of the last clip, one frame per thread, "
                                          f"C oracle -O3 -march=x86-64-v3, {reps} repetitions",
                                "seconds": round(secs, 3)}
+        # tree_offsets of the last clip: propagate is single-threaded in the
+        # reference (mbtree.rs:167-223); repeated until about half a second
+        gb_all = [b for b in tblocks[-1].cpu().numpy().view(pcme.TREE_BLOCK_DTYPE).reshape(dc.n, nb)]
+        refs = [(t - 1 if t else -1, -1) for t in range(dc.n)]
+        secs, reps = 0.0, 0
+        while secs < 0.5 and reps < 200:
+            t1 = time.perf_counter()
+            cprop = O.mbtree_propagate(gb_all, refs, dc.mbw, dc.mbh)
+            coff = O.mbtree_offsets(gb_all[0], cprop[0])
+            secs += time.perf_counter() - t1
+            reps += 1
+        gp = props[-1].cpu().numpy()
+        parity = parity and bool(np.array_equal(gp.view(np.uint32), cprop.view(np.uint32)) and
+                                 np.allclose(toffs[-1].cpu().numpy(), coff, rtol=1e-6, atol=1e-5))
+        out["propagate_cpu"] = {"ms": round(1e3 * secs / reps, 4), "cores": 1, "kind": "port",
+                                "sample": f"propagate over the last clip's {dc.n} nodes + offsets of the first, "
+                                          f"C oracle, {reps} repetitions"}
         out["parity_sample"] = parity
     return out
 

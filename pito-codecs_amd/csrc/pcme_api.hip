@@ -790,8 +790,8 @@ int pcme_mbtree_propagate(const pcme_tree_node *nodes, int32_t n, int32_t bw, in
       mask[nd++] = 1 << l;
     }
     if (!nd) continue;
-    const unsigned threads = (unsigned)(nd * nb);
-    hipLaunchKernelGGL(tree_prop_kernel, dim3((threads + 255) / 256), dim3(256), 0, st, nodes[i].blocks,
+    const unsigned tiles = (unsigned)(((bw + 15) / 16) * ((bh + 15) / 16));
+    hipLaunchKernelGGL(tree_prop_kernel, dim3(tiles, (unsigned)nd), dim3(256), 0, st, nodes[i].blocks,
                        prop + (size_t)i * nb, dst[0], dst[1], mask[0], mask[1], dv->dwin + i, bw, bh);
     HIPCHK(hipGetLastError());
   }

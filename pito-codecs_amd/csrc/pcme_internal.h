@@ -39,6 +39,7 @@ struct TreeNodeArgs {
   const pcme_tree_block *b[LA_CHUNK];
 };
 #define TREE_WIN_EMPTY (1 << 29)  // tree_window_kernel's bound of a node no block of which feeds a cell
+#define TREE_LDS 4096             // tree_prop_kernel: sources staged per 16x16 tile (16 B each)
 static_assert(sizeof(CplxArgs) <= 4000 && sizeof(TreeArgs) <= 4000, "lookahead jobs exceed the kernel arguments");
 
 #ifdef __HIP__

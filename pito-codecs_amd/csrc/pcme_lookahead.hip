@@ -512,58 +512,118 @@ __global__ void __launch_bounds__(256) tree_zero_kernel(float *__restrict__ p, s
   if (i < n) p[i] = 0.0f;
 }
 
-// One node of mbtree.rs:169-222.  Thread (d, cell) owns cell `cell` of
-// destination d (dst[d], fed by the lists in mask[d]) and gathers what the
-// node's blocks move into it: the sources in the window `win` allows, in
-// raster order, list 0 before list 1 -- the order the reference adds them
-// into that cell -- each `+=` rounded on its own, so the cell's sum is the
-// reference's to the bit.  mine holds the node's own propagated costs,
-// complete because the nodes after it ran first.
+// What one source block of mbtree.rs:169-222 hands on: the lists that feed
+// the destination (bit l: list l), a = amount * w / 64 (the per-unit-area
+// share) and its MVs.
+struct TreeSrc {
+  float a;
+  uint32_t lists;
+  int16_t mv[4];
+};
+
+__device__ __forceinline__ TreeSrc tree_src(const pcme_tree_block &q, float m, int mask) {
+#pragma clang fp contract(off)
+  TreeSrc e;
+  e.a = 0.0f;
+  e.lists = 0;
+  e.mv[0] = q.mv[0].x;
+  e.mv[1] = q.mv[0].y;
+  e.mv[2] = q.mv[1].x;
+  e.mv[3] = q.mv[1].y;
+  if (q.list == 0 || q.intra == 0) return e;
+  const uint32_t lm = (q.list == 3 ? 3u : (q.list == 1 ? 1u : 2u)) & (uint32_t)mask;
+  if (!lm) return e;
+  const float intra = (float)q.intra;
+  const float inter0 = (float)q.inter;
+  const float inter = inter0 < intra ? inter0 : intra;
+  const float amount = (intra + m) * (intra - inter) / intra;
+  if (amount <= 0.0f) return e;
+  const float wt = q.list == 3 ? 0.5f : 1.0f;
+  e.a = amount * wt / 64.0f;
+  e.lists = lm;
+  return e;
+}
+
+// The share source e (block (sx, sy)) adds into cell (tx, ty), in the
+// reference's list order.
+__device__ __forceinline__ float tree_add(float acc, const TreeSrc &e, int sx, int sy, int tx, int ty) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int l = 0; l < 2; l++) {
+    if (!((e.lists >> l) & 1u)) continue;
+    const int mx = e.mv[2 * l], my = e.mv[2 * l + 1];
+    const int dx = tx - (sx + (mx >> 3)), dy = ty - (sy + (my >> 3));
+    if ((unsigned)dx > 1u || (unsigned)dy > 1u) continue;
+    const int fx = mx & 7, fy = my & 7;
+    const int area = (dx ? fx : 8 - fx) * (dy ? fy : 8 - fy);
+    if (area == 0) continue;
+    acc += e.a * (float)area;
+  }
+  return acc;
+}
+
+// One node of mbtree.rs:169-222.  Workgroup (tile, d) owns a 16x16 tile of
+// cells of destination d (dst[d], fed by the lists in mask[d]); thread
+// (lx, ly) owns one cell and gathers what the node's blocks move into it:
+// the sources the window `win` allows, in raster order, list 0 before list
+// 1 -- the order the reference adds them into that cell -- each `+=`
+// rounded on its own, so the cell's sum is the reference's to the bit.  The
+// sources the tile's cells can see are staged in LDS first (each source's
+// share computed once); a rectangle larger than TREE_LDS falls back to
+// reading the blocks from memory per cell.  mine holds the node's own
+// propagated costs, complete because the nodes after it ran first.
 __global__ void __launch_bounds__(256) tree_prop_kernel(const pcme_tree_block *__restrict__ src,
                                                         const float *__restrict__ mine, float *dst0, float *dst1,
                                                         int mask0, int mask1, const int4 *__restrict__ win, int bw,
                                                         int bh) {
-#pragma clang fp contract(off)
-  const int nb = bw * bh;
-  const int t = (int)(blockIdx.x * 256 + threadIdx.x);
-  const int d = t >= nb;
-  const int cell = t - d * nb;
-  if (d && !dst1) return;
-  if (cell >= nb) return;
+  __shared__ TreeSrc S[TREE_LDS];
+  const int d = (int)blockIdx.y;
   float *dst = d ? dst1 : dst0;
   const int mask = d ? mask1 : mask0;
-  const int tx = cell % bw, ty = cell / bw;
+  const int tiles_x = (bw + 15) >> 4;
+  const int tx0 = ((int)blockIdx.x % tiles_x) << 4, ty0 = ((int)blockIdx.x / tiles_x) << 4;
+  const int tx1 = min(tx0 + 15, bw - 1), ty1 = min(ty0 + 15, bh - 1);
+  const int tx = tx0 + (int)(threadIdx.x & 15), ty = ty0 + (int)(threadIdx.x >> 4);
+  const bool own = tx < bw && ty < bh;
   const int4 w = *win;
+  // sources that land on cell x: bx in [x - qx_max - 1, x - qx_min]
+  const int sx0 = max(0, tx0 - w.y - 1), sx1 = min(bw - 1, tx1 - w.x);
+  const int sy0 = max(0, ty0 - w.w - 1), sy1 = min(bh - 1, ty1 - w.z);
+  const int SX = sx1 - sx0 + 1, SY = sy1 - sy0 + 1;
+  if (SX <= 0 || SY <= 0) return;  // nothing lands on this tile (uniform)
   const int x0 = max(0, tx - w.y - 1), x1 = min(bw - 1, tx - w.x);
   const int y0 = max(0, ty - w.w - 1), y1 = min(bh - 1, ty - w.z);
-  float acc = dst[cell];
-  for (int sy = y0; sy <= y1; sy++)
-    for (int sx = x0; sx <= x1; sx++) {
-      const int k = sy * bw + sx;
-      const pcme_tree_block q = src[k];
-      if (q.list == 0 || q.intra == 0) continue;
-      const int lm = (q.list == 3 ? 3 : (q.list == 1 ? 1 : 2)) & mask;
-      if (!lm) continue;
-      const float intra = (float)q.intra;
-      const float inter0 = (float)q.inter;
-      const float inter = inter0 < intra ? inter0 : intra;
-      const float amount = (intra + mine[k]) * (intra - inter) / intra;
-      if (amount <= 0.0f) continue;
-      const float wt = q.list == 3 ? 0.5f : 1.0f;
-#pragma unroll
-      for (int l = 0; l < 2; l++) {
-        if (!((lm >> l) & 1)) continue;
-        const int mx = q.mv[l].x, my = q.mv[l].y;
-        const int dx = tx - (sx + (mx >> 3)), dy = ty - (sy + (my >> 3));
-        if ((unsigned)dx > 1u || (unsigned)dy > 1u) continue;
-        const int fx = mx & 7, fy = my & 7;
-        const int area = (dx ? fx : 8 - fx) * (dy ? fy : 8 - fy);
-        if (area == 0) continue;
-        const float a = amount * wt / 64.0f;
-        acc += a * (float)area;
+  if (SX * SY > TREE_LDS) {
+    if (!own) return;
+    float acc = dst[ty * bw + tx];
+    for (int sy = y0; sy <= y1; sy++)
+      for (int sx = x0; sx <= x1; sx++) {
+        const int k = sy * bw + sx;
+        const pcme_tree_block q = src[k];
+        if (q.list == 0 || q.intra == 0) continue;
+        acc = tree_add(acc, tree_src(q, mine[k], mask), sx, sy, tx, ty);
       }
+    dst[ty * bw + tx] = acc;
+    return;
+  }
+  for (int j = (int)threadIdx.x; j < SX * SY; j += 256) {
+    const int k = (sy0 + j / SX) * bw + sx0 + j % SX;
+    const pcme_tree_block q = src[k];
+    const bool feeds = q.list != 0 && q.intra != 0;
+    S[j] = tree_src(q, feeds ? mine[k] : 0.0f, mask);
+  }
+  __syncthreads();
+  if (!own) return;
+  float acc = dst[ty * bw + tx];
+  for (int sy = y0; sy <= y1; sy++) {
+    const TreeSrc *row = S + (sy - sy0) * SX - sx0;
+    for (int sx = x0; sx <= x1; sx++) {
+      const TreeSrc e = row[sx];
+      if (!e.lists) continue;
+      acc = tree_add(acc, e, sx, sy, tx, ty);
     }
-  dst[cell] = acc;
+  }
+  dst[ty * bw + tx] = acc;
 }
 
 // mbtree.rs:225-231: -2 log2((max(intra, 1) + prop) / max(intra, 1)).

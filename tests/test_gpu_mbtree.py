@@ -59,6 +59,7 @@ def _bits_equal(got, want, what):
     (2, 12, 30, 17, 600),     # MVs across the whole picture
     (3, 3, 1, 1, 9),
     (4, 70, 8, 6, 16),        # two window launches
+    (5, 4, 120, 68, 300),     # source rectangles beyond the LDS stage: the per-cell path
 ])
 def test_propagate_random_nodes(gpu, oracle, seed, n, bw, bh, mvmax):
     say("propagate random", seed, n, bw, bh, mvmax)
