@@ -35,11 +35,13 @@ __device__ __forceinline__ uint32_t red4(uint32_t v) {
 }
 
 // Sums of a paired search whose lower half runs 4 lanes per candidate and
-// upper half 2 (a 16x8 or 8x16 search beside an 8x8 one).
+// upper half 2 (a 16x8 or 8x16 search beside an 8x8 one); lanes 48-51, idle
+// in the upper half, carry one more 4-lane candidate of the lower search.
 __device__ __forceinline__ uint32_t red42(uint32_t v) {
   v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_QP_1032, 0xF, 0xF, false);
   const uint32_t w = v + (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_QP_2301, 0xF, 0xF, false);
-  return __lane_id() < 32 ? w : v;
+  const uint32_t l = __lane_id();
+  return (l < 32 || (l >> 2) == 12) ? w : v;  // quads on lanes 0-31 and 48-51
 }
 
 __device__ __forceinline__ uint32_t rdlane(uint32_t v, int l) {

@@ -583,14 +583,18 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
   constexpr int LPC = B8 ? 2 : 4;         // reduction code of the cost helpers
   constexpr int BLS = B8 ? 2 : 4;         // lanes per B candidate
   const uint32_t l = lane_id();
-  const bool g = l >= 32;
+  // With an 8x8 B, lanes 48-63 hold no B candidate: lanes 48-51 carry A's
+  // data (an A candidate's 4 lanes) and lanes 52-53 B's, inactive in every
+  // scan; they price the integer best of each search in the half-pel pass.
+  const bool ca = B8 && (l >> 2) == 12u, cb = B8 && (l >> 1) == 26u;
+  const bool g = l >= 32 && !ca;
   Lane L;
   if (!g) {
     L.slot = (int)((l >> 2) & 7u);
     L.sub = (int)(l & 3u);
     L.sx = AW == 16 ? (L.sub & 1) * 8 : 0;
     L.sy = AW == 16 ? (L.sub >> 1) * 4 : L.sub * 4;
-    L.active = true;
+    L.active = !ca;
   } else {
     L.slot = (int)(((l - 32) / BLS) & 7u);
     L.sub = (int)((l - 32) % BLS);
@@ -762,16 +766,38 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
   ensure_sp<BW, BH>(c, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
   v2s sp[16];
   src_pairs(s, sp);
-  {
+  if constexpr (B8) {
+    // slice.rs:1688-1705 in one pass: lanes 48-51 / 52-53 price the integer
+    // best of A / B (the SATD the half-pel round must beat), the scan lanes
+    // the half-pel candidates around it.
+    const int ox = gsel(g, A.x, B.x), oy = gsel(g, A.y, B.y);
+    const int cx = ox + qdx * 2, cy = oy + qdy * 2;
+    const Box ok = g ? B.ok : A.ok;
+    const bool valid = L.active & in_box(ok, cx, cy);
+    const bool centre = ca | cb;
+    Lane Lc = L;
+    Lc.active = L.active | centre;
+    const int px = centre ? ox : cx, py = centre ? oy : cy;
+    const uint32_t k = satd_cost<LPC, true>(c, Lc, sp, bx, by, px, py, valid ? cx : ox, valid ? cy : oy, mvpx,
+                                            mvpy, lsp, gsel(g, A.sx0, B.sx0), gsel(g, A.sy0, B.sy0));
+    A.cost = rdlane(k, 48);
+    B.cost = rdlane(k, 52);
+    uint32_t ka, kb;
+    min_key_pair<B8>(k, valid, L.slot, ka, kb);
+    const int ja = (int)(ka & 7u), jb = (int)(kb & 7u);
+    take_key(A, ka, sq_dx(ja) * 2, sq_dy(ja) * 2);
+    take_key(B, kb, sq_dx(jb) * 2, sq_dy(jb) * 2);
+  } else {
     const int ox = gsel(g, A.x, B.x), oy = gsel(g, A.y, B.y);
     const uint32_t k = satd_cost<LPC, true>(c, L, sp, bx, by, ox, oy, ox, oy, mvpx, mvpy, lsp,
                                             gsel(g, A.sx0, B.sx0), gsel(g, A.sy0, B.sy0));
     A.cost = rdlane(k, 0);
     B.cost = rdlane(k, 32);
   }
-  // slice.rs:1689-1721: half-pel round, then 1 + subpel_rounds quarter-pel.
+  // slice.rs:1689-1721: half-pel round (fused above for an 8x8 B), then
+  // 1 + subpel_rounds quarter-pel.
 #pragma unroll
-  for (int step = 2; step >= 1; step--) {
+  for (int step = B8 ? 1 : 2; step >= 1; step--) {
     const int rounds = step == 1 ? 1 + c.subpel : 1;
     A.act = B.act = true;
     A.it = B.it = 0;
