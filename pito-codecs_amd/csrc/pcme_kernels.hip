@@ -558,6 +558,22 @@ __device__ __forceinline__ void min_key_pair(uint32_t cost, bool valid, int slot
   kb = rdlane(k, 63);
 }
 
+template <int CTL>
+__device__ __forceinline__ uint32_t dpp_min(uint32_t k) {
+  const uint32_t t = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)k, CTL, 0xF, 0xF, false);
+  return t < k ? t : k;
+}
+
+// Least of ka over the wave and least of kb over the wave (keys of lanes
+// outside a set are all ones): each row of 16 reduced with DPP, the four
+// rows read out.
+__device__ __forceinline__ void min_key_two(uint32_t ka, uint32_t kb, uint32_t &ma, uint32_t &mb) {
+  ka = dpp_min<DPP_ROW_MIRROR>(dpp_min<DPP_ROW_HALF_MIRROR>(dpp_min<DPP_QP_2301>(dpp_min<DPP_QP_1032>(ka))));
+  kb = dpp_min<DPP_ROW_MIRROR>(dpp_min<DPP_ROW_HALF_MIRROR>(dpp_min<DPP_QP_2301>(dpp_min<DPP_QP_1032>(kb))));
+  ma = min(min(rdlane(ka, 0), rdlane(ka, 16)), min(rdlane(ka, 32), rdlane(ka, 48)));
+  mb = min(min(rdlane(kb, 0), rdlane(kb, 16)), min(rdlane(kb, 32), rdlane(kb, 48)));
+}
+
 __device__ __forceinline__ int gsel(bool g, int a, int b) { return g ? b : a; }
 
 // Apply a group's key: take the candidate when its cost is below the
@@ -625,6 +641,12 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
     prefetch_win<AW, AH>(c, A.bx, A.by, a0x, a0y, A.wx0, A.wy0, lds);
     prefetch_win<BW, BH>(c, B.bx, B.by, b0x, b0y, B.wx0, B.wy0, lds + LDS_PER_SEARCH / 4);
   }
+  // With an 8x8 B (the full search) the candidate pass's idle slots (3-7,
+  // and lanes 48-51 / 52-53) price the first hexagon round around slot 0's
+  // position (the rounded predictor, which also wins ties): when the pass's
+  // best lands there, that round is already done (hk*).
+  constexpr bool SPEC = B8 && !PR;
+  uint32_t hka = 0xFFFFFFFFu, hkb = 0xFFFFFFFFu;
   {
     // cands = [seed, mvs[0] of the search's own mode]
     const int r0x = (c0x + 2) & ~3, r0y = (c0y + 2) & ~3;
@@ -636,15 +658,29 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
     int hx = 0, hy = 0;
     if constexpr (PR) hint_cand(hv, L.slot - 3, nh, hx, hy);
     const bool hs = PR && L.slot >= 3;
-    const int cx = L.slot == 1 ? r0x : (L.slot == 2 ? r1x : (hs ? hx : px));
-    const int cy = L.slot == 1 ? r0y : (L.slot == 2 ? r1y : (hs ? hy : py));
     const Box ok = g ? B.ok : A.ok;
+    // speculative hex point hi of lane's search: in the hex box of centre0 = p
+    const bool hl = SPEC && ((L.slot >= 3 && L.active) || ca || cb);
+    const int hi = hl ? ((ca | cb) ? 5 : L.slot - 3) : 0;
+    const int shx = px + hex_dx(hi) * 4, shy = py + hex_dy(hi) * 4;
+    const int rr = 4 * c.range + 3;
+    const bool vh = hl && c.range > 0 && (shx >= max(ok.xlo, px - rr)) && (shx <= min(ok.xhi, px + rr)) &&
+                    (shy >= max(ok.ylo, py - rr)) && (shy <= min(ok.yhi, py + rr));
+    const int cx = hl ? shx : (L.slot == 1 ? r0x : (L.slot == 2 ? r1x : (hs ? hx : px)));
+    const int cy = hl ? shy : (L.slot == 1 ? r0y : (L.slot == 2 ? r1y : (hs ? hy : py)));
     const bool valid =
-        L.active & ((L.slot == 0) | (((L.slot <= 2) | (hs & (L.slot - 3 < nh))) & in_box(ok, cx, cy)));
-    const uint32_t k = sad_cost<false, LPC>(c, L, s, bx, by, cx, cy, valid ? cx : px, valid ? cy : py, mvpx, mvpy,
+        !hl & L.active & ((L.slot == 0) | (((L.slot <= 2) | (hs & (L.slot - 3 < nh))) & in_box(ok, cx, cy)));
+    Lane Lk = L;
+    Lk.active = L.active | (SPEC & (ca | cb));
+    const bool use = valid | vh;
+    const uint32_t k = sad_cost<false, LPC>(c, Lk, s, bx, by, cx, cy, use ? cx : px, use ? cy : py, mvpx, mvpy,
                                             lw, 0, 0);
     uint32_t ka, kb;
     min_key_pair<B8>(k, valid, L.slot, ka, kb);
+    if constexpr (SPEC) {
+      const uint32_t hk = vh ? ((k << 3) | (uint32_t)hi) : 0xFFFFFFFFu;
+      min_key_two(g ? 0xFFFFFFFFu : hk, g ? hk : 0xFFFFFFFFu, hka, hkb);
+    }
     A.cost = ka >> 3;
     B.cost = kb >> 3;
     const int wa = (int)(ka & 7u), wb = (int)(kb & 7u);
@@ -700,6 +736,18 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
   const int hdx = hex_dx(L.slot) * 4, hdy = hex_dy(L.slot) * 4;
   A.act = B.act = range > 0 && !PR;
   A.it = B.it = 0;
+  if constexpr (SPEC) {
+    // the first round, priced in the candidate pass, applies when centre0
+    // is the position it was priced around
+    if (A.act && A.x == a0x && A.y == a0y) {
+      const int j = (int)(hka & 7u);
+      A.act = take_key(A, hka, hex_dx(j) * 4, hex_dy(j) * 4) && ++A.it < range;
+    }
+    if (B.act && B.x == b0x && B.y == b0y) {
+      const int j = (int)(hkb & 7u);
+      B.act = take_key(B, hkb, hex_dx(j) * 4, hex_dy(j) * 4) && ++B.it < range;
+    }
+  }
   while (A.act | B.act) {
     if (A.act) ensure_win<AW, AH>(c, A.bx, A.by, A.x, A.y, 2, A.wx0, A.wy0, lds);
     if (B.act) ensure_win<BW, BH>(c, B.bx, B.by, B.x, B.y, 2, B.wx0, B.wy0, lds + LDS_PER_SEARCH / 4);
