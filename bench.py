@@ -346,9 +346,12 @@ def lookahead_leg(args, pcme, torch, clips, stream, threads):
         sums.append(s)
         tblocks.append(tb)
         tnodes.append([L.Node(tb[t], (t - 1 if t else None, None)) for t in range(n)])
-        props.append(torch.empty((n, nb), dtype=torch.float32, device="cuda"))
+        props.append(None)  # set by each step
         toffs.append(torch.empty(nb, dtype=torch.float32, device="cuda"))
         blocks_per_step += n * nb
+    groups = {}
+    for k, dc in enumerate(clips):
+        groups.setdefault((dc.mbw, dc.mbh), []).append(k)
     ev = []
 
     def lowres_all():
@@ -370,10 +373,13 @@ def lookahead_leg(args, pcme, torch, clips, stream, threads):
         L.run_analyse(tjobs, sp)
         if record:
             e[3].record(stream)
-        for dc, nodes, pr, of, tb in zip(clips, tnodes, props, toffs, tblocks):
-            L.propagate(nodes, dc.mbw, dc.mbh, out=pr, stream=sp)
-            pcme._check(pcme.load().pcme_mbtree_offsets(tb[0].data_ptr(), pr[0].data_ptr(), dc.mbw * dc.mbh,
-                                                        of.data_ptr(), sp))
+        # clips of one size go in one call (independent queues share launches)
+        for (mbw, mbh), ks in groups.items():
+            got = L.propagate_queues([tnodes[k] for k in ks], mbw, mbh, stream=sp)
+            for k, pr in zip(ks, got):
+                props[k] = pr
+                pcme._check(pcme.load().pcme_mbtree_offsets(tblocks[k][0].data_ptr(), pr[0].data_ptr(),
+                                                            mbw * mbh, toffs[k].data_ptr(), sp))
         if record:
             e[4].record(stream)
             ev.append(e)

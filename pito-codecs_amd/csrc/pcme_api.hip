@@ -773,27 +773,74 @@ int pcme_mbtree_propagate(const pcme_tree_node *nodes, int32_t n, int32_t bw, in
     hipLaunchKernelGGL(tree_window_kernel, dim3((unsigned)m), dim3(256), 0, st, A, nb, dv->dwin + c0);
     HIPCHK(hipGetLastError());
   }
-  for (int i = n - 1; i >= 0; i--) {
-    // The destinations of node i: a list feeds node ref[l] when that is an
-    // earlier node (mbtree.rs:191-194); two lists on one node share a thread.
-    float *dst[2] = {nullptr, nullptr};
-    int mask[2] = {0, 0}, nd = 0;
+  // Rounds: node i runs once every node that feeds it (a later node whose
+  // reference is i) has run -- its own costs are then complete -- and every
+  // later node feeding one of its destinations has run, so each cell still
+  // receives its adds in the reference's order (nodes last to first).  The
+  // highest node not yet run always qualifies.  Within a round no two jobs
+  // share a destination, and no job's destination is a node of the round.
+  // A single chain runs one node per round; independent queues in one call
+  // (refs never crossing) share rounds.
+  std::vector<std::vector<int>> feeders((size_t)n);
+  for (int j = 0; j < n; j++)
+    for (int l = 0; l < 2; l++) {
+      const int r = nodes[j].ref[l];
+      if (r >= 0 && r < j && (l == 0 || nodes[j].ref[0] != r)) feeders[(size_t)r].push_back(j);
+    }
+  std::vector<char> done((size_t)n, 0);
+  std::vector<int> left((size_t)n, 0);  // feeders of i not yet run
+  for (int i = 0; i < n; i++) left[(size_t)i] = (int)feeders[(size_t)i].size();
+  // unrun feeders of r above node i: counted from the top as nodes run
+  auto ready = [&](int i) {
+    if (left[(size_t)i]) return false;
     for (int l = 0; l < 2; l++) {
       const int r = nodes[i].ref[l];
       if (r < 0 || r >= i) continue;
-      float *p = prop + (size_t)r * nb;
-      if (nd == 1 && dst[0] == p) {
-        mask[0] |= 1 << l;
-        continue;
-      }
-      dst[nd] = p;
-      mask[nd++] = 1 << l;
+      for (int j : feeders[(size_t)r])
+        if (j > i && !done[(size_t)j]) return false;
     }
-    if (!nd) continue;
+    return true;
+  };
+  std::vector<TreePropJob> jobs;
+  std::vector<int> round;
+  int remaining = n;
+  while (remaining > 0) {
+    round.clear();
+    jobs.clear();
+    for (int i = n - 1; i >= 0; i--)
+      if (!done[(size_t)i] && ready(i)) round.push_back(i);
+    for (int i : round) {
+      for (int l = 0; l < 2; l++) {
+        const int r = nodes[i].ref[l];
+        if (r < 0 || r >= i) continue;
+        if (l == 1 && nodes[i].ref[0] == r) continue;  // one job for both lists
+        TreePropJob J;
+        J.src = nodes[i].blocks;
+        J.mine = prop + (size_t)i * nb;
+        J.dst = prop + (size_t)r * nb;
+        J.win = dv->dwin + i;
+        J.mask = (1 << l) | (l == 0 && nodes[i].ref[1] == r ? 2 : 0);
+        J.pad_ = 0;
+        jobs.push_back(J);
+      }
+    }
     const unsigned tiles = (unsigned)(((bw + 15) / 16) * ((bh + 15) / 16));
-    hipLaunchKernelGGL(tree_prop_kernel, dim3(tiles, (unsigned)nd), dim3(256), 0, st, nodes[i].blocks,
-                       prop + (size_t)i * nb, dst[0], dst[1], mask[0], mask[1], dv->dwin + i, bw, bh);
-    HIPCHK(hipGetLastError());
+    for (size_t c0 = 0; c0 < jobs.size(); c0 += TREE_JOBS) {
+      const size_t m = std::min((size_t)TREE_JOBS, jobs.size() - c0);
+      TreePropArgs A;
+      memset(&A, 0, sizeof A);
+      for (size_t k = 0; k < m; k++) A.j[k] = jobs[c0 + k];
+      hipLaunchKernelGGL(tree_prop_kernel, dim3(tiles, (unsigned)m), dim3(256), 0, st, A, bw, bh);
+      HIPCHK(hipGetLastError());
+    }
+    for (int i : round) {
+      done[(size_t)i] = 1;
+      remaining--;
+      for (int l = 0; l < 2; l++) {
+        const int r = nodes[i].ref[l];
+        if (r >= 0 && r < i && (l == 0 || nodes[i].ref[0] != r)) left[(size_t)r]--;
+      }
+    }
   }
   return PCME_OK;
 }

@@ -40,6 +40,20 @@ struct TreeNodeArgs {
 };
 #define TREE_WIN_EMPTY (1 << 29)  // tree_window_kernel's bound of a node no block of which feeds a cell
 #define TREE_LDS 4096             // tree_prop_kernel: sources staged per 16x16 tile (16 B each)
+// One (node, destination) pair of a propagation round.
+struct TreePropJob {
+  const pcme_tree_block *src;  // the node's blocks
+  const float *mine;           // the node's propagated costs
+  float *dst;                  // the destination node's propagated costs
+  const int4 *win;             // the node's window (tree_window_kernel)
+  int mask;                    // lists of the node that feed dst (bit l)
+  int pad_;
+};
+#define TREE_JOBS 64
+struct TreePropArgs {
+  TreePropJob j[TREE_JOBS];
+};
+static_assert(sizeof(TreePropArgs) <= 4000, "propagation jobs exceed the kernel arguments");
 static_assert(sizeof(CplxArgs) <= 4000 && sizeof(TreeArgs) <= 4000, "lookahead jobs exceed the kernel arguments");
 
 #ifdef __HIP__
@@ -82,9 +96,7 @@ __global__ void aq_var_kernel(const uint8_t *__restrict__ y, int ys, const uint8
 __global__ void aq_mean_kernel(float *__restrict__ e, int n, float strength);
 __global__ void tree_window_kernel(const TreeNodeArgs A, int nb, int4 *__restrict__ win);
 __global__ void tree_zero_kernel(float *__restrict__ p, size_t n);
-__global__ void tree_prop_kernel(const pcme_tree_block *__restrict__ src, const float *__restrict__ mine,
-                                 float *dst0, float *dst1, int mask0, int mask1, const int4 *__restrict__ win,
-                                 int bw, int bh);
+__global__ void tree_prop_kernel(const TreePropArgs A, int bw, int bh);
 __global__ void tree_offsets_kernel(const pcme_tree_block *__restrict__ b, const float *__restrict__ prop, int n,
                                     float *__restrict__ out);
 #endif

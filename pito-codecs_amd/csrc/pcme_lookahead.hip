@@ -562,9 +562,12 @@ __device__ __forceinline__ float tree_add(float acc, const TreeSrc &e, int sx, i
   return acc;
 }
 
-// One node of mbtree.rs:169-222.  Workgroup (tile, d) owns a 16x16 tile of
-// cells of destination d (dst[d], fed by the lists in mask[d]); thread
-// (lx, ly) owns one cell and gathers what the node's blocks move into it:
+// One round of mbtree.rs:169-222: job j = one node feeding one destination
+// (A.j[j]: the node's blocks and own costs, the destination's costs, the
+// lists that feed it, the node's window); no two jobs of a round write the
+// same destination or read one another's.  Workgroup (tile, j) owns a 16x16
+// tile of cells of job j's destination; thread (lx, ly) owns one cell and
+// gathers what the node's blocks move into it:
 // the sources the window `win` allows, in raster order, list 0 before list
 // 1 -- the order the reference adds them into that cell -- each `+=`
 // rounded on its own, so the cell's sum is the reference's to the bit.  The
@@ -572,14 +575,14 @@ __device__ __forceinline__ float tree_add(float acc, const TreeSrc &e, int sx, i
 // share computed once); a rectangle larger than TREE_LDS falls back to
 // reading the blocks from memory per cell.  mine holds the node's own
 // propagated costs, complete because the nodes after it ran first.
-__global__ void __launch_bounds__(256) tree_prop_kernel(const pcme_tree_block *__restrict__ src,
-                                                        const float *__restrict__ mine, float *dst0, float *dst1,
-                                                        int mask0, int mask1, const int4 *__restrict__ win, int bw,
-                                                        int bh) {
+__global__ void __launch_bounds__(256) tree_prop_kernel(const TreePropArgs A, int bw, int bh) {
   __shared__ TreeSrc S[TREE_LDS];
-  const int d = (int)blockIdx.y;
-  float *dst = d ? dst1 : dst0;
-  const int mask = d ? mask1 : mask0;
+  const TreePropJob &J = A.j[blockIdx.y];
+  const pcme_tree_block *__restrict__ src = J.src;
+  const float *__restrict__ mine = J.mine;
+  float *dst = J.dst;
+  const int mask = J.mask;
+  const int4 *win = J.win;
   const int tiles_x = (bw + 15) >> 4;
   const int tx0 = ((int)blockIdx.x % tiles_x) << 4, ty0 = ((int)blockIdx.x / tiles_x) << 4;
   const int tx1 = min(tx0 + 15, bw - 1), ty1 = min(ty0 + 15, bh - 1);

@@ -197,6 +197,27 @@ def propagate(nodes: list[Node], bw: int, bh: int, out=None, stream=None):
     return out
 
 
+def propagate_queues(queues: list[list[Node]], bw: int, bh: int, stream=None):
+    """propagate of several independent lookahead queues of one block grid
+    in one call: the library runs every queue's next node in the same
+    launch.  queues = [[Node, ...], ...] with refs local to their queue (a
+    reference at or after its node, or past the queue's end, feeds
+    nothing).  Returns one [len(q), bh*bw] float32 tensor per queue (views
+    of one buffer)."""
+    flat, base = [], 0
+    for q in queues:
+        for k, nd in enumerate(q):
+            refs = tuple(None if (r is None or r < 0 or r >= k) else r + base for r in nd.refs)
+            flat.append(Node(nd.costs, refs))
+        base += len(q)
+    prop = propagate(flat, bw, bh, stream=stream)
+    out, base = [], 0
+    for q in queues:
+        out.append(prop[base:base + len(q)])
+        base += len(q)
+    return out
+
+
 def offsets(costs, prop, stream=None):
     """mbtree::offsets of one node: its costs (device TREE_BLOCK_DTYPE
     records) and propagated cost (device float32, one per block) -> a
@@ -231,4 +252,5 @@ def aq_offsets(clip: DeviceClip, t: int, strength: float = AQ_STRENGTH, stream=N
 
 
 __all__ = ["Complexity", "LowRes", "aq_offsets", "AQ_STRENGTH", "lowres_frames", "complexity", "complexity_batch", "complexity_jobs",
-           "run_complexity", "analyse", "analyse_batch", "tree_jobs", "run_analyse", "Node", "propagate", "offsets"]
+           "run_complexity", "analyse", "analyse_batch", "tree_jobs", "run_analyse", "Node", "propagate", "propagate_queues",
+           "offsets"]

@@ -175,3 +175,29 @@ def test_propagate_replays_in_a_hip_graph(gpu, oracle):
         torch.cuda.synchronize()
         _bits_equal(prop.cpu().numpy(), want, ("replay", rep))
         np.testing.assert_allclose(off.cpu().numpy(), want_off, rtol=1e-6, atol=1e-5)
+
+
+@pytest.mark.parametrize("seed,nq,bw,bh", [(0, 5, 20, 12), (1, 70, 6, 5)])
+def test_propagate_independent_queues_in_one_call(gpu, oracle, seed, nq, bw, bh):
+    """Several queues in one call share the launches (70 queues: a round of
+    more than 64 jobs); each queue's costs equal the oracle's for that
+    queue alone, bit for bit."""
+    import torch
+    from pcme import lookahead as L
+    say("propagate queues", seed, nq, bw, bh)
+    rng = np.random.default_rng(0x7EE5 + seed)
+    queues, host = [], []
+    for q in range(nq):
+        n = int(rng.integers(1, 9))
+        if q % 3 == 0:   # the encoder's P/B queue shape
+            order, refs = _queue(n, 2)
+            nodes, _ = _random_nodes(rng, len(order), bw, bh, 20)
+        else:
+            nodes, refs = _random_nodes(rng, n, bw, bh, 20)
+        dev = [torch.as_tensor(b.view(np.uint8).copy()).cuda() for b in nodes]
+        queues.append([L.Node(d, tuple(None if r < 0 else r for r in rf)) for d, rf in zip(dev, refs)])
+        host.append((nodes, refs))
+    props = L.propagate_queues(queues, bw, bh)
+    torch.cuda.synchronize()
+    for q, ((nodes, refs), pr) in enumerate(zip(host, props)):
+        _bits_equal(pr.cpu().numpy(), oracle.mbtree_propagate(nodes, refs, bw, bh), ("queue", q))
