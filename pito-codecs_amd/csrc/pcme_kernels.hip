@@ -590,12 +590,16 @@ __device__ __forceinline__ bool take_key(PState &S, uint32_t key, int dx, int dy
 // partition) at the same time.  src* = block top-left of each source.
 // nh >= 0: pruned search with hint candidates 0..nh-1 of hv (see search()).
 template <int AW, int AH, int BW, int BH, bool PR>
-__device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B, const uint8_t *srcA,
-                                            const uint8_t *srcB, int sstride, int c0x, int c0y, int c1ax,
-                                            int c1ay, int c1bx, int c1by, uint32_t hv, int nh,
-                                            uint32_t *lds, SRes &ra, SRes &rb) {
-  static_assert(AW * AH == 128 && BW == 8, "pair shapes");
-  constexpr bool B8 = BH == 8;            // B is an 8x8 partition: 2 lanes per candidate
+__device__ __forceinline__ void search_pair(const SCtx &cA, const SCtx &cB, PState &A, PState &B,
+                                            const uint8_t *srcA, const uint8_t *srcB, int sstride, int c0ax,
+                                            int c0ay, int c0bx, int c0by, int c1ax, int c1ay, int c1bx, int c1by,
+                                            uint32_t hv, int nh, uint32_t *lds, SRes &ra, SRes &rb) {
+  // cA / cB: the reference pictures of A and B (the same context for two
+  // partitions of one list, two for a list-0 / list-1 pair); lam, range,
+  // subpel and the plane geometry are common to both.
+  const SCtx &c = cA;
+  static_assert(AW * AH == 128 && (BW * BH == 128 || (BW == 8 && BH == 8)), "pair shapes");
+  constexpr bool B8 = BW == 8 && BH == 8;  // B is an 8x8 partition: 2 lanes per candidate
   constexpr int LPC = B8 ? 2 : 4;         // reduction code of the cost helpers
   constexpr int BLS = B8 ? 2 : 4;         // lanes per B candidate
   const uint32_t l = lane_id();
@@ -604,6 +608,8 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
   // scan; they price the integer best of each search in the half-pel pass.
   const bool ca = B8 && (l >> 2) == 12u, cb = B8 && (l >> 1) == 26u;
   const bool g = l >= 32 && !ca;
+  SCtx cl = cA;  // the lane's search's context, for reads from memory
+  cl.pl[0] = g ? cB.pl[0] : cA.pl[0];
   Lane L;
   if (!g) {
     L.slot = (int)((l >> 2) & 7u);
@@ -614,8 +620,8 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
   } else {
     L.slot = (int)(((l - 32) / BLS) & 7u);
     L.sub = (int)((l - 32) % BLS);
-    L.sx = 0;
-    L.sy = L.sub * 4;
+    L.sx = BW == 16 ? (L.sub & 1) * 8 : 0;
+    L.sy = BW == 16 ? (L.sub >> 1) * 4 : L.sub * 4;
     L.active = !B8 || l < 48;
   }
   SrcRegs s;
@@ -624,8 +630,8 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
   uint32_t *lsp = lw + WIN_BYTES / 4;
   const int bx = gsel(g, A.bx, B.bx), by = gsel(g, A.by, B.by);
   const int mvpx = gsel(g, A.mvpx, B.mvpx), mvpy = gsel(g, A.mvpy, B.mvpy);
-  A.ok = legal_box(c, A.bx, A.by, AW, AH);
-  B.ok = legal_box(c, B.bx, B.by, BW, BH);
+  A.ok = legal_box(cA, A.bx, A.by, AW, AH);
+  B.ok = legal_box(cB, B.bx, B.by, BW, BH);
   // slice.rs:1617-1631 (see search()).
   int a0x = (A.mvpx + 2) & ~3, a0y = (A.mvpy + 2) & ~3;
   if (!in_box(A.ok, a0x, a0y)) { a0x = 0; a0y = 0; }
@@ -635,11 +641,11 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
   A.wx0 = A.wy0 = B.wx0 = B.wy0 = 1 << 20;
   A.sx0 = A.sy0 = B.sx0 = B.sy0 = 1 << 20;
   if constexpr (PR) {
-    prefetch_sp<AW, AH, 3>(c, A.bx, A.by, a0x, a0y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
-    prefetch_sp<BW, BH, 3>(c, B.bx, B.by, b0x, b0y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
+    prefetch_sp<AW, AH, 3>(cA, A.bx, A.by, a0x, a0y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
+    prefetch_sp<BW, BH, 3>(cB, B.bx, B.by, b0x, b0y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
   } else {
-    prefetch_win<AW, AH>(c, A.bx, A.by, a0x, a0y, A.wx0, A.wy0, lds);
-    prefetch_win<BW, BH>(c, B.bx, B.by, b0x, b0y, B.wx0, B.wy0, lds + LDS_PER_SEARCH / 4);
+    prefetch_win<AW, AH>(cA, A.bx, A.by, a0x, a0y, A.wx0, A.wy0, lds);
+    prefetch_win<BW, BH>(cB, B.bx, B.by, b0x, b0y, B.wx0, B.wy0, lds + LDS_PER_SEARCH / 4);
   }
   // With an 8x8 B (the full search) the candidate pass's idle slots (3-7,
   // and lanes 48-51 / 52-53) price the first hexagon round around slot 0's
@@ -649,7 +655,9 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
   uint32_t hka = 0xFFFFFFFFu, hkb = 0xFFFFFFFFu;
   {
     // cands = [seed, mvs[0] of the search's own mode]
-    const int r0x = (c0x + 2) & ~3, r0y = (c0y + 2) & ~3;
+    const int r0ax = (c0ax + 2) & ~3, r0ay = (c0ay + 2) & ~3;
+    const int r0bx = (c0bx + 2) & ~3, r0by = (c0by + 2) & ~3;
+    const int r0x = gsel(g, r0ax, r0bx), r0y = gsel(g, r0ay, r0by);
     const int r1ax = (c1ax + 2) & ~3, r1ay = (c1ay + 2) & ~3;
     const int r1bx = (c1bx + 2) & ~3, r1by = (c1by + 2) & ~3;
     const int r1x = gsel(g, r1ax, r1bx), r1y = gsel(g, r1ay, r1by);
@@ -673,7 +681,7 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
     Lane Lk = L;
     Lk.active = L.active | (SPEC & (ca | cb));
     const bool use = valid | vh;
-    const uint32_t k = sad_cost<false, LPC>(c, Lk, s, bx, by, cx, cy, use ? cx : px, use ? cy : py, mvpx, mvpy,
+    const uint32_t k = sad_cost<false, LPC>(cl, Lk, s, bx, by, cx, cy, use ? cx : px, use ? cy : py, mvpx, mvpy,
                                             lw, 0, 0);
     uint32_t ka, kb;
     min_key_pair<B8>(k, valid, L.slot, ka, kb);
@@ -690,10 +698,10 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
       B.x = (int)rdlane((uint32_t)cx, 32 + BLS * wb);
       B.y = (int)rdlane((uint32_t)cy, 32 + BLS * wb);
     } else {
-      A.x = wa == 1 ? r0x : (wa == 2 ? r1ax : a0x);
-      A.y = wa == 1 ? r0y : (wa == 2 ? r1ay : a0y);
-      B.x = wb == 1 ? r0x : (wb == 2 ? r1bx : b0x);
-      B.y = wb == 1 ? r0y : (wb == 2 ? r1by : b0y);
+      A.x = wa == 1 ? r0ax : (wa == 2 ? r1ax : a0x);
+      A.y = wa == 1 ? r0ay : (wa == 2 ? r1ay : a0y);
+      B.x = wb == 1 ? r0bx : (wb == 2 ? r1bx : b0x);
+      B.y = wb == 1 ? r0by : (wb == 2 ? r1by : b0y);
     }
   }
   if (PR && nh > 5) {
@@ -703,7 +711,7 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
     const Box ok = g ? B.ok : A.ok;
     const int ox = gsel(g, A.x, B.x), oy = gsel(g, A.y, B.y);
     const bool valid = L.active & (L.slot + 5 < nh) & in_box(ok, hx, hy);
-    const uint32_t k = sad_cost<false, LPC>(c, L, s, bx, by, hx, hy, valid ? hx : ox, valid ? hy : oy, mvpx, mvpy,
+    const uint32_t k = sad_cost<false, LPC>(cl, L, s, bx, by, hx, hy, valid ? hx : ox, valid ? hy : oy, mvpx, mvpy,
                                             lw, 0, 0);
     uint32_t ka, kb;
     min_key_pair<B8>(k, valid, L.slot, ka, kb);
@@ -749,8 +757,8 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
     }
   }
   while (A.act | B.act) {
-    if (A.act) ensure_win<AW, AH>(c, A.bx, A.by, A.x, A.y, 2, A.wx0, A.wy0, lds);
-    if (B.act) ensure_win<BW, BH>(c, B.bx, B.by, B.x, B.y, 2, B.wx0, B.wy0, lds + LDS_PER_SEARCH / 4);
+    if (A.act) ensure_win<AW, AH>(cA, A.bx, A.by, A.x, A.y, 2, A.wx0, A.wy0, lds);
+    if (B.act) ensure_win<BW, BH>(cB, B.bx, B.by, B.x, B.y, 2, B.wx0, B.wy0, lds + LDS_PER_SEARCH / 4);
     const int ox = gsel(g, A.x, B.x), oy = gsel(g, A.y, B.y);
     const int cx = ox + hdx, cy = oy + hdy;
     Box hb = g ? B.hb : A.hb;
@@ -773,20 +781,20 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
   // pruned search reads the full-pel plane of its sub-pel region, staged
   // once with margin 3).
   if constexpr (!PR) {
-    prefetch_sp<AW, AH, 3>(c, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
-    prefetch_sp<BW, BH, 3>(c, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
+    prefetch_sp<AW, AH, 3>(cA, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
+    prefetch_sp<BW, BH, 3>(cB, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
   }
   const int qdx = sq_dx(L.slot), qdy = sq_dy(L.slot);
   A.act = B.act = true;
   A.it = B.it = 0;
   while (A.act | B.act) {
     if constexpr (PR) {
-      if (A.act) ensure_sp<AW, AH, 3>(c, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
+      if (A.act) ensure_sp<AW, AH, 3>(cA, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
       if (B.act)
-        ensure_sp<BW, BH, 3>(c, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
+        ensure_sp<BW, BH, 3>(cB, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
     } else {
-      if (A.act) ensure_win<AW, AH>(c, A.bx, A.by, A.x, A.y, 1, A.wx0, A.wy0, lds);
-      if (B.act) ensure_win<BW, BH>(c, B.bx, B.by, B.x, B.y, 1, B.wx0, B.wy0, lds + LDS_PER_SEARCH / 4);
+      if (A.act) ensure_win<AW, AH>(cA, A.bx, A.by, A.x, A.y, 1, A.wx0, A.wy0, lds);
+      if (B.act) ensure_win<BW, BH>(cB, B.bx, B.by, B.x, B.y, 1, B.wx0, B.wy0, lds + LDS_PER_SEARCH / 4);
     }
     const int ox = gsel(g, A.x, B.x), oy = gsel(g, A.y, B.y);
     const int cx = ox + qdx * 4, cy = oy + qdy * 4;
@@ -810,8 +818,8 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
     }
   }
   // Sub-pel planes of both, then the SATD cost at each integer best.
-  ensure_sp<AW, AH>(c, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
-  ensure_sp<BW, BH>(c, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
+  ensure_sp<AW, AH>(cA, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
+  ensure_sp<BW, BH>(cB, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
   v2s sp[16];
   src_pairs(s, sp);
   if constexpr (B8) {
@@ -850,8 +858,8 @@ __device__ __forceinline__ void search_pair(const SCtx &c, PState &A, PState &B,
     A.act = B.act = true;
     A.it = B.it = 0;
     while (A.act | B.act) {
-      if (A.act) ensure_sp<AW, AH>(c, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
-      if (B.act) ensure_sp<BW, BH>(c, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
+      if (A.act) ensure_sp<AW, AH>(cA, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
+      if (B.act) ensure_sp<BW, BH>(cB, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
       const int ox = gsel(g, A.x, B.x), oy = gsel(g, A.y, B.y);
       const int cx = ox + qdx * step, cy = oy + qdy * step;
       const Box ok = g ? B.ok : A.ok;
@@ -1074,7 +1082,7 @@ __device__ __forceinline__ void search_part_12(const SCtx &c, const Nbr &n, cons
   uint32_t pa = mvp<1, 0, 0, 0, 4, 1>(n, ca), pb = mvp<2, 0, 0, 0, 2, 3>(n, cb);
   A.bx = mbx; A.by = mby; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
   B.bx = mbx; B.by = mby; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
-  search_pair<16, 8, 8, 16, PR>(c, A, B, src, src, sstride, mv_x(seed), mv_y(seed), 0, 0, 0, 0, hv, nh, lds, xa,
+  search_pair<16, 8, 8, 16, PR>(c, c, A, B, src, src, sstride, mv_x(seed), mv_y(seed), mv_x(seed), mv_y(seed), 0, 0, 0, 0, hv, nh, lds, xa,
                                 xb);
   ra.mv[0] = ca.mv[0] = pack_mv(xa.x, xa.y); ra.mvp[0] = pa; ra.total = xa.cost;
   rb.mv[0] = cb.mv[0] = pack_mv(xb.x, xb.y); rb.mvp[0] = pb; rb.total = xb.cost;
@@ -1083,8 +1091,46 @@ __device__ __forceinline__ void search_part_12(const SCtx &c, const Nbr &n, cons
   pb = mvp<2, 1, 2, 0, 2, 4>(n, cb);
   A.bx = mbx; A.by = mby + 8; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
   B.bx = mbx + 8; B.by = mby; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
-  search_pair<16, 8, 8, 16, PR>(c, A, B, src + 8 * sstride, src + 8, sstride, mv_x(seed), mv_y(seed), mv_x(ra.mv[0]),
+  search_pair<16, 8, 8, 16, PR>(c, c, A, B, src + 8 * sstride, src + 8, sstride, mv_x(seed), mv_y(seed), mv_x(seed), mv_y(seed), mv_x(ra.mv[0]),
               mv_y(ra.mv[0]), mv_x(rb.mv[0]), mv_y(rb.mv[0]), hv, nh, lds, xa, xb);
+  ra.mv[1] = pack_mv(xa.x, xa.y); ra.mvp[1] = pa; ra.total += xa.cost;
+  rb.mv[1] = pack_mv(xb.x, xb.y); rb.mvp[1] = pb; rb.total += xb.cost;
+}
+
+// search_part (slice.rs:1810-1868) of one mode (PART 1: 16x8, 2: 8x16) on
+// list 0 (ra) and list 1 (rb) together: partition p of both lists in one
+// paired search, each list with its own reference (c[l]), neighbours (n[l])
+// and seed (seed[l]), as analyse_b searches them one after the other.
+template <int PART, class N>
+__device__ __forceinline__ void search_part_lists(const SCtx *c, const N *n, const uint8_t *src, int sstride,
+                                                  int mbx, int mby, const uint32_t *seed, PartRes &ra,
+                                                  PartRes &rb, uint32_t *lds) {
+  constexpr int W = rect_w(PART), H = rect_h(PART);
+  constexpr int X0 = rect_x(PART, 0), Y0 = rect_y(PART, 0), X1 = rect_x(PART, 1), Y1 = rect_y(PART, 1);
+  Cur ca = {}, cb = {};
+#pragma unroll
+  for (int i = 0; i < 4; i++) { ra.mv[i] = ra.mvp[i] = rb.mv[i] = rb.mvp[i] = 0u; }
+  PState A, B;
+  SRes xa, xb;
+  uint32_t pa = mvp<PART, 0, X0 / 4, Y0 / 4, W / 4, shape_of(PART, 0)>(n[0], ca);
+  uint32_t pb = mvp<PART, 0, X0 / 4, Y0 / 4, W / 4, shape_of(PART, 0)>(n[1], cb);
+  A.bx = B.bx = mbx + X0; A.by = B.by = mby + Y0;
+  A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
+  B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
+  const uint8_t *s0 = src + Y0 * sstride + X0;
+  search_pair<W, H, W, H, false>(c[0], c[1], A, B, s0, s0, sstride, mv_x(seed[0]), mv_y(seed[0]), mv_x(seed[1]),
+                                 mv_y(seed[1]), 0, 0, 0, 0, 0u, -1, lds, xa, xb);
+  ra.mv[0] = ca.mv[0] = pack_mv(xa.x, xa.y); ra.mvp[0] = pa; ra.total = xa.cost;
+  rb.mv[0] = cb.mv[0] = pack_mv(xb.x, xb.y); rb.mvp[0] = pb; rb.total = xb.cost;
+  pa = mvp<PART, 1, X1 / 4, Y1 / 4, W / 4, shape_of(PART, 1)>(n[0], ca);
+  pb = mvp<PART, 1, X1 / 4, Y1 / 4, W / 4, shape_of(PART, 1)>(n[1], cb);
+  A.bx = B.bx = mbx + X1; A.by = B.by = mby + Y1;
+  A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
+  B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
+  const uint8_t *s1 = src + Y1 * sstride + X1;
+  search_pair<W, H, W, H, false>(c[0], c[1], A, B, s1, s1, sstride, mv_x(seed[0]), mv_y(seed[0]), mv_x(seed[1]),
+                                 mv_y(seed[1]), mv_x(ra.mv[0]), mv_y(ra.mv[0]), mv_x(rb.mv[0]), mv_y(rb.mv[0]), 0u,
+                                 -1, lds, xa, xb);
   ra.mv[1] = pack_mv(xa.x, xa.y); ra.mvp[1] = pa; ra.total += xa.cost;
   rb.mv[1] = pack_mv(xb.x, xb.y); rb.mvp[1] = pb; rb.total += xb.cost;
 }
@@ -1108,7 +1154,7 @@ __device__ __forceinline__ void search_part_123(const SCtx &c, const Nbr &n, con
   uint32_t pa = mvp<1, 0, 0, 0, 4, 1>(n, c1), pb = mvp<3, 0, 0, 0, 2, 0>(n, c3);
   A.bx = mbx; A.by = mby; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
   B.bx = mbx; B.by = mby; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
-  search_pair<16, 8, 8, 8, false>(c, A, B, src, src, sstride, sx, sy, 0, 0, 0, 0, nohint, -1, lds, xa, xb);
+  search_pair<16, 8, 8, 8, false>(c, c, A, B, src, src, sstride, sx, sy, sx, sy, 0, 0, 0, 0, nohint, -1, lds, xa, xb);
   r1.mv[0] = c1.mv[0] = pack_mv(xa.x, xa.y); r1.mvp[0] = pa; r1.total = xa.cost;
   r3.mv[0] = c3.mv[0] = pack_mv(xb.x, xb.y); r3.mvp[0] = pb; r3.total = xb.cost;
   // step 2: 8x16 left (x 0, y 0, w 2) with 8x8 top-right (x 2, y 0)
@@ -1116,7 +1162,7 @@ __device__ __forceinline__ void search_part_123(const SCtx &c, const Nbr &n, con
   pb = mvp<3, 1, 2, 0, 2, 0>(n, c3);
   A.bx = mbx; A.by = mby; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
   B.bx = mbx + 8; B.by = mby; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
-  search_pair<8, 16, 8, 8, false>(c, A, B, src, src + 8, sstride, sx, sy, 0, 0, mv_x(r3.mv[0]), mv_y(r3.mv[0]),
+  search_pair<8, 16, 8, 8, false>(c, c, A, B, src, src + 8, sstride, sx, sy, sx, sy, 0, 0, mv_x(r3.mv[0]), mv_y(r3.mv[0]),
                                   nohint, -1, lds, xa, xb);
   r2.mv[0] = c2.mv[0] = pack_mv(xa.x, xa.y); r2.mvp[0] = pa; r2.total = xa.cost;
   r3.mv[1] = c3.mv[1] = pack_mv(xb.x, xb.y); r3.mvp[1] = pb; r3.total += xb.cost;
@@ -1125,7 +1171,7 @@ __device__ __forceinline__ void search_part_123(const SCtx &c, const Nbr &n, con
   pb = mvp<3, 2, 0, 2, 2, 0>(n, c3);
   A.bx = mbx; A.by = mby + 8; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
   B.bx = mbx; B.by = mby + 8; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
-  search_pair<16, 8, 8, 8, false>(c, A, B, src + 8 * sstride, src + 8 * sstride, sstride, sx, sy,
+  search_pair<16, 8, 8, 8, false>(c, c, A, B, src + 8 * sstride, src + 8 * sstride, sstride, sx, sy, sx, sy,
                                   mv_x(r1.mv[0]), mv_y(r1.mv[0]), mv_x(r3.mv[0]), mv_y(r3.mv[0]), nohint, -1, lds,
                                   xa, xb);
   r1.mv[1] = pack_mv(xa.x, xa.y); r1.mvp[1] = pa; r1.total += xa.cost;
@@ -1135,7 +1181,7 @@ __device__ __forceinline__ void search_part_123(const SCtx &c, const Nbr &n, con
   pb = mvp<3, 3, 2, 2, 2, 0>(n, c3);
   A.bx = mbx + 8; A.by = mby; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
   B.bx = mbx + 8; B.by = mby + 8; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
-  search_pair<8, 16, 8, 8, false>(c, A, B, src + 8, src + 8 * sstride + 8, sstride, sx, sy, mv_x(r2.mv[0]),
+  search_pair<8, 16, 8, 8, false>(c, c, A, B, src + 8, src + 8 * sstride + 8, sstride, sx, sy, sx, sy, mv_x(r2.mv[0]),
                                   mv_y(r2.mv[0]), mv_x(r3.mv[0]), mv_y(r3.mv[0]), nohint, -1, lds, xa, xb);
   r2.mv[1] = pack_mv(xa.x, xa.y); r2.mvp[1] = pa; r2.total += xa.cost;
   r3.mv[3] = pack_mv(xb.x, xb.y); r3.mvp[3] = pb; r3.total += xb.cost;
@@ -1412,10 +1458,10 @@ __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, con
   }
 }
 
-// The full-search kernel carries no hint code (165 VGPRs, 3 waves per
+// The full-search kernel carries no hint code (140 VGPRs, 3 waves per
 // SIMD); the hinted kernel runs jobs with and without hints (its two MB
-// paths together need 2 waves per SIMD's registers; forcing 3 spills and
-// runs slower).
+// paths together need 2 waves per SIMD's registers, 205 VGPRs; forcing 3
+// spills and runs slower).
 __global__ void __launch_bounds__(64) search_frames_kernel(const DevJob *__restrict__ jobs,
                                                            const int2 *__restrict__ tickets, int ntickets,
                                                            uint32_t *ctl, uint32_t spin_limit) {
@@ -1733,9 +1779,10 @@ __global__ void __launch_bounds__(64) b_analyse_part_kernel(const pcme_b_job *__
   const uint32_t mv16[2] = {rdlane(rec[12 + (lane & 1)], 0), rdlane(rec[12 + (lane & 1)], 1)};
   uint32_t pmv[2][2];
   PartRes rp[2];
+  // both lists' searches in two paired steps (partition 0, then 1)
+  search_part_lists<K + 1>(c, n, J.src, J.sstride, px, py, mv16, rp[0], rp[1], s_lds);
 #pragma unroll
   for (int l = 0; l < 2; l++) {
-    search_part<K + 1, false>(c[l], n[l], J.src, J.sstride, px, py, mv16[l], 0u, -1, rp[l], s_lds);
     pmv[l][0] = rp[l].mv[0];
     pmv[l][1] = rp[l].mv[1];
   }
