@@ -11,6 +11,13 @@
  * hipStream_t, NULL = the default stream) and are asynchronous unless
  * stated otherwise.
  *
+ * Workspaces: pcme_search_part_batch, pcme_b_analyse_batch and
+ * pcme_mbtree_propagate keep a per-device buffer (job table, node windows)
+ * that each call rewrites.  Calls from several host threads are serialised
+ * by the library, but their GPU work is not: calls on one device must use
+ * one stream, or must not overlap in time.  Plans (pcme_plan_*) have their
+ * own workspaces.
+ *
  * Errors: 0 = success, negative = failure (PCME_E*).  The reference's search
  * is infallible and its frame threads turn a panic into a sticky
  * Error::Failed for the frame (enc/mod.rs:228-233, 630-642); a caller maps a

@@ -759,6 +759,7 @@ int pcme_mbtree_propagate(const pcme_tree_node *nodes, int32_t n, int32_t bw, in
   Dev *dv;
   int rc = get_dev(&dv);
   if (rc) return rc;
+  std::lock_guard<std::mutex> g(dv->mu);
   if ((rc = grow(&dv->dwin, &dv->dwin_cap, (size_t)n))) return rc;
   hipStream_t st = (hipStream_t)stream;
   const int nb = bw * bh;
