@@ -352,6 +352,16 @@ def lookahead_leg(args, pcme, torch, clips, stream, threads):
     groups = {}
     for k, dc in enumerate(clips):
         groups.setdefault((dc.mbw, dc.mbh), []).append(k)
+    # node tables and output buffers built once, like the job tables above
+    tables = {}
+    for (mbw, mbh), ks in groups.items():
+        arr, n = L.tree_nodes([tnodes[k] for k in ks])
+        buf = torch.empty((n, mbw * mbh), dtype=torch.float32, device="cuda")
+        tables[(mbw, mbh)] = (arr, n, buf)
+        base = 0
+        for k in ks:
+            props[k] = buf[base:base + clips[k].n]
+            base += clips[k].n
     ev = []
 
     def lowres_all():
@@ -374,12 +384,12 @@ def lookahead_leg(args, pcme, torch, clips, stream, threads):
         if record:
             e[3].record(stream)
         # clips of one size go in one call (independent queues share launches)
-        for (mbw, mbh), ks in groups.items():
-            got = L.propagate_queues([tnodes[k] for k in ks], mbw, mbh, stream=sp)
-            for k, pr in zip(ks, got):
-                props[k] = pr
-                pcme._check(pcme.load().pcme_mbtree_offsets(tblocks[k][0].data_ptr(), pr[0].data_ptr(),
-                                                            mbw * mbh, toffs[k].data_ptr(), sp))
+        for (mbw, mbh), (arr, n, buf) in tables.items():
+            L.run_propagate(arr, n, mbw, mbh, buf, sp)
+        for k in range(len(clips)):
+            nb = clips[k].mbw * clips[k].mbh
+            pcme._check(pcme.load().pcme_mbtree_offsets(tblocks[k][0].data_ptr(), props[k][0].data_ptr(), nb,
+                                                        toffs[k].data_ptr(), sp))
         if record:
             e[4].record(stream)
             ev.append(e)

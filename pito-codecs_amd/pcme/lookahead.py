@@ -197,6 +197,29 @@ def propagate(nodes: list[Node], bw: int, bh: int, out=None, stream=None):
     return out
 
 
+def tree_nodes(queues: list[list[Node]]):
+    """The pcme_tree_node table of several independent queues laid out one
+    after the other (refs local to each queue; a reference at or after its
+    node, or past the queue's end, feeds nothing): (ctypes array, n)."""
+    n = sum(len(q) for q in queues)
+    arr = (TreeNode * max(n, 1))()
+    base = 0
+    for q in queues:
+        for k, nd in enumerate(q):
+            arr[base + k].blocks = nd.costs.data_ptr()
+            for l in (0, 1):
+                r = nd.refs[l]
+                arr[base + k].ref[l] = -1 if (r is None or r < 0 or r >= k) else r + base
+        base += len(q)
+    return arr, n
+
+
+def run_propagate(arr, n: int, bw: int, bh: int, prop, stream=None) -> None:
+    """pcme_mbtree_propagate over a tree_nodes table into prop (device,
+    [n, bh*bw] float32)."""
+    _check(load().pcme_mbtree_propagate(arr, n, bw, bh, prop.data_ptr() if n else None, _stream_ptr(stream)))
+
+
 def propagate_queues(queues: list[list[Node]], bw: int, bh: int, stream=None):
     """propagate of several independent lookahead queues of one block grid
     in one call: the library runs every queue's next node in the same
@@ -204,13 +227,11 @@ def propagate_queues(queues: list[list[Node]], bw: int, bh: int, stream=None):
     reference at or after its node, or past the queue's end, feeds
     nothing).  Returns one [len(q), bh*bw] float32 tensor per queue (views
     of one buffer)."""
-    flat, base = [], 0
-    for q in queues:
-        for k, nd in enumerate(q):
-            refs = tuple(None if (r is None or r < 0 or r >= k) else r + base for r in nd.refs)
-            flat.append(Node(nd.costs, refs))
-        base += len(q)
-    prop = propagate(flat, bw, bh, stream=stream)
+    torch = _torch()
+    arr, n = tree_nodes(queues)
+    dev = next((nd.costs.device for q in queues for nd in q), "cuda")
+    prop = torch.empty((n, bw * bh), dtype=torch.float32, device=dev)
+    run_propagate(arr, n, bw, bh, prop, stream)
     out, base = [], 0
     for q in queues:
         out.append(prop[base:base + len(q)])
@@ -253,4 +274,4 @@ def aq_offsets(clip: DeviceClip, t: int, strength: float = AQ_STRENGTH, stream=N
 
 __all__ = ["Complexity", "LowRes", "aq_offsets", "AQ_STRENGTH", "lowres_frames", "complexity", "complexity_batch", "complexity_jobs",
            "run_complexity", "analyse", "analyse_batch", "tree_jobs", "run_analyse", "Node", "propagate", "propagate_queues",
-           "offsets"]
+           "tree_nodes", "run_propagate", "offsets"]
