@@ -623,6 +623,11 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
         "hints_ms": round(hints_ms, 3), "search_ms": round(search_ms, 3),
         "decision_agreement": round(j["agreement"], 5),
         "part_agreement": round(j["part_agreement"], 5),
+        # hinted mode of the pruned MBs: 16x16 only / 16x8+8x16 (two modes) / 8x8
+        "pruned_modes": [round(float(x), 4) for x in
+                         np.array([1, 0, 0, 0, 0, 1, 1, 0, 0, 0, 0, 1]).reshape(3, 4) @
+                         np.bincount(hs["part"][hs["prune"] != 0], minlength=4) /
+                         max(int((hs["prune"] != 0).sum()), 1)],
         "jcost_increase": round(j["jcost_increase"], 6),
         "pruned_fraction": round(float((hs["prune"] != 0).mean()), 5),
         "ncand": args.ncand, "confidence_threshold": args.thr, "mv_mass_threshold": thr_mv,
