@@ -228,6 +228,10 @@ def propagate_queues(queues: list[list[Node]], bw: int, bh: int, stream=None):
     nothing).  Returns one [len(q), bh*bw] float32 tensor per queue (views
     of one buffer)."""
     torch = _torch()
+    for qi, q in enumerate(queues):
+        for k, nd in enumerate(q):
+            if int(nd.costs.numel()) < bw * bh * TREE_BLOCK_DTYPE.itemsize:
+                raise PcmeError(PCME_EINVAL, f"propagate_queues: queue {qi} node {k} holds fewer than {bw * bh} blocks")
     arr, n = tree_nodes(queues)
     dev = next((nd.costs.device for q in queues for nd in q), "cuda")
     prop = torch.empty((n, bw * bh), dtype=torch.float32, device=dev)

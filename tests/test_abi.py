@@ -238,3 +238,13 @@ def test_mbtree_propagate_and_offsets_check_arguments(pcme_lib):
     assert lib.pcme_mbtree_offsets(None, 256, 16, 512, None) == pcme_lib.PCME_EINVAL
     assert lib.pcme_mbtree_offsets(64, 256, -1, 512, None) == pcme_lib.PCME_EINVAL
     assert lib.pcme_mbtree_offsets(64, 257, 16, 512, None) == pcme_lib.PCME_EINVAL
+
+
+def test_propagate_queues_rejects_short_nodes(pcme_lib):
+    """A node whose blocks tensor is shorter than the grid is refused on the
+    host, before any device call."""
+    import torch
+    from pcme import lookahead as L
+    short = torch.zeros(19 * 20, dtype=torch.uint8)
+    with pytest.raises(pcme_lib.PcmeError):
+        L.propagate_queues([[L.Node(short)]], 5, 4)
