@@ -98,6 +98,10 @@ CASES = [
     (200, 16, 3, 16, 1, True, 28, dict(nrect=2, rect_lo=4, rect_hi=12, vmax=2.0, pan_max=1.0)),
     (16, 200, 3, 16, 1, True, 28, dict(nrect=2, rect_lo=4, rect_hi=12, vmax=2.0, pan_max=1.0)),
     (100, 70, 3, 16, 1, True, 28, dict(nrect=2, rect_lo=8, rect_hi=30, vmax=3.0, pan_max=1.0)),
+    # tiny ranges with partitions: hexagon boxes of +-1 and +-2 px around each centre (the first
+    # round priced in the candidate pass must clamp like the descent), motion beyond the range
+    (320, 240, 4, 1, 1, True, 27, dict(nrect=4, rect_lo=24, rect_hi=80, vmax=6.0, pan_max=2.0)),
+    (320, 240, 4, 2, 1, True, 32, dict(nrect=4, rect_lo=24, rect_hi=80, vmax=6.0, pan_max=2.0)),
 ]
 
 
