@@ -170,7 +170,8 @@ struct Plan {
   int ntickets = 0;
   uint32_t *xchg = nullptr;
   size_t nwords = 0;
-  uint32_t *ctl = nullptr;  // [0] tickets taken, [1] stall flag, [2] epoch, [3] clear request
+  uint32_t *ctl = nullptr;  // [1] stall flag, [2] epoch, [3] clear request, [8..15] tickets taken per
+                            // queue, [16..24] queue q's tickets = [ctl[16+q], ctl[17+q]) (CTL_BYTES)
   int resident = 0;         // resident waves of the search kernel on this device
   bool hinted = false;      // some job carries pruner hints: the hinted kernel
   int grid = 0;             // waves launched (persistent grid)
@@ -228,17 +229,36 @@ static int plan_create(Dev *dv, const pcme_frame_job *jobs, int njobs, Plan **ou
     words += (size_t)J.prm.mbw * J.prm.mbh * 4;
     maxh = std::max(maxh, J.prm.mbh);
   }
-  // Tickets in (row, job) order: a row's dependency (row-1 of the same job)
-  // always has a smaller ticket.
+  // Eight ticket queues, one per XCD (a wave takes its own XCD's queue
+  // first): queue q holds a contiguous run of jobs of about an eighth of
+  // the MBs, so neighbouring rows of a frame, and a frame and the next one
+  // (whose reference it is), share an L2.  Within a queue tickets are in
+  // (row, job) order: a row's dependency (row-1 of the same job, in the same
+  // queue) always has a smaller ticket.
+  std::vector<int> qof((size_t)njobs);
+  {
+    size_t total = 0, cum = 0;
+    for (int i = 0; i < njobs; i++) total += (size_t)dj[(size_t)i].mbw * dj[(size_t)i].mbh;
+    for (int i = 0; i < njobs; i++) {
+      const size_t m = (size_t)dj[(size_t)i].mbw * dj[(size_t)i].mbh;
+      qof[(size_t)i] = (int)std::min<size_t>(PCME_QUEUES - 1, (cum + m / 2) * PCME_QUEUES / std::max<size_t>(total, 1));
+      cum += m;
+    }
+  }
   std::vector<int2> tk;
   tk.reserve(words / 4);
-  for (int y = 0; y < maxh; y++)
-    for (int i = 0; i < njobs; i++)
-      if (y < dj[(size_t)i].mbh) tk.push_back(make_int2(i, y));
+  uint32_t qoff[PCME_QUEUES + 1];
+  for (int q = 0; q < PCME_QUEUES; q++) {
+    qoff[q] = (uint32_t)tk.size();
+    for (int y = 0; y < maxh; y++)
+      for (int i = 0; i < njobs; i++)
+        if (qof[(size_t)i] == q && y < dj[(size_t)i].mbh) tk.push_back(make_int2(i, y));
+  }
+  qoff[PCME_QUEUES] = (uint32_t)tk.size();
   p->ntickets = (int)tk.size();
   p->nwords = words;
   p->grid = std::min(p->ntickets, p->resident);
-  if (hipMalloc((void **)&p->xchg, words * 4) != hipSuccess || hipMalloc((void **)&p->ctl, 64) != hipSuccess ||
+  if (hipMalloc((void **)&p->xchg, words * 4) != hipSuccess || hipMalloc((void **)&p->ctl, CTL_BYTES) != hipSuccess ||
       hipMalloc((void **)&p->djobs, dj.size() * sizeof(DevJob)) != hipSuccess ||
       hipMalloc((void **)&p->dtickets, tk.size() * sizeof(int2)) != hipSuccess) {
     plan_free(p);
@@ -250,7 +270,8 @@ static int plan_create(Dev *dv, const pcme_frame_job *jobs, int njobs, Plan **ou
     off += (size_t)dj[(size_t)i].mbw * dj[(size_t)i].mbh * 4;
   }
   // ctl[2] = 0: the first launch takes tag 1 and clears the exchange words.
-  if (hipMemset(p->ctl, 0, 64) != hipSuccess ||
+  if (hipMemset(p->ctl, 0, CTL_BYTES) != hipSuccess ||
+      hipMemcpy(p->ctl + 16, qoff, sizeof qoff, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(p->djobs, dj.data(), dj.size() * sizeof(DevJob), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(p->dtickets, tk.data(), tk.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) {
     plan_free(p);

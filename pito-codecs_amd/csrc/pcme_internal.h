@@ -25,6 +25,11 @@ struct DevJob {
   int pad_;
 };
 
+// Ticket queues of the frame kernels (one per XCD) and the size of a
+// plan's control block (see frames_body).
+#define PCME_QUEUES 8
+#define CTL_BYTES 128
+
 // Lookahead jobs travel in the kernel arguments, LA_CHUNK per launch
 // (64 x 40 B of the 4 KiB kernel-argument space).
 #define LA_CHUNK 64

@@ -1362,15 +1362,33 @@ __device__ __forceinline__ uint32_t load_hint(const pcme_hint *h) {
 template <bool HINTS>
 __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
                                             int ntickets, uint32_t *ctl, uint32_t spin_limit, uint32_t *s_lds) {
-  uint32_t *counter = ctl;
   uint32_t *err = ctl + 1;
   const uint32_t tag = ctl[2] & 0xFFu;  // this launch's epoch (set by prep_kernel)
   const int l = (int)lane_id();
+  // Ticket queue q holds tickets [ctl[16+q], ctl[17+q]), taken in order
+  // through the counter ctl[8+q].  Workgroups are dealt to the eight XCDs
+  // in turn, so a wave starts with its own XCD's queue; an exhausted queue
+  // passes the wave to the next, and after all eight it ends.  Any queue's
+  // ticket depends only on earlier tickets of the same queue, which running
+  // waves took, so taking from another XCD's queue keeps the walk
+  // deadlock-free.
+  uint32_t q = blockIdx.x & (PCME_QUEUES - 1);
+  int passed = 0;
   for (;;) {
-    uint32_t t = 0;
-    if (l == 0) t = atomicAdd(counter, 1u);
-    t = rdlane(t, 0);
-    if (t >= (uint32_t)ntickets) return;
+    uint32_t t = 0xFFFFFFFFu;
+    while (passed < PCME_QUEUES) {
+      uint32_t v = 0;
+      if (l == 0) v = atomicAdd(ctl + 8 + q, 1u);
+      v = rdlane(v, 0);
+      const uint32_t lo = ctl[16 + q], hi = ctl[17 + q];
+      if (v < hi - lo) {
+        t = lo + v;
+        break;
+      }
+      q = (q + 1) & (PCME_QUEUES - 1);
+      passed++;
+    }
+    if (passed >= PCME_QUEUES || t >= (uint32_t)ntickets) return;
     {
       uint32_t e = 0;
       if (l == 0) e = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -2152,9 +2170,7 @@ __global__ void __launch_bounds__(256) pred_frame_kernel(const uint8_t *__restri
   }
 }
 
-// Reset the ticket counter and the error flag; optionally clear the
-// exchange words (on tag wrap-around).
-// Per launch: reset the ticket counter and the stall flag and advance the
+// Per launch: reset the ticket counters and the stall flag and advance the
 // launch epoch ctl[2], whose low byte tags the exchange words.  Tag 0 is
 // skipped; when the tag wraps to 1, ctl[3] asks clear_kernel to zero the
 // exchange words so no word of 255 launches ago can pass for a fresh one.
@@ -2168,6 +2184,7 @@ __global__ void prep_kernel(uint32_t *ctl) {
     ctl[2] = e;
     ctl[3] = (e & 0xFFu) == 1u ? 1u : 0u;
   }
+  if (blockIdx.x == 0 && threadIdx.x < PCME_QUEUES) ctl[8 + threadIdx.x] = 0u;  // per-queue ticket counters
 }
 
 __global__ void __launch_bounds__(256) clear_kernel(const uint32_t *ctl, uint32_t *xchg, size_t nwords) {
