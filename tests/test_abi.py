@@ -248,3 +248,18 @@ def test_propagate_queues_rejects_short_nodes(pcme_lib):
     short = torch.zeros(19 * 20, dtype=torch.uint8)
     with pytest.raises(pcme_lib.PcmeError):
         L.propagate_queues([[L.Node(short)]], 5, 4)
+
+
+def test_tree_nodes_lays_out_independent_queues(pcme_lib):
+    """tree_nodes concatenates queues and shifts their refs; a ref at or
+    after its node, past its queue, or None becomes -1."""
+    import torch
+    from pcme import lookahead as L
+    b = torch.zeros(8 * 20, dtype=torch.uint8)
+    q0 = [L.Node(b), L.Node(b, (0, None)), L.Node(b, (1, 0))]
+    q1 = [L.Node(b, (None, 1)), L.Node(b, (0, 5)), L.Node(b, (1, 0))]
+    arr, n = L.tree_nodes([q0, q1])
+    assert n == 6
+    refs = [tuple(arr[k].ref) for k in range(n)]
+    assert refs == [(-1, -1), (0, -1), (1, 0), (-1, -1), (3, -1), (-1, 3)]
+    assert all(arr[k].blocks == b.data_ptr() for k in range(n))
