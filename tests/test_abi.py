@@ -261,5 +261,5 @@ def test_tree_nodes_lays_out_independent_queues(pcme_lib):
     arr, n = L.tree_nodes([q0, q1])
     assert n == 6
     refs = [tuple(arr[k].ref) for k in range(n)]
-    assert refs == [(-1, -1), (0, -1), (1, 0), (-1, -1), (3, -1), (-1, 3)]
+    assert refs == [(-1, -1), (0, -1), (1, 0), (-1, -1), (3, -1), (4, 3)]
     assert all(arr[k].blocks == b.data_ptr() for k in range(n))
