@@ -11,12 +11,19 @@
  * hipStream_t, NULL = the default stream) and are asynchronous unless
  * stated otherwise.
  *
- * Workspaces: pcme_search_part_batch, pcme_b_analyse_batch and
- * pcme_mbtree_propagate keep a per-device buffer (job table, node windows)
- * that each call rewrites.  Calls from several host threads are serialised
- * by the library, but their GPU work is not: calls on one device must use
- * one stream, or must not overlap in time.  Plans (pcme_plan_*) have their
- * own workspaces.
+ * Threads and streams: every entry point may be called from several host
+ * threads at once, each on its own stream, as the reference's frame threads
+ * call its search (enc/mod.rs:207-238).  The per-call entry points
+ * (pcme_search_batch, pcme_search_part_batch, pcme_b_analyse_batch,
+ * pcme_mbtree_propagate) take a workspace of their own for each call (job
+ * table, node windows) from a per-device pool; a workspace returns to the
+ * pool when the call's last kernel has completed (a HIP event), so calls on
+ * different streams run concurrently and never share a buffer a running
+ * kernel reads.  Launches of one plan (pcme_plan_*, and the plan
+ * pcme_search_frames caches per device) share that plan's workspace and
+ * are serialised on the GPU: each eager launch waits for the previous one,
+ * whatever stream it was issued on.  Results never depend on the stream,
+ * the thread or the batch composition.
  *
  * Errors: 0 = success, negative = failure (PCME_E*).  The reference's search
  * is infallible and its frame threads turn a panic into a sticky
@@ -171,13 +178,22 @@ int pcme_luma_planes(const uint8_t *src, int64_t src_pitch, int32_t sstride,
  * array; the library copies it.  Requires 1 <= me_range <= 1024,
  * 0 <= subpel_rounds <= 8, 0 <= qp <= 51, mbw, mbh >= 1.  Blocks until the
  * job table is uploaded; the search itself runs asynchronously on stream.
- * Calls on one device are serialised by the library. */
+ * The device's cached plan is reused while the job table is unchanged; its
+ * launches run one after another on the GPU (see above).  A stall of an
+ * asynchronous call is kept until pcme_search_frames_check reports it. */
 int pcme_search_frames(const pcme_frame_job *jobs, int njobs, void *stream);
 
 /* Same as pcme_search_frames, then waits for the stream and reports a
  * dependency-wait stall (PCME_ESTALL) if one happened. */
 int pcme_search_frames_sync(const pcme_frame_job *jobs, int njobs,
                             void *stream);
+
+/* Waits for `stream` and for the last launch of the current device's
+ * cached pcme_search_frames plan, and returns PCME_ESTALL if any launch of
+ * that plan since the previous check (or pcme_search_frames_sync) stalled;
+ * the flag is then cleared.  PCME_OK when no plan is cached.  No
+ * counterpart in the reference (its search cannot stall). */
+int pcme_search_frames_check(void *stream);
 
 /* Plans: the same ME-only field search, split into a set-up step and a
  * launch step, for callers that replay one frame-step layout many times
@@ -190,10 +206,14 @@ int pcme_search_frames_sync(const pcme_frame_job *jobs, int njobs,
  * change between launches.
  * pcme_plan_launch enqueues kernels only -- no host synchronisation, copies
  * or memsets -- so a stream capture of it replays correctly.  Launches of one
- * plan must not overlap (they share the plan's workspace); different plans
- * may run concurrently.
- * pcme_plan_check waits for `stream` and returns PCME_ESTALL if a
- * dependency wait of the last launch exceeded its bound.
+ * plan share the plan's workspace: an eager launch makes its stream wait for
+ * the plan's previous eager launch (a HIP event), so they never overlap;
+ * inside a stream capture the graph's order applies.  Different plans may
+ * run concurrently.
+ * pcme_plan_check waits for `stream` and the plan's last launch and returns
+ * PCME_ESTALL if a dependency wait of any launch since the previous check
+ * exceeded its bound (a sticky flag; replays of a captured launch set it
+ * too), clearing it.
  * pcme_plan_set_grid sets the persistent grid (waves; 0 = every resident
  * wave) and returns the value in effect.  pcme_plan_autotune times the plan
  * over a set of grid sizes (`reps` launches each, on `stream`) and keeps the
@@ -436,8 +456,12 @@ typedef struct pcme_tree_node {
 /* mbtree::propagate (mbtree.rs:167-223; the encoder's tree_offsets,
  * mod.rs:819-840, runs it over the lookahead queue): prop (device, n*bw*bh
  * floats, node-major, raster order) receives every node's propagated cost.
- * Bit-identical to the reference.  Nodes run last to first, one launch
- * each, on `stream`. */
+ * Bit-identical to the reference.  The nodes run in rounds, one launch per
+ * round on `stream`: a node joins a round once every node that feeds it has
+ * run and every later node that feeds one of its destinations has run, so
+ * each cell still receives its adds in the reference's order (nodes last to
+ * first, blocks in raster order, list 0 before list 1).  One chain is one
+ * node per round; independent queues in one call share rounds. */
 int pcme_mbtree_propagate(const pcme_tree_node *nodes, int32_t n, int32_t bw, int32_t bh, float *prop,
                           void *stream);
 

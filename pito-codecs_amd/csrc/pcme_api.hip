@@ -37,27 +37,35 @@ void set_error(const char *msg) { snprintf(g_err, sizeof g_err, "%s", msg); }
 
 struct Plan;
 
-// Per-device state: the per-call search workspace, the upload staging and
-// the plan cached by pcme_search_frames.
+// One per-call workspace: a device buffer for a call's job table (or node
+// windows) and the pinned host staging of its upload.  A call takes a free
+// workspace, uploads into it, launches on the caller's stream and records
+// `ev` there after its last kernel; the workspace is free again once `ev`
+// has completed.  So calls on different streams never share a buffer that
+// a running kernel still reads.
+struct CallWs {
+  void *dev = nullptr;
+  size_t cap = 0;
+  unsigned char *host = nullptr;
+  size_t hcap = 0;
+  hipEvent_t ev = nullptr;
+  bool recorded = false;
+};
+
+// Per-device state: the per-call workspaces, the plan cached by
+// pcme_search_frames and the event that orders its launches.
 struct Dev {
   int id = -1;
   std::mutex mu;
   int resident_waves[2] = {0, 0};  // full-search / hinted kernel
-  pcme_search_job *dsj = nullptr;
-  size_t dsj_cap = 0;
-  pcme_part_job *dpj = nullptr;     // per-call search_part jobs
-  size_t dpj_cap = 0;
-  pcme_b_job *dbj = nullptr;        // B macroblock jobs
-  size_t dbj_cap = 0;
-  int4 *dwin = nullptr;             // macroblock-tree landing windows, one per node
-  size_t dwin_cap = 0;
-  unsigned char *hstage = nullptr;  // pinned upload staging
-  size_t hstage_cap = 0;
-  hipEvent_t stage_ev = nullptr;
-  bool stage_pending = false;
-  Plan *cached = nullptr;           // plan of the last pcme_search_frames call
+  std::vector<CallWs *> ws;        // per-call workspaces (at most kMaxWs)
+  Plan *cached = nullptr;          // plan of the last pcme_search_frames call
   std::vector<pcme_frame_job> cached_jobs;
 };
+
+// Workspaces per device: beyond this many calls in flight a call waits for
+// the oldest one to complete.
+static const size_t kMaxWs = 64;
 
 static std::mutex g_mu;
 static std::map<int, Dev *> g_devs;
@@ -94,48 +102,73 @@ static int get_dev(Dev **out) {
     if (per_cu < 1) per_cu = 1;
     dv->resident_waves[v] = per_cu * p.multiProcessorCount;
   }
-  HIPCHK(hipEventCreateWithFlags(&dv->stage_ev, hipEventDisableTiming));
   g_devs[d] = dv;
   *out = dv;
   return PCME_OK;
 }
 
-template <typename T>
-static int grow(T **p, size_t *cap, size_t n) {
-  if (*cap >= n && *p) return PCME_OK;
-  size_t nc = std::max(n, *cap * 2);
-  if (*p) {
-    HIPCHK(hipDeviceSynchronize());
-    HIPCHK(hipFree(*p));
-    *p = nullptr;
-    *cap = 0;
+// A free workspace of at least `bytes` (device) and `hbytes` (pinned host)
+// for a call on this device; the caller holds dv->mu.  Prefers a free
+// workspace that is large enough, grows a free one otherwise, creates one
+// while fewer than kMaxWs exist, and else waits for the oldest to finish.
+static int take_ws(Dev *dv, size_t bytes, size_t hbytes, CallWs **out) {
+  CallWs *pick = nullptr;
+  for (CallWs *w : dv->ws) {
+    bool free_ = !w->recorded || hipEventQuery(w->ev) == hipSuccess;
+    if (!free_) continue;
+    if (!pick || (w->cap >= bytes && pick->cap < bytes)) pick = w;
+    if (pick->cap >= bytes && pick->hcap >= hbytes) break;
   }
-  if (hipMalloc((void **)p, nc * sizeof(T)) != hipSuccess)
-    return fail(PCME_ENOMEM, "hipMalloc of %zu bytes failed", nc * sizeof(T));
-  *cap = nc;
+  if (!pick && dv->ws.size() < kMaxWs) {
+    pick = new CallWs();
+    if (hipEventCreateWithFlags(&pick->ev, hipEventDisableTiming) != hipSuccess) {
+      delete pick;
+      return fail(PCME_EHIP, "hipEventCreateWithFlags failed");
+    }
+    dv->ws.push_back(pick);
+  }
+  if (!pick) {
+    pick = dv->ws.front();
+    HIPCHK(hipEventSynchronize(pick->ev));
+  }
+  pick->recorded = false;
+  if (pick->cap < bytes) {
+    if (pick->dev) HIPCHK(hipFree(pick->dev));
+    pick->dev = nullptr;
+    pick->cap = 0;
+    if (hipMalloc(&pick->dev, bytes) != hipSuccess) return fail(PCME_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
+    pick->cap = bytes;
+  }
+  if (pick->hcap < hbytes) {
+    if (pick->host) HIPCHK(hipHostFree(pick->host));
+    pick->host = nullptr;
+    pick->hcap = 0;
+    HIPCHK(hipHostMalloc((void **)&pick->host, hbytes, hipHostMallocDefault));
+    pick->hcap = hbytes;
+  }
+  *out = pick;
   return PCME_OK;
 }
 
-// Copy `bytes` from host memory to device memory on `stream` through the
-// pinned staging buffer (waits for the previous staged copy first).
-static int upload(Dev *dv, void *dst, const void *src, size_t bytes, hipStream_t stream) {
-  if (dv->stage_pending) {
-    HIPCHK(hipEventSynchronize(dv->stage_ev));
-    dv->stage_pending = false;
-  }
-  if (dv->hstage_cap < bytes) {
-    if (dv->hstage) HIPCHK(hipHostFree(dv->hstage));
-    dv->hstage = nullptr;
-    size_t nc = std::max(bytes, dv->hstage_cap * 2);
-    HIPCHK(hipHostMalloc((void **)&dv->hstage, nc, hipHostMallocDefault));
-    dv->hstage_cap = nc;
-  }
-  memcpy(dv->hstage, src, bytes);
-  HIPCHK(hipMemcpyAsync(dst, dv->hstage, bytes, hipMemcpyHostToDevice, stream));
-  HIPCHK(hipEventRecord(dv->stage_ev, stream));
-  dv->stage_pending = true;
+// Copy `bytes` of host memory into the workspace's device buffer on
+// `stream`, through its pinned staging.
+static int ws_upload(CallWs *w, const void *src, size_t bytes, hipStream_t stream) {
+  memcpy(w->host, src, bytes);
+  HIPCHK(hipMemcpyAsync(w->dev, w->host, bytes, hipMemcpyHostToDevice, stream));
   return PCME_OK;
 }
+
+// Marks a taken workspace busy until the work enqueued on `stream` by the
+// time the guard ends has completed -- on every return path, so a call that
+// fails after a launch never frees a buffer a kernel still reads.  Declared
+// after the device lock, so it runs while the lock is held.
+struct WsGuard {
+  CallWs *w = nullptr;
+  hipStream_t s = nullptr;
+  ~WsGuard() {
+    if (w && hipEventRecord(w->ev, s) == hipSuccess) w->recorded = true;
+  }
+};
 
 // slice.rs:52-68
 static uint32_t lambda_of(int qp) {
@@ -170,11 +203,15 @@ struct Plan {
   int ntickets = 0;
   uint32_t *xchg = nullptr;
   size_t nwords = 0;
-  uint32_t *ctl = nullptr;  // [1] stall flag, [2] epoch, [3] clear request, [8..15] tickets taken per
-                            // queue, [16..24] queue q's tickets = [ctl[16+q], ctl[17+q]) (CTL_BYTES)
+  uint32_t *ctl = nullptr;  // [1] stall flag of the running launch, [2] epoch, [3] clear request,
+                            // [4] sticky stall flag (set by any launch, cleared by plan_check),
+                            // [8..15] tickets taken per queue, [16..24] queue q's tickets =
+                            // [ctl[16+q], ctl[17+q]) (CTL_BYTES)
   int resident = 0;         // resident waves of the search kernel on this device
   bool hinted = false;      // some job carries pruner hints: the hinted kernel
   int grid = 0;             // waves launched (persistent grid)
+  hipEvent_t ev = nullptr;  // recorded after each eager launch: the next launch, on any stream, waits
+  bool ev_recorded = false;
 };
 
 static int check_jobs(const pcme_frame_job *jobs, int njobs) {
@@ -198,6 +235,7 @@ static void plan_free(Plan *p) {
   if (p->dtickets) (void)hipFree(p->dtickets);
   if (p->xchg) (void)hipFree(p->xchg);
   if (p->ctl) (void)hipFree(p->ctl);
+  if (p->ev) (void)hipEventDestroy(p->ev);
   delete p;
 }
 
@@ -260,7 +298,8 @@ static int plan_create(Dev *dv, const pcme_frame_job *jobs, int njobs, Plan **ou
   p->grid = std::min(p->ntickets, p->resident);
   if (hipMalloc((void **)&p->xchg, words * 4) != hipSuccess || hipMalloc((void **)&p->ctl, CTL_BYTES) != hipSuccess ||
       hipMalloc((void **)&p->djobs, dj.size() * sizeof(DevJob)) != hipSuccess ||
-      hipMalloc((void **)&p->dtickets, tk.size() * sizeof(int2)) != hipSuccess) {
+      hipMalloc((void **)&p->dtickets, tk.size() * sizeof(int2)) != hipSuccess ||
+      hipEventCreateWithFlags(&p->ev, hipEventDisableTiming) != hipSuccess) {
     plan_free(p);
     return fail(PCME_ENOMEM, "plan allocation of %zu exchange words failed", words);
   }
@@ -281,7 +320,15 @@ static int plan_create(Dev *dv, const pcme_frame_job *jobs, int njobs, Plan **ou
   return PCME_OK;
 }
 
+// Launches of one plan share its workspace, so they run one after another:
+// an eager launch waits (on the GPU) for the plan's previous eager launch,
+// whatever stream either was issued on.  Under stream capture the graph's
+// own order applies and no event is used.
 static int plan_launch(Plan *p, hipStream_t stream) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HIPCHK(hipStreamIsCapturing(stream, &cs));
+  const bool eager = cs == hipStreamCaptureStatusNone;
+  if (eager && p->ev_recorded) HIPCHK(hipStreamWaitEvent(stream, p->ev, 0));
   hipLaunchKernelGGL(prep_kernel, dim3(1), dim3(64), 0, stream, p->ctl);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(clear_kernel, dim3(512), dim3(256), 0, stream, p->ctl, p->xchg, p->nwords);
@@ -290,14 +337,24 @@ static int plan_launch(Plan *p, hipStream_t stream) {
                      dim3(64), 0, stream, p->djobs, p->dtickets,
                      p->ntickets, p->ctl, kSpinLimit);
   HIPCHK(hipGetLastError());
+  if (eager) {
+    HIPCHK(hipEventRecord(p->ev, stream));
+    p->ev_recorded = true;
+  }
   return PCME_OK;
 }
 
+// Waits for `stream` and reports whether any launch of the plan since the
+// previous check stalled (the sticky flag ctl[4]), clearing the flag.
 static int plan_check(Plan *p, hipStream_t stream) {
   HIPCHK(hipStreamSynchronize(stream));
+  if (p->ev_recorded) HIPCHK(hipEventSynchronize(p->ev));
   uint32_t flag = 0;
-  HIPCHK(hipMemcpy(&flag, p->ctl + 1, 4, hipMemcpyDeviceToHost));
-  if (flag) return fail(PCME_ESTALL, "a wavefront dependency wait exceeded its bound");
+  HIPCHK(hipMemcpy(&flag, p->ctl + 4, 4, hipMemcpyDeviceToHost));
+  if (flag) {
+    HIPCHK(hipMemset(p->ctl + 4, 0, 4));
+    return fail(PCME_ESTALL, "a wavefront dependency wait exceeded its bound");
+  }
   return PCME_OK;
 }
 
@@ -421,6 +478,15 @@ int pcme_search_frames_sync(const pcme_frame_job *jobs, int njobs, void *stream)
   return search_frames_impl(jobs, njobs, (hipStream_t)stream, true);
 }
 
+int pcme_search_frames_check(void *stream) {
+  Dev *dv = nullptr;
+  int rc;
+  if ((rc = get_dev(&dv))) return rc;
+  std::lock_guard<std::mutex> g(dv->mu);
+  if (!dv->cached) return PCME_OK;
+  return plan_check(dv->cached, (hipStream_t)stream);
+}
+
 int pcme_plan_create(const pcme_frame_job *jobs, int njobs, pcme_plan **out) {
   if (!out) return fail(PCME_EINVAL, "pcme_plan_create: null out");
   *out = nullptr;
@@ -532,11 +598,16 @@ int pcme_search_batch(const pcme_search_job *jobs, int njobs, pcme_mv *out_mv, u
   Dev *dv;
   int rc = get_dev(&dv);
   if (rc) return rc;
+  const size_t bytes = sizeof(pcme_search_job) * (size_t)njobs;
   std::lock_guard<std::mutex> g(dv->mu);
-  if ((rc = grow(&dv->dsj, &dv->dsj_cap, (size_t)njobs))) return rc;
-  if ((rc = upload(dv, dv->dsj, jobs, sizeof(pcme_search_job) * (size_t)njobs, (hipStream_t)stream))) return rc;
-  hipLaunchKernelGGL(search_batch_kernel, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream, dv->dsj, njobs,
-                     out_mv, out_cost);
+  WsGuard wg;
+  wg.s = (hipStream_t)stream;
+  CallWs *w = nullptr;
+  if ((rc = take_ws(dv, bytes, bytes, &w))) return rc;
+  wg.w = w;
+  if ((rc = ws_upload(w, jobs, bytes, (hipStream_t)stream))) return rc;
+  hipLaunchKernelGGL(search_batch_kernel, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream,
+                     (const pcme_search_job *)w->dev, njobs, out_mv, out_cost);
   HIPCHK(hipGetLastError());
   return PCME_OK;
 }
@@ -572,11 +643,16 @@ int pcme_search_part_batch(const pcme_part_job *jobs, int njobs, pcme_part_out *
   Dev *dv;
   int rc = get_dev(&dv);
   if (rc) return rc;
+  const size_t bytes = sizeof(pcme_part_job) * (size_t)njobs;
   std::lock_guard<std::mutex> g(dv->mu);
-  if ((rc = grow(&dv->dpj, &dv->dpj_cap, (size_t)njobs))) return rc;
-  if ((rc = upload(dv, dv->dpj, jobs, sizeof(pcme_part_job) * (size_t)njobs, (hipStream_t)stream))) return rc;
-  hipLaunchKernelGGL(search_part_batch_kernel, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream, dv->dpj,
-                     njobs, out);
+  WsGuard wg;
+  wg.s = (hipStream_t)stream;
+  CallWs *w = nullptr;
+  if ((rc = take_ws(dv, bytes, bytes, &w))) return rc;
+  wg.w = w;
+  if ((rc = ws_upload(w, jobs, bytes, (hipStream_t)stream))) return rc;
+  hipLaunchKernelGGL(search_part_batch_kernel, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream,
+                     (const pcme_part_job *)w->dev, njobs, out);
   HIPCHK(hipGetLastError());
   return PCME_OK;
 }
@@ -613,17 +689,22 @@ int pcme_b_analyse_batch(const pcme_b_job *jobs, int njobs, pcme_b_out *out, voi
   Dev *dv;
   int rc = get_dev(&dv);
   if (rc) return rc;
+  const size_t bytes = sizeof(pcme_b_job) * (size_t)njobs;
   std::lock_guard<std::mutex> g(dv->mu);
-  if ((rc = grow(&dv->dbj, &dv->dbj_cap, (size_t)njobs))) return rc;
-  if ((rc = upload(dv, dv->dbj, jobs, sizeof(pcme_b_job) * (size_t)njobs, (hipStream_t)stream))) return rc;
-  hipLaunchKernelGGL(b_analyse_16_kernel, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream, dv->dbj, njobs,
+  WsGuard wg;
+  wg.s = (hipStream_t)stream;
+  CallWs *w = nullptr;
+  if ((rc = take_ws(dv, bytes, bytes, &w))) return rc;
+  wg.w = w;
+  if ((rc = ws_upload(w, jobs, bytes, (hipStream_t)stream))) return rc;
+  const pcme_b_job *dj = (const pcme_b_job *)w->dev;
+  hipLaunchKernelGGL(b_analyse_16_kernel, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream, dj, njobs, out);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(b_analyse_part_kernel<0>, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream, dj, njobs,
                      out);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(b_analyse_part_kernel<0>, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream, dv->dbj,
-                     njobs, out);
-  HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(b_analyse_part_kernel<1>, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream, dv->dbj,
-                     njobs, out);
+  hipLaunchKernelGGL(b_analyse_part_kernel<1>, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream, dj, njobs,
+                     out);
   HIPCHK(hipGetLastError());
   return PCME_OK;
 }
@@ -781,7 +862,12 @@ int pcme_mbtree_propagate(const pcme_tree_node *nodes, int32_t n, int32_t bw, in
   int rc = get_dev(&dv);
   if (rc) return rc;
   std::lock_guard<std::mutex> g(dv->mu);
-  if ((rc = grow(&dv->dwin, &dv->dwin_cap, (size_t)n))) return rc;
+  WsGuard wg;
+  wg.s = (hipStream_t)stream;
+  CallWs *w = nullptr;
+  if ((rc = take_ws(dv, (size_t)n * sizeof(int4), 0, &w))) return rc;
+  wg.w = w;
+  int4 *dwin = (int4 *)w->dev;
   hipStream_t st = (hipStream_t)stream;
   const int nb = bw * bh;
   const size_t total = (size_t)n * (size_t)nb;
@@ -792,7 +878,7 @@ int pcme_mbtree_propagate(const pcme_tree_node *nodes, int32_t n, int32_t bw, in
     TreeNodeArgs A;
     memset(&A, 0, sizeof A);
     for (int i = 0; i < m; i++) A.b[i] = nodes[c0 + i].blocks;
-    hipLaunchKernelGGL(tree_window_kernel, dim3((unsigned)m), dim3(256), 0, st, A, nb, dv->dwin + c0);
+    hipLaunchKernelGGL(tree_window_kernel, dim3((unsigned)m), dim3(256), 0, st, A, nb, dwin + c0);
     HIPCHK(hipGetLastError());
   }
   // Rounds: node i runs once every node that feeds it (a later node whose
@@ -840,7 +926,7 @@ int pcme_mbtree_propagate(const pcme_tree_node *nodes, int32_t n, int32_t bw, in
         J.src = nodes[i].blocks;
         J.mine = prop + (size_t)i * nb;
         J.dst = prop + (size_t)r * nb;
-        J.win = dv->dwin + i;
+        J.win = dwin + i;
         J.mask = (1 << l) | (l == 0 && nodes[i].ref[1] == r ? 2 : 0);
         J.pad_ = 0;
         jobs.push_back(J);

@@ -1334,7 +1334,10 @@ __device__ __forceinline__ bool fetch_words(const uint32_t *xw, uint32_t tag, ui
     bool ok = l >= 4 || (v >> 24) == tag;
     if (__ballot(!ok) == 0) break;
     if (++spins > spin_limit) {
-      if (l == 0) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (l == 0) {
+        __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(err + 3, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // sticky: ctl[4]
+      }
       return false;
     }
     if ((spins & 63u) == 0) {
@@ -2170,7 +2173,8 @@ __global__ void __launch_bounds__(256) pred_frame_kernel(const uint8_t *__restri
   }
 }
 
-// Per launch: reset the ticket counters and the stall flag and advance the
+// Per launch: reset the ticket counters and the stall flag ctl[1] (ctl[4],
+// the sticky copy, is cleared only by the host's check) and advance the
 // launch epoch ctl[2], whose low byte tags the exchange words.  Tag 0 is
 // skipped; when the tag wraps to 1, ctl[3] asks clear_kernel to zero the
 // exchange words so no word of 255 launches ago can pass for a fresh one.
