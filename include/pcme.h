@@ -188,6 +188,22 @@ int pcme_search_frames(const pcme_frame_job *jobs, int njobs, void *stream);
 int pcme_search_frames_sync(const pcme_frame_job *jobs, int njobs,
                             void *stream);
 
+/* pcme_search_frames_sync without any allocation: no counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code
+ * under stream capture set-up, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same results as pcme_search_frames_sync for the same
+ * jobs.  Its workspace (control block, job table, tickets and 16 bytes of
+ * exchange words per macroblock) lives in a 16 MiB array the library keeps
+ * on each device until the process ends; a call whose jobs need more (about
+ * 125 1080p frames) or more than 1024 jobs fails with PCME_EINVAL.  Every
+ * call on a device shares that array: the library serialises these calls
+ * and each one waits for its kernels before it returns (it is synchronous,
+ * and reports PCME_ESTALL like pcme_search_frames_sync); calls from
+ * several processes each use their own copy.  The caller passes the
+ * inputs and outputs as for pcme_search_frames. */
+int pcme_search_frames_noalloc(const pcme_frame_job *jobs, int njobs,
+                               void *stream);
+
 /* Waits for `stream` and for the last launch of the current device's
  * cached pcme_search_frames plan, and returns PCME_ESTALL if any launch of
  * that plan since the previous check (or pcme_search_frames_sync) stalled;

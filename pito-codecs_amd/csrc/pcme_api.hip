@@ -67,7 +67,19 @@ struct Dev {
   std::vector<CallWs *> ws;        // per-call workspaces (at most kMaxWs)
   Plan *cached = nullptr;          // plan of the last pcme_search_frames call
   std::vector<pcme_frame_job> cached_jobs;
+  uint8_t *scratch = nullptr;      // this device's copy of `scratch` (hipGetSymbolAddress)
+  bool scratch_init = false;       // its control block has been cleared once
 };
+
+// Workspace of pcme_search_frames_noalloc: a control block, the job table,
+// the tickets and the exchange words of one call, in a file-scope device
+// array, so that call allocates nothing.  Each device holds its own copy
+// until the process ends; host code reaches it through Dev::scratch (the
+// address hipGetSymbolAddress returns on that device), never through the
+// name, which in host code is a host-side stand-in.
+#define PCME_NOALLOC_BYTES (16u << 20)
+#define PCME_NOALLOC_MAX_JOBS 1024
+static __device__ __align__(256) uint8_t scratch[PCME_NOALLOC_BYTES];
 
 // Workspaces per device: with this many in flight on other streams a call
 // waits for the oldest one to complete.
@@ -108,6 +120,8 @@ static int get_dev(Dev **out) {
     if (per_cu < 1) per_cu = 1;
     dv->resident_waves[v] = per_cu * p.multiProcessorCount;
   }
+  HIPCHK(hipGetSymbolAddress((void **)&dv->scratch, HIP_SYMBOL(scratch)));
+  if (!dv->scratch) return fail(PCME_EHIP, "hipGetSymbolAddress(scratch) returned null");
   g_devs[d] = dv;
   *out = dv;
   return PCME_OK;
@@ -408,6 +422,110 @@ static int search_frames_impl(const pcme_frame_job *jobs, int njobs, hipStream_t
   return sync ? plan_check(dv->cached, stream) : PCME_OK;
 }
 
+// Host-side tables of pcme_search_frames_noalloc, static so the call
+// allocates nothing; g_na_mu serialises their use across threads.
+static std::mutex g_na_mu;
+static DevJob g_na_jobs[PCME_NOALLOC_MAX_JOBS];
+static int2 g_na_tickets[PCME_NOALLOC_BYTES / 8];
+
+static size_t up256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// The ME-only field search of pcme_search_frames_sync with its workspace in
+// `scratch`: the same job layout, tickets, queues and kernels as a plan.
+static int search_frames_noalloc(const pcme_frame_job *jobs, int njobs, hipStream_t stream) {
+  if (njobs == 0) return PCME_OK;
+  int rc;
+  if ((rc = check_jobs(jobs, njobs))) return rc;
+  if (njobs > PCME_NOALLOC_MAX_JOBS)
+    return fail(PCME_EINVAL, "pcme_search_frames_noalloc: %d jobs exceed the limit of %d", njobs,
+                PCME_NOALLOC_MAX_JOBS);
+  size_t words = 0, mbs = 0;
+  int maxh = 0, hinted = 0;
+  for (int i = 0; i < njobs; i++) {
+    words += (size_t)jobs[i].prm.mbw * jobs[i].prm.mbh * 4;
+    mbs += (size_t)jobs[i].prm.mbw * jobs[i].prm.mbh;
+    maxh = std::max(maxh, jobs[i].prm.mbh);
+    hinted |= jobs[i].hint != nullptr;
+  }
+  size_t ntk = 0;
+  for (int i = 0; i < njobs; i++) ntk += (size_t)jobs[i].prm.mbh;
+  const size_t off_jobs = up256(CTL_BYTES), off_tk = off_jobs + up256((size_t)njobs * sizeof(DevJob));
+  const size_t off_x = off_tk + up256(ntk * sizeof(int2)), need = off_x + words * 4;
+  if (need > PCME_NOALLOC_BYTES)
+    return fail(PCME_EINVAL,
+                "pcme_search_frames_noalloc: %zu macroblocks need %zu bytes of workspace, more than its %u "
+                "(about %u 1080p frames)",
+                mbs, need, PCME_NOALLOC_BYTES, (unsigned)(PCME_NOALLOC_BYTES / (8160u * 16u + 600u)));
+  Dev *dv = nullptr;
+  if ((rc = get_dev(&dv))) return rc;
+  std::lock_guard<std::mutex> g0(g_na_mu);
+  std::lock_guard<std::mutex> g(dv->mu);
+  uint8_t *base = dv->scratch;
+  uint32_t *ctl = (uint32_t *)base;
+  DevJob *djobs = (DevJob *)(base + off_jobs);
+  int2 *dtk = (int2 *)(base + off_tk);
+  uint32_t *xchg = (uint32_t *)(base + off_x);
+  // the job table and tickets as plan_create builds them
+  size_t off = 0, total = mbs, cum = 0;
+  int qof[PCME_NOALLOC_MAX_JOBS];
+  for (int i = 0; i < njobs; i++) {
+    const pcme_frame_job &J = jobs[i];
+    DevJob &d = g_na_jobs[i];
+    d.src = J.src;
+    d.pl[0] = J.ref; d.pl[1] = J.ref_h; d.pl[2] = J.ref_v; d.pl[3] = J.ref_c;
+    d.out = J.out;
+    d.hint = J.hint;
+    d.xchg = xchg + off;
+    d.mbw = J.prm.mbw;
+    d.mbh = J.prm.mbh;
+    d.stride = 16 * J.prm.mbw + 2 * PCME_PAD;
+    d.me_range = J.prm.me_range;
+    d.subpel_rounds = J.prm.subpel_rounds;
+    d.partitions = J.prm.partitions ? 1 : 0;
+    d.lam = lambda_of(J.prm.qp);
+    d.pad_ = 0;
+    const size_t m = (size_t)J.prm.mbw * J.prm.mbh;
+    off += m * 4;
+    qof[i] = (int)std::min<size_t>(PCME_QUEUES - 1, (cum + m / 2) * PCME_QUEUES / std::max<size_t>(total, 1));
+    cum += m;
+  }
+  uint32_t qoff[PCME_QUEUES + 1];
+  size_t nt = 0;
+  for (int q = 0; q < PCME_QUEUES; q++) {
+    qoff[q] = (uint32_t)nt;
+    for (int y = 0; y < maxh; y++)
+      for (int i = 0; i < njobs; i++)
+        if (qof[i] == q && y < g_na_jobs[i].mbh) g_na_tickets[nt++] = make_int2(i, y);
+  }
+  qoff[PCME_QUEUES] = (uint32_t)nt;
+  if (!dv->scratch_init) {  // the epoch word starts at zero on each device
+    HIPCHK(hipMemsetAsync(ctl, 0, CTL_BYTES, stream));
+    dv->scratch_init = true;
+  }
+  // every call clears its exchange words and the sticky stall flag: the
+  // array keeps the previous call's contents, whatever its layout was
+  HIPCHK(hipMemsetAsync(ctl + 4, 0, 4, stream));
+  HIPCHK(hipMemsetAsync(xchg, 0, words * 4, stream));
+  HIPCHK(hipMemcpyAsync(ctl + 16, qoff, sizeof qoff, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(djobs, g_na_jobs, (size_t)njobs * sizeof(DevJob), hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(dtk, g_na_tickets, nt * sizeof(int2), hipMemcpyHostToDevice, stream));
+  const int grid = std::min((int)nt, dv->resident_waves[hinted ? 1 : 0]);
+  hipLaunchKernelGGL(prep_kernel, dim3(1), dim3(64), 0, stream, ctl);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(clear_kernel, dim3(512), dim3(256), 0, stream, ctl, xchg, words);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(hinted ? search_frames_hinted_kernel : search_frames_kernel, dim3((unsigned)grid), dim3(64), 0,
+                     stream, djobs, dtk, (int)nt, ctl, kSpinLimit);
+  HIPCHK(hipGetLastError());
+  // the host tables are reused by the next call: wait here (the call is
+  // synchronous like pcme_search_frames_sync) and report a stall
+  HIPCHK(hipStreamSynchronize(stream));
+  uint32_t flag = 0;
+  HIPCHK(hipMemcpy(&flag, ctl + 4, 4, hipMemcpyDeviceToHost));
+  if (flag) return fail(PCME_ESTALL, "a wavefront dependency wait exceeded its bound");
+  return PCME_OK;
+}
+
 }  // namespace pcme
 
 using namespace pcme;
@@ -502,6 +620,10 @@ int pcme_search_frames(const pcme_frame_job *jobs, int njobs, void *stream) {
 
 int pcme_search_frames_sync(const pcme_frame_job *jobs, int njobs, void *stream) {
   return search_frames_impl(jobs, njobs, (hipStream_t)stream, true);
+}
+
+int pcme_search_frames_noalloc(const pcme_frame_job *jobs, int njobs, void *stream) {
+  return search_frames_noalloc(jobs, njobs, (hipStream_t)stream);
 }
 
 int pcme_search_frames_check(void *stream) {

@@ -166,7 +166,8 @@ assert HINT_DTYPE.itemsize == 40
 EXPORTS = (
     "pcme_version", "pcme_last_error", "pcme_device_count", "pcme_lambda",
     "pcme_qpel_table", "pcme_pad_planes", "pcme_hpel", "pcme_luma_planes", "pcme_search_frames",
-    "pcme_search_frames_sync", "pcme_search_frames_check", "pcme_search_batch", "pcme_pred_frame",
+    "pcme_search_frames_sync", "pcme_search_frames_check",
+    "pcme_search_frames_noalloc", "pcme_search_batch", "pcme_pred_frame",
     "pcme_synth_clip", "pcme_plan_create", "pcme_plan_launch", "pcme_plan_check",
     "pcme_plan_set_grid", "pcme_plan_autotune", "pcme_plan_destroy", "pcme_set_spin_limit",
     "pcme_net_create", "pcme_net_destroy", "pcme_net_get_params", "pcme_net_set_params",
@@ -207,7 +208,9 @@ def load() -> ctypes.CDLL:
         "pcme_search_frames": (c.c_int, [c.POINTER(FrameJob), c.c_int, vp]),
         "pcme_search_frames_sync": (c.c_int, [c.POINTER(FrameJob), c.c_int, vp]),
         "pcme_search_frames_check": (c.c_int, [vp]),
+        "pcme_search_frames_noalloc": (c.c_int, [c.POINTER(FrameJob), c.c_int, vp]),
         "pcme_search_frames_check": (c.c_int, [vp]),
+        "pcme_search_frames_noalloc": (c.c_int, [c.POINTER(FrameJob), c.c_int, vp]),
         "pcme_search_batch": (c.c_int, [c.POINTER(SearchJob), c.c_int, vp, vp, vp]),
         "pcme_search_part_batch": (c.c_int, [c.POINTER(PartJob), c.c_int, vp, vp]),
         "pcme_b_analyse_batch": (c.c_int, [c.POINTER(BJob), c.c_int, vp, vp]),
