@@ -56,6 +56,13 @@ __device__ __forceinline__ uint32_t se_bits(int v) {
   return 2u * (31u - (uint32_t)__builtin_clz(kp)) + 1u;
 }
 
+// Rate term of a candidate (slice.rs:1615, 1686): (lam * mv_bits) >> 8.
+// lam <= 21362 and the bits of two components within +-4096 stay below 64,
+// so the product is a full-rate 24-bit multiply.
+__device__ __forceinline__ uint32_t mv_cost(uint32_t lam, int dx, int dy) {
+  return __umul24(lam, se_bits(dx) + se_bits(dy)) >> 8;
+}
+
 typedef short v2s __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ v2s as_v2s(uint32_t u) { return __builtin_bit_cast(v2s, u); }
 

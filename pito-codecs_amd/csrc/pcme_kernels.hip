@@ -162,13 +162,13 @@ __device__ __forceinline__ uint32_t sad_cost(const SCtx &c, const Lane &L, const
     if (LDS)
       ld8_at(lw, (y - wy0 + r) * PITCH + (x - wx0), a, b);
     else
-      ld8_at(gplane(c.pl[0]), (y + r + PCME_PAD) * c.stride + x + PCME_PAD, a, b);
+      ld8_at(gplane(c.pl[0]), (int)__umul24((uint32_t)(y + r + PCME_PAD), (uint32_t)c.stride) + x + PCME_PAD, a, b);
     acc = __builtin_amdgcn_sad_u8(a, s.raw[2 * r], acc);
     acc = __builtin_amdgcn_sad_u8(b, s.raw[2 * r + 1], acc);
   }
   acc = L.active ? acc : 0u;
   acc = LPC == 8 ? red8(acc) : (LPC == 4 ? red4(acc) : red42(acc));
-  return acc + ((c.lam * (se_bits(cx - mvpx) + se_bits(cy - mvpy))) >> 8);
+  return acc + mv_cost(c.lam, cx - mvpx, cy - mvpy);
 }
 
 __device__ __forceinline__ const uint8_t *plane_sel(const SCtx &c, uint32_t p) {
@@ -205,7 +205,7 @@ __device__ __forceinline__ uint32_t satd_cost(const SCtx &c, const Lane &L, cons
   uint32_t acc = satd_strip8(sp, P);
   acc = L.active ? acc : 0u;
   acc = LPC == 8 ? red8(acc) : (LPC == 4 ? red4(acc) : red42(acc));
-  return acc + ((c.lam * (se_bits(cx - mvpx) + se_bits(cy - mvpy))) >> 8);
+  return acc + mv_cost(c.lam, cx - mvpx, cy - mvpy);
 }
 
 // Per-slot candidate offsets, 4-bit fields (value + bias) per slot.
@@ -271,6 +271,7 @@ __device__ __forceinline__ void dma_region(const uint8_t *plane, int stride, int
   int r0 = l / CPR, c0 = l - (l / CPR) * CPR;
   asm volatile("" : "+v"(r0), "+v"(c0));
   const uint8_t *base = plane + (size_t)((oy + PCME_PAD) * stride + ox + PCME_PAD);
+  const uint32_t ustride = (uint32_t)stride;
 #pragma unroll
   for (int k = 0; k < IT; k++) {
     const int q = (64 * k) / CPR, m = (64 * k) % CPR;
@@ -278,10 +279,19 @@ __device__ __forceinline__ void dma_region(const uint8_t *plane, int stride, int
     const bool cy = t >= CPR;
     const int row = r0 + q + (cy ? 1 : 0), col = cy ? t - CPR : t;
     if (64 * k + 63 < N || l + 64 * k < N)
-      __builtin_amdgcn_global_load_lds((gvptr)(base + row * stride + 16 * col), (lvptr)(dst + 256 * k), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((gvptr)(base + __umul24((uint32_t)row, ustride) + 16 * col),
+                                       (lvptr)(dst + 256 * k), 16, 0, 0);
   }
 }
 __device__ __forceinline__ void dma_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// Stage a 16x16 block (rows `stride` apart, 4-byte aligned) into 256 bytes
+// of LDS, row r at dst + 16 r: lane r < 16 moves row r with one 16-byte
+// LDS-DMA load.  The caller waits (dma_wait) before reading it.
+__device__ __forceinline__ void dma_block16(const uint8_t *src, int stride, uint32_t *dst) {
+  const uint32_t l = lane_id();
+  if (l < 16u) __builtin_amdgcn_global_load_lds((gvptr)(src + __umul24(l, (uint32_t)stride)), (lvptr)dst, 16, 0, 0);
+}
 
 // Origin of a full-pel window centred on the block at MV (bxv, byv).
 __device__ __forceinline__ void win_origin(const SCtx &c, int bx, int by, int bxv, int byv, int &wx0, int &wy0) {
@@ -344,6 +354,39 @@ __device__ __forceinline__ void ensure_sp(const SCtx &c, int bx, int by, int bxv
   __syncthreads();
 }
 
+// Staged regions of one LDS search area (full-pel window origin, sub-pel
+// region origin; 1 << 20 = nothing staged).  A row walker keeps them across
+// the searches of its row: every pass re-checks that its reads lie inside
+// the staged region (ensure_win / ensure_sp) and re-stages when they do not,
+// so a region staged by an earlier search of the same reference picture is
+// reused whenever it still covers, and its HBM round trip disappears.
+struct Area {
+  int wx0, wy0, sx0, sy0;
+};
+__device__ __forceinline__ Area no_area() {
+  Area a;
+  a.wx0 = a.wy0 = a.sx0 = a.sy0 = 1 << 20;
+  return a;
+}
+
+// Whether the staged full-pel window holds every read of a W x H block at
+// MV (mx, my) moved by up to R px (the reads of sad_cost<true>: 8 bytes from
+// three aligned dwords per strip row, up to 4 bytes past the block).
+template <int W, int H>
+__device__ __forceinline__ bool win_covers(int bx, int by, int mx, int my, int R, int wx0, int wy0) {
+  const int X = bx + (mx >> 2), Y = by + (my >> 2);
+  return (X - R - wx0 >= 0) & (X + R + W + 4 - wx0 <= WIN_W) & (Y - R - wy0 >= 0) & (Y + R + H - wy0 <= WIN_H);
+}
+
+// Whether the staged sub-pel region serves every sub-pel round around a
+// best up to R px from (mx, my) (the check of ensure_sp, widened by R).
+template <int W, int H>
+__device__ __forceinline__ bool sp_covers(int bx, int by, int mx, int my, int R, int sx0, int sy0) {
+  const int X = bx + (mx >> 2), Y = by + (my >> 2);
+  return (X - R - 1 - sx0 >= 0) & (X + R + W + 6 - sx0 <= SP_W) & (Y - R - 1 - sy0 >= 0) &
+         (Y + R + H + 2 - sy0 <= SP_H);
+}
+
 // Hint candidate i (lane-varying) of the MB's hint record hv (lane j < 10
 // holds dword j of its pcme_hint; cand[k] is dword k), rounded like the
 // search_part candidates (slice.rs:1622); i outside [0, nh) reads as 0.
@@ -364,7 +407,7 @@ __device__ __forceinline__ void hint_cand(uint32_t hv, int i, int nh, int &x, in
 template <int W, int H, bool PR>
 __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const Lane &L, int bx, int by, int mvpx,
                                        int mvpy, int c0x, int c0y, int c1x, int c1y, int ncand, uint32_t hv,
-                                       int nh, uint32_t *lds) {
+                                       int nh, uint32_t *lds, Area &ar) {
   uint32_t *lw = lds;
   uint32_t *lsp = lds + WIN_BYTES / 4;
   const Box ok = legal_box(c, bx, by, W, H);
@@ -377,11 +420,24 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
   // Staging runs beside the candidate pass (its HBM round trip): the full
   // window around the rounded predictor (the usual winner), or for a pruned
   // search the sub-pel planes its square rounds read.
-  int wx0 = 1 << 20, wy0 = 1 << 20, sx0 = 1 << 20, sy0 = 1 << 20;
-  if constexpr (PR)
+  // A pruned search starts with nothing staged; a full search keeps the
+  // regions its area holds (a region of an earlier search of the row).
+  if constexpr (PR) ar = no_area();
+  int wx0 = ar.wx0, wy0 = ar.wy0, sx0 = ar.sx0, sy0 = ar.sy0;
+  bool lds_pass = false;
+  if constexpr (PR) {
     prefetch_sp<W, H, 3>(c, bx, by, bxv, byv, sx0, sy0, lsp);
-  else
-    prefetch_win<W, H>(c, bx, by, bxv, byv, wx0, wy0, lw);
+  } else {
+    // the candidate pass reads the staged window when it holds every
+    // candidate, else HBM beside a fresh window
+    const int r0x = (c0x + 2) & ~3, r0y = (c0y + 2) & ~3;
+    const int r1x = (c1x + 2) & ~3, r1y = (c1y + 2) & ~3;
+    const bool v1 = (ncand >= 1) & in_box(ok, r0x, r0y), v2 = (ncand >= 2) & in_box(ok, r1x, r1y);
+    lds_pass = win_covers<W, H>(bx, by, bxv, byv, 0, wx0, wy0) &
+               (!v1 | win_covers<W, H>(bx, by, r0x, r0y, 0, wx0, wy0)) &
+               (!v2 | win_covers<W, H>(bx, by, r1x, r1y, 0, wx0, wy0));
+    if (!lds_pass) prefetch_win<W, H>(c, bx, by, bxv, byv, wx0, wy0, lw);
+  }
   uint32_t bcost;
   if constexpr (!PR) {
     const int r0x = (c0x + 2) & ~3, r0y = (c0y + 2) & ~3;
@@ -389,8 +445,9 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
     const int cx = L.slot == 1 ? r0x : (L.slot == 2 ? r1x : bxv);
     const int cy = L.slot == 1 ? r0y : (L.slot == 2 ? r1y : byv);
     const bool valid = (L.slot == 0) | ((L.slot <= ncand) & in_box(ok, cx, cy));
-    const uint32_t k = sad_cost<false>(c, L, s, bx, by, cx, cy, valid ? cx : bxv, valid ? cy : byv, mvpx, mvpy,
-                                       lw, 0, 0);
+    const int ux = valid ? cx : bxv, uy = valid ? cy : byv;
+    const uint32_t k = lds_pass ? sad_cost<true>(c, L, s, bx, by, cx, cy, ux, uy, mvpx, mvpy, lw, wx0, wy0)
+                                : sad_cost<false>(c, L, s, bx, by, cx, cy, ux, uy, mvpx, mvpy, lw, 0, 0);
     const uint32_t key = min_key(k, valid, L.slot);
     bcost = key >> 3;
     const int w = (int)(key & 7u);
@@ -460,7 +517,9 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
   // sub-pel planes once (margin 3) and reads the full-pel plane among them.
   // The sub-pel planes stage while the square rounds run: two rounds move
   // the best by at most 2 px, which margin 3 covers.
-  if constexpr (!PR) prefetch_sp<W, H, 3>(c, bx, by, bxv, byv, sx0, sy0, lsp);
+  if constexpr (!PR) {
+    if (!sp_covers<W, H>(bx, by, bxv, byv, 2, sx0, sy0)) prefetch_sp<W, H, 3>(c, bx, by, bxv, byv, sx0, sy0, lsp);
+  }
   const int qdx = sq_dx(L.slot), qdy = sq_dy(L.slot);
   for (int it = 0; it < 2; it++) {
     if constexpr (PR)
@@ -507,6 +566,7 @@ __device__ __forceinline__ SRes search(const SCtx &c, const SrcRegs &s, const La
       byv += sq_dy(j) * step;
     }
   }
+  ar.wx0 = wx0; ar.wy0 = wy0; ar.sx0 = sx0; ar.sy0 = sy0;
   SRes r;
   r.x = bxv;
   r.y = byv;
@@ -593,7 +653,8 @@ template <int AW, int AH, int BW, int BH, bool PR>
 __device__ __forceinline__ void search_pair(const SCtx &cA, const SCtx &cB, PState &A, PState &B,
                                             const uint8_t *srcA, const uint8_t *srcB, int sstride, int c0ax,
                                             int c0ay, int c0bx, int c0by, int c1ax, int c1ay, int c1bx, int c1by,
-                                            uint32_t hv, int nh, uint32_t *lds, SRes &ra, SRes &rb) {
+                                            uint32_t hv, int nh, uint32_t *lds, Area &arA, Area &arB, SRes &ra,
+                                            SRes &rb) {
   // cA / cB: the reference pictures of A and B (the same context for two
   // partitions of one list, two for a list-0 / list-1 pair); lam, range,
   // subpel and the plane geometry are common to both.
@@ -637,15 +698,37 @@ __device__ __forceinline__ void search_pair(const SCtx &cA, const SCtx &cB, PSta
   if (!in_box(A.ok, a0x, a0y)) { a0x = 0; a0y = 0; }
   int b0x = (B.mvpx + 2) & ~3, b0y = (B.mvpy + 2) & ~3;
   if (!in_box(B.ok, b0x, b0y)) { b0x = 0; b0y = 0; }
-  // Staging beside the candidate pass (see search()).
-  A.wx0 = A.wy0 = B.wx0 = B.wy0 = 1 << 20;
-  A.sx0 = A.sy0 = B.sx0 = B.sy0 = 1 << 20;
+  // Staging beside the candidate pass (see search()).  A full search keeps
+  // the regions its area holds; the candidate pass reads LDS when both
+  // windows hold every read of it (the candidates and, with an 8x8 B, the
+  // first hexagon round around the rounded predictor).
+  if constexpr (PR) {
+    arA = no_area();
+    arB = no_area();
+  }
+  A.wx0 = arA.wx0; A.wy0 = arA.wy0; A.sx0 = arA.sx0; A.sy0 = arA.sy0;
+  B.wx0 = arB.wx0; B.wy0 = arB.wy0; B.sx0 = arB.sx0; B.sy0 = arB.sy0;
+  bool lds_pass = false;
   if constexpr (PR) {
     prefetch_sp<AW, AH, 3>(cA, A.bx, A.by, a0x, a0y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
     prefetch_sp<BW, BH, 3>(cB, B.bx, B.by, b0x, b0y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
   } else {
-    prefetch_win<AW, AH>(cA, A.bx, A.by, a0x, a0y, A.wx0, A.wy0, lds);
-    prefetch_win<BW, BH>(cB, B.bx, B.by, b0x, b0y, B.wx0, B.wy0, lds + LDS_PER_SEARCH / 4);
+    constexpr int HR = (BW == 8 && BH == 8) ? 2 : 0;  // hexagon points priced in the pass
+    const int r0ax = (c0ax + 2) & ~3, r0ay = (c0ay + 2) & ~3, r1ax = (c1ax + 2) & ~3, r1ay = (c1ay + 2) & ~3;
+    const int r0bx = (c0bx + 2) & ~3, r0by = (c0by + 2) & ~3, r1bx = (c1bx + 2) & ~3, r1by = (c1by + 2) & ~3;
+    const bool va1 = in_box(A.ok, r0ax, r0ay), va2 = in_box(A.ok, r1ax, r1ay);
+    const bool vb1 = in_box(B.ok, r0bx, r0by), vb2 = in_box(B.ok, r1bx, r1by);
+    const bool ca_ = win_covers<AW, AH>(A.bx, A.by, a0x, a0y, HR, A.wx0, A.wy0) &
+                     (!va1 | win_covers<AW, AH>(A.bx, A.by, r0ax, r0ay, 0, A.wx0, A.wy0)) &
+                     (!va2 | win_covers<AW, AH>(A.bx, A.by, r1ax, r1ay, 0, A.wx0, A.wy0));
+    const bool cb_ = win_covers<BW, BH>(B.bx, B.by, b0x, b0y, HR, B.wx0, B.wy0) &
+                     (!vb1 | win_covers<BW, BH>(B.bx, B.by, r0bx, r0by, 0, B.wx0, B.wy0)) &
+                     (!vb2 | win_covers<BW, BH>(B.bx, B.by, r1bx, r1by, 0, B.wx0, B.wy0));
+    lds_pass = ca_ & cb_;
+    if (!lds_pass) {
+      prefetch_win<AW, AH>(cA, A.bx, A.by, a0x, a0y, A.wx0, A.wy0, lds);
+      prefetch_win<BW, BH>(cB, B.bx, B.by, b0x, b0y, B.wx0, B.wy0, lds + LDS_PER_SEARCH / 4);
+    }
   }
   // With an 8x8 B (the full search) the candidate pass's idle slots (3-7,
   // and lanes 48-51 / 52-53) price the first hexagon round around slot 0's
@@ -681,8 +764,10 @@ __device__ __forceinline__ void search_pair(const SCtx &cA, const SCtx &cB, PSta
     Lane Lk = L;
     Lk.active = L.active | (SPEC & (ca | cb));
     const bool use = valid | vh;
-    const uint32_t k = sad_cost<false, LPC>(cl, Lk, s, bx, by, cx, cy, use ? cx : px, use ? cy : py, mvpx, mvpy,
-                                            lw, 0, 0);
+    const int ux = use ? cx : px, uy = use ? cy : py;
+    const uint32_t k = lds_pass ? sad_cost<true, LPC>(cl, Lk, s, bx, by, cx, cy, ux, uy, mvpx, mvpy, lw,
+                                                      gsel(g, A.wx0, B.wx0), gsel(g, A.wy0, B.wy0))
+                                : sad_cost<false, LPC>(cl, Lk, s, bx, by, cx, cy, ux, uy, mvpx, mvpy, lw, 0, 0);
     uint32_t ka, kb;
     min_key_pair<B8>(k, valid, L.slot, ka, kb);
     if constexpr (SPEC) {
@@ -781,8 +866,10 @@ __device__ __forceinline__ void search_pair(const SCtx &cA, const SCtx &cB, PSta
   // pruned search reads the full-pel plane of its sub-pel region, staged
   // once with margin 3).
   if constexpr (!PR) {
-    prefetch_sp<AW, AH, 3>(cA, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
-    prefetch_sp<BW, BH, 3>(cB, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
+    if (!sp_covers<AW, AH>(A.bx, A.by, A.x, A.y, 2, A.sx0, A.sy0))
+      prefetch_sp<AW, AH, 3>(cA, A.bx, A.by, A.x, A.y, A.sx0, A.sy0, lds + WIN_BYTES / 4);
+    if (!sp_covers<BW, BH>(B.bx, B.by, B.x, B.y, 2, B.sx0, B.sy0))
+      prefetch_sp<BW, BH, 3>(cB, B.bx, B.by, B.x, B.y, B.sx0, B.sy0, lds + (LDS_PER_SEARCH + WIN_BYTES) / 4);
   }
   const int qdx = sq_dx(L.slot), qdy = sq_dy(L.slot);
   A.act = B.act = true;
@@ -882,6 +969,8 @@ __device__ __forceinline__ void search_pair(const SCtx &cA, const SCtx &cB, PSta
       }
     }
   }
+  arA.wx0 = A.wx0; arA.wy0 = A.wy0; arA.sx0 = A.sx0; arA.sy0 = A.sy0;
+  arB.wx0 = B.wx0; arB.wy0 = B.wy0; arB.sx0 = B.sx0; arB.sy0 = B.sy0;
   ra.x = A.x; ra.y = A.y; ra.cost = A.cost;
   rb.x = B.x; rb.y = B.y; rb.cost = B.cost;
 }
@@ -1035,7 +1124,7 @@ struct PartRes {
 template <int PART, int P, bool PR, class N>
 __device__ __forceinline__ void part_step(const SCtx &c, const N &n, Cur &cur, const uint8_t *src,
                                           int sstride, int mbx, int mby, uint32_t seed, uint32_t hv,
-                                          int nh, PartRes &res, uint32_t *lds) {
+                                          int nh, PartRes &res, uint32_t *lds, Area *ar) {
   if constexpr (P < part_count(PART)) {
     constexpr int X = rect_x(PART, P), Y = rect_y(PART, P);
     constexpr int W = rect_w(PART), H = rect_h(PART);
@@ -1045,13 +1134,13 @@ __device__ __forceinline__ void part_step(const SCtx &c, const N &n, Cur &cur, c
     load_src(src + Y * sstride + X, sstride, L, s);
     const uint32_t m0 = P == 0 ? 0u : res.mv[0];
     SRes r = search<W, H, PR>(c, s, L, mbx + X, mby + Y, mv_x(pv), mv_y(pv), mv_x(seed), mv_y(seed), mv_x(m0),
-                          mv_y(m0), 2, hv, nh, lds);
+                          mv_y(m0), 2, hv, nh, lds, ar[0]);
     const uint32_t m = pack_mv(r.x, r.y);
     res.mv[P] = m;
     res.mvp[P] = pv;
     res.total += r.cost;
     cur.mv[P] = m;
-    part_step<PART, P + 1, PR>(c, n, cur, src, sstride, mbx, mby, seed, hv, nh, res, lds);
+    part_step<PART, P + 1, PR>(c, n, cur, src, sstride, mbx, mby, seed, hv, nh, res, lds, ar);
   }
 }
 
@@ -1059,12 +1148,12 @@ __device__ __forceinline__ void part_step(const SCtx &c, const N &n, Cur &cur, c
 template <int PART, bool PR, class N>
 __device__ __forceinline__ void search_part(const SCtx &c, const N &n, const uint8_t *src, int sstride,
                                             int mbx, int mby, uint32_t seed, uint32_t hv, int nh,
-                                            PartRes &res, uint32_t *lds) {
+                                            PartRes &res, uint32_t *lds, Area *ar) {
   Cur cur = {};
 #pragma unroll
   for (int i = 0; i < 4; i++) { res.mv[i] = res.mvp[i] = 0u; }
   res.total = 0;
-  part_step<PART, 0, PR>(c, n, cur, src, sstride, mbx, mby, seed, hv, nh, res, lds);
+  part_step<PART, 0, PR>(c, n, cur, src, sstride, mbx, mby, seed, hv, nh, res, lds, ar);
 }
 
 // search_part (slice.rs:1810-1868) of P16x8 (ra) and P8x16 (rb) together:
@@ -1072,7 +1161,7 @@ __device__ __forceinline__ void search_part(const SCtx &c, const N &n, const uin
 template <bool PR>
 __device__ __forceinline__ void search_part_12(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
                                                int mbx, int mby, uint32_t seed, uint32_t hv, int nh,
-                                               PartRes &ra, PartRes &rb, uint32_t *lds) {
+                                               PartRes &ra, PartRes &rb, uint32_t *lds, Area *ar) {
   Cur ca = {}, cb = {};
 #pragma unroll
   for (int i = 0; i < 4; i++) { ra.mv[i] = ra.mvp[i] = rb.mv[i] = rb.mvp[i] = 0u; }
@@ -1082,8 +1171,8 @@ __device__ __forceinline__ void search_part_12(const SCtx &c, const Nbr &n, cons
   uint32_t pa = mvp<1, 0, 0, 0, 4, 1>(n, ca), pb = mvp<2, 0, 0, 0, 2, 3>(n, cb);
   A.bx = mbx; A.by = mby; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
   B.bx = mbx; B.by = mby; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
-  search_pair<16, 8, 8, 16, PR>(c, c, A, B, src, src, sstride, mv_x(seed), mv_y(seed), mv_x(seed), mv_y(seed), 0, 0, 0, 0, hv, nh, lds, xa,
-                                xb);
+  search_pair<16, 8, 8, 16, PR>(c, c, A, B, src, src, sstride, mv_x(seed), mv_y(seed), mv_x(seed), mv_y(seed), 0, 0, 0, 0, hv, nh, lds, ar[0],
+                                ar[1], xa, xb);
   ra.mv[0] = ca.mv[0] = pack_mv(xa.x, xa.y); ra.mvp[0] = pa; ra.total = xa.cost;
   rb.mv[0] = cb.mv[0] = pack_mv(xb.x, xb.y); rb.mvp[0] = pb; rb.total = xb.cost;
   // partition 1: 16x8 bottom (y 2 blocks), 8x16 right (x 2 blocks)
@@ -1092,7 +1181,7 @@ __device__ __forceinline__ void search_part_12(const SCtx &c, const Nbr &n, cons
   A.bx = mbx; A.by = mby + 8; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
   B.bx = mbx + 8; B.by = mby; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
   search_pair<16, 8, 8, 16, PR>(c, c, A, B, src + 8 * sstride, src + 8, sstride, mv_x(seed), mv_y(seed), mv_x(seed), mv_y(seed), mv_x(ra.mv[0]),
-              mv_y(ra.mv[0]), mv_x(rb.mv[0]), mv_y(rb.mv[0]), hv, nh, lds, xa, xb);
+              mv_y(ra.mv[0]), mv_x(rb.mv[0]), mv_y(rb.mv[0]), hv, nh, lds, ar[0], ar[1], xa, xb);
   ra.mv[1] = pack_mv(xa.x, xa.y); ra.mvp[1] = pa; ra.total += xa.cost;
   rb.mv[1] = pack_mv(xb.x, xb.y); rb.mvp[1] = pb; rb.total += xb.cost;
 }
@@ -1104,7 +1193,7 @@ __device__ __forceinline__ void search_part_12(const SCtx &c, const Nbr &n, cons
 template <int PART, class N>
 __device__ __forceinline__ void search_part_lists(const SCtx *c, const N *n, const uint8_t *src, int sstride,
                                                   int mbx, int mby, const uint32_t *seed, PartRes &ra,
-                                                  PartRes &rb, uint32_t *lds) {
+                                                  PartRes &rb, uint32_t *lds, Area *ar) {
   constexpr int W = rect_w(PART), H = rect_h(PART);
   constexpr int X0 = rect_x(PART, 0), Y0 = rect_y(PART, 0), X1 = rect_x(PART, 1), Y1 = rect_y(PART, 1);
   Cur ca = {}, cb = {};
@@ -1119,7 +1208,7 @@ __device__ __forceinline__ void search_part_lists(const SCtx *c, const N *n, con
   B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
   const uint8_t *s0 = src + Y0 * sstride + X0;
   search_pair<W, H, W, H, false>(c[0], c[1], A, B, s0, s0, sstride, mv_x(seed[0]), mv_y(seed[0]), mv_x(seed[1]),
-                                 mv_y(seed[1]), 0, 0, 0, 0, 0u, -1, lds, xa, xb);
+                                 mv_y(seed[1]), 0, 0, 0, 0, 0u, -1, lds, ar[0], ar[1], xa, xb);
   ra.mv[0] = ca.mv[0] = pack_mv(xa.x, xa.y); ra.mvp[0] = pa; ra.total = xa.cost;
   rb.mv[0] = cb.mv[0] = pack_mv(xb.x, xb.y); rb.mvp[0] = pb; rb.total = xb.cost;
   pa = mvp<PART, 1, X1 / 4, Y1 / 4, W / 4, shape_of(PART, 1)>(n[0], ca);
@@ -1130,7 +1219,7 @@ __device__ __forceinline__ void search_part_lists(const SCtx *c, const N *n, con
   const uint8_t *s1 = src + Y1 * sstride + X1;
   search_pair<W, H, W, H, false>(c[0], c[1], A, B, s1, s1, sstride, mv_x(seed[0]), mv_y(seed[0]), mv_x(seed[1]),
                                  mv_y(seed[1]), mv_x(ra.mv[0]), mv_y(ra.mv[0]), mv_x(rb.mv[0]), mv_y(rb.mv[0]), 0u,
-                                 -1, lds, xa, xb);
+                                 -1, lds, ar[0], ar[1], xa, xb);
   ra.mv[1] = pack_mv(xa.x, xa.y); ra.mvp[1] = pa; ra.total += xa.cost;
   rb.mv[1] = pack_mv(xb.x, xb.y); rb.mvp[1] = pb; rb.total += xb.cost;
 }
@@ -1142,7 +1231,7 @@ __device__ __forceinline__ void search_part_lists(const SCtx *c, const N *n, con
 // one-at-a-time order, so the results are the same.
 __device__ __forceinline__ void search_part_123(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
                                                 int mbx, int mby, uint32_t seed, PartRes &r1, PartRes &r2,
-                                                PartRes &r3, uint32_t *lds) {
+                                                PartRes &r3, uint32_t *lds, Area *ar) {
   Cur c1 = {}, c2 = {}, c3 = {};
 #pragma unroll
   for (int i = 0; i < 4; i++) { r1.mv[i] = r1.mvp[i] = r2.mv[i] = r2.mvp[i] = r3.mv[i] = r3.mvp[i] = 0u; }
@@ -1154,7 +1243,8 @@ __device__ __forceinline__ void search_part_123(const SCtx &c, const Nbr &n, con
   uint32_t pa = mvp<1, 0, 0, 0, 4, 1>(n, c1), pb = mvp<3, 0, 0, 0, 2, 0>(n, c3);
   A.bx = mbx; A.by = mby; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
   B.bx = mbx; B.by = mby; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
-  search_pair<16, 8, 8, 8, false>(c, c, A, B, src, src, sstride, sx, sy, sx, sy, 0, 0, 0, 0, nohint, -1, lds, xa, xb);
+  search_pair<16, 8, 8, 8, false>(c, c, A, B, src, src, sstride, sx, sy, sx, sy, 0, 0, 0, 0, nohint, -1, lds, ar[0],
+                                  ar[1], xa, xb);
   r1.mv[0] = c1.mv[0] = pack_mv(xa.x, xa.y); r1.mvp[0] = pa; r1.total = xa.cost;
   r3.mv[0] = c3.mv[0] = pack_mv(xb.x, xb.y); r3.mvp[0] = pb; r3.total = xb.cost;
   // step 2: 8x16 left (x 0, y 0, w 2) with 8x8 top-right (x 2, y 0)
@@ -1163,7 +1253,7 @@ __device__ __forceinline__ void search_part_123(const SCtx &c, const Nbr &n, con
   A.bx = mbx; A.by = mby; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
   B.bx = mbx + 8; B.by = mby; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
   search_pair<8, 16, 8, 8, false>(c, c, A, B, src, src + 8, sstride, sx, sy, sx, sy, 0, 0, mv_x(r3.mv[0]), mv_y(r3.mv[0]),
-                                  nohint, -1, lds, xa, xb);
+                                  nohint, -1, lds, ar[0], ar[1], xa, xb);
   r2.mv[0] = c2.mv[0] = pack_mv(xa.x, xa.y); r2.mvp[0] = pa; r2.total = xa.cost;
   r3.mv[1] = c3.mv[1] = pack_mv(xb.x, xb.y); r3.mvp[1] = pb; r3.total += xb.cost;
   // step 3: 16x8 bottom (y 2 blocks) with 8x8 bottom-left (x 0, y 2)
@@ -1173,7 +1263,7 @@ __device__ __forceinline__ void search_part_123(const SCtx &c, const Nbr &n, con
   B.bx = mbx; B.by = mby + 8; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
   search_pair<16, 8, 8, 8, false>(c, c, A, B, src + 8 * sstride, src + 8 * sstride, sstride, sx, sy, sx, sy,
                                   mv_x(r1.mv[0]), mv_y(r1.mv[0]), mv_x(r3.mv[0]), mv_y(r3.mv[0]), nohint, -1, lds,
-                                  xa, xb);
+                                  ar[0], ar[1], xa, xb);
   r1.mv[1] = pack_mv(xa.x, xa.y); r1.mvp[1] = pa; r1.total += xa.cost;
   r3.mv[2] = c3.mv[2] = pack_mv(xb.x, xb.y); r3.mvp[2] = pb; r3.total += xb.cost;
   // step 4: 8x16 right (x 2 blocks) with 8x8 bottom-right (x 2, y 2)
@@ -1182,7 +1272,8 @@ __device__ __forceinline__ void search_part_123(const SCtx &c, const Nbr &n, con
   A.bx = mbx + 8; A.by = mby; A.mvpx = mv_x(pa); A.mvpy = mv_y(pa);
   B.bx = mbx + 8; B.by = mby + 8; B.mvpx = mv_x(pb); B.mvpy = mv_y(pb);
   search_pair<8, 16, 8, 8, false>(c, c, A, B, src + 8, src + 8 * sstride + 8, sstride, sx, sy, sx, sy, mv_x(r2.mv[0]),
-                                  mv_y(r2.mv[0]), mv_x(r3.mv[0]), mv_y(r3.mv[0]), nohint, -1, lds, xa, xb);
+                                  mv_y(r2.mv[0]), mv_x(r3.mv[0]), mv_y(r3.mv[0]), nohint, -1, lds, ar[0], ar[1], xa,
+                                  xb);
   r2.mv[1] = pack_mv(xa.x, xa.y); r2.mvp[1] = pa; r2.total += xa.cost;
   r3.mv[3] = pack_mv(xb.x, xb.y); r3.mvp[3] = pb; r3.total += xb.cost;
 }
@@ -1224,9 +1315,9 @@ __device__ __forceinline__ void take_mode(MbRes &o, const PartRes &r, int part, 
 // against the running best (slice.rs:1925-1945).
 template <bool PR>
 __device__ __forceinline__ void mode_12(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride, int mbx,
-                                        int mby, uint32_t hv, int nh, MbRes &o, uint32_t *lds) {
+                                        int mby, uint32_t hv, int nh, MbRes &o, uint32_t *lds, Area *ar) {
   PartRes r, r2;
-  search_part_12<PR>(c, n, src, sstride, mbx, mby, o.m16, hv, nh, r, r2, lds);
+  search_part_12<PR>(c, n, src, sstride, mbx, mby, o.m16, hv, nh, r, r2, lds, ar);
   o.cost[1] = r.total;
   uint32_t cc = r.total + ((c.lam * 3u) >> 8);
   bool t = cc < o.best_cost;
@@ -1244,9 +1335,9 @@ __device__ __forceinline__ void mode_12(const SCtx &c, const Nbr &n, const uint8
 // Mode P8x8 and its choice.
 template <bool PR>
 __device__ __forceinline__ void mode_3(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride, int mbx,
-                                       int mby, uint32_t hv, int nh, MbRes &o, uint32_t *lds) {
+                                       int mby, uint32_t hv, int nh, MbRes &o, uint32_t *lds, Area *ar) {
   PartRes r;
-  search_part<3, PR>(c, n, src, sstride, mbx, mby, o.m16, hv, nh, r, lds);
+  search_part<3, PR>(c, n, src, sstride, mbx, mby, o.m16, hv, nh, r, lds, ar);
   o.cost[3] = r.total;
   const uint32_t cc = r.total + ((c.lam * 10u) >> 8);
   const bool t = cc < o.best_cost;
@@ -1262,10 +1353,10 @@ __device__ __forceinline__ void mode_3(const SCtx &c, const Nbr &n, const uint8_
 template <bool PR>
 __device__ __forceinline__ void analyse_mb(const SCtx &c, const Nbr &n, const uint8_t *src, int sstride,
                                            int mbx, int mby, int partitions, uint32_t hv, int nh,
-                                           int hpart, MbRes &o, uint32_t *lds) {
+                                           int hpart, MbRes &o, uint32_t *lds, Area *ar) {
   const uint32_t skip = pskip(n);
   PartRes r16;
-  search_part<0, PR>(c, n, src, sstride, mbx, mby, skip, hv, nh, r16, lds);
+  search_part<0, PR>(c, n, src, sstride, mbx, mby, skip, hv, nh, r16, lds, ar);
   const uint32_t cost16 = r16.total;
   o.part = 0;
   take_if(o.best, r16, true);
@@ -1278,12 +1369,12 @@ __device__ __forceinline__ void analyse_mb(const SCtx &c, const Nbr &n, const ui
   if (PR) {
     // A pruned MB searches the hinted mode only: P16x8 with P8x16, or P8x8.
     if (split & ((hpart == 1) | (hpart == 2)))
-      mode_12<PR>(c, n, src, sstride, mbx, mby, hv, nh, o, lds);
+      mode_12<PR>(c, n, src, sstride, mbx, mby, hv, nh, o, lds, ar);
     else if (split & (hpart == 3))
-      mode_3<PR>(c, n, src, sstride, mbx, mby, hv, nh, o, lds);
+      mode_3<PR>(c, n, src, sstride, mbx, mby, hv, nh, o, lds, ar);
   } else if (split) {
     PartRes r1, r2, r3;
-    search_part_123(c, n, src, sstride, mbx, mby, o.m16, r1, r2, r3, lds);
+    search_part_123(c, n, src, sstride, mbx, mby, o.m16, r1, r2, r3, lds, ar);
     take_mode(o, r1, 1, (c.lam * 3u) >> 8);
     take_mode(o, r2, 2, (c.lam * 3u) >> 8);
     take_mode(o, r3, 3, (c.lam * 10u) >> 8);
@@ -1364,7 +1455,8 @@ __device__ __forceinline__ uint32_t load_hint(const pcme_hint *h) {
 
 template <bool HINTS>
 __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
-                                            int ntickets, uint32_t *ctl, uint32_t spin_limit, uint32_t *s_lds) {
+                                            int ntickets, uint32_t *ctl, uint32_t spin_limit, uint32_t *s_lds,
+                                            uint32_t *s_src) {
   uint32_t *err = ctl + 1;
   const uint32_t tag = ctl[2] & 0xFFu;  // this launch's epoch (set by prep_kernel)
   const int l = (int)lane_id();
@@ -1429,6 +1521,12 @@ __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, con
       if (ok && mbw > 1) ok = fetch_words(xup + 4, tag, err, spin_limit, nxt);
     }
     if (!ok) return;
+    // The MB's source block travels into LDS one MB ahead of its use (two
+    // buffers), so no search of the row waits for a source load; the staged
+    // reference regions of the row's searches are kept from MB to MB.
+    const uint8_t *srow = J.src + (size_t)(16 * my + PCME_PAD) * stride + PCME_PAD;
+    dma_block16(srow, stride, s_src);
+    Area ar[2] = {no_area(), no_area()};
     // Hint records travel one MB ahead of their use (a vector load whose
     // round trip hides under the previous MB's search).
     const bool hinted = HINTS && J.hint != nullptr;
@@ -1447,7 +1545,9 @@ __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, con
       n.aTL = my > 0 && mx > 0;
       n.aTR = my > 0 && mx + 1 < mbw;
       n.TR = nxt[0];
-      const uint8_t *src = J.src + (size_t)(16 * my + PCME_PAD) * stride + 16 * mx + PCME_PAD;
+      dma_wait();  // this MB's source block
+      if (mx + 1 < mbw) dma_block16(srow + 16 * (mx + 1), stride, s_src + 64 * ((mx + 1) & 1));
+      const uint8_t *src = (const uint8_t *)(s_src + 64 * (mx & 1));
       // Pruner hint of this MB; a full search without one.
       uint32_t hv = 0u;
       int nh = -1, hpart = 0;
@@ -1462,9 +1562,9 @@ __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, con
       }
       MbRes o;
       if (HINTS && nh >= 0)
-        analyse_mb<true>(c, n, src, stride, 16 * mx, 16 * my, J.partitions, hv, nh, hpart, o, s_lds);
+        analyse_mb<true>(c, n, src, 16, 16 * mx, 16 * my, J.partitions, hv, nh, hpart, o, s_lds, ar);
       else
-        analyse_mb<false>(c, n, src, stride, 16 * mx, 16 * my, J.partitions, hv, nh, hpart, o, s_lds);
+        analyse_mb<false>(c, n, src, 16, 16 * mx, 16 * my, J.partitions, hv, nh, hpart, o, s_lds, ar);
       store_out(J.out + (size_t)my * mbw + mx, o);
       // Publish the bottom row (blocks 12..15) for the row below.
       if (l < 4) {
@@ -1479,22 +1579,28 @@ __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, con
   }
 }
 
-// The full-search kernel carries no hint code (140 VGPRs, 3 waves per
-// SIMD); the hinted kernel runs jobs with and without hints (its two MB
-// paths together need 2 waves per SIMD's registers, 205 VGPRs; forcing 3
-// spills and runs slower).
+// The full-search kernel carries no hint code (128 VGPRs); the hinted
+// kernel runs jobs with and without hints (its two MB paths together need
+// 168 VGPRs, 2 waves per SIMD; forcing 3 spills and runs slower).  Each
+// workgroup (one wave) holds FRAME_LDS_WORDS of LDS: the two search areas,
+// the two MB source blocks and padding that keeps a CU at 12 waves, 3 per
+// SIMD.  With 4 per SIMD (128 VGPRs allow it) the dispatcher packed some
+// SIMDs fuller than others and the persistent walkers ran 5% slower.
+#define FRAME_LDS_WORDS (13312 / 4)
+static_assert(FRAME_LDS_WORDS * 4 >= LDS_PER_WAVE + 512 && 163840 / (FRAME_LDS_WORDS * 4) == 12,
+              "frame kernels: room for the areas and source blocks, 12 waves per CU");
 __global__ void __launch_bounds__(64) search_frames_kernel(const DevJob *__restrict__ jobs,
                                                            const int2 *__restrict__ tickets, int ntickets,
                                                            uint32_t *ctl, uint32_t spin_limit) {
-  __shared__ __attribute__((aligned(16))) uint32_t s_lds[LDS_PER_WAVE / 4];
-  frames_body<false>(jobs, tickets, ntickets, ctl, spin_limit, s_lds);
+  __shared__ __attribute__((aligned(16))) uint32_t s_lds[FRAME_LDS_WORDS];
+  frames_body<false>(jobs, tickets, ntickets, ctl, spin_limit, s_lds, s_lds + LDS_PER_WAVE / 4);
 }
 
 __global__ void __launch_bounds__(64)
 search_frames_hinted_kernel(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets, int ntickets,
                             uint32_t *ctl, uint32_t spin_limit) {
-  __shared__ __attribute__((aligned(16))) uint32_t s_lds[LDS_PER_WAVE / 4];
-  frames_body<true>(jobs, tickets, ntickets, ctl, spin_limit, s_lds);
+  __shared__ __attribute__((aligned(16))) uint32_t s_lds[FRAME_LDS_WORDS];
+  frames_body<true>(jobs, tickets, ntickets, ctl, spin_limit, s_lds, s_lds + LDS_PER_WAVE / 4);
 }
 
 // -------------------------------------------------- per-call search_part
@@ -1541,11 +1647,12 @@ __global__ void __launch_bounds__(64) search_part_batch_kernel(const pcme_part_j
   const uint32_t seed = pack_mv(J.seed.x, J.seed.y);
   const int mbx = 16 * J.mx, mby = 16 * J.my;
   PartRes r;
+  Area ar[2] = {no_area(), no_area()};
   switch (J.part) {
-    case 0: search_part<0, false>(c, n, J.src, J.sstride, mbx, mby, seed, 0u, -1, r, s_lds); break;
-    case 1: search_part<1, false>(c, n, J.src, J.sstride, mbx, mby, seed, 0u, -1, r, s_lds); break;
-    case 2: search_part<2, false>(c, n, J.src, J.sstride, mbx, mby, seed, 0u, -1, r, s_lds); break;
-    default: search_part<3, false>(c, n, J.src, J.sstride, mbx, mby, seed, 0u, -1, r, s_lds); break;
+    case 0: search_part<0, false>(c, n, J.src, J.sstride, mbx, mby, seed, 0u, -1, r, s_lds, ar); break;
+    case 1: search_part<1, false>(c, n, J.src, J.sstride, mbx, mby, seed, 0u, -1, r, s_lds, ar); break;
+    case 2: search_part<2, false>(c, n, J.src, J.sstride, mbx, mby, seed, 0u, -1, r, s_lds, ar); break;
+    default: search_part<3, false>(c, n, J.src, J.sstride, mbx, mby, seed, 0u, -1, r, s_lds, ar); break;
   }
   const int lane = (int)lane_id();
   if (lane < 9) {
@@ -1731,8 +1838,11 @@ __global__ void __launch_bounds__(64) b_analyse_16_kernel(const pcme_b_job *__re
   }
   // 16x16 per list, seeded with the direct MVs (slice.rs:2232-2244)
   PartRes r16[2];
-  search_part<0, false>(c[0], n[0], J.src, J.sstride, px, py, dmv[0][0], 0u, -1, r16[0], s_lds);
-  search_part<0, false>(c[1], n[1], J.src, J.sstride, px, py, dmv[1][0], 0u, -1, r16[1], s_lds);
+  // each list's search starts with nothing staged (the two lists read
+  // different pictures)
+  Area ar0[2] = {no_area(), no_area()}, ar1[2] = {no_area(), no_area()};
+  search_part<0, false>(c[0], n[0], J.src, J.sstride, px, py, dmv[0][0], 0u, -1, r16[0], s_lds, ar0);
+  search_part<0, false>(c[1], n[1], J.src, J.sstride, px, py, dmv[1][0], 0u, -1, r16[1], s_lds, ar1);
   __syncthreads();
   mc_rect(c[0], px, py, 0, 0, 16, 16, r16[0].mv[0], s_pred[0]);
   mc_rect(c[1], px, py, 0, 0, 16, 16, r16[1].mv[0], s_pred[1]);
@@ -1801,7 +1911,8 @@ __global__ void __launch_bounds__(64) b_analyse_part_kernel(const pcme_b_job *__
   uint32_t pmv[2][2];
   PartRes rp[2];
   // both lists' searches in two paired steps (partition 0, then 1)
-  search_part_lists<K + 1>(c, n, J.src, J.sstride, px, py, mv16, rp[0], rp[1], s_lds);
+  Area ar[2] = {no_area(), no_area()};  // list 0 in area 0, list 1 in area 1
+  search_part_lists<K + 1>(c, n, J.src, J.sstride, px, py, mv16, rp[0], rp[1], s_lds, ar);
 #pragma unroll
   for (int l = 0; l < 2; l++) {
     pmv[l][0] = rp[l].mv[0];
@@ -1870,19 +1981,24 @@ __global__ void __launch_bounds__(64) search_batch_kernel(const pcme_search_job 
   int c1x = J.ncands > 1 ? J.cands[1].x : 0, c1y = J.ncands > 1 ? J.cands[1].y : 0;
   SRes r;
   r.x = r.y = 0; r.cost = 0;
+  Area ar = no_area();
   const uint32_t hv = 0u;  // full search: no hints
   if (J.w == 16 && J.h == 16) {
     Lane L = lane_layout<16, 16>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
-    r = search<16, 16, false>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hv, -1, s_lds);
+    r = search<16, 16, false>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hv, -1, s_lds,
+                           ar);
   } else if (J.w == 16 && J.h == 8) {
     Lane L = lane_layout<16, 8>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
-    r = search<16, 8, false>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hv, -1, s_lds);
+    r = search<16, 8, false>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hv, -1, s_lds,
+                           ar);
   } else if (J.w == 8 && J.h == 16) {
     Lane L = lane_layout<8, 16>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
-    r = search<8, 16, false>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hv, -1, s_lds);
+    r = search<8, 16, false>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hv, -1, s_lds,
+                           ar);
   } else {
     Lane L = lane_layout<8, 8>(); SrcRegs s; load_src(J.src, J.sstride, L, s);
-    r = search<8, 8, false>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hv, -1, s_lds);
+    r = search<8, 8, false>(c, s, L, J.bx, J.by, J.mvp.x, J.mvp.y, c0x, c0y, c1x, c1y, J.ncands, hv, -1, s_lds,
+                           ar);
   }
   if (lane_id() == 0) {
     pcme_mv m;
