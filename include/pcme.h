@@ -372,6 +372,56 @@ typedef struct pcme_b_out {
 int pcme_b_analyse_batch(const pcme_b_job *jobs, int njobs, pcme_b_out *out,
                          void *stream);
 
+/* ---- early P-skip gate (analyse_p, slice.rs:1884-1920) ----------------
+ * Before searching an MB the encoder predicts it at its pskip_mv
+ * (pred_block: mc_luma + mc_chroma), and when that MV is legal and the
+ * prediction's sad16 is below 2 * 16 * v4[qp % 6][0] << (qp / 6), codes the
+ * residual (code_inter, slice.rs:943-1001: 4x4 and/or 8x8 transform,
+ * dead-zone quantiser, decimation; chroma DC and AC); a coded block pattern
+ * of 0 makes the MB a PSkip that is never searched.  One job per MB; the
+ * caller supplies the skip MV it derived from its committed neighbours
+ * (pskip_mv, slice.rs:484-494).  flags: the encoder's EncParams t8x8,
+ * adaptive_t8 and cabac (the level clamp of CAVLC). */
+#define PCME_SKIP_T8X8 1
+#define PCME_SKIP_ADAPTIVE_T8 2
+#define PCME_SKIP_CABAC 4
+typedef struct pcme_skip_job {
+  const uint8_t *src_y;    /* (device) top-left of the 16x16 source MB */
+  const uint8_t *src_u;    /* (device) top-left of its 8x8 chroma blocks */
+  const uint8_t *src_v;
+  int32_t ystride, cstride;
+  const uint8_t *ref[4];   /* (device) padded reference luma: full, H, V, C */
+  const uint8_t *ref_u;    /* (device) padded reference chroma planes, */
+  const uint8_t *ref_v;    /* stride 8*mbw + 64, pad 32 */
+  int32_t mbw, mbh, mx, my;
+  pcme_mv skip_mv;
+  int32_t qp;
+  int32_t flags;           /* PCME_SKIP_* */
+  int32_t chroma_qp_offset;
+  int64_t vlim;            /* SliceEnc::vlim of the MB's row */
+} pcme_skip_job;
+
+/* The gate's outcome.  16 bytes. */
+typedef struct pcme_skip_out {
+  uint32_t sad;            /* sad16 of the prediction (0 when !mv_ok) */
+  uint32_t thresh;         /* the bound sad16 must stay below */
+  uint8_t mv_ok;           /* the skip MV is legal for the 16x16 block */
+  uint8_t coded;           /* code_inter ran (mv_ok and sad < thresh) */
+  uint8_t cbp;             /* its coded block pattern, luma bits 0-3 and
+                              chroma << 4; 0xFF when not coded */
+  uint8_t skip;            /* early PSkip: coded and cbp == 0 */
+  uint8_t t8x8;            /* code_inter's transform_8x8 flag */
+  uint8_t reserved[3];
+} pcme_skip_out;
+
+/* The gate for n MBs, one wave each; out[i] (device) gets jobs[i]'s
+ * outcome (jobs is a host array).  Bit-identical to the reference,
+ * including the adaptive 8x8/4x4 choice (its float rate comparison is
+ * evaluated in the reference's operation order, with the C library's
+ * log2 tabulated on the host). */
+int pcme_skip_gate_batch(const pcme_skip_job *jobs, int njobs, pcme_skip_out *out,
+                         void *stream);
+
 /* Motion-compensated prediction of the chosen partitions (pred_block,
  * slice.rs:1725-1749 = mc_luma picture.rs:415-449 + mc_chroma
  * picture.rs:451-487) for every MB of a searched frame.  ref planes and

@@ -80,6 +80,11 @@ OHINT_DTYPE = np.dtype([
     ("part", "u1"), ("ncand", "u1"), ("prune", "u1"), ("reserved", "u1"),
 ])
 
+# oskip_out has the layout of pcme_skip_out
+OSKIP_DTYPE = np.dtype([("sad", "<u4"), ("thresh", "<u4"), ("mv_ok", "u1"), ("coded", "u1"), ("cbp", "u1"),
+                        ("skip", "u1"), ("t8x8", "u1"), ("reserved", "u1", (3,))])
+SKIP_T8X8, SKIP_ADAPTIVE_T8, SKIP_CABAC = 1, 2, 4
+
 _lib = None
 
 
@@ -180,6 +185,27 @@ def load() -> ctypes.CDLL:
     lib.o_block_var.argtypes = [P, c.c_int, c.c_int, c.c_int]
     lib.o_aq_offsets.restype = None
     lib.o_aq_offsets.argtypes = [P, c.c_int, P, P, c.c_int, c.c_int, c.c_int, c.c_float, P]
+    lib.o_chroma_qp.restype = c.c_int
+    lib.o_chroma_qp.argtypes = [c.c_int]
+    lib.o_lambda_ssd.restype = c.c_double
+    lib.o_lambda_ssd.argtypes = [c.c_int]
+    lib.o_quant_tables.restype = None
+    lib.o_quant_tables.argtypes = [P]
+    lib.o_fdct4.restype = None
+    lib.o_fdct4.argtypes = [P, P]
+    lib.o_decimate_score.restype = c.c_uint32
+    lib.o_decimate_score.argtypes = [P, c.c_int]
+    lib.o_code_luma4.restype = c.c_int
+    lib.o_code_luma4.argtypes = [P, P, c.c_int, c.c_int, c.c_int, P, P]
+    lib.o_code_luma8.restype = c.c_int
+    lib.o_code_luma8.argtypes = [P, P, c.c_int, c.c_int, P, P]
+    lib.o_code_chroma.restype = c.c_int
+    lib.o_code_chroma.argtypes = [P, P, P, P, c.c_int, c.c_int, c.c_int, c.c_int]
+    lib.o_code_inter.restype = c.c_int
+    lib.o_code_inter.argtypes = [P, P, P, P, P, P, c.c_int, c.c_int, c.c_int, c.POINTER(c.c_int)]
+    lib.o_skip_gate.restype = None
+    lib.o_skip_gate.argtypes = [c.POINTER(oparams), c.POINTER(oref), P, P, c.c_int, c.c_int, c.c_int, omv, P, P, P,
+                                c.c_int, c.c_int, c.c_int, P]
     lib.o_lookahead_mt.restype = c.c_double
     lib.o_lookahead_mt.argtypes = [P, c.c_int, c.c_int, c.c_int, c.c_int, P, P]
     _lib = lib
@@ -424,3 +450,72 @@ def mbtree_offsets(blocks: np.ndarray, prop: np.ndarray) -> np.ndarray:
     out = np.zeros(b.size, np.float32)
     load().o_mbtree_offsets(_p(b), _p(p), b.size, _p(out))
     return out
+
+
+# ------------------------------------------- early P-skip gate (skip_oracle.c)
+
+def quant_tables():
+    """QuantTables::new -> dict of int32 arrays mf4/v4 [6,16], mf8/v8 [6,64]."""
+    buf = np.zeros(6 * 16 * 2 + 6 * 64 * 2, np.int32)
+    load().o_quant_tables(_p(buf))
+    mf4, v4 = buf[:96].reshape(6, 16), buf[96:192].reshape(6, 16)
+    mf8, v8 = buf[192:576].reshape(6, 64), buf[576:].reshape(6, 64)
+    return {"mf4": mf4, "v4": v4, "mf8": mf8, "v8": v8}
+
+
+def fdct4(d) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(d, np.int32).reshape(16))
+    o = np.zeros(16, np.int32)
+    load().o_fdct4(_p(a), _p(o))
+    return o.reshape(4, 4)
+
+
+def decimate_score(levels) -> int:
+    a = np.ascontiguousarray(np.asarray(levels, np.int16))
+    return int(load().o_decimate_score(_p(a), a.size))
+
+
+def code_luma4(src, pred, qp, decimate=True, cabac=True):
+    s = np.ascontiguousarray(src, np.uint8).reshape(256)
+    p = np.ascontiguousarray(pred, np.uint8).reshape(256)
+    lv = np.zeros((16, 16), np.int16)
+    rec = np.zeros(256, np.uint8)
+    cbp = load().o_code_luma4(_p(s), _p(p), qp, int(decimate), int(cabac), _p(lv), _p(rec))
+    return cbp, lv, rec.reshape(16, 16)
+
+
+def code_luma8(src, pred, qp, decimate=True):
+    s = np.ascontiguousarray(src, np.uint8).reshape(256)
+    p = np.ascontiguousarray(pred, np.uint8).reshape(256)
+    lv = np.zeros((4, 64), np.int16)
+    rec = np.zeros(256, np.uint8)
+    cbp = load().o_code_luma8(_p(s), _p(p), qp, int(decimate), _p(lv), _p(rec))
+    return cbp, lv, rec.reshape(16, 16)
+
+
+def code_chroma(su, sv, pu, pv, qp, offset=0, intra=False, cabac=True) -> int:
+    a = [np.ascontiguousarray(x, np.uint8).reshape(64) for x in (su, sv, pu, pv)]
+    return int(load().o_code_chroma(*[_p(x) for x in a], qp, offset, int(intra), int(cabac)))
+
+
+def code_inter(sy, su, sv, py, pu, pv, qp, flags, offset=0):
+    """code_inter's (cbp, t8x8)."""
+    a = [np.ascontiguousarray(x, np.uint8).reshape(-1) for x in (sy, su, sv, py, pu, pv)]
+    t8 = ctypes.c_int()
+    cbp = load().o_code_inter(*[_p(x) for x in a], qp, flags, offset, ctypes.byref(t8))
+    return int(cbp), int(t8.value)
+
+
+def skip_gate(prm: oparams, ref: Ref, ref_u: np.ndarray, ref_v: np.ndarray, mx: int, my: int, skip_mv,
+              src_y, src_u, src_v, qp: int, flags: int, offset: int = 0) -> np.ndarray:
+    """The early P-skip gate of one MB -> one OSKIP_DTYPE record.  ref_u /
+    ref_v are the padded chroma planes of the reference."""
+    out = np.zeros(1, OSKIP_DTYPE)
+    u = np.ascontiguousarray(ref_u, np.uint8)
+    v = np.ascontiguousarray(ref_v, np.uint8)
+    sy = np.ascontiguousarray(src_y, np.uint8)
+    su = np.ascontiguousarray(src_u, np.uint8)
+    sv = np.ascontiguousarray(src_v, np.uint8)
+    load().o_skip_gate(ctypes.byref(prm), ctypes.byref(ref.view), _p(u), _p(v), u.shape[1], mx, my, omv(*skip_mv),
+                       _p(sy), _p(su), _p(sv), qp, flags, offset, _p(out))
+    return out[0]

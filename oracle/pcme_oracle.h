@@ -259,3 +259,35 @@ void o_aq_offsets(const uint8_t *y, int ys, const uint8_t *u, const uint8_t *v, 
                   float strength, float *out);
 double o_b_frame_mt(const oparams *p, const ombinfo2 *mbs, const ombinfo2 *cols, const oref *r0, const oref *r1,
                     const uint8_t *src, int sstride, int pad, uint32_t lam, int threads, ob_out *out);
+
+/* ---- early P-skip gate (skip_oracle.c; slice.rs:1884-1920, 679-1001,
+ * transform.rs, residual.rs) ---- */
+typedef struct {
+  int32_t mf4[6][16], v4[6][16], mf8[6][64], v8[6][64];
+} oquant;
+/* flags of the gate: the encoder's t8x8, adaptive_t8 and cabac params */
+#define OSKIP_T8X8 1
+#define OSKIP_ADAPTIVE_T8 2
+#define OSKIP_CABAC 4
+/* Layout of pcme_skip_out (include/pcme.h). */
+typedef struct {
+  uint32_t sad, thresh;
+  uint8_t mv_ok, coded, cbp, skip, t8x8, reserved[3];
+} oskip_out;
+void o_quant_tables(oquant *q);
+int o_chroma_qp(int qpi);
+double o_lambda_ssd(int qp);
+void o_fdct4(const int32_t d[16], int32_t o[16]);
+uint32_t o_decimate_score(const int16_t *levels, int n);
+int o_code_luma4(const uint8_t src[256], const uint8_t pred[256], int qp, int decimate, int cabac,
+                 int16_t levels[16][16], uint8_t rec[256]);
+int o_code_luma8(const uint8_t src[256], const uint8_t pred[256], int qp, int decimate, int16_t levels[4][64],
+                 uint8_t rec[256]);
+int o_code_chroma(const uint8_t su[64], const uint8_t sv[64], const uint8_t pu[64], const uint8_t pv[64], int qp,
+                  int chroma_qp_offset, int intra, int cabac);
+int o_code_inter(const uint8_t sy[256], const uint8_t su[64], const uint8_t sv[64], const uint8_t py[256],
+                 const uint8_t pu[64], const uint8_t pv[64], int qp, int flags, int chroma_qp_offset,
+                 int *t8x8_out);
+void o_skip_gate(const oparams *p, const oref *r, const uint8_t *ru, const uint8_t *rv, int cstride, int mx, int my,
+                 omv skip_mv, const uint8_t *src_y, const uint8_t *src_u, const uint8_t *src_v, int qp, int flags,
+                 int chroma_qp_offset, oskip_out *out);

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -69,7 +70,12 @@ struct Dev {
   std::vector<pcme_frame_job> cached_jobs;
   uint8_t *scratch = nullptr;      // this device's copy of `scratch` (hipGetSymbolAddress)
   bool scratch_init = false;       // its control block has been cleared once
+  SkipTables *skip_tab = nullptr;  // this device's copy of `skip_tables`, filled on first use
 };
+
+// Tables of the early P-skip gate; filled on a device by the first
+// pcme_skip_gate_batch call there.
+static __device__ __align__(256) SkipTables skip_tables;
 
 // Workspace of pcme_search_frames_noalloc: a control block, the job table,
 // the tickets and the exchange words of one call, in a file-scope device
@@ -853,6 +859,55 @@ int pcme_b_analyse_batch(const pcme_b_job *jobs, int njobs, pcme_b_out *out, voi
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(b_analyse_part_kernel<1>, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream, dj, njobs,
                      out);
+  HIPCHK(hipGetLastError());
+  return PCME_OK;
+}
+
+static_assert(sizeof(pcme_skip_job) == 120 && sizeof(pcme_skip_out) == 16, "pcme_skip_* layout");
+
+int pcme_skip_gate_batch(const pcme_skip_job *jobs, int njobs, pcme_skip_out *out, void *stream) {
+  if (njobs < 0 || (njobs > 0 && (!jobs || !out)) || ((uintptr_t)out & 3u))
+    return fail(PCME_EINVAL, "pcme_skip_gate_batch: bad arguments");
+  for (int i = 0; i < njobs; i++) {
+    const pcme_skip_job &J = jobs[i];
+    bool ok = J.src_y && J.src_u && J.src_v && J.ref_u && J.ref_v;
+    for (int k = 0; k < 4; k++) ok = ok && J.ref[k];
+    if (!ok) return fail(PCME_EINVAL, "job %d has a null pointer", i);
+    if (J.ystride < 16 || J.cstride < 8) return fail(PCME_EINVAL, "job %d: source strides %d/%d", i, J.ystride, J.cstride);
+    if (J.mbw < 1 || J.mbh < 1 || J.mx < 0 || J.my < 0 || J.mx >= J.mbw || J.my >= J.mbh)
+      return fail(PCME_EINVAL, "job %d: MB outside the picture", i);
+    if (J.qp < 0 || J.qp > 51) return fail(PCME_EINVAL, "job %d: qp %d outside [0, 51]", i, J.qp);
+    if (J.flags & ~(PCME_SKIP_T8X8 | PCME_SKIP_ADAPTIVE_T8 | PCME_SKIP_CABAC))
+      return fail(PCME_EINVAL, "job %d: unknown flags 0x%x", i, J.flags);
+    if (J.chroma_qp_offset < -12 || J.chroma_qp_offset > 12)
+      return fail(PCME_EINVAL, "job %d: chroma_qp_offset %d outside [-12, 12]", i, J.chroma_qp_offset);
+    if (J.vlim < 16LL * J.my + 17 || J.vlim > 16LL * J.mbh + 44)
+      return fail(PCME_EINVAL, "job %d: vlim %lld outside [16*my+17, 16*mbh+44]", i, (long long)J.vlim);
+  }
+  if (njobs == 0) return PCME_OK;
+  Dev *dv;
+  int rc = get_dev(&dv);
+  if (rc) return rc;
+  const size_t bytes = sizeof(pcme_skip_job) * (size_t)njobs;
+  std::lock_guard<std::mutex> g(dv->mu);
+  if (!dv->skip_tab) {
+    SkipTables *dst = nullptr;
+    HIPCHK(hipGetSymbolAddress((void **)&dst, HIP_SYMBOL(skip_tables)));
+    if (!dst) return fail(PCME_EHIP, "hipGetSymbolAddress(skip_tables) returned null");
+    std::unique_ptr<SkipTables> h(new SkipTables());
+    build_skip_tables(h.get());
+    HIPCHK(hipMemcpy(dst, h.get(), sizeof(SkipTables), hipMemcpyHostToDevice));
+    HIPCHK(hipStreamSynchronize(nullptr));  // the copy has landed before any stream reads it
+    dv->skip_tab = dst;
+  }
+  WsGuard wg;
+  wg.s = (hipStream_t)stream;
+  CallWs *w = nullptr;
+  if ((rc = take_ws(dv, wg.s, bytes, bytes, &w))) return rc;
+  wg.w = w;
+  if ((rc = ws_upload(w, jobs, bytes, (hipStream_t)stream))) return rc;
+  hipLaunchKernelGGL(skip_gate_kernel, dim3((unsigned)njobs), dim3(64), 0, (hipStream_t)stream,
+                     (const pcme_skip_job *)w->dev, njobs, (const SkipTables *)dv->skip_tab, out);
   HIPCHK(hipGetLastError());
   return PCME_OK;
 }

@@ -106,6 +106,23 @@ __global__ void tree_offsets_kernel(const pcme_tree_block *__restrict__ b, const
                                     float *__restrict__ out);
 #endif
 
+// Tables of the early P-skip gate, one copy per device: the quantiser
+// (transform.rs QuantTables::new), Lambdas::new().ssd (slice.rs:52-68) and
+// the rate estimate of a level, 3 + 2 log2|v| (0.4 for 0) for |v| <= 32768
+// (slice.rs:968-974), all computed on the host with the C library.
+struct SkipTables {
+  int32_t mf4[6][16], v4[6][16], mf8[6][64], v8[6][64];
+  double lam_ssd[52];
+  double lb[32769];
+};
+
+#ifdef __HIP__
+__global__ void skip_gate_kernel(const pcme_skip_job *__restrict__ jobs, int njobs, const SkipTables *__restrict__ T,
+                                 pcme_skip_out *out);
+#endif
+// Fills the tables (host side).
+void build_skip_tables(SkipTables *t);
+
 // Message returned by pcme_last_error() on this thread (host side).
 void set_error(const char *msg);
 

@@ -32,17 +32,6 @@
 namespace pcme {
 
 
-// Quarter-pel source table (picture.rs:387-404), 8 bits per position q:
-// [1:0] plane A, [2] dx A, [3] dy A, [5:4] plane B, [6] dx B, [7] dy B.
-#define QW0 0x41111000u
-#define QW1 0x61312120u
-#define QW2 0x63333222u
-#define QW3 0x96939282u
-
-__device__ __forceinline__ uint32_t qpel_entry(uint32_t q) {
-  uint32_t w = (q & 8u) ? ((q & 4u) ? QW3 : QW2) : ((q & 4u) ? QW1 : QW0);
-  return (w >> ((q & 3u) * 8u)) & 0xFFu;
-}
 
 // --------------------------------------------------------- search context
 

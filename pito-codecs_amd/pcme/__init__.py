@@ -102,6 +102,29 @@ class BJob(ctypes.Structure):
 assert ctypes.sizeof(BJob) == 328
 
 
+# pcme_skip_job flags
+SKIP_T8X8, SKIP_ADAPTIVE_T8, SKIP_CABAC = 1, 2, 4
+
+
+class SkipJob(ctypes.Structure):
+    """pcme_skip_job: the early P-skip gate of one MB (source blocks and
+    reference planes on the device)."""
+    _fields_ = [("src_y", ctypes.c_void_p), ("src_u", ctypes.c_void_p), ("src_v", ctypes.c_void_p),
+                ("ystride", ctypes.c_int32), ("cstride", ctypes.c_int32),
+                ("ref", ctypes.c_void_p * 4), ("ref_u", ctypes.c_void_p), ("ref_v", ctypes.c_void_p),
+                ("mbw", ctypes.c_int32), ("mbh", ctypes.c_int32), ("mx", ctypes.c_int32), ("my", ctypes.c_int32),
+                ("skip_mv", Mv), ("qp", ctypes.c_int32), ("flags", ctypes.c_int32),
+                ("chroma_qp_offset", ctypes.c_int32), ("vlim", ctypes.c_int64)]
+
+
+assert ctypes.sizeof(SkipJob) == 120
+
+# pcme_skip_out (16 bytes)
+SKIP_OUT_DTYPE = np.dtype([("sad", "<u4"), ("thresh", "<u4"), ("mv_ok", "u1"), ("coded", "u1"), ("cbp", "u1"),
+                           ("skip", "u1"), ("t8x8", "u1"), ("reserved", "u1", (3,))])
+assert SKIP_OUT_DTYPE.itemsize == 16
+
+
 class SynthCfg(ctypes.Structure):
     _fields_ = [("w", ctypes.c_int32), ("h", ctypes.c_int32),
                 ("frames", ctypes.c_int32), ("nrect", ctypes.c_int32),
@@ -173,7 +196,7 @@ EXPORTS = (
     "pcme_net_create", "pcme_net_destroy", "pcme_net_get_params", "pcme_net_set_params",
     "pcme_net_inputs", "pcme_net_forward", "pcme_net_grad", "pcme_net_train_step", "pcme_net_hints",
     "pcme_lowres", "pcme_complexity", "pcme_mbtree_analyse", "pcme_search_part_batch", "pcme_b_analyse_batch",
-    "pcme_aq_offsets", "pcme_mbtree_propagate", "pcme_mbtree_offsets",
+    "pcme_aq_offsets", "pcme_mbtree_propagate", "pcme_mbtree_offsets", "pcme_skip_gate_batch",
 )
 
 _lib = None
@@ -209,11 +232,10 @@ def load() -> ctypes.CDLL:
         "pcme_search_frames_sync": (c.c_int, [c.POINTER(FrameJob), c.c_int, vp]),
         "pcme_search_frames_check": (c.c_int, [vp]),
         "pcme_search_frames_noalloc": (c.c_int, [c.POINTER(FrameJob), c.c_int, vp]),
-        "pcme_search_frames_check": (c.c_int, [vp]),
-        "pcme_search_frames_noalloc": (c.c_int, [c.POINTER(FrameJob), c.c_int, vp]),
         "pcme_search_batch": (c.c_int, [c.POINTER(SearchJob), c.c_int, vp, vp, vp]),
         "pcme_search_part_batch": (c.c_int, [c.POINTER(PartJob), c.c_int, vp, vp]),
         "pcme_b_analyse_batch": (c.c_int, [c.POINTER(BJob), c.c_int, vp, vp]),
+        "pcme_skip_gate_batch": (c.c_int, [c.POINTER(SkipJob), c.c_int, vp, vp]),
         "pcme_pred_frame": (c.c_int, [vp, vp, vp, vp, vp, vp, vp, c.c_int32, c.c_int32, vp, vp, vp, vp]),
         "pcme_synth_clip": (c.c_int, [c.POINTER(SynthCfg), vp, vp, vp, c.c_int32]),
         "pcme_plan_create": (c.c_int, [c.POINTER(FrameJob), c.c_int, c.POINTER(vp)]),
@@ -409,12 +431,6 @@ def search_frames_check(stream=None) -> None:
     _check(load().pcme_search_frames_check(_stream_ptr(stream)))
 
 
-def search_frames_check(stream=None) -> None:
-    """pcme_search_frames_check: raises PcmeError(PCME_ESTALL) if any
-    pcme_search_frames launch of this device stalled since the last check."""
-    _check(load().pcme_search_frames_check(_stream_ptr(stream)))
-
-
 class Plan:
     """A reusable frame-search plan (pcme_plan_*): set up once, launch many
     times; launches are graph-capturable."""
@@ -477,6 +493,14 @@ def b_analyse_batch(jobs: list[BJob], out_ptr: int, stream=None) -> None:
     lib = load()
     arr = (BJob * len(jobs))(*jobs)
     _check(lib.pcme_b_analyse_batch(arr, len(jobs), out_ptr, _stream_ptr(stream)))
+
+
+def skip_gate_batch(jobs: list[SkipJob], out_ptr: int, stream=None) -> None:
+    """pcme_skip_gate_batch: out_ptr (device) receives len(jobs)
+    SKIP_OUT_DTYPE records."""
+    lib = load()
+    arr = (SkipJob * len(jobs))(*jobs)
+    _check(lib.pcme_skip_gate_batch(arr, len(jobs), out_ptr, _stream_ptr(stream)))
 
 
 def pred_frame(clip: DeviceClip, ref: int, mb_ptr: int, pred_y_ptr: int, pred_u_ptr: int,

@@ -174,6 +174,31 @@ def test_search_part_batch_checks_arguments(pcme_lib):
         assert lib.pcme_search_part_batch(job(), 1, out, None) == pcme_lib.PCME_ENODEV
 
 
+def test_skip_gate_batch_checks_arguments(pcme_lib):
+    lib = pcme_lib.load()
+
+    def job(**kw):
+        j = pcme_lib.SkipJob()
+        j.src_y, j.src_u, j.src_v, j.ref_u, j.ref_v = 64, 64, 64, 64, 64
+        j.ref[:] = [64, 64, 64, 64]
+        j.ystride, j.cstride, j.mbw, j.mbh, j.qp, j.vlim = 16, 8, 4, 4, 27, 4 * 16 + 44
+        for k, v in kw.items():
+            setattr(j, k, v)
+        return (pcme_lib.SkipJob * 1)(j)
+
+    out = 64
+    for bad in (job(src_u=None), job(qp=52), job(qp=-1), job(flags=8), job(chroma_qp_offset=13), job(mx=4),
+                job(my=-1), job(vlim=16), job(vlim=4 * 16 + 45), job(ystride=15), job(cstride=7)):
+        assert lib.pcme_skip_gate_batch(bad, 1, out, None) == pcme_lib.PCME_EINVAL
+    j = job()
+    j[0].ref[2] = None
+    assert lib.pcme_skip_gate_batch(j, 1, out, None) == pcme_lib.PCME_EINVAL
+    assert lib.pcme_skip_gate_batch(job(), 1, 66, None) == pcme_lib.PCME_EINVAL  # misaligned out
+    assert lib.pcme_skip_gate_batch(job(), 0, out, None) == pcme_lib.PCME_OK
+    if pcme_lib.device_count() == 0:
+        assert lib.pcme_skip_gate_batch(job(), 1, out, None) == pcme_lib.PCME_ENODEV
+
+
 def test_b_analyse_batch_checks_arguments(pcme_lib):
     lib = pcme_lib.load()
 
