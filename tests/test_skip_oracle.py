@@ -154,3 +154,54 @@ def test_chroma_dc_alone(oracle):
     assert oracle.code_chroma(z + 1, z, z, z, 27) == 0
     assert oracle.code_chroma(z + 2, z, z, z, 27) == 1
     assert oracle.code_chroma(z, z - 2, z, z, 27) == 1
+
+
+def test_adaptive_transform_choice_by_hand(oracle):
+    """code_inter's adaptive choice j8 <= j4, j = ssd(src, rec) + lam * bits,
+    on residuals whose levels, reconstructions and rates follow by hand
+    (lam = 0.85 * 2^((qp-12)/3); a level v costs 3 + 2 log2|v|, a zero 0.4,
+    a coded 4x4 block 1 more; the 8x8 transform of a flat block is 4096 d at
+    DC, of the +-1 step s = (1,1,1,1,-1,-1,-1,-1) the odd rows 62, -22, 14, -10):
+    * QP 12: the top-left 8x8 holds +6 in its first and last 4x4 block, 0 in
+      the other two.  4x4: DC level (96*13107 + 22016) >> 17 = 9, dequantised
+      9*40 = 360 -> (360+32) >> 6 = 6: exact, j4 = 0.85 * (2*(1 + 9.34 + 15*0.4)
+      + 2*(1 + 16*0.4)) = 40.4.  8x8: the block is 3*(1 + s x s), so DC
+      2048*6 -> level 9, and the odd-odd coefficients 3*o[k]*o[l] with MF
+      round(2^36 / (578^2*18)) = 11428, f = 2^24*43/256: (1,1) 11532 -> 8,
+      (1,3) 4092 -> 2 (twice), (3,3), (1,5), (1,7) -> 1 (five), (3,5) 924 -> 0;
+      bits 9.34 + 9 + 2*5 + 5*3 + 55*0.4 = 65.34, j8 >= 0.85 * 65.34 = 55.5 >
+      j4: 4x4 wins, cbp 0x1 without the 8x8 flag; 8x8 alone keeps it.
+    * QP 27, d = 6 everywhere: 4x4 level 1 (score 12, kept), rec pred + 4
+      (14 << 4 = 224 -> 4), ssd 1024, j4 = 1024 + 27.2 * 160 = 5376.  8x8
+      level 3 (score 64, kept), dequantised (3*448 + 2) >> 2 = 336 -> 5, ssd
+      256, bits 4 * (3 + 2 log2 3 + 63 * 0.4), j8 = 3669.  8x8 wins: cbp 0xF
+      with the 8x8 flag.
+    * QP 27, d = 3: 4x4 level 1 per block (kept, cbp 0xF) but 8x8 level 1
+      (score 3 < 4, decimated): j8 = ssd 2304 + 0 < j4 = 256 + 4352, so the
+      adaptive choice codes nothing: an early PSkip that 4x4 alone misses.
+    * QP 27, d = 2: both transforms code nothing, j8 = j4 (the tie goes to
+      8x8, whose flag is only reported with a coded luma block): cbp 0."""
+    T8, AD = oracle.SKIP_T8X8, oracle.SKIP_ADAPTIVE_T8
+    pred = np.full((16, 16), 100, np.uint8)
+    c = np.full((8, 8), 128, np.uint8)
+
+    def run(res, qp, flags):
+        return oracle.code_inter((pred.astype(int) + res).astype(np.uint8), c, c, pred, c, c, qp, flags)
+
+    lam12, lam27 = oracle.load().o_lambda_ssd(12), oracle.load().o_lambda_ssd(27)
+    assert abs(lam12 - 0.85) < 1e-12 and abs(lam27 - 27.2) < 1e-12
+    assert oracle.quant_tables()["mf8"][0, 9] == 11428
+    diag = np.zeros((16, 16), int)
+    diag[0:4, 0:4] = diag[4:8, 4:8] = 6
+    j4 = lam12 * (2 * (1 + 3 + 2 * np.log2(9) + 15 * 0.4) + 2 * (1 + 16 * 0.4))
+    j8_min = lam12 * (3 + 2 * np.log2(9) + 9 + 2 * 5 + 5 * 3 + 55 * 0.4)
+    assert j4 == pytest.approx(40.36, abs=0.01) and j8_min == pytest.approx(55.54, abs=0.01)
+    assert run(diag, 12, T8 | AD) == (0x01, 0)
+    assert run(diag, 12, T8) == (0x01, 1) and run(diag, 12, 0) == (0x01, 0)
+    _, lv8, _ = oracle.code_luma8((pred.astype(int) + diag).astype(np.uint8), pred, 12)
+    assert sorted(np.abs(lv8[0][lv8[0] != 0]).tolist()) == [1, 1, 1, 1, 1, 2, 2, 8, 9]
+    j8 = 256 + lam27 * 4 * (3 + 2 * np.log2(3) + 63 * 0.4)
+    assert abs(j8 - 3669.0) < 1.0 and 1024 + lam27 * 160 == pytest.approx(5376)
+    assert run(6, 27, T8 | AD) == (0x0F, 1)
+    assert run(3, 27, T8 | AD) == (0, 0) and run(3, 27, 0) == (0x0F, 0)
+    assert run(2, 27, T8 | AD) == (0, 0)
