@@ -59,6 +59,9 @@ def parse():
     ap.add_argument("--cpu-frames-1t", type=int, default=4, help="P frames of the one-thread CPU sample")
     ap.add_argument("--grid", type=int, default=0,
                     help="persistent grid in waves (skips the autotuner; 0: autotune)")
+    ap.add_argument("--config", default="",
+                    help="kernel configuration waves_per_group,waves_per_cu,reuse_regions (with --grid: skips "
+                         "the autotuner)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the timed steps (no graph, critical-path, pruned, lookahead, B or CPU legs): "
                          "the command the profiles are taken with")
@@ -185,13 +188,7 @@ def main():
     prepare()
     if args.kernels_only:
         args.no_graph = args.no_pruned = args.no_lookahead = args.no_b = args.no_cpu = True
-    tune = None
-    if args.grid > 0:
-        tune = {"waves": plan.set_grid(args.grid), "search_ms": None, "fixed": True}
-    elif not args.no_autotune:
-        waves, tms = plan.autotune(reps=2, stream=sp)
-        tune = {"waves": waves, "search_ms": round(tms, 3)}
-        print(f"[bench] autotune: {waves} waves, {tms:.2f} ms", file=sys.stderr, flush=True)
+    tune = fix_or_tune(plan, args, sp)
 
     ev = []  # (start, end) events around each search launch, on the launch stream
 
@@ -251,6 +248,8 @@ def main():
     crit = None
     if args.workload in ("1080p", "4k") and not args.kernels_only:
         p1 = pcme.Plan(jobs[:1])
+        if tune:  # the step's kernel configuration, one walker per row
+            p1.set_config(0, tune["waves_per_group"], tune["waves_per_cu"], tune["reuse_regions"])
         p1.launch(sp)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -316,6 +315,21 @@ def main():
         line["cpu_baseline"] = cpu
         line["parity_sample"] = bool(parity)
     print(json.dumps(line), flush=True)
+
+
+def fix_or_tune(plan, args, stream) -> dict | None:
+    """The plan's kernel configuration: the one --config / --grid name, or
+    the autotuner's choice over every built configuration and grid size."""
+    if args.grid > 0 or args.config:
+        wpg, wcu, reuse = (int(v) for v in (args.config or "1,12,1").split(","))
+        t = plan.set_config(args.grid, wpg, wcu, reuse)
+        return dict(t.as_dict(), fixed=True, ms=None)
+    if args.no_autotune:
+        return None
+    t = plan.autotune_ex(reps=2, stream=stream)
+    print(f"[bench] autotune: {t.waves} waves, {t.waves_per_group} per group, {t.waves_per_cu} per CU, "
+          f"reuse {t.reuse_regions}: {t.ms:.2f} ms", file=sys.stderr, flush=True)
+    return dict(t.as_dict(), ms=round(t.ms, 3))
 
 
 def cpu_share() -> int:
@@ -642,12 +656,7 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
     infer()
     if args.kernels_only:
         args.no_graph = args.no_pruned = args.no_lookahead = args.no_b = args.no_cpu = True
-    tune = None
-    if args.grid > 0:
-        tune = {"waves": plan.set_grid(args.grid), "search_ms": None, "fixed": True}
-    elif not args.no_autotune:
-        waves, tms = plan.autotune(reps=2, stream=sp)
-        tune = {"waves": waves, "search_ms": round(tms, 3)}
+    tune = fix_or_tune(plan, args, sp)
     ev_h, ev_s = [], []
 
     def step(record):

@@ -235,12 +235,40 @@ int pcme_search_frames_check(void *stream);
  * over a set of grid sizes (`reps` launches each, on `stream`) and keeps the
  * fastest; it writes the chosen wave count and its time. */
 typedef struct pcme_plan pcme_plan;
+
+/* A plan's kernel configuration (no counterpart in the reference: how the
+ * search is laid out on the GPU, never what it computes -- every
+ * configuration gives the same records).  waves: the persistent grid (0 =
+ * every wave the configuration keeps resident); waves_per_group: row
+ * walkers per workgroup (1, 2 or 4); waves_per_cu: the occupancy cap the
+ * LDS each walker reserves sets (12 or 8; for a plan with pruner hints its
+ * kernel's own); reuse_regions: 1 = a walker keeps staged LDS regions from
+ * MB to MB, 0 = every search stages its own; ms: the time per launch the
+ * autotuner measured.  The built configurations are (1, 12, 1) -- the
+ * default --, (2, 12, 1), (4, 12, 1), (1, 8, 1), (2, 8, 1) and (1, 12, 0);
+ * a plan with hints runs (1, *, 1) only.
+ * pcme_plan_get_config reads the plan's configuration; pcme_plan_set_config
+ * selects one (PCME_EINVAL for a configuration that is not built);
+ * pcme_plan_autotune_ex times every built configuration over the grid
+ * sizes of pcme_plan_autotune (`reps` launches each, on `stream`), keeps
+ * the fastest and writes it to *best. */
+typedef struct pcme_tune {
+  int32_t waves;
+  int32_t waves_per_group;
+  int32_t waves_per_cu;
+  int32_t reuse_regions;
+  float ms;
+} pcme_tune;
+
 int pcme_plan_create(const pcme_frame_job *jobs, int njobs, pcme_plan **out);
 int pcme_plan_launch(pcme_plan *plan, void *stream);
 int pcme_plan_check(pcme_plan *plan, void *stream);
 int pcme_plan_set_grid(pcme_plan *plan, int32_t waves);
 int pcme_plan_autotune(pcme_plan *plan, void *stream, int32_t reps,
                        int32_t *best_waves, float *best_ms);
+int pcme_plan_get_config(pcme_plan *plan, pcme_tune *out);
+int pcme_plan_set_config(pcme_plan *plan, const pcme_tune *cfg);
+int pcme_plan_autotune_ex(pcme_plan *plan, void *stream, int32_t reps, pcme_tune *best);
 void pcme_plan_destroy(pcme_plan *plan);
 
 /* Test hook: bound of every wavefront dependency wait, in polls (default

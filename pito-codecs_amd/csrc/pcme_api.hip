@@ -65,6 +65,8 @@ struct Dev {
   int id = -1;
   std::mutex mu;
   int resident_waves[2] = {0, 0};  // full-search / hinted kernel
+  int resident_var[8] = {0};       // resident waves of each full-search configuration (kVariants)
+  int cus = 1;                     // compute units
   std::vector<CallWs *> ws;        // per-call workspaces (at most kMaxWs)
   Plan *cached = nullptr;          // plan of the last pcme_search_frames call
   std::vector<pcme_frame_job> cached_jobs;
@@ -86,6 +88,30 @@ static __device__ __align__(256) SkipTables skip_tables;
 #define PCME_NOALLOC_BYTES (16u << 20)
 #define PCME_NOALLOC_MAX_JOBS 1024
 static __device__ __align__(256) uint8_t scratch[PCME_NOALLOC_BYTES];
+
+// Configurations of the full-search kernel the autotuner chooses from
+// (search_frames_var_kernel): row walkers per workgroup, the occupancy cap
+// its LDS reservation sets, and whether staged LDS regions are kept along a
+// row.  Entry 0 is search_frames_kernel, the default.
+struct Variant {
+  int wpg, wcu;
+  bool reuse;
+};
+static const Variant kVariants[] = {{1, 12, true}, {2, 12, true}, {4, 12, true},
+                                    {1, 8, true},  {2, 8, true},  {1, 12, false}};
+static const int kNumVariants = (int)(sizeof kVariants / sizeof kVariants[0]);
+static_assert(sizeof kVariants / sizeof kVariants[0] <= 8, "Dev::resident_var holds 8 configurations");
+
+static const void *variant_fn(int v) {
+  switch (v) {
+    case 1: return (const void *)search_frames_var_kernel<2, 12, true>;
+    case 2: return (const void *)search_frames_var_kernel<4, 12, true>;
+    case 3: return (const void *)search_frames_var_kernel<1, 8, true>;
+    case 4: return (const void *)search_frames_var_kernel<2, 8, true>;
+    case 5: return (const void *)search_frames_var_kernel<1, 12, false>;
+    default: return (const void *)search_frames_kernel;
+  }
+}
 
 // Workspaces per device: with this many in flight on other streams a call
 // waits for the oldest one to complete.
@@ -125,6 +151,13 @@ static int get_dev(Dev **out) {
         &per_cu, v ? search_frames_hinted_kernel : search_frames_kernel, 64, 0));
     if (per_cu < 1) per_cu = 1;
     dv->resident_waves[v] = per_cu * p.multiProcessorCount;
+  }
+  dv->cus = p.multiProcessorCount;
+  for (int v = 0; v < kNumVariants; v++) {
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, variant_fn(v), 64 * kVariants[v].wpg, 0));
+    if (per_cu < 1) per_cu = 1;
+    dv->resident_var[v] = per_cu * kVariants[v].wpg * p.multiProcessorCount;
   }
   HIPCHK(hipGetSymbolAddress((void **)&dv->scratch, HIP_SYMBOL(scratch)));
   if (!dv->scratch) return fail(PCME_EHIP, "hipGetSymbolAddress(scratch) returned null");
@@ -256,6 +289,9 @@ struct Plan {
   int resident = 0;         // resident waves of the search kernel on this device
   bool hinted = false;      // some job carries pruner hints: the hinted kernel
   int grid = 0;             // waves launched (persistent grid)
+  int variant = 0;          // configuration of the full-search kernel (kVariants)
+  const int *resident_var = nullptr;  // the device's resident waves per configuration
+  int cus = 1;              // compute units of the device
   hipEvent_t ev = nullptr;  // recorded after each eager launch: the next launch, on any stream, waits
   bool ev_recorded = false;
 };
@@ -285,12 +321,19 @@ static void plan_free(Plan *p) {
   delete p;
 }
 
+// Resident waves of the plan's kernel configuration on its device.
+static int plan_resident(const Plan *p) {
+  return p->hinted ? p->resident : p->resident_var[p->variant];
+}
+
 static int plan_create(Dev *dv, const pcme_frame_job *jobs, int njobs, Plan **out) {
   Plan *p = new Plan();
   p->dev = dv->id;
   p->njobs = njobs;
   for (int i = 0; i < njobs; i++) p->hinted |= jobs[i].hint != nullptr;
   p->resident = dv->resident_waves[p->hinted ? 1 : 0];
+  p->resident_var = dv->resident_var;
+  p->cus = dv->cus;
   std::vector<DevJob> dj((size_t)njobs);
   size_t words = 0;
   int maxh = 0;
@@ -379,9 +422,27 @@ static int plan_launch(Plan *p, hipStream_t stream) {
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(clear_kernel, dim3(512), dim3(256), 0, stream, p->ctl, p->xchg, p->nwords);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(p->hinted ? search_frames_hinted_kernel : search_frames_kernel, dim3((unsigned)p->grid),
-                     dim3(64), 0, stream, p->djobs, p->dtickets,
-                     p->ntickets, p->ctl, kSpinLimit);
+  if (p->hinted) {
+    hipLaunchKernelGGL(search_frames_hinted_kernel, dim3((unsigned)p->grid), dim3(64), 0, stream, p->djobs,
+                       p->dtickets, p->ntickets, p->ctl, kSpinLimit);
+  } else {
+    const int wpg = kVariants[p->variant].wpg;
+    const dim3 g((unsigned)((p->grid + wpg - 1) / wpg)), b((unsigned)(64 * wpg));
+    switch (p->variant) {
+      case 1: hipLaunchKernelGGL((search_frames_var_kernel<2, 12, true>), g, b, 0, stream, p->djobs, p->dtickets,
+                                 p->ntickets, p->ctl, kSpinLimit); break;
+      case 2: hipLaunchKernelGGL((search_frames_var_kernel<4, 12, true>), g, b, 0, stream, p->djobs, p->dtickets,
+                                 p->ntickets, p->ctl, kSpinLimit); break;
+      case 3: hipLaunchKernelGGL((search_frames_var_kernel<1, 8, true>), g, b, 0, stream, p->djobs, p->dtickets,
+                                 p->ntickets, p->ctl, kSpinLimit); break;
+      case 4: hipLaunchKernelGGL((search_frames_var_kernel<2, 8, true>), g, b, 0, stream, p->djobs, p->dtickets,
+                                 p->ntickets, p->ctl, kSpinLimit); break;
+      case 5: hipLaunchKernelGGL((search_frames_var_kernel<1, 12, false>), g, b, 0, stream, p->djobs, p->dtickets,
+                                 p->ntickets, p->ctl, kSpinLimit); break;
+      default: hipLaunchKernelGGL(search_frames_kernel, g, b, 0, stream, p->djobs, p->dtickets, p->ntickets, p->ctl,
+                                  kSpinLimit); break;
+    }
+  }
   HIPCHK(hipGetLastError());
   if (eager) {
     HIPCHK(hipEventRecord(p->ev, stream));
@@ -667,9 +728,39 @@ int pcme_plan_check(pcme_plan *plan, void *stream) {
 int pcme_plan_set_grid(pcme_plan *plan, int32_t waves) {
   if (!plan || waves < 0) return fail(PCME_EINVAL, "pcme_plan_set_grid: bad arguments");
   Plan *p = (Plan *)plan;
-  int w = waves == 0 ? p->resident : waves;
+  int w = waves == 0 ? plan_resident(p) : waves;
   p->grid = std::max(1, std::min(w, p->ntickets));
   return p->grid;
+}
+
+int pcme_plan_get_config(pcme_plan *plan, pcme_tune *out) {
+  if (!plan || !out) return fail(PCME_EINVAL, "pcme_plan_get_config: bad arguments");
+  const Plan *p = (const Plan *)plan;
+  const Variant &v = kVariants[p->variant];
+  out->waves = p->grid;
+  out->waves_per_group = v.wpg;
+  out->waves_per_cu = p->hinted ? std::max(1, p->resident / std::max(1, p->cus)) : v.wcu;
+  out->reuse_regions = v.reuse ? 1 : 0;
+  out->ms = 0.f;
+  return PCME_OK;
+}
+
+int pcme_plan_set_config(pcme_plan *plan, const pcme_tune *cfg) {
+  if (!plan || !cfg || cfg->waves < 0) return fail(PCME_EINVAL, "pcme_plan_set_config: bad arguments");
+  Plan *p = (Plan *)plan;
+  int v = -1;
+  for (int i = 0; i < kNumVariants; i++)
+    if (kVariants[i].wpg == cfg->waves_per_group && kVariants[i].wcu == cfg->waves_per_cu &&
+        kVariants[i].reuse == (cfg->reuse_regions != 0))
+      v = i;
+  if (v < 0)
+    return fail(PCME_EINVAL, "no kernel configuration with %d waves per group, %d waves per CU, region reuse %d",
+                cfg->waves_per_group, cfg->waves_per_cu, cfg->reuse_regions);
+  if (p->hinted && v != 0) return fail(PCME_EINVAL, "a plan with pruner hints runs the default configuration only");
+  p->variant = v;
+  const int w = cfg->waves == 0 ? plan_resident(p) : cfg->waves;
+  p->grid = std::max(1, std::min(w, p->ntickets));
+  return PCME_OK;
 }
 
 int pcme_plan_autotune(pcme_plan *plan, void *stream, int32_t reps, int32_t *best_waves, float *best_ms) {
@@ -687,7 +778,7 @@ int pcme_plan_autotune(pcme_plan *plan, void *stream, int32_t reps, int32_t *bes
   float bms = 1e30f;
   int rc = PCME_OK;
   for (int f : fr) {
-    int w = std::max(1, std::min(p->ntickets, p->resident * f / 16));
+    int w = std::max(1, std::min(p->ntickets, plan_resident(p) * f / 16));
     p->grid = w;
     if ((rc = plan_launch(p, st))) break;  // warm-up
     HIPCHK(hipEventRecord(e0, st));
@@ -709,6 +800,55 @@ int pcme_plan_autotune(pcme_plan *plan, void *stream, int32_t reps, int32_t *bes
   p->grid = best;
   if (best_waves) *best_waves = best;
   if (best_ms) *best_ms = bms;
+  return rc;
+}
+
+// Times `reps` launches of the plan as configured (after one untimed).
+static int time_plan(Plan *p, hipStream_t st, int reps, hipEvent_t e0, hipEvent_t e1, float *ms) {
+  int rc;
+  if ((rc = plan_launch(p, st))) return rc;
+  HIPCHK(hipEventRecord(e0, st));
+  for (int r = 0; r < reps; r++)
+    if ((rc = plan_launch(p, st))) return rc;
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipEventSynchronize(e1));
+  if ((rc = plan_check(p, st))) return rc;
+  HIPCHK(hipEventElapsedTime(ms, e0, e1));
+  *ms /= (float)reps;
+  return PCME_OK;
+}
+
+int pcme_plan_autotune_ex(pcme_plan *plan, void *stream, int32_t reps, pcme_tune *best) {
+  if (!plan || reps < 1) return fail(PCME_EINVAL, "pcme_plan_autotune_ex: bad arguments");
+  Plan *p = (Plan *)plan;
+  hipStream_t st = (hipStream_t)stream;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const int fr[] = {16, 14, 12, 10, 8, 6};
+  int bv = p->variant, bg = p->grid, rc = PCME_OK;
+  float bms = 1e30f;
+  for (int v = 0; v < (p->hinted ? 1 : kNumVariants) && !rc; v++) {
+    p->variant = v;
+    for (int f : fr) {
+      p->grid = std::max(1, std::min(p->ntickets, plan_resident(p) * f / 16));
+      float ms = 0.f;
+      if ((rc = time_plan(p, st, reps, e0, e1, &ms))) break;
+      if (ms < bms) {
+        bms = ms;
+        bv = v;
+        bg = p->grid;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  p->variant = bv;
+  p->grid = bg;
+  if (best) {
+    pcme_plan_get_config(plan, best);
+    best->ms = bms;
+  }
   return rc;
 }
 

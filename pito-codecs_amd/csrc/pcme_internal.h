@@ -66,6 +66,9 @@ __global__ void search_frames_kernel(const DevJob *__restrict__ jobs, const int2
                                      int ntickets, uint32_t *ctl, uint32_t spin_limit);
 __global__ void search_frames_hinted_kernel(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
                                             int ntickets, uint32_t *ctl, uint32_t spin_limit);
+template <int WPG, int WCU, bool REUSE>
+__global__ void search_frames_var_kernel(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
+                                         int ntickets, uint32_t *ctl, uint32_t spin_limit);
 __global__ void search_batch_kernel(const pcme_search_job *__restrict__ jobs, int njobs,
                                     pcme_mv *out_mv, uint32_t *out_cost);
 __global__ void search_part_batch_kernel(const pcme_part_job *__restrict__ jobs, int njobs, pcme_part_out *out);

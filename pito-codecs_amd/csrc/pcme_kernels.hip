@@ -274,6 +274,12 @@ __device__ __forceinline__ void dma_region(const uint8_t *plane, int stride, int
 }
 __device__ __forceinline__ void dma_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
+// Order this wave's LDS accesses around an LDS-DMA re-stage: every LDS read
+// issued before has returned (so the DMA cannot overwrite data still being
+// read), and no LDS access moves across.  Each wave stages into its own LDS
+// area, so no other wave is involved (workgroups may hold several walkers).
+__device__ __forceinline__ void lds_fence() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
 // Stage a 16x16 block (rows `stride` apart, 4-byte aligned) into 256 bytes
 // of LDS, row r at dst + 16 r: lane r < 16 moves row r with one 16-byte
 // LDS-DMA load.  The caller waits (dma_wait) before reading it.
@@ -294,7 +300,7 @@ template <int W, int H>
 __device__ __forceinline__ void prefetch_win(const SCtx &c, int bx, int by, int bxv, int byv, int &wx0, int &wy0,
                                              uint32_t *lw) {
   win_origin(c, bx, by, bxv, byv, wx0, wy0);
-  __syncthreads();
+  lds_fence();
   dma_region<WIN_W, WIN_H>(c.pl[0], c.stride, wx0, wy0, lw);
 }
 
@@ -307,10 +313,10 @@ __device__ __forceinline__ void ensure_win(const SCtx &c, int bx, int by, int bx
   const bool ok = (X - R - wx0 >= 0) & (X + R + W + 4 - wx0 <= WIN_W) & (Y - R - wy0 >= 0) & (Y + R + H - wy0 <= WIN_H);
   if (ok) return;
   win_origin(c, bx, by, bxv, byv, wx0, wy0);
-  __syncthreads();
+  lds_fence();
   dma_region<WIN_W, WIN_H>(c.pl[0], c.stride, wx0, wy0, lw);
   dma_wait();
-  __syncthreads();
+  lds_fence();
 }
 
 // Same for the four sub-pel planes: candidates of a round lie within +-1 px
@@ -324,7 +330,7 @@ __device__ __forceinline__ void prefetch_sp(const SCtx &c, int bx, int by, int b
   const int X = bx + (bxv >> 2), Y = by + (byv >> 2);
   sx0 = clampi((X - RM) & ~3, -PCME_PAD, c.pw + PCME_PAD - SP_W);
   sy0 = clampi(Y - RM, -PCME_PAD, c.ph + PCME_PAD - SP_H);
-  __syncthreads();
+  lds_fence();
 #pragma unroll
   for (int p = 0; p < 4; p++) dma_region<SP_W, SP_H>(c.pl[p], c.stride, sx0, sy0, lsp + p * (SP_PLANE / 4));
 }
@@ -340,7 +346,7 @@ __device__ __forceinline__ void ensure_sp(const SCtx &c, int bx, int by, int bxv
   if (ok) return;
   prefetch_sp<W, H, RM>(c, bx, by, bxv, byv, sx0, sy0, lsp);
   dma_wait();
-  __syncthreads();
+  lds_fence();
 }
 
 // Staged regions of one LDS search area (full-pel window origin, sub-pel
@@ -1442,7 +1448,7 @@ __device__ __forceinline__ uint32_t load_hint(const pcme_hint *h) {
   return v;
 }
 
-template <bool HINTS>
+template <bool HINTS, bool REUSE = true>
 __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
                                             int ntickets, uint32_t *ctl, uint32_t spin_limit, uint32_t *s_lds,
                                             uint32_t *s_src) {
@@ -1550,6 +1556,10 @@ __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, con
         }
       }
       MbRes o;
+      if (!REUSE) {  // stage every search's regions afresh
+        ar[0] = no_area();
+        ar[1] = no_area();
+      }
       if (HINTS && nh >= 0)
         analyse_mb<true>(c, n, src, 16, 16 * mx, 16 * my, J.partitions, hv, nh, hpart, o, s_lds, ar);
       else
@@ -1591,6 +1601,31 @@ search_frames_hinted_kernel(const DevJob *__restrict__ jobs, const int2 *__restr
   __shared__ __attribute__((aligned(16))) uint32_t s_lds[FRAME_LDS_WORDS];
   frames_body<true>(jobs, tickets, ntickets, ctl, spin_limit, s_lds, s_lds + LDS_PER_WAVE / 4);
 }
+
+// Configurations of the full-search kernel for the autotuner: WPG row
+// walkers (waves) per workgroup, each reserving the LDS that caps a CU at
+// WCU waves, and with or without the staged regions kept along a row.
+// search_frames_kernel is <1, 12, true>.
+template <int WPG, int WCU, bool REUSE>
+__global__ void __launch_bounds__(64 * WPG) search_frames_var_kernel(const DevJob *__restrict__ jobs,
+                                                                     const int2 *__restrict__ tickets, int ntickets,
+                                                                     uint32_t *ctl, uint32_t spin_limit) {
+  constexpr int kWords = (163840 / WCU / 4) & ~3;
+  static_assert(kWords * 4 >= LDS_PER_WAVE + 512 && 163840 / (kWords * 4) == WCU, "LDS per wave for WCU waves per CU");
+  __shared__ __attribute__((aligned(16))) uint32_t s_lds[WPG * kWords];
+  uint32_t *mine = s_lds + (threadIdx.x >> 6) * kWords;
+  frames_body<false, REUSE>(jobs, tickets, ntickets, ctl, spin_limit, mine, mine + LDS_PER_WAVE / 4);
+}
+template __global__ void search_frames_var_kernel<2, 12, true>(const DevJob *__restrict__, const int2 *__restrict__,
+                                                                int, uint32_t *, uint32_t);
+template __global__ void search_frames_var_kernel<4, 12, true>(const DevJob *__restrict__, const int2 *__restrict__,
+                                                                int, uint32_t *, uint32_t);
+template __global__ void search_frames_var_kernel<1, 8, true>(const DevJob *__restrict__, const int2 *__restrict__,
+                                                               int, uint32_t *, uint32_t);
+template __global__ void search_frames_var_kernel<2, 8, true>(const DevJob *__restrict__, const int2 *__restrict__,
+                                                               int, uint32_t *, uint32_t);
+template __global__ void search_frames_var_kernel<1, 12, false>(const DevJob *__restrict__, const int2 *__restrict__,
+                                                                 int, uint32_t *, uint32_t);
 
 // -------------------------------------------------- per-call search_part
 

@@ -125,6 +125,19 @@ SKIP_OUT_DTYPE = np.dtype([("sad", "<u4"), ("thresh", "<u4"), ("mv_ok", "u1"), (
 assert SKIP_OUT_DTYPE.itemsize == 16
 
 
+class Tune(ctypes.Structure):
+    """pcme_tune: a plan's kernel configuration (see include/pcme.h)."""
+    _fields_ = [("waves", ctypes.c_int32), ("waves_per_group", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32),
+                ("reuse_regions", ctypes.c_int32), ("ms", ctypes.c_float)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# The kernel configurations libpcme builds: (waves_per_group, waves_per_cu, reuse_regions).
+CONFIGS = ((1, 12, 1), (2, 12, 1), (4, 12, 1), (1, 8, 1), (2, 8, 1), (1, 12, 0))
+
+
 class SynthCfg(ctypes.Structure):
     _fields_ = [("w", ctypes.c_int32), ("h", ctypes.c_int32),
                 ("frames", ctypes.c_int32), ("nrect", ctypes.c_int32),
@@ -193,6 +206,7 @@ EXPORTS = (
     "pcme_search_frames_noalloc", "pcme_search_batch", "pcme_pred_frame",
     "pcme_synth_clip", "pcme_plan_create", "pcme_plan_launch", "pcme_plan_check",
     "pcme_plan_set_grid", "pcme_plan_autotune", "pcme_plan_destroy", "pcme_set_spin_limit",
+    "pcme_plan_get_config", "pcme_plan_set_config", "pcme_plan_autotune_ex",
     "pcme_net_create", "pcme_net_destroy", "pcme_net_get_params", "pcme_net_set_params",
     "pcme_net_inputs", "pcme_net_forward", "pcme_net_grad", "pcme_net_train_step", "pcme_net_hints",
     "pcme_lowres", "pcme_complexity", "pcme_mbtree_analyse", "pcme_search_part_batch", "pcme_b_analyse_batch",
@@ -244,6 +258,9 @@ def load() -> ctypes.CDLL:
         "pcme_plan_set_grid": (c.c_int, [vp, c.c_int32]),
         "pcme_plan_autotune": (c.c_int, [vp, vp, c.c_int32, c.POINTER(c.c_int32), c.POINTER(c.c_float)]),
         "pcme_plan_destroy": (None, [vp]),
+        "pcme_plan_get_config": (c.c_int, [vp, c.POINTER(Tune)]),
+        "pcme_plan_set_config": (c.c_int, [vp, c.POINTER(Tune)]),
+        "pcme_plan_autotune_ex": (c.c_int, [vp, vp, c.c_int32, c.POINTER(Tune)]),
         "pcme_set_spin_limit": (c.c_uint32, [c.c_uint32]),
         "pcme_lowres": (c.c_int, [vp, c.c_int64, c.c_int32, c.c_int32, c.c_int32, c.c_int32, vp, c.c_int64,
                                   c.c_int32, vp]),
@@ -460,6 +477,24 @@ class Plan:
         _check(load().pcme_plan_autotune(self.handle, _stream_ptr(stream), int(reps), ctypes.byref(w),
                                          ctypes.byref(ms)))
         return int(w.value), float(ms.value)
+
+    def get_config(self) -> Tune:
+        t = Tune()
+        _check(load().pcme_plan_get_config(self.handle, ctypes.byref(t)))
+        return t
+
+    def set_config(self, waves: int = 0, waves_per_group: int = 1, waves_per_cu: int = 12,
+                   reuse_regions: int = 1) -> Tune:
+        t = Tune(int(waves), int(waves_per_group), int(waves_per_cu), int(reuse_regions), 0.0)
+        _check(load().pcme_plan_set_config(self.handle, ctypes.byref(t)))
+        return self.get_config()
+
+    def autotune_ex(self, reps: int = 3, stream=None) -> Tune:
+        """Every built kernel configuration over the grid sizes; keeps and
+        returns the fastest."""
+        t = Tune()
+        _check(load().pcme_plan_autotune_ex(self.handle, _stream_ptr(stream), int(reps), ctypes.byref(t)))
+        return t
 
     def close(self) -> None:
         if self.handle:

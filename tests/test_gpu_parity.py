@@ -384,6 +384,43 @@ def test_autotune_keeps_results(gpu, oracle):
     plan.close()
 
 
+def test_every_kernel_configuration_gives_the_same_field(gpu, oracle):
+    """Each built configuration (row walkers per workgroup, occupancy cap,
+    LDS region reuse) searches two 640x368 P frames bit-exact with the
+    oracle, at its resident grid and at a small one; autotune_ex then keeps
+    a built configuration and its field is exact too."""
+    import torch
+    w, h, n = 640, 360, 3
+    y, _, _ = _clip(gpu, w, h, n)
+    mbw, mbh = gpu.mb_dims(w, h)
+    dc = gpu.DeviceClip(y, mbw, mbh)
+    dc.pad()
+    dc.hpel(0, n - 1)
+    outs = torch.zeros((n - 1, mbw * mbh * gpu.MB_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+    plan = gpu.Plan(_jobs_for(gpu, dc, outs, 32, 1, True, 27))
+    want = oracle.search_clip_me(y, 32, 1, True, 27)
+    for wpg, wcu, reuse in gpu.CONFIGS:
+        for waves in (0, 9):
+            say("configuration", wpg, wcu, reuse, "waves", waves)
+            t = plan.set_config(waves, wpg, wcu, reuse)
+            assert (t.waves_per_group, t.waves_per_cu, t.reuse_regions) == (wpg, wcu, reuse)
+            assert t.waves >= 1 and (waves == 0 or t.waves == waves)
+            outs.zero_()
+            plan.launch()
+            plan.check()
+            assert_same_field(_field(gpu, outs, n - 1, mbh, mbw), want, f"config {wpg}/{wcu}/{reuse} waves {waves}")
+    with pytest.raises(gpu.PcmeError):
+        plan.set_config(0, 3, 12, 1)
+    say("autotune_ex")
+    best = plan.autotune_ex(reps=1)
+    assert (best.waves_per_group, best.waves_per_cu, best.reuse_regions) in gpu.CONFIGS and best.ms > 0
+    outs.zero_()
+    plan.launch()
+    plan.check()
+    assert_same_field(_field(gpu, outs, n - 1, mbh, mbw), want, "autotuned")
+    plan.close()
+
+
 def test_stall_path_reports_estall(gpu):
     """With the dependency-wait bound forced to one poll, rows that must wait
     give up, raise the flag, every wave exits, and the call fails with
