@@ -1,7 +1,8 @@
-// h264dec.cpp -- a minimal H.264 decoder on the host (libpcme_dec.so): the
-// Baseline-profile subset an encoder of this project emits or checks against
-// (include/pcme_dec.h lists it).  Its job is to show that a bitstream
-// decodes bit-exact; it is plain scalar C++ and makes no claim to speed.
+// h264.cpp -- the host-side H.264 codec of libpcme_h264.so: a minimal
+// decoder (include/pcme_dec.h), the Baseline-profile subset that shows a
+// bitstream decodes bit-exact, and the field coder (include/pcme_enc.h) that
+// codes a motion field into such a bitstream with the decoder's own
+// reconstruction.  Plain scalar C++: no claim to speed.
 //
 // The steps follow ITU-T H.264 (clause numbers below) in the structure of
 // the reference's decoder:
@@ -15,6 +16,7 @@
 //   deblocking                            8.7        (dec/deblock.rs)
 //   reference marking, list init/modification 8.2.4, 8.2.5 (dec/picture.rs)
 #include "../../include/pcme_dec.h"
+#include "../../include/pcme_enc.h"
 
 #include <stdarg.h>
 #include <stdio.h>
@@ -1151,6 +1153,40 @@ void luma_residual(Dec *d, int mx, int my, int bx, int by, const int *zz, int qp
   add4(&P.y[(size_t)(16 * my + 4 * by) * P.w + 16 * mx + 4 * bx], P.w, r);
 }
 
+// An inter MB's prediction (all 16 4x4 blocks, luma and chroma) plus its
+// luma residual (levels in zigzag order per block, blocks of the 8x8s whose
+// cbp bit is set).
+void recon_inter(Dec *d, int mx, int my, const Mb &m, const int (*acl)[16], int cbp_l, int qp) {
+  for (int blk = 0; blk < 16; blk++) {
+    const int bx = blk & 3, by = blk >> 2, b8 = (by >> 1) * 2 + (bx >> 1);
+    mc_block(d, *d->list0[m.ref[b8]], mx, my, bx, by, m.mv[blk][0], m.mv[blk][1]);
+  }
+  for (int blk = 0; blk < 16; blk++) {
+    const int bx = blk_x(blk), by = blk_y(blk);
+    if ((cbp_l >> (blk >> 2)) & 1) luma_residual(d, mx, my, bx, by, acl[blk], qp, false, 0);
+  }
+}
+
+// 8.5.11: the chroma residual of one component (DC levels dc[4], AC levels
+// ac[4][1..15] in zigzag order) added to the prediction in pl.
+void chroma_residual(Dec *d, int mx, int my, int qpc, const int *dc, const int (*ac)[16], std::vector<uint8_t> &pl) {
+  const int c0 = dc[0], c1 = dc[1], c2 = dc[2], c3 = dc[3];
+  const int f[4] = {c0 + c1 + c2 + c3, c0 - c1 + c2 - c3, c0 + c1 - c2 - c3, c0 - c1 - c2 + c3};
+  const int ls = 16 * kNorm4[qpc % 6][0];
+  const int cw = d->cur->w / 2;
+  for (int i = 0; i < 4; i++) {
+    int cc[16], r[16];
+    for (int k = 0; k < 16; k++) cc[k] = 0;
+    for (int k = 1; k < 16; k++) {
+      const int pos = kZigzag4[k];
+      cc[pos] = ac[i][k] ? scale4(ac[i][k], qpc, pos) : 0;
+    }
+    cc[0] = ((f[i] * ls) << (qpc / 6)) >> 5;
+    idct4(cc, r);
+    add4(&pl[(size_t)(8 * my + 4 * (i >> 1)) * cw + 8 * mx + 4 * (i & 1)], cw, r);
+  }
+}
+
 int decode_mb(Dec *d, Bits &b, int addr, int slice_type, int &qp) {
   const int mx = addr % d->mbw, my = addr / d->mbw;
   Mb &m = d->mbs[addr];
@@ -1370,14 +1406,7 @@ int decode_mb(Dec *d, Bits &b, int addr, int slice_type, int &qp) {
 
   // luma prediction + residual
   if (m.kind == kPInter) {
-    for (int blk = 0; blk < 16; blk++) {
-      const int bx = blk & 3, by = blk >> 2, b8 = (by >> 1) * 2 + (bx >> 1);
-      mc_block(d, *d->list0[m.ref[b8]], mx, my, bx, by, m.mv[blk][0], m.mv[blk][1]);
-    }
-    for (int blk = 0; blk < 16; blk++) {
-      const int bx = blk_x(blk), by = blk_y(blk);
-      if ((cbp_l >> (blk >> 2)) & 1) luma_residual(d, mx, my, bx, by, acl[blk], qp, false, 0);
-    }
+    recon_inter(d, mx, my, m, acl, cbp_l, qp);
   } else if (m.kind == kI4x4) {
     uint32_t dn = 0;
     for (int blk = 0; blk < 16; blk++) {
@@ -1421,24 +1450,7 @@ int decode_mb(Dec *d, Bits &b, int addr, int slice_type, int &qp) {
       int rc = pred_chroma(d, mx, my, m, chroma_mode, pl);
       if (rc) return rc;
     }
-    if (!cbp_c) continue;
-    const int qpc = m.qpc;
-    // 8.5.11: 2x2 chroma DC
-    const int c0 = cdc[c][0], c1 = cdc[c][1], c2 = cdc[c][2], c3 = cdc[c][3];
-    const int f[4] = {c0 + c1 + c2 + c3, c0 - c1 + c2 - c3, c0 + c1 - c2 - c3, c0 - c1 - c2 + c3};
-    const int ls = 16 * kNorm4[qpc % 6][0];
-    for (int i = 0; i < 4; i++) {
-      int cc[16], r[16];
-      for (int k = 0; k < 16; k++) cc[k] = 0;
-      for (int k = 1; k < 16; k++) {
-        const int pos = kZigzag4[k];
-        cc[pos] = cac[c][i][k] ? scale4(cac[c][i][k], qpc, pos) : 0;
-      }
-      cc[0] = ((f[i] * ls) << (qpc / 6)) >> 5;
-      idct4(cc, r);
-      const int cw = P.w / 2;
-      add4(&pl[(size_t)(8 * my + 4 * (i >> 1)) * cw + 8 * mx + 4 * (i & 1)], cw, r);
-    }
+    if (cbp_c) chroma_residual(d, mx, my, m.qpc, cdc[c], cac[c], pl);
   }
   return PCME_OK;
 }
@@ -1838,5 +1850,577 @@ int pcme_h264_mc_luma(const uint8_t *plane, int32_t stride, int32_t w, int32_t h
 }
 
 const char *pcme_h264dec_last_error(void) { return g_err; }
+
+}  // extern "C"
+
+// ================================================================ encoder
+//
+// The field coder of include/pcme_enc.h.  It writes the syntax the decoder
+// above reads and reconstructs every MB with the decoder's own functions
+// (mc_block, luma_residual, chroma_residual, deblock_picture), so its
+// reconstruction is the one any decoder of the stream produces.
+
+namespace {
+
+class BitWriter {
+ public:
+  void u(int k, uint32_t v) {
+    for (int i = k - 1; i >= 0; i--) bit((v >> i) & 1u);
+  }
+  void ue(uint32_t v) {
+    const uint32_t x = v + 1;
+    int n = 0;
+    while ((x >> (n + 1)) != 0) n++;
+    u(n, 0);
+    u(n + 1, x);
+  }
+  void se(int32_t v) { ue(v > 0 ? (uint32_t)(2 * v - 1) : (uint32_t)(-2 * v)); }
+  void bit(uint32_t b) {
+    cur_ = (uint8_t)((cur_ << 1) | b);
+    if (++n_ == 8) {
+      buf_.push_back(cur_);
+      cur_ = 0;
+      n_ = 0;
+    }
+  }
+  bool aligned() const { return n_ == 0; }
+  void align_zero() {
+    while (n_) bit(0);
+  }
+  void trailing() {  // rbsp_trailing_bits
+    bit(1);
+    align_zero();
+  }
+  const std::vector<uint8_t> &bytes() const { return buf_; }
+
+ private:
+  std::vector<uint8_t> buf_;
+  uint8_t cur_ = 0;
+  int n_ = 0;
+};
+
+// A NAL unit: start code, header, the RBSP with emulation prevention (7.4.1).
+void put_nal(std::vector<uint8_t> &out, int ref_idc, int type, const std::vector<uint8_t> &rbsp) {
+  out.insert(out.end(), {0, 0, 0, 1, (uint8_t)((ref_idc << 5) | type)});
+  int zeros = 0;
+  for (uint8_t b : rbsp) {
+    if (zeros >= 2 && b <= 3) {
+      out.push_back(3);
+      zeros = 0;
+    }
+    out.push_back(b);
+    zeros = b == 0 ? zeros + 1 : 0;
+  }
+}
+
+// QUANT4 (the multipliers that invert normAdjust4x4): by position class
+const int kQuant4[6][3] = {{13107, 5243, 8066}, {11916, 4660, 7490}, {10082, 4194, 6554},
+                           {9362, 3647, 5825},  {8192, 3355, 5243},  {7282, 2893, 4559}};
+inline int mf4(int m, int pos) {
+  const int i = pos >> 2, j = pos & 3;
+  return kQuant4[m][(i & 1) == 0 && (j & 1) == 0 ? 0 : ((i & 1) && (j & 1) ? 1 : 2)];
+}
+
+// Forward 4x4 core transform (the inverse of 8.5.12's up to scaling).
+void fdct4(const int *d, int *o) {
+  int t[16];
+  for (int i = 0; i < 4; i++) {
+    const int *s = d + 4 * i;
+    const int s03 = s[0] + s[3], d03 = s[0] - s[3], s12 = s[1] + s[2], d12 = s[1] - s[2];
+    t[4 * i] = s03 + s12;
+    t[4 * i + 1] = 2 * d03 + d12;
+    t[4 * i + 2] = s03 - s12;
+    t[4 * i + 3] = d03 - 2 * d12;
+  }
+  for (int j = 0; j < 4; j++) {
+    const int s03 = t[j] + t[12 + j], d03 = t[j] - t[12 + j], s12 = t[4 + j] + t[8 + j], d12 = t[4 + j] - t[8 + j];
+    o[j] = s03 + s12;
+    o[4 + j] = 2 * d03 + d12;
+    o[8 + j] = s03 - s12;
+    o[12 + j] = d03 - 2 * d12;
+  }
+}
+
+// Dead-zone quantiser: |level| = (|w| * mf + f) >> qbits, f = 2^qbits / 6.
+inline int quant(int w, int mf, int qbits) {
+  const int64_t f = ((int64_t)1 << qbits) / 6;
+  const int a = (int)(((int64_t)(w < 0 ? -w : w) * mf + f) >> qbits);
+  return w < 0 ? -a : a;
+}
+
+// residual_block_cavlc (the writer of 9.2): lv[0..n-1] in scan order.
+void write_block(BitWriter &bw, int nc, const int *lv, int n) {
+  int pos[16], tc = 0;
+  for (int k = 0; k < n; k++)
+    if (lv[k]) pos[tc++] = k;
+  int t1 = 0;
+  for (int i = tc - 1; i >= 0 && t1 < 3 && (lv[pos[i]] == 1 || lv[pos[i]] == -1); i--) t1++;
+  if (nc >= 8) {
+    bw.u(6, tc == 0 ? 3u : (uint32_t)(((tc - 1) << 2) | t1));
+  } else {
+    const int cls = nc == -1 ? 3 : (nc < 2 ? 0 : (nc < 4 ? 1 : 2));
+    for (int i = 0; i < kCoeffTokenN[cls]; i++) {
+      const Vlc4 &e = kCoeffToken[cls][i];
+      if (e.tc == tc && e.t1 == t1) {
+        bw.u(e.len, e.val);
+        break;
+      }
+    }
+  }
+  if (tc == 0) return;
+  // levels from the highest frequency down
+  int suffix_len = (tc > 10 && t1 < 3) ? 1 : 0;
+  for (int i = 0; i < tc; i++) {
+    const int level = lv[pos[tc - 1 - i]];
+    if (i < t1) {
+      bw.u(1, level < 0 ? 1u : 0u);
+      continue;
+    }
+    int code = level > 0 ? 2 * level - 2 : -2 * level - 1;
+    if (i == t1 && t1 < 3) code -= 2;
+    int prefix, ssize = 0, suffix = 0;
+    if (suffix_len == 0) {
+      if (code < 14) prefix = code;
+      else if (code < 30) { prefix = 14; ssize = 4; suffix = code - 14; }
+      else { prefix = 15; ssize = 12; suffix = code - 30; }
+    } else if (code < (15 << suffix_len)) {
+      prefix = code >> suffix_len;
+      ssize = suffix_len;
+      suffix = code & ((1 << suffix_len) - 1);
+    } else {
+      prefix = 15;
+      ssize = 12;
+      suffix = code - (15 << suffix_len);
+    }
+    bw.u(prefix, 0);
+    bw.u(1, 1);
+    if (ssize) bw.u(ssize, (uint32_t)suffix);
+    if (suffix_len == 0) suffix_len = 1;
+    if (std::abs(level) > (3 << (suffix_len - 1)) && suffix_len < 6) suffix_len++;
+  }
+  int zeros = pos[tc - 1] + 1 - tc;
+  if (tc < n) {
+    const Vlc2 &e = n == 4 ? kTotalZerosDc[tc - 1][zeros] : kTotalZeros4[tc - 1][zeros];
+    bw.u(e.len, e.val);
+  }
+  for (int i = tc - 1; i > 0 && zeros > 0; i--) {
+    const int run = pos[i] - pos[i - 1] - 1;
+    const Vlc2 &e = kRunBefore[std::min(zeros, 7) - 1][run];
+    bw.u(e.len, e.val);
+    zeros -= run;
+  }
+}
+
+}  // namespace
+
+struct pcme_h264enc {
+  int w = 0, h = 0, mbw = 0, mbh = 0;
+  Sps sps;
+  Pps pps;
+  pcme_h264dec d;                 // picture under construction, MB records, reference list
+  std::unique_ptr<Picture> ref;   // the previous reconstruction
+  int frame_num = 0;
+  bool started = false;
+  int n_skip = 0, n_inter = 0, n_pcm = 0;
+  std::vector<uint8_t> sy, su, sv;  // the coded-size source (edges replicated)
+};
+
+namespace {
+
+using Enc = pcme_h264enc;
+
+void load_source(Enc *e, const uint8_t *y, int ys, const uint8_t *u, const uint8_t *v, int cs) {
+  const int W = 16 * e->mbw, H = 16 * e->mbh;
+  e->sy.resize((size_t)W * H);
+  e->su.resize((size_t)W * H / 4);
+  e->sv.resize((size_t)W * H / 4);
+  for (int r = 0; r < H; r++)
+    for (int c = 0; c < W; c++) e->sy[(size_t)r * W + c] = y[(size_t)std::min(r, e->h - 1) * ys + std::min(c, e->w - 1)];
+  for (int r = 0; r < H / 2; r++)
+    for (int c = 0; c < W / 2; c++) {
+      const size_t o = (size_t)std::min(r, e->h / 2 - 1) * cs + std::min(c, e->w / 2 - 1);
+      e->su[(size_t)r * (W / 2) + c] = u[o];
+      e->sv[(size_t)r * (W / 2) + c] = v[o];
+    }
+}
+
+void write_sps_pps(Enc *e, std::vector<uint8_t> &out) {
+  BitWriter b;
+  b.u(8, 66);    // profile_idc: Baseline
+  b.u(8, 0xC0);  // constraint_set0_flag, constraint_set1_flag
+  b.u(8, 51);    // level_idc
+  b.ue(0);       // seq_parameter_set_id
+  b.ue(12);      // log2_max_frame_num_minus4: 16 bits
+  b.ue(2);       // pic_order_cnt_type
+  b.ue(1);       // max_num_ref_frames
+  b.u(1, 0);     // gaps_in_frame_num_value_allowed_flag
+  b.ue((uint32_t)e->mbw - 1);
+  b.ue((uint32_t)e->mbh - 1);
+  b.u(1, 1);  // frame_mbs_only_flag
+  b.u(1, 1);  // direct_8x8_inference_flag
+  const int cr = (16 * e->mbw - e->w) / 2, cb = (16 * e->mbh - e->h) / 2;
+  b.u(1, (cr || cb) ? 1u : 0u);
+  if (cr || cb) {
+    b.ue(0);
+    b.ue((uint32_t)cr);
+    b.ue(0);
+    b.ue((uint32_t)cb);
+  }
+  b.u(1, 0);  // vui_parameters_present_flag
+  b.trailing();
+  put_nal(out, 3, 7, b.bytes());
+  BitWriter p;
+  p.ue(0);    // pic_parameter_set_id
+  p.ue(0);    // seq_parameter_set_id
+  p.u(1, 0);  // entropy_coding_mode_flag: CAVLC
+  p.u(1, 0);  // bottom_field_pic_order_in_frame_present_flag
+  p.ue(0);    // num_slice_groups_minus1
+  p.ue(0);    // num_ref_idx_l0_default_active_minus1
+  p.ue(0);    // num_ref_idx_l1_default_active_minus1
+  p.u(1, 0);  // weighted_pred_flag
+  p.u(2, 0);  // weighted_bipred_idc
+  p.se(0);    // pic_init_qp_minus26
+  p.se(0);    // pic_init_qs_minus26
+  p.se(0);    // chroma_qp_index_offset
+  p.u(1, 0);  // deblocking_filter_control_present_flag
+  p.u(1, 0);  // constrained_intra_pred_flag
+  p.u(1, 0);  // redundant_pic_cnt_present_flag
+  p.trailing();
+  put_nal(out, 3, 8, p.bytes());
+}
+
+void begin_picture(Enc *e) {
+  pcme_h264dec &d = e->d;
+  d.csps = &e->sps;
+  d.cpps = &e->pps;
+  d.mbw = e->mbw;
+  d.mbh = e->mbh;
+  d.cur.reset(new Picture());
+  Picture &p = *d.cur;
+  p.id = d.next_id++;
+  p.w = 16 * e->mbw;
+  p.h = 16 * e->mbh;
+  p.y.assign((size_t)p.w * p.h, 0);
+  p.u.assign((size_t)p.w * p.h / 4, 0);
+  p.v.assign((size_t)p.w * p.h / 4, 0);
+  p.frame_num = e->frame_num;
+  for (int i = 0; i < 4; i++) p.crop[i] = e->sps.crop[i];
+  d.mbs.assign((size_t)e->mbw * e->mbh, Mb());
+  d.slices.assign(1, SliceInfo());
+  d.list0.clear();
+  if (e->ref) d.list0.push_back(e->ref.get());
+  e->n_skip = e->n_inter = e->n_pcm = 0;
+}
+
+void end_picture(Enc *e) {
+  deblock_picture(&e->d);
+  e->ref = std::move(e->d.cur);
+  e->frame_num = (e->frame_num + 1) & 0xFFFF;
+}
+
+int code_idr(Enc *e, int qp, BitWriter &b) {
+  b.ue(0);       // first_mb_in_slice
+  b.ue(7);       // slice_type: I (all slices)
+  b.ue(0);       // pic_parameter_set_id
+  b.u(16, 0);    // frame_num
+  b.ue(0);       // idr_pic_id
+  b.u(1, 0);     // no_output_of_prior_pics_flag
+  b.u(1, 0);     // long_term_reference_flag
+  b.se(qp - 26);  // slice_qp_delta
+  Picture &P = *e->d.cur;
+  for (int addr = 0; addr < e->mbw * e->mbh; addr++) {
+    const int mx = addr % e->mbw, my = addr / e->mbw;
+    b.ue(25);  // I_PCM
+    b.align_zero();
+    Mb &m = e->d.mbs[addr];
+    m.kind = kIPCM;
+    m.slice = 0;
+    m.qp = 0;
+    m.qpc = kQpc[0];
+    for (int i = 0; i < 16; i++) {
+      m.tc[i] = 16;
+      m.i4[i] = 2;
+      m.mv[i][0] = m.mv[i][1] = m.mvp[i][0] = m.mvp[i][1] = 0;
+    }
+    for (int i = 0; i < 4; i++) {
+      m.tcc[0][i] = m.tcc[1][i] = 16;
+      m.ref[i] = -1;
+      m.refpic[i] = -1;
+    }
+    for (int r = 0; r < 16; r++)
+      for (int c = 0; c < 16; c++) {
+        const uint8_t s = e->sy[(size_t)(16 * my + r) * P.w + 16 * mx + c];
+        b.u(8, s);
+        P.y[(size_t)(16 * my + r) * P.w + 16 * mx + c] = s;
+      }
+    for (int k = 0; k < 2; k++)
+      for (int r = 0; r < 8; r++)
+        for (int c = 0; c < 8; c++) {
+          const size_t o = (size_t)(8 * my + r) * (P.w / 2) + 8 * mx + c;
+          const uint8_t s = (k ? e->sv : e->su)[o];
+          b.u(8, s);
+          (k ? P.v : P.u)[o] = s;
+        }
+    e->n_pcm++;
+  }
+  b.trailing();
+  return PCME_OK;
+}
+
+int code_p(Enc *e, const pcme_mb_out *field, int qp, BitWriter &b) {
+  b.ue(0);        // first_mb_in_slice
+  b.ue(5);        // slice_type: P (all slices)
+  b.ue(0);        // pic_parameter_set_id
+  b.u(16, (uint32_t)e->frame_num);
+  b.u(1, 0);      // num_ref_idx_active_override_flag
+  b.u(1, 0);      // ref_pic_list_modification_flag_l0
+  b.u(1, 0);      // adaptive_ref_pic_marking_mode_flag
+  b.se(qp - 26);  // slice_qp_delta
+  pcme_h264dec *d = &e->d;
+  Picture &P = *d->cur;
+  const int W = P.w, CW = W / 2;
+  const int qpc = kQpc[clip3(0, 51, qp)];
+  const int ref_id = e->ref->id;
+  int skip_run = 0;
+  for (int addr = 0; addr < e->mbw * e->mbh; addr++) {
+    const int mx = addr % e->mbw, my = addr / e->mbw;
+    const pcme_mb_out &f = field[addr];
+    Mb &m = d->mbs[addr];
+    m.slice = 0;
+    m.kind = kPInter;
+    m.part = (int8_t)f.part;
+    m.qp = qp;
+    m.qpc = qpc;
+    for (int i = 0; i < 4; i++) {
+      m.ref[i] = 0;
+      m.refpic[i] = ref_id;
+      m.sub[i] = f.part == 3 ? 0 : -1;
+      m.tcc[0][i] = m.tcc[1][i] = 0;
+    }
+    for (int i = 0; i < 16; i++) {
+      m.i4[i] = 2;
+      m.tc[i] = 0;
+      m.mv[i][0] = f.mv[i].x;
+      m.mv[i][1] = f.mv[i].y;
+    }
+    // the partitions (x, y, w, h in samples) and their predictors in decoding order
+    static const int kParts[4][4][4] = {{{0, 0, 16, 16}},
+                                        {{0, 0, 16, 8}, {0, 8, 16, 8}},
+                                        {{0, 0, 8, 16}, {8, 0, 8, 16}},
+                                        {{0, 0, 8, 8}, {8, 0, 8, 8}, {0, 8, 8, 8}, {8, 8, 8, 8}}};
+    const int np = f.part == 0 ? 1 : (f.part == 3 ? 4 : 2);
+    int mvd[4][2];
+    uint32_t done = 0;
+    for (int p = 0; p < np; p++) {
+      const int *q = kParts[f.part][p];
+      const int blk = (q[1] / 4) * 4 + q[0] / 4;
+      int px, py;
+      mv_pred(d, mx, my, m, done, q[0], q[1], q[2], q[3], 0, f.part == 3 ? 0 : f.part, p, px, py);
+      mvd[p][0] = m.mv[blk][0] - px;
+      mvd[p][1] = m.mv[blk][1] - py;
+      for (int by = q[1] / 4; by < (q[1] + q[3]) / 4; by++)
+        for (int bx = q[0] / 4; bx < (q[0] + q[2]) / 4; bx++) {
+          if (m.mv[by * 4 + bx][0] != m.mv[blk][0] || m.mv[by * 4 + bx][1] != m.mv[blk][1])
+            return fail("MB %d: the field's MVs differ inside partition %d", addr, p);
+          m.mvp[by * 4 + bx][0] = (int16_t)px;
+          m.mvp[by * 4 + bx][1] = (int16_t)py;
+          done |= 1u << (by * 4 + bx);
+        }
+    }
+    // prediction (into the picture), then the residual levels
+    for (int blk = 0; blk < 16; blk++)
+      mc_block(d, *e->ref, mx, my, blk & 3, blk >> 2, m.mv[blk][0], m.mv[blk][1]);
+    int acl[16][16], cdc[2][4], cac[2][4][16];
+    int cbp_l = 0;
+    const int qbits = 15 + qp / 6;
+    for (int blk = 0; blk < 16; blk++) {
+      const int bx = blk_x(blk), by = blk_y(blk);
+      int r[16], w[16];
+      for (int y = 0; y < 4; y++)
+        for (int x = 0; x < 4; x++) {
+          const size_t o = (size_t)(16 * my + 4 * by + y) * W + 16 * mx + 4 * bx + x;
+          r[y * 4 + x] = (int)e->sy[o] - (int)P.y[o];
+        }
+      fdct4(r, w);
+      int nz = 0;
+      for (int k = 0; k < 16; k++) {
+        const int pos = kZigzag4[k];
+        acl[blk][k] = quant(w[pos], mf4(qp % 6, pos), qbits);
+        nz |= acl[blk][k] != 0;
+      }
+      if (nz) cbp_l |= 1 << (blk >> 2);
+    }
+    int cbp_c = 0;
+    const int qbc = 15 + qpc / 6;
+    for (int c = 0; c < 2; c++) {
+      const std::vector<uint8_t> &src = c ? e->sv : e->su;
+      const std::vector<uint8_t> &pr = c ? P.v : P.u;
+      int dc[4];
+      for (int i = 0; i < 4; i++) {
+        int r[16], w[16];
+        for (int y = 0; y < 4; y++)
+          for (int x = 0; x < 4; x++) {
+            const size_t o = (size_t)(8 * my + 4 * (i >> 1) + y) * CW + 8 * mx + 4 * (i & 1) + x;
+            r[y * 4 + x] = (int)src[o] - (int)pr[o];
+          }
+        fdct4(r, w);
+        dc[i] = w[0];
+        cac[c][i][0] = 0;
+        for (int k = 1; k < 16; k++) {
+          const int pos = kZigzag4[k];
+          cac[c][i][k] = quant(w[pos], mf4(qpc % 6, pos), qbc);
+          if (cac[c][i][k]) cbp_c = 2;
+        }
+      }
+      const int hd[4] = {dc[0] + dc[1] + dc[2] + dc[3], dc[0] - dc[1] + dc[2] - dc[3], dc[0] + dc[1] - dc[2] - dc[3],
+                         dc[0] - dc[1] - dc[2] + dc[3]};
+      for (int i = 0; i < 4; i++) {
+        cdc[c][i] = quant(hd[i], mf4(qpc % 6, 0), qbc + 1);
+        if (cdc[c][i] && cbp_c == 0) cbp_c = 1;
+      }
+    }
+    // P_Skip: the 16x16 MV is the skip MV and nothing is coded
+    if (f.part == 0 && cbp_l == 0 && cbp_c == 0) {
+      int sx, sy;
+      pskip_mv(d, mx, my, m, sx, sy);
+      if (sx == m.mv[0][0] && sy == m.mv[0][1]) {
+        m.kind = kPSkip;
+        m.part = -1;
+        skip_run++;
+        e->n_skip++;
+        continue;  // the prediction is the reconstruction
+      }
+    }
+    b.ue((uint32_t)skip_run);
+    skip_run = 0;
+    b.ue((uint32_t)f.part);  // P_L0_16x16, P_L0_L0_16x8, P_L0_L0_8x16, P_8x8
+    if (f.part == 3)
+      for (int s = 0; s < 4; s++) b.ue(0);  // sub_mb_type P_L0_8x8
+    for (int p = 0; p < np; p++) {
+      b.se(mvd[p][0]);
+      b.se(mvd[p][1]);
+    }
+    const int cbp = cbp_l | (cbp_c << 4);
+    int code = 0;
+    while (kCbpInter[code] != cbp) code++;
+    b.ue((uint32_t)code);
+    if (cbp) b.se(0);  // mb_qp_delta
+    for (int b8 = 0; b8 < 4; b8++) {
+      if (!((cbp_l >> b8) & 1)) continue;
+      for (int i = 0; i < 4; i++) {
+        const int blk = b8 * 4 + i, bx = blk_x(blk), by = blk_y(blk);
+        write_block(b, coeff_nc(d, mx, my, m, -1, bx, by), acl[blk], 16);
+        int tc = 0;
+        for (int k = 0; k < 16; k++) tc += acl[blk][k] != 0;
+        m.tc[by * 4 + bx] = (uint8_t)tc;
+      }
+    }
+    if (cbp_c)
+      for (int c = 0; c < 2; c++) write_block(b, -1, cdc[c], 4);
+    if (cbp_c == 2)
+      for (int c = 0; c < 2; c++)
+        for (int i = 0; i < 4; i++) {
+          write_block(b, coeff_nc(d, mx, my, m, c, i & 1, i >> 1), cac[c][i] + 1, 15);
+          int tc = 0;
+          for (int k = 1; k < 16; k++) tc += cac[c][i][k] != 0;
+          m.tcc[c][i] = (uint8_t)tc;
+        }
+    // reconstruction: the decoder's own steps on the same levels
+    for (int blk = 0; blk < 16; blk++) {
+      const int bx = blk_x(blk), by = blk_y(blk);
+      if ((cbp_l >> (blk >> 2)) & 1) luma_residual(d, mx, my, bx, by, acl[blk], qp, false, 0);
+    }
+    if (cbp_c)
+      for (int c = 0; c < 2; c++) chroma_residual(d, mx, my, qpc, cdc[c], cac[c], c ? P.v : P.u);
+    e->n_inter++;
+  }
+  if (skip_run) b.ue((uint32_t)skip_run);
+  b.trailing();
+  return PCME_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pcme_h264enc_create(int32_t width, int32_t height, pcme_h264enc **out) {
+  if (!out || width < 16 || height < 16 || (width & 1) || (height & 1) || width > 8192 || height > 8192)
+    return fail("pcme_h264enc_create: bad arguments");
+  Enc *e = new Enc();
+  e->w = width;
+  e->h = height;
+  e->mbw = (width + 15) / 16;
+  e->mbh = (height + 15) / 16;
+  Sps &s = e->sps;
+  s.valid = true;
+  s.profile = 66;
+  s.level = 51;
+  s.log2_max_frame_num = 16;
+  s.poc_type = 2;
+  s.max_num_ref_frames = 1;
+  s.mbw = e->mbw;
+  s.mbh = e->mbh;
+  s.crop[1] = (16 * e->mbw - width) / 2;
+  s.crop[3] = (16 * e->mbh - height) / 2;
+  e->pps.valid = true;
+  *out = e;
+  return PCME_OK;
+}
+
+void pcme_h264enc_destroy(pcme_h264enc *enc) { delete enc; }
+
+int pcme_h264enc_picture(pcme_h264enc *e, const uint8_t *y, int32_t ystride, const uint8_t *u, const uint8_t *v,
+                         int32_t cstride, const pcme_mb_out *mbs, int32_t qp, uint8_t *out, size_t cap, size_t *len) {
+  if (!e || !y || !u || !v || !out || !len || ystride < e->w || cstride < e->w / 2 || qp < 0 || qp > 51)
+    return fail("pcme_h264enc_picture: bad arguments");
+  if (mbs && !e->ref) return fail("pcme_h264enc_picture: a P picture before the first IDR");
+  if (mbs)
+    for (int i = 0; i < e->mbw * e->mbh; i++) {
+      if (mbs[i].part > 3) return fail("MB %d: partition %d", i, mbs[i].part);
+      for (int k = 0; k < 16; k++)
+        if (std::abs(mbs[i].mv[k].x) > 2048 || std::abs(mbs[i].mv[k].y) > 2048)
+          return fail("MB %d: MV (%d, %d) beyond +-2048", i, mbs[i].mv[k].x, mbs[i].mv[k].y);
+    }
+  load_source(e, y, ystride, u, v, cstride);
+  std::vector<uint8_t> nals;
+  if (!mbs) {
+    if (!e->started) write_sps_pps(e, nals);
+    e->started = true;
+    e->frame_num = 0;
+  }
+  begin_picture(e);
+  BitWriter b;
+  const int rc = mbs ? code_p(e, mbs, qp, b) : code_idr(e, qp, b);
+  if (rc) {
+    e->d.cur.reset();
+    return rc;
+  }
+  end_picture(e);
+  put_nal(nals, mbs ? 2 : 3, mbs ? 1 : 5, b.bytes());
+  if (nals.size() > cap) return fail("pcme_h264enc_picture: %zu bytes do not fit in %zu", nals.size(), cap);
+  memcpy(out, nals.data(), nals.size());
+  *len = nals.size();
+  return PCME_OK;
+}
+
+int pcme_h264enc_recon(pcme_h264enc *e, uint8_t *y, int32_t ystride, uint8_t *u, uint8_t *v, int32_t cstride) {
+  if (!e || !y || !u || !v || !e->ref || ystride < e->w || cstride < e->w / 2)
+    return fail("pcme_h264enc_recon: bad arguments");
+  const Picture &p = *e->ref;
+  for (int r = 0; r < e->h; r++) memcpy(y + (size_t)r * ystride, &p.y[(size_t)r * p.w], e->w);
+  for (int r = 0; r < e->h / 2; r++) {
+    memcpy(u + (size_t)r * cstride, &p.u[(size_t)r * (p.w / 2)], e->w / 2);
+    memcpy(v + (size_t)r * cstride, &p.v[(size_t)r * (p.w / 2)], e->w / 2);
+  }
+  return PCME_OK;
+}
+
+int pcme_h264enc_stats(pcme_h264enc *e, int32_t *skipped, int32_t *inter, int32_t *pcm) {
+  if (!e) return fail("pcme_h264enc_stats: null coder");
+  if (skipped) *skipped = e->n_skip;
+  if (inter) *inter = e->n_inter;
+  if (pcm) *pcm = e->n_pcm;
+  return PCME_OK;
+}
 
 }  // extern "C"

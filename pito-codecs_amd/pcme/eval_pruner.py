@@ -1,16 +1,23 @@
-"""BD-rate of the learned search pruner under the P-frame coding model of
-pcme.rdmodel, on a held-out 1080p clip at QP 22, 27, 32 and 37.
+"""BD-rate of the learned search pruner on a held-out clip (1080p by
+default) at QP 22, 27, 32 and 37.
 
-    python -m pcme.eval_pruner [--frames 16] [--seed 0x5EED9200] [--out report.json]
+    python -m pcme.eval_pruner [--frames 16] [--seed 0x5EED9200] [--coder h264|model] [--out report.json]
 
 For each QP the same frames are searched three ways: the plain search, the
 pruned search with that QP's network (5 candidates; full search where
-the partition head's confidence is below 0.7) and the same pruned search with empty hints (the
-method without the network).  Every field is then coded by the model --
-motion-compensated prediction of its chosen partitions (pcme_pred_frame),
-luma residual through the H.264 4x4 transform and quantiser, run/level and
-MV-difference bits -- and the (rate, PSNR) points of each method give its
-BD-rate against the plain search.
+the partition head's confidence is below 0.7) and the same pruned search
+with empty hints (the method without the network).  Every field is then
+coded, and the (rate, PSNR) points of each method give its BD-rate against
+the plain search:
+
+* coder "h264" (the default): the field coder (pcme.h264enc) writes a
+  Baseline CAVLC H.264 stream -- an I_PCM IDR, then every P picture with the
+  field's partitions and MVs against the previous reconstruction, 4x4
+  residual, P_Skip -- and the rate is the P pictures' bits, the distortion
+  their mean luma PSNR; every stream is decoded by the minimal decoder
+  (pcme.h264dec) and must reproduce the coder's reconstruction bit for bit;
+* coder "model": the P-frame coding model of pcme.rdmodel (luma only,
+  bits estimated from run/level and MV-difference Exp-Golomb lengths).
 """
 from __future__ import annotations
 
@@ -58,8 +65,21 @@ def rd_point(dc: DeviceClip, field: np.ndarray, qp: int) -> tuple[float, float]:
     return bits, R.psnr(sse, npix)
 
 
+def rd_point_h264(y, u, v, field: np.ndarray, qp: int) -> tuple[float, float]:
+    """(bits, PSNR) of the P pictures of the clip coded into a real stream
+    with `field`; the stream must decode to the coder's reconstruction."""
+    from . import h264dec, h264enc
+    c = h264enc.code_clip(y, u, v, field, qp)
+    frames = h264dec.decode_annexb(c.stream, per_access_unit=False)
+    if len(frames) != len(c.recon) or not all(
+            np.array_equal(f.y, r[0]) and np.array_equal(f.u, r[1]) and np.array_equal(f.v, r[2])
+            for f, r in zip(frames, c.recon)):
+        raise AssertionError("the coded stream does not decode to the coder's reconstruction")
+    return float(c.p_bits), c.p_psnr
+
+
 def evaluate(frames: int, seed: int, ncand: int = 5, thr_part: float = 0.7, thr_mv: float = 0.0,
-             log=print, w: int = 1920, h: int = 1080, weights: str | None = None) -> dict:
+             log=print, w: int = 1920, h: int = 1080, weights: str | None = None, coder: str = "h264") -> dict:
     y, u, v = synth_clip(clip_spec(seed, frames + 1, w, h))
     mbw, mbh = mb_dims(w, h)
     dc = DeviceClip(y, mbw, mbh, u, v)
@@ -76,12 +96,13 @@ def evaluate(frames: int, seed: int, ncand: int = 5, thr_part: float = 0.7, thr_
         bare, _ = P.search_clip(dc, 32, qp, prepared=True, hints_in=none)
         net.close()
         for k, f in (("plain", plain), ("pruned", pruned), ("no_network", bare)):
-            pts[k].append(rd_point(dc, f, qp))
+            pts[k].append(rd_point_h264(y, u, v, f, qp) if coder == "h264" else rd_point(dc, f, qp))
         judge[qp] = {"pruned": P.judge(plain, pruned), "no_network": P.judge(plain, bare),
                      "pruned_fraction": float(np.mean(hints["prune"] != 0))}
         log(f"qp {qp}: " + ", ".join(f"{k} {pts[k][-1][0] / 1e3:.0f} kbit {pts[k][-1][1]:.3f} dB" for k in pts))
     return {
         "frames": frames, "seed": seed, "size": [w, h], "qps": list(QPS), "ncand": ncand, "thr_part": thr_part, "thr_mv": thr_mv,
+        "coder": coder, "decoded_bit_exact": coder == "h264",
         "points": {k: [[round(r, 1), round(q, 4)] for r, q in v] for k, v in pts.items()},
         "bd_rate_pruned_pct": round(R.bd_rate(pts["plain"], pts["pruned"]), 4),
         "bd_rate_no_network_pct": round(R.bd_rate(pts["plain"], pts["no_network"]), 4),
@@ -99,14 +120,17 @@ def main(argv=None) -> int:
     ap.add_argument("--thr", type=float, default=0.7, help="partition-head confidence threshold")
     ap.add_argument("--thr-mv", type=float, default=0.0, help="motion-head candidate-mass threshold")
     ap.add_argument("--weights", default=None, help="pruner weights (.npz; default: the committed ones)")
+    ap.add_argument("--coder", choices=("h264", "model"), default="h264")
+    ap.add_argument("--size", default="1920x1080")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     import torch
     if not torch.cuda.is_available():
         print("the evaluation needs the GPU", file=sys.stderr)
         return 2
+    w, h = (int(t) for t in a.size.split("x"))
     rep = evaluate(a.frames, a.seed, a.ncand, a.thr, a.thr_mv, log=lambda s: print(s, file=sys.stderr, flush=True),
-                   weights=a.weights)
+                   w=w, h=h, weights=a.weights, coder=a.coder)
     print(json.dumps(rep))
     if a.out:
         with open(a.out, "w") as f:

@@ -1,4 +1,4 @@
-"""The minimal H.264 decoder (include/pcme_dec.h, libpcme_dec.so) under the
+"""The minimal H.264 decoder (include/pcme_dec.h, libpcme_h264.so) under the
 reference's names: ``Decoder`` with ``decode(packet)`` and ``flush()``
 returning decoded frames (src/h264/dec/decoder.rs), ``decode_annexb`` and
 ``frames_digest`` as the reference's test harness uses them
@@ -14,7 +14,7 @@ import numpy as np
 
 from . import PCME_OK, PcmeError
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lib", "libpcme_dec.so")
+LIB_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lib", "libpcme_h264.so")
 
 # pcme_h264_mb (144 bytes)
 MB_DTYPE = np.dtype([("kind", "i1"), ("part", "i1"), ("sub", "i1", (4,)), ("ref", "i1", (4,)), ("qp", "u1"),
