@@ -99,27 +99,35 @@ def measured_traffic(workload):
     return None
 
 
-BDRATE_SUMMARY = os.path.join(ROOT, "profiles", "pruner_bdrate_1080p.md")
+BDRATE_SUMMARY = os.path.join(ROOT, "profiles", "pruner_bdrate_bitstreams.md")
+WORKLOAD_SIZES = {"1080p": ["1920x1080"], "4k": ["3840x2160"],
+                  "batch": ["352x288", "640x360", "1280x720", "1920x1080"]}
 
 
-def measured_bd_rate(ncand, thr_part, thr_mv):
-    """The pruned search's BD-rate against the plain search under the P-frame
-    coding model (pcme/rdmodel.py, QP 22/27/32/37, held-out 1080p clip), from
-    the evaluation summarised in profiles/ for the same confidence rule;
-    None for other rules or without the summary."""
-    if not os.path.exists(BDRATE_SUMMARY):
+def measured_bd_rate(ncand, thr_part, thr_mv, workload):
+    """The pruned search's BD-rate against the plain search on real
+    bitstreams (the field coder, QP 22/27/32/37, held-out clips of each
+    picture size of the workload, every stream decoded bit-exact), from the
+    evaluation summarised in profiles/ for the default confidence rule; None
+    for other rules or without the summary."""
+    if not os.path.exists(BDRATE_SUMMARY) or (ncand, thr_part, thr_mv) != (5, 0.7, 0.0):
         return None
-    key = f"`thr_part {thr_part:g}, thr_mv {thr_mv:g}`: "
-    if ncand != 5:
-        return None
+    per = {}
     with open(BDRATE_SUMMARY) as f:
         for ln in f:
-            if ln.startswith(key):
-                kv = dict(x.split("=") for x in ln[len(key):].split())
-                return {"cubic_pct": float(kv["bd_rate_cubic_pct"]), "pchip_pct": float(kv["bd_rate_pchip_pct"]),
-                        "model": "pcme/rdmodel.py P-frame coding model, QP 22/27/32/37",
-                        "source": "profiles/pruner_bdrate_1080p.md"}
-    return None
+            if not ln.startswith("`") or "`: " not in ln:
+                continue
+            key, rest = ln[1:].split("`: ", 1)
+            size = key.split()[0]
+            kv = dict(x.split("=") for x in rest.split())
+            per.setdefault(size, []).append((float(kv["bd_rate_cubic_pct"]), float(kv["bd_rate_pchip_pct"])))
+    sizes = {z: per[z] for z in WORKLOAD_SIZES[workload] if z in per}
+    if not sizes:
+        return None
+    return {"per_size": {z: {"cubic_pct": [c for c, _ in v], "pchip_pct": [q for _, q in v]} for z, v in sizes.items()},
+            "max_pct": max(max(c, q) for v in sizes.values() for c, q in v),
+            "coder": "field coder (pcme/h264enc.py), Baseline CAVLC bitstreams decoded bit-exact, QP 22/27/32/37",
+            "source": "profiles/pruner_bdrate_bitstreams.md"}
 
 
 def clip_specs(pcme, workload):
@@ -713,7 +721,7 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
         "net_roofline": {"bound": "mfma", "kernel": "hints_kernel", "achieved": round(tflops, 3),
                          "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5), "macs_per_mb": NET_MACS_PER_MB},
-        "bd_rate": measured_bd_rate(args.ncand, args.thr, thr_mv) if args.workload == "1080p" else None,
+        "bd_rate": measured_bd_rate(args.ncand, args.thr, thr_mv, args.workload),
     }
 
 

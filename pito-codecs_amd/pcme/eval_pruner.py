@@ -34,9 +34,18 @@ from . import rdmodel as R
 QPS = (22, 27, 32, 37)
 
 
+# The benchmark workloads' generators by picture size: (rects, smallest and
+# largest rect side, largest speed in px/frame, zoom per frame).
+GENERATORS = {(352, 288): (8, 24, 120, 6.0, 0.0), (640, 360): (8, 24, 180, 6.0, 0.0),
+              (1280, 720): (10, 64, 300, 8.0, 0.0), (1920, 1080): (12, 64, 400, 8.0, 0.0),
+              (3840, 2160): (8, 128, 900, 16.0, 0.002)}
+
+
 def clip_spec(seed: int, frames: int, w: int = 1920, h: int = 1080) -> ClipSpec:
-    """The 1080p workload's generator (12 rects of 64-400 px at up to 8 px/frame) at w x h."""
-    return ClipSpec(w, h, frames, nrect=12, rect_lo=64, rect_hi=400, vmax=8.0, pan_max=1.0, seed=seed)
+    """A held-out clip of the workload generator of that size (the 1080p one
+    for other sizes)."""
+    n, lo, hi, v, z = GENERATORS.get((w, h), GENERATORS[(1920, 1080)])
+    return ClipSpec(w, h, frames, nrect=n, rect_lo=lo, rect_hi=hi, vmax=v, pan_max=1.0, zoom=z, seed=seed)
 
 
 def source_mbs(dc: DeviceClip, t: int) -> np.ndarray:

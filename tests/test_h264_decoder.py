@@ -1,4 +1,4 @@
-"""The minimal H.264 decoder (libpcme_dec.so, include/pcme_dec.h) against
+"""The minimal H.264 decoder (libpcme_h264.so, include/pcme_dec.h) against
 the reference's own fixture, and the oracle's motion compensation and MV
 prediction against the decoder.
 
