@@ -85,17 +85,19 @@ def parse():
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "kernels_1080p_traffic.md")
 
 
-def measured_traffic(workload):
-    """HBM-side bytes per launch of the search kernel on the 1080p workload,
-    from the FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (scaled by
-    a 1 GiB copy measured in the same runs); None for other workloads or
-    when the summary is absent."""
+def measured_traffic(workload, reuse_regions=1):
+    """HBM-side bytes per launch of the search kernel on the 1080p workload
+    in the given LDS-region-reuse configuration, from the FETCH_SIZE /
+    WRITE_SIZE runs summarised in profiles/ (scaled by a 1 GiB copy measured
+    in the same runs); None for other workloads or when the summary is
+    absent."""
     if workload != "1080p" or not os.path.exists(TRAFFIC_SUMMARY):
         return None
+    key = f"`search_frames_kernel` reuse_regions={int(bool(reuse_regions))}: bytes_per_launch="
     with open(TRAFFIC_SUMMARY) as f:
         for ln in f:
-            if ln.startswith("`search_frames_kernel`: bytes_per_launch="):
-                return int(ln.split("=", 1)[1])
+            if ln.startswith(key):
+                return int(ln.split("=", 2)[2])
     return None
 
 
@@ -276,6 +278,7 @@ def main():
         print(f"[bench] one frame alone {one_ms:.2f} ms ({1e3 * one_ms / path:.1f} us per MB on the critical path)",
               file=sys.stderr, flush=True)
         p1.close()
+    reuse = tune["reuse_regions"] if tune else 1  # the plan's default keeps regions in LDS
     achieved_gbs = BYTES_PER_MB * mbs_per_step / (search_ms * 1e-3) / 1e9
     wl = {"1080p": "plain GPU ME-only search, 1920x1080 60 frames IPPP, 1 ref, QP 27, me_range 32 + qpel, "
                    "partitions 16x16/16x8/8x16/8x8, 12 rects 64-400 px, v in [-8,8] px/frame",
@@ -293,9 +296,9 @@ def main():
         "roofline": None,  # set below from the oracle's operation counts (cpu_baseline leg)
         "hbm_roofline": {"bound": "hbm", "kernel": "search_frames_kernel", "achieved": round(achieved_gbs, 2),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
-                         "traffic": measured_traffic(args.workload),
+                         "traffic": measured_traffic(args.workload, reuse),
                          "traffic_source": "profiles/kernels_1080p_traffic.md (FETCH_SIZE/WRITE_SIZE runs)"
-                         if measured_traffic(args.workload) is not None else None,
+                         if measured_traffic(args.workload, reuse) is not None else None,
                          "algorithmic_bytes_per_mb": BYTES_PER_MB, "kernel_ms": round(search_ms, 3)},
         "autotune": tune,
         "graph": graph,
@@ -316,7 +319,7 @@ def main():
         line["roofline"] = {"bound": "valu", "kernel": "search_frames_kernel", "achieved": round(valu_tops, 3),
                             "peak": round(VALU_PEAK_TOPS, 2), "unit": "T lane-op/s",
                             "frac": round(valu_tops / VALU_PEAK_TOPS, 4),
-                            "traffic": measured_traffic(args.workload),
+                            "traffic": measured_traffic(args.workload, reuse),
                             "lane_ops_per_mb": round(per_mb_ops, 1), "mbs_per_launch": mbs_per_step,
                             "kernel_ms": round(search_ms, 3),
                             "peak_basis": "256 CU x 64 lanes x 2.4 GHz, one 32-bit lane-op per lane per clock"}
