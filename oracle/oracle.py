@@ -58,7 +58,7 @@ class ojob(ctypes.Structure):
 OMB_DTYPE = np.dtype([
     ("mv", "<i2", (16, 2)), ("mvs", "<i2", (4, 2)), ("mvps", "<i2", (4, 2)),
     ("cost", "<u4", (4,)), ("best_cost", "<u4"), ("skip_mv", "<i2", (2,)),
-    ("mv16", "<i2", (2,)), ("part", "u1"), ("reserved", "u1", (3,)),
+    ("mv16", "<i2", (2,)), ("part", "u1"), ("ref", "u1"), ("reserved", "u1", (2,)),
 ])
 
 # otree_block has the layout of pcme_tree_block (mbtree.rs Block)
@@ -154,6 +154,11 @@ def load() -> ctypes.CDLL:
     lib.o_search_frame_ex.restype = None
     lib.o_search_frame_ex.argtypes = [c.c_int, c.c_int, c.c_int, c.c_int, c.c_int, c.c_uint32, P, c.c_int,
                                       c.c_int, c.POINTER(oref), P, P, c.POINTER(ocounters)]
+    lib.o_search_frame_mr.restype = None
+    lib.o_search_frame_mr.argtypes = [c.c_int, c.c_int, c.c_int, c.c_int, c.c_int, c.c_uint32, P, c.c_int,
+                                      c.c_int, c.POINTER(oref), c.c_int, P, P, c.POINTER(ocounters)]
+    lib.o_ref_bits.restype = c.c_uint32
+    lib.o_ref_bits.argtypes = [c.c_int, c.c_int]
     lib.o_search_frame.restype = None
     lib.o_search_frame.argtypes = [c.c_int, c.c_int, c.c_int, c.c_int, c.c_int, c.c_uint32, P, c.c_int,
                                    c.c_int, c.POINTER(oref), P, c.POINTER(ocounters)]
@@ -281,6 +286,44 @@ def search_frame(src_padded: np.ndarray, ref: Ref, mbw: int, mbh: int, me_range:
                           _p(s), s.shape[1], LUMA_PAD, ctypes.byref(ref.view), _p(h) if h is not None else None,
                           _p(out), ctypes.byref(counters) if counters is not None else None)
     return out.reshape(mbh, mbw)
+
+
+def search_frame_mr(src_padded: np.ndarray, refs: list, mbw: int, mbh: int, me_range: int, subpel_rounds: int,
+                    partitions: bool, qp: int, hints: np.ndarray | None = None,
+                    counters: ocounters | None = None):
+    """ME-only field of one frame over several references (refs[0] the
+    newest, each a Ref): o_search_frame_mr."""
+    lib = load()
+    out = np.zeros(mbw * mbh, OMB_DTYPE)
+    s = np.ascontiguousarray(src_padded)
+    views = (oref * len(refs))(*[r.view for r in refs])
+    h = None
+    if hints is not None:
+        h = np.ascontiguousarray(hints.reshape(-1).view(OHINT_DTYPE))
+        assert h.size == mbw * mbh
+    lib.o_search_frame_mr(mbw, mbh, me_range, subpel_rounds, 1 if partitions else 0, lib.o_lambda(qp), _p(s),
+                          s.shape[1], LUMA_PAD, views, len(refs), _p(h) if h is not None else None, _p(out),
+                          ctypes.byref(counters) if counters is not None else None)
+    return out.reshape(mbh, mbw)
+
+
+def search_clip_mr(y_frames: np.ndarray, nref: int, me_range: int, subpel_rounds: int, partitions: bool, qp: int,
+                   hints: np.ndarray | None = None):
+    """ME-only field of every P frame of an IPPP clip over the nref previous
+    frames (fewer at the start: frame t has min(t, nref) references)."""
+    n, h, w = y_frames.shape
+    mbw, mbh = (w + 15) // 16, (h + 15) // 16
+    padded = [pad_luma(y_frames[i], mbw, mbh) for i in range(n)]
+    refs = [None] * n
+    res = []
+    for t in range(1, n):
+        for k in range(max(0, t - nref), t):
+            if refs[k] is None:
+                refs[k] = Ref(padded[k])
+        rl = [refs[t - 1 - k] for k in range(min(t, nref))]
+        res.append(search_frame_mr(padded[t], rl, mbw, mbh, me_range, subpel_rounds, partitions, qp,
+                                   None if hints is None else hints[t - 1]))
+    return np.stack(res) if res else np.zeros((0, mbh, mbw), OMB_DTYPE)
 
 
 def search_clip_me(y_frames: np.ndarray, me_range: int, subpel_rounds: int, partitions: bool, qp: int,

@@ -641,7 +641,7 @@ static int shape_of(int part, int q) {
 }
 
 static uint32_t search_part_ex(const oparams *p, const ombinfo *mbs, int mx,
-                               int my, const oref *r, int part,
+                               int my, const oref *r, int rf, int part,
                                const uint8_t *src16, uint32_t lam, omv seed,
                                const omv *hc, int nh, omv mvs[4],
                                omv mvps[4], ocounters *ctr);
@@ -651,12 +651,15 @@ uint32_t o_search_part(const oparams *p, const ombinfo *mbs, int mx, int my,
                        const oref *r, int part, const uint8_t *src16,
                        uint32_t lam, omv seed, omv mvs[4], omv mvps[4],
                        ocounters *ctr) {
-  return search_part_ex(p, mbs, mx, my, r, part, src16, lam, seed, NULL, -1,
+  return search_part_ex(p, mbs, mx, my, r, 0, part, src16, lam, seed, NULL, -1,
                         mvs, mvps, ctr);
 }
 
+/* With reference index rf (list 0): the predictor takes the neighbours of
+ * that index (o_mvp's rf) and the searched partitions are committed with
+ * it; rf 0 is the reference's search_part. */
 static uint32_t search_part_ex(const oparams *p, const ombinfo *mbs, int mx,
-                               int my, const oref *r, int part,
+                               int my, const oref *r, int rf, int part,
                                const uint8_t *src16, uint32_t lam, omv seed,
                                const omv *hc, int nh, omv mvs[4],
                                omv mvps[4], ocounters *ctr) {
@@ -671,7 +674,7 @@ static uint32_t search_part_ex(const oparams *p, const ombinfo *mbs, int mx,
     int x, y, w, h;
     part_rect(part, q, &x, &y, &w, &h);
     uint16_t mask = mask_of(part, q);
-    omv mvp = o_mvp(p, mbs, &info, mask, mx, my, x / 4, y / 4, w / 4, 0,
+    omv mvp = o_mvp(p, mbs, &info, mask, mx, my, x / 4, y / 4, w / 4, (int8_t)rf,
                     shape_of(part, q));
     omv cands[2] = {seed, mvs[0]};
     uint32_t cost;
@@ -684,7 +687,7 @@ static uint32_t search_part_ex(const oparams *p, const ombinfo *mbs, int mx,
       for (int bx = x / 4; bx < (x + w) / 4; bx++) info.mv[by * 4 + bx] = mv;
     for (int by8 = y / 8; by8 < (y + h + 7) / 8; by8++)
       for (int bx8 = x / 8; bx8 < (x + w + 7) / 8; bx8++)
-        info.ref_idx[by8 * 2 + bx8] = 0;
+        info.ref_idx[by8 * 2 + bx8] = (int8_t)rf;
   }
   return total;
 }
@@ -698,12 +701,41 @@ void o_analyse_mb(const oparams *p, ombinfo *mbs, int mx, int my,
                   ctr);
 }
 
+/* Length of ref_idx's te(v) code with num_ref_idx_active = nref (H.264
+ * 7.4.5.1, 9.1): none with one reference, one bit with two, else ue(v). */
+uint32_t o_ref_bits(int r, int nref) {
+  if (nref <= 1) return 0;
+  if (nref == 2) return 1;
+  uint32_t k = (uint32_t)r + 1, n = 0;
+  while (k > 1) {
+    k >>= 1;
+    n++;
+  }
+  return 2 * n + 1;
+}
+
 /* o_analyse_mb; a pruned MB (h->prune) passes the hint candidates to every
  * search and searches only the hinted modes. */
 void o_analyse_mb_ex(const oparams *p, ombinfo *mbs, int mx, int my,
                      const oref *r, const uint8_t *src_plane, int sstride,
                      int spad, uint32_t lam, const ohint *h, omb_out *out,
                      ocounters *ctr) {
+  o_analyse_mb_mr(p, mbs, mx, my, r, 1, src_plane, sstride, spad, lam, h, out, ctr);
+}
+
+/* o_analyse_mb_ex over nref references of list 0 (refs[0] = ref_idx 0, the
+ * newest).  No counterpart in the reference, whose P slices use one
+ * reference: the P16x16 search runs on every reference (a pruned MB: on
+ * reference 0 only), each with the predictor of its index and with
+ * (lam * te(v) bits of the index) >> 8 added to its cost; the least total
+ * (the lower index on ties) gives the MB's reference, and the partition
+ * searches run on it, each partition paying the index's rate.  A reference
+ * other than the newest is read without the row limit (it is complete).
+ * With nref = 1 this is o_analyse_mb_ex. */
+void o_analyse_mb_mr(const oparams *p, ombinfo *mbs, int mx, int my,
+                     const oref *refs, int nref, const uint8_t *src_plane,
+                     int sstride, int spad, uint32_t lam, const ohint *h,
+                     omb_out *out, ocounters *ctr) {
   const int pr = h != NULL && h->prune != 0;
   const omv *hc = pr ? h->cand : NULL;
   const int nh = pr ? (h->ncand > 8 ? 8 : (int)h->ncand) : -1;
@@ -713,9 +745,26 @@ void o_analyse_mb_ex(const oparams *p, ombinfo *mbs, int mx, int my,
     memcpy(src16 + rr * 16,
            src_plane + (size_t)(py + rr + spad) * sstride + px + spad, 16);
   omv skip = o_pskip_mv(p, mbs, mx, my);
+  oparams pall = *p;
+  pall.vlim = o_vlim(p->mbh - 1, p->mbh);
   omv mv16[4], mvp16[4];
-  uint32_t cost16 = search_part_ex(p, mbs, mx, my, r, OP_16x16, src16, lam,
-                                   skip, hc, nh, mv16, mvp16, ctr);
+  uint32_t cost16 = 0;
+  int rs = 0;
+  for (int rf = 0; rf < (pr ? 1 : nref); rf++) {
+    omv m[4], pp[4];
+    const uint32_t c =
+        search_part_ex(rf == 0 ? p : &pall, mbs, mx, my, &refs[rf], rf, OP_16x16, src16, lam, skip, hc, nh, m, pp,
+                       ctr) +
+        ((lam * o_ref_bits(rf, nref)) >> 8);
+    if (rf == 0 || c < cost16) {
+      cost16 = c;
+      rs = rf;
+      memcpy(mv16, m, sizeof m);
+      memcpy(mvp16, pp, sizeof pp);
+    }
+  }
+  const oparams *ps = rs == 0 ? p : &pall;
+  const uint32_t rcost = (lam * o_ref_bits(rs, nref)) >> 8;
   int best_part = OP_16x16;
   omv best_mvs[4], best_mvps[4];
   memcpy(best_mvs, mv16, sizeof mv16);
@@ -730,8 +779,9 @@ void o_analyse_mb_ex(const oparams *p, ombinfo *mbs, int mx, int my,
                                    : (h->part == OP_16x16 || parts[k] == OP_8x8)))
         continue;
       omv m[4], pp[4];
-      uint32_t c = search_part_ex(p, mbs, mx, my, r, parts[k], src16, lam,
-                                  mv16[0], hc, nh, m, pp, ctr);
+      uint32_t c = search_part_ex(ps, mbs, mx, my, &refs[rs], rs, parts[k], src16, lam,
+                                  mv16[0], hc, nh, m, pp, ctr) +
+                   (uint32_t)part_count(parts[k]) * rcost;
       out->cost[parts[k]] = c;
       uint32_t extra = parts[k] == OP_8x8 ? 10 : 3;
       uint64_t cc = (uint64_t)c + (uint64_t)((lam * extra) >> 8);
@@ -753,7 +803,7 @@ void o_analyse_mb_ex(const oparams *p, ombinfo *mbs, int mx, int my,
       for (int bx = x / 4; bx < (x + w) / 4; bx++) mi->mv[by * 4 + bx] = best_mvs[q];
     for (int by8 = y / 8; by8 < (y + h + 7) / 8; by8++)
       for (int bx8 = x / 8; bx8 < (x + w + 7) / 8; bx8++)
-        mi->ref_idx[by8 * 2 + bx8] = 0;
+        mi->ref_idx[by8 * 2 + bx8] = (int8_t)rs;
   }
   memcpy(out->mv, mi->mv, sizeof mi->mv);
   memcpy(out->mvs, best_mvs, sizeof best_mvs);
@@ -762,7 +812,8 @@ void o_analyse_mb_ex(const oparams *p, ombinfo *mbs, int mx, int my,
   out->skip_mv = skip;
   out->mv16 = mv16[0];
   out->part = (uint8_t)best_part;
-  out->pad_[0] = out->pad_[1] = out->pad_[2] = 0;
+  out->ref = (uint8_t)rs;
+  out->pad_[0] = out->pad_[1] = 0;
   if (ctr) ctr->mbs++;
 }
 
@@ -778,12 +829,20 @@ void o_search_frame_ex(int mbw, int mbh, int me_range, int subpel_rounds,
                        int partitions, uint32_t lam, const uint8_t *src,
                        int sstride, int spad, const oref *r,
                        const ohint *hints, omb_out *out, ocounters *ctr) {
+  o_search_frame_mr(mbw, mbh, me_range, subpel_rounds, partitions, lam, src, sstride, spad, r, 1, hints, out,
+                    ctr);
+}
+
+void o_search_frame_mr(int mbw, int mbh, int me_range, int subpel_rounds,
+                       int partitions, uint32_t lam, const uint8_t *src,
+                       int sstride, int spad, const oref *refs, int nref,
+                       const ohint *hints, omb_out *out, ocounters *ctr) {
   ombinfo *mbs = (ombinfo *)calloc((size_t)mbw * mbh, sizeof(ombinfo));
   oparams p = {mbw, mbh, me_range, subpel_rounds, partitions, 0};
   for (int my = 0; my < mbh; my++) {
     p.vlim = o_vlim(my, mbh);
     for (int mx = 0; mx < mbw; mx++)
-      o_analyse_mb_ex(&p, mbs, mx, my, r, src, sstride, spad, lam,
+      o_analyse_mb_mr(&p, mbs, mx, my, refs, nref, src, sstride, spad, lam,
                       hints ? &hints[my * mbw + mx] : NULL,
                       &out[my * mbw + mx], ctr);
   }

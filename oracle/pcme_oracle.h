@@ -81,7 +81,8 @@ typedef struct {
   omv skip_mv;
   omv mv16;        /* 16x16 search result, seed of the other modes */
   uint8_t part;
-  uint8_t pad_[3];
+  uint8_t ref;     /* the MB's reference index (o_analyse_mb_mr) */
+  uint8_t pad_[2];
 } omb_out;
 
 /* Learned-pruner hint (layout of pcme_hint, include/pcme.h).  No
@@ -179,6 +180,20 @@ void o_analyse_mb_ex(const oparams *p, ombinfo *mbs, int mx, int my,
                      const oref *r, const uint8_t *src_plane, int sstride,
                      int spad, uint32_t lam, const ohint *h, omb_out *out,
                      ocounters *ctr);
+/* o_analyse_mb_ex over nref references (refs[0] the newest): see
+ * pcme_oracle.c.  No counterpart in the reference (P slices there use one
+ * reference); with nref = 1 it is o_analyse_mb_ex. */
+void o_analyse_mb_mr(const oparams *p, ombinfo *mbs, int mx, int my,
+                     const oref *refs, int nref, const uint8_t *src_plane,
+                     int sstride, int spad, uint32_t lam, const ohint *h,
+                     omb_out *out, ocounters *ctr);
+/* te(v) length of reference index r among nref. */
+uint32_t o_ref_bits(int r, int nref);
+/* o_search_frame_ex over nref references. */
+void o_search_frame_mr(int mbw, int mbh, int me_range, int subpel_rounds,
+                       int partitions, uint32_t lam, const uint8_t *src,
+                       int sstride, int spad, const oref *refs, int nref,
+                       const ohint *hints, omb_out *out, ocounters *ctr);
 /* o_search_frame with per-MB hints (mbw*mbh, raster; NULL = none). */
 void o_search_frame_ex(int mbw, int mbh, int me_range, int subpel_rounds,
                        int partitions, uint32_t lam, const uint8_t *src,

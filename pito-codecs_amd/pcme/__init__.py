@@ -187,7 +187,7 @@ assert TREE_BLOCK_DTYPE.itemsize == 20
 MB_DTYPE = np.dtype([
     ("mv", "<i2", (16, 2)), ("mvs", "<i2", (4, 2)), ("mvps", "<i2", (4, 2)),
     ("cost", "<u4", (4,)), ("best_cost", "<u4"), ("skip_mv", "<i2", (2,)),
-    ("mv16", "<i2", (2,)), ("part", "u1"), ("reserved", "u1", (3,)),
+    ("mv16", "<i2", (2,)), ("part", "u1"), ("ref", "u1"), ("reserved", "u1", (2,)),
 ])
 assert MB_DTYPE.itemsize == 128
 

@@ -3,7 +3,7 @@ from __future__ import annotations
 
 import numpy as np
 
-FIELDS = ("mv", "mvs", "mvps", "cost", "best_cost", "skip_mv", "mv16", "part")
+FIELDS = ("mv", "mvs", "mvps", "cost", "best_cost", "skip_mv", "mv16", "part", "ref")
 
 
 def assert_same_field(got: np.ndarray, want: np.ndarray, what: str = "") -> None:

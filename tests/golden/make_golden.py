@@ -18,7 +18,7 @@ import pysearch  # noqa: E402
 DTYPE = np.dtype([
     ("mv", "<i2", (16, 2)), ("mvs", "<i2", (4, 2)), ("mvps", "<i2", (4, 2)),
     ("cost", "<u4", (4,)), ("best_cost", "<u4"), ("skip_mv", "<i2", (2,)),
-    ("mv16", "<i2", (2,)), ("part", "u1"), ("reserved", "u1", (3,)),
+    ("mv16", "<i2", (2,)), ("part", "u1"), ("ref", "u1"), ("reserved", "u1", (2,)),
 ])
 
 

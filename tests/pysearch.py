@@ -76,12 +76,20 @@ def satd(d):
 class Frame:
     """ME-only field analysis of one P frame (raster order)."""
 
-    def __init__(self, src_frame, ref_frame, me_range, subpel_rounds, partitions, qp):
+    def __init__(self, src_frame, ref_frame, me_range, subpel_rounds, partitions, qp, older_refs=()):
+        """older_refs: further references of list 0 after ref_frame (reference
+        indices 1, 2, ...), for the multi-reference extension."""
         h, w = src_frame.shape
         self.mbw, self.mbh = (w + 15) // 16, (h + 15) // 16
         self.src = pad_plane(src_frame, self.mbw, self.mbh)
         G = pad_plane(ref_frame, self.mbw, self.mbh)
         self.planes = [G] + half_planes(G)
+        self.ref_planes = [self.planes]
+        for f in older_refs:
+            Gr = pad_plane(f, self.mbw, self.mbh)
+            self.ref_planes.append([Gr] + half_planes(Gr))
+        self.rf = 0      # reference index being searched
+        self.reff = {}   # committed reference index per MB
         self.range = me_range
         self.subpel = subpel_rounds
         self.partitions = partitions
@@ -128,11 +136,11 @@ class Frame:
             r = y * 4 + x
             if r not in done:
                 return (False, -1, (0, 0))
-            return (True, 0, cur[r])
+            return (True, self.rf, cur[r])
         nx, ny = mb
         if nx < 0 or ny < 0 or nx >= self.mbw:
             return (False, -1, (0, 0))
-        return (True, 0, self.mvf[(nx, ny)][by * 4 + bx])
+        return (True, self.reff.get((nx, ny), 0), self.mvf[(nx, ny)][by * 4 + bx])
 
     def mvp(self, mx, my, cur, done, x, y, w, shape, part):
         a = self.nb_motion(mx, my, cur, done, x - 1, y)
@@ -141,18 +149,18 @@ class Frame:
         if not c[0]:
             c = self.nb_motion(mx, my, cur, done, x - 1, y - 1)
         if shape == 1:  # 16x8
-            if part == 0 and b[1] == 0:
+            if part == 0 and b[1] == self.rf:
                 return b[2]
-            if part == 1 and a[1] == 0:
+            if part == 1 and a[1] == self.rf:
                 return a[2]
         elif shape == 2:  # 8x16
-            if part == 0 and a[1] == 0:
+            if part == 0 and a[1] == self.rf:
                 return a[2]
-            if part == 1 and c[1] == 0:
+            if part == 1 and c[1] == self.rf:
                 return c[2]
         if not b[0] and not c[0] and a[0]:
             b = c = a
-        m = [a[1] == 0, b[1] == 0, c[1] == 0]
+        m = [a[1] == self.rf, b[1] == self.rf, c[1] == self.rf]
         if m == [True, False, False]:
             return a[2]
         if m == [False, True, False]:
@@ -253,15 +261,38 @@ class Frame:
                     done.add(by * 4 + bx)
         return mvs, mvps, total
 
+    def use_ref(self, r):
+        self.rf = r
+        self.planes = self.ref_planes[r]
+
     def analyse(self, mx, my):
-        vlim = self.vlim(my)
+        self.use_ref(0)
         skip = self.pskip(mx, my)
-        mv16, mvp16, c16 = self.search_part(mx, my, 0, skip, vlim)
+        # every reference, each paying the rate of its index (te(v)); the
+        # least total, the lower index on ties; older references whole
+        nref = len(self.ref_planes)
+
+        def ref_cost(r):
+            bits = 0 if nref == 1 else (1 if nref == 2 else ue_len(r))
+            return (self.lam * bits) >> 8
+
+        found = None
+        for r in range(nref):
+            self.use_ref(r)
+            vl = self.vlim(my) if r == 0 else self.vlim(self.mbh - 1)
+            m, p, c = self.search_part(mx, my, 0, skip, vl)
+            c += ref_cost(r)
+            if found is None or c < found[3]:
+                found = (r, m, p, c)
+        rs, mv16, mvp16, c16 = found
+        self.use_ref(rs)
+        vlim = self.vlim(my) if rs == 0 else self.vlim(self.mbh - 1)
         best = (c16 + (self.lam >> 8), 0, mv16, mvp16)
         costs = [c16, 0xFFFFFFFF, 0xFFFFFFFF, 0xFFFFFFFF]
         if self.partitions and c16 > 1024:
             for mode, extra in ((1, 3), (2, 3), (3, 10)):
                 m, p, c = self.search_part(mx, my, mode, mv16[0], vlim)
+                c += len(PARTS[mode]) * ref_cost(rs)
                 costs[mode] = c
                 cc = c + ((self.lam * extra) >> 8)
                 if cc < best[0]:
@@ -273,8 +304,10 @@ class Frame:
                 for bx in range(x // 4, (x + w) // 4):
                     field[by * 4 + bx] = mvs[p]
         self.mvf[(mx, my)] = field
+        self.reff[(mx, my)] = rs
+        self.use_ref(0)
         return dict(mv=field, mvs=mvs, mvps=mvps, cost=costs, best_cost=cost, skip_mv=skip, mv16=mv16[0],
-                    part=mode)
+                    part=mode, ref=rs)
 
     def run(self):
         return [[self.analyse(mx, my) for mx in range(self.mbw)] for my in range(self.mbh)]
@@ -298,5 +331,6 @@ def to_records(rows, dtype):
             o["skip_mv"] = np.array(r["skip_mv"])
             o["mv16"] = np.array(r["mv16"])
             o["part"] = r["part"]
+            o["ref"] = r.get("ref", 0)
             out[my, mx] = o
     return out
