@@ -79,7 +79,9 @@ typedef struct pcme_mb_out {
   pcme_mv skip_mv;      /* pskip_mv (slice.rs:484-494), the 16x16 seed */
   pcme_mv mv16;         /* 16x16 result mv16[0], the seed of the other modes */
   uint8_t part;         /* best_part (PCME_P*) */
-  uint8_t reserved[3];
+  uint8_t ref;          /* the MB's reference index (list 0); 0 with one
+                           reference */
+  uint8_t reserved[2];
 } pcme_mb_out;
 
 /* Search parameters (EncParams subset, slice.rs:21-34). */
@@ -109,6 +111,14 @@ typedef struct pcme_hint {
   uint8_t reserved;
 } pcme_hint;
 
+/* The four padded planes of one reference picture (device): luma and its
+ * H, V, C half-pel planes. */
+typedef struct pcme_ref_planes {
+  const uint8_t *luma, *h, *v, *c;
+} pcme_ref_planes;
+
+#define PCME_MAX_REFS 4
+
 /* One P frame of the batched ME-only field search.  All pointers (device). */
 typedef struct pcme_frame_job {
   const uint8_t *src;      /* padded source luma of the frame */
@@ -120,6 +130,19 @@ typedef struct pcme_frame_job {
   const pcme_hint *hint;   /* NULL: full search of every MB; else mbw*mbh
                               hints, raster order (pruned search) */
   pcme_params prm;
+  /* Multi-reference search (no counterpart in the reference, whose P
+   * slices use one reference): nref references of list 0, `ref` being
+   * index 0 (the newest) and older[k] index k + 1.  0 or 1: `ref` only.
+   * Each MB's P16x16 search runs on every reference, each with the
+   * predictor of its index and (lambda * te(v) bits of the index) >> 8 on
+   * its cost; the least total (the lower index on ties) is the MB's
+   * reference (pcme_mb_out.ref) and its partition searches run on it, each
+   * partition paying the index's rate.  References other than the newest
+   * are read whole (no row limit).  A pruned MB searches reference 0
+   * only. */
+  int32_t nref;
+  int32_t reserved;
+  pcme_ref_planes older[PCME_MAX_REFS - 1];
 } pcme_frame_job;
 
 /* ---- library ------------------------------------------------------- */

@@ -64,7 +64,7 @@ struct CallWs {
 struct Dev {
   int id = -1;
   std::mutex mu;
-  int resident_waves[2] = {0, 0};  // full-search / hinted kernel
+  int resident_waves[4] = {0, 0, 0, 0};  // full-search / hinted kernel; the same with several references
   int resident_var[8] = {0};       // resident waves of each full-search configuration (kVariants)
   int cus = 1;                     // compute units
   std::vector<CallWs *> ws;        // per-call workspaces (at most kMaxWs)
@@ -126,6 +126,14 @@ static bool is_gfx950(int d) {
   return strncmp(p.gcnArchName, "gfx950", 6) == 0;
 }
 
+typedef void (*FrameKernel)(const DevJob *, const int2 *, int, uint32_t *, uint32_t);
+// The frame-search kernel of a job table: with or without pruner hints, one
+// reference or several.
+static FrameKernel frame_kernel(bool hinted, bool mr) {
+  if (mr) return hinted ? search_frames_mr_hinted_kernel : search_frames_mr_kernel;
+  return hinted ? search_frames_hinted_kernel : search_frames_kernel;
+}
+
 // The current device's state; the first call per device validates it.
 // Later calls only read the current device id, so entry points stay cheap
 // and safe inside a stream capture.
@@ -145,10 +153,9 @@ static int get_dev(Dev **out) {
   dv->id = d;
   hipDeviceProp_t p;
   HIPCHK(hipGetDeviceProperties(&p, d));
-  for (int v = 0; v < 2; v++) {
+  for (int v = 0; v < 4; v++) {
     int per_cu = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, v ? search_frames_hinted_kernel : search_frames_kernel, 64, 0));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, frame_kernel(v & 1, v >= 2), 64, 0));
     if (per_cu < 1) per_cu = 1;
     dv->resident_waves[v] = per_cu * p.multiProcessorCount;
   }
@@ -288,6 +295,7 @@ struct Plan {
                             // [ctl[16+q], ctl[17+q]) (CTL_BYTES)
   int resident = 0;         // resident waves of the search kernel on this device
   bool hinted = false;      // some job carries pruner hints: the hinted kernel
+  bool mr = false;          // some job has several references: the multi-reference kernels
   int grid = 0;             // waves launched (persistent grid)
   int variant = 0;          // configuration of the full-search kernel (kVariants)
   const int *resident_var = nullptr;  // the device's resident waves per configuration
@@ -307,8 +315,40 @@ static int check_jobs(const pcme_frame_job *jobs, int njobs) {
           (uintptr_t)jobs[i].ref_c) & 3u) != 0)
       return fail(PCME_EINVAL, "job %d: plane pointers must be 4-byte aligned", i);
     if (((uintptr_t)jobs[i].hint & 3u) != 0) return fail(PCME_EINVAL, "job %d: hints must be 4-byte aligned", i);
+    if (jobs[i].nref < 0 || jobs[i].nref > PCME_MAX_REFS)
+      return fail(PCME_EINVAL, "job %d: nref %d not in [0, %d]", i, jobs[i].nref, PCME_MAX_REFS);
+    for (int k = 0; k + 1 < jobs[i].nref; k++) {
+      const pcme_ref_planes &o = jobs[i].older[k];
+      if (!o.luma || !o.h || !o.v || !o.c) return fail(PCME_EINVAL, "job %d: reference %d has a null plane", i, k + 1);
+      if ((((uintptr_t)o.luma | (uintptr_t)o.h | (uintptr_t)o.v | (uintptr_t)o.c) & 3u) != 0)
+        return fail(PCME_EINVAL, "job %d: reference %d: plane pointers must be 4-byte aligned", i, k + 1);
+    }
   }
   return PCME_OK;
+}
+
+// The device copy of a frame job (its exchange words are set by the caller).
+static DevJob dev_job(const pcme_frame_job &J) {
+  DevJob d;
+  memset(&d, 0, sizeof d);
+  d.src = J.src;
+  d.pl[0] = J.ref; d.pl[1] = J.ref_h; d.pl[2] = J.ref_v; d.pl[3] = J.ref_c;
+  d.out = J.out;
+  d.hint = J.hint;
+  d.xchg = nullptr;
+  d.mbw = J.prm.mbw;
+  d.mbh = J.prm.mbh;
+  d.stride = 16 * J.prm.mbw + 2 * PCME_PAD;
+  d.me_range = J.prm.me_range;
+  d.subpel_rounds = J.prm.subpel_rounds;
+  d.partitions = J.prm.partitions ? 1 : 0;
+  d.lam = lambda_of(J.prm.qp);
+  d.nref = std::max(1, (int)J.nref);
+  for (int k = 0; k + 1 < d.nref; k++) {
+    d.opl[k][0] = J.older[k].luma; d.opl[k][1] = J.older[k].h;
+    d.opl[k][2] = J.older[k].v; d.opl[k][3] = J.older[k].c;
+  }
+  return d;
 }
 
 static void plan_free(Plan *p) {
@@ -323,15 +363,18 @@ static void plan_free(Plan *p) {
 
 // Resident waves of the plan's kernel configuration on its device.
 static int plan_resident(const Plan *p) {
-  return p->hinted ? p->resident : p->resident_var[p->variant];
+  return (p->hinted || p->mr) ? p->resident : p->resident_var[p->variant];
 }
 
 static int plan_create(Dev *dv, const pcme_frame_job *jobs, int njobs, Plan **out) {
   Plan *p = new Plan();
   p->dev = dv->id;
   p->njobs = njobs;
-  for (int i = 0; i < njobs; i++) p->hinted |= jobs[i].hint != nullptr;
-  p->resident = dv->resident_waves[p->hinted ? 1 : 0];
+  for (int i = 0; i < njobs; i++) {
+    p->hinted |= jobs[i].hint != nullptr;
+    p->mr |= jobs[i].nref > 1;
+  }
+  p->resident = dv->resident_waves[(p->hinted ? 1 : 0) + (p->mr ? 2 : 0)];
   p->resident_var = dv->resident_var;
   p->cus = dv->cus;
   std::vector<DevJob> dj((size_t)njobs);
@@ -339,20 +382,7 @@ static int plan_create(Dev *dv, const pcme_frame_job *jobs, int njobs, Plan **ou
   int maxh = 0;
   for (int i = 0; i < njobs; i++) {
     const pcme_frame_job &J = jobs[i];
-    DevJob &d = dj[(size_t)i];
-    d.src = J.src;
-    d.pl[0] = J.ref; d.pl[1] = J.ref_h; d.pl[2] = J.ref_v; d.pl[3] = J.ref_c;
-    d.out = J.out;
-    d.hint = J.hint;
-    d.xchg = nullptr;
-    d.mbw = J.prm.mbw;
-    d.mbh = J.prm.mbh;
-    d.stride = 16 * J.prm.mbw + 2 * PCME_PAD;
-    d.me_range = J.prm.me_range;
-    d.subpel_rounds = J.prm.subpel_rounds;
-    d.partitions = J.prm.partitions ? 1 : 0;
-    d.lam = lambda_of(J.prm.qp);
-    d.pad_ = 0;
+    dj[(size_t)i] = dev_job(J);
     words += (size_t)J.prm.mbw * J.prm.mbh * 4;
     maxh = std::max(maxh, J.prm.mbh);
   }
@@ -422,8 +452,8 @@ static int plan_launch(Plan *p, hipStream_t stream) {
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(clear_kernel, dim3(512), dim3(256), 0, stream, p->ctl, p->xchg, p->nwords);
   HIPCHK(hipGetLastError());
-  if (p->hinted) {
-    hipLaunchKernelGGL(search_frames_hinted_kernel, dim3((unsigned)p->grid), dim3(64), 0, stream, p->djobs,
+  if (p->hinted || p->mr) {
+    hipLaunchKernelGGL(frame_kernel(p->hinted, p->mr), dim3((unsigned)p->grid), dim3(64), 0, stream, p->djobs,
                        p->dtickets, p->ntickets, p->ctl, kSpinLimit);
   } else {
     const int wpg = kVariants[p->variant].wpg;
@@ -507,12 +537,13 @@ static int search_frames_noalloc(const pcme_frame_job *jobs, int njobs, hipStrea
     return fail(PCME_EINVAL, "pcme_search_frames_noalloc: %d jobs exceed the limit of %d", njobs,
                 PCME_NOALLOC_MAX_JOBS);
   size_t words = 0, mbs = 0;
-  int maxh = 0, hinted = 0;
+  int maxh = 0, hinted = 0, mr = 0;
   for (int i = 0; i < njobs; i++) {
     words += (size_t)jobs[i].prm.mbw * jobs[i].prm.mbh * 4;
     mbs += (size_t)jobs[i].prm.mbw * jobs[i].prm.mbh;
     maxh = std::max(maxh, jobs[i].prm.mbh);
     hinted |= jobs[i].hint != nullptr;
+    mr |= jobs[i].nref > 1;
   }
   size_t ntk = 0;
   for (int i = 0; i < njobs; i++) ntk += (size_t)jobs[i].prm.mbh;
@@ -538,19 +569,8 @@ static int search_frames_noalloc(const pcme_frame_job *jobs, int njobs, hipStrea
   for (int i = 0; i < njobs; i++) {
     const pcme_frame_job &J = jobs[i];
     DevJob &d = g_na_jobs[i];
-    d.src = J.src;
-    d.pl[0] = J.ref; d.pl[1] = J.ref_h; d.pl[2] = J.ref_v; d.pl[3] = J.ref_c;
-    d.out = J.out;
-    d.hint = J.hint;
+    d = dev_job(J);
     d.xchg = xchg + off;
-    d.mbw = J.prm.mbw;
-    d.mbh = J.prm.mbh;
-    d.stride = 16 * J.prm.mbw + 2 * PCME_PAD;
-    d.me_range = J.prm.me_range;
-    d.subpel_rounds = J.prm.subpel_rounds;
-    d.partitions = J.prm.partitions ? 1 : 0;
-    d.lam = lambda_of(J.prm.qp);
-    d.pad_ = 0;
     const size_t m = (size_t)J.prm.mbw * J.prm.mbh;
     off += m * 4;
     qof[i] = (int)std::min<size_t>(PCME_QUEUES - 1, (cum + m / 2) * PCME_QUEUES / std::max<size_t>(total, 1));
@@ -576,12 +596,12 @@ static int search_frames_noalloc(const pcme_frame_job *jobs, int njobs, hipStrea
   HIPCHK(hipMemcpyAsync(ctl + 16, qoff, sizeof qoff, hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync(djobs, g_na_jobs, (size_t)njobs * sizeof(DevJob), hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync(dtk, g_na_tickets, nt * sizeof(int2), hipMemcpyHostToDevice, stream));
-  const int grid = std::min((int)nt, dv->resident_waves[hinted ? 1 : 0]);
+  const int grid = std::min((int)nt, dv->resident_waves[(hinted ? 1 : 0) + (mr ? 2 : 0)]);
   hipLaunchKernelGGL(prep_kernel, dim3(1), dim3(64), 0, stream, ctl);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(clear_kernel, dim3(512), dim3(256), 0, stream, ctl, xchg, words);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(hinted ? search_frames_hinted_kernel : search_frames_kernel, dim3((unsigned)grid), dim3(64), 0,
+  hipLaunchKernelGGL(frame_kernel(hinted != 0, mr != 0), dim3((unsigned)grid), dim3(64), 0,
                      stream, djobs, dtk, (int)nt, ctl, kSpinLimit);
   HIPCHK(hipGetLastError());
   // the host tables are reused by the next call: wait here (the call is
@@ -739,7 +759,7 @@ int pcme_plan_get_config(pcme_plan *plan, pcme_tune *out) {
   const Variant &v = kVariants[p->variant];
   out->waves = p->grid;
   out->waves_per_group = v.wpg;
-  out->waves_per_cu = p->hinted ? std::max(1, p->resident / std::max(1, p->cus)) : v.wcu;
+  out->waves_per_cu = (p->hinted || p->mr) ? std::max(1, p->resident / std::max(1, p->cus)) : v.wcu;
   out->reuse_regions = v.reuse ? 1 : 0;
   out->ms = 0.f;
   return PCME_OK;
@@ -756,7 +776,8 @@ int pcme_plan_set_config(pcme_plan *plan, const pcme_tune *cfg) {
   if (v < 0)
     return fail(PCME_EINVAL, "no kernel configuration with %d waves per group, %d waves per CU, region reuse %d",
                 cfg->waves_per_group, cfg->waves_per_cu, cfg->reuse_regions);
-  if (p->hinted && v != 0) return fail(PCME_EINVAL, "a plan with pruner hints runs the default configuration only");
+  if ((p->hinted || p->mr) && v != 0)
+    return fail(PCME_EINVAL, "a plan with pruner hints or several references runs the default configuration only");
   p->variant = v;
   const int w = cfg->waves == 0 ? plan_resident(p) : cfg->waves;
   p->grid = std::max(1, std::min(w, p->ntickets));
@@ -828,7 +849,7 @@ int pcme_plan_autotune_ex(pcme_plan *plan, void *stream, int32_t reps, pcme_tune
   const int fr[] = {16, 14, 12, 10, 8, 6};
   int bv = p->variant, bg = p->grid, rc = PCME_OK;
   float bms = 1e30f;
-  for (int v = 0; v < (p->hinted ? 1 : kNumVariants) && !rc; v++) {
+  for (int v = 0; v < ((p->hinted || p->mr) ? 1 : kNumVariants) && !rc; v++) {
     p->variant = v;
     for (int f : fr) {
       p->grid = std::max(1, std::min(p->ntickets, plan_resident(p) * f / 16));
@@ -1004,6 +1025,8 @@ int pcme_b_analyse_batch(const pcme_b_job *jobs, int njobs, pcme_b_out *out, voi
 }
 
 static_assert(sizeof(pcme_skip_job) == 120 && sizeof(pcme_skip_out) == 16, "pcme_skip_* layout");
+static_assert(sizeof(pcme_frame_job) == 184 && offsetof(pcme_frame_job, older) == 88 && sizeof(pcme_mb_out) == 128,
+              "pcme_frame_job / pcme_mb_out layout");
 
 int pcme_skip_gate_batch(const pcme_skip_job *jobs, int njobs, pcme_skip_out *out, void *stream) {
   if (njobs < 0 || (njobs > 0 && (!jobs || !out)) || ((uintptr_t)out & 3u))

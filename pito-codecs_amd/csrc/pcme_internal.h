@@ -22,7 +22,8 @@ struct DevJob {
   int mbw, mbh, stride;
   int me_range, subpel_rounds, partitions;
   uint32_t lam;
-  int pad_;
+  int nref;                    // references of list 0 (1..PCME_MAX_REFS)
+  const uint8_t *opl[3][4];    // planes of references 1..nref-1
 };
 
 // Ticket queues of the frame kernels (one per XCD) and the size of a
@@ -66,6 +67,10 @@ __global__ void search_frames_kernel(const DevJob *__restrict__ jobs, const int2
                                      int ntickets, uint32_t *ctl, uint32_t spin_limit);
 __global__ void search_frames_hinted_kernel(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
                                             int ntickets, uint32_t *ctl, uint32_t spin_limit);
+__global__ void search_frames_mr_kernel(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
+                                        int ntickets, uint32_t *ctl, uint32_t spin_limit);
+__global__ void search_frames_mr_hinted_kernel(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
+                                               int ntickets, uint32_t *ctl, uint32_t spin_limit);
 template <int WPG, int WCU, bool REUSE>
 __global__ void search_frames_var_kernel(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
                                          int ntickets, uint32_t *ctl, uint32_t spin_limit);

@@ -980,12 +980,18 @@ __device__ __forceinline__ int mv_y(uint32_t p) { return (int)p >> 16; }
 
 // Neighbour state of the MB being analysed in the ME-only field: every
 // coded MB is PInter with ref 0, so an available neighbour has ref 0.
+// With several references (the multi-reference field, one reference index
+// per MB) the neighbours carry their MB's index and rf is the index being
+// searched (the current MB's earlier partitions have it); with one they
+// are all zero, constants the compiler folds away.
 struct Nbr {
   bool aL, aT, aTR, aTL;
   uint32_t L[4];  // left MB, blocks 3, 7, 11, 15
   uint32_t T[4];  // top MB, blocks 12..15
   uint32_t TR;    // top-right MB, block 12
   uint32_t TL;    // top-left MB, block 15
+  int rL, rT, rTR, rTL;  // the neighbours' reference indices
+  int rf;                // reference index of the search
 };
 
 struct NbRes {
@@ -1026,22 +1032,25 @@ __device__ __forceinline__ NbRes nb(const Nbr &n, const Cur &cur) {
   if constexpr (BX >= 0 && BX < 4 && BY >= 0 && BY < 4) {
     constexpr int k = part_of(PART, BY * 4 + BX);
     if constexpr (k < P) {
-      r.avail = true; r.ref = 0; r.mv = cur.mv[k];
+      r.avail = true; r.ref = n.rf; r.mv = cur.mv[k];
     }
   } else if constexpr (BY >= 4 || (BY >= 0 && BX >= 4)) {
   } else if constexpr (BY < 0) {
     if constexpr (BX < 0) {
-      if (n.aTL) { r.avail = true; r.ref = 0; r.mv = n.TL; }
+      if (n.aTL) { r.avail = true; r.ref = n.rTL; r.mv = n.TL; }
     } else if constexpr (BX < 4) {
-      if (n.aT) { r.avail = true; r.ref = 0; r.mv = n.T[BX]; }
+      if (n.aT) { r.avail = true; r.ref = n.rT; r.mv = n.T[BX]; }
     } else {
-      if (n.aTR) { r.avail = true; r.ref = 0; r.mv = n.TR; }
+      if (n.aTR) { r.avail = true; r.ref = n.rTR; r.mv = n.TR; }
     }
   } else {
-    if (n.aL) { r.avail = true; r.ref = 0; r.mv = n.L[BY]; }
+    if (n.aL) { r.avail = true; r.ref = n.rL; r.mv = n.L[BY]; }
   }
   return r;
 }
+
+__device__ __forceinline__ int search_rf(const Nbr &n) { return n.rf; }
+__device__ __forceinline__ int search_rf(const NbrG &) { return 0; }
 
 template <int PART, int P, int BX, int BY>
 __device__ __forceinline__ NbRes nb(const NbrG &n, const Cur &cur) {
@@ -1072,7 +1081,8 @@ __device__ __forceinline__ int median3(int a, int b, int c) {
   return mx < t ? mx : t;
 }
 
-// mvp (slice.rs:434-482) with rf = 0.  SHAPE: 0 Normal, 1 Top16x8,
+// mvp (slice.rs:434-482) for reference index rf (search_rf: 0 except in
+// the multi-reference field).  SHAPE: 0 Normal, 1 Top16x8,
 // 2 Bottom16x8, 3 Left8x16, 4 Right8x16.  Returns a packed MV.
 template <int PART, int P, int X, int Y, int WB, int SHAPE, class N>
 __device__ __forceinline__ uint32_t mvp(const N &n, const Cur &cur) {
@@ -1084,7 +1094,7 @@ __device__ __forceinline__ uint32_t mvp(const N &n, const Cur &cur) {
   if (!a.avail) { a.ref = -1; a.mv = 0u; }
   if (!b.avail) { b.ref = -1; b.mv = 0u; }
   if (!cc.avail) { cc.ref = -1; cc.mv = 0u; }
-  const int rf = 0;
+  const int rf = search_rf(n);
   if (SHAPE == 1 && b.ref == rf) return b.mv;
   if (SHAPE == 2 && a.ref == rf) return a.mv;
   if (SHAPE == 3 && a.ref == rf) return a.mv;
@@ -1094,8 +1104,10 @@ __device__ __forceinline__ uint32_t mvp(const N &n, const Cur &cur) {
   return pack_mv(median3(mv_x(a.mv), mv_x(b.mv), mv_x(cc.mv)), median3(mv_y(a.mv), mv_y(b.mv), mv_y(cc.mv)));
 }
 
-// pskip_mv (slice.rs:484-494)
-__device__ __forceinline__ uint32_t pskip(const Nbr &n) {
+// pskip_mv (slice.rs:484-494): reference index 0
+__device__ __forceinline__ uint32_t pskip(const Nbr &nbr) {
+  Nbr n = nbr;
+  n.rf = 0;
   Cur cur = {};
   NbRes a = nb<0, 0, -1, 0>(n, cur);
   NbRes b = nb<0, 0, 0, -1>(n, cur);
@@ -1289,6 +1301,7 @@ __device__ __forceinline__ void take_if(PartRes &dst, const PartRes &src, bool t
 
 struct MbRes {
   int part;
+  int ref;  // the MB's reference index (multi-reference field; else 0)
   PartRes best;
   uint32_t cost[4];
   uint32_t best_cost;
@@ -1354,6 +1367,7 @@ __device__ __forceinline__ void analyse_mb(const SCtx &c, const Nbr &n, const ui
   search_part<0, PR>(c, n, src, sstride, mbx, mby, skip, hv, nh, r16, lds, ar);
   const uint32_t cost16 = r16.total;
   o.part = 0;
+  o.ref = 0;
   take_if(o.best, r16, true);
   o.best_cost = cost16 + (c.lam >> 8);
   o.cost[0] = cost16;
@@ -1376,6 +1390,99 @@ __device__ __forceinline__ void analyse_mb(const SCtx &c, const Nbr &n, const ui
   }
 }
 
+// (lam * te(v) bits of reference index r among nref) >> 8: no bits with one
+// reference, one with two, else ue(v).
+__device__ __forceinline__ uint32_t ref_rate(uint32_t lam, int r, int nref) {
+  const uint32_t bits = nref <= 1 ? 0u : (nref == 2 ? 1u : 2u * (31u - (uint32_t)__builtin_clz((uint32_t)r + 1u)) + 1u);
+  return __umul24(lam, bits) >> 8;
+}
+
+// Search context of reference index r >= 1 of job J (its planes; read
+// whole, as an older reference is complete).
+__device__ __forceinline__ SCtx older_ctx(const DevJob &J, const SCtx &c0, int r) {
+  SCtx c = c0;
+  const uint8_t *const *pl = J.opl[r - 1];
+  c.pl[0] = pl[0]; c.pl[1] = pl[1]; c.pl[2] = pl[2]; c.pl[3] = pl[3];
+  c.vlim = c.ph + 44;
+  return c;
+}
+
+// Multi-reference analysis of one MB (no counterpart in the reference, whose
+// P slices use one reference): the P16x16 search on each reference of the
+// job (a pruned MB: reference 0 only), each with the predictor of its index
+// and its index's rate added; the least total, the lower index on ties,
+// gives the MB's reference, and the partition modes are searched on it, each
+// partition paying the rate.  The LDS areas hold the regions of reference
+// ar_ref and are restaged when the reference changes.
+template <bool PR>
+__device__ __forceinline__ void analyse_mb_mr(const DevJob &J, const SCtx &c0, const Nbr &nb0, const uint8_t *src,
+                                              int mbx, int mby, uint32_t hv, int nh, int hpart, MbRes &o,
+                                              uint32_t *lds, Area *ar, int &ar_ref) {
+  Nbr n = nb0;
+  const uint32_t skip = pskip(n);
+  const int nsearch = PR ? 1 : J.nref;
+  PartRes b16;
+  uint32_t cost16 = 0u;
+  int rs = 0;
+  for (int r = 0; r < nsearch; r++) {
+    const SCtx cr = r == 0 ? c0 : older_ctx(J, c0, r);
+    if (ar_ref != r) {
+      ar[0] = no_area();
+      ar[1] = no_area();
+      ar_ref = r;
+    }
+    n.rf = r;
+    PartRes r16;
+    search_part<0, PR>(cr, n, src, 16, mbx, mby, skip, hv, nh, r16, lds, ar);
+    const uint32_t t = r16.total + ref_rate(c0.lam, r, J.nref);
+    const bool take = r == 0 || t < cost16;
+    cost16 = take ? t : cost16;
+    rs = take ? r : rs;
+    take_if(b16, r16, take);
+  }
+  const SCtx cs = rs == 0 ? c0 : older_ctx(J, c0, rs);
+  if (ar_ref != rs) {
+    ar[0] = no_area();
+    ar[1] = no_area();
+    ar_ref = rs;
+  }
+  n.rf = rs;
+  const uint32_t rr = ref_rate(c0.lam, rs, J.nref);
+  o.part = 0;
+  o.ref = rs;
+  take_if(o.best, b16, true);
+  o.best_cost = cost16 + (c0.lam >> 8);
+  o.cost[0] = cost16;
+  o.cost[1] = o.cost[2] = o.cost[3] = 0xFFFFFFFFu;
+  o.skip = skip;
+  o.m16 = b16.mv[0];
+  const bool split = J.partitions && cost16 > 1024u;
+  if (PR) {
+    if (split & ((hpart == 1) | (hpart == 2))) {
+      PartRes a, b;
+      search_part_12<PR>(cs, n, src, 16, mbx, mby, o.m16, hv, nh, a, b, lds, ar);
+      a.total += 2u * rr;
+      b.total += 2u * rr;
+      take_mode(o, a, 1, (c0.lam * 3u) >> 8);
+      take_mode(o, b, 2, (c0.lam * 3u) >> 8);
+    } else if (split & (hpart == 3)) {
+      PartRes a;
+      search_part<3, PR>(cs, n, src, 16, mbx, mby, o.m16, hv, nh, a, lds, ar);
+      a.total += 4u * rr;
+      take_mode(o, a, 3, (c0.lam * 10u) >> 8);
+    }
+  } else if (split) {
+    PartRes r1, r2, r3;
+    search_part_123(cs, n, src, 16, mbx, mby, o.m16, r1, r2, r3, lds, ar);
+    r1.total += 2u * rr;
+    r2.total += 2u * rr;
+    r3.total += 4u * rr;
+    take_mode(o, r1, 1, (c0.lam * 3u) >> 8);
+    take_mode(o, r2, 2, (c0.lam * 3u) >> 8);
+    take_mode(o, r3, 3, (c0.lam * 10u) >> 8);
+  }
+}
+
 // Packed MV of 4x4 block r after cand_inter_info (slice.rs:1751-1785).
 __device__ __forceinline__ uint32_t blk_mv(const MbRes &o, int r) {
   int k = o.part == 0 ? 0 : o.part == 1 ? (r >> 3) : o.part == 2 ? ((r & 3) >> 1) : (((r >> 3) << 1) | ((r & 3) >> 1));
@@ -1393,31 +1500,35 @@ __device__ __forceinline__ void store_out(pcme_mb_out *out, const MbRes &o) {
   const uint32_t vm = sel4(l < 16 ? kb : q, o.best.mv[0], o.best.mv[1], o.best.mv[2], o.best.mv[3]);
   const uint32_t vp = sel4(q, o.best.mvp[0], o.best.mvp[1], o.best.mvp[2], o.best.mvp[3]);
   const uint32_t vc = sel4(q, o.cost[0], o.cost[1], o.cost[2], o.cost[3]);
-  const uint32_t vx = sel4(q, o.best_cost, o.skip, o.m16, (uint32_t)pt);
+  const uint32_t vx = sel4(q, o.best_cost, o.skip, o.m16, (uint32_t)pt | ((uint32_t)o.ref << 8));
   const uint32_t v = l < 20 ? vm : (l < 24 ? vp : (l < 28 ? vc : vx));
   if (l < 32) ((uint32_t *)out)[l] = v;
 }
 
 // ------------------------------------------------------ frame search kernel
 
-// Exchange word: tag in bits 24-31, x in 23-12, y in 11-0 (12-bit two's
-// complement; legal MVs lie within +-2000).
+// Exchange word: the launch tag in bits 26-31, the MB's reference index in
+// 24-25, x in 23-12, y in 11-0 (12-bit two's complement; legal MVs lie
+// within +-2000).
+#define XTAG(v) ((v) >> 26)
 __device__ __forceinline__ int sext12(uint32_t v) { return (int)((v & 0xFFFu) ^ 0x800u) - 0x800; }
-__device__ __forceinline__ uint32_t xword(uint32_t tag, uint32_t mv) {
-  return (tag << 24) | (((uint32_t)mv_x(mv) & 0xFFFu) << 12) | ((uint32_t)mv_y(mv) & 0xFFFu);
+__device__ __forceinline__ uint32_t xword(uint32_t tag, uint32_t ref, uint32_t mv) {
+  return (tag << 26) | (ref << 24) | (((uint32_t)mv_x(mv) & 0xFFFu) << 12) | ((uint32_t)mv_y(mv) & 0xFFFu);
 }
 __device__ __forceinline__ uint32_t xword_mv(uint32_t w) { return pack_mv(sext12(w >> 12), sext12(w)); }
+__device__ __forceinline__ int xword_ref(uint32_t w) { return (int)((w >> 24) & 3u); }
 
 // Wait until the 4 exchange words of one MB carry `tag`; returns their MVs
-// (packed) in w[0..3] (uniform).  False on stall or on a raised error flag.
+// (packed) in w[0..3] and the MB's reference index in *ref (uniform).  False
+// on stall or on a raised error flag.
 __device__ __forceinline__ bool fetch_words(const uint32_t *xw, uint32_t tag, uint32_t *err,
-                                            uint32_t spin_limit, uint32_t w[4]) {
+                                            uint32_t spin_limit, uint32_t w[4], int *ref) {
   int l = (int)lane_id();
   uint32_t spins = 0;
   uint32_t v = 0;
   for (;;) {
     if (l < 4) v = __hip_atomic_load(xw + l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    bool ok = l >= 4 || (v >> 24) == tag;
+    bool ok = l >= 4 || XTAG(v) == tag;
     if (__ballot(!ok) == 0) break;
     if (++spins > spin_limit) {
       if (l == 0) {
@@ -1433,6 +1544,7 @@ __device__ __forceinline__ bool fetch_words(const uint32_t *xw, uint32_t tag, ui
     }
     __builtin_amdgcn_s_sleep(2);
   }
+  *ref = xword_ref(rdlane(v, 0));
   v = xword_mv(v);
 #pragma unroll
   for (int i = 0; i < 4; i++) w[i] = rdlane(v, i);
@@ -1448,12 +1560,12 @@ __device__ __forceinline__ uint32_t load_hint(const pcme_hint *h) {
   return v;
 }
 
-template <bool HINTS, bool REUSE = true>
+template <bool HINTS, bool REUSE = true, bool MR = false>
 __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
                                             int ntickets, uint32_t *ctl, uint32_t spin_limit, uint32_t *s_lds,
                                             uint32_t *s_src) {
   uint32_t *err = ctl + 1;
-  const uint32_t tag = ctl[2] & 0xFFu;  // this launch's epoch (set by prep_kernel)
+  const uint32_t tag = ctl[2] & 0x3Fu;  // this launch's epoch (set by prep_kernel)
   const int l = (int)lane_id();
   // Ticket queue q holds tickets [ctl[16+q], ctl[17+q]), taken in order
   // through the counter ctl[8+q].  Workgroups are dealt to the eight XCDs
@@ -1508,13 +1620,18 @@ __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, con
 #pragma unroll
     for (int i = 0; i < 4; i++) { n.L[i] = 0u; n.T[i] = 0u; }
     n.TR = n.TL = 0u;
+    n.rL = n.rT = n.rTR = n.rTL = n.rf = 0;
     // Row above: n.T = MB x, nxt = MB x+1; n.TL = block 15 of MB x-1.
+    // With one reference every index read stays in rdummy and n's indices
+    // are the constant 0.
     uint32_t nxt[4] = {0u, 0u, 0u, 0u};
+    int rnxt = 0, rdummy = 0;
     bool ok = true;
     if (my > 0) {
-      ok = fetch_words(xup, tag, err, spin_limit, n.T);
-      if (ok && mbw > 1) ok = fetch_words(xup + 4, tag, err, spin_limit, nxt);
+      ok = fetch_words(xup, tag, err, spin_limit, n.T, MR ? &n.rT : &rdummy);
+      if (ok && mbw > 1) ok = fetch_words(xup + 4, tag, err, spin_limit, nxt, MR ? &rnxt : &rdummy);
     }
+    int ar_ref = 0;  // the reference whose regions the areas hold (several references)
     if (!ok) return;
     // The MB's source block travels into LDS one MB ahead of its use (two
     // buffers), so no search of the row waits for a source load; the staged
@@ -1532,14 +1649,19 @@ __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, con
         n.TL = n.T[3];
 #pragma unroll
         for (int i = 0; i < 4; i++) n.T[i] = nxt[i];
+        if constexpr (MR) {
+          n.rTL = n.rT;
+          n.rT = rnxt;
+        }
         if (mx + 1 < mbw) {
-          if (!fetch_words(xup + (size_t)(mx + 1) * 4, tag, err, spin_limit, nxt)) return;
+          if (!fetch_words(xup + (size_t)(mx + 1) * 4, tag, err, spin_limit, nxt, MR ? &rnxt : &rdummy)) return;
         }
       }
       n.aT = my > 0;
       n.aTL = my > 0 && mx > 0;
       n.aTR = my > 0 && mx + 1 < mbw;
       n.TR = nxt[0];
+      if constexpr (MR) n.rTR = rnxt;
       dma_wait();  // this MB's source block
       if (mx + 1 < mbw) dma_block16(srow + 16 * (mx + 1), stride, s_src + 64 * ((mx + 1) & 1));
       const uint8_t *src = (const uint8_t *)(s_src + 64 * (mx & 1));
@@ -1560,20 +1682,28 @@ __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, con
         ar[0] = no_area();
         ar[1] = no_area();
       }
-      if (HINTS && nh >= 0)
-        analyse_mb<true>(c, n, src, 16, 16 * mx, 16 * my, J.partitions, hv, nh, hpart, o, s_lds, ar);
-      else
-        analyse_mb<false>(c, n, src, 16, 16 * mx, 16 * my, J.partitions, hv, nh, hpart, o, s_lds, ar);
+      if constexpr (MR) {
+        if (HINTS && nh >= 0)
+          analyse_mb_mr<true>(J, c, n, src, 16 * mx, 16 * my, hv, nh, hpart, o, s_lds, ar, ar_ref);
+        else
+          analyse_mb_mr<false>(J, c, n, src, 16 * mx, 16 * my, hv, nh, hpart, o, s_lds, ar, ar_ref);
+      } else {
+        if (HINTS && nh >= 0)
+          analyse_mb<true>(c, n, src, 16, 16 * mx, 16 * my, J.partitions, hv, nh, hpart, o, s_lds, ar);
+        else
+          analyse_mb<false>(c, n, src, 16, 16 * mx, 16 * my, J.partitions, hv, nh, hpart, o, s_lds, ar);
+      }
       store_out(J.out + (size_t)my * mbw + mx, o);
       // Publish the bottom row (blocks 12..15) for the row below.
       if (l < 4) {
-        uint32_t wv = xword(tag, blk_mv(o, 12 + l));
+        uint32_t wv = xword(tag, MR ? (uint32_t)o.ref : 0u, blk_mv(o, 12 + l));
         __hip_atomic_store(xrow + (size_t)mx * 4 + l, wv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
       // Left column (blocks 3, 7, 11, 15) for the next MB of this row.
       n.aL = true;
 #pragma unroll
       for (int i = 0; i < 4; i++) n.L[i] = blk_mv(o, 4 * i + 3);
+      if constexpr (MR) n.rL = o.ref;
     }
   }
 }
@@ -1600,6 +1730,22 @@ search_frames_hinted_kernel(const DevJob *__restrict__ jobs, const int2 *__restr
                             uint32_t *ctl, uint32_t spin_limit) {
   __shared__ __attribute__((aligned(16))) uint32_t s_lds[FRAME_LDS_WORDS];
   frames_body<true>(jobs, tickets, ntickets, ctl, spin_limit, s_lds, s_lds + LDS_PER_WAVE / 4);
+}
+
+// The multi-reference field (jobs with nref > 1; the same walk, each MB
+// searched on every reference): full search, and with pruner hints.
+__global__ void __launch_bounds__(64) search_frames_mr_kernel(const DevJob *__restrict__ jobs,
+                                                              const int2 *__restrict__ tickets, int ntickets,
+                                                              uint32_t *ctl, uint32_t spin_limit) {
+  __shared__ __attribute__((aligned(16))) uint32_t s_lds[FRAME_LDS_WORDS];
+  frames_body<false, true, true>(jobs, tickets, ntickets, ctl, spin_limit, s_lds, s_lds + LDS_PER_WAVE / 4);
+}
+
+__global__ void __launch_bounds__(64) search_frames_mr_hinted_kernel(const DevJob *__restrict__ jobs,
+                                                                     const int2 *__restrict__ tickets, int ntickets,
+                                                                     uint32_t *ctl, uint32_t spin_limit) {
+  __shared__ __attribute__((aligned(16))) uint32_t s_lds[FRAME_LDS_WORDS];
+  frames_body<true, true, true>(jobs, tickets, ntickets, ctl, spin_limit, s_lds, s_lds + LDS_PER_WAVE / 4);
 }
 
 // Configurations of the full-search kernel for the autotuner: WPG row
@@ -2322,11 +2468,11 @@ __global__ void __launch_bounds__(256) pred_frame_kernel(const uint8_t *__restri
 __global__ void prep_kernel(uint32_t *ctl) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     uint32_t e = ctl[2] + 1u;
-    if ((e & 0xFFu) == 0u) e += 1u;
+    if ((e & 0x3Fu) == 0u) e += 1u;  // tag 0 is a cleared word
     ctl[0] = 0u;
     ctl[1] = 0u;
     ctl[2] = e;
-    ctl[3] = (e & 0xFFu) == 1u ? 1u : 0u;
+    ctl[3] = (e & 0x3Fu) == 1u ? 1u : 0u;  // the tags wrapped: clear the words
   }
   if (blockIdx.x == 0 && threadIdx.x < PCME_QUEUES) ctl[8 + threadIdx.x] = 0u;  // per-queue ticket counters
 }

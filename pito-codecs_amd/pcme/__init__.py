@@ -52,11 +52,23 @@ class Params(ctypes.Structure):
                 ("partitions", ctypes.c_int32), ("qp", ctypes.c_int32)]
 
 
+class RefPlanes(ctypes.Structure):
+    _fields_ = [("luma", ctypes.c_void_p), ("h", ctypes.c_void_p), ("v", ctypes.c_void_p), ("c", ctypes.c_void_p)]
+
+
+MAX_REFS = 4
+
+
 class FrameJob(ctypes.Structure):
     _fields_ = [("src", ctypes.c_void_p), ("ref", ctypes.c_void_p),
                 ("ref_h", ctypes.c_void_p), ("ref_v", ctypes.c_void_p),
                 ("ref_c", ctypes.c_void_p), ("out", ctypes.c_void_p),
-                ("hint", ctypes.c_void_p), ("prm", Params)]
+                ("hint", ctypes.c_void_p), ("prm", Params),
+                ("nref", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("older", RefPlanes * (MAX_REFS - 1))]
+
+
+assert ctypes.sizeof(FrameJob) == 184
 
 
 class SearchJob(ctypes.Structure):
@@ -426,11 +438,22 @@ class DeviceClip:
 
 
 def frame_job(clip: DeviceClip, cur: int, ref: int, out_ptr: int, me_range: int, subpel_rounds: int,
-              partitions: bool, qp: int, ref_clip: DeviceClip | None = None, hint_ptr: int | None = None) -> FrameJob:
+              partitions: bool, qp: int, ref_clip: DeviceClip | None = None, hint_ptr: int | None = None,
+              older: list[int] | tuple = ()) -> FrameJob:
+    """One P frame: picture `cur` of `clip` against picture `ref` (of
+    ref_clip, default the same clip); `older` lists further reference
+    pictures of the clip (reference indices 1, 2, ...), for the
+    multi-reference search."""
     rc = clip if ref_clip is None else ref_clip
-    return FrameJob(clip.plane_ptr(0, cur), rc.plane_ptr(0, ref), rc.plane_ptr(1, ref),
-                    rc.plane_ptr(2, ref), rc.plane_ptr(3, ref), out_ptr, hint_ptr,
-                    Params(clip.mbw, clip.mbh, me_range, subpel_rounds, 1 if partitions else 0, qp))
+    if len(older) > MAX_REFS - 1:
+        raise PcmeError(PCME_EINVAL, f"at most {MAX_REFS} references")
+    j = FrameJob(clip.plane_ptr(0, cur), rc.plane_ptr(0, ref), rc.plane_ptr(1, ref),
+                 rc.plane_ptr(2, ref), rc.plane_ptr(3, ref), out_ptr, hint_ptr,
+                 Params(clip.mbw, clip.mbh, me_range, subpel_rounds, 1 if partitions else 0, qp))
+    j.nref = 1 + len(older)
+    for k, t in enumerate(older):
+        j.older[k] = RefPlanes(rc.plane_ptr(0, t), rc.plane_ptr(1, t), rc.plane_ptr(2, t), rc.plane_ptr(3, t))
+    return j
 
 
 def search_frames(jobs: list[FrameJob], stream=None, sync: bool = True) -> None:
@@ -547,6 +570,25 @@ def pred_frame(clip: DeviceClip, ref: int, mb_ptr: int, pred_y_ptr: int, pred_u_
                                clip.plane_ptr(3, ref), clip.cplanes[0, ref].data_ptr(),
                                clip.cplanes[1, ref].data_ptr(), mb_ptr, clip.mbw, clip.mbh,
                                pred_y_ptr, pred_u_ptr, pred_v_ptr, _stream_ptr(stream)))
+
+
+def search_clip_mr(clip: DeviceClip, nref: int, me_range: int, subpel_rounds: int, partitions: bool, qp: int,
+                   stream=None, prepared: bool = False):
+    """Multi-reference ME-only field of every P frame of an IPPP clip: frame
+    t against its min(t, nref) previous frames, the newest first.  Returns
+    the (n-1, mbh, mbw) MB_DTYPE records."""
+    torch = clip.torch
+    if not prepared:
+        clip.pad(stream)
+        clip.hpel(0, clip.n - 1, stream)
+    nb = clip.n - 1
+    out = torch.empty((max(nb, 1), clip.mbh * clip.mbw * MB_DTYPE.itemsize), dtype=torch.uint8,
+                      device=clip.raw.device)
+    jobs = [frame_job(clip, t, t - 1, out[t - 1].data_ptr(), me_range, subpel_rounds, partitions, qp,
+                      older=[t - 1 - k for k in range(1, min(t, nref))])
+            for t in range(1, clip.n)]
+    search_frames(jobs, stream, sync=True)
+    return out[:nb].cpu().numpy().view(MB_DTYPE).reshape(nb, clip.mbh, clip.mbw)
 
 
 def search_clip_me(clip: DeviceClip, me_range: int, subpel_rounds: int, partitions: bool, qp: int,
