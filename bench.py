@@ -39,8 +39,9 @@ METRIC = "1080p H.264 ME Mblocks/s: plain GPU vs CNN-pruned at <=1% BD-rate; GB/
 HBM_PEAK_GBS = 8000.0
 # Integer VALU peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3 T lane-op/s
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
-# Algorithmic HBM bytes of the search per MB: 256 B source luma, 4 x 256 B
-# reference planes read once, 128 B record, 32 B exchange words.
+# Algorithmic HBM bytes of the search per MB (one reference): 256 B source
+# luma, 4 x 256 B reference planes read once, 128 B record, 32 B exchange
+# words; each further reference adds its four planes.
 BYTES_PER_MB = 256 + 4 * 256 + 128 + 32
 # Pruner network multiply-adds per MB (include/pcme_net.h): conv1 144 px x
 # 32 x 18, conv2 36 x 32 x 288, conv3 36 x 32 x 288, conv4 9 x 32 x 288,
@@ -71,6 +72,8 @@ def parse():
     ap.add_argument("--no-pruned", action="store_true", help="skip the CNN-pruned leg")
     ap.add_argument("--no-lookahead", action="store_true", help="skip the lowres-lookahead leg")
     ap.add_argument("--no-b", action="store_true", help="skip the B-picture search leg")
+    ap.add_argument("--nref", type=int, default=0,
+                    help="references of list 0 per P frame (0: the workload's own, 4 for 4k, else 1)")
     ap.add_argument("--ncand", type=int, default=5, help="pruner candidates per MB")
     ap.add_argument("--weights", default=None, help="pruner weights (.npz; default: the committed ones)")
     ap.add_argument("--thr", type=float, default=0.7, help="pruner confidence threshold (partition head)")
@@ -85,13 +88,13 @@ def parse():
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "kernels_1080p_traffic.md")
 
 
-def measured_traffic(workload, reuse_regions=1):
+def measured_traffic(workload, reuse_regions=1, nref=1):
     """HBM-side bytes per launch of the search kernel on the 1080p workload
     in the given LDS-region-reuse configuration, from the FETCH_SIZE /
     WRITE_SIZE runs summarised in profiles/ (scaled by a 1 GiB copy measured
     in the same runs); None for other workloads or when the summary is
     absent."""
-    if workload != "1080p" or not os.path.exists(TRAFFIC_SUMMARY):
+    if workload != "1080p" or nref != 1 or not os.path.exists(TRAFFIC_SUMMARY):
         return None
     key = f"`search_frames_kernel` reuse_regions={int(bool(reuse_regions))}: bytes_per_launch="
     with open(TRAFFIC_SUMMARY) as f:
@@ -130,6 +133,12 @@ def measured_bd_rate(ncand, thr_part, thr_mv, workload):
             "max_pct": max(max(c, q) for v in sizes.values() for c, q in v),
             "coder": "field coder (pcme/h264enc.py), Baseline CAVLC bitstreams decoded bit-exact, QP 22/27/32/37",
             "source": "profiles/pruner_bdrate_bitstreams.md"}
+
+
+def older_refs(t, nref):
+    """Pictures of references 1..nref-1 of P frame t (the frames before
+    t - 1, nearest first; fewer at the clip's start)."""
+    return [t - 1 - k for k in range(1, min(t, nref))]
 
 
 def clip_specs(pcme, workload):
@@ -172,6 +181,10 @@ def main():
         del a, b
 
     specs = clip_specs(pcme, args.workload)
+    if args.nref == 0:
+        args.nref = 4 if args.workload == "4k" else 1
+    if not 1 <= args.nref <= pcme.MAX_REFS:
+        sys.exit(f"bench.py: --nref must be 1..{pcme.MAX_REFS}")
     threads = cpu_share()
     t0 = time.time()
     clips, outs, jobs, host = [], [], [], []
@@ -180,7 +193,8 @@ def main():
         mbw, mbh = pcme.mb_dims(spec.w, spec.h)
         dc = pcme.DeviceClip(y, mbw, mbh)
         o = torch.empty((spec.frames - 1, mbw * mbh * pcme.MB_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
-        jobs += [pcme.frame_job(dc, t, t - 1, o[t - 1].data_ptr(), rng, 1, True, qp) for t in range(1, spec.frames)]
+        jobs += [pcme.frame_job(dc, t, t - 1, o[t - 1].data_ptr(), rng, 1, True, qp, older=older_refs(t, args.nref))
+                 for t in range(1, spec.frames)]
         clips.append(dc)
         outs.append(o)
         host.append((y, mbw, mbh, qp, rng))
@@ -279,12 +293,16 @@ def main():
               file=sys.stderr, flush=True)
         p1.close()
     reuse = tune["reuse_regions"] if tune else 1  # the plan's default keeps regions in LDS
-    achieved_gbs = BYTES_PER_MB * mbs_per_step / (search_ms * 1e-3) / 1e9
-    wl = {"1080p": "plain GPU ME-only search, 1920x1080 60 frames IPPP, 1 ref, QP 27, me_range 32 + qpel, "
+    bytes_per_mb = BYTES_PER_MB + 4 * 256 * (args.nref - 1)
+    achieved_gbs = bytes_per_mb * mbs_per_step / (search_ms * 1e-3) / 1e9
+    kname = "search_frames_mr_kernel" if args.nref > 1 else "search_frames_kernel"
+    refs = f"{args.nref} ref" + ("s" if args.nref > 1 else "")
+    wl = {"1080p": f"plain GPU ME-only search, 1920x1080 60 frames IPPP, {refs}, QP 27, me_range 32 + qpel, "
                    "partitions 16x16/16x8/8x16/8x8, 12 rects 64-400 px, v in [-8,8] px/frame",
           "batch": "plain GPU ME-only search, 64 clips (16 each of 352x288, 640x360, 1280x720, 1920x1080), "
-                   "30 frames each IPPP, QP per clip from {22,27,32,37}, me_range 32, one persistent launch",
-          "4k": "plain GPU ME-only search, 3840x2160 120 frames IPPP, 1 ref, QP 27, me_range 64 + qpel, "
+                   f"30 frames each IPPP, {refs}, QP per clip from {{22,27,32,37}}, me_range 32, one persistent "
+                   "launch",
+          "4k": f"plain GPU ME-only search, 3840x2160 120 frames IPPP, {refs}, QP 27, me_range 64 + qpel, "
                 "8 rects 128-900 px, v in [-16,16] px/frame, 0.2%/frame zoom"}[args.workload]
     line = {
         "metric": METRIC, "value": round(value, 4), "unit": "Mblocks/s", "n_gpus": 1,
@@ -294,12 +312,12 @@ def main():
         "config": {"workload": wl, "mbs_per_step": mbs_per_step, "p_frames": len(jobs),
                    "parallelism": "single GPU"},
         "roofline": None,  # set below from the oracle's operation counts (cpu_baseline leg)
-        "hbm_roofline": {"bound": "hbm", "kernel": "search_frames_kernel", "achieved": round(achieved_gbs, 2),
+        "hbm_roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved_gbs, 2),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
-                         "traffic": measured_traffic(args.workload, reuse),
+                         "traffic": measured_traffic(args.workload, reuse, args.nref),
                          "traffic_source": "profiles/kernels_1080p_traffic.md (FETCH_SIZE/WRITE_SIZE runs)"
-                         if measured_traffic(args.workload, reuse) is not None else None,
-                         "algorithmic_bytes_per_mb": BYTES_PER_MB, "kernel_ms": round(search_ms, 3)},
+                         if measured_traffic(args.workload, reuse, args.nref) is not None else None,
+                         "algorithmic_bytes_per_mb": bytes_per_mb, "kernel_ms": round(search_ms, 3)},
         "autotune": tune,
         "graph": graph,
         "critical_path": crit,
@@ -316,10 +334,10 @@ def main():
     if not args.no_cpu:
         cpu, per_mb_ops, parity = cpu_leg(args, pcme, host[-1], outs[-1])
         valu_tops = per_mb_ops * mbs_per_step / (search_ms * 1e-3) / 1e12
-        line["roofline"] = {"bound": "valu", "kernel": "search_frames_kernel", "achieved": round(valu_tops, 3),
+        line["roofline"] = {"bound": "valu", "kernel": kname, "achieved": round(valu_tops, 3),
                             "peak": round(VALU_PEAK_TOPS, 2), "unit": "T lane-op/s",
                             "frac": round(valu_tops / VALU_PEAK_TOPS, 4),
-                            "traffic": measured_traffic(args.workload, reuse),
+                            "traffic": measured_traffic(args.workload, reuse, args.nref),
                             "lane_ops_per_mb": round(per_mb_ops, 1), "mbs_per_launch": mbs_per_step,
                             "kernel_ms": round(search_ms, 3),
                             "peak_basis": "256 CU x 64 lanes x 2.4 GHz, one 32-bit lane-op per lane per clock"}
@@ -379,23 +397,25 @@ def cpu_leg(args, pcme, host, out):
     y, mbw, mbh, qp, rng = host
     nP = y.shape[0] - 1
     k = min(args.cpu_frames, nP)
-    src = [O.pad_luma(y[t], mbw, mbh) for t in range(1, k + 1)]
-    ref = [O.pad_luma(y[t - 1], mbw, mbh) for t in range(1, k + 1)]
+    pads = [O.pad_luma(y[t], mbw, mbh) for t in range(0, k + 1)]
+    src = pads[1:k + 1]
+    ref = pads[0:k]
+    older = [[pads[u] for u in older_refs(t, args.nref)] for t in range(1, k + 1)]
     nproc = len(os.sched_getaffinity(0))
     threads = max(1, min(cpu_share(), k))
-    res, _, ctr = O.search_jobs_mt(src, ref, mbw, mbh, rng, 1, True, qp, threads, count=True)
+    res, _, ctr = O.search_jobs_mt(src, ref, mbw, mbh, rng, 1, True, qp, threads, count=True, older=older)
     gpu_res = out.cpu().numpy().view(pcme.MB_DTYPE).reshape(nP, mbh, mbw)
     parity = all(np.array_equal(np.ascontiguousarray(gpu_res[i]).view(np.uint8),
                                 np.ascontiguousarray(res[i]).view(np.uint8)) for i in range(k))
     # Lane-op count of the same work (the GPU replays the oracle's path):
     # SAD 0.25/px, MC+SATD 3.75/px.
     per_mb_ops = (0.25 * ctr.sad_px + 3.75 * ctr.satd_px) / max(ctr.mbs, 1)
-    runs = sorted(O.search_jobs_mt(src, ref, mbw, mbh, rng, 1, True, qp, threads, count=False)[1]
+    runs = sorted(O.search_jobs_mt(src, ref, mbw, mbh, rng, 1, True, qp, threads, count=False, older=older)[1]
                   for _ in range(3))
     secs = runs[1]
     k1 = min(args.cpu_frames_1t, k)
-    runs1 = sorted(O.search_jobs_mt(src[:k1], ref[:k1], mbw, mbh, rng, 1, True, qp, 1, count=False)[1]
-                   for _ in range(3))
+    runs1 = sorted(O.search_jobs_mt(src[:k1], ref[:k1], mbw, mbh, rng, 1, True, qp, 1, count=False,
+                                    older=older[:k1])[1] for _ in range(3))
     cpu_val = k * mbw * mbh / secs / 1e6
     one_val = k1 * mbw * mbh / runs1[1] / 1e6
     print(f"[bench] cpu baseline: {cpu_val:.3f} Mblocks/s on {threads} threads ({nproc} in the affinity mask), "
@@ -647,7 +667,8 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
         nb, nmb = dc.n - 1, dc.mbw * dc.mbh
         o = torch.empty((nb, nmb * pcme.MB_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
         h = torch.zeros((nb, nmb * pcme.HINT_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
-        js = [pcme.frame_job(dc, t, t - 1, o[t - 1].data_ptr(), rng, 1, True, qp, hint_ptr=h[t - 1].data_ptr())
+        js = [pcme.frame_job(dc, t, t - 1, o[t - 1].data_ptr(), rng, 1, True, qp, hint_ptr=h[t - 1].data_ptr(),
+                             older=older_refs(t, args.nref))
               for t in range(1, dc.n)]
         groups.setdefault(qp, []).extend(js)
         jobs += js

@@ -50,6 +50,13 @@ int pcme_h264enc_picture(pcme_h264enc *enc, const uint8_t *y, int32_t ystride, c
 int pcme_h264enc_recon(pcme_h264enc *enc, uint8_t *y, int32_t ystride, uint8_t *u, uint8_t *v,
                        int32_t cstride);
 
+/* Reference indices of the following P pictures (1..PCME_MAX_REFS, default
+ * 1): list 0 is the last nref reconstructions, newest first (fewer after an
+ * IDR picture), num_ref_idx_l0_active the count in use, and each MB's
+ * pcme_mb_out.ref names its reference (coded as te(v); P_Skip only with
+ * index 0).  A record naming an index beyond those in use is refused. */
+int pcme_h264enc_set_refs(pcme_h264enc *enc, int32_t nref);
+
 /* Macroblock counts of the last coded picture: P_Skip, other P, I_PCM. */
 int pcme_h264enc_stats(pcme_h264enc *enc, int32_t *skipped, int32_t *inter, int32_t *pcm);
 

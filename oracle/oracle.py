@@ -51,7 +51,8 @@ class ocounters(ctypes.Structure):
 class ojob(ctypes.Structure):
     _fields_ = [("src", ctypes.c_void_p), ("ref", ctypes.c_void_p), ("out", ctypes.c_void_p),
                 ("mbw", ctypes.c_int), ("mbh", ctypes.c_int), ("me_range", ctypes.c_int),
-                ("subpel_rounds", ctypes.c_int), ("partitions", ctypes.c_int), ("qp", ctypes.c_int)]
+                ("subpel_rounds", ctypes.c_int), ("partitions", ctypes.c_int), ("qp", ctypes.c_int),
+                ("nref", ctypes.c_int), ("older", ctypes.c_void_p * 3)]
 
 
 # omb_out has the layout of pcme_mb_out
@@ -339,18 +340,23 @@ def search_clip_me(y_frames: np.ndarray, me_range: int, subpel_rounds: int, part
     return np.stack(res) if res else np.zeros((0, mbh, mbw), OMB_DTYPE)
 
 
-def search_jobs_mt(srcs, refs, mbw, mbh, me_range, subpel_rounds, partitions, qp, threads, count=True):
+def search_jobs_mt(srcs, refs, mbw, mbh, me_range, subpel_rounds, partitions, qp, threads, count=True,
+                   older=None):
     """Frame-threaded CPU run (half-pel build + search per job); returns
     (results, wall_seconds, counters); with count False the run keeps no
-    counters (a timed run) and the counters returned are zero."""
+    counters (a timed run) and the counters returned are zero.  older[i]:
+    padded luma of job i's further references (indices 1, 2, ...)."""
     lib = load()
     n = len(srcs)
     outs = [np.zeros(mbw * mbh, OMB_DTYPE) for _ in range(n)]
     keep = [np.ascontiguousarray(a) for a in list(srcs) + list(refs)]
+    olds = [[np.ascontiguousarray(a) for a in (older[i] if older else [])] for i in range(n)]
     jobs = (ojob * n)()
     for i in range(n):
         jobs[i] = ojob(_p(keep[i]), _p(keep[n + i]), _p(outs[i]), mbw, mbh, me_range, subpel_rounds,
-                       1 if partitions else 0, qp)
+                       1 if partitions else 0, qp, 1 + len(olds[i]))
+        for k, a in enumerate(olds[i]):
+            jobs[i].older[k] = _p(a)
     ctr = ocounters()
     secs = lib.o_search_jobs_mt(jobs, n, threads, ctypes.byref(ctr) if count else None)
     return [o.reshape(mbh, mbw) for o in outs], secs, ctr

@@ -873,12 +873,19 @@ static void *worker(void *arg) {
     int stride = 16 * jb->mbw + 2 * O_LUMA_PAD;
     int rows = 16 * jb->mbh + 2 * O_LUMA_PAD;
     size_t n = (size_t)stride * rows;
-    uint8_t *hp = (uint8_t *)malloc(3 * n);
-    o_hpel(jb->ref, stride, rows, hp, hp + n, hp + 2 * n);
-    oref r = {{jb->ref, hp, hp + n, hp + 2 * n}, stride, O_LUMA_PAD};
-    o_search_frame(jb->mbw, jb->mbh, jb->me_range, jb->subpel_rounds,
-                   jb->partitions, o_lambda(jb->qp), jb->src, stride,
-                   O_LUMA_PAD, &r, jb->out, pl->count ? &c : NULL);
+    const int nref = jb->nref < 1 ? 1 : (jb->nref > 4 ? 4 : jb->nref);
+    uint8_t *hp = (uint8_t *)malloc(3 * n * (size_t)nref);
+    oref r[4];
+    for (int k = 0; k < nref; k++) {
+      const uint8_t *luma = k == 0 ? jb->ref : jb->older[k - 1];
+      uint8_t *h = hp + 3 * n * (size_t)k;
+      o_hpel(luma, stride, rows, h, h + n, h + 2 * n);
+      oref rk = {{luma, h, h + n, h + 2 * n}, stride, O_LUMA_PAD};
+      r[k] = rk;
+    }
+    o_search_frame_mr(jb->mbw, jb->mbh, jb->me_range, jb->subpel_rounds,
+                      jb->partitions, o_lambda(jb->qp), jb->src, stride,
+                      O_LUMA_PAD, r, nref, NULL, jb->out, pl->count ? &c : NULL);
     free(hp);
   }
   pthread_mutex_lock(&pl->mu);

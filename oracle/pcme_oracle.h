@@ -208,6 +208,8 @@ typedef struct {
   const uint8_t *ref;   /* padded reference luma (edges extended) */
   omb_out *out;         /* mbw*mbh */
   int mbw, mbh, me_range, subpel_rounds, partitions, qp;
+  int nref;                    /* 0 or 1: `ref` only; else o_search_frame_mr */
+  const uint8_t *older[3];     /* padded luma of references 1..nref-1 */
 } ojob;
 double o_search_jobs_mt(const ojob *jobs, int njobs, int threads,
                         ocounters *ctr);

@@ -2018,7 +2018,8 @@ struct pcme_h264enc {
   Sps sps;
   Pps pps;
   pcme_h264dec d;                 // picture under construction, MB records, reference list
-  std::unique_ptr<Picture> ref;   // the previous reconstruction
+  std::deque<std::unique_ptr<Picture>> dpb;  // reconstructions of the reference pictures, newest first
+  int nref = 1;                   // reference indices of the following P pictures (set_refs)
   int frame_num = 0;
   bool started = false;
   int n_skip = 0, n_inter = 0, n_pcm = 0;
@@ -2052,7 +2053,7 @@ void write_sps_pps(Enc *e, std::vector<uint8_t> &out) {
   b.ue(0);       // seq_parameter_set_id
   b.ue(12);      // log2_max_frame_num_minus4: 16 bits
   b.ue(2);       // pic_order_cnt_type
-  b.ue(1);       // max_num_ref_frames
+  b.ue((uint32_t)e->sps.max_num_ref_frames);
   b.u(1, 0);     // gaps_in_frame_num_value_allowed_flag
   b.ue((uint32_t)e->mbw - 1);
   b.ue((uint32_t)e->mbh - 1);
@@ -2108,13 +2109,15 @@ void begin_picture(Enc *e) {
   d.mbs.assign((size_t)e->mbw * e->mbh, Mb());
   d.slices.assign(1, SliceInfo());
   d.list0.clear();
-  if (e->ref) d.list0.push_back(e->ref.get());
+  for (const auto &r : e->dpb) d.list0.push_back(r.get());  // short-term, descending frame_num
   e->n_skip = e->n_inter = e->n_pcm = 0;
 }
 
 void end_picture(Enc *e) {
   deblock_picture(&e->d);
-  e->ref = std::move(e->d.cur);
+  // sliding window (8.2.5.3) over max_num_ref_frames pictures
+  e->dpb.push_front(std::move(e->d.cur));
+  while ((int)e->dpb.size() > e->sps.max_num_ref_frames) e->dpb.pop_back();
   e->frame_num = (e->frame_num + 1) & 0xFFFF;
 }
 
@@ -2167,20 +2170,31 @@ int code_idr(Enc *e, int qp, BitWriter &b) {
   return PCME_OK;
 }
 
+// te(v) (9.1): one inverted bit when the range is 1, else ue(v).
+void te(BitWriter &b, int range, uint32_t v) {
+  if (range == 1)
+    b.u(1, v ? 0u : 1u);
+  else
+    b.ue(v);
+}
+
 int code_p(Enc *e, const pcme_mb_out *field, int qp, BitWriter &b) {
+  pcme_h264dec *d = &e->d;
+  const int nra = std::min(e->nref, (int)d->list0.size());  // num_ref_idx_l0_active
+  for (int i = 0; i < e->mbw * e->mbh; i++)
+    if (field[i].ref >= nra) return fail("MB %d: reference index %d beyond the %d in use", i, field[i].ref, nra);
   b.ue(0);        // first_mb_in_slice
   b.ue(5);        // slice_type: P (all slices)
   b.ue(0);        // pic_parameter_set_id
   b.u(16, (uint32_t)e->frame_num);
-  b.u(1, 0);      // num_ref_idx_active_override_flag
+  b.u(1, nra > 1 ? 1u : 0u);  // num_ref_idx_active_override_flag
+  if (nra > 1) b.ue((uint32_t)nra - 1);
   b.u(1, 0);      // ref_pic_list_modification_flag_l0
   b.u(1, 0);      // adaptive_ref_pic_marking_mode_flag
   b.se(qp - 26);  // slice_qp_delta
-  pcme_h264dec *d = &e->d;
   Picture &P = *d->cur;
   const int W = P.w, CW = W / 2;
   const int qpc = kQpc[clip3(0, 51, qp)];
-  const int ref_id = e->ref->id;
   int skip_run = 0;
   for (int addr = 0; addr < e->mbw * e->mbh; addr++) {
     const int mx = addr % e->mbw, my = addr / e->mbw;
@@ -2191,9 +2205,10 @@ int code_p(Enc *e, const pcme_mb_out *field, int qp, BitWriter &b) {
     m.part = (int8_t)f.part;
     m.qp = qp;
     m.qpc = qpc;
+    const Picture &R = *d->list0[f.ref];
     for (int i = 0; i < 4; i++) {
-      m.ref[i] = 0;
-      m.refpic[i] = ref_id;
+      m.ref[i] = (int8_t)f.ref;
+      m.refpic[i] = R.id;
       m.sub[i] = f.part == 3 ? 0 : -1;
       m.tcc[0][i] = m.tcc[1][i] = 0;
     }
@@ -2215,7 +2230,7 @@ int code_p(Enc *e, const pcme_mb_out *field, int qp, BitWriter &b) {
       const int *q = kParts[f.part][p];
       const int blk = (q[1] / 4) * 4 + q[0] / 4;
       int px, py;
-      mv_pred(d, mx, my, m, done, q[0], q[1], q[2], q[3], 0, f.part == 3 ? 0 : f.part, p, px, py);
+      mv_pred(d, mx, my, m, done, q[0], q[1], q[2], q[3], f.ref, f.part == 3 ? 0 : f.part, p, px, py);
       mvd[p][0] = m.mv[blk][0] - px;
       mvd[p][1] = m.mv[blk][1] - py;
       for (int by = q[1] / 4; by < (q[1] + q[3]) / 4; by++)
@@ -2229,7 +2244,7 @@ int code_p(Enc *e, const pcme_mb_out *field, int qp, BitWriter &b) {
     }
     // prediction (into the picture), then the residual levels
     for (int blk = 0; blk < 16; blk++)
-      mc_block(d, *e->ref, mx, my, blk & 3, blk >> 2, m.mv[blk][0], m.mv[blk][1]);
+      mc_block(d, R, mx, my, blk & 3, blk >> 2, m.mv[blk][0], m.mv[blk][1]);
     int acl[16][16], cdc[2][4], cac[2][4][16];
     int cbp_l = 0;
     const int qbits = 15 + qp / 6;
@@ -2279,8 +2294,8 @@ int code_p(Enc *e, const pcme_mb_out *field, int qp, BitWriter &b) {
         if (cdc[c][i] && cbp_c == 0) cbp_c = 1;
       }
     }
-    // P_Skip: the 16x16 MV is the skip MV and nothing is coded
-    if (f.part == 0 && cbp_l == 0 && cbp_c == 0) {
+    // P_Skip: reference 0, the 16x16 MV is the skip MV and nothing is coded
+    if (f.part == 0 && f.ref == 0 && cbp_l == 0 && cbp_c == 0) {
       int sx, sy;
       pskip_mv(d, mx, my, m, sx, sy);
       if (sx == m.mv[0][0] && sy == m.mv[0][1]) {
@@ -2296,6 +2311,8 @@ int code_p(Enc *e, const pcme_mb_out *field, int qp, BitWriter &b) {
     b.ue((uint32_t)f.part);  // P_L0_16x16, P_L0_L0_16x8, P_L0_L0_8x16, P_8x8
     if (f.part == 3)
       for (int s = 0; s < 4; s++) b.ue(0);  // sub_mb_type P_L0_8x8
+    if (nra > 1)
+      for (int p = 0; p < np; p++) te(b, nra - 1, f.ref);  // ref_idx_l0 of each (sub-)partition
     for (int p = 0; p < np; p++) {
       b.se(mvd[p][0]);
       b.se(mvd[p][1]);
@@ -2357,7 +2374,7 @@ int pcme_h264enc_create(int32_t width, int32_t height, pcme_h264enc **out) {
   s.level = 51;
   s.log2_max_frame_num = 16;
   s.poc_type = 2;
-  s.max_num_ref_frames = 1;
+  s.max_num_ref_frames = PCME_MAX_REFS;
   s.mbw = e->mbw;
   s.mbh = e->mbh;
   s.crop[1] = (16 * e->mbw - width) / 2;
@@ -2373,10 +2390,11 @@ int pcme_h264enc_picture(pcme_h264enc *e, const uint8_t *y, int32_t ystride, con
                          int32_t cstride, const pcme_mb_out *mbs, int32_t qp, uint8_t *out, size_t cap, size_t *len) {
   if (!e || !y || !u || !v || !out || !len || ystride < e->w || cstride < e->w / 2 || qp < 0 || qp > 51)
     return fail("pcme_h264enc_picture: bad arguments");
-  if (mbs && !e->ref) return fail("pcme_h264enc_picture: a P picture before the first IDR");
+  if (mbs && e->dpb.empty()) return fail("pcme_h264enc_picture: a P picture before the first IDR");
   if (mbs)
     for (int i = 0; i < e->mbw * e->mbh; i++) {
       if (mbs[i].part > 3) return fail("MB %d: partition %d", i, mbs[i].part);
+      if (mbs[i].ref >= PCME_MAX_REFS) return fail("MB %d: reference index %d", i, mbs[i].ref);
       for (int k = 0; k < 16; k++)
         if (std::abs(mbs[i].mv[k].x) > 2048 || std::abs(mbs[i].mv[k].y) > 2048)
           return fail("MB %d: MV (%d, %d) beyond +-2048", i, mbs[i].mv[k].x, mbs[i].mv[k].y);
@@ -2387,6 +2405,7 @@ int pcme_h264enc_picture(pcme_h264enc *e, const uint8_t *y, int32_t ystride, con
     if (!e->started) write_sps_pps(e, nals);
     e->started = true;
     e->frame_num = 0;
+    e->dpb.clear();  // an IDR picture empties the reference list (8.2.5.1)
   }
   begin_picture(e);
   BitWriter b;
@@ -2404,14 +2423,21 @@ int pcme_h264enc_picture(pcme_h264enc *e, const uint8_t *y, int32_t ystride, con
 }
 
 int pcme_h264enc_recon(pcme_h264enc *e, uint8_t *y, int32_t ystride, uint8_t *u, uint8_t *v, int32_t cstride) {
-  if (!e || !y || !u || !v || !e->ref || ystride < e->w || cstride < e->w / 2)
+  if (!e || !y || !u || !v || e->dpb.empty() || ystride < e->w || cstride < e->w / 2)
     return fail("pcme_h264enc_recon: bad arguments");
-  const Picture &p = *e->ref;
+  const Picture &p = *e->dpb.front();
   for (int r = 0; r < e->h; r++) memcpy(y + (size_t)r * ystride, &p.y[(size_t)r * p.w], e->w);
   for (int r = 0; r < e->h / 2; r++) {
     memcpy(u + (size_t)r * cstride, &p.u[(size_t)r * (p.w / 2)], e->w / 2);
     memcpy(v + (size_t)r * cstride, &p.v[(size_t)r * (p.w / 2)], e->w / 2);
   }
+  return PCME_OK;
+}
+
+int pcme_h264enc_set_refs(pcme_h264enc *e, int32_t nref) {
+  if (!e || nref < 1 || nref > PCME_MAX_REFS) return fail("pcme_h264enc_set_refs: nref %d not in [1, %d]", nref,
+                                                         PCME_MAX_REFS);
+  e->nref = nref;
   return PCME_OK;
 }
 

@@ -15,7 +15,7 @@ from . import MB_DTYPE, PCME_OK, PcmeError
 from . import h264dec
 
 EXPORTS = ("pcme_h264enc_create", "pcme_h264enc_destroy", "pcme_h264enc_picture", "pcme_h264enc_recon",
-           "pcme_h264enc_stats")
+           "pcme_h264enc_set_refs", "pcme_h264enc_stats")
 
 
 def load() -> ctypes.CDLL:
@@ -28,6 +28,7 @@ def load() -> ctypes.CDLL:
             "pcme_h264enc_picture": (c.c_int, [vp, vp, c.c_int32, vp, vp, c.c_int32, vp, c.c_int32, vp, c.c_size_t,
                                                c.POINTER(c.c_size_t)]),
             "pcme_h264enc_recon": (c.c_int, [vp, vp, c.c_int32, vp, vp, c.c_int32]),
+            "pcme_h264enc_set_refs": (c.c_int, [vp, c.c_int32]),
             "pcme_h264enc_stats": (c.c_int, [vp, c.POINTER(c.c_int32), c.POINTER(c.c_int32),
                                              c.POINTER(c.c_int32)]),
         }
@@ -48,12 +49,14 @@ class Encoder:
     """One coder per clip: picture() codes an IDR (field None) or a P
     picture with the given field (mbw*mbh MB_DTYPE records)."""
 
-    def __init__(self, width: int, height: int):
+    def __init__(self, width: int, height: int, nref: int = 1):
         self._lib = load()
         self.width, self.height = width, height
         self.mbw, self.mbh = (width + 15) // 16, (height + 15) // 16
         self._h = ctypes.c_void_p()
         _check(self._lib.pcme_h264enc_create(width, height, ctypes.byref(self._h)))
+        if nref != 1:
+            _check(self._lib.pcme_h264enc_set_refs(self._h, int(nref)))
         self._buf = np.empty(16 * self.mbw * 16 * self.mbh * 3 + (1 << 16), np.uint8)
 
     def picture(self, y, u, v, field=None, qp: int = 27) -> bytes:
@@ -112,13 +115,14 @@ class Coded:
     recon: list
 
 
-def code_clip(y, u, v, fields, qp: int) -> Coded:
+def code_clip(y, u, v, fields, qp: int, nref: int = 1) -> Coded:
     """IDR of frame 0 (I_PCM), then P pictures 1..n-1, picture t coded with
-    fields[t-1].  The rate counts the P pictures only (the IDR is the same
-    lossless picture whatever the field) and the distortion is their mean
-    luma PSNR."""
+    fields[t-1] (with nref > 1 a multi-reference field: picture t's list 0
+    is the reconstructions of t-1 .. t-nref).  The rate counts the P
+    pictures only (the IDR is the same lossless picture whatever the field)
+    and the distortion is their mean luma PSNR."""
     n, h, w = y.shape
-    enc = Encoder(w, h)
+    enc = Encoder(w, h, nref)
     out = [enc.picture(y[0], u[0], v[0], None, qp)]
     rec, ps = [enc.recon()], []
     for t in range(1, n):

@@ -125,3 +125,57 @@ def test_rejects_bad_input(codec, pcme_lib):
     with pytest.raises(PcmeError):
         enc.Encoder(63, 48)
     e.close()
+
+
+@pytest.mark.parametrize("nref,qp", [(2, 27), (4, 32)])
+def test_multiref_field_decodes_to_the_reconstruction(codec, pcme_lib, oracle, nref, qp):
+    """The oracle's multi-reference field (o_search_frame_mr) coded with
+    nref reference indices: the decoder's output equals the coder's
+    reconstruction, and each coded P MB carries the field's reference."""
+    from pcme.h264dec import KIND_P
+    enc, dec = codec
+    y, u, v = _clip(pcme_lib, 176, 144, 6, vmax=4.0)
+    y = y.copy()
+    y[2:] = ((y[2:].astype(int) + y[:-2].astype(int)) // 2).astype(np.uint8)
+    field = oracle.search_clip_mr(y, nref, 16, 1, True, qp)
+    assert (field["ref"] > 0).any()
+    c = enc.code_clip(y, u, v, field, qp, nref=nref)
+    frames = _roundtrip(dec, c)
+    coded = 0
+    for t, f in enumerate(frames[1:]):
+        mbs = f.mbs.reshape(field[t].shape)
+        p = mbs["kind"] == KIND_P
+        assert np.array_equal(mbs["ref"][p][:, 0], field[t]["ref"][p])
+        coded += int(p.sum())
+    assert coded > 50
+
+
+def test_older_reference_saves_bits(codec, pcme_lib, oracle):
+    """Pictures that alternate between two scenes: with four references each
+    P picture from the third on is coded from the identical picture two
+    back, at a fraction of the one-reference bits."""
+    enc, dec = codec
+    a, b = (_clip(pcme_lib, 176, 144, 1, seed=s) for s in (0x5EED7710, 0x5EED7711))
+    y, u, v = (np.concatenate([a[k], b[k], a[k], b[k], a[k]]) for k in range(3))
+    one = enc.code_clip(y, u, v, oracle.search_clip_mr(y, 1, 16, 1, True, 27), 27)
+    four = enc.code_clip(y, u, v, oracle.search_clip_mr(y, 4, 16, 1, True, 27), 27, nref=4)
+    _roundtrip(dec, one)
+    _roundtrip(dec, four)
+    assert four.p_bits < 0.6 * one.p_bits and four.p_psnr >= one.p_psnr - 0.1
+
+
+def test_multiref_rejects_an_index_beyond_those_in_use(codec, pcme_lib):
+    from pcme import PcmeError
+    enc, _ = codec
+    y, u, v = _clip(pcme_lib, 64, 48, 3)
+    e = enc.Encoder(64, 48, nref=2)
+    e.picture(y[0], u[0], v[0], None, 27)
+    f = _zero_field(pcme_lib, 64, 48, 2)[0]
+    f["ref"][0, 0] = 1  # one reconstruction so far: only index 0 is in use
+    with pytest.raises(PcmeError, match="beyond"):
+        e.picture(y[1], u[1], v[1], f, 27)
+    f["ref"][0, 0] = 0
+    e.picture(y[1], u[1], v[1], f, 27)
+    f["ref"][0, 0] = 1
+    e.picture(y[2], u[2], v[2], f, 27)  # two reconstructions now
+    e.close()
