@@ -109,7 +109,7 @@ WORKLOAD_SIZES = {"1080p": ["1920x1080"], "4k": ["3840x2160"],
                   "batch": ["352x288", "640x360", "1280x720", "1920x1080"]}
 
 
-def measured_bd_rate(ncand, thr_part, thr_mv, workload):
+def measured_bd_rate(ncand, thr_part, thr_mv, workload, nref=1):
     """The pruned search's BD-rate against the plain search on real
     bitstreams (the field coder, QP 22/27/32/37, held-out clips of each
     picture size of the workload, every stream decoded bit-exact), from the
@@ -125,6 +125,8 @@ def measured_bd_rate(ncand, thr_part, thr_mv, workload):
             key, rest = ln[1:].split("`: ", 1)
             size = key.split()[0]
             kv = dict(x.split("=") for x in rest.split())
+            if int(kv.get("nref", "1")) != nref:
+                continue
             per.setdefault(size, []).append((float(kv["bd_rate_cubic_pct"]), float(kv["bd_rate_pchip_pct"])))
     sizes = {z: per[z] for z in WORKLOAD_SIZES[workload] if z in per}
     if not sizes:
@@ -132,6 +134,7 @@ def measured_bd_rate(ncand, thr_part, thr_mv, workload):
     return {"per_size": {z: {"cubic_pct": [c for c, _ in v], "pchip_pct": [q for _, q in v]} for z, v in sizes.items()},
             "max_pct": max(max(c, q) for v in sizes.values() for c, q in v),
             "coder": "field coder (pcme/h264enc.py), Baseline CAVLC bitstreams decoded bit-exact, QP 22/27/32/37",
+            "references": nref,
             "source": "profiles/pruner_bdrate_bitstreams.md"}
 
 
@@ -745,7 +748,7 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
         "net_roofline": {"bound": "mfma", "kernel": "hints_kernel", "achieved": round(tflops, 3),
                          "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5), "macs_per_mb": NET_MACS_PER_MB},
-        "bd_rate": measured_bd_rate(args.ncand, args.thr, thr_mv, args.workload),
+        "bd_rate": measured_bd_rate(args.ncand, args.thr, thr_mv, args.workload, args.nref),
     }
 
 

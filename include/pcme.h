@@ -138,8 +138,8 @@ typedef struct pcme_frame_job {
    * its cost; the least total (the lower index on ties) is the MB's
    * reference (pcme_mb_out.ref) and its partition searches run on it, each
    * partition paying the index's rate.  References other than the newest
-   * are read whole (no row limit).  A pruned MB searches reference 0
-   * only. */
+   * are read whole (no row limit).  A pruned MB's hint candidates (motion
+   * over one picture) are scaled by each reference's distance, index + 1. */
   int32_t nref;
   int32_t reserved;
   pcme_ref_planes older[PCME_MAX_REFS - 1];

@@ -725,11 +725,12 @@ void o_analyse_mb_ex(const oparams *p, ombinfo *mbs, int mx, int my,
 
 /* o_analyse_mb_ex over nref references of list 0 (refs[0] = ref_idx 0, the
  * newest).  No counterpart in the reference, whose P slices use one
- * reference: the P16x16 search runs on every reference (a pruned MB: on
- * reference 0 only), each with the predictor of its index and with
- * (lam * te(v) bits of the index) >> 8 added to its cost; the least total
- * (the lower index on ties) gives the MB's reference, and the partition
- * searches run on it, each partition paying the index's rate.  A reference
+ * reference: the P16x16 search runs on every reference, each with the
+ * predictor of its index and with (lam * te(v) bits of the index) >> 8
+ * added to its cost; the least total (the lower index on ties) gives the
+ * MB's reference, and the partition searches run on it, each partition
+ * paying the index's rate.  A pruned MB's hint candidates (motion over one
+ * picture) are scaled by the reference's distance, index + 1.  A reference
  * other than the newest is read without the row limit (it is complete).
  * With nref = 1 this is o_analyse_mb_ex. */
 void o_analyse_mb_mr(const oparams *p, ombinfo *mbs, int mx, int my,
@@ -750,11 +751,14 @@ void o_analyse_mb_mr(const oparams *p, ombinfo *mbs, int mx, int my,
   omv mv16[4], mvp16[4];
   uint32_t cost16 = 0;
   int rs = 0;
-  for (int rf = 0; rf < (pr ? 1 : nref); rf++) {
+  omv hs[4][8];  /* the hint candidates scaled to each reference's distance */
+  for (int rf = 0; rf < nref && pr; rf++)
+    for (int i = 0; i < 8; i++) hs[rf][i] = mvnew((int32_t)hc[i].x * (rf + 1), (int32_t)hc[i].y * (rf + 1));
+  for (int rf = 0; rf < nref; rf++) {
     omv m[4], pp[4];
     const uint32_t c =
-        search_part_ex(rf == 0 ? p : &pall, mbs, mx, my, &refs[rf], rf, OP_16x16, src16, lam, skip, hc, nh, m, pp,
-                       ctr) +
+        search_part_ex(rf == 0 ? p : &pall, mbs, mx, my, &refs[rf], rf, OP_16x16, src16, lam, skip,
+                       pr ? hs[rf] : NULL, nh, m, pp, ctr) +
         ((lam * o_ref_bits(rf, nref)) >> 8);
     if (rf == 0 || c < cost16) {
       cost16 = c;
@@ -780,7 +784,7 @@ void o_analyse_mb_mr(const oparams *p, ombinfo *mbs, int mx, int my,
         continue;
       omv m[4], pp[4];
       uint32_t c = search_part_ex(ps, mbs, mx, my, &refs[rs], rs, parts[k], src16, lam,
-                                  mv16[0], hc, nh, m, pp, ctr) +
+                                  mv16[0], pr ? hs[rs] : NULL, nh, m, pp, ctr) +
                    (uint32_t)part_count(parts[k]) * rcost;
       out->cost[parts[k]] = c;
       uint32_t extra = parts[k] == OP_8x8 ? 10 : 3;

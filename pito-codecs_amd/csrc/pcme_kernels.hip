@@ -1407,24 +1407,30 @@ __device__ __forceinline__ SCtx older_ctx(const DevJob &J, const SCtx &c0, int r
   return c;
 }
 
+// The hint dwords (lane i: dword i) with the candidates (dwords 0-7) scaled
+// by s: motion over one picture taken over s.
+__device__ __forceinline__ uint32_t scale_hint(uint32_t hv, int s) {
+  return lane_id() < 8u ? pack_mv(mv_x(hv) * s, mv_y(hv) * s) : hv;
+}
+
 // Multi-reference analysis of one MB (no counterpart in the reference, whose
 // P slices use one reference): the P16x16 search on each reference of the
-// job (a pruned MB: reference 0 only), each with the predictor of its index
-// and its index's rate added; the least total, the lower index on ties,
-// gives the MB's reference, and the partition modes are searched on it, each
-// partition paying the rate.  The LDS areas hold the regions of reference
-// ar_ref and are restaged when the reference changes.
+// job, each with the predictor of its index and its index's rate added (a
+// pruned MB's hint candidates scaled by the reference's distance, index +
+// 1); the least total, the lower index on ties, gives the MB's reference,
+// and the partition modes are searched on it, each partition paying the
+// rate.  The LDS areas hold the regions of reference ar_ref and are
+// restaged when the reference changes.
 template <bool PR>
 __device__ __forceinline__ void analyse_mb_mr(const DevJob &J, const SCtx &c0, const Nbr &nb0, const uint8_t *src,
                                               int mbx, int mby, uint32_t hv, int nh, int hpart, MbRes &o,
                                               uint32_t *lds, Area *ar, int &ar_ref) {
   Nbr n = nb0;
   const uint32_t skip = pskip(n);
-  const int nsearch = PR ? 1 : J.nref;
   PartRes b16;
   uint32_t cost16 = 0u;
   int rs = 0;
-  for (int r = 0; r < nsearch; r++) {
+  for (int r = 0; r < J.nref; r++) {
     const SCtx cr = r == 0 ? c0 : older_ctx(J, c0, r);
     if (ar_ref != r) {
       ar[0] = no_area();
@@ -1433,7 +1439,7 @@ __device__ __forceinline__ void analyse_mb_mr(const DevJob &J, const SCtx &c0, c
     }
     n.rf = r;
     PartRes r16;
-    search_part<0, PR>(cr, n, src, 16, mbx, mby, skip, hv, nh, r16, lds, ar);
+    search_part<0, PR>(cr, n, src, 16, mbx, mby, skip, PR ? scale_hint(hv, r + 1) : hv, nh, r16, lds, ar);
     const uint32_t t = r16.total + ref_rate(c0.lam, r, J.nref);
     const bool take = r == 0 || t < cost16;
     cost16 = take ? t : cost16;
@@ -1447,6 +1453,7 @@ __device__ __forceinline__ void analyse_mb_mr(const DevJob &J, const SCtx &c0, c
     ar_ref = rs;
   }
   n.rf = rs;
+  if (PR) hv = scale_hint(hv, rs + 1);
   const uint32_t rr = ref_rate(c0.lam, rs, J.nref);
   o.part = 0;
   o.ref = rs;

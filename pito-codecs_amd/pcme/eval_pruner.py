@@ -1,7 +1,8 @@
 """BD-rate of the learned search pruner on a held-out clip (1080p by
 default) at QP 22, 27, 32 and 37.
 
-    python -m pcme.eval_pruner [--frames 16] [--seed 0x5EED9200] [--coder h264|model] [--out report.json]
+    python -m pcme.eval_pruner [--frames 16] [--seed 0x5EED9200] [--coder h264|model] [--size WxH] [--nref N]
+                               [--out report.json]
 
 For each QP the same frames are searched three ways: the plain search, the
 pruned search with that QP's network (5 candidates; full search where
@@ -74,11 +75,12 @@ def rd_point(dc: DeviceClip, field: np.ndarray, qp: int) -> tuple[float, float]:
     return bits, R.psnr(sse, npix)
 
 
-def rd_point_h264(y, u, v, field: np.ndarray, qp: int) -> tuple[float, float]:
+def rd_point_h264(y, u, v, field: np.ndarray, qp: int, nref: int = 1) -> tuple[float, float]:
     """(bits, PSNR) of the P pictures of the clip coded into a real stream
-    with `field`; the stream must decode to the coder's reconstruction."""
+    with `field` (nref reference indices); the stream must decode to the
+    coder's reconstruction."""
     from . import h264dec, h264enc
-    c = h264enc.code_clip(y, u, v, field, qp)
+    c = h264enc.code_clip(y, u, v, field, qp, nref=nref)
     frames = h264dec.decode_annexb(c.stream, per_access_unit=False)
     if len(frames) != len(c.recon) or not all(
             np.array_equal(f.y, r[0]) and np.array_equal(f.u, r[1]) and np.array_equal(f.v, r[2])
@@ -88,30 +90,39 @@ def rd_point_h264(y, u, v, field: np.ndarray, qp: int) -> tuple[float, float]:
 
 
 def evaluate(frames: int, seed: int, ncand: int = 5, thr_part: float = 0.7, thr_mv: float = 0.0,
-             log=print, w: int = 1920, h: int = 1080, weights: str | None = None, coder: str = "h264") -> dict:
+             log=print, w: int = 1920, h: int = 1080, weights: str | None = None, coder: str = "h264",
+             nref: int = 1) -> dict:
     y, u, v = synth_clip(clip_spec(seed, frames + 1, w, h))
     mbw, mbh = mb_dims(w, h)
     dc = DeviceClip(y, mbw, mbh, u, v)
     dc.prepare()
+    rng = 64 if (w, h) == (3840, 2160) else 32
+    if coder != "h264" and nref != 1:
+        raise ValueError("the coding model codes one reference")
     pts = {"plain": [], "pruned": [], "no_network": []}
+    if nref > 1:
+        pts["plain_1ref"] = []
     judge = {}
     for qp in QPS:
         net = P.Net(seed=1, max_batch=64, weights=weights or P.WEIGHTS, qp=qp)
-        plain, _ = P.search_clip(dc, 32, qp, prepared=True)
-        pruned, hints = P.search_clip(dc, 32, qp, net=net, ncand=ncand, thr_part=thr_part, thr_mv=thr_mv,
-                                      prepared=True)
+        plain, _ = P.search_clip(dc, rng, qp, prepared=True, nref=nref)
+        pruned, hints = P.search_clip(dc, rng, qp, net=net, ncand=ncand, thr_part=thr_part, thr_mv=thr_mv,
+                                      prepared=True, nref=nref)
         none = np.zeros(hints.shape, HINT_DTYPE)
         none["prune"] = 1
-        bare, _ = P.search_clip(dc, 32, qp, prepared=True, hints_in=none)
+        bare, _ = P.search_clip(dc, rng, qp, prepared=True, hints_in=none, nref=nref)
         net.close()
-        for k, f in (("plain", plain), ("pruned", pruned), ("no_network", bare)):
-            pts[k].append(rd_point_h264(y, u, v, f, qp) if coder == "h264" else rd_point(dc, f, qp))
+        fields = [("plain", plain, nref), ("pruned", pruned, nref), ("no_network", bare, nref)]
+        if nref > 1:
+            fields.append(("plain_1ref", P.search_clip(dc, rng, qp, prepared=True)[0], 1))
+        for k, f, nr in fields:
+            pts[k].append(rd_point_h264(y, u, v, f, qp, nr) if coder == "h264" else rd_point(dc, f, qp))
         judge[qp] = {"pruned": P.judge(plain, pruned), "no_network": P.judge(plain, bare),
                      "pruned_fraction": float(np.mean(hints["prune"] != 0))}
         log(f"qp {qp}: " + ", ".join(f"{k} {pts[k][-1][0] / 1e3:.0f} kbit {pts[k][-1][1]:.3f} dB" for k in pts))
-    return {
+    rep = {
         "frames": frames, "seed": seed, "size": [w, h], "qps": list(QPS), "ncand": ncand, "thr_part": thr_part, "thr_mv": thr_mv,
-        "coder": coder, "decoded_bit_exact": coder == "h264",
+        "nref": nref, "me_range": rng, "coder": coder, "decoded_bit_exact": coder == "h264",
         "points": {k: [[round(r, 1), round(q, 4)] for r, q in v] for k, v in pts.items()},
         "bd_rate_pruned_pct": round(R.bd_rate(pts["plain"], pts["pruned"]), 4),
         "bd_rate_no_network_pct": round(R.bd_rate(pts["plain"], pts["no_network"]), 4),
@@ -119,6 +130,10 @@ def evaluate(frames: int, seed: int, ncand: int = 5, thr_part: float = 0.7, thr_
         "bd_rate_no_network_pchip_pct": round(R.bd_rate_pchip(pts["plain"], pts["no_network"]), 4),
         "judge": {str(q): j for q, j in judge.items()},
     }
+    if nref > 1:  # the references' own gain: the plain field over nref references against one
+        rep["bd_rate_plain_vs_1ref_pct"] = round(R.bd_rate(pts["plain_1ref"], pts["plain"]), 4)
+        rep["bd_rate_plain_vs_1ref_pchip_pct"] = round(R.bd_rate_pchip(pts["plain_1ref"], pts["plain"]), 4)
+    return rep
 
 
 def main(argv=None) -> int:
@@ -131,6 +146,7 @@ def main(argv=None) -> int:
     ap.add_argument("--weights", default=None, help="pruner weights (.npz; default: the committed ones)")
     ap.add_argument("--coder", choices=("h264", "model"), default="h264")
     ap.add_argument("--size", default="1920x1080")
+    ap.add_argument("--nref", type=int, default=1, help="references of list 0 (the 4K workload: 4)")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     import torch
@@ -139,7 +155,7 @@ def main(argv=None) -> int:
         return 2
     w, h = (int(t) for t in a.size.split("x"))
     rep = evaluate(a.frames, a.seed, a.ncand, a.thr, a.thr_mv, log=lambda s: print(s, file=sys.stderr, flush=True),
-                   w=w, h=h, weights=a.weights, coder=a.coder)
+                   w=w, h=h, weights=a.weights, coder=a.coder, nref=a.nref)
     print(json.dumps(rep))
     if a.out:
         with open(a.out, "w") as f:

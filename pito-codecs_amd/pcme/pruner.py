@@ -226,11 +226,12 @@ def train(net: Net, data: TrainSet, epochs: int = 3, batch: int = 1024, lr: floa
 
 def search_clip(clip: DeviceClip, me_range: int, qp: int, net: Net | None = None, ncand: int = 8,
                 thr_part: float = 0.7, thr_mv: float = 0.0, stream=None, prepared: bool = False,
-                hints_in: np.ndarray | None = None):
+                hints_in: np.ndarray | None = None, nref: int = 1):
     """ME-only field of every P frame of a device clip: the full search
     (net None), the pruned search with the network's hints, or with given
-    hint records `hints_in` ([P frames, mbh, mbw] of HINT_DTYPE).  Returns
-    (field, hints) as numpy records (hints None for the full search)."""
+    hint records `hints_in` ([P frames, mbh, mbw] of HINT_DTYPE); frame t
+    searches its min(t, nref) previous frames.  Returns (field, hints) as
+    numpy records (hints None for the full search)."""
     from . import search_frames
     torch = clip.torch
     dev = clip.raw.device
@@ -249,7 +250,8 @@ def search_clip(clip: DeviceClip, me_range: int, qp: int, net: Net | None = None
     elif net is not None:
         hints = torch.zeros((nb, nmb * HINT_DTYPE.itemsize), dtype=torch.uint8, device=dev)
         hp = [hints[t].data_ptr() for t in range(nb)]
-    jobs = [frame_job(clip, t, t - 1, out[t - 1].data_ptr(), me_range, 1, True, qp, hint_ptr=hp[t - 1])
+    jobs = [frame_job(clip, t, t - 1, out[t - 1].data_ptr(), me_range, 1, True, qp, hint_ptr=hp[t - 1],
+                      older=[t - 1 - k for k in range(1, min(t, nref))])
             for t in range(1, clip.n)]
     if net is not None and hints_in is None:
         net.hints(jobs, ncand, thr_part, thr_mv, stream)

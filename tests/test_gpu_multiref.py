@@ -65,7 +65,8 @@ def test_repeating_pictures_closed_form(gpu, oracle):
 
 
 def test_multiref_with_pruner_hints(gpu, oracle):
-    """Pruned MBs search reference 0 only; the others every reference."""
+    """Pruned MBs search every reference with their hint candidates scaled
+    to its distance."""
     import torch
     import pcme.pruner as P
     say("multiref + network hints 320x240, 3 refs")
@@ -87,7 +88,6 @@ def test_multiref_with_pruner_hints(gpu, oracle):
     want = oracle.search_clip_mr(y, 3, 16, 1, True, 27, hints=hs)
     for t in range(nb):
         assert_same_field(field[t], want[t], f"frame {t + 1}")
-    assert np.all(field["ref"][hs["prune"] != 0] == 0)
 
 
 def test_multiref_entry_points_and_tag_wrap(gpu):

@@ -141,3 +141,39 @@ def test_costs_are_satd_plus_rates(oracle):
         assert int(r["cost"][0]) == satd_textbook(sblk, pred) + rate + ((lam * _ue(rf)) >> 8), (mx, my)
         checked += 1
     assert checked >= 4
+
+
+def test_pruned_mbs_search_every_reference(oracle):
+    """Pictures A, B, A, B, A with every MB pruned (hints of one zero
+    candidate): the pruned search of reference 1 (distance 2) finds the
+    identical picture, so the pruned field takes index 1 like the plain one;
+    and on a clip moving (5, 3) px per picture, hints of the one-picture
+    motion (20, 12) quarter samples, scaled to distance 2, (40, 24), find
+    picture t-2 at that MV where the source repeats it."""
+    A, B = _noise(11, 64, 48), _noise(12, 64, 48)
+    y = np.stack([A, B, A, B, A])
+    mbw, mbh = 4, 3
+    h = np.zeros((4, mbh * mbw), oracle.OHINT_DTYPE)
+    h["prune"] = 1
+    h["ncand"] = 1
+    got = oracle.search_clip_mr(y, 2, 16, 1, True, 27, hints=h.reshape(4, mbh, mbw))[3]
+    assert np.all(got["ref"] == 1) and np.all(got["mv16"] == 0)
+    # a moving picture and noise in turn: picture 4 repeats picture 2 moved
+    # twice.  MB (0, 0) has no neighbour, so only the scaled hint can reach
+    # the 10 px motion: from the unscaled (5, 3) px the two square rounds and
+    # the sub-pel rounds reach 8 px at most.
+    base = _frames(13, 96, 64, 5, shift=(5, 3))
+    y2 = base.copy()
+    y2[1] = _noise(14, 96, 64)
+    y2[3] = _noise(15, 96, 64)
+    mbw, mbh = 6, 4
+    h2 = np.zeros((4, mbh * mbw), oracle.OHINT_DTYPE)
+    h2["prune"] = 1
+    h2["ncand"] = 1
+    h2["cand"][:, :, 0, 0] = 20  # picture t at x shows picture t-1 at x + (5, 3) px
+    h2["cand"][:, :, 0, 1] = 12
+    got2 = oracle.search_clip_mr(y2, 2, 8, 1, True, 27, hints=h2.reshape(4, mbh, mbw))[3]
+    assert got2[0, 0]["ref"] == 1 and tuple(got2[0, 0]["mv16"]) == (40, 24)
+    inner = got2[:-1, :-1]
+    assert np.mean(inner["ref"] == 1) > 0.9
+    assert np.mean((inner["mv16"][..., 0] == 40) & (inner["mv16"][..., 1] == 24)) > 0.9
