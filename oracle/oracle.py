@@ -74,6 +74,10 @@ OB_DTYPE = np.dtype([
     ("reserved", "u1", (4,)),
 ])
 assert OB_DTYPE.itemsize == 136
+# ob_mb has the layout of pcme_b_mb
+OBMB_DTYPE = np.dtype([("search", OB_DTYPE), ("mv", "<i2", (2, 16, 2)), ("ref", "i1", (2, 4)),
+                       ("best_cost", "<u4"), ("mode", "u1"), ("dir", "u1"), ("reserved", "u1", (2,))])
+assert OBMB_DTYPE.itemsize == 280
 
 # ohint has the layout of pcme_hint
 OHINT_DTYPE = np.dtype([
@@ -180,6 +184,9 @@ def load() -> ctypes.CDLL:
     lib.o_b_analyse.restype = None
     lib.o_b_analyse.argtypes = [c.POINTER(oparams), P, P, c.c_int, c.c_int, c.POINTER(oref), c.POINTER(oref), P,
                                 c.c_uint32, P]
+    lib.o_b_field.restype = None
+    lib.o_b_field.argtypes = [c.POINTER(oparams), P, c.POINTER(oref), c.POINTER(oref), P, c.c_int, c.c_int,
+                              c.c_uint32, P]
     lib.o_b_frame_mt.restype = c.c_double
     lib.o_b_frame_mt.argtypes = [c.POINTER(oparams), P, P, c.POINTER(oref), c.POINTER(oref), P, c.c_int, c.c_int,
                                  c.c_uint32, c.c_int, P]
@@ -458,6 +465,16 @@ def b_frame_mt(prm: oparams, mbs, cols, r0: Ref, r1: Ref, src_padded: np.ndarray
                               ctypes.byref(r0.view), ctypes.byref(r1.view), _p(s), s.shape[1], LUMA_PAD, lam,
                               threads, _p(out))
     return sec, out
+
+
+def b_field(prm: oparams, cols, r0: Ref, r1: Ref, src_padded: np.ndarray, lam: int) -> np.ndarray:
+    """o_b_field: the ME-only field of one B picture -> [mbh*mbw] OBMB_DTYPE
+    records (cols: mbw*mbh ombinfo2 of the list-1 picture)."""
+    out = np.zeros(prm.mbw * prm.mbh, OBMB_DTYPE)
+    s = np.ascontiguousarray(src_padded, np.uint8)
+    load().o_b_field(ctypes.byref(prm), ctypes.cast(cols, ctypes.c_void_p), ctypes.byref(r0.view),
+                     ctypes.byref(r1.view), _p(s), s.shape[1], LUMA_PAD, lam, _p(out))
+    return out
 
 
 # -------------------------------------------------------- AQ (lookahead.rs)

@@ -1393,6 +1393,83 @@ double o_b_frame_mt(const oparams *p, const ombinfo2 *mbs, const ombinfo2 *cols,
   return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 }
 
+/* slice.rs:2189-2377 with rd off and without BSkip and intra: the decision
+ * among the searches of b_analyse_proj, then cand_inter_info's commit
+ * (slice.rs:1751-1785): ref_idx per 8x8 and MV per 4x4 and list, zero MVs
+ * where a list is unused. */
+static void b_decide(int partitions, const ob_out *o, ob_mb *m) {
+  m->search = *o;
+  uint64_t best = 0;
+  int mode = -1;
+  if (o->direct_ok) {
+    best = o->direct_cost;
+    mode = 0;
+  }
+  if (mode < 0 || (uint64_t)o->cost[o->dir] < best) {
+    best = o->cost[o->dir];
+    mode = 1;
+  }
+  if (partitions && best > 256u * 4u)
+    for (int k = 0; k < 2; k++)
+      if ((uint64_t)o->part_cost[k] < best) {
+        best = o->part_cost[k];
+        mode = 2 + k;
+      }
+  m->mode = (uint8_t)mode;
+  m->dir = (uint8_t)o->dir;
+  m->best_cost = (uint32_t)best;
+  m->reserved[0] = m->reserved[1] = 0;
+  for (int l = 0; l < 2; l++)
+    for (int r = 0; r < 16; r++) {
+      const int q = ((r >> 3) << 1) | ((r & 3) >> 1);  /* the 8x8 of 4x4 block r */
+      int8_t rf;
+      omv mv;
+      if (mode == 0) {
+        rf = o->direct_ref[l][q];
+        mv = o->direct_mv[l][q];
+      } else if (mode == 1) {
+        rf = (o->dir == 2 || o->dir == (uint32_t)l) ? 0 : -1;
+        mv = o->mv16[l];
+      } else {
+        const int k = mode - 2;
+        const int pi = k == 0 ? (r >> 3) : ((r & 3) >> 1);  /* 16x8: top/bottom; 8x16: left/right */
+        const int d = o->part_dir[k][pi];
+        rf = (d == 2 || d == l) ? 0 : -1;
+        mv = o->part_mv[k][l][pi];
+      }
+      if (rf < 0) mv = mvnew(0, 0);
+      m->mv[l][r] = mv;
+      m->ref[l][q] = rf;
+    }
+}
+
+void o_b_field(const oparams *p0, const ombinfo2 *cols, const oref *r0, const oref *r1, const uint8_t *src,
+               int sstride, int pad, uint32_t lam, ob_mb *out) {
+  const int n = p0->mbw * p0->mbh;
+  ombinfo *proj = (ombinfo *)calloc(2 * (size_t)n, sizeof(ombinfo));
+  const ombinfo *pl[2] = {proj, proj + n};
+  oparams p = *p0;
+  for (int my = 0; my < p.mbh; my++) {
+    p.vlim = o_vlim(my, p.mbh);
+    for (int mx = 0; mx < p.mbw; mx++) {
+      const int i = my * p.mbw + mx;
+      uint8_t s16[256];
+      for (int r = 0; r < 16; r++)
+        memcpy(s16 + 16 * r, src + (size_t)(pad + 16 * my + r) * sstride + pad + 16 * mx, 16);
+      ob_out o;
+      b_analyse_proj(&p, pl, &cols[i], mx, my, r0, r1, s16, lam, &o);
+      b_decide(p.partitions, &o, &out[i]);
+      for (int l = 0; l < 2; l++) {
+        ombinfo *c = proj + (size_t)l * n + i;
+        c->kind = OK_PINTER;
+        memcpy(c->ref_idx, out[i].ref[l], 4);
+        memcpy(c->mv, out[i].mv[l], sizeof c->mv);
+      }
+    }
+  }
+  free(proj);
+}
+
 /* ------------------------------------------------------------- AQ (C8) */
 
 /* dsp.rs:104-114 block_var: sum of squares minus square of sum over w*h,

@@ -277,6 +277,25 @@ void o_aq_offsets(const uint8_t *y, int ys, const uint8_t *u, const uint8_t *v, 
 double o_b_frame_mt(const oparams *p, const ombinfo2 *mbs, const ombinfo2 *cols, const oref *r0, const oref *r1,
                     const uint8_t *src, int sstride, int pad, uint32_t lam, int threads, ob_out *out);
 
+/* Layout of pcme_b_mb (include/pcme.h): one MB of the ME-only B field. */
+typedef struct {
+  ob_out search;
+  omv mv[2][16];     /* committed MbInfo.mv per list (zero where unused) */
+  int8_t ref[2][4];  /* committed ref_idx per list and 8x8 (-1: list unused) */
+  uint32_t best_cost;
+  uint8_t mode;      /* 0 direct, 1 16x16, 2 16x8, 3 8x16 */
+  uint8_t dir;       /* the 16x16 direction (0 list 0, 1 list 1, 2 bi) */
+  uint8_t reserved[2];
+} ob_mb;
+/* The ME-only field of one B picture (analyse_b's decision without BSkip,
+ * RD and intra, slice.rs:2189-2377 with rd off): every MB in raster order
+ * searched as o_b_analyse over the MBs already committed, then committed
+ * as direct (when its MVs are legal), the 16x16 direction when cheaper
+ * (strict <), then 16x8 and 8x16 when partitions and the best cost >
+ * 1024, each when cheaper; cols = the list-1 picture's MbInfo. */
+void o_b_field(const oparams *p, const ombinfo2 *cols, const oref *r0, const oref *r1, const uint8_t *src,
+               int sstride, int pad, uint32_t lam, ob_mb *out);
+
 /* ---- early P-skip gate (skip_oracle.c; slice.rs:1884-1920, 679-1001,
  * transform.rs, residual.rs) ---- */
 typedef struct {
