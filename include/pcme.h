@@ -423,6 +423,51 @@ typedef struct pcme_b_out {
 int pcme_b_analyse_batch(const pcme_b_job *jobs, int njobs, pcme_b_out *out,
                          void *stream);
 
+/* ---- the ME-only field of a whole B picture -------------------------- */
+/* The co-located MB of the list-1 picture (direct_spatial, slice.rs:
+ * 2057-2071).  44 bytes. */
+typedef struct pcme_b_col {
+  pcme_mv mv[2][4];        /* MVs of its 4x4 blocks 0, 3, 12, 15 per list */
+  int8_t ref[2][4];        /* its ref_idx per 8x8 per list */
+  int32_t intra;           /* nonzero: the co-located MB is intra */
+} pcme_b_col;
+
+/* One B picture: the source and both references as the frame jobs lay
+ * them out (padded luma, stride 16*mbw + 128, pad 64; device). */
+typedef struct pcme_b_field_job {
+  const uint8_t *src;      /* padded source luma of the B picture */
+  const uint8_t *ref0[4];  /* list-0 picture: full, H, V, C planes */
+  const uint8_t *ref1[4];  /* list-1 picture */
+  const pcme_b_col *col;   /* mbw*mbh co-located MBs, raster order */
+  int32_t mbw, mbh;
+  int32_t qp, me_range, subpel_rounds, partitions;
+} pcme_b_field_job;
+
+/* One MB of the B field: its search (pcme_b_out) over the MBs committed
+ * before it, and what it commits -- analyse_b's decision with rd off and
+ * without BSkip and intra (slice.rs:2189-2377): direct when its MVs are
+ * legal, the 16x16 direction when strictly cheaper, then 16x8 and 8x16
+ * when `partitions` and the best cost exceeds 1024, each when strictly
+ * cheaper; committed as cand_inter_info does (slice.rs:1751-1785).
+ * 280 bytes. */
+typedef struct pcme_b_mb {
+  pcme_b_out search;
+  pcme_mv mv[2][16];       /* MbInfo.mv per list, zero where a list is unused */
+  int8_t ref[2][4];        /* MbInfo.ref_idx per list and 8x8, -1 unused */
+  uint32_t best_cost;
+  uint8_t mode;            /* 0 direct, 1 16x16, 2 16x8, 3 8x16 */
+  uint8_t dir;             /* the 16x16 direction: 0 list 0, 1 list 1, 2 bi */
+  uint8_t reserved[2];
+} pcme_b_mb;
+
+/* The whole field of one B picture in raster dependency order, on the
+ * device: its anti-diagonals (mx + 2 my constant) one after another, each
+ * diagonal's MBs in parallel (a job-building kernel reads the committed
+ * neighbours, the B search kernels, a commit kernel).  out: mbw*mbh
+ * records (device), raster order.  No counterpart in the reference, whose
+ * encoder walks the picture on the CPU. */
+int pcme_b_field(const pcme_b_field_job *job, pcme_b_mb *out, void *stream);
+
 /* ---- early P-skip gate (analyse_p, slice.rs:1884-1920) ----------------
  * Before searching an MB the encoder predicts it at its pskip_mv
  * (pred_block: mc_luma + mc_chroma), and when that MV is legal and the

@@ -1024,6 +1024,77 @@ int pcme_b_analyse_batch(const pcme_b_job *jobs, int njobs, pcme_b_out *out, voi
   return PCME_OK;
 }
 
+static_assert(sizeof(pcme_b_field_job) == 104, "pcme_b_field_job layout");
+
+int pcme_b_field(const pcme_b_field_job *job, pcme_b_mb *out, void *stream) {
+  if (!job || !out || ((uintptr_t)out & 7u)) return fail(PCME_EINVAL, "pcme_b_field: bad arguments");
+  const pcme_b_field_job &F = *job;
+  if (F.mbw < 1 || F.mbh < 1 || F.mbw > 4096 || F.mbh > 4096 || !F.src || !F.col || ((uintptr_t)F.src & 3u) ||
+      ((uintptr_t)F.col & 3u))
+    return fail(PCME_EINVAL, "pcme_b_field: bad picture size, source or co-located array");
+  for (int k = 0; k < 4; k++)
+    if (!F.ref0[k] || !F.ref1[k] || (((uintptr_t)F.ref0[k] | (uintptr_t)F.ref1[k]) & 3u))
+      return fail(PCME_EINVAL, "pcme_b_field: reference plane null or misaligned");
+  if (F.qp < 0 || F.qp > 51 || F.me_range < 0 || F.me_range > 1024 || F.subpel_rounds < 0 || F.subpel_rounds > 8)
+    return fail(PCME_EINVAL, "pcme_b_field: qp, me_range or subpel_rounds out of range");
+  Dev *dv;
+  int rc = get_dev(&dv);
+  if (rc) return rc;
+  const int mbw = F.mbw, mbh = F.mbh, n = mbw * mbh, nd = mbw + 2 * (mbh - 1);
+  const int stride = 16 * mbw + 2 * PCME_PAD;
+  const uint32_t lam = lambda_of(F.qp);
+  // the job table in diagonal order: diagonal d = mx + 2 my, rows ascending
+  std::vector<pcme_b_job> jobs((size_t)n);
+  std::vector<int> off((size_t)nd + 1, 0);
+  int k = 0;
+  for (int d = 0; d < nd; d++) {
+    off[(size_t)d] = k;
+    for (int my = 0; my < mbh; my++) {
+      const int mx = d - 2 * my;
+      if (mx < 0 || mx >= mbw) continue;
+      pcme_b_job &J = jobs[(size_t)k++];
+      memset(&J, 0, sizeof J);
+      J.src = F.src + (size_t)(PCME_PAD + 16 * my) * stride + PCME_PAD + 16 * mx;
+      J.sstride = stride;
+      for (int q = 0; q < 4; q++) {
+        J.ref0[q] = F.ref0[q];
+        J.ref1[q] = F.ref1[q];
+      }
+      J.mbw = mbw;
+      J.mbh = mbh;
+      J.mx = mx;
+      J.my = my;
+      J.lam = lam;
+      J.me_range = F.me_range;
+      J.subpel_rounds = F.subpel_rounds;
+      const int need = 16 * (my + 1) + 96;  // frame.rs:152 with picture.rs:261-265
+      J.vlim = need >= 16 * mbh ? 16 * mbh + 44 : need;
+    }
+  }
+  off[(size_t)nd] = k;
+  const size_t jb = sizeof(pcme_b_job) * (size_t)n, rb = sizeof(pcme_b_out) * (size_t)n;
+  std::lock_guard<std::mutex> g(dv->mu);
+  WsGuard wg;
+  wg.s = (hipStream_t)stream;
+  CallWs *w = nullptr;
+  if ((rc = take_ws(dv, wg.s, jb + rb, jb, &w))) return rc;
+  wg.w = w;
+  if ((rc = ws_upload(w, jobs.data(), jb, wg.s))) return rc;
+  pcme_b_job *dj = (pcme_b_job *)w->dev;
+  pcme_b_out *dr = (pcme_b_out *)((uint8_t *)w->dev + jb);
+  for (int d = 0; d < nd; d++) {
+    const int o = off[(size_t)d], c = off[(size_t)d + 1] - o;
+    const unsigned tb = (unsigned)((c + 255) / 256);
+    hipLaunchKernelGGL(b_gather_kernel, dim3(tb), dim3(256), 0, wg.s, dj + o, c, out, F.col);
+    hipLaunchKernelGGL(b_analyse_16_kernel, dim3((unsigned)c), dim3(64), 0, wg.s, dj + o, c, dr + o);
+    hipLaunchKernelGGL(b_analyse_part_kernel<0>, dim3((unsigned)c), dim3(64), 0, wg.s, dj + o, c, dr + o);
+    hipLaunchKernelGGL(b_analyse_part_kernel<1>, dim3((unsigned)c), dim3(64), 0, wg.s, dj + o, c, dr + o);
+    hipLaunchKernelGGL(b_commit_kernel, dim3(tb), dim3(256), 0, wg.s, dj + o, c, dr + o, out, F.partitions ? 1 : 0);
+    HIPCHK(hipGetLastError());
+  }
+  return PCME_OK;
+}
+
 static_assert(sizeof(pcme_skip_job) == 120 && sizeof(pcme_skip_out) == 16, "pcme_skip_* layout");
 static_assert(sizeof(pcme_frame_job) == 184 && offsetof(pcme_frame_job, older) == 88 && sizeof(pcme_mb_out) == 128,
               "pcme_frame_job / pcme_mb_out layout");

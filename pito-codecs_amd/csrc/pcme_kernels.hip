@@ -2136,6 +2136,110 @@ __global__ void __launch_bounds__(64) b_analyse_part_kernel(const pcme_b_job *__
 template __global__ void b_analyse_part_kernel<0>(const pcme_b_job *__restrict__, int, pcme_b_out *);
 template __global__ void b_analyse_part_kernel<1>(const pcme_b_job *__restrict__, int, pcme_b_out *);
 
+// ------------------------------------------------------ the B picture field
+
+static_assert(sizeof(pcme_b_col) == 44 && sizeof(pcme_b_mb) == 280, "pcme_b_col / pcme_b_mb layout");
+
+// The 8x8 (raster) holding 4x4 block r.
+__device__ __forceinline__ int blk8(int r) { return ((r >> 3) << 1) | ((r & 3) >> 1); }
+
+// One diagonal of the B field, before its search: each job's neighbour
+// blocks (both lists: left blocks 3, 7, 11, 15, top 12..15, top-right 12,
+// top-left 15; -2 outside the picture) read from the records the earlier
+// diagonals committed, and its co-located MB.  One thread per job.
+__global__ void __launch_bounds__(256) b_gather_kernel(pcme_b_job *__restrict__ jobs, int n,
+                                                       const pcme_b_mb *__restrict__ field,
+                                                       const pcme_b_col *__restrict__ col) {
+  const int k = (int)(blockIdx.x * 256 + threadIdx.x);
+  if (k >= n) return;
+  pcme_b_job &J = jobs[k];
+  const int mx = J.mx, my = J.my, mbw = J.mbw;
+  // (MB offset, 4x4 block) of the ten neighbour blocks
+  const int ndx[10] = {-1, -1, -1, -1, 0, 0, 0, 0, 1, -1};
+  const int ndy[10] = {0, 0, 0, 0, -1, -1, -1, -1, -1, -1};
+  const int nbk[10] = {3, 7, 11, 15, 12, 13, 14, 15, 12, 15};
+  for (int i = 0; i < 10; i++) {
+    const int x = mx + ndx[i], y = my + ndy[i];
+    const bool av = x >= 0 && y >= 0 && x < mbw;
+    for (int l = 0; l < 2; l++) {
+      pcme_nb_block b;
+      b.reserved[0] = b.reserved[1] = b.reserved[2] = 0;
+      if (av) {
+        const pcme_b_mb &m = field[(size_t)y * mbw + x];
+        b.mv = m.mv[l][nbk[i]];
+        b.ref = m.ref[l][blk8(nbk[i])];
+      } else {
+        b.mv.x = b.mv.y = 0;
+        b.ref = -2;
+      }
+      J.nb[l][i] = b;
+    }
+  }
+  const pcme_b_col &c = col[(size_t)my * mbw + mx];
+  for (int l = 0; l < 2; l++)
+    for (int q = 0; q < 4; q++) {
+      J.col_mv[l][q] = c.mv[l][q];
+      J.col_ref[l][q] = c.ref[l][q];
+    }
+  J.col_intra = c.intra;
+}
+
+// One diagonal of the B field, after its search: analyse_b's decision with
+// rd off and without BSkip and intra (slice.rs:2189-2377) and the commit of
+// cand_inter_info (slice.rs:1751-1785) into the MB's record.  One thread
+// per job.
+__global__ void __launch_bounds__(256) b_commit_kernel(const pcme_b_job *__restrict__ jobs, int n,
+                                                       const pcme_b_out *__restrict__ res,
+                                                       pcme_b_mb *__restrict__ field, int partitions) {
+  const int k = (int)(blockIdx.x * 256 + threadIdx.x);
+  if (k >= n) return;
+  const pcme_b_out &o = res[k];
+  pcme_b_mb &m = field[(size_t)jobs[k].my * jobs[k].mbw + jobs[k].mx];
+  uint32_t best = 0u;
+  int mode = -1;
+  if (o.direct_ok) {
+    best = o.direct_cost;
+    mode = 0;
+  }
+  if (mode < 0 || o.cost[o.dir] < best) {
+    best = o.cost[o.dir];
+    mode = 1;
+  }
+  if (partitions && best > 1024u)
+    for (int p = 0; p < 2; p++)
+      if (o.part_cost[p] < best) {
+        best = o.part_cost[p];
+        mode = 2 + p;
+      }
+  m.search = o;
+  m.best_cost = best;
+  m.mode = (uint8_t)mode;
+  m.dir = (uint8_t)o.dir;
+  m.reserved[0] = m.reserved[1] = 0;
+  for (int l = 0; l < 2; l++)
+    for (int r = 0; r < 16; r++) {
+      const int q = blk8(r);
+      int rf;
+      pcme_mv mv;
+      if (mode == 0) {
+        rf = o.direct_ref[l][q];
+        mv = o.direct_mv[l][q];
+      } else if (mode == 1) {
+        rf = (o.dir == 2u || o.dir == (uint32_t)l) ? 0 : -1;
+        mv = o.mv16[l];
+      } else {
+        const int p = mode - 2;
+        const int pi = p == 0 ? (r >> 3) : ((r & 3) >> 1);
+        const int d = o.part_dir[p][pi];
+        rf = (d == 2 || d == l) ? 0 : -1;
+        mv = o.part_mv[p][l][pi];
+      }
+      if (rf < 0) mv.x = mv.y = 0;
+      m.mv[l][r] = mv;
+      m.ref[l][q] = (int8_t)rf;
+    }
+}
+
 // -------------------------------------------------- per-call search batch
 
 __global__ void __launch_bounds__(64) search_batch_kernel(const pcme_search_job *__restrict__ jobs,

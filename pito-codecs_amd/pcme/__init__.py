@@ -114,6 +114,15 @@ class BJob(ctypes.Structure):
 assert ctypes.sizeof(BJob) == 328
 
 
+class BFieldJob(ctypes.Structure):
+    _fields_ = [("src", ctypes.c_void_p), ("ref0", ctypes.c_void_p * 4), ("ref1", ctypes.c_void_p * 4),
+                ("col", ctypes.c_void_p), ("mbw", ctypes.c_int32), ("mbh", ctypes.c_int32), ("qp", ctypes.c_int32),
+                ("me_range", ctypes.c_int32), ("subpel_rounds", ctypes.c_int32), ("partitions", ctypes.c_int32)]
+
+
+assert ctypes.sizeof(BFieldJob) == 104
+
+
 # pcme_skip_job flags
 SKIP_T8X8, SKIP_ADAPTIVE_T8, SKIP_CABAC = 1, 2, 4
 
@@ -187,6 +196,12 @@ B_OUT_DTYPE = np.dtype([
     ("reserved", "u1", (4,)),
 ])
 assert B_OUT_DTYPE.itemsize == 136
+# pcme_b_col (44 bytes) and pcme_b_mb (280 bytes)
+B_COL_DTYPE = np.dtype([("mv", "<i2", (2, 4, 2)), ("ref", "i1", (2, 4)), ("intra", "<i4")])
+assert B_COL_DTYPE.itemsize == 44
+B_MB_DTYPE = np.dtype([("search", B_OUT_DTYPE), ("mv", "<i2", (2, 16, 2)), ("ref", "i1", (2, 4)),
+                       ("best_cost", "<u4"), ("mode", "u1"), ("dir", "u1"), ("reserved", "u1", (2,))])
+assert B_MB_DTYPE.itemsize == 280
 
 # pcme_cplx_block (12 bytes) and pcme_tree_block (20 bytes) as numpy records
 CPLX_BLOCK_DTYPE = np.dtype([("intra", "<u4"), ("inter", "<u4"), ("mv", "<i2", (2,))])
@@ -222,6 +237,7 @@ EXPORTS = (
     "pcme_net_create", "pcme_net_destroy", "pcme_net_get_params", "pcme_net_set_params",
     "pcme_net_inputs", "pcme_net_forward", "pcme_net_grad", "pcme_net_train_step", "pcme_net_hints",
     "pcme_lowres", "pcme_complexity", "pcme_mbtree_analyse", "pcme_search_part_batch", "pcme_b_analyse_batch",
+    "pcme_b_field",
     "pcme_aq_offsets", "pcme_mbtree_propagate", "pcme_mbtree_offsets", "pcme_skip_gate_batch",
 )
 
@@ -261,6 +277,7 @@ def load() -> ctypes.CDLL:
         "pcme_search_batch": (c.c_int, [c.POINTER(SearchJob), c.c_int, vp, vp, vp]),
         "pcme_search_part_batch": (c.c_int, [c.POINTER(PartJob), c.c_int, vp, vp]),
         "pcme_b_analyse_batch": (c.c_int, [c.POINTER(BJob), c.c_int, vp, vp]),
+        "pcme_b_field": (c.c_int, [c.POINTER(BFieldJob), vp, vp]),
         "pcme_skip_gate_batch": (c.c_int, [c.POINTER(SkipJob), c.c_int, vp, vp]),
         "pcme_pred_frame": (c.c_int, [vp, vp, vp, vp, vp, vp, vp, c.c_int32, c.c_int32, vp, vp, vp, vp]),
         "pcme_synth_clip": (c.c_int, [c.POINTER(SynthCfg), vp, vp, vp, c.c_int32]),

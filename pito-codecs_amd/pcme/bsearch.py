@@ -91,4 +91,40 @@ def b_analyse(jobs: np.ndarray, out_ptr: int, stream=None) -> None:
                                        _stream_ptr(stream)))
 
 
-__all__ = ["B_JOB_DTYPE", "B_OUT_DTYPE", "NB_DTYPE", "b_jobs", "b_analyse", "vlim_rows"]
+def b_field(cur: DeviceClip, t: int, ref0: tuple[DeviceClip, int], ref1: tuple[DeviceClip, int], col, qp: int,
+            me_range: int = 32, subpel_rounds: int = 1, partitions: bool = True, stream=None):
+    """pcme_b_field: the ME-only field of B picture t of `cur` (all of its
+    MBs, each searched over the MBs committed before it) between list-0
+    picture ref0 and list-1 picture ref1 ((clip, index) pairs, prepared).
+    col: [mbh, mbw] B_COL_DTYPE records of the list-1 picture's MBs (host).
+    Returns the [mbh, mbw] B_MB_DTYPE records."""
+    from . import BFieldJob, B_COL_DTYPE, B_MB_DTYPE
+    torch = cur.torch
+    dev = cur.raw.device
+    c = np.ascontiguousarray(np.asarray(col).reshape(-1).view(B_COL_DTYPE))
+    assert c.size == cur.mbw * cur.mbh
+    dcol = torch.as_tensor(c.view(np.uint8)).to(dev)
+    out = torch.empty(cur.mbw * cur.mbh * B_MB_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    (c0, t0), (c1, t1) = ref0, ref1
+    job = BFieldJob(cur.plane_ptr(0, t), (ctypes.c_void_p * 4)(*[c0.plane_ptr(k, t0) for k in range(4)]),
+                    (ctypes.c_void_p * 4)(*[c1.plane_ptr(k, t1) for k in range(4)]), dcol.data_ptr(), cur.mbw,
+                    cur.mbh, int(qp), int(me_range), int(subpel_rounds), 1 if partitions else 0)
+    _check(load().pcme_b_field(ctypes.byref(job), out.data_ptr(), _stream_ptr(stream)))
+    torch.cuda.synchronize()
+    return out.cpu().numpy().view(B_MB_DTYPE).reshape(cur.mbh, cur.mbw)
+
+
+def col_records(field_p, mbh: int, mbw: int) -> np.ndarray:
+    """B_COL_DTYPE records of a list-1 picture coded as a P picture with the
+    ME-only field (MB_DTYPE records): list 0 = its MVs and reference indices,
+    list 1 unused, never intra."""
+    from . import B_COL_DTYPE
+    f = np.asarray(field_p).reshape(mbh, mbw)
+    col = np.zeros((mbh, mbw), B_COL_DTYPE)
+    col["mv"][:, :, 0] = f["mv"][:, :, list(_CORNER)]
+    col["ref"][:, :, 0] = f["ref"][:, :, None]
+    col["ref"][:, :, 1] = -1
+    return col
+
+
+__all__ = ["B_JOB_DTYPE", "B_OUT_DTYPE", "NB_DTYPE", "b_jobs", "b_analyse", "b_field", "col_records", "vlim_rows"]

@@ -1,0 +1,54 @@
+"""The ME-only field of a whole B picture on the GPU (pcme_b_field: its
+anti-diagonals one after another, each MB searched over the MBs committed
+before it), bit for bit against the oracle's o_b_field (analyse_b's
+searches and decision with rd off, raster order), at 320x240 with and
+without partitions and at 1080p.
+
+Each case prints what it runs before the GPU step so a failing run names
+its case."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _oracle_cols(oracle, field_p):
+    """ombinfo2 of the list-1 picture from its P field (list 0 only)."""
+    mbh, mbw = field_p.shape
+    cols = (oracle.ombinfo2 * (mbw * mbh))()
+    for i, r in enumerate(field_p.reshape(-1)):
+        cols[i].kind = oracle.OK_PINTER
+        for q in range(4):
+            cols[i].ref_idx[0][q] = int(r["ref"])
+            cols[i].ref_idx[1][q] = -1
+        for b in range(16):
+            cols[i].mv[0][b] = oracle.omv(int(r["mv"][b][0]), int(r["mv"][b][1]))
+    return cols
+
+
+@pytest.mark.parametrize("w,h,qp,partitions", [(320, 240, 27, True), (320, 240, 32, False), (1920, 1080, 27, True)])
+def test_b_field_matches_oracle(gpu, oracle, w, h, qp, partitions):
+    from pcme import bsearch as B
+    say("B field", w, h, qp, partitions)
+    y, _, _ = gpu.synth_clip(gpu.ClipSpec(w, h, 3, seed=0x5EED6000 + w, nrect=6, vmax=6.0))
+    mbw, mbh = gpu.mb_dims(w, h)
+    dc = gpu.DeviceClip(y, mbw, mbh)
+    dc.prepare()
+    fp = oracle.search_clip_me(y[1:3], 32, 1, True, qp)[0]  # picture 2 coded as P from picture 1
+    got = B.b_field(dc, 1, (dc, 0), (dc, 2), B.col_records(fp, mbh, mbw), qp, 32, 1, partitions)
+    prm = oracle.oparams(mbw, mbh, 32, 1, 1 if partitions else 0, 0)
+    want = oracle.b_field(prm, _oracle_cols(oracle, fp), oracle.Ref(oracle.pad_luma(y[0], mbw, mbh)),
+                          oracle.Ref(oracle.pad_luma(y[2], mbw, mbh)), oracle.pad_luma(y[1], mbw, mbh),
+                          oracle.load().o_lambda(qp)).reshape(mbh, mbw)
+    g = np.ascontiguousarray(got).view(np.uint8).reshape(mbh * mbw, -1)
+    wt = np.ascontiguousarray(want).view(np.uint8).reshape(mbh * mbw, -1)
+    bad = np.nonzero((g != wt).any(axis=1))[0]
+    assert bad.size == 0, f"{bad.size} of {mbw * mbh} B records differ, first MB {divmod(int(bad[0]), mbw)}"
+    modes = np.bincount(got["mode"].reshape(-1), minlength=4)
+    assert modes[0] > 0 and modes[1] > 0 and (modes[2] + modes[3] > 0) == partitions
