@@ -644,6 +644,66 @@ def b_leg(args, pcme, torch, dc, out, qp, me_range, stream, threads):
                                        f"{reps} repetitions",
                              "seconds": round(secs, 3)}
         o["parity_sample"] = bool(got.tobytes() == want.tobytes())
+    o["field"] = b_field_leg(args, pcme, torch, dc, f[1], qp, me_range, stream)
+    return o
+
+
+def b_field_leg(args, pcme, torch, dc, field2, qp, me_range, stream):
+    """The ME-only field of the same B picture as one dependent whole
+    (pcme_b_field: each MB searched over the MBs committed before it, the
+    picture's anti-diagonals in order), the co-located MBs the P field of
+    frame 2; the C oracle's o_b_field (raster order, one thread: the
+    picture is one dependent chain) times the same picture and the records
+    are compared."""
+    import numpy as np
+    from pcme import bsearch as B
+    from oracle import oracle as O
+    mbw, mbh = dc.mbw, dc.mbh
+    col = B.col_records(field2, mbh, mbw)
+    n = mbw * mbh
+    dcol = torch.as_tensor(np.ascontiguousarray(col.reshape(-1)).view(np.uint8)).to("cuda")
+    dout = torch.empty(n * pcme.B_MB_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    sp = stream.cuda_stream
+
+    def run():
+        B.b_field_launch(dc, 1, (dc, 0), (dc, 2), dcol.data_ptr(), dout.data_ptr(), qp, me_range, 1, True, sp)
+
+    for _ in range(max(1, args.warmup)):
+        run()
+    reps = max(3, args.steps)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+        run()
+    b.record(stream)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    got = dout.cpu().numpy().view(pcme.B_MB_DTYPE).reshape(mbh, mbw)
+    diag = mbw + 2 * (mbh - 1)
+    o = {"workload": f"ME-only field of the B picture, {n} MBs in {diag} anti-diagonals (5 launches each), "
+                     "co-located MBs from the P field of frame 2",
+         "ms_per_picture": round(ms, 3), "mblocks_per_s": round(n / (ms * 1e-3) / 1e6, 4),
+         "modes": [int(x) for x in np.bincount(got["mode"].reshape(-1), minlength=4)]}
+    print(f"[bench] B field: {ms:.3f} ms per picture", file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        cols = (O.ombinfo2 * n)()
+        for i, r in enumerate(np.asarray(field2).reshape(-1)):
+            cols[i].kind = O.OK_PINTER
+            for q in range(4):
+                cols[i].ref_idx[0][q] = int(r["ref"])
+                cols[i].ref_idx[1][q] = -1
+            for k in range(16):
+                cols[i].mv[0][k] = O.omv(int(r["mv"][k][0]), int(r["mv"][k][1]))
+        planes = dc.planes[0, :3].cpu().numpy()
+        r0, r1 = O.Ref(planes[0]), O.Ref(planes[2])
+        prm = O.oparams(mbw, mbh, me_range, 1, 1, 0)
+        t0 = time.perf_counter()
+        want = O.b_field(prm, cols, r0, r1, planes[1], pcme.lam(qp))
+        secs = time.perf_counter() - t0
+        o["cpu_baseline"] = {"value": round(n / secs / 1e6, 5), "unit": "Mblocks/s", "cores": 1, "kind": "port",
+                             "sample": "the same B picture, C oracle o_b_field -O3 -march=x86-64-v3, one run",
+                             "seconds": round(secs, 3)}
+        o["parity_sample"] = bool(np.ascontiguousarray(got).tobytes() == want.tobytes())
     return o
 
 

@@ -91,6 +91,20 @@ def b_analyse(jobs: np.ndarray, out_ptr: int, stream=None) -> None:
                                        _stream_ptr(stream)))
 
 
+def b_field_launch(cur: DeviceClip, t: int, ref0: tuple[DeviceClip, int], ref1: tuple[DeviceClip, int],
+                   col_ptr: int, out_ptr: int, qp: int, me_range: int = 32, subpel_rounds: int = 1,
+                   partitions: bool = True, stream=None) -> None:
+    """pcme_b_field on device buffers: col_ptr holds mbw*mbh B_COL_DTYPE
+    records, out_ptr receives mbw*mbh B_MB_DTYPE records (stream order, no
+    synchronisation)."""
+    from . import BFieldJob
+    (c0, t0), (c1, t1) = ref0, ref1
+    job = BFieldJob(cur.plane_ptr(0, t), (ctypes.c_void_p * 4)(*[c0.plane_ptr(k, t0) for k in range(4)]),
+                    (ctypes.c_void_p * 4)(*[c1.plane_ptr(k, t1) for k in range(4)]), col_ptr, cur.mbw, cur.mbh,
+                    int(qp), int(me_range), int(subpel_rounds), 1 if partitions else 0)
+    _check(load().pcme_b_field(ctypes.byref(job), out_ptr, _stream_ptr(stream)))
+
+
 def b_field(cur: DeviceClip, t: int, ref0: tuple[DeviceClip, int], ref1: tuple[DeviceClip, int], col, qp: int,
             me_range: int = 32, subpel_rounds: int = 1, partitions: bool = True, stream=None):
     """pcme_b_field: the ME-only field of B picture t of `cur` (all of its
@@ -98,18 +112,15 @@ def b_field(cur: DeviceClip, t: int, ref0: tuple[DeviceClip, int], ref1: tuple[D
     picture ref0 and list-1 picture ref1 ((clip, index) pairs, prepared).
     col: [mbh, mbw] B_COL_DTYPE records of the list-1 picture's MBs (host).
     Returns the [mbh, mbw] B_MB_DTYPE records."""
-    from . import BFieldJob, B_COL_DTYPE, B_MB_DTYPE
+    from . import B_COL_DTYPE, B_MB_DTYPE
     torch = cur.torch
     dev = cur.raw.device
     c = np.ascontiguousarray(np.asarray(col).reshape(-1).view(B_COL_DTYPE))
     assert c.size == cur.mbw * cur.mbh
     dcol = torch.as_tensor(c.view(np.uint8)).to(dev)
     out = torch.empty(cur.mbw * cur.mbh * B_MB_DTYPE.itemsize, dtype=torch.uint8, device=dev)
-    (c0, t0), (c1, t1) = ref0, ref1
-    job = BFieldJob(cur.plane_ptr(0, t), (ctypes.c_void_p * 4)(*[c0.plane_ptr(k, t0) for k in range(4)]),
-                    (ctypes.c_void_p * 4)(*[c1.plane_ptr(k, t1) for k in range(4)]), dcol.data_ptr(), cur.mbw,
-                    cur.mbh, int(qp), int(me_range), int(subpel_rounds), 1 if partitions else 0)
-    _check(load().pcme_b_field(ctypes.byref(job), out.data_ptr(), _stream_ptr(stream)))
+    b_field_launch(cur, t, ref0, ref1, dcol.data_ptr(), out.data_ptr(), qp, me_range, subpel_rounds, partitions,
+                   stream)
     torch.cuda.synchronize()
     return out.cpu().numpy().view(B_MB_DTYPE).reshape(cur.mbh, cur.mbw)
 
@@ -127,4 +138,5 @@ def col_records(field_p, mbh: int, mbw: int) -> np.ndarray:
     return col
 
 
-__all__ = ["B_JOB_DTYPE", "B_OUT_DTYPE", "NB_DTYPE", "b_jobs", "b_analyse", "b_field", "col_records", "vlim_rows"]
+__all__ = ["B_JOB_DTYPE", "B_OUT_DTYPE", "NB_DTYPE", "b_jobs", "b_analyse", "b_field", "b_field_launch", "col_records",
+           "vlim_rows"]
