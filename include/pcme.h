@@ -15,8 +15,9 @@
  * threads at once, each on its own stream, as the reference's frame threads
  * call its search (enc/mod.rs:207-238).  The per-call entry points
  * (pcme_search_batch, pcme_search_part_batch, pcme_b_analyse_batch,
- * pcme_mbtree_propagate) take a workspace of their own for each call (job
- * table, node windows) from a per-device pool; a workspace returns to the
+ * pcme_b_field, pcme_skip_gate_batch, pcme_mbtree_propagate) take a
+ * workspace of their own for each call (job table, node windows) from a
+ * per-device pool; a workspace returns to the
  * pool when the call's last kernel has completed (a HIP event), so calls on
  * different streams run concurrently and never share a buffer a running
  * kernel reads.  Launches of one plan (pcme_plan_*, and the plan
