@@ -76,7 +76,9 @@ def parse():
                     help="references of list 0 per P frame (0: the workload's own, 4 for 4k, else 1)")
     ap.add_argument("--ncand", type=int, default=5, help="pruner candidates per MB")
     ap.add_argument("--weights", default=None, help="pruner weights (.npz; default: the committed ones)")
-    ap.add_argument("--thr", type=float, default=0.7, help="pruner confidence threshold (partition head)")
+    ap.add_argument("--thr", default=None,
+                    help="pruner confidence threshold (partition head): one for every QP (0.7), per QP "
+                         "(22:0.7,27:0.7,32:0.8,37:0.9), or the per-QP default rule (omitted)")
     ap.add_argument("--thr-mv", type=float, default=0.0,
                     help="threshold on the motion head's candidate mass (0: the partition head decides)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -113,9 +115,10 @@ def measured_bd_rate(ncand, thr_part, thr_mv, workload, nref=1):
     """The pruned search's BD-rate against the plain search on real
     bitstreams (the field coder, QP 22/27/32/37, held-out clips of each
     picture size of the workload, every stream decoded bit-exact), from the
-    evaluation summarised in profiles/ for the default confidence rule; None
-    for other rules or without the summary."""
-    if not os.path.exists(BDRATE_SUMMARY) or (ncand, thr_part, thr_mv) != (5, 0.7, 0.0):
+    evaluation summarised in profiles/ for the default confidence rule (the
+    per-QP thresholds, thr_part None); None for other rules or without the
+    summary."""
+    if not os.path.exists(BDRATE_SUMMARY) or (ncand, thr_part, thr_mv) != (5, None, 0.0):
         return None
     per = {}
     with open(BDRATE_SUMMARY) as f:
@@ -723,6 +726,7 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
         return {"skipped": "no trained weights (pcme/pruner_weights.npz)"}
     sp = stream.cuda_stream
     thr_mv = args.thr_mv
+    thr_rule = P.parse_thr(args.thr)
     nets, jobs, pouts, hints, groups = {}, [], [], [], {}
     for dc, (spec, qp, rng) in zip(clips, specs):
         if qp not in nets:
@@ -745,7 +749,7 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
 
     def infer():
         for qp, js in groups.items():
-            nets[qp].hints(js, args.ncand, args.thr, thr_mv, sp)
+            nets[qp].hints(js, args.ncand, P.thr_part_for(qp, thr_rule), thr_mv, sp)
 
     prepare()
     infer()
@@ -803,12 +807,14 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
                          max(int((hs["prune"] != 0).sum()), 1)],
         "jcost_increase": round(j["jcost_increase"], 6),
         "pruned_fraction": round(float((hs["prune"] != 0).mean()), 5),
-        "ncand": args.ncand, "confidence_threshold": args.thr, "mv_mass_threshold": thr_mv,
+        "ncand": args.ncand,
+        "confidence_threshold": {str(q): P.thr_part_for(q, thr_rule) for q in sorted(nets)},
+        "mv_mass_threshold": thr_mv,
         "autotune": tune,
         "net_roofline": {"bound": "mfma", "kernel": "hints_kernel", "achieved": round(tflops, 3),
                          "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5), "macs_per_mb": NET_MACS_PER_MB},
-        "bd_rate": measured_bd_rate(args.ncand, args.thr, thr_mv, args.workload, args.nref),
+        "bd_rate": measured_bd_rate(args.ncand, thr_rule, thr_mv, args.workload, args.nref),
     }
 
 

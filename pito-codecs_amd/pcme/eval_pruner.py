@@ -89,7 +89,7 @@ def rd_point_h264(y, u, v, field: np.ndarray, qp: int, nref: int = 1) -> tuple[f
     return float(c.p_bits), c.p_psnr
 
 
-def evaluate(frames: int, seed: int, ncand: int = 5, thr_part: float = 0.7, thr_mv: float = 0.0,
+def evaluate(frames: int, seed: int, ncand: int = 5, thr_part=None, thr_mv: float = 0.0,
              log=print, w: int = 1920, h: int = 1080, weights: str | None = None, coder: str = "h264",
              nref: int = 1) -> dict:
     y, u, v = synth_clip(clip_spec(seed, frames + 1, w, h))
@@ -106,7 +106,8 @@ def evaluate(frames: int, seed: int, ncand: int = 5, thr_part: float = 0.7, thr_
     for qp in QPS:
         net = P.Net(seed=1, max_batch=64, weights=weights or P.WEIGHTS, qp=qp)
         plain, _ = P.search_clip(dc, rng, qp, prepared=True, nref=nref)
-        pruned, hints = P.search_clip(dc, rng, qp, net=net, ncand=ncand, thr_part=thr_part, thr_mv=thr_mv,
+        pruned, hints = P.search_clip(dc, rng, qp, net=net, ncand=ncand, thr_part=P.thr_part_for(qp, thr_part),
+                                      thr_mv=thr_mv,
                                       prepared=True, nref=nref)
         none = np.zeros(hints.shape, HINT_DTYPE)
         none["prune"] = 1
@@ -121,7 +122,8 @@ def evaluate(frames: int, seed: int, ncand: int = 5, thr_part: float = 0.7, thr_
                      "pruned_fraction": float(np.mean(hints["prune"] != 0))}
         log(f"qp {qp}: " + ", ".join(f"{k} {pts[k][-1][0] / 1e3:.0f} kbit {pts[k][-1][1]:.3f} dB" for k in pts))
     rep = {
-        "frames": frames, "seed": seed, "size": [w, h], "qps": list(QPS), "ncand": ncand, "thr_part": thr_part, "thr_mv": thr_mv,
+        "frames": frames, "seed": seed, "size": [w, h], "qps": list(QPS), "ncand": ncand,
+        "thr_part": {str(q): P.thr_part_for(q, thr_part) for q in QPS}, "thr_mv": thr_mv,
         "nref": nref, "me_range": rng, "coder": coder, "decoded_bit_exact": coder == "h264",
         "points": {k: [[round(r, 1), round(q, 4)] for r, q in v] for k, v in pts.items()},
         "bd_rate_pruned_pct": round(R.bd_rate(pts["plain"], pts["pruned"]), 4),
@@ -141,7 +143,9 @@ def main(argv=None) -> int:
     ap.add_argument("--frames", type=int, default=16, help="P frames per QP")
     ap.add_argument("--seed", type=lambda s: int(s, 0), default=0x5EED9200)
     ap.add_argument("--ncand", type=int, default=5)
-    ap.add_argument("--thr", type=float, default=0.7, help="partition-head confidence threshold")
+    ap.add_argument("--thr", default=None,
+                    help="partition-head confidence threshold: one for every QP (0.7), per QP "
+                         "(22:0.7,27:0.7,32:0.8,37:0.9), or the per-QP default (omitted)")
     ap.add_argument("--thr-mv", type=float, default=0.0, help="motion-head candidate-mass threshold")
     ap.add_argument("--weights", default=None, help="pruner weights (.npz; default: the committed ones)")
     ap.add_argument("--coder", choices=("h264", "model"), default="h264")
@@ -154,7 +158,7 @@ def main(argv=None) -> int:
         print("the evaluation needs the GPU", file=sys.stderr)
         return 2
     w, h = (int(t) for t in a.size.split("x"))
-    rep = evaluate(a.frames, a.seed, a.ncand, a.thr, a.thr_mv, log=lambda s: print(s, file=sys.stderr, flush=True),
+    rep = evaluate(a.frames, a.seed, a.ncand, P.parse_thr(a.thr), a.thr_mv, log=lambda s: print(s, file=sys.stderr, flush=True),
                    w=w, h=h, weights=a.weights, coder=a.coder, nref=a.nref)
     print(json.dumps(rep))
     if a.out:

@@ -24,6 +24,33 @@ MV_SIDE = 17
 INPUT = 1152
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "pruner_weights.npz")
 
+# Partition-head confidence below which an MB falls back to the full search,
+# per QP (one network per QP).  At the high QPs a pruned MB's lost mode or
+# motion costs more of the picture's rate, so those networks prune only more
+# confident MBs.
+THR_PART = {22: 0.7, 27: 0.7, 32: 0.8, 37: 0.9}
+
+
+def parse_thr(text: str | None):
+    """A fallback rule from the command line: None or "" (THR_PART), one
+    threshold for every QP ("0.7"), or per-QP thresholds
+    ("22:0.7,27:0.7,32:0.8,37:0.9")."""
+    if text is None or text == "":
+        return None
+    if ":" not in text:
+        return float(text)
+    return {int(k): float(v) for k, v in (kv.split(":") for kv in text.split(","))}
+
+
+def thr_part_for(qp: int, thr=None) -> float:
+    """The fallback threshold of QP `qp` under rule `thr` (see parse_thr):
+    THR_PART's entry when `thr` is None, 0.7 for a QP the table lacks."""
+    if thr is None:
+        thr = THR_PART
+    if isinstance(thr, dict):
+        return float(thr.get(qp, 0.7))
+    return float(thr)
+
 _sig_done = False
 
 

@@ -84,7 +84,8 @@ def test_pruned_1080p_committed_weights(gpu, oracle, clip1080, qp):
     y, dc = clip1080
     say("pruned 1080p, committed weights, 2 P frames, qp", qp)
     net = P.Net(seed=1, max_batch=1024, weights=P.WEIGHTS, qp=qp)
-    field, hints = P.search_clip(dc, 32, qp, net=net, ncand=5, thr_part=0.7, thr_mv=0.0, prepared=True)
+    field, hints = P.search_clip(dc, 32, qp, net=net, ncand=5, thr_part=P.thr_part_for(qp), thr_mv=0.0,
+                                 prepared=True)
     net.close()
     assert 0.0 < float(np.mean(hints["prune"])) < 1.0  # both the pruned and the fallback path run
     mbw, mbh = dc.mbw, dc.mbh
