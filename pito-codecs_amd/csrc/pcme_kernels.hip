@@ -1567,6 +1567,27 @@ __device__ __forceinline__ uint32_t load_hint(const pcme_hint *h) {
   return v;
 }
 
+// Timeline build (`make timeline`, -DPCME_TIMELINE; never the product
+// library): the frame walkers stamp four moments of every MB of one job
+// (g_tl_job) with the device's constant-rate clock (s_memrealtime, 100 MHz):
+// [0] the MB's turn begins, [1] its top-right neighbour's words have been
+// seen, [2] its analysis is done, [3] its own words are published.  The
+// stamps show the per-MB analysis time, the wait for the row above and the
+// hand-off latency between rows.
+#ifdef PCME_TIMELINE
+#define TL_MBS 32768
+__device__ unsigned long long g_timeline[TL_MBS * 4];
+__device__ int g_tl_job;
+#define TL_STAMP(i)                                                                     \
+  do {                                                                                  \
+    if (tl_on && l == 0) g_timeline[tl_mb * 4 + (i)] = __builtin_amdgcn_s_memrealtime(); \
+  } while (0)
+#else
+#define TL_STAMP(i) \
+  do {              \
+  } while (0)
+#endif
+
 template <bool HINTS, bool REUSE = true, bool MR = false>
 __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
                                             int ntickets, uint32_t *ctl, uint32_t spin_limit, uint32_t *s_lds,
@@ -1652,6 +1673,11 @@ __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, con
     uint32_t hnext = 0u;
     if (hinted) hnext = load_hint(J.hint + (size_t)my * mbw);
     for (int mx = 0; mx < mbw; mx++) {
+#ifdef PCME_TIMELINE
+      const bool tl_on = tk.x == g_tl_job && my * mbw + mx < TL_MBS;
+      const size_t tl_mb = (size_t)(my * mbw + mx);
+#endif
+      TL_STAMP(0);
       if (my > 0 && mx > 0) {
         n.TL = n.T[3];
 #pragma unroll
@@ -1664,6 +1690,7 @@ __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, con
           if (!fetch_words(xup + (size_t)(mx + 1) * 4, tag, err, spin_limit, nxt, MR ? &rnxt : &rdummy)) return;
         }
       }
+      TL_STAMP(1);
       n.aT = my > 0;
       n.aTL = my > 0 && mx > 0;
       n.aTR = my > 0 && mx + 1 < mbw;
@@ -1700,12 +1727,14 @@ __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, con
         else
           analyse_mb<false>(c, n, src, 16, 16 * mx, 16 * my, J.partitions, hv, nh, hpart, o, s_lds, ar);
       }
+      TL_STAMP(2);
       store_out(J.out + (size_t)my * mbw + mx, o);
       // Publish the bottom row (blocks 12..15) for the row below.
       if (l < 4) {
         uint32_t wv = xword(tag, MR ? (uint32_t)o.ref : 0u, blk_mv(o, 12 + l));
         __hip_atomic_store(xrow + (size_t)mx * 4 + l, wv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
+      TL_STAMP(3);
       // Left column (blocks 3, 7, 11, 15) for the next MB of this row.
       n.aL = true;
 #pragma unroll
@@ -2595,3 +2624,22 @@ __global__ void __launch_bounds__(256) clear_kernel(const uint32_t *ctl, uint32_
 }
 
 }  // namespace pcme
+
+#ifdef PCME_TIMELINE
+// Timeline build only: choose the recorded job and clear its stamps; read
+// them back (4 x u64 per MB, raster order, 0 = not reached).
+extern "C" int pcme_timeline_set(int job) {
+  static unsigned long long zero[TL_MBS * 4];
+  if (hipMemcpyToSymbol(HIP_SYMBOL(pcme::g_tl_job), &job, sizeof job, 0, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpyToSymbol(HIP_SYMBOL(pcme::g_timeline), zero, sizeof zero, 0, hipMemcpyHostToDevice) != hipSuccess)
+    return PCME_EHIP;
+  return PCME_OK;
+}
+extern "C" int pcme_timeline_read(unsigned long long *dst, int mbs) {
+  if (!dst || mbs < 0 || mbs > TL_MBS) return PCME_EINVAL;
+  return hipMemcpyFromSymbol(dst, HIP_SYMBOL(pcme::g_timeline), (size_t)mbs * 32, 0, hipMemcpyDeviceToHost) ==
+                 hipSuccess
+             ? PCME_OK
+             : PCME_EHIP;
+}
+#endif
