@@ -265,12 +265,14 @@ typedef struct pcme_plan pcme_plan;
  * configuration gives the same records).  waves: the persistent grid (0 =
  * every wave the configuration keeps resident); waves_per_group: row
  * walkers per workgroup (1, 2 or 4); waves_per_cu: the occupancy cap the
- * LDS each walker reserves sets (12 or 8; for a plan with pruner hints its
- * kernel's own); reuse_regions: 1 = a walker keeps staged LDS regions from
- * MB to MB, 0 = every search stages its own; ms: the time per launch the
- * autotuner measured.  The built configurations are (1, 12, 1) -- the
- * default --, (2, 12, 1), (4, 12, 1), (1, 8, 1), (2, 8, 1) and (1, 12, 0);
- * a plan with hints runs (1, *, 1) only.
+ * LDS each walker reserves sets (16, 12 or 8; for a plan with pruner hints
+ * its kernel's own; at 16 a walker stages each MB's source block after the
+ * previous MB instead of one MB ahead); reuse_regions: 1 = a walker keeps
+ * staged LDS regions from MB to MB, 0 = every search stages its own; ms:
+ * the time per launch the autotuner measured.  The built configurations are
+ * (1, 12, 1) -- the default --, (2, 12, 1), (4, 12, 1), (1, 8, 1),
+ * (2, 8, 1), (1, 12, 0), (4, 16, 0) and (4, 16, 1); a plan with hints runs
+ * (1, *, 1) only.
  * pcme_plan_get_config reads the plan's configuration; pcme_plan_set_config
  * selects one (PCME_EINVAL for a configuration that is not built);
  * pcme_plan_autotune_ex times every built configuration over the grid

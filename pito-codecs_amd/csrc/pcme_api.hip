@@ -98,7 +98,8 @@ struct Variant {
   bool reuse;
 };
 static const Variant kVariants[] = {{1, 12, true}, {2, 12, true}, {4, 12, true},
-                                    {1, 8, true},  {2, 8, true},  {1, 12, false}};
+                                    {1, 8, true},  {2, 8, true},  {1, 12, false},
+                                    {4, 16, false}, {4, 16, true}};
 static const int kNumVariants = (int)(sizeof kVariants / sizeof kVariants[0]);
 static_assert(sizeof kVariants / sizeof kVariants[0] <= 8, "Dev::resident_var holds 8 configurations");
 
@@ -109,6 +110,8 @@ static const void *variant_fn(int v) {
     case 3: return (const void *)search_frames_var_kernel<1, 8, true>;
     case 4: return (const void *)search_frames_var_kernel<2, 8, true>;
     case 5: return (const void *)search_frames_var_kernel<1, 12, false>;
+    case 6: return (const void *)search_frames_var_kernel<4, 16, false>;
+    case 7: return (const void *)search_frames_var_kernel<4, 16, true>;
     default: return (const void *)search_frames_kernel;
   }
 }
@@ -468,6 +471,10 @@ static int plan_launch(Plan *p, hipStream_t stream) {
       case 4: hipLaunchKernelGGL((search_frames_var_kernel<2, 8, true>), g, b, 0, stream, p->djobs, p->dtickets,
                                  p->ntickets, p->ctl, kSpinLimit); break;
       case 5: hipLaunchKernelGGL((search_frames_var_kernel<1, 12, false>), g, b, 0, stream, p->djobs, p->dtickets,
+                                 p->ntickets, p->ctl, kSpinLimit); break;
+      case 6: hipLaunchKernelGGL((search_frames_var_kernel<4, 16, false>), g, b, 0, stream, p->djobs, p->dtickets,
+                                 p->ntickets, p->ctl, kSpinLimit); break;
+      case 7: hipLaunchKernelGGL((search_frames_var_kernel<4, 16, true>), g, b, 0, stream, p->djobs, p->dtickets,
                                  p->ntickets, p->ctl, kSpinLimit); break;
       default: hipLaunchKernelGGL(search_frames_kernel, g, b, 0, stream, p->djobs, p->dtickets, p->ntickets, p->ctl,
                                   kSpinLimit); break;

@@ -1588,7 +1588,10 @@ __device__ int g_tl_job;
   } while (0)
 #endif
 
-template <bool HINTS, bool REUSE = true, bool MR = false>
+// SRC2: the MB source blocks travel one MB ahead through two LDS buffers;
+// without it one buffer is refilled at the end of each MB (the 16-waves-
+// per-CU configurations, whose LDS share has room for one).
+template <bool HINTS, bool REUSE = true, bool MR = false, bool SRC2 = true>
 __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
                                             int ntickets, uint32_t *ctl, uint32_t spin_limit, uint32_t *s_lds,
                                             uint32_t *s_src) {
@@ -1697,8 +1700,8 @@ __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, con
       n.TR = nxt[0];
       if constexpr (MR) n.rTR = rnxt;
       dma_wait();  // this MB's source block
-      if (mx + 1 < mbw) dma_block16(srow + 16 * (mx + 1), stride, s_src + 64 * ((mx + 1) & 1));
-      const uint8_t *src = (const uint8_t *)(s_src + 64 * (mx & 1));
+      if (SRC2 && mx + 1 < mbw) dma_block16(srow + 16 * (mx + 1), stride, s_src + 64 * ((mx + 1) & 1));
+      const uint8_t *src = (const uint8_t *)(s_src + (SRC2 ? 64 * (mx & 1) : 0));
       // Pruner hint of this MB; a full search without one.
       uint32_t hv = 0u;
       int nh = -1, hpart = 0;
@@ -1728,6 +1731,10 @@ __device__ __forceinline__ void frames_body(const DevJob *__restrict__ jobs, con
           analyse_mb<false>(c, n, src, 16, 16 * mx, 16 * my, J.partitions, hv, nh, hpart, o, s_lds, ar);
       }
       TL_STAMP(2);
+      if (!SRC2 && mx + 1 < mbw) {
+        lds_fence();  // every read of this MB's source block has returned
+        dma_block16(srow + 16 * (mx + 1), stride, s_src);
+      }
       store_out(J.out + (size_t)my * mbw + mx, o);
       // Publish the bottom row (blocks 12..15) for the row below.
       if (l < 4) {
@@ -1788,15 +1795,19 @@ __global__ void __launch_bounds__(64) search_frames_mr_hinted_kernel(const DevJo
 // walkers (waves) per workgroup, each reserving the LDS that caps a CU at
 // WCU waves, and with or without the staged regions kept along a row.
 // search_frames_kernel is <1, 12, true>.
+// With 16 waves per CU (4 per SIMD, 128 VGPRs) a wave's LDS share holds the
+// two search areas and one source block.
 template <int WPG, int WCU, bool REUSE>
-__global__ void __launch_bounds__(64 * WPG) search_frames_var_kernel(const DevJob *__restrict__ jobs,
+__global__ void __launch_bounds__(64 * WPG, WCU > 12 ? 4 : 1) search_frames_var_kernel(const DevJob *__restrict__ jobs,
                                                                      const int2 *__restrict__ tickets, int ntickets,
                                                                      uint32_t *ctl, uint32_t spin_limit) {
+  constexpr bool SRC2 = WCU <= 12;
   constexpr int kWords = (163840 / WCU / 4) & ~3;
-  static_assert(kWords * 4 >= LDS_PER_WAVE + 512 && 163840 / (kWords * 4) == WCU, "LDS per wave for WCU waves per CU");
+  static_assert(kWords * 4 >= LDS_PER_WAVE + (SRC2 ? 512 : 256) && 163840 / (kWords * 4) == WCU,
+                "LDS per wave for WCU waves per CU");
   __shared__ __attribute__((aligned(16))) uint32_t s_lds[WPG * kWords];
   uint32_t *mine = s_lds + (threadIdx.x >> 6) * kWords;
-  frames_body<false, REUSE>(jobs, tickets, ntickets, ctl, spin_limit, mine, mine + LDS_PER_WAVE / 4);
+  frames_body<false, REUSE, false, SRC2>(jobs, tickets, ntickets, ctl, spin_limit, mine, mine + LDS_PER_WAVE / 4);
 }
 template __global__ void search_frames_var_kernel<2, 12, true>(const DevJob *__restrict__, const int2 *__restrict__,
                                                                 int, uint32_t *, uint32_t);
@@ -1808,6 +1819,10 @@ template __global__ void search_frames_var_kernel<2, 8, true>(const DevJob *__re
                                                                int, uint32_t *, uint32_t);
 template __global__ void search_frames_var_kernel<1, 12, false>(const DevJob *__restrict__, const int2 *__restrict__,
                                                                  int, uint32_t *, uint32_t);
+template __global__ void search_frames_var_kernel<4, 16, false>(const DevJob *__restrict__, const int2 *__restrict__,
+                                                                 int, uint32_t *, uint32_t);
+template __global__ void search_frames_var_kernel<4, 16, true>(const DevJob *__restrict__, const int2 *__restrict__,
+                                                                int, uint32_t *, uint32_t);
 
 // -------------------------------------------------- per-call search_part
 

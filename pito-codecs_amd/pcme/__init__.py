@@ -156,7 +156,7 @@ class Tune(ctypes.Structure):
 
 
 # The kernel configurations libpcme builds: (waves_per_group, waves_per_cu, reuse_regions).
-CONFIGS = ((1, 12, 1), (2, 12, 1), (4, 12, 1), (1, 8, 1), (2, 8, 1), (1, 12, 0))
+CONFIGS = ((1, 12, 1), (2, 12, 1), (4, 12, 1), (1, 8, 1), (2, 8, 1), (1, 12, 0), (4, 16, 0), (4, 16, 1))
 
 
 class SynthCfg(ctypes.Structure):
