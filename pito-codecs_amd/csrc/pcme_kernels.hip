@@ -1791,6 +1791,19 @@ __global__ void __launch_bounds__(64) search_frames_mr_hinted_kernel(const DevJo
   frames_body<true, true, true>(jobs, tickets, ntickets, ctl, spin_limit, s_lds, s_lds + LDS_PER_WAVE / 4);
 }
 
+// The multi-reference field at 16 waves per CU: 4 walkers per workgroup,
+// each with the two search areas and one source block (the 4K workload is
+// bound by throughput, where the fourth wave per SIMD pays).
+#define MR16_WORDS (163840 / 16 / 4)
+static_assert(MR16_WORDS * 4 >= LDS_PER_WAVE + 256, "multi-reference walkers at 16 per CU");
+__global__ void __launch_bounds__(256, 4) search_frames_mr16_kernel(const DevJob *__restrict__ jobs,
+                                                                   const int2 *__restrict__ tickets, int ntickets,
+                                                                   uint32_t *ctl, uint32_t spin_limit) {
+  __shared__ __attribute__((aligned(16))) uint32_t s_lds[4 * MR16_WORDS];
+  uint32_t *mine = s_lds + (threadIdx.x >> 6) * MR16_WORDS;
+  frames_body<false, true, true, false>(jobs, tickets, ntickets, ctl, spin_limit, mine, mine + LDS_PER_WAVE / 4);
+}
+
 // Configurations of the full-search kernel for the autotuner: WPG row
 // walkers (waves) per workgroup, each reserving the LDS that caps a CU at
 // WCU waves, and with or without the staged regions kept along a row.

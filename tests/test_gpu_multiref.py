@@ -122,3 +122,36 @@ def test_multiref_entry_points_and_tag_wrap(gpu):
     f0 = first.view(gpu.MB_DTYPE).reshape(4, mbh, mbw)
     for k, name in ((1, "plan x130"), (2, "per-call"), (3, "noalloc")):
         assert_same_field(outs[k].cpu().numpy().view(gpu.MB_DTYPE).reshape(4, mbh, mbw), f0, name)
+
+
+def test_multiref_plan_configurations(gpu, oracle):
+    """A multi-reference plan runs the default configuration and (4, 16, 1)
+    (four walkers per workgroup at 16 per CU), at its resident grid and a
+    small one, bit for bit the oracle's field; other configurations are
+    refused, and autotune_ex keeps one of the two."""
+    import torch
+    w, h, nref, rng, qp = 320, 240, 3, 16, 27
+    say("multiref plan configurations", w, h, nref)
+    y = _synth(gpu, w, h, nref + 2, 0x5EED5A00, nrect=5, vmax=5.0)
+    dc = _dclip(gpu, y)
+    nb = dc.n - 1
+    out = torch.zeros((nb, dc.mbh * dc.mbw * gpu.MB_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+    jobs = [gpu.frame_job(dc, t, t - 1, out[t - 1].data_ptr(), rng, 1, True, qp,
+                          older=[t - 1 - k for k in range(1, min(t, nref))]) for t in range(1, dc.n)]
+    plan = gpu.Plan(jobs)
+    want = oracle.search_clip_mr(y, nref, rng, 1, True, qp)
+    for wpg, wcu, reuse in ((1, 12, 1), (4, 16, 1)):
+        for waves in (0, 6):
+            say("configuration", wpg, wcu, reuse, "waves", waves)
+            t = plan.set_config(waves, wpg, wcu, reuse)
+            assert (t.waves_per_group, t.waves_per_cu) == (wpg, wcu)
+            out.zero_()
+            plan.launch()
+            plan.check()
+            got = out.cpu().numpy().view(gpu.MB_DTYPE).reshape(nb, dc.mbh, dc.mbw)
+            for k in range(nb):
+                assert_same_field(got[k], want[k], f"config {wpg}/{wcu} waves {waves} frame {k + 1}")
+    with pytest.raises(gpu.PcmeError):
+        plan.set_config(0, 1, 12, 0)
+    best = plan.autotune_ex(reps=1)
+    assert (best.waves_per_group, best.waves_per_cu) in ((1, 12), (4, 16))
