@@ -271,8 +271,8 @@ typedef struct pcme_plan pcme_plan;
  * staged LDS regions from MB to MB, 0 = every search stages its own; ms:
  * the time per launch the autotuner measured.  The built configurations are
  * (1, 12, 1) -- the default --, (2, 12, 1), (4, 12, 1), (1, 8, 1),
- * (2, 8, 1), (1, 12, 0), (4, 16, 0) and (4, 16, 1); a plan with hints runs
- * (1, *, 1) only.
+ * (2, 8, 1), (1, 12, 0), (4, 16, 0) and (4, 16, 1); a plan with pruner
+ * hints or with several references runs its kernel's default or (4, 16, 1).
  * pcme_plan_get_config reads the plan's configuration; pcme_plan_set_config
  * selects one (PCME_EINVAL for a configuration that is not built);
  * pcme_plan_autotune_ex times every built configuration over the grid

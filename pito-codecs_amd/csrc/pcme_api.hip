@@ -66,7 +66,7 @@ struct Dev {
   std::mutex mu;
   int resident_waves[4] = {0, 0, 0, 0};  // full-search / hinted kernel; the same with several references
   int resident_var[8] = {0};       // resident waves of each full-search configuration (kVariants)
-  int resident_mr16 = 0;           // resident waves of the multi-reference kernel at 16 per CU
+  int resident_16[3] = {0, 0, 0};  // resident waves at 16 per CU: hinted, multi-reference, both
   int cus = 1;                     // compute units
   std::vector<CallWs *> ws;        // per-call workspaces (at most kMaxWs)
   Plan *cached = nullptr;          // plan of the last pcme_search_frames call
@@ -102,10 +102,10 @@ static const Variant kVariants[] = {{1, 12, true}, {2, 12, true}, {4, 12, true},
                                     {1, 8, true},  {2, 8, true},  {1, 12, false},
                                     {4, 16, false}, {4, 16, true}};
 static const int kNumVariants = (int)(sizeof kVariants / sizeof kVariants[0]);
-// A multi-reference plan without hints runs the default configuration or
-// (4, 16, 1), kVariants[kMr16] (search_frames_mr16_kernel).
-static const int kMr16 = 7;
-static_assert(kMr16 < kNumVariants, "kMr16 names a built configuration");
+// A plan with pruner hints, several references or both runs its kernel's
+// default configuration or (4, 16, 1), kVariants[k16] (kernel16()).
+static const int k16 = 7;
+static_assert(k16 < kNumVariants, "k16 names a built configuration");
 static_assert(sizeof kVariants / sizeof kVariants[0] <= 8, "Dev::resident_var holds 8 configurations");
 
 static const void *variant_fn(int v) {
@@ -142,6 +142,12 @@ static FrameKernel frame_kernel(bool hinted, bool mr) {
   return hinted ? search_frames_hinted_kernel : search_frames_kernel;
 }
 
+// The same kinds at 16 waves per CU (4 walkers per workgroup of 256).
+static FrameKernel kernel16(bool hinted, bool mr) {
+  if (mr) return hinted ? search_frames_mr_hinted16_kernel : search_frames_mr16_kernel;
+  return search_frames_hinted16_kernel;
+}
+
 // The current device's state; the first call per device validates it.
 // Later calls only read the current device id, so entry points stay cheap
 // and safe inside a stream capture.
@@ -168,10 +174,10 @@ static int get_dev(Dev **out) {
     dv->resident_waves[v] = per_cu * p.multiProcessorCount;
   }
   dv->cus = p.multiProcessorCount;
-  {
+  for (int k = 0; k < 3; k++) {
     int per_cu = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, search_frames_mr16_kernel, 256, 0));
-    dv->resident_mr16 = std::max(1, per_cu) * 4 * p.multiProcessorCount;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kernel16(k != 1, k != 0), 256, 0));
+    dv->resident_16[k] = std::max(1, per_cu) * 4 * p.multiProcessorCount;
   }
   for (int v = 0; v < kNumVariants; v++) {
     int per_cu = 0;
@@ -310,9 +316,9 @@ struct Plan {
   bool hinted = false;      // some job carries pruner hints: the hinted kernel
   bool mr = false;          // some job has several references: the multi-reference kernels
   int grid = 0;             // waves launched (persistent grid)
-  int variant = 0;          // configuration of the kernel (kVariants; a multi-reference plan: 0 or kMr16)
+  int variant = 0;          // configuration of the kernel (kVariants; with hints or references: 0 or k16)
   const int *resident_var = nullptr;  // the device's resident waves per configuration
-  int resident_mr16 = 0;    // the device's resident waves of search_frames_mr16_kernel
+  int resident16 = 0;       // the device's resident waves of the plan's kernel at 16 per CU
   int cus = 1;              // compute units of the device
   hipEvent_t ev = nullptr;  // recorded after each eager launch: the next launch, on any stream, waits
   bool ev_recorded = false;
@@ -376,18 +382,19 @@ static void plan_free(Plan *p) {
 }
 
 // Resident waves of the plan's kernel configuration on its device.
-static bool plan_mr16(const Plan *p) { return p->mr && !p->hinted && p->variant == kMr16; }
+// Whether the plan runs the 16-per-CU kernel of its kind (hinted,
+// multi-reference or both).
+static bool plan_16(const Plan *p) { return (p->hinted || p->mr) && p->variant == k16; }
 
 static int plan_resident(const Plan *p) {
-  if (plan_mr16(p)) return p->resident_mr16;
+  if (plan_16(p)) return p->resident16;
   return (p->hinted || p->mr) ? p->resident : p->resident_var[p->variant];
 }
 
-// The configurations a plan can run: every built one for the full search,
-// the default or (4, 16, 1) with several references, the default with hints.
+// The configurations a plan can run: every built one for the full search;
+// the default or (4, 16, 1) with pruner hints or several references.
 static bool plan_allows(const Plan *p, int v) {
-  if (p->hinted) return v == 0;
-  if (p->mr) return v == 0 || v == kMr16;
+  if (p->hinted || p->mr) return v == 0 || v == k16;
   return v >= 0 && v < kNumVariants;
 }
 
@@ -401,7 +408,7 @@ static int plan_create(Dev *dv, const pcme_frame_job *jobs, int njobs, Plan **ou
   }
   p->resident = dv->resident_waves[(p->hinted ? 1 : 0) + (p->mr ? 2 : 0)];
   p->resident_var = dv->resident_var;
-  p->resident_mr16 = dv->resident_mr16;
+  p->resident16 = dv->resident_16[p->hinted && p->mr ? 2 : (p->mr ? 1 : 0)];
   p->cus = dv->cus;
   std::vector<DevJob> dj((size_t)njobs);
   size_t words = 0;
@@ -478,8 +485,8 @@ static int plan_launch(Plan *p, hipStream_t stream) {
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(clear_kernel, dim3(512), dim3(256), 0, stream, p->ctl, p->xchg, p->nwords);
   HIPCHK(hipGetLastError());
-  if (plan_mr16(p)) {
-    hipLaunchKernelGGL(search_frames_mr16_kernel, dim3((unsigned)((p->grid + 3) / 4)), dim3(256), 0, stream,
+  if (plan_16(p)) {
+    hipLaunchKernelGGL(kernel16(p->hinted, p->mr), dim3((unsigned)((p->grid + 3) / 4)), dim3(256), 0, stream,
                        p->djobs, p->dtickets, p->ntickets, p->ctl, kSpinLimit);
   } else if (p->hinted || p->mr) {
     hipLaunchKernelGGL(frame_kernel(p->hinted, p->mr), dim3((unsigned)p->grid), dim3(64), 0, stream, p->djobs,
@@ -793,7 +800,7 @@ int pcme_plan_get_config(pcme_plan *plan, pcme_tune *out) {
   out->waves = p->grid;
   out->waves_per_group = v.wpg;
   out->waves_per_cu =
-      ((p->hinted || p->mr) && !plan_mr16(p)) ? std::max(1, p->resident / std::max(1, p->cus)) : v.wcu;
+      ((p->hinted || p->mr) && !plan_16(p)) ? std::max(1, p->resident / std::max(1, p->cus)) : v.wcu;
   out->reuse_regions = v.reuse ? 1 : 0;
   out->ms = 0.f;
   return PCME_OK;
@@ -811,8 +818,8 @@ int pcme_plan_set_config(pcme_plan *plan, const pcme_tune *cfg) {
     return fail(PCME_EINVAL, "no kernel configuration with %d waves per group, %d waves per CU, region reuse %d",
                 cfg->waves_per_group, cfg->waves_per_cu, cfg->reuse_regions);
   if (!plan_allows(p, v))
-    return fail(PCME_EINVAL, "a plan with pruner hints runs the default configuration only, one with several "
-                             "references the default or (4, 16, 1)");
+    return fail(PCME_EINVAL, "a plan with pruner hints or several references runs the default configuration or "
+                             "(4, 16, 1)");
   p->variant = v;
   const int w = cfg->waves == 0 ? plan_resident(p) : cfg->waves;
   p->grid = std::max(1, std::min(w, p->ntickets));

@@ -71,6 +71,10 @@ __global__ void search_frames_mr_kernel(const DevJob *__restrict__ jobs, const i
                                         int ntickets, uint32_t *ctl, uint32_t spin_limit);
 __global__ void search_frames_mr_hinted_kernel(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
                                                int ntickets, uint32_t *ctl, uint32_t spin_limit);
+__global__ void search_frames_hinted16_kernel(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
+                                              int ntickets, uint32_t *ctl, uint32_t spin_limit);
+__global__ void search_frames_mr_hinted16_kernel(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
+                                                 int ntickets, uint32_t *ctl, uint32_t spin_limit);
 __global__ void search_frames_mr16_kernel(const DevJob *__restrict__ jobs, const int2 *__restrict__ tickets,
                                           int ntickets, uint32_t *ctl, uint32_t spin_limit);
 template <int WPG, int WCU, bool REUSE>

@@ -1804,6 +1804,29 @@ __global__ void __launch_bounds__(256, 4) search_frames_mr16_kernel(const DevJob
   frames_body<false, true, true, false>(jobs, tickets, ntickets, ctl, spin_limit, mine, mine + LDS_PER_WAVE / 4);
 }
 
+// The hinted field (pruner hints, one reference) at 16 waves per CU, for
+// workloads bound by throughput: its two MB paths need 168 VGPRs, so at
+// 128 some spill to scratch (96 bytes per lane).
+__global__ void __launch_bounds__(256, 4) search_frames_hinted16_kernel(const DevJob *__restrict__ jobs,
+                                                                       const int2 *__restrict__ tickets,
+                                                                       int ntickets, uint32_t *ctl,
+                                                                       uint32_t spin_limit) {
+  __shared__ __attribute__((aligned(16))) uint32_t s_lds[4 * MR16_WORDS];
+  uint32_t *mine = s_lds + (threadIdx.x >> 6) * MR16_WORDS;
+  frames_body<true, true, false, false>(jobs, tickets, ntickets, ctl, spin_limit, mine, mine + LDS_PER_WAVE / 4);
+}
+
+// The multi-reference field with pruner hints at 16 waves per CU (166
+// VGPRs at the default occupancy; some spill at 128).
+__global__ void __launch_bounds__(256, 4) search_frames_mr_hinted16_kernel(const DevJob *__restrict__ jobs,
+                                                                          const int2 *__restrict__ tickets,
+                                                                          int ntickets, uint32_t *ctl,
+                                                                          uint32_t spin_limit) {
+  __shared__ __attribute__((aligned(16))) uint32_t s_lds[4 * MR16_WORDS];
+  uint32_t *mine = s_lds + (threadIdx.x >> 6) * MR16_WORDS;
+  frames_body<true, true, true, false>(jobs, tickets, ntickets, ctl, spin_limit, mine, mine + LDS_PER_WAVE / 4);
+}
+
 // Configurations of the full-search kernel for the autotuner: WPG row
 // walkers (waves) per workgroup, each reserving the LDS that caps a CU at
 // WCU waves, and with or without the staged regions kept along a row.

@@ -88,6 +88,17 @@ def test_multiref_with_pruner_hints(gpu, oracle):
     want = oracle.search_clip_mr(y, 3, 16, 1, True, 27, hints=hs)
     for t in range(nb):
         assert_same_field(field[t], want[t], f"frame {t + 1}")
+    # the same table through a plan at 16 waves per CU (4 walkers per group)
+    plan = gpu.Plan(jobs)
+    for waves in (0, 6):
+        say("multiref + hints, configuration 4/16/1, waves", waves)
+        plan.set_config(waves, 4, 16, 1)
+        out.zero_()
+        plan.launch()
+        plan.check()
+        got = out.cpu().numpy().view(gpu.MB_DTYPE).reshape(nb, mbh, mbw)
+        for t in range(nb):
+            assert_same_field(got[t], want[t], f"4/16 waves {waves} frame {t + 1}")
 
 
 def test_multiref_entry_points_and_tag_wrap(gpu):

@@ -239,3 +239,45 @@ def test_bd_rate_model_runs_on_product_fields(gpu):
         assert r == sorted(r, reverse=True) and q == sorted(q, reverse=True), (k, pts[k])
     assert abs(R.bd_rate([tuple(p) for p in pts["plain"]], [tuple(p) for p in pts["plain"]])) < 1e-9
     assert np.isfinite(rep["bd_rate_pruned_pct"]) and np.isfinite(rep["bd_rate_no_network_pct"])
+
+
+def test_hinted_plan_configurations(gpu, oracle):
+    """A plan with pruner hints runs the default configuration and (4, 16, 1)
+    (the hinted kernel at 16 waves per CU), at its resident grid and a small
+    one, bit for bit the oracle's pruned field on the same hints (pruned and
+    fallback MBs mixed); autotune_ex keeps one of the two."""
+    import torch
+    import pcme.pruner as P
+    say("hinted plan configurations 320x240 x 3 frames")
+    y, dc = _dclip(gpu, 320, 240, 4, 0x5EED0207, nrect=4, rect_lo=24, rect_hi=80)
+    net = P.Net(seed=1, max_batch=256, weights=P.WEIGHTS, qp=27)
+    nb, nmb = dc.n - 1, dc.mbw * dc.mbh
+    out = torch.zeros((nb, nmb * gpu.MB_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+    hbuf = torch.zeros((nb, nmb * gpu.HINT_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+    jobs = [gpu.frame_job(dc, t, t - 1, out[t - 1].data_ptr(), 16, 1, True, 27, hint_ptr=hbuf[t - 1].data_ptr())
+            for t in range(1, dc.n)]
+    net.hints(jobs, 5, P.thr_part_for(27), 0.0)
+    net.close()
+    hints = hbuf.cpu().numpy().view(gpu.HINT_DTYPE).reshape(nb, -1)
+    assert 0.0 < float(np.mean(hints["prune"])) < 1.0
+    want = []
+    for t in range(1, dc.n):
+        src = oracle.pad_luma(y[t], dc.mbw, dc.mbh)
+        ref = oracle.Ref(oracle.pad_luma(y[t - 1], dc.mbw, dc.mbh))
+        want.append(oracle.search_frame(src, ref, dc.mbw, dc.mbh, 16, 1, True, 27, hints=hints[t - 1]))
+    plan = gpu.Plan(jobs)
+    for wpg, wcu in ((1, 12), (4, 16)):
+        for waves in (0, 6):
+            say("configuration", wpg, wcu, "waves", waves)
+            t = plan.set_config(waves, wpg, wcu, 1)
+            assert (t.waves_per_group, t.waves_per_cu) == (wpg, wcu)
+            out.zero_()
+            plan.launch()
+            plan.check()
+            got = out.cpu().numpy().view(gpu.MB_DTYPE).reshape(nb, dc.mbh, dc.mbw)
+            for k in range(nb):
+                assert_same_field(got[k], want[k], f"config {wpg}/{wcu} waves {waves} frame {k + 1}")
+    with pytest.raises(gpu.PcmeError):
+        plan.set_config(0, 2, 12, 1)
+    best = plan.autotune_ex(reps=1)
+    assert (best.waves_per_group, best.waves_per_cu) in ((1, 12), (4, 16))
