@@ -1127,12 +1127,8 @@ int pcme_b_field(const pcme_b_field_job *job, pcme_b_mb *out, void *stream) {
   pcme_b_out *dr = (pcme_b_out *)((uint8_t *)w->dev + jb);
   for (int d = 0; d < nd; d++) {
     const int o = off[(size_t)d], c = off[(size_t)d + 1] - o;
-    const unsigned tb = (unsigned)((c + 255) / 256);
-    hipLaunchKernelGGL(b_gather_kernel, dim3(tb), dim3(256), 0, wg.s, dj + o, c, out, F.col);
-    hipLaunchKernelGGL(b_analyse_16_kernel, dim3((unsigned)c), dim3(64), 0, wg.s, dj + o, c, dr + o);
-    hipLaunchKernelGGL(b_analyse_part_kernel<0>, dim3((unsigned)c), dim3(64), 0, wg.s, dj + o, c, dr + o);
-    hipLaunchKernelGGL(b_analyse_part_kernel<1>, dim3((unsigned)c), dim3(64), 0, wg.s, dj + o, c, dr + o);
-    hipLaunchKernelGGL(b_commit_kernel, dim3(tb), dim3(256), 0, wg.s, dj + o, c, dr + o, out, F.partitions ? 1 : 0);
+    hipLaunchKernelGGL(b_diag_kernel, dim3((unsigned)c), dim3(128), 0, wg.s, dj + o, c, dr + o, out, F.col,
+                       F.partitions ? 1 : 0);
     HIPCHK(hipGetLastError());
   }
   return PCME_OK;

@@ -2016,25 +2016,32 @@ __device__ __forceinline__ void b_ctx(const pcme_b_job &J, SCtx *c) {
   }
 }
 
+// Orders this wave's LDS writes before its later LDS reads by other lanes
+// (the B analysis runs one MB per wave).
+__device__ __forceinline__ void bwave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // analyse_b's motion search (slice.rs:2189-2377 without BSkip, RD and
 // intra), first half: direct_spatial and the direct prediction's cost, the
-// list-0/list-1 16x16 search_part and the direction costs.  Writes dwords
-// 0-21 of the record.  One MB per wave.
-__global__ void __launch_bounds__(64) b_analyse_16_kernel(const pcme_b_job *__restrict__ jobs, int njobs,
-                                                          pcme_b_out *out) {
-  __shared__ __attribute__((aligned(16))) uint32_t s_lds[LDS_PER_WAVE / 4];
-  __shared__ __attribute__((aligned(16))) uint8_t s_pred[3][256];
-  const int j = blockIdx.x;
-  if (j >= njobs) return;
-  const pcme_b_job &J = jobs[j];
-  const int lane = (int)lane_id();
-  SCtx c[2];
-  b_ctx(J, c);
-  const NbrG n[2] = {nbr_of(J.nb[0]), nbr_of(J.nb[1])};
-  const int px = 16 * J.mx, py = 16 * J.my;
-  const uint32_t lam = J.lam;
-  // direct_spatial (slice.rs:2030-2084)
+// list-0/list-1 16x16 search_part and the direction costs, written as
+// dwords 0-21 of the record.  In pieces, so that the two lists' searches
+// can run on two waves (b_diag_kernel) or one after the other
+// (b_analyse_16).
+
+// direct_spatial's reference indices and MVs per 8x8 and whether every
+// direct MV is legal (slice.rs:2030-2084).  Wave-uniform.
+struct BDirect {
   int refs[2];
+  uint32_t dmv[2][4];
+  bool ok;
+};
+
+__device__ __forceinline__ BDirect b_direct(const pcme_b_job &J, const SCtx *c, const NbrG *n) {
+  const int px = 16 * J.mx, py = 16 * J.my;
+  BDirect D;
+  int *refs = D.refs;
   uint32_t dmvp[2] = {0u, 0u};
 #pragma unroll
   for (int l = 0; l < 2; l++) {
@@ -2051,7 +2058,7 @@ __global__ void __launch_bounds__(64) b_analyse_16_kernel(const pcme_b_job *__re
     for (int l = 0; l < 2; l++)
       if (refs[l] >= 0) dmvp[l] = mvp16_rf(n[l], refs[l]);
   }
-  uint32_t dmv[2][4];
+  uint32_t (*dmv)[4] = D.dmv;
   bool direct_ok = true;
 #pragma unroll
   for (int q = 0; q < 4; q++) {
@@ -2070,6 +2077,20 @@ __global__ void __launch_bounds__(64) b_analyse_16_kernel(const pcme_b_job *__re
       }
     }
   }
+  D.ok = direct_ok;
+  return D;
+}
+
+// The direct prediction's cost (SATD + (2 lam) >> 8), all ones when a
+// direct MV is illegal.  One wave; s_pred[2] holds the prediction after.
+__device__ __forceinline__ uint32_t b_direct_cost(const pcme_b_job &J, const SCtx *c, const BDirect &D,
+                                                  uint8_t (*s_pred)[256]) {
+  const int lane = (int)lane_id();
+  const int px = 16 * J.mx, py = 16 * J.my;
+  const uint32_t lam = J.lam;
+  const int *refs = D.refs;
+  const uint32_t (*dmv)[4] = D.dmv;
+  const bool direct_ok = D.ok;
   uint32_t direct_cost = 0xFFFFFFFFu;
   if (direct_ok) {
     for (int i = lane; i < 256; i += 64) {
@@ -2089,23 +2110,40 @@ __global__ void __launch_bounds__(64) b_analyse_16_kernel(const pcme_b_job *__re
         v = mc_px(c[1], px + x, py + y, m1);
       s_pred[2][i] = (uint8_t)v;
     }
-    __syncthreads();
+    bwave_sync();
     direct_cost = satd_rect<16, 16>(J.src, J.sstride, s_pred[2], 0, 0) + ((lam * 2u) >> 8);
-    __syncthreads();
+    bwave_sync();
   }
-  // 16x16 per list, seeded with the direct MVs (slice.rs:2232-2244)
-  PartRes r16[2];
-  // each list's search starts with nothing staged (the two lists read
-  // different pictures)
-  Area ar0[2] = {no_area(), no_area()}, ar1[2] = {no_area(), no_area()};
-  search_part<0, false>(c[0], n[0], J.src, J.sstride, px, py, dmv[0][0], 0u, -1, r16[0], s_lds, ar0);
-  search_part<0, false>(c[1], n[1], J.src, J.sstride, px, py, dmv[1][0], 0u, -1, r16[1], s_lds, ar1);
-  __syncthreads();
+  return direct_cost;
+}
+
+// One list's 16x16 search_part, seeded with its direct MV
+// (slice.rs:2232-2244); each list's search starts with nothing staged (the
+// two lists read different pictures).
+__device__ __forceinline__ void b_search16(const pcme_b_job &J, const SCtx &c, const NbrG &n, uint32_t seed,
+                                           uint32_t *s_lds, PartRes &r) {
+  Area ar[2] = {no_area(), no_area()};
+  search_part<0, false>(c, n, J.src, J.sstride, 16 * J.mx, 16 * J.my, seed, 0u, -1, r, s_lds, ar);
+}
+
+// The bi prediction's cost, the direction and dwords 0-21 of the record
+// from both lists' 16x16 results (mv[0], mvp[0], total); mv16 = the two
+// lists' MVs.  One wave.
+__device__ __forceinline__ void b_finish16(const pcme_b_job &J, const SCtx *c, const BDirect &D, uint32_t direct_cost,
+                                           const PartRes *r16, uint8_t (*s_pred)[256], pcme_b_out *rec,
+                                           uint32_t mv16[2]) {
+  const int lane = (int)lane_id();
+  const int px = 16 * J.mx, py = 16 * J.my;
+  const uint32_t lam = J.lam;
+  const int *refs = D.refs;
+  const uint32_t (*dmv)[4] = D.dmv;
+  const bool direct_ok = D.ok;
+  bwave_sync();
   mc_rect(c[0], px, py, 0, 0, 16, 16, r16[0].mv[0], s_pred[0]);
   mc_rect(c[1], px, py, 0, 0, 16, 16, r16[1].mv[0], s_pred[1]);
-  __syncthreads();
+  bwave_sync();
   avg_buf(s_pred[2], s_pred[0], s_pred[1]);
-  __syncthreads();
+  bwave_sync();
   uint32_t bits = 0;
 #pragma unroll
   for (int l = 0; l < 2; l++)
@@ -2142,29 +2180,48 @@ __global__ void __launch_bounds__(64) b_analyse_16_kernel(const pcme_b_job *__re
   } else if (lane == 21) {
     w = dir;
   }
-  if (lane < 22 || lane == 33) ((uint32_t *)&out[j])[lane] = w;  // dword 33: reserved, zero
+  if (lane < 22 || lane == 33) ((uint32_t *)rec)[lane] = w;  // dword 33: reserved, zero
+  mv16[0] = r16[0].mv[0];
+  mv16[1] = r16[1].mv[0];
 }
 
-// Second half: b_partition (slice.rs:2086-2187) of 16x8 (K = 0) or 8x16
-// (K = 1), seeded with the 16x16 MVs the first half wrote (dwords 12-13 of
-// the record).  Writes part_mv[K], part_cost[K] and part_dir[K].  One MB
-// per wave.
-template <int K>
-__global__ void __launch_bounds__(64) b_analyse_part_kernel(const pcme_b_job *__restrict__ jobs, int njobs,
-                                                            pcme_b_out *out) {
+// The first half on one wave.
+__device__ __forceinline__ void b_analyse_16(const pcme_b_job &J, pcme_b_out *rec, uint32_t *s_lds,
+                                             uint8_t (*s_pred)[256], uint32_t mv16[2]) {
+  SCtx c[2];
+  b_ctx(J, c);
+  const NbrG n[2] = {nbr_of(J.nb[0]), nbr_of(J.nb[1])};
+  const BDirect D = b_direct(J, c, n);
+  const uint32_t direct_cost = b_direct_cost(J, c, D, s_pred);
+  PartRes r16[2];
+  b_search16(J, c[0], n[0], D.dmv[0][0], s_lds, r16[0]);
+  b_search16(J, c[1], n[1], D.dmv[1][0], s_lds, r16[1]);
+  b_finish16(J, c, D, direct_cost, r16, s_pred, rec, mv16);
+}
+
+__global__ void __launch_bounds__(64) b_analyse_16_kernel(const pcme_b_job *__restrict__ jobs, int njobs,
+                                                          pcme_b_out *out) {
   __shared__ __attribute__((aligned(16))) uint32_t s_lds[LDS_PER_WAVE / 4];
   __shared__ __attribute__((aligned(16))) uint8_t s_pred[3][256];
   const int j = blockIdx.x;
   if (j >= njobs) return;
-  const pcme_b_job &J = jobs[j];
+  uint32_t mv16[2];
+  b_analyse_16(jobs[j], &out[j], s_lds, s_pred, mv16);
+}
+
+// Second half: b_partition (slice.rs:2086-2187) of 16x8 (K = 0) or 8x16
+// (K = 1), seeded with the two lists' 16x16 MVs mv16 (packed; dwords 12-13
+// of the record).  Writes part_mv[K], part_cost[K] and part_dir[K] of *rec.
+// One wave, LDS as b_analyse_16.
+template <int K>
+__device__ __forceinline__ void b_analyse_part(const pcme_b_job &J, pcme_b_out *rec, uint32_t *s_lds,
+                                               uint8_t (*s_pred)[256], const uint32_t mv16[2]) {
   const int lane = (int)lane_id();
   SCtx c[2];
   b_ctx(J, c);
   const NbrG n[2] = {nbr_of(J.nb[0]), nbr_of(J.nb[1])};
   const int px = 16 * J.mx, py = 16 * J.my;
   const uint32_t lam = J.lam;
-  const uint32_t *rec = (const uint32_t *)&out[j];
-  const uint32_t mv16[2] = {rdlane(rec[12 + (lane & 1)], 0), rdlane(rec[12 + (lane & 1)], 1)};
   uint32_t pmv[2][2];
   PartRes rp[2];
   // both lists' searches in two paired steps (partition 0, then 1)
@@ -2175,7 +2232,7 @@ __global__ void __launch_bounds__(64) b_analyse_part_kernel(const pcme_b_job *__
     pmv[l][0] = rp[l].mv[0];
     pmv[l][1] = rp[l].mv[1];
   }
-  __syncthreads();
+  bwave_sync();
 #pragma unroll
   for (int l = 0; l < 2; l++)
 #pragma unroll
@@ -2185,9 +2242,9 @@ __global__ void __launch_bounds__(64) b_analyse_part_kernel(const pcme_b_job *__
       else
         mc_rect(c[l], px, py, 8 * pi, 0, 8, 16, rp[l].mv[pi], s_pred[l]);
     }
-  __syncthreads();
+  bwave_sync();
   avg_buf(s_pred[2], s_pred[0], s_pred[1]);
-  __syncthreads();
+  bwave_sync();
   uint32_t tot = (lam * 8u) >> 8, pdir[2];
 #pragma unroll
   for (int pi = 0; pi < 2; pi++) {
@@ -2205,12 +2262,26 @@ __global__ void __launch_bounds__(64) b_analyse_part_kernel(const pcme_b_job *__
     tot += best;
     pdir[pi] = bd;
   }
-  uint32_t *o = (uint32_t *)&out[j];
+  uint32_t *o = (uint32_t *)rec;
   // part_mv[K][l][pi] = dword 22 + 4K + 2l + pi
   const uint32_t v = lane == 0 ? pmv[0][0] : (lane == 1 ? pmv[0][1] : (lane == 2 ? pmv[1][0] : pmv[1][1]));
   if (lane < 4) o[22 + 4 * K + lane] = v;
   if (lane == 4) o[30 + K] = tot;
   if (lane == 5 || lane == 6) ((uint8_t *)o)[128 + 2 * K + (lane - 5)] = (uint8_t)pdir[lane - 5];
+}
+
+// The second half over independent MBs whose first half an earlier launch
+// wrote (the 16x16 MVs read from dwords 12-13 of the record).
+template <int K>
+__global__ void __launch_bounds__(64) b_analyse_part_kernel(const pcme_b_job *__restrict__ jobs, int njobs,
+                                                            pcme_b_out *out) {
+  __shared__ __attribute__((aligned(16))) uint32_t s_lds[LDS_PER_WAVE / 4];
+  __shared__ __attribute__((aligned(16))) uint8_t s_pred[3][256];
+  const int j = blockIdx.x;
+  if (j >= njobs) return;
+  const uint32_t *rec = (const uint32_t *)&out[j];
+  const uint32_t mv16[2] = {rdlane(rec[12], 0), rdlane(rec[13], 0)};
+  b_analyse_part<K>(jobs[j], &out[j], s_lds, s_pred, mv16);
 }
 
 template __global__ void b_analyse_part_kernel<0>(const pcme_b_job *__restrict__, int, pcme_b_out *);
@@ -2223,16 +2294,12 @@ static_assert(sizeof(pcme_b_col) == 44 && sizeof(pcme_b_mb) == 280, "pcme_b_col 
 // The 8x8 (raster) holding 4x4 block r.
 __device__ __forceinline__ int blk8(int r) { return ((r >> 3) << 1) | ((r & 3) >> 1); }
 
-// One diagonal of the B field, before its search: each job's neighbour
-// blocks (both lists: left blocks 3, 7, 11, 15, top 12..15, top-right 12,
-// top-left 15; -2 outside the picture) read from the records the earlier
-// diagonals committed, and its co-located MB.  One thread per job.
-__global__ void __launch_bounds__(256) b_gather_kernel(pcme_b_job *__restrict__ jobs, int n,
-                                                       const pcme_b_mb *__restrict__ field,
-                                                       const pcme_b_col *__restrict__ col) {
-  const int k = (int)(blockIdx.x * 256 + threadIdx.x);
-  if (k >= n) return;
-  pcme_b_job &J = jobs[k];
+// Before an MB's search: its neighbour blocks (both lists: left blocks 3,
+// 7, 11, 15, top 12..15, top-right 12, top-left 15; -2 outside the picture)
+// read from the records the earlier diagonals committed, and its co-located
+// MB, into its job.  One thread.
+__device__ __forceinline__ void b_gather(pcme_b_job &J, const pcme_b_mb *__restrict__ field,
+                                         const pcme_b_col *__restrict__ col) {
   const int mx = J.mx, my = J.my, mbw = J.mbw;
   // (MB offset, 4x4 block) of the ten neighbour blocks
   const int ndx[10] = {-1, -1, -1, -1, 0, 0, 0, 0, 1, -1};
@@ -2264,17 +2331,12 @@ __global__ void __launch_bounds__(256) b_gather_kernel(pcme_b_job *__restrict__ 
   J.col_intra = c.intra;
 }
 
-// One diagonal of the B field, after its search: analyse_b's decision with
-// rd off and without BSkip and intra (slice.rs:2189-2377) and the commit of
-// cand_inter_info (slice.rs:1751-1785) into the MB's record.  One thread
-// per job.
-__global__ void __launch_bounds__(256) b_commit_kernel(const pcme_b_job *__restrict__ jobs, int n,
-                                                       const pcme_b_out *__restrict__ res,
-                                                       pcme_b_mb *__restrict__ field, int partitions) {
-  const int k = (int)(blockIdx.x * 256 + threadIdx.x);
-  if (k >= n) return;
-  const pcme_b_out &o = res[k];
-  pcme_b_mb &m = field[(size_t)jobs[k].my * jobs[k].mbw + jobs[k].mx];
+// After an MB's search: analyse_b's decision with rd off and without BSkip
+// and intra (slice.rs:2189-2377) and the commit of cand_inter_info
+// (slice.rs:1751-1785) into the MB's record.  One thread.
+__device__ __forceinline__ void b_commit(const pcme_b_job &J, const pcme_b_out &o, pcme_b_mb *__restrict__ field,
+                                         int partitions) {
+  pcme_b_mb &m = field[(size_t)J.my * J.mbw + J.mx];
   uint32_t best = 0u;
   int mode = -1;
   if (o.direct_ok) {
@@ -2318,6 +2380,75 @@ __global__ void __launch_bounds__(256) b_commit_kernel(const pcme_b_job *__restr
       m.mv[l][r] = mv;
       m.ref[l][q] = (int8_t)rf;
     }
+}
+
+// One anti-diagonal of the B field in one launch: a workgroup of two waves
+// per MB copies its job into LDS and gathers its neighbours from the
+// records earlier diagonals committed; wave 0 prices direct and searches
+// list 0's 16x16 while wave 1 searches list 1's, wave 0 completes the first
+// half, both waves then run the partition searches (16x8 on wave 0, 8x16
+// on wave 1), and the decision is committed.  The MBs of a diagonal depend
+// only on earlier diagonals (earlier launches).  The job and the record
+// the waves build stay in LDS: written and read within the launch, they
+// never pass through the scalar cache.
+__global__ void __launch_bounds__(128) b_diag_kernel(const pcme_b_job *__restrict__ jobs, int n, pcme_b_out *res,
+                                                     pcme_b_mb *__restrict__ field,
+                                                     const pcme_b_col *__restrict__ col, int partitions) {
+  __shared__ __attribute__((aligned(16))) uint32_t s_lds[2][LDS_PER_WAVE / 4];
+  __shared__ __attribute__((aligned(16))) uint8_t s_pred[2][3][256];
+  __shared__ __attribute__((aligned(16))) pcme_b_job s_job;
+  __shared__ __attribute__((aligned(16))) pcme_b_out s_rec;
+  __shared__ uint32_t s_mv16[3];
+  static_assert(sizeof(pcme_b_job) % 4 == 0 && sizeof(pcme_b_job) / 4 <= 128 && sizeof(pcme_b_out) / 4 <= 128,
+                "one dword per thread");
+  const int k = blockIdx.x;
+  if (k >= n) return;
+  const int t = (int)threadIdx.x, w = t >> 6;
+  if (t < (int)(sizeof(pcme_b_job) / 4)) ((uint32_t *)&s_job)[t] = ((const uint32_t *)&jobs[k])[t];
+  if (t < (int)(sizeof(pcme_b_out) / 4)) ((uint32_t *)&s_rec)[t] = 0u;
+  __syncthreads();
+  if (t == 0) b_gather(s_job, field, col);
+  __syncthreads();  // the job's neighbours
+  const pcme_b_job &J = s_job;
+  SCtx c[2];
+  b_ctx(J, c);
+  const NbrG nb[2] = {nbr_of(J.nb[0]), nbr_of(J.nb[1])};
+  const BDirect D = b_direct(J, c, nb);
+  // the two lists' 16x16 searches side by side (list 1 on wave 1)
+  PartRes r16[2];
+  uint32_t direct_cost = 0u;
+  if (w == 0) {
+    direct_cost = b_direct_cost(J, c, D, s_pred[0]);
+    b_search16(J, c[0], nb[0], D.dmv[0][0], s_lds[0], r16[0]);
+  } else {
+    b_search16(J, c[1], nb[1], D.dmv[1][0], s_lds[1], r16[1]);
+    if (lane_id() == 0) {
+      s_mv16[0] = r16[1].mv[0];
+      s_mv16[1] = r16[1].mvp[0];
+      s_mv16[2] = r16[1].total;
+    }
+  }
+  __syncthreads();  // list 1's 16x16 result
+  if (w == 0) {
+    r16[1].mv[0] = s_mv16[0];
+    r16[1].mvp[0] = s_mv16[1];
+    r16[1].total = s_mv16[2];
+    uint32_t mv16[2];
+    b_finish16(J, c, D, direct_cost, r16, s_pred[0], &s_rec, mv16);
+    if (lane_id() == 0) {
+      s_mv16[0] = mv16[0];
+      s_mv16[1] = mv16[1];
+    }
+  }
+  __syncthreads();  // the first half's record and MVs
+  const uint32_t mv16[2] = {s_mv16[0], s_mv16[1]};
+  if (w == 0)
+    b_analyse_part<0>(J, &s_rec, s_lds[0], s_pred[0], mv16);
+  else
+    b_analyse_part<1>(J, &s_rec, s_lds[1], s_pred[1], mv16);
+  __syncthreads();  // the whole record
+  if (t < (int)(sizeof(pcme_b_out) / 4)) ((uint32_t *)&res[k])[t] = ((const uint32_t *)&s_rec)[t];
+  if (t == 0) b_commit(J, s_rec, field, partitions);
 }
 
 // -------------------------------------------------- per-call search batch
