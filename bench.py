@@ -683,7 +683,7 @@ def b_field_leg(args, pcme, torch, dc, field2, qp, me_range, stream):
     ms = a.elapsed_time(b) / reps
     got = dout.cpu().numpy().view(pcme.B_MB_DTYPE).reshape(mbh, mbw)
     diag = mbw + 2 * (mbh - 1)
-    o = {"workload": f"ME-only field of the B picture, {n} MBs in {diag} anti-diagonals (one launch each), "
+    o = {"workload": f"ME-only field of the B picture, {n} MBs, one launch of row walkers (critical path {diag} MBs), "
                      "co-located MBs from the P field of frame 2",
          "ms_per_picture": round(ms, 3), "mblocks_per_s": round(n / (ms * 1e-3) / 1e6, 4),
          "modes": [int(x) for x in np.bincount(got["mode"].reshape(-1), minlength=4)]}

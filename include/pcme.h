@@ -464,11 +464,12 @@ typedef struct pcme_b_mb {
 } pcme_b_mb;
 
 /* The whole field of one B picture in raster dependency order, on the
- * device: its anti-diagonals (mx + 2 my constant) one after another, each
- * diagonal's MBs in parallel (a job-building kernel reads the committed
- * neighbours, the B search kernels, a commit kernel).  out: mbw*mbh
- * records (device), raster order.  No counterpart in the reference, whose
- * encoder walks the picture on the CPU. */
+ * device, in one launch of persistent row walkers (two waves per MB row;
+ * an MB waits for the row above's MB x+1, so the rows advance as a
+ * wavefront).  out: mbw*mbh records (device), raster order.  No
+ * counterpart in the reference, whose encoder walks the picture on the
+ * CPU.  A wait that exceeds the library's spin limit (which the wavefront
+ * never reaches) ends the launch with the records incomplete. */
 int pcme_b_field(const pcme_b_field_job *job, pcme_b_mb *out, void *stream);
 
 /* ---- early P-skip gate (analyse_p, slice.rs:1884-1920) ----------------

@@ -1083,19 +1083,14 @@ int pcme_b_field(const pcme_b_field_job *job, pcme_b_mb *out, void *stream) {
   Dev *dv;
   int rc = get_dev(&dv);
   if (rc) return rc;
-  const int mbw = F.mbw, mbh = F.mbh, n = mbw * mbh, nd = mbw + 2 * (mbh - 1);
+  const int mbw = F.mbw, mbh = F.mbh, n = mbw * mbh;
   const int stride = 16 * mbw + 2 * PCME_PAD;
   const uint32_t lam = lambda_of(F.qp);
-  // the job table in diagonal order: diagonal d = mx + 2 my, rows ascending
+  // the job table in raster order (row walkers, one workgroup per row)
   std::vector<pcme_b_job> jobs((size_t)n);
-  std::vector<int> off((size_t)nd + 1, 0);
-  int k = 0;
-  for (int d = 0; d < nd; d++) {
-    off[(size_t)d] = k;
-    for (int my = 0; my < mbh; my++) {
-      const int mx = d - 2 * my;
-      if (mx < 0 || mx >= mbw) continue;
-      pcme_b_job &J = jobs[(size_t)k++];
+  for (int my = 0; my < mbh; my++)
+    for (int mx = 0; mx < mbw; mx++) {
+      pcme_b_job &J = jobs[(size_t)my * mbw + mx];
       memset(&J, 0, sizeof J);
       J.src = F.src + (size_t)(PCME_PAD + 16 * my) * stride + PCME_PAD + 16 * mx;
       J.sstride = stride;
@@ -1113,24 +1108,24 @@ int pcme_b_field(const pcme_b_field_job *job, pcme_b_mb *out, void *stream) {
       const int need = 16 * (my + 1) + 96;  // frame.rs:152 with picture.rs:261-265
       J.vlim = need >= 16 * mbh ? 16 * mbh + 44 : need;
     }
-  }
-  off[(size_t)nd] = k;
+  // workspace: jobs | records | exchange words (8 per MB) | 2 control words
   const size_t jb = sizeof(pcme_b_job) * (size_t)n, rb = sizeof(pcme_b_out) * (size_t)n;
+  const size_t xwords = (size_t)n * 8, xb = xwords * 4;
   std::lock_guard<std::mutex> g(dv->mu);
   WsGuard wg;
   wg.s = (hipStream_t)stream;
   CallWs *w = nullptr;
-  if ((rc = take_ws(dv, wg.s, jb + rb, jb, &w))) return rc;
+  if ((rc = take_ws(dv, wg.s, jb + rb + xb + 64, jb, &w))) return rc;
   wg.w = w;
   if ((rc = ws_upload(w, jobs.data(), jb, wg.s))) return rc;
   pcme_b_job *dj = (pcme_b_job *)w->dev;
   pcme_b_out *dr = (pcme_b_out *)((uint8_t *)w->dev + jb);
-  for (int d = 0; d < nd; d++) {
-    const int o = off[(size_t)d], c = off[(size_t)d + 1] - o;
-    hipLaunchKernelGGL(b_diag_kernel, dim3((unsigned)c), dim3(128), 0, wg.s, dj + o, c, dr + o, out, F.col,
-                       F.partitions ? 1 : 0);
-    HIPCHK(hipGetLastError());
-  }
+  uint32_t *xw = (uint32_t *)((uint8_t *)w->dev + jb + rb);
+  uint32_t *ctl = (uint32_t *)((uint8_t *)w->dev + jb + rb + xb);
+  hipLaunchKernelGGL(b_walk_clear_kernel, dim3(64), dim3(256), 0, wg.s, xw, xwords, ctl);
+  hipLaunchKernelGGL(b_walk_kernel, dim3((unsigned)std::min(mbh, 1024)), dim3(128), 0, wg.s, dj, mbw, mbh, dr, out,
+                     F.col, xw, ctl, F.partitions ? 1 : 0, kSpinLimit);
+  HIPCHK(hipGetLastError());
   return PCME_OK;
 }
 

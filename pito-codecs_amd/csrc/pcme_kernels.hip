@@ -2334,9 +2334,7 @@ __device__ __forceinline__ void b_gather(pcme_b_job &J, const pcme_b_mb *__restr
 // After an MB's search: analyse_b's decision with rd off and without BSkip
 // and intra (slice.rs:2189-2377) and the commit of cand_inter_info
 // (slice.rs:1751-1785) into the MB's record.  One thread.
-__device__ __forceinline__ void b_commit(const pcme_b_job &J, const pcme_b_out &o, pcme_b_mb *__restrict__ field,
-                                         int partitions) {
-  pcme_b_mb &m = field[(size_t)J.my * J.mbw + J.mx];
+__device__ __forceinline__ void b_commit(const pcme_b_out &o, pcme_b_mb &m, int partitions) {
   uint32_t best = 0u;
   int mode = -1;
   if (o.direct_ok) {
@@ -2382,73 +2380,186 @@ __device__ __forceinline__ void b_commit(const pcme_b_job &J, const pcme_b_out &
     }
 }
 
-// One anti-diagonal of the B field in one launch: a workgroup of two waves
-// per MB copies its job into LDS and gathers its neighbours from the
-// records earlier diagonals committed; wave 0 prices direct and searches
-// list 0's 16x16 while wave 1 searches list 1's, wave 0 completes the first
-// half, both waves then run the partition searches (16x8 on wave 0, 8x16
-// on wave 1), and the decision is committed.  The MBs of a diagonal depend
-// only on earlier diagonals (earlier launches).  The job and the record
-// the waves build stay in LDS: written and read within the launch, they
-// never pass through the scalar cache.
-__global__ void __launch_bounds__(128) b_diag_kernel(const pcme_b_job *__restrict__ jobs, int n, pcme_b_out *res,
-                                                     pcme_b_mb *__restrict__ field,
-                                                     const pcme_b_col *__restrict__ col, int partitions) {
+// The B field in one launch: persistent row walkers (a workgroup of two
+// waves walks one MB row left to right; rows are taken in order from a
+// ticket counter).  MB (x, y) waits for the row above's MB x+1 (x for the
+// last column), publishes its bottom blocks 12-15 of both lists as eight
+// exchange words (the P walker's format: tag, list used, MV) for the row
+// below, and keeps its right column (blocks 3, 7, 11, 15) in LDS for the
+// next MB of its row.  Per MB: its job is copied into LDS and its
+// neighbour blocks filled from those two sources and its co-located MB;
+// wave 0 prices direct and searches list 0's 16x16 while wave 1 searches
+// list 1's; wave 0 prices bi and fills the first half of the record; the
+// two waves run the 16x8 and 8x16 b_partition searches; one thread
+// commits.  The job and record stay in LDS.  ctl[0]: row tickets, ctl[1]:
+// stall flag (a wait past spin_limit polls ends the launch).
+__device__ __forceinline__ uint32_t bword(uint32_t tag, int ref, pcme_mv m) {
+  return xword(tag, ref >= 0 ? 1u : 0u, pack_mv(m.x, m.y));
+}
+
+// Exchange word q once it carries `tag`; raises *stall past spin_limit polls.
+__device__ __forceinline__ uint32_t bwait(const uint32_t *q, uint32_t tag, uint32_t spin_limit, int *stall,
+                                          uint32_t *ctl) {
+  uint32_t v = 0, spins = 0;
+  for (;;) {
+    v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (XTAG(v) == tag) return v;
+    if (++spins > spin_limit) {
+      *stall = 1;
+      __hip_atomic_store(ctl + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return v;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+}
+
+__global__ void __launch_bounds__(128) b_walk_kernel(const pcme_b_job *__restrict__ jobs, int mbw, int mbh,
+                                                     pcme_b_out *res, pcme_b_mb *__restrict__ field,
+                                                     const pcme_b_col *__restrict__ col, uint32_t *xw,
+                                                     uint32_t *ctl, int partitions, uint32_t spin_limit) {
   __shared__ __attribute__((aligned(16))) uint32_t s_lds[2][LDS_PER_WAVE / 4];
   __shared__ __attribute__((aligned(16))) uint8_t s_pred[2][3][256];
   __shared__ __attribute__((aligned(16))) pcme_b_job s_job;
   __shared__ __attribute__((aligned(16))) pcme_b_out s_rec;
+  __shared__ __attribute__((aligned(16))) pcme_b_mb s_mb;
   __shared__ uint32_t s_mv16[3];
-  static_assert(sizeof(pcme_b_job) % 4 == 0 && sizeof(pcme_b_job) / 4 <= 128 && sizeof(pcme_b_out) / 4 <= 128,
+  __shared__ uint32_t s_above[3][8];  // exchange words of the row above: MB x-1, x, x+1
+  __shared__ pcme_nb_block s_left[2][4];  // this row's previous MB: blocks 3, 7, 11, 15 per list
+  __shared__ int s_ctl[2];            // the row taken; a stall
+  static_assert(sizeof(pcme_b_job) % 4 == 0 && sizeof(pcme_b_job) / 4 <= 128 && sizeof(pcme_b_out) / 4 <= 128 &&
+                    sizeof(pcme_b_mb) % 4 == 0 && sizeof(pcme_b_mb) / 4 <= 128,
                 "one dword per thread");
-  const int k = blockIdx.x;
-  if (k >= n) return;
+  const uint32_t tag = 1u;
   const int t = (int)threadIdx.x, w = t >> 6;
-  if (t < (int)(sizeof(pcme_b_job) / 4)) ((uint32_t *)&s_job)[t] = ((const uint32_t *)&jobs[k])[t];
-  if (t < (int)(sizeof(pcme_b_out) / 4)) ((uint32_t *)&s_rec)[t] = 0u;
-  __syncthreads();
-  if (t == 0) b_gather(s_job, field, col);
-  __syncthreads();  // the job's neighbours
-  const pcme_b_job &J = s_job;
-  SCtx c[2];
-  b_ctx(J, c);
-  const NbrG nb[2] = {nbr_of(J.nb[0]), nbr_of(J.nb[1])};
-  const BDirect D = b_direct(J, c, nb);
-  // the two lists' 16x16 searches side by side (list 1 on wave 1)
-  PartRes r16[2];
-  uint32_t direct_cost = 0u;
-  if (w == 0) {
-    direct_cost = b_direct_cost(J, c, D, s_pred[0]);
-    b_search16(J, c[0], nb[0], D.dmv[0][0], s_lds[0], r16[0]);
-  } else {
-    b_search16(J, c[1], nb[1], D.dmv[1][0], s_lds[1], r16[1]);
-    if (lane_id() == 0) {
-      s_mv16[0] = r16[1].mv[0];
-      s_mv16[1] = r16[1].mvp[0];
-      s_mv16[2] = r16[1].total;
+  for (;;) {
+    if (t == 0) {
+      s_ctl[0] = (int)atomicAdd(ctl, 1u);
+      s_ctl[1] = 0;
+    }
+    __syncthreads();
+    const int my = s_ctl[0];
+    if (my >= mbh) return;
+    for (int mx = 0; mx < mbw; mx++) {
+      const int k = my * mbw + mx;
+      // wait for the row above's MB x+1 (x in the last column); its words
+      // and those of MB x (and x-1) read earlier give the top neighbours
+      if (t < 8 && my > 0) {
+        const int dx = mx + 1 < mbw ? 1 : 0;
+        if (mx == 0) {
+          s_above[1][t] = bwait(xw + (size_t)(k - mbw) * 8 + t, tag, spin_limit, &s_ctl[1], ctl);  // MB x
+        } else {
+          s_above[0][t] = s_above[1][t];
+          s_above[1][t] = s_above[2][t];
+        }
+        s_above[2][t] = bwait(xw + (size_t)(k - mbw + dx) * 8 + t, tag, spin_limit, &s_ctl[1], ctl);
+      }
+      if (t < (int)(sizeof(pcme_b_job) / 4)) ((uint32_t *)&s_job)[t] = ((const uint32_t *)&jobs[k])[t];
+      if (t < (int)(sizeof(pcme_b_out) / 4)) ((uint32_t *)&s_rec)[t] = 0u;
+      __syncthreads();
+      if (s_ctl[1]) return;  // a stall: the launch ends (every wave reads the same flag)
+      if (t < 20) {
+        // neighbour block i of list l: left[4] (blocks 3, 7, 11, 15 of MB
+        // x-1), top[4] (12..15 of the MB above), top-right (12 of x+1),
+        // top-left (15 of x-1); -2 outside the picture
+        const int l = t / 10, i = t % 10;
+        pcme_nb_block b;
+        b.reserved[0] = b.reserved[1] = b.reserved[2] = 0;
+        b.mv.x = b.mv.y = 0;
+        b.ref = -2;
+        if (i < 4) {
+          if (mx > 0) b = s_left[l][i];
+        } else if (my > 0) {
+          const bool tr = i == 8, tl = i == 9;
+          const bool av = tr ? mx + 1 < mbw : (tl ? mx > 0 : true);
+          if (av) {
+            const int slot = tr ? 2 : (tl ? 0 : 1);
+            const int blk = tr ? 0 : (tl ? 3 : i - 4);
+            const uint32_t v = s_above[slot][4 * l + blk];
+            b.ref = (int8_t)(xword_ref(v) ? 0 : -1);
+            const uint32_t m = b.ref >= 0 ? xword_mv(v) : 0u;
+            b.mv.x = (int16_t)mv_x(m);
+            b.mv.y = (int16_t)mv_y(m);
+          }
+        }
+        s_job.nb[l][i] = b;
+      } else if (t >= 32 && t < 40) {
+        const int l = (t - 32) >> 2, q = (t - 32) & 3;
+        const pcme_b_col &c = col[k];
+        s_job.col_mv[l][q] = c.mv[l][q];
+        s_job.col_ref[l][q] = c.ref[l][q];
+      } else if (t == 40) {
+        s_job.col_intra = col[k].intra;
+      }
+      __syncthreads();  // the job's neighbours
+      const pcme_b_job &J = s_job;
+      SCtx c[2];
+      b_ctx(J, c);
+      const NbrG nb[2] = {nbr_of(J.nb[0]), nbr_of(J.nb[1])};
+      const BDirect D = b_direct(J, c, nb);
+      PartRes r16[2];
+      uint32_t direct_cost = 0u;
+      if (w == 0) {
+        direct_cost = b_direct_cost(J, c, D, s_pred[0]);
+        b_search16(J, c[0], nb[0], D.dmv[0][0], s_lds[0], r16[0]);
+      } else {
+        b_search16(J, c[1], nb[1], D.dmv[1][0], s_lds[1], r16[1]);
+        if (lane_id() == 0) {
+          s_mv16[0] = r16[1].mv[0];
+          s_mv16[1] = r16[1].mvp[0];
+          s_mv16[2] = r16[1].total;
+        }
+      }
+      __syncthreads();  // list 1's 16x16 result
+      if (w == 0) {
+        r16[1].mv[0] = s_mv16[0];
+        r16[1].mvp[0] = s_mv16[1];
+        r16[1].total = s_mv16[2];
+        uint32_t mv16[2];
+        b_finish16(J, c, D, direct_cost, r16, s_pred[0], &s_rec, mv16);
+        if (lane_id() == 0) {
+          s_mv16[0] = mv16[0];
+          s_mv16[1] = mv16[1];
+        }
+      }
+      __syncthreads();  // the first half's record and MVs
+      const uint32_t mv16[2] = {s_mv16[0], s_mv16[1]};
+      if (w == 0)
+        b_analyse_part<0>(J, &s_rec, s_lds[0], s_pred[0], mv16);
+      else
+        b_analyse_part<1>(J, &s_rec, s_lds[1], s_pred[1], mv16);
+      __syncthreads();  // the whole record
+      if (t < (int)(sizeof(pcme_b_out) / 4)) ((uint32_t *)&res[k])[t] = ((const uint32_t *)&s_rec)[t];
+      if (t == 0) b_commit(s_rec, s_mb, partitions);
+      __syncthreads();  // the committed record
+      if (t < (int)(sizeof(pcme_b_mb) / 4)) ((uint32_t *)&field[k])[t] = ((const uint32_t *)&s_mb)[t];
+      if (t < 16) {
+        // lanes 0-7: the bottom blocks 12-15 of both lists for the row below;
+        // lanes 8-15: the right column 3, 7, 11, 15 for the next MB
+        const int l = (t & 7) >> 2, i = t & 3;
+        const pcme_b_mb &m = s_mb;
+        const int blk = t < 8 ? 12 + i : 4 * i + 3;
+        const pcme_mv mv = m.mv[l][blk];
+        const int rf = m.ref[l][blk8(blk)];
+        if (t < 8) {
+          __hip_atomic_store(xw + (size_t)k * 8 + t, bword(tag, rf, mv), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          pcme_nb_block b;
+          b.reserved[0] = b.reserved[1] = b.reserved[2] = 0;
+          b.ref = (int8_t)rf;
+          b.mv = rf >= 0 ? mv : pcme_mv{0, 0};
+          s_left[l][i] = b;
+        }
+      }
+      __syncthreads();  // s_left, s_above and s_job are rewritten for the next MB
     }
   }
-  __syncthreads();  // list 1's 16x16 result
-  if (w == 0) {
-    r16[1].mv[0] = s_mv16[0];
-    r16[1].mvp[0] = s_mv16[1];
-    r16[1].total = s_mv16[2];
-    uint32_t mv16[2];
-    b_finish16(J, c, D, direct_cost, r16, s_pred[0], &s_rec, mv16);
-    if (lane_id() == 0) {
-      s_mv16[0] = mv16[0];
-      s_mv16[1] = mv16[1];
-    }
-  }
-  __syncthreads();  // the first half's record and MVs
-  const uint32_t mv16[2] = {s_mv16[0], s_mv16[1]};
-  if (w == 0)
-    b_analyse_part<0>(J, &s_rec, s_lds[0], s_pred[0], mv16);
-  else
-    b_analyse_part<1>(J, &s_rec, s_lds[1], s_pred[1], mv16);
-  __syncthreads();  // the whole record
-  if (t < (int)(sizeof(pcme_b_out) / 4)) ((uint32_t *)&res[k])[t] = ((const uint32_t *)&s_rec)[t];
-  if (t == 0) b_commit(J, s_rec, field, partitions);
+}
+
+// Clears the walker's exchange words and control words before a launch.
+__global__ void __launch_bounds__(256) b_walk_clear_kernel(uint32_t *xw, size_t nwords, uint32_t *ctl) {
+  const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i0 < 2) ctl[i0] = 0u;
+  for (size_t i = i0; i < nwords; i += (size_t)gridDim.x * blockDim.x) xw[i] = 0u;
 }
 
 // -------------------------------------------------- per-call search batch
