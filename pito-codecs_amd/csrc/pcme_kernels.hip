@@ -2027,7 +2027,7 @@ __device__ __forceinline__ void bwave_sync() {
 // intra), first half: direct_spatial and the direct prediction's cost, the
 // list-0/list-1 16x16 search_part and the direction costs, written as
 // dwords 0-21 of the record.  In pieces, so that the two lists' searches
-// can run on two waves (b_diag_kernel) or one after the other
+// can run on two waves (b_walk_kernel) or one after the other
 // (b_analyse_16).
 
 // direct_spatial's reference indices and MVs per 8x8 and whether every
