@@ -301,7 +301,7 @@ def main():
     reuse = tune["reuse_regions"] if tune else 1  # the plan's default keeps regions in LDS
     bytes_per_mb = BYTES_PER_MB + 4 * 256 * (args.nref - 1)
     achieved_gbs = bytes_per_mb * mbs_per_step / (search_ms * 1e-3) / 1e9
-    kname = "search_frames_mr_kernel" if args.nref > 1 else "search_frames_kernel"
+    kname = search_kernel_name(tune, args.nref > 1, False)
     refs = f"{args.nref} ref" + ("s" if args.nref > 1 else "")
     wl = {"1080p": f"plain GPU ME-only search, 1920x1080 60 frames IPPP, {refs}, QP 27, me_range 32 + qpel, "
                    "partitions 16x16/16x8/8x16/8x8, 12 rects 64-400 px, v in [-8,8] px/frame",
@@ -365,6 +365,22 @@ def fix_or_tune(plan, args, stream) -> dict | None:
     print(f"[bench] autotune: {t.waves} waves, {t.waves_per_group} per group, {t.waves_per_cu} per CU, "
           f"reuse {t.reuse_regions}: {t.ms:.2f} ms", file=sys.stderr, flush=True)
     return dict(t.as_dict(), ms=round(t.ms, 3))
+
+
+def search_kernel_name(tune: dict | None, mr: bool, hinted: bool) -> str:
+    """The frame-search kernel a plan launches at configuration ``tune``
+    (None: the plan's default, one walker per group at 12 per CU with
+    regions reused): the plain search has a kernel per configuration, the
+    searches with references or hints one at 12 and one at 16 walkers per
+    CU."""
+    wpg, wcu, reuse = ((tune["waves_per_group"], tune["waves_per_cu"], tune["reuse_regions"])
+                       if tune else (1, 12, 1))
+    if mr or hinted:
+        base = "search_frames_" + "_".join(n for n, on in (("mr", mr), ("hinted", hinted)) if on)
+        return base + ("16_kernel" if wcu == 16 else "_kernel")
+    if (wpg, wcu, reuse) == (1, 12, 1):
+        return "search_frames_kernel"
+    return f"search_frames_var_kernel<{wpg}, {wcu}, {'true' if reuse else 'false'}>"
 
 
 def cpu_share() -> int:
@@ -654,7 +670,7 @@ def b_leg(args, pcme, torch, dc, out, qp, me_range, stream, threads):
 def b_field_leg(args, pcme, torch, dc, field2, qp, me_range, stream):
     """The ME-only field of the same B picture as one dependent whole
     (pcme_b_field: each MB searched over the MBs committed before it, the
-    picture's anti-diagonals in order), the co-located MBs the P field of
+    rows walked as a wavefront in one launch), the co-located MBs the P field of
     frame 2; the C oracle's o_b_field (raster order, one thread: the
     picture is one dependent chain) times the same picture and the records
     are compared."""
@@ -680,6 +696,7 @@ def b_field_leg(args, pcme, torch, dc, field2, qp, me_range, stream):
         run()
     b.record(stream)
     torch.cuda.synchronize()
+    B.b_field_check(sp)  # no timed launch stalled
     ms = a.elapsed_time(b) / reps
     got = dout.cpu().numpy().view(pcme.B_MB_DTYPE).reshape(mbh, mbw)
     diag = mbw + 2 * (mbh - 1)
@@ -811,6 +828,7 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
         "confidence_threshold": {str(q): P.thr_part_for(q, thr_rule) for q in sorted(nets)},
         "mv_mass_threshold": thr_mv,
         "autotune": tune,
+        "search_kernel": search_kernel_name(tune, args.nref > 1, True),
         "net_roofline": {"bound": "mfma", "kernel": "hints_kernel", "achieved": round(tflops, 3),
                          "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5), "macs_per_mb": NET_MACS_PER_MB},

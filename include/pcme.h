@@ -468,9 +468,17 @@ typedef struct pcme_b_mb {
  * an MB waits for the row above's MB x+1, so the rows advance as a
  * wavefront).  out: mbw*mbh records (device), raster order.  No
  * counterpart in the reference, whose encoder walks the picture on the
- * CPU.  A wait that exceeds the library's spin limit (which the wavefront
- * never reaches) ends the launch with the records incomplete. */
+ * CPU.  Asynchronous on stream.  A wait that exceeds the library's spin
+ * limit (which the wavefront never reaches) ends the launch with the
+ * records incomplete and is kept until pcme_b_field_check reports it. */
 int pcme_b_field(const pcme_b_field_job *job, pcme_b_mb *out, void *stream);
+
+/* Waits for `stream` and returns PCME_ESTALL if any pcme_b_field launch on
+ * the current device since the previous check stalled (its records are
+ * incomplete); the flag is then cleared.  Launches on other streams are
+ * covered once they have completed.  No counterpart in the reference (its
+ * B analysis cannot stall). */
+int pcme_b_field_check(void *stream);
 
 /* ---- early P-skip gate (analyse_p, slice.rs:1884-1920) ----------------
  * Before searching an MB the encoder predicts it at its pskip_mv

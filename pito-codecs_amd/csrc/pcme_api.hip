@@ -74,11 +74,17 @@ struct Dev {
   uint8_t *scratch = nullptr;      // this device's copy of `scratch` (hipGetSymbolAddress)
   bool scratch_init = false;       // its control block has been cleared once
   SkipTables *skip_tab = nullptr;  // this device's copy of `skip_tables`, filled on first use
+  uint32_t *b_stall = nullptr;     // this device's copy of `b_field_stall` (first pcme_b_field call)
 };
 
 // Tables of the early P-skip gate; filled on a device by the first
 // pcme_skip_gate_batch call there.
 static __device__ __align__(256) SkipTables skip_tables;
+
+// Sticky stall word of the B field walkers: set by a launch whose
+// dependency wait passed its bound, read and cleared by pcme_b_field_check.
+// Host code reaches it through Dev::b_stall.
+static __device__ uint32_t b_field_stall;
 
 // Workspace of pcme_search_frames_noalloc: a control block, the job table,
 // the tickets and the exchange words of one call, in a file-scope device
@@ -1112,6 +1118,12 @@ int pcme_b_field(const pcme_b_field_job *job, pcme_b_mb *out, void *stream) {
   const size_t jb = sizeof(pcme_b_job) * (size_t)n, rb = sizeof(pcme_b_out) * (size_t)n;
   const size_t xwords = (size_t)n * 8, xb = xwords * 4;
   std::lock_guard<std::mutex> g(dv->mu);
+  if (!dv->b_stall) {
+    uint32_t *p = nullptr;
+    HIPCHK(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(b_field_stall)));
+    if (!p) return fail(PCME_EHIP, "hipGetSymbolAddress(b_field_stall) returned null");
+    dv->b_stall = p;
+  }
   WsGuard wg;
   wg.s = (hipStream_t)stream;
   CallWs *w = nullptr;
@@ -1124,9 +1136,24 @@ int pcme_b_field(const pcme_b_field_job *job, pcme_b_mb *out, void *stream) {
   uint32_t *ctl = (uint32_t *)((uint8_t *)w->dev + jb + rb + xb);
   hipLaunchKernelGGL(b_walk_clear_kernel, dim3(64), dim3(256), 0, wg.s, xw, xwords, ctl);
   hipLaunchKernelGGL(b_walk_kernel, dim3((unsigned)std::min(mbh, 1024)), dim3(128), 0, wg.s, dj, mbw, mbh, dr, out,
-                     F.col, xw, ctl, F.partitions ? 1 : 0, kSpinLimit);
+                     F.col, xw, ctl, dv->b_stall, F.partitions ? 1 : 0, kSpinLimit);
   HIPCHK(hipGetLastError());
   return PCME_OK;
+}
+
+int pcme_b_field_check(void *stream) {
+  Dev *dv;
+  int rc = get_dev(&dv);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(dv->mu);
+  if (!dv->b_stall) return PCME_OK;  // no B field launched on this device
+  HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+  uint32_t v = 0u;
+  HIPCHK(hipMemcpy(&v, dv->b_stall, sizeof v, hipMemcpyDeviceToHost));
+  if (v == 0u) return PCME_OK;
+  const uint32_t zero = 0u;
+  HIPCHK(hipMemcpy(dv->b_stall, &zero, sizeof zero, hipMemcpyHostToDevice));
+  return fail(PCME_ESTALL, "pcme_b_field: a dependency wait exceeded its bound; the records are incomplete");
 }
 
 static_assert(sizeof(pcme_skip_job) == 120 && sizeof(pcme_skip_out) == 16, "pcme_skip_* layout");

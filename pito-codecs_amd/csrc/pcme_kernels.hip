@@ -2392,14 +2392,18 @@ __device__ __forceinline__ void b_commit(const pcme_b_out &o, pcme_b_mb &m, int 
 // list 1's; wave 0 prices bi and fills the first half of the record; the
 // two waves run the 16x8 and 8x16 b_partition searches; one thread
 // commits.  The job and record stay in LDS.  ctl[0]: row tickets, ctl[1]:
-// stall flag (a wait past spin_limit polls ends the launch).
+// stall flag (a wait past spin_limit polls ends the launch: the walkers
+// leave their rows and take no more), *sticky: the same, kept across
+// launches until the host reads and clears it.
 __device__ __forceinline__ uint32_t bword(uint32_t tag, int ref, pcme_mv m) {
   return xword(tag, ref >= 0 ? 1u : 0u, pack_mv(m.x, m.y));
 }
 
-// Exchange word q once it carries `tag`; raises *stall past spin_limit polls.
+// Exchange word q once it carries `tag`; past spin_limit polls raises
+// *stall (this workgroup), ctl[1] (this launch) and *sticky (kept for the
+// host's check).
 __device__ __forceinline__ uint32_t bwait(const uint32_t *q, uint32_t tag, uint32_t spin_limit, int *stall,
-                                          uint32_t *ctl) {
+                                          uint32_t *ctl, uint32_t *sticky) {
   uint32_t v = 0, spins = 0;
   for (;;) {
     v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -2407,6 +2411,7 @@ __device__ __forceinline__ uint32_t bwait(const uint32_t *q, uint32_t tag, uint3
     if (++spins > spin_limit) {
       *stall = 1;
       __hip_atomic_store(ctl + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(sticky, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return v;
     }
     __builtin_amdgcn_s_sleep(2);
@@ -2416,7 +2421,8 @@ __device__ __forceinline__ uint32_t bwait(const uint32_t *q, uint32_t tag, uint3
 __global__ void __launch_bounds__(128) b_walk_kernel(const pcme_b_job *__restrict__ jobs, int mbw, int mbh,
                                                      pcme_b_out *res, pcme_b_mb *__restrict__ field,
                                                      const pcme_b_col *__restrict__ col, uint32_t *xw,
-                                                     uint32_t *ctl, int partitions, uint32_t spin_limit) {
+                                                     uint32_t *ctl, uint32_t *sticky, int partitions,
+                                                     uint32_t spin_limit) {
   __shared__ __attribute__((aligned(16))) uint32_t s_lds[2][LDS_PER_WAVE / 4];
   __shared__ __attribute__((aligned(16))) uint8_t s_pred[2][3][256];
   __shared__ __attribute__((aligned(16))) pcme_b_job s_job;
@@ -2434,11 +2440,11 @@ __global__ void __launch_bounds__(128) b_walk_kernel(const pcme_b_job *__restric
   for (;;) {
     if (t == 0) {
       s_ctl[0] = (int)atomicAdd(ctl, 1u);
-      s_ctl[1] = 0;
+      s_ctl[1] = __hip_atomic_load(ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
     }
     __syncthreads();
     const int my = s_ctl[0];
-    if (my >= mbh) return;
+    if (my >= mbh || s_ctl[1]) return;  // done, or another row stalled
     for (int mx = 0; mx < mbw; mx++) {
       const int k = my * mbw + mx;
       // wait for the row above's MB x+1 (x in the last column); its words
@@ -2446,12 +2452,12 @@ __global__ void __launch_bounds__(128) b_walk_kernel(const pcme_b_job *__restric
       if (t < 8 && my > 0) {
         const int dx = mx + 1 < mbw ? 1 : 0;
         if (mx == 0) {
-          s_above[1][t] = bwait(xw + (size_t)(k - mbw) * 8 + t, tag, spin_limit, &s_ctl[1], ctl);  // MB x
+          s_above[1][t] = bwait(xw + (size_t)(k - mbw) * 8 + t, tag, spin_limit, &s_ctl[1], ctl, sticky);  // MB x
         } else {
           s_above[0][t] = s_above[1][t];
           s_above[1][t] = s_above[2][t];
         }
-        s_above[2][t] = bwait(xw + (size_t)(k - mbw + dx) * 8 + t, tag, spin_limit, &s_ctl[1], ctl);
+        s_above[2][t] = bwait(xw + (size_t)(k - mbw + dx) * 8 + t, tag, spin_limit, &s_ctl[1], ctl, sticky);
       }
       if (t < (int)(sizeof(pcme_b_job) / 4)) ((uint32_t *)&s_job)[t] = ((const uint32_t *)&jobs[k])[t];
       if (t < (int)(sizeof(pcme_b_out) / 4)) ((uint32_t *)&s_rec)[t] = 0u;

@@ -96,13 +96,19 @@ def b_field_launch(cur: DeviceClip, t: int, ref0: tuple[DeviceClip, int], ref1: 
                    partitions: bool = True, stream=None) -> None:
     """pcme_b_field on device buffers: col_ptr holds mbw*mbh B_COL_DTYPE
     records, out_ptr receives mbw*mbh B_MB_DTYPE records (stream order, no
-    synchronisation)."""
+    synchronisation; b_field_check reports a stall)."""
     from . import BFieldJob
     (c0, t0), (c1, t1) = ref0, ref1
     job = BFieldJob(cur.plane_ptr(0, t), (ctypes.c_void_p * 4)(*[c0.plane_ptr(k, t0) for k in range(4)]),
                     (ctypes.c_void_p * 4)(*[c1.plane_ptr(k, t1) for k in range(4)]), col_ptr, cur.mbw, cur.mbh,
                     int(qp), int(me_range), int(subpel_rounds), 1 if partitions else 0)
     _check(load().pcme_b_field(ctypes.byref(job), out_ptr, _stream_ptr(stream)))
+
+
+def b_field_check(stream=None) -> None:
+    """pcme_b_field_check: waits for the stream; PcmeError(PCME_ESTALL) if a
+    B field launch on the device stalled since the previous check."""
+    _check(load().pcme_b_field_check(_stream_ptr(stream)))
 
 
 def b_field(cur: DeviceClip, t: int, ref0: tuple[DeviceClip, int], ref1: tuple[DeviceClip, int], col, qp: int,
@@ -121,6 +127,7 @@ def b_field(cur: DeviceClip, t: int, ref0: tuple[DeviceClip, int], ref1: tuple[D
     out = torch.empty(cur.mbw * cur.mbh * B_MB_DTYPE.itemsize, dtype=torch.uint8, device=dev)
     b_field_launch(cur, t, ref0, ref1, dcol.data_ptr(), out.data_ptr(), qp, me_range, subpel_rounds, partitions,
                    stream)
+    b_field_check(stream)
     torch.cuda.synchronize()
     return out.cpu().numpy().view(B_MB_DTYPE).reshape(cur.mbh, cur.mbw)
 

@@ -1,8 +1,9 @@
-"""The ME-only field of a whole B picture on the GPU (pcme_b_field: its
-anti-diagonals one after another, each MB searched over the MBs committed
-before it), bit for bit against the oracle's o_b_field (analyse_b's
-searches and decision with rd off, raster order), at 320x240 with and
-without partitions and at 1080p.
+"""The ME-only field of a whole B picture on the GPU (pcme_b_field: one
+launch of row walkers advancing as a wavefront, each MB searched over the
+MBs committed before it), bit for bit against the oracle's o_b_field
+(analyse_b's searches and decision with rd off, raster order), at 320x240
+with and without partitions and at 1080p; a one-row and a one-column
+picture; and the stall path (pcme_b_field_check).
 
 Each case prints what it runs before the GPU step so a failing run names
 its case."""
@@ -32,11 +33,13 @@ def _oracle_cols(oracle, field_p):
     return cols
 
 
-@pytest.mark.parametrize("w,h,qp,partitions", [(320, 240, 27, True), (320, 240, 32, False), (1920, 1080, 27, True)])
+@pytest.mark.parametrize("w,h,qp,partitions", [(320, 240, 27, True), (320, 240, 32, False), (1920, 1080, 27, True),
+                                                (320, 16, 27, True), (16, 240, 22, True)])
 def test_b_field_matches_oracle(gpu, oracle, w, h, qp, partitions):
     from pcme import bsearch as B
     say("B field", w, h, qp, partitions)
-    y, _, _ = gpu.synth_clip(gpu.ClipSpec(w, h, 3, seed=0x5EED6000 + w, nrect=6, vmax=6.0))
+    y, _, _ = gpu.synth_clip(gpu.ClipSpec(w, h, 3, seed=0x5EED6000 + w + h, nrect=6 if min(w, h) > 16 else 1,
+                                          vmax=6.0))
     mbw, mbh = gpu.mb_dims(w, h)
     dc = gpu.DeviceClip(y, mbw, mbh)
     dc.prepare()
@@ -50,5 +53,38 @@ def test_b_field_matches_oracle(gpu, oracle, w, h, qp, partitions):
     wt = np.ascontiguousarray(want).view(np.uint8).reshape(mbh * mbw, -1)
     bad = np.nonzero((g != wt).any(axis=1))[0]
     assert bad.size == 0, f"{bad.size} of {mbw * mbh} B records differ, first MB {divmod(int(bad[0]), mbw)}"
-    modes = np.bincount(got["mode"].reshape(-1), minlength=4)
-    assert modes[0] > 0 and modes[1] > 0 and (modes[2] + modes[3] > 0) == partitions
+    if min(mbw, mbh) > 1:
+        modes = np.bincount(got["mode"].reshape(-1), minlength=4)
+        assert modes[0] > 0 and modes[1] > 0 and (modes[2] + modes[3] > 0) == partitions
+
+
+def test_b_field_stall_reports_estall(gpu, oracle):
+    """With the dependency-wait bound forced to one poll, rows that must wait
+    give up, every walker leaves, and pcme_b_field_check reports
+    PCME_ESTALL (once: the flag is then clear); with the default bound the
+    next field is whole and equal to one computed before."""
+    import torch
+    from pcme import bsearch as B
+    say("B field stall path")
+    lib = gpu.load()
+    w, h, qp = 320, 240, 27
+    y, _, _ = gpu.synth_clip(gpu.ClipSpec(w, h, 3, seed=0x5EED6100, nrect=6, vmax=6.0))
+    mbw, mbh = gpu.mb_dims(w, h)
+    dc = gpu.DeviceClip(y, mbw, mbh)
+    dc.prepare()
+    fp = oracle.search_clip_me(y[1:3], 32, 1, True, qp)[0]
+    col = B.col_records(fp, mbh, mbw)
+    first = B.b_field(dc, 1, (dc, 0), (dc, 2), col, qp)
+    dcol = torch.as_tensor(np.ascontiguousarray(col.reshape(-1)).view(np.uint8)).to("cuda")
+    dout = torch.zeros(mbw * mbh * gpu.B_MB_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    old = lib.pcme_set_spin_limit(1)
+    try:
+        B.b_field_launch(dc, 1, (dc, 0), (dc, 2), dcol.data_ptr(), dout.data_ptr(), qp)
+        with pytest.raises(gpu.PcmeError) as e:
+            B.b_field_check()
+        assert e.value.code == gpu.PCME_ESTALL
+        B.b_field_check()  # cleared
+    finally:
+        lib.pcme_set_spin_limit(old)
+    again = B.b_field(dc, 1, (dc, 0), (dc, 2), col, qp)
+    assert again.tobytes() == first.tobytes()
