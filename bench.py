@@ -48,6 +48,8 @@ BYTES_PER_MB = 256 + 4 * 256 + 128 + 32
 # fc 293 x 32.
 NET_MACS_PER_MB = 144 * 32 * 18 + 36 * 32 * 288 * 2 + 9 * 32 * 288 + 293 * 32
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+# B pictures of the batched B field leg (pcme_b_fields, one launch)
+B_BATCH = 16
 
 
 def parse():
@@ -663,21 +665,25 @@ def b_leg(args, pcme, torch, dc, out, qp, me_range, stream, threads):
                                        f"{reps} repetitions",
                              "seconds": round(secs, 3)}
         o["parity_sample"] = bool(got.tobytes() == want.tobytes())
-    o["field"] = b_field_leg(args, pcme, torch, dc, f[1], qp, me_range, stream)
+    o["field"] = b_field_leg(args, pcme, torch, dc, f, qp, me_range, stream)
     return o
 
 
-def b_field_leg(args, pcme, torch, dc, field2, qp, me_range, stream):
+def b_field_leg(args, pcme, torch, dc, fields, qp, me_range, stream):
     """The ME-only field of the same B picture as one dependent whole
     (pcme_b_field: each MB searched over the MBs committed before it, the
     rows walked as a wavefront in one launch), the co-located MBs the P field of
     frame 2; the C oracle's o_b_field (raster order, one thread: the
     picture is one dependent chain) times the same picture and the records
-    are compared."""
+    are compared.  Then B_BATCH pictures of the clip (1, 3, 5, ..., each
+    between its neighbours, the co-located MBs the P field of the next
+    frame) in one pcme_b_fields launch, the walkers taking their rows side
+    by side; picture 1's records are compared with the single field's."""
     import numpy as np
     from pcme import bsearch as B
     from oracle import oracle as O
     mbw, mbh = dc.mbw, dc.mbh
+    field2 = fields[1]
     col = B.col_records(field2, mbh, mbw)
     n = mbw * mbh
     dcol = torch.as_tensor(np.ascontiguousarray(col.reshape(-1)).view(np.uint8)).to("cuda")
@@ -705,6 +711,31 @@ def b_field_leg(args, pcme, torch, dc, field2, qp, me_range, stream):
          "ms_per_picture": round(ms, 3), "mblocks_per_s": round(n / (ms * 1e-3) / 1e6, 4),
          "modes": [int(x) for x in np.bincount(got["mode"].reshape(-1), minlength=4)]}
     print(f"[bench] B field: {ms:.3f} ms per picture", file=sys.stderr, flush=True)
+    nb = min(B_BATCH, (dc.n - 1) // 2)
+    ts = [1 + 2 * i for i in range(nb)]
+    dcols = [torch.as_tensor(np.ascontiguousarray(B.col_records(fields[t], mbh, mbw).reshape(-1)).view(np.uint8))
+             .to("cuda") for t in ts]
+    douts = torch.empty((nb, n * pcme.B_MB_DTYPE.itemsize), dtype=torch.uint8, device="cuda")
+    bjobs = [B.b_field_job(dc, t, (dc, t - 1), (dc, t + 1), dcol.data_ptr(), qp, me_range, 1, True)
+             for t, dcol in zip(ts, dcols)]
+    optrs = [douts[i].data_ptr() for i in range(nb)]
+    for _ in range(max(1, args.warmup)):
+        B.b_fields_launch(bjobs, optrs, sp)
+    a.record(stream)
+    for _ in range(reps):
+        B.b_fields_launch(bjobs, optrs, sp)
+    b.record(stream)
+    torch.cuda.synchronize()
+    B.b_field_check(sp)
+    bms = a.elapsed_time(b) / reps
+    same = bool(douts[0].cpu().numpy().tobytes() == np.ascontiguousarray(got).tobytes())
+    o["batch"] = {"workload": f"{nb} B pictures of the clip (frames {ts[0]}, {ts[1]}, ..., {ts[-1]}) in one "
+                              "pcme_b_fields launch", "pictures": nb, "ms_per_launch": round(bms, 3),
+                  "ms_per_picture": round(bms / nb, 3), "mblocks_per_s": round(nb * n / (bms * 1e-3) / 1e6, 4),
+                  "speedup_vs_one_per_launch": round(ms * nb / bms, 2),
+                  "picture_1_same_as_single_field": same}
+    print(f"[bench] B fields: {nb} pictures in {bms:.3f} ms ({bms / nb:.3f} ms per picture)", file=sys.stderr,
+          flush=True)
     if not args.no_cpu:
         cols = (O.ombinfo2 * n)()
         for i, r in enumerate(np.asarray(field2).reshape(-1)):

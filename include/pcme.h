@@ -473,6 +473,15 @@ typedef struct pcme_b_mb {
  * records incomplete and is kept until pcme_b_field_check reports it. */
 int pcme_b_field(const pcme_b_field_job *job, pcme_b_mb *out, void *stream);
 
+/* pcme_b_field for njobs independent B pictures (1 <= njobs <= 1024, at
+ * most 2^22 MBs in all; sizes and parameters may differ) in one launch:
+ * the walkers take the pictures' rows in (row, picture) order, so the
+ * pictures advance side by side -- the device form of the reference's
+ * frame threads, which analyse several pictures at once.  outs[i]: the
+ * mbw*mbh records of picture i (device).  Results are those of one
+ * pcme_b_field call per picture; stalls as for pcme_b_field. */
+int pcme_b_fields(const pcme_b_field_job *jobs, int njobs, pcme_b_mb *const *outs, void *stream);
+
 /* Waits for `stream` and returns PCME_ESTALL if any pcme_b_field launch on
  * the current device since the previous check stalled (its records are
  * incomplete); the flag is then cleared.  Launches on other streams are

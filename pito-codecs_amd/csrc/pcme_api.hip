@@ -68,6 +68,7 @@ struct Dev {
   int resident_var[8] = {0};       // resident waves of each full-search configuration (kVariants)
   int resident_16[3] = {0, 0, 0};  // resident waves at 16 per CU: hinted, multi-reference, both
   int cus = 1;                     // compute units
+  int b_walk_resident = 1;         // resident workgroups of b_walk_kernel
   std::vector<CallWs *> ws;        // per-call workspaces (at most kMaxWs)
   Plan *cached = nullptr;          // plan of the last pcme_search_frames call
   std::vector<pcme_frame_job> cached_jobs;
@@ -190,6 +191,11 @@ static int get_dev(Dev **out) {
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, variant_fn(v), 64 * kVariants[v].wpg, 0));
     if (per_cu < 1) per_cu = 1;
     dv->resident_var[v] = per_cu * kVariants[v].wpg * p.multiProcessorCount;
+  }
+  {
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)b_walk_kernel, 128, 0));
+    dv->b_walk_resident = std::max(1, per_cu) * p.multiProcessorCount;
   }
   HIPCHK(hipGetSymbolAddress((void **)&dv->scratch, HIP_SYMBOL(scratch)));
   if (!dv->scratch) return fail(PCME_EHIP, "hipGetSymbolAddress(scratch) returned null");
@@ -1075,48 +1081,85 @@ int pcme_b_analyse_batch(const pcme_b_job *jobs, int njobs, pcme_b_out *out, voi
 
 static_assert(sizeof(pcme_b_field_job) == 104, "pcme_b_field_job layout");
 
-int pcme_b_field(const pcme_b_field_job *job, pcme_b_mb *out, void *stream) {
-  if (!job || !out || ((uintptr_t)out & 7u)) return fail(PCME_EINVAL, "pcme_b_field: bad arguments");
-  const pcme_b_field_job &F = *job;
+static int check_b_field_job(const pcme_b_field_job &F, int i) {
   if (F.mbw < 1 || F.mbh < 1 || F.mbw > 4096 || F.mbh > 4096 || !F.src || !F.col || ((uintptr_t)F.src & 3u) ||
       ((uintptr_t)F.col & 3u))
-    return fail(PCME_EINVAL, "pcme_b_field: bad picture size, source or co-located array");
+    return fail(PCME_EINVAL, "pcme_b_fields: picture %d: bad size, source or co-located array", i);
   for (int k = 0; k < 4; k++)
     if (!F.ref0[k] || !F.ref1[k] || (((uintptr_t)F.ref0[k] | (uintptr_t)F.ref1[k]) & 3u))
-      return fail(PCME_EINVAL, "pcme_b_field: reference plane null or misaligned");
+      return fail(PCME_EINVAL, "pcme_b_fields: picture %d: reference plane null or misaligned", i);
   if (F.qp < 0 || F.qp > 51 || F.me_range < 0 || F.me_range > 1024 || F.subpel_rounds < 0 || F.subpel_rounds > 8)
-    return fail(PCME_EINVAL, "pcme_b_field: qp, me_range or subpel_rounds out of range");
+    return fail(PCME_EINVAL, "pcme_b_fields: picture %d: qp, me_range or subpel_rounds out of range", i);
+  return PCME_OK;
+}
+
+// Upper bounds of one pcme_b_fields call: pictures, and MBs over all of them.
+static const int kMaxBPics = 1024;
+static const size_t kMaxBMbs = (size_t)1 << 22;
+
+int pcme_b_fields(const pcme_b_field_job *jobs, int njobs, pcme_b_mb *const *outs, void *stream) {
+  if (njobs < 0 || njobs > kMaxBPics || (njobs > 0 && (!jobs || !outs)))
+    return fail(PCME_EINVAL, "pcme_b_fields: bad arguments (njobs %d, at most %d)", njobs, kMaxBPics);
+  int rc;
+  size_t n = 0;
+  int maxh = 0;
+  for (int i = 0; i < njobs; i++) {
+    if (!outs[i] || ((uintptr_t)outs[i] & 7u)) return fail(PCME_EINVAL, "pcme_b_fields: output %d null or misaligned", i);
+    if ((rc = check_b_field_job(jobs[i], i))) return rc;
+    n += (size_t)jobs[i].mbw * jobs[i].mbh;
+    maxh = std::max(maxh, jobs[i].mbh);
+  }
+  if (n > kMaxBMbs) return fail(PCME_EINVAL, "pcme_b_fields: %zu MBs in one call, at most %zu", n, kMaxBMbs);
+  if (njobs == 0) return PCME_OK;
   Dev *dv;
-  int rc = get_dev(&dv);
-  if (rc) return rc;
-  const int mbw = F.mbw, mbh = F.mbh, n = mbw * mbh;
-  const int stride = 16 * mbw + 2 * PCME_PAD;
-  const uint32_t lam = lambda_of(F.qp);
-  // the job table in raster order (row walkers, one workgroup per row)
-  std::vector<pcme_b_job> jobs((size_t)n);
-  for (int my = 0; my < mbh; my++)
-    for (int mx = 0; mx < mbw; mx++) {
-      pcme_b_job &J = jobs[(size_t)my * mbw + mx];
-      memset(&J, 0, sizeof J);
-      J.src = F.src + (size_t)(PCME_PAD + 16 * my) * stride + PCME_PAD + 16 * mx;
-      J.sstride = stride;
-      for (int q = 0; q < 4; q++) {
-        J.ref0[q] = F.ref0[q];
-        J.ref1[q] = F.ref1[q];
+  if ((rc = get_dev(&dv))) return rc;
+  // host image: per-MB jobs (raster within each picture) | pictures | tickets
+  std::vector<pcme_b_job> bj(n);
+  std::vector<BPic> pics((size_t)njobs);
+  size_t k0 = 0;
+  for (int i = 0; i < njobs; i++) {
+    const pcme_b_field_job &F = jobs[i];
+    const int mbw = F.mbw, mbh = F.mbh;
+    const int stride = 16 * mbw + 2 * PCME_PAD;
+    const uint32_t lam = lambda_of(F.qp);
+    pics[(size_t)i] = BPic{F.col, outs[i], (int)k0, mbw, mbh, F.partitions ? 1 : 0};
+    for (int my = 0; my < mbh; my++)
+      for (int mx = 0; mx < mbw; mx++) {
+        pcme_b_job &J = bj[k0 + (size_t)my * mbw + mx];
+        memset(&J, 0, sizeof J);
+        J.src = F.src + (size_t)(PCME_PAD + 16 * my) * stride + PCME_PAD + 16 * mx;
+        J.sstride = stride;
+        for (int q = 0; q < 4; q++) {
+          J.ref0[q] = F.ref0[q];
+          J.ref1[q] = F.ref1[q];
+        }
+        J.mbw = mbw;
+        J.mbh = mbh;
+        J.mx = mx;
+        J.my = my;
+        J.lam = lam;
+        J.me_range = F.me_range;
+        J.subpel_rounds = F.subpel_rounds;
+        const int need = 16 * (my + 1) + 96;  // frame.rs:152 with picture.rs:261-265
+        J.vlim = need >= 16 * mbh ? 16 * mbh + 44 : need;
       }
-      J.mbw = mbw;
-      J.mbh = mbh;
-      J.mx = mx;
-      J.my = my;
-      J.lam = lam;
-      J.me_range = F.me_range;
-      J.subpel_rounds = F.subpel_rounds;
-      const int need = 16 * (my + 1) + 96;  // frame.rs:152 with picture.rs:261-265
-      J.vlim = need >= 16 * mbh ? 16 * mbh + 44 : need;
-    }
-  // workspace: jobs | records | exchange words (8 per MB) | 2 control words
-  const size_t jb = sizeof(pcme_b_job) * (size_t)n, rb = sizeof(pcme_b_out) * (size_t)n;
-  const size_t xwords = (size_t)n * 8, xb = xwords * 4;
+    k0 += (size_t)mbw * mbh;
+  }
+  // (row, picture) order: the row above of a picture is an earlier ticket
+  std::vector<int2> tk;
+  for (int y = 0; y < maxh; y++)
+    for (int i = 0; i < njobs; i++)
+      if (y < jobs[i].mbh) tk.push_back(make_int2(i, y));
+  const size_t jb = sizeof(pcme_b_job) * n, pb = sizeof(BPic) * (size_t)njobs, tb = sizeof(int2) * tk.size();
+  const size_t hb = (jb + pb + 15) / 16 * 16 + tb;  // uploaded: jobs | pictures | tickets
+  const size_t po = jb, to = (jb + pb + 15) / 16 * 16;
+  const size_t ro = (hb + 255) / 256 * 256, rb = sizeof(pcme_b_out) * n;
+  const size_t xo = ro + rb, xwords = n * 8, xb = xwords * 4;
+  const size_t co = xo + xb;
+  std::vector<uint8_t> img(hb);
+  memcpy(img.data(), bj.data(), jb);
+  memcpy(img.data() + po, pics.data(), pb);
+  memcpy(img.data() + to, tk.data(), tb);
   std::lock_guard<std::mutex> g(dv->mu);
   if (!dv->b_stall) {
     uint32_t *p = nullptr;
@@ -1127,18 +1170,26 @@ int pcme_b_field(const pcme_b_field_job *job, pcme_b_mb *out, void *stream) {
   WsGuard wg;
   wg.s = (hipStream_t)stream;
   CallWs *w = nullptr;
-  if ((rc = take_ws(dv, wg.s, jb + rb + xb + 64, jb, &w))) return rc;
+  if ((rc = take_ws(dv, wg.s, co + 64, hb, &w))) return rc;
   wg.w = w;
-  if ((rc = ws_upload(w, jobs.data(), jb, wg.s))) return rc;
-  pcme_b_job *dj = (pcme_b_job *)w->dev;
-  pcme_b_out *dr = (pcme_b_out *)((uint8_t *)w->dev + jb);
-  uint32_t *xw = (uint32_t *)((uint8_t *)w->dev + jb + rb);
-  uint32_t *ctl = (uint32_t *)((uint8_t *)w->dev + jb + rb + xb);
+  if ((rc = ws_upload(w, img.data(), hb, wg.s))) return rc;
+  uint8_t *base = (uint8_t *)w->dev;
+  uint32_t *xw = (uint32_t *)(base + xo);
+  uint32_t *ctl = (uint32_t *)(base + co);
   hipLaunchKernelGGL(b_walk_clear_kernel, dim3(64), dim3(256), 0, wg.s, xw, xwords, ctl);
-  hipLaunchKernelGGL(b_walk_kernel, dim3((unsigned)std::min(mbh, 1024)), dim3(128), 0, wg.s, dj, mbw, mbh, dr, out,
-                     F.col, xw, ctl, dv->b_stall, F.partitions ? 1 : 0, kSpinLimit);
+  // as many walkers as are resident at once (or rows)
+  const unsigned grid = (unsigned)std::min<size_t>(tk.size(), (size_t)dv->b_walk_resident);
+  hipLaunchKernelGGL(b_walk_kernel, dim3(grid), dim3(128), 0, wg.s, (const pcme_b_job *)base,
+                     (const BPic *)(base + po), (const int2 *)(base + to), (int)tk.size(), (pcme_b_out *)(base + ro),
+                     xw, ctl, dv->b_stall, kSpinLimit);
   HIPCHK(hipGetLastError());
   return PCME_OK;
+}
+
+int pcme_b_field(const pcme_b_field_job *job, pcme_b_mb *out, void *stream) {
+  if (!job || !out) return fail(PCME_EINVAL, "pcme_b_field: bad arguments");
+  pcme_b_mb *const outs[1] = {out};
+  return pcme_b_fields(job, 1, outs, stream);
 }
 
 int pcme_b_field_check(void *stream) {

@@ -86,9 +86,16 @@ __global__ void search_part_batch_kernel(const pcme_part_job *__restrict__ jobs,
 __global__ void b_analyse_16_kernel(const pcme_b_job *__restrict__ jobs, int njobs, pcme_b_out *out);
 template <int K>
 __global__ void b_analyse_part_kernel(const pcme_b_job *__restrict__ jobs, int njobs, pcme_b_out *out);
-__global__ void b_walk_kernel(const pcme_b_job *__restrict__ jobs, int mbw, int mbh, pcme_b_out *res,
-                              pcme_b_mb *__restrict__ field, const pcme_b_col *__restrict__ col, uint32_t *xw,
-                              uint32_t *ctl, uint32_t *sticky, int partitions, uint32_t spin_limit);
+// One picture of a B field launch: its MBs are the launch's jobs, records
+// and exchange words (8 per MB) [job0, job0 + mbw*mbh), raster order.
+struct BPic {
+  const pcme_b_col *col;  // mbw*mbh co-located MBs
+  pcme_b_mb *field;       // mbw*mbh committed records
+  int job0, mbw, mbh, partitions;
+};
+__global__ void b_walk_kernel(const pcme_b_job *__restrict__ jobs, const BPic *__restrict__ pics,
+                              const int2 *__restrict__ tickets, int ntickets, pcme_b_out *res, uint32_t *xw,
+                              uint32_t *ctl, uint32_t *sticky, uint32_t spin_limit);
 __global__ void b_walk_clear_kernel(uint32_t *xw, size_t nwords, uint32_t *ctl);
 __global__ void pad_planes_kernel(const uint8_t *__restrict__ src, int64_t src_pitch, int sstride, int w,
                                   int h, uint8_t *__restrict__ dst, int64_t dst_pitch, int dstride,

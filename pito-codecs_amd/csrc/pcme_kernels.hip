@@ -2380,9 +2380,11 @@ __device__ __forceinline__ void b_commit(const pcme_b_out &o, pcme_b_mb &m, int 
     }
 }
 
-// The B field in one launch: persistent row walkers (a workgroup of two
-// waves walks one MB row left to right; rows are taken in order from a
-// ticket counter).  MB (x, y) waits for the row above's MB x+1 (x for the
+// B fields in one launch: persistent row walkers (a workgroup of two waves
+// walks one MB row of one picture left to right; rows are taken through a
+// ticket counter from a table of (picture, row) tickets in (row, picture)
+// order, so a row's dependency, the row above of its picture, always holds
+// an earlier ticket, taken by a running walker).  MB (x, y) waits for the row above's MB x+1 (x for the
 // last column), publishes its bottom blocks 12-15 of both lists as eight
 // exchange words (the P walker's format: tag, list used, MV) for the row
 // below, and keeps its right column (blocks 3, 7, 11, 15) in LDS for the
@@ -2391,7 +2393,9 @@ __device__ __forceinline__ void b_commit(const pcme_b_out &o, pcme_b_mb &m, int 
 // wave 0 prices direct and searches list 0's 16x16 while wave 1 searches
 // list 1's; wave 0 prices bi and fills the first half of the record; the
 // two waves run the 16x8 and 8x16 b_partition searches; one thread
-// commits.  The job and record stay in LDS.  ctl[0]: row tickets, ctl[1]:
+// commits.  The job and record stay in LDS.  Picture p's MBs are jobs,
+// records and exchange words [job0, job0 + mbw*mbh) of the launch (BPic).
+// ctl[0]: tickets taken, ctl[1]:
 // stall flag (a wait past spin_limit polls ends the launch: the walkers
 // leave their rows and take no more), *sticky: the same, kept across
 // launches until the host reads and clears it.
@@ -2418,11 +2422,11 @@ __device__ __forceinline__ uint32_t bwait(const uint32_t *q, uint32_t tag, uint3
   }
 }
 
-__global__ void __launch_bounds__(128) b_walk_kernel(const pcme_b_job *__restrict__ jobs, int mbw, int mbh,
-                                                     pcme_b_out *res, pcme_b_mb *__restrict__ field,
-                                                     const pcme_b_col *__restrict__ col, uint32_t *xw,
-                                                     uint32_t *ctl, uint32_t *sticky, int partitions,
-                                                     uint32_t spin_limit) {
+__global__ void __launch_bounds__(128) b_walk_kernel(const pcme_b_job *__restrict__ jobs,
+                                                     const BPic *__restrict__ pics,
+                                                     const int2 *__restrict__ tickets, int ntickets,
+                                                     pcme_b_out *res, uint32_t *xw, uint32_t *ctl,
+                                                     uint32_t *sticky, uint32_t spin_limit) {
   __shared__ __attribute__((aligned(16))) uint32_t s_lds[2][LDS_PER_WAVE / 4];
   __shared__ __attribute__((aligned(16))) uint8_t s_pred[2][3][256];
   __shared__ __attribute__((aligned(16))) pcme_b_job s_job;
@@ -2431,7 +2435,7 @@ __global__ void __launch_bounds__(128) b_walk_kernel(const pcme_b_job *__restric
   __shared__ uint32_t s_mv16[3];
   __shared__ uint32_t s_above[3][8];  // exchange words of the row above: MB x-1, x, x+1
   __shared__ pcme_nb_block s_left[2][4];  // this row's previous MB: blocks 3, 7, 11, 15 per list
-  __shared__ int s_ctl[2];            // the row taken; a stall
+  __shared__ int s_ctl[2];            // the ticket taken; a stall
   static_assert(sizeof(pcme_b_job) % 4 == 0 && sizeof(pcme_b_job) / 4 <= 128 && sizeof(pcme_b_out) / 4 <= 128 &&
                     sizeof(pcme_b_mb) % 4 == 0 && sizeof(pcme_b_mb) / 4 <= 128,
                 "one dword per thread");
@@ -2443,8 +2447,16 @@ __global__ void __launch_bounds__(128) b_walk_kernel(const pcme_b_job *__restric
       s_ctl[1] = __hip_atomic_load(ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
     }
     __syncthreads();
-    const int my = s_ctl[0];
-    if (my >= mbh || s_ctl[1]) return;  // done, or another row stalled
+    if (s_ctl[0] >= ntickets || s_ctl[1]) return;  // done, or another row stalled
+    // ticket = (picture, row), read-only in this launch
+    const int2 tk = tickets[s_ctl[0]];
+    const BPic &P = pics[tk.x];
+    const int my = tk.y, mbw = P.mbw, partitions = P.partitions;
+    const pcme_b_job *pj = jobs + P.job0;
+    pcme_b_out *pres = res + P.job0;
+    uint32_t *pxw = xw + (size_t)P.job0 * 8;
+    pcme_b_mb *field = P.field;
+    const pcme_b_col *col = P.col;
     for (int mx = 0; mx < mbw; mx++) {
       const int k = my * mbw + mx;
       // wait for the row above's MB x+1 (x in the last column); its words
@@ -2452,14 +2464,14 @@ __global__ void __launch_bounds__(128) b_walk_kernel(const pcme_b_job *__restric
       if (t < 8 && my > 0) {
         const int dx = mx + 1 < mbw ? 1 : 0;
         if (mx == 0) {
-          s_above[1][t] = bwait(xw + (size_t)(k - mbw) * 8 + t, tag, spin_limit, &s_ctl[1], ctl, sticky);  // MB x
+          s_above[1][t] = bwait(pxw + (size_t)(k - mbw) * 8 + t, tag, spin_limit, &s_ctl[1], ctl, sticky);  // MB x
         } else {
           s_above[0][t] = s_above[1][t];
           s_above[1][t] = s_above[2][t];
         }
-        s_above[2][t] = bwait(xw + (size_t)(k - mbw + dx) * 8 + t, tag, spin_limit, &s_ctl[1], ctl, sticky);
+        s_above[2][t] = bwait(pxw + (size_t)(k - mbw + dx) * 8 + t, tag, spin_limit, &s_ctl[1], ctl, sticky);
       }
-      if (t < (int)(sizeof(pcme_b_job) / 4)) ((uint32_t *)&s_job)[t] = ((const uint32_t *)&jobs[k])[t];
+      if (t < (int)(sizeof(pcme_b_job) / 4)) ((uint32_t *)&s_job)[t] = ((const uint32_t *)&pj[k])[t];
       if (t < (int)(sizeof(pcme_b_out) / 4)) ((uint32_t *)&s_rec)[t] = 0u;
       __syncthreads();
       if (s_ctl[1]) return;  // a stall: the launch ends (every wave reads the same flag)
@@ -2534,7 +2546,7 @@ __global__ void __launch_bounds__(128) b_walk_kernel(const pcme_b_job *__restric
       else
         b_analyse_part<1>(J, &s_rec, s_lds[1], s_pred[1], mv16);
       __syncthreads();  // the whole record
-      if (t < (int)(sizeof(pcme_b_out) / 4)) ((uint32_t *)&res[k])[t] = ((const uint32_t *)&s_rec)[t];
+      if (t < (int)(sizeof(pcme_b_out) / 4)) ((uint32_t *)&pres[k])[t] = ((const uint32_t *)&s_rec)[t];
       if (t == 0) b_commit(s_rec, s_mb, partitions);
       __syncthreads();  // the committed record
       if (t < (int)(sizeof(pcme_b_mb) / 4)) ((uint32_t *)&field[k])[t] = ((const uint32_t *)&s_mb)[t];
@@ -2547,7 +2559,7 @@ __global__ void __launch_bounds__(128) b_walk_kernel(const pcme_b_job *__restric
         const pcme_mv mv = m.mv[l][blk];
         const int rf = m.ref[l][blk8(blk)];
         if (t < 8) {
-          __hip_atomic_store(xw + (size_t)k * 8 + t, bword(tag, rf, mv), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(pxw + (size_t)k * 8 + t, bword(tag, rf, mv), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
           pcme_nb_block b;
           b.reserved[0] = b.reserved[1] = b.reserved[2] = 0;

@@ -239,6 +239,7 @@ EXPORTS = (
     "pcme_lowres", "pcme_complexity", "pcme_mbtree_analyse", "pcme_search_part_batch", "pcme_b_analyse_batch",
     "pcme_b_field",
     "pcme_b_field_check",
+    "pcme_b_fields",
     "pcme_aq_offsets", "pcme_mbtree_propagate", "pcme_mbtree_offsets", "pcme_skip_gate_batch",
 )
 
@@ -280,6 +281,7 @@ def load() -> ctypes.CDLL:
         "pcme_b_analyse_batch": (c.c_int, [c.POINTER(BJob), c.c_int, vp, vp]),
         "pcme_b_field": (c.c_int, [c.POINTER(BFieldJob), vp, vp]),
         "pcme_b_field_check": (c.c_int, [vp]),
+        "pcme_b_fields": (c.c_int, [c.POINTER(BFieldJob), c.c_int, c.POINTER(vp), vp]),
         "pcme_skip_gate_batch": (c.c_int, [c.POINTER(SkipJob), c.c_int, vp, vp]),
         "pcme_pred_frame": (c.c_int, [vp, vp, vp, vp, vp, vp, vp, c.c_int32, c.c_int32, vp, vp, vp, vp]),
         "pcme_synth_clip": (c.c_int, [c.POINTER(SynthCfg), vp, vp, vp, c.c_int32]),

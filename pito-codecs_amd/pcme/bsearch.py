@@ -91,18 +91,36 @@ def b_analyse(jobs: np.ndarray, out_ptr: int, stream=None) -> None:
                                        _stream_ptr(stream)))
 
 
+def b_field_job(cur: DeviceClip, t: int, ref0: tuple[DeviceClip, int], ref1: tuple[DeviceClip, int],
+                col_ptr: int, qp: int, me_range: int = 32, subpel_rounds: int = 1, partitions: bool = True):
+    """The pcme_b_field_job of B picture t of `cur` between list-0 picture
+    ref0 and list-1 picture ref1 ((clip, index) pairs, prepared); col_ptr:
+    mbw*mbh B_COL_DTYPE records on the device."""
+    from . import BFieldJob
+    (c0, t0), (c1, t1) = ref0, ref1
+    return BFieldJob(cur.plane_ptr(0, t), (ctypes.c_void_p * 4)(*[c0.plane_ptr(k, t0) for k in range(4)]),
+                     (ctypes.c_void_p * 4)(*[c1.plane_ptr(k, t1) for k in range(4)]), col_ptr, cur.mbw, cur.mbh,
+                     int(qp), int(me_range), int(subpel_rounds), 1 if partitions else 0)
+
+
 def b_field_launch(cur: DeviceClip, t: int, ref0: tuple[DeviceClip, int], ref1: tuple[DeviceClip, int],
                    col_ptr: int, out_ptr: int, qp: int, me_range: int = 32, subpel_rounds: int = 1,
                    partitions: bool = True, stream=None) -> None:
     """pcme_b_field on device buffers: col_ptr holds mbw*mbh B_COL_DTYPE
     records, out_ptr receives mbw*mbh B_MB_DTYPE records (stream order, no
     synchronisation; b_field_check reports a stall)."""
-    from . import BFieldJob
-    (c0, t0), (c1, t1) = ref0, ref1
-    job = BFieldJob(cur.plane_ptr(0, t), (ctypes.c_void_p * 4)(*[c0.plane_ptr(k, t0) for k in range(4)]),
-                    (ctypes.c_void_p * 4)(*[c1.plane_ptr(k, t1) for k in range(4)]), col_ptr, cur.mbw, cur.mbh,
-                    int(qp), int(me_range), int(subpel_rounds), 1 if partitions else 0)
+    job = b_field_job(cur, t, ref0, ref1, col_ptr, qp, me_range, subpel_rounds, partitions)
     _check(load().pcme_b_field(ctypes.byref(job), out_ptr, _stream_ptr(stream)))
+
+
+def b_fields_launch(jobs, out_ptrs, stream=None) -> None:
+    """pcme_b_fields: the B field jobs (b_field_job) in one launch, picture
+    i's records to out_ptrs[i] (stream order, no synchronisation)."""
+    from . import BFieldJob
+    assert len(jobs) == len(out_ptrs)
+    arr = (BFieldJob * len(jobs))(*jobs)
+    outs = (ctypes.c_void_p * len(jobs))(*out_ptrs)
+    _check(load().pcme_b_fields(arr, len(jobs), outs, _stream_ptr(stream)))
 
 
 def b_field_check(stream=None) -> None:

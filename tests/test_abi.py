@@ -227,6 +227,45 @@ def test_b_analyse_batch_checks_arguments(pcme_lib):
         assert lib.pcme_b_analyse_batch(job(), 1, 64, None) == pcme_lib.PCME_ENODEV
 
 
+def test_b_fields_check_arguments(pcme_lib):
+    """pcme_b_field / pcme_b_fields refuse bad pictures and outputs before
+    touching a device; no pictures is a no-op."""
+    import ctypes
+    lib = pcme_lib.load()
+
+    def job(**kw):
+        j = pcme_lib.BFieldJob()
+        j.src, j.col, j.mbw, j.mbh, j.qp, j.me_range, j.subpel_rounds, j.partitions = 64, 128, 4, 3, 27, 16, 1, 1
+        for k in range(4):
+            j.ref0[k] = 256
+            j.ref1[k] = 512
+        for k, v in kw.items():
+            setattr(j, k, v)
+        return j
+
+    def fields(*jobs, outs=None):
+        arr = (pcme_lib.BFieldJob * max(1, len(jobs)))(*jobs)
+        o = (ctypes.c_void_p * max(1, len(jobs)))(*(outs if outs is not None else [1024] * len(jobs)))
+        return lib.pcme_b_fields(arr, len(jobs), o, None)
+
+    assert lib.pcme_b_fields(None, 0, None, None) == pcme_lib.PCME_OK
+    assert lib.pcme_b_fields(None, -1, None, None) == pcme_lib.PCME_EINVAL
+    assert lib.pcme_b_fields(None, 1, None, None) == pcme_lib.PCME_EINVAL
+    for bad in (job(src=None), job(col=None), job(mbw=0), job(mbh=5000), job(qp=52), job(me_range=-1),
+                job(subpel_rounds=9), job(src=66)):
+        assert fields(job(), bad) == pcme_lib.PCME_EINVAL
+        assert lib.pcme_b_field(ctypes.byref(bad), 1024, None) == pcme_lib.PCME_EINVAL
+    j = job()
+    j.ref1[3] = None
+    assert fields(j) == pcme_lib.PCME_EINVAL
+    assert fields(job(), job(), outs=[1024, None]) == pcme_lib.PCME_EINVAL
+    assert fields(job(), outs=[1028]) == pcme_lib.PCME_EINVAL  # records are 8-byte aligned
+    assert fields(*[job()] * 1025) == pcme_lib.PCME_EINVAL  # at most 1024 pictures
+    assert fields(job(mbw=4096, mbh=4096)) == pcme_lib.PCME_EINVAL  # at most 2^22 MBs
+    if pcme_lib.device_count() == 0:
+        assert fields(job(), job(mbw=7)) == pcme_lib.PCME_ENODEV
+
+
 def test_aq_offsets_checks_arguments(pcme_lib):
     lib = pcme_lib.load()
     if pcme_lib.device_count() == 0:

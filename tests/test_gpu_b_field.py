@@ -33,6 +33,24 @@ def _oracle_cols(oracle, field_p):
     return cols
 
 
+def _oracle_field(oracle, y, t, fp, qp, partitions, me_range=32):
+    """o_b_field of picture t of y between t-1 and t+1, the co-located MBs
+    from P field fp of picture t+1."""
+    mbh, mbw = fp.shape
+    prm = oracle.oparams(mbw, mbh, me_range, 1, 1 if partitions else 0, 0)
+    return oracle.b_field(prm, _oracle_cols(oracle, fp), oracle.Ref(oracle.pad_luma(y[t - 1], mbw, mbh)),
+                          oracle.Ref(oracle.pad_luma(y[t + 1], mbw, mbh)), oracle.pad_luma(y[t], mbw, mbh),
+                          oracle.load().o_lambda(qp)).reshape(mbh, mbw)
+
+
+def _assert_same_b(got, want, what):
+    mbh, mbw = want.shape
+    g = np.ascontiguousarray(got).view(np.uint8).reshape(mbh * mbw, -1)
+    wt = np.ascontiguousarray(want).view(np.uint8).reshape(mbh * mbw, -1)
+    bad = np.nonzero((g != wt).any(axis=1))[0]
+    assert bad.size == 0, f"{what}: {bad.size} of {mbw * mbh} B records differ, first MB {divmod(int(bad[0]), mbw)}"
+
+
 @pytest.mark.parametrize("w,h,qp,partitions", [(320, 240, 27, True), (320, 240, 32, False), (1920, 1080, 27, True),
                                                 (320, 16, 27, True), (16, 240, 22, True)])
 def test_b_field_matches_oracle(gpu, oracle, w, h, qp, partitions):
@@ -45,17 +63,54 @@ def test_b_field_matches_oracle(gpu, oracle, w, h, qp, partitions):
     dc.prepare()
     fp = oracle.search_clip_me(y[1:3], 32, 1, True, qp)[0]  # picture 2 coded as P from picture 1
     got = B.b_field(dc, 1, (dc, 0), (dc, 2), B.col_records(fp, mbh, mbw), qp, 32, 1, partitions)
-    prm = oracle.oparams(mbw, mbh, 32, 1, 1 if partitions else 0, 0)
-    want = oracle.b_field(prm, _oracle_cols(oracle, fp), oracle.Ref(oracle.pad_luma(y[0], mbw, mbh)),
-                          oracle.Ref(oracle.pad_luma(y[2], mbw, mbh)), oracle.pad_luma(y[1], mbw, mbh),
-                          oracle.load().o_lambda(qp)).reshape(mbh, mbw)
-    g = np.ascontiguousarray(got).view(np.uint8).reshape(mbh * mbw, -1)
-    wt = np.ascontiguousarray(want).view(np.uint8).reshape(mbh * mbw, -1)
-    bad = np.nonzero((g != wt).any(axis=1))[0]
-    assert bad.size == 0, f"{bad.size} of {mbw * mbh} B records differ, first MB {divmod(int(bad[0]), mbw)}"
+    want = _oracle_field(oracle, y, 1, fp, qp, partitions)
+    _assert_same_b(got, want, f"{w}x{h}")
     if min(mbw, mbh) > 1:
         modes = np.bincount(got["mode"].reshape(-1), minlength=4)
         assert modes[0] > 0 and modes[1] > 0 and (modes[2] + modes[3] > 0) == partitions
+
+
+def test_b_fields_one_launch_matches_oracle(gpu, oracle):
+    """pcme_b_fields: five B pictures of three clips (two sizes, QP 22-37,
+    with and without partitions, me_range 16 and 32) in one launch, each
+    bit for bit its oracle field; then the same jobs again in another
+    order, the records unchanged."""
+    import torch
+    from pcme import bsearch as B
+    clips = []
+    for k, (w, h, n) in enumerate(((320, 240, 5), (176, 144, 3), (640, 368, 3))):
+        y, _, _ = gpu.synth_clip(gpu.ClipSpec(w, h, n, seed=0x5EED6200 + k, nrect=5, vmax=6.0))
+        mbw, mbh = gpu.mb_dims(w, h)
+        dc = gpu.DeviceClip(y, mbw, mbh)
+        dc.prepare()
+        clips.append((y, dc))
+    # (clip, B picture, qp, partitions, me_range)
+    cases = [(0, 1, 27, True, 32), (0, 3, 32, True, 32), (1, 1, 37, False, 32), (2, 1, 22, True, 16),
+             (0, 2, 27, True, 24)]
+    jobs, outs, want, keep = [], [], [], []
+    for ci, t, qp, part, rng in cases:
+        y, dc = clips[ci]
+        fp = oracle.search_clip_me(y[t:t + 2], rng, 1, True, qp)[0]
+        col = B.col_records(fp, dc.mbh, dc.mbw)
+        dcol = torch.as_tensor(np.ascontiguousarray(col.reshape(-1)).view(np.uint8)).to("cuda")
+        o = torch.full((dc.mbw * dc.mbh * gpu.B_MB_DTYPE.itemsize,), 0x5A, dtype=torch.uint8, device="cuda")
+        jobs.append(B.b_field_job(dc, t, (dc, t - 1), (dc, t + 1), dcol.data_ptr(), qp, rng, 1, part))
+        outs.append(o)
+        keep.append(dcol)
+        want.append(_oracle_field(oracle, y, t, fp, qp, part, rng))
+    say("B fields, five pictures in one launch")
+    B.b_fields_launch(jobs, [o.data_ptr() for o in outs])
+    B.b_field_check()
+    got = [o.cpu().numpy().view(gpu.B_MB_DTYPE).reshape(w_.shape) for o, w_ in zip(outs, want)]
+    for k, (g, w_) in enumerate(zip(got, want)):
+        _assert_same_b(g, w_, f"picture {k} {cases[k]}")
+    say("B fields, reversed order")
+    for o in outs:
+        o.fill_(0xA5)
+    B.b_fields_launch(jobs[::-1], [o.data_ptr() for o in outs[::-1]])
+    B.b_field_check()
+    for k, (o, g) in enumerate(zip(outs, got)):
+        assert o.cpu().numpy().tobytes() == g.tobytes(), f"picture {k} changed with the order"
 
 
 def test_b_field_stall_reports_estall(gpu, oracle):
