@@ -89,18 +89,20 @@ def parse():
     return ap.parse_args()
 
 
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "kernels_1080p_traffic.md")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "search_frames_traffic.md")
 
 
-def measured_traffic(workload, reuse_regions=1, nref=1):
-    """HBM-side bytes per launch of the search kernel on the 1080p workload
-    in the given LDS-region-reuse configuration, from the FETCH_SIZE /
-    WRITE_SIZE runs summarised in profiles/ (scaled by a 1 GiB copy measured
-    in the same runs); None for other workloads or when the summary is
-    absent."""
-    if workload != "1080p" or nref != 1 or not os.path.exists(TRAFFIC_SUMMARY):
+def measured_traffic(workload, tune, nref):
+    """HBM-side bytes per launch of the search kernel on the workload with
+    nref references at the plan's kernel configuration (tune; None: the
+    default 1,12,1), from the FETCH_SIZE / WRITE_SIZE runs summarised in
+    profiles/ (scaled by a 1 GiB copy measured in the same runs); None for a
+    combination not measured there or when the summary is absent."""
+    if not os.path.exists(TRAFFIC_SUMMARY):
         return None
-    key = f"`search_frames_kernel` reuse_regions={int(bool(reuse_regions))}: bytes_per_launch="
+    wpg, wcu, reuse = ((tune["waves_per_group"], tune["waves_per_cu"], tune["reuse_regions"])
+                       if tune else (1, 12, 1))
+    key = f"`search` workload={workload} nref={nref} config={wpg},{wcu},{reuse}: bytes_per_launch="
     with open(TRAFFIC_SUMMARY) as f:
         for ln in f:
             if ln.startswith(key):
@@ -300,7 +302,6 @@ def main():
         print(f"[bench] one frame alone {one_ms:.2f} ms ({1e3 * one_ms / path:.1f} us per MB on the critical path)",
               file=sys.stderr, flush=True)
         p1.close()
-    reuse = tune["reuse_regions"] if tune else 1  # the plan's default keeps regions in LDS
     bytes_per_mb = BYTES_PER_MB + 4 * 256 * (args.nref - 1)
     achieved_gbs = bytes_per_mb * mbs_per_step / (search_ms * 1e-3) / 1e9
     kname = search_kernel_name(tune, args.nref > 1, False)
@@ -322,9 +323,9 @@ def main():
         "roofline": None,  # set below from the oracle's operation counts (cpu_baseline leg)
         "hbm_roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved_gbs, 2),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
-                         "traffic": measured_traffic(args.workload, reuse, args.nref),
-                         "traffic_source": "profiles/kernels_1080p_traffic.md (FETCH_SIZE/WRITE_SIZE runs)"
-                         if measured_traffic(args.workload, reuse, args.nref) is not None else None,
+                         "traffic": measured_traffic(args.workload, tune, args.nref),
+                         "traffic_source": "profiles/search_frames_traffic.md (FETCH_SIZE/WRITE_SIZE runs)"
+                         if measured_traffic(args.workload, tune, args.nref) is not None else None,
                          "algorithmic_bytes_per_mb": bytes_per_mb, "kernel_ms": round(search_ms, 3)},
         "autotune": tune,
         "graph": graph,
@@ -345,7 +346,7 @@ def main():
         line["roofline"] = {"bound": "valu", "kernel": kname, "achieved": round(valu_tops, 3),
                             "peak": round(VALU_PEAK_TOPS, 2), "unit": "T lane-op/s",
                             "frac": round(valu_tops / VALU_PEAK_TOPS, 4),
-                            "traffic": measured_traffic(args.workload, reuse, args.nref),
+                            "traffic": measured_traffic(args.workload, tune, args.nref),
                             "lane_ops_per_mb": round(per_mb_ops, 1), "mbs_per_launch": mbs_per_step,
                             "kernel_ms": round(search_ms, 3),
                             "peak_basis": "256 CU x 64 lanes x 2.4 GHz, one 32-bit lane-op per lane per clock"}
