@@ -81,6 +81,10 @@ def parse():
     ap.add_argument("--thr", default=None,
                     help="pruner confidence threshold (partition head): one for every QP (0.7), per QP "
                          "(22:0.7,27:0.7,32:0.8,37:0.9), or the per-QP default rule (omitted)")
+    ap.add_argument("--overlap", type=int, default=1,
+                    help="pruned step with the network and the search overlapped: the frames in this many "
+                         "chunks, each chunk's search on a stream of its own once its hints are done (1: the "
+                         "network, then one search launch)")
     ap.add_argument("--thr-mv", type=float, default=0.0,
                     help="threshold on the motion head's candidate mass (0: the partition head decides)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -106,7 +110,7 @@ def measured_traffic(workload, tune, nref):
     with open(TRAFFIC_SUMMARY) as f:
         for ln in f:
             if ln.startswith(key):
-                return int(ln.split("=", 2)[2])
+                return int(ln[len(key):].split()[0])
     return None
 
 
@@ -776,7 +780,7 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
     sp = stream.cuda_stream
     thr_mv = args.thr_mv
     thr_rule = P.parse_thr(args.thr)
-    nets, jobs, pouts, hints, groups = {}, [], [], [], {}
+    nets, jobs, pouts, hints, groups, per_clip = {}, [], [], [], {}, []
     for dc, (spec, qp, rng) in zip(clips, specs):
         if qp not in nets:
             nets[qp] = P.Net(seed=1, max_batch=32, weights=wpath, qp=qp)
@@ -787,6 +791,7 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
                              older=older_refs(t, args.nref))
               for t in range(1, dc.n)]
         groups.setdefault(qp, []).extend(js)
+        per_clip.append((qp, js))
         jobs += js
         pouts.append(o)
         hints.append(h)
@@ -833,6 +838,16 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
     hints_ms = sum(a.elapsed_time(b) for a, b in ev_h) / len(ev_h)
     search_ms = sum(a.elapsed_time(b) for a, b in ev_s) / len(ev_s)
     ms_step = 1e3 * dt / args.steps
+    serial_ms = ms_step
+    overlap = None
+    if args.overlap > 1:
+        serial_out = [o.clone() for o in pouts]
+        overlap = overlapped_pruned(args, pcme, torch, P, wpath, per_clip, thr_rule, stream, prepare, tune)
+        torch.cuda.synchronize()
+        overlap["records_identical"] = all(bool(torch.equal(a, b)) for a, b in zip(serial_out, pouts))
+        if not overlap["records_identical"]:
+            raise SystemExit("[bench] the overlapped pruned step's records differ from the serial step's")
+        ms_step = overlap["ms_per_step"]
     plain = np.concatenate([o.cpu().numpy().view(pcme.MB_DTYPE).reshape(-1) for o in outs])
     pruned = np.concatenate([o.cpu().numpy().view(pcme.MB_DTYPE).reshape(-1) for o in pouts])
     hs = np.concatenate([h.cpu().numpy().view(pcme.HINT_DTYPE).reshape(-1) for h in hints])
@@ -840,13 +855,18 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
     tflops = 2.0 * NET_MACS_PER_MB * mbs_per_step / (hints_ms * 1e-3) / 1e12
     for n in nets.values():
         n.close()
-    print(f"[bench] pruned: {ms_step:.2f} ms/step (hints {hints_ms:.2f} ms, search {search_ms:.2f} ms), "
+    print(f"[bench] pruned: {ms_step:.2f} ms/step (serial {serial_ms:.2f}: hints {hints_ms:.2f} ms, "
+          f"search {search_ms:.2f} ms), "
           f"agreement {j['agreement']:.4f}, J-cost +{100 * j['jcost_increase']:.3f}%", file=sys.stderr, flush=True)
     return {
         "mblocks_per_s": round(mbs_per_step / (ms_step * 1e-3) / 1e6, 4),
         "ms_per_step": round(ms_step, 3),
         "speedup_vs_plain": round(plain_ms_step / ms_step, 3),
         "hints_ms": round(hints_ms, 3), "search_ms": round(search_ms, 3),
+        # the network, then the search, one launch each
+        "serial_ms_per_step": round(serial_ms, 3),
+        "serial_speedup_vs_plain": round(plain_ms_step / serial_ms, 3),
+        "overlap": overlap,
         "decision_agreement": round(j["agreement"], 5),
         "part_agreement": round(j["part_agreement"], 5),
         # hinted mode of the pruned MBs: 16x16 only / 16x8+8x16 (two modes) / 8x8
@@ -866,6 +886,69 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
                          "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5), "macs_per_mb": NET_MACS_PER_MB},
         "bd_rate": measured_bd_rate(args.ncand, thr_rule, thr_mv, args.workload, args.nref),
     }
+
+
+def overlapped_pruned(args, pcme, torch, P, wpath, per_clip, thr_rule, stream, prepare, tune):
+    """The pruned step with the network and the search overlapped.  Every
+    clip's P frames are cut into ``args.overlap`` contiguous ranges; chunk i
+    holds range i of every clip and has its own network job table and frame
+    plan.  The networks of all chunks run in order on the main stream; chunk
+    i's search launches on a stream of its own as soon as chunk i's hints are
+    done, so it runs beside the network of the later chunks and beside the
+    other chunks' searches.  Every wait is on work enqueued before it, and
+    the searches' walkers wait only on tickets of their own launch, so no
+    launch can wait on another.  The records equal the serial step's: a
+    frame's search reads only its own planes and hints."""
+    k = args.overlap
+    chunks = [{} for _ in range(k)]
+    for qp, js in per_clip:
+        n = len(js)
+        for i in range(k):
+            part = js[i * n // k:(i + 1) * n // k]
+            if part:
+                chunks[i].setdefault(qp, []).extend(part)
+    chunks = [c for c in chunks if c]
+    nets = {(qp, i): P.Net(seed=1, max_batch=32, weights=wpath, qp=qp)
+            for i, c in enumerate(chunks) for qp in c}
+    plans = [pcme.Plan([j for js in c.values() for j in js]) for c in chunks]
+    for pl in plans:
+        if tune:
+            pl.set_config(0, tune["waves_per_group"], tune["waves_per_cu"], tune["reuse_regions"])
+    streams = [torch.cuda.Stream() for _ in chunks]
+    sp = stream.cuda_stream
+
+    def step():
+        prepare()
+        done = []
+        for i, c in enumerate(chunks):
+            for qp, js in c.items():
+                nets[(qp, i)].hints(js, args.ncand, P.thr_part_for(qp, thr_rule), args.thr_mv, sp)
+            e = torch.cuda.Event()
+            e.record(stream)
+            done.append(e)
+        for pl, st, e in zip(plans, streams, done):
+            st.wait_event(e)
+            pl.launch(st.cuda_stream)
+        for st in streams:
+            stream.wait_stream(st)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    for pl, st in zip(plans, streams):
+        pl.check(st.cuda_stream)
+        pl.close()
+    for n in nets.values():
+        n.close()
+    ms = 1e3 * dt / args.steps
+    print(f"[bench] pruned, network and search overlapped in {len(chunks)} chunks: {ms:.2f} ms/step",
+          file=sys.stderr, flush=True)
+    return {"chunks": len(chunks), "ms_per_step": round(ms, 3)}
 
 
 if __name__ == "__main__":
