@@ -85,6 +85,9 @@ def parse():
                     help="pruned step with the network and the search overlapped: the frames in this many "
                          "chunks, each chunk's search on a stream of its own once its hints are done (1: the "
                          "network, then one search launch)")
+    ap.add_argument("--overlap-grid", type=float, default=1.0,
+                    help="with --overlap: each chunk's persistent grid is this fraction of the tuned grid "
+                         "divided by the chunk count, so the chunks' walkers are resident together")
     ap.add_argument("--thr-mv", type=float, default=0.0,
                     help="threshold on the motion head's candidate mass (0: the partition head decides)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -913,7 +916,8 @@ def overlapped_pruned(args, pcme, torch, P, wpath, per_clip, thr_rule, stream, p
     plans = [pcme.Plan([j for js in c.values() for j in js]) for c in chunks]
     for pl in plans:
         if tune:
-            pl.set_config(0, tune["waves_per_group"], tune["waves_per_cu"], tune["reuse_regions"])
+            waves = max(64, int(tune["waves"] * args.overlap_grid / len(plans)))
+            pl.set_config(waves, tune["waves_per_group"], tune["waves_per_cu"], tune["reuse_regions"])
     streams = [torch.cuda.Stream() for _ in chunks]
     sp = stream.cuda_stream
 
@@ -940,6 +944,7 @@ def overlapped_pruned(args, pcme, torch, P, wpath, per_clip, thr_rule, stream, p
         step()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    waves_of = [pl.get_config().waves for pl in plans]
     for pl, st in zip(plans, streams):
         pl.check(st.cuda_stream)
         pl.close()
@@ -948,7 +953,8 @@ def overlapped_pruned(args, pcme, torch, P, wpath, per_clip, thr_rule, stream, p
     ms = 1e3 * dt / args.steps
     print(f"[bench] pruned, network and search overlapped in {len(chunks)} chunks: {ms:.2f} ms/step",
           file=sys.stderr, flush=True)
-    return {"chunks": len(chunks), "ms_per_step": round(ms, 3)}
+    return {"chunks": len(chunks), "ms_per_step": round(ms, 3),
+            "waves_per_chunk": [pl_waves for pl_waves in waves_of]}
 
 
 if __name__ == "__main__":
