@@ -222,8 +222,7 @@ def main():
     sp = stream.cuda_stream
 
     def prepare():
-        for dc in clips:
-            dc.prepare(sp)
+        pcme.prepare_clips(clips, sp)
 
     plan = pcme.Plan(jobs)
     prepare()
@@ -265,8 +264,7 @@ def main():
         s2.wait_stream(stream)
         with torch.cuda.stream(s2):
             with torch.cuda.graph(g, stream=s2, capture_error_mode="thread_local"):
-                for dc in clips:
-                    dc.prepare(s2.cuda_stream)
+                pcme.prepare_clips(clips, s2.cuda_stream)
                 plan.launch(s2.cuda_stream)
         torch.cuda.synchronize()
         g.replay()
@@ -801,8 +799,7 @@ def pruned_leg(args, pcme, torch, clips, outs, specs, stream, mbs_per_step, plai
     plan = pcme.Plan(jobs)
 
     def prepare():
-        for dc in clips:
-            dc.prepare(sp)
+        pcme.prepare_clips(clips, sp)
 
     def infer():
         for qp, js in groups.items():

@@ -189,6 +189,20 @@ int pcme_luma_planes(const uint8_t *src, int64_t src_pitch, int32_t sstride,
                      int32_t w, int32_t h, uint8_t *full, uint8_t *hp,
                      uint8_t *vp, uint8_t *cp, int64_t pitch, int32_t n,
                      int32_t mbw, int32_t mbh, void *stream);
+/* The arguments of one pcme_luma_planes call. */
+typedef struct pcme_planes_job {
+  const uint8_t *src;
+  uint8_t *full, *hp, *vp, *cp;
+  int64_t src_pitch, pitch;
+  int32_t sstride, w, h, n, mbw, mbh;
+} pcme_planes_job;
+/* pcme_luma_planes of several clips, of any sizes, in one launch per 16
+ * clips: the same planes bit for bit.  Each clip is the reference's import
+ * + extend + half-pel planes of its frames (mod.rs:842-873, frame.rs:58-122,
+ * picture.rs:318-380); the reference builds them per frame on each encoder's
+ * frame threads, which a batch of clips replaces.  Every job is checked
+ * before anything is launched; no device memory is allocated. */
+int pcme_luma_planes_batch(const pcme_planes_job *jobs, int32_t njobs, void *stream);
 
 /* ---- the search ----------------------------------------------------- */
 /* ME-only field over a batch of P frames: for every macroblock of every

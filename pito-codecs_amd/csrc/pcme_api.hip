@@ -753,6 +753,49 @@ int pcme_luma_planes(const uint8_t *src, int64_t src_pitch, int32_t sstride, int
   return PCME_OK;
 }
 
+static int lp_launch(const LpBatch &b, int64_t blocks, hipStream_t s) {
+  hipLaunchKernelGGL(luma_planes_batch_kernel, dim3((unsigned)blocks), dim3(256), 0, s, b);
+  HIPCHK(hipGetLastError());
+  return PCME_OK;
+}
+
+int pcme_luma_planes_batch(const pcme_planes_job *jobs, int32_t njobs, void *stream) {
+  if (njobs < 0 || (njobs > 0 && !jobs)) return fail(PCME_EINVAL, "pcme_luma_planes_batch: bad job array");
+  for (int i = 0; i < njobs; i++) {  // every job checked before anything is launched
+    const pcme_planes_job &J = jobs[i];
+    if (J.n < 0 || J.mbw < 1 || J.mbh < 1 || J.w < 1 || J.h < 1 || J.w > 16 * J.mbw || J.h > 16 * J.mbh ||
+        J.sstride < J.w || !J.src || !J.full || !J.hp || !J.vp || !J.cp)
+      return fail(PCME_EINVAL, "pcme_luma_planes_batch: job %d: bad geometry", i);
+    const int64_t stride = 16 * J.mbw + 2 * PCME_PAD, rows = 16 * J.mbh + 2 * PCME_PAD;
+    if (J.pitch < stride * rows || (J.pitch & 3) ||
+        (((uintptr_t)J.full | (uintptr_t)J.hp | (uintptr_t)J.vp | (uintptr_t)J.cp) & 3))
+      return fail(PCME_EINVAL, "pcme_luma_planes_batch: job %d: pitch smaller than a plane or not 4-byte aligned",
+                  i);
+  }
+  Dev *dv;
+  int rc = get_dev(&dv);
+  if (rc) return rc;
+  const hipStream_t s = (hipStream_t)stream;
+  LpBatch b{};
+  int64_t blocks = 0;
+  for (int i = 0; i < njobs; i++) {
+    const pcme_planes_job &J = jobs[i];
+    if (J.n == 0) continue;
+    const int stride = 16 * J.mbw + 2 * PCME_PAD, rows = 16 * J.mbh + 2 * PCME_PAD;
+    const int ngx = stride / 4, nbands = (rows + LP_TH - 1) / LP_TH, bpf = (ngx * nbands + 255) / 256;
+    if (b.n == LP_MAXJOBS || blocks + (int64_t)bpf * J.n > INT32_MAX) {
+      if ((rc = lp_launch(b, blocks, s))) return rc;
+      b.n = 0;
+      blocks = 0;
+    }
+    if ((int64_t)bpf * J.n > INT32_MAX) return fail(PCME_EINVAL, "pcme_luma_planes_batch: job %d too large", i);
+    b.j[b.n++] = LpJob{J.src, J.full, J.hp, J.vp, J.cp, J.src_pitch, J.pitch, J.sstride, J.w, J.h, stride, rows,
+                       ngx, nbands, bpf, (int)blocks, 0};
+    blocks += (int64_t)bpf * J.n;
+  }
+  return b.n ? lp_launch(b, blocks, s) : PCME_OK;
+}
+
 int pcme_search_frames(const pcme_frame_job *jobs, int njobs, void *stream) {
   return search_frames_impl(jobs, njobs, (hipStream_t)stream, false);
 }

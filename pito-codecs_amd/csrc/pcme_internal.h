@@ -108,6 +108,20 @@ __global__ void luma_planes_kernel(const uint8_t *__restrict__ src, int64_t src_
                                    uint8_t *__restrict__ vp, uint8_t *__restrict__ cp, int64_t pitch, int stride,
                                    int rows);
 #define LP_TH 32  // rows of a luma_planes_kernel band
+// One clip of luma_planes_batch_kernel: ngx 4-column groups x nbands bands
+// per frame, bpf blocks per frame, its first block at `first`.
+struct LpJob {
+  const uint8_t *src;
+  uint8_t *full, *hp, *vp, *cp;
+  int64_t src_pitch, pitch;
+  int sstride, w, h, stride, rows, ngx, nbands, bpf, first, unused;
+};
+#define LP_MAXJOBS 16  // clips per launch (the table travels in the kernel arguments)
+struct LpBatch {
+  LpJob j[LP_MAXJOBS];
+  int n;
+};
+__global__ void luma_planes_batch_kernel(const LpBatch jobs);
 __global__ void pred_frame_kernel(const uint8_t *__restrict__ pl0, const uint8_t *__restrict__ pl1,
                                   const uint8_t *__restrict__ pl2, const uint8_t *__restrict__ pl3,
                                   const uint8_t *__restrict__ pu, const uint8_t *__restrict__ pv,

@@ -2796,16 +2796,12 @@ __device__ __forceinline__ uint32_t lp_cbyte(int v) {
   return (uint32_t)b;
 }
 
-__global__ void __launch_bounds__(256) luma_planes_kernel(const uint8_t *__restrict__ src, int64_t src_pitch,
-                                                          int sstride, int w, int h, uint8_t *__restrict__ full,
-                                                          uint8_t *__restrict__ hp, uint8_t *__restrict__ vp,
-                                                          uint8_t *__restrict__ cp, int64_t pitch, int stride,
-                                                          int rows) {
-  const int x0 = (blockIdx.x * 256 + threadIdx.x) * 4;
-  if (x0 >= stride) return;  // stride is a multiple of 16
-  const int y0 = blockIdx.y * LP_TH;
-  const uint8_t *s = src + (size_t)blockIdx.z * src_pitch;
-  const size_t po = (size_t)blockIdx.z * pitch;
+// The band of LP_TH output rows from y0 and 4 output columns from x0 of one
+// frame: source frame s, planes at full/hp/vp/cp + po.
+__device__ __forceinline__ void lp_band(const uint8_t *__restrict__ s, int sstride, int w, int h,
+                                        uint8_t *__restrict__ full, uint8_t *__restrict__ hp,
+                                        uint8_t *__restrict__ vp, uint8_t *__restrict__ cp, size_t po, int stride,
+                                        int rows, int x0, int y0) {
   const int sx0 = x0 - PCME_PAD;
   // Interior lanes read three aligned dwords of a row; lanes near the
   // picture's left/right edges gather clamped bytes.
@@ -2866,6 +2862,34 @@ __global__ void __launch_bounds__(256) luma_planes_kernel(const uint8_t *__restr
     *(uint32_t *)(vp + o) = vv;
     *(uint32_t *)(cp + o) = cv;
   }
+}
+
+__global__ void __launch_bounds__(256) luma_planes_kernel(const uint8_t *__restrict__ src, int64_t src_pitch,
+                                                          int sstride, int w, int h, uint8_t *__restrict__ full,
+                                                          uint8_t *__restrict__ hp, uint8_t *__restrict__ vp,
+                                                          uint8_t *__restrict__ cp, int64_t pitch, int stride,
+                                                          int rows) {
+  const int x0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+  if (x0 >= stride) return;  // stride is a multiple of 16
+  lp_band(src + (size_t)blockIdx.z * src_pitch, sstride, w, h, full, hp, vp, cp, (size_t)blockIdx.z * pitch,
+          stride, rows, x0, blockIdx.y * LP_TH);
+}
+
+// Several clips of any sizes in one launch.  The blocks of clip k start at
+// jobs.j[k].first, bpf per frame; a frame's (band, 4-column group) pairs are
+// numbered band-major and dealt 256 to a block, so a narrow picture fills
+// its blocks' lanes (one band per block would leave most of them idle).
+__global__ void __launch_bounds__(256) luma_planes_batch_kernel(const LpBatch jobs) {
+  const int b = blockIdx.x;
+  int k = 0;
+  while (k + 1 < jobs.n && b >= jobs.j[k + 1].first) k++;
+  const LpJob &J = jobs.j[k];
+  const int local = b - J.first, frame = local / J.bpf;
+  const int idx = (local - frame * J.bpf) * 256 + (int)threadIdx.x;
+  if (idx >= J.ngx * J.nbands) return;
+  const int band = idx / J.ngx;
+  lp_band(J.src + (size_t)frame * J.src_pitch, J.sstride, J.w, J.h, J.full, J.hp, J.vp, J.cp,
+          (size_t)frame * J.pitch, J.stride, J.rows, (idx - band * J.ngx) * 4, band * LP_TH);
 }
 
 // ------------------------------------------------------------ pred kernel

@@ -59,6 +59,14 @@ class RefPlanes(ctypes.Structure):
 MAX_REFS = 4
 
 
+class PlanesJob(ctypes.Structure):
+    """pcme_planes_job: the arguments of one pcme_luma_planes call."""
+    _fields_ = [("src", ctypes.c_void_p), ("full", ctypes.c_void_p), ("hp", ctypes.c_void_p),
+                ("vp", ctypes.c_void_p), ("cp", ctypes.c_void_p), ("src_pitch", ctypes.c_int64),
+                ("pitch", ctypes.c_int64), ("sstride", ctypes.c_int32), ("w", ctypes.c_int32),
+                ("h", ctypes.c_int32), ("n", ctypes.c_int32), ("mbw", ctypes.c_int32), ("mbh", ctypes.c_int32)]
+
+
 class FrameJob(ctypes.Structure):
     _fields_ = [("src", ctypes.c_void_p), ("ref", ctypes.c_void_p),
                 ("ref_h", ctypes.c_void_p), ("ref_v", ctypes.c_void_p),
@@ -228,7 +236,7 @@ assert HINT_DTYPE.itemsize == 40
 # Every symbol include/pcme.h and include/pcme_synth.h declare.
 EXPORTS = (
     "pcme_version", "pcme_last_error", "pcme_device_count", "pcme_lambda",
-    "pcme_qpel_table", "pcme_pad_planes", "pcme_hpel", "pcme_luma_planes", "pcme_search_frames",
+    "pcme_qpel_table", "pcme_pad_planes", "pcme_hpel", "pcme_luma_planes", "pcme_luma_planes_batch", "pcme_search_frames",
     "pcme_search_frames_sync", "pcme_search_frames_check",
     "pcme_search_frames_noalloc", "pcme_search_batch", "pcme_pred_frame",
     "pcme_synth_clip", "pcme_plan_create", "pcme_plan_launch", "pcme_plan_check",
@@ -272,6 +280,7 @@ def load() -> ctypes.CDLL:
         "pcme_hpel": (c.c_int, [vp, vp, vp, vp, c.c_int64, c.c_int32, c.c_int32, c.c_int32, vp]),
         "pcme_luma_planes": (c.c_int, [vp, c.c_int64, c.c_int32, c.c_int32, c.c_int32, vp, vp, vp, vp, c.c_int64,
                                        c.c_int32, c.c_int32, c.c_int32, vp]),
+        "pcme_luma_planes_batch": (c.c_int, [c.POINTER(PlanesJob), c.c_int32, vp]),
         "pcme_search_frames": (c.c_int, [c.POINTER(FrameJob), c.c_int, vp]),
         "pcme_search_frames_sync": (c.c_int, [c.POINTER(FrameJob), c.c_int, vp]),
         "pcme_search_frames_check": (c.c_int, [vp]),
@@ -425,6 +434,12 @@ class DeviceClip:
         _check(lib.pcme_pad_planes(self.raw.data_ptr(), self.h * self.w, self.w, self.w, self.h,
                                    self.planes[0].data_ptr(), self.pitch, self.stride,
                                    16 * self.mbw, 16 * self.mbh, LUMA_PAD, self.n, s))
+        self.pad_chroma(stream)
+
+    def pad_chroma(self, stream=None) -> None:
+        """The padded chroma planes of every frame (pcme_pad_planes); nothing
+        for a luma-only clip."""
+        lib, s = load(), _stream_ptr(stream)
         if self.has_chroma:
             ch, cw = self.h // 2, self.w // 2
             for k in range(2):
@@ -440,13 +455,7 @@ class DeviceClip:
         _check(lib.pcme_luma_planes(self.raw.data_ptr(), self.h * self.w, self.w, self.w, self.h,
                                     self.plane_ptr(0, 0), self.plane_ptr(1, 0), self.plane_ptr(2, 0),
                                     self.plane_ptr(3, 0), self.pitch, self.n, self.mbw, self.mbh, s))
-        if self.has_chroma:
-            ch, cw = self.h // 2, self.w // 2
-            for k in range(2):
-                _check(lib.pcme_pad_planes(self.craw[k].data_ptr(), ch * cw, cw, cw, ch,
-                                           self.cplanes[k].data_ptr(), self.crows * self.cstride,
-                                           self.cstride, 8 * self.mbw, 8 * self.mbh, CHROMA_PAD,
-                                           self.n, s))
+        self.pad_chroma(stream)
 
     def hpel(self, first: int = 0, count: int | None = None, stream=None) -> None:
         """Half-pel planes of frames [first, first+count) (picture.rs:318-380)."""
@@ -456,6 +465,20 @@ class DeviceClip:
         lib, s = load(), _stream_ptr(stream)
         _check(lib.pcme_hpel(self.plane_ptr(0, first), self.plane_ptr(1, first), self.plane_ptr(2, first),
                              self.plane_ptr(3, first), self.pitch, count, self.mbw, self.mbh, s))
+
+
+def prepare_clips(clips: list[DeviceClip], stream=None) -> None:
+    """DeviceClip.prepare of every clip, the luma planes of all of them
+    through pcme_luma_planes_batch (one launch per 16 clips)."""
+    if not clips:
+        return
+    jobs = (PlanesJob * len(clips))(*[
+        PlanesJob(dc.raw.data_ptr(), dc.plane_ptr(0, 0), dc.plane_ptr(1, 0), dc.plane_ptr(2, 0),
+                  dc.plane_ptr(3, 0), dc.h * dc.w, dc.pitch, dc.w, dc.w, dc.h, dc.n, dc.mbw, dc.mbh)
+        for dc in clips])
+    _check(load().pcme_luma_planes_batch(jobs, len(clips), _stream_ptr(stream)))
+    for dc in clips:
+        dc.pad_chroma(stream)
 
 
 def frame_job(clip: DeviceClip, cur: int, ref: int, out_ptr: int, me_range: int, subpel_rounds: int,

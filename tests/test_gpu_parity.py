@@ -87,6 +87,32 @@ def test_fused_luma_planes_match_pad_and_hpel(gpu, oracle, w, h):
         np.testing.assert_array_equal(pb[1 + k, 1], want[k], err_msg=f"oracle plane {'HVC'[k]}")
 
 
+def test_luma_planes_batch_matches_one_clip_at_a_time(gpu):
+    """pcme_luma_planes_batch over clips of mixed sizes (more than one launch's
+    16, narrow pictures whose blocks hold several bands, an empty job array)
+    writes every clip's planes as pcme_luma_planes of that clip alone."""
+    import torch
+    sizes = [(352, 288, 3), (640, 360, 2), (1280, 720, 2), (1920, 1080, 2), (100, 70, 3), (16, 16, 2),
+             (34, 18, 2), (52, 200, 2), (200, 16, 3)] * 2
+    say("luma planes batch", len(sizes), "clips")
+    want, got = [], []
+    for i, (w, h, n) in enumerate(sizes):
+        y, _, _ = _clip(gpu, w, h, n)
+        mbw, mbh = gpu.mb_dims(w, h)
+        a = gpu.DeviceClip(y, mbw, mbh)
+        a.planes.fill_(0x5A)
+        a.prepare()
+        b = gpu.DeviceClip(y, mbw, mbh)
+        b.planes.fill_(0xA5)
+        want.append(a)
+        got.append(b)
+    gpu.prepare_clips(got)
+    torch.cuda.synchronize()
+    for (w, h, n), a, b in zip(sizes, want, got):
+        assert torch.equal(a.planes, b.planes), f"{w}x{h} x{n}"
+    assert gpu.load().pcme_luma_planes_batch(None, 0, None) == 0
+
+
 CASES = [
     # (w, h, frames, me_range, subpel, partitions, qp, generator kwargs)
     (320, 240, 8, 16, 1, True, 28, dict(nrect=4, rect_lo=24, rect_hi=80, vmax=3.0, pan_max=1.0)),
